@@ -1,0 +1,259 @@
+/*
+ * mrh.h — C-ABI of the MI355X wavefront path-tracing core.
+ *
+ * Plain pointers and sizes only.  "d_" arguments are device pointers in the
+ * context's device memory (the caller owns them; the library never frees
+ * caller memory).  Every call is asynchronous on the context's stream unless
+ * it says otherwise; mrh_sync() waits for it.  Every entry point returns an
+ * mrh_status; on failure mrh_last_error() holds the message.  No C++
+ * exceptions cross this boundary.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to the reference checkout).
+ */
+#ifndef MRH_H
+#define MRH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MRH_OK = 0,
+    MRH_ERR_INVALID = 1,      /* bad argument (TracerError::..., SceneError::...) */
+    MRH_ERR_HIP = 2,          /* HIP runtime error */
+    MRH_ERR_OOM = 3,          /* device allocation failed (TracerException on OOM, CudaCheck.h:32-42) */
+    MRH_ERR_UNSUPPORTED = 4,  /* option or type not implemented */
+    MRH_ERR_BUDGET = 5        /* request exceeds the memory budget */
+} mrh_status;
+
+typedef struct mrh_context mrh_context;
+typedef struct mrh_lbvh mrh_lbvh;
+typedef struct mrh_scene mrh_scene;
+
+/* ------------------------------------------------------------------ */
+/* Context: device, stream, scratch memory, per-kernel timers.         */
+/* Replaces CudaSystem::Initialize + CudaGPU (TracerCUDA/CudaSystem.h:97-118,
+ * CudaSystem.cpp:109-122) as created by TracerSystemCUDA::Initialize
+ * (TracerSystemCUDA.cpp:19).                                           */
+/* ------------------------------------------------------------------ */
+int32_t     mrh_context_create(int32_t device, mrh_context** out);
+int32_t     mrh_context_destroy(mrh_context* ctx);
+const char* mrh_last_error(const mrh_context* ctx);
+int32_t     mrh_sync(mrh_context* ctx);
+/* hipStream_t of the context, returned as void* */
+int32_t     mrh_stream(mrh_context* ctx, void** stream);
+/* Copy between any two device/host pointers on the context stream
+ * (hipMemcpyDefault), then wait for it.  Utility for hosts that keep their
+ * buffers outside any framework. */
+int32_t     mrh_memcpy(mrh_context* ctx, void* dst, const void* src, size_t bytes);
+/* Per-kernel hipEvent timing on the context stream (the reference's
+ * cudaEvent timers, WFPGTracer.cu:592-596).  When enabled every launch is
+ * bracketed by events; mrh_profile_get returns total ms and launches for a
+ * kernel name, summed since the last mrh_profile_reset. */
+int32_t     mrh_profile_enable(mrh_context* ctx, int32_t on);
+int32_t     mrh_profile_reset(mrh_context* ctx);
+int32_t     mrh_profile_get(mrh_context* ctx, const char* kernel, double* total_ms,
+                            uint64_t* launches);
+
+/* ------------------------------------------------------------------ */
+/* Morton codes.  Replaces KCGenerateAABBs + ReduceArrayGPU + FindMortonDelta
+ * + KCGenMortonCodes (TracerCUDA/LinearBVH.hpp:103-166, 400-414, 446-477).
+ * d_tris: float[n][9] triangle corners.  bits = 30 writes uint32 codes,
+ * bits = 63 writes uint64 codes.  Quantisation: centroid = min + 0.5*span,
+ * delta = float(max span * 2^-b) (b = 10 or 21), IEEE division, clamp to
+ * 2^b - 1.                                                              */
+/* ------------------------------------------------------------------ */
+int32_t mrh_morton_codes(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t bits,
+                         void* d_codes);
+
+/* ------------------------------------------------------------------ */
+/* Onesweep LSD radix sort of (key, value) pairs.  Replaces
+ * RadixSortValKeyGPU (TracerCUDA/ParallelRadixSort.cuh) and the two
+ * cub::DeviceRadixSort::SortPairsDescending passes of RayMemory::SortKeys
+ * (TracerCUDA/RayMemory.cu:187-228).  Stable.  key_bytes = 4 or 8; bits
+ * [begin_bit, end_bit) are sorted; d_vals_in = NULL means values 0..n-1.
+ * Input and output ranges must not overlap.                             */
+/* ------------------------------------------------------------------ */
+int32_t mrh_sort_pairs(mrh_context* ctx, const void* d_keys_in, const uint32_t* d_vals_in,
+                       void* d_keys_out, uint32_t* d_vals_out, uint32_t n, uint32_t key_bytes,
+                       uint32_t begin_bit, uint32_t end_bit, int32_t descending);
+
+/* Partition ray ids by HitKey.  Replaces RayMemory::SortKeys + Partition
+ * (TracerCUDA/RayMemory.cu:187-312) and RayCaster::PartitionRaysWRTWork
+ * (TracerCUDA/RayCaster.cu:21-37): a stable DESCENDING sort of (key, id)
+ * over bits [sort_begin_bit, end_bit), d_order receives the ids.
+ * d_offsets receives 2^(end_bit - part_begin_bit) + 1 entries: entry j is
+ * the first sorted position whose bits [part_begin_bit, end_bit) equal
+ * (2^w - 1 - j), the last entry is n; so batch b's rays are
+ * d_order[d_offsets[2^w-1-b] .. d_offsets[2^w-b]), the NullBatch (0) tail
+ * last.  end_bit - part_begin_bit <= 8 and part_begin_bit >= sort_begin_bit.
+ * No host round trip (the reference copies the split list to a host
+ * std::set every bounce, RayMemory.cu:275-297).                          */
+int32_t mrh_partition_by_key(mrh_context* ctx, const uint32_t* d_keys, uint32_t n,
+                             uint32_t sort_begin_bit, uint32_t part_begin_bit, uint32_t end_bit,
+                             uint32_t* d_order, uint32_t* d_offsets);
+
+/* ------------------------------------------------------------------ */
+/* LBVH.  Replaces LinearBVHCPU::Construct (TracerCUDA/LinearBVH.hpp:423-596)
+ * and, as the ray-casting accelerator, GPUAccBVHGroup::ConstructAccelerator
+ * (TracerCUDA/GPUAcceleratorBVH.hpp:353-640).  Topology in the reference's
+ * numbering: leaves are nodes [0,n) in sorted order, internal node p is
+ * node n+p with its split between sorted leaves p and p+1, the parent is
+ * chosen by ChooseParentAndUpdate (LinearBVH.hpp:43-100); duplicate codes are
+ * ordered by the (code, index) tie-break instead of aborting
+ * (LinearBVH.hpp:506-521).                                                */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint32_t    n;            /* leaves */
+    uint32_t    bits;         /* 30 or 63 */
+    uint32_t    root;         /* global node index of the root */
+    uint32_t    max_depth;    /* deepest leaf (root = depth 0) */
+    const void* d_sorted_codes;   /* uint32[n] or uint64[n] */
+    const uint32_t* d_sorted_idx; /* primitive of sorted leaf i */
+    const uint32_t* d_left;       /* [n-1] global node index */
+    const uint32_t* d_right;      /* [n-1] */
+    const uint32_t* d_parent;     /* [2n-1]; root holds 0xFFFFFFFF */
+    const float*    d_node_aabb;  /* [(2n-1)][6] min.xyz max.xyz */
+} mrh_lbvh_view;
+
+/* d_tris is copied; the caller may free it after the call returns.
+ * Synchronous (returns after the build). */
+int32_t mrh_lbvh_build(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t bits,
+                       mrh_lbvh** out);
+int32_t mrh_lbvh_view_get(const mrh_lbvh* bvh, mrh_lbvh_view* out);
+int32_t mrh_lbvh_destroy(mrh_lbvh* bvh);
+/* Rebuilds in place from new corners of the same count (timed builds). */
+int32_t mrh_lbvh_rebuild(mrh_context* ctx, mrh_lbvh* bvh, const float* d_tris);
+
+/* ------------------------------------------------------------------ */
+/* Ray casting.  Replaces GPUAccBVHGroup::Hit -> KCIntersectBVH /
+ * KCIntersectBVHStackless + AcceptHit (TracerCUDA/GPUAcceleratorBVHKC.cuh:
+ * 229-511, GPUAcceleratorCommonKC.cuh:33-78) and GPUBaseAcceleratorI::Hit
+ * (GPUAcceleratorI.h:149-166).
+ * d_rays: 7 planes of float[pitch]: ox, oy, oz, dx, dy, dz, tmax.
+ * Outputs: d_prim (primitive index, 0xFFFFFFFF = miss), d_t, d_bary
+ * (2 planes of float[n]: weights of corners 0 and 1).  Closest hit accepts
+ * t in [tmin, tmax).  d_stats (optional): 2 uint32 per ray = internal-node
+ * visits and triangle tests.                                             */
+/* ------------------------------------------------------------------ */
+int32_t mrh_trace_closest(mrh_context* ctx, const mrh_lbvh* bvh, const float* d_rays,
+                          uint32_t n, uint32_t pitch, float tmin, uint32_t* d_prim,
+                          float* d_t, float* d_bary, uint32_t* d_stats);
+/* any hit (shadow rays): d_hit[i] = 1 if some t in [tmin, tmax) exists */
+int32_t mrh_trace_any(mrh_context* ctx, const mrh_lbvh* bvh, const float* d_rays, uint32_t n,
+                      uint32_t pitch, float tmin, uint32_t* d_hit);
+
+/* ------------------------------------------------------------------ */
+/* Scenes and the plain wavefront path tracer.                          */
+/* Replaces GPUSceneI + the Triangle primitive group, the LambertC / Metal /
+ * Refract material groups and primitive area lights
+ * (TracerCUDA/GPUPrimitiveTriangle.h, SimpleMaterialsKC.cuh:16-303,
+ * MetalGlassMaterialsKC.cuh:11-222, GPULightP.cuh:133-140), and
+ * PathTracer::GenerateWork/Render/Finalize (TracerCUDA/PathTracer.cu:144-313). */
+/* ------------------------------------------------------------------ */
+
+/* HitKey work batches (RayLib/HitStructs.h:42-77: key = batch:8 | id:24) */
+#define MRH_BATCH_NULL       0u
+#define MRH_BATCH_LAMBERT    1u
+#define MRH_BATCH_METAL      2u
+#define MRH_BATCH_DIELECTRIC 3u
+#define MRH_BATCH_EMISSIVE   4u
+#define MRH_KEY(batch, id)   (((uint32_t)(batch) << 24) | ((uint32_t)(id) & 0xFFFFFFu))
+
+typedef struct {
+    const float*    tris;        /* host float[n][9] */
+    const uint32_t* prim_key;    /* host uint32[n] HitKey per triangle */
+    uint32_t        n;
+    const float*    lambert;     /* host float[n_lambert][3] albedo */
+    uint32_t        n_lambert;
+    const float*    metal;       /* host float[n_metal][10]: roughness, eta rgb, k rgb, specular rgb */
+    uint32_t        n_metal;
+    const float*    dielectric;  /* host float[n_dielectric][4]: albedo rgb, ior */
+    uint32_t        n_dielectric;
+    const float*    emissive;    /* host float[n_emissive][3] radiance */
+    uint32_t        n_emissive;
+    uint32_t        morton_bits; /* 30 or 63 */
+} mrh_scene_desc;
+
+typedef struct {
+    float    position[3];
+    float    gaze[3];        /* point looked at */
+    float    up[3];
+    float    fov_x;          /* radians; fov_y follows the aspect ratio */
+    float    near_plane;
+    float    far_plane;
+} mrh_camera_desc;
+
+typedef struct {
+    uint32_t width, height;
+    uint32_t samples;        /* strata per axis ("Samples"), spp = samples^2 */
+    uint32_t max_depth;      /* "MaxDepth" */
+    uint32_t rr_start;       /* "RussianRouletteStart" */
+    uint32_t nee;            /* "NextEventEstimation": only 0 is implemented */
+    uint32_t seed;
+    uint32_t count_traversal; /* 1: also count node visits / triangle tests (slower) */
+} mrh_render_params;
+
+typedef struct {
+    uint64_t segments;        /* rays traced over all bounces */
+    uint32_t bounce_rays[32]; /* live rays per bounce (first 32 bounces) */
+    uint32_t bounces;
+    double   render_ms;       /* wall time of the bounce loop on the stream */
+    uint64_t node_visits;     /* internal-node visits (count_traversal = 1) */
+    uint64_t tri_tests;       /* triangle tests (count_traversal = 1) */
+} mrh_render_stats;
+
+/* Uploads the scene and builds its LBVH (synchronous). */
+int32_t mrh_scene_create(mrh_context* ctx, const mrh_scene_desc* desc, mrh_scene** out);
+int32_t mrh_scene_destroy(mrh_scene* scene);
+int32_t mrh_scene_bvh(const mrh_scene* scene, const mrh_lbvh** out);
+/* Device bytes held by the scene (TracerSystemI/GPUTracerI::TotalGPUMemoryUsed) */
+int32_t mrh_scene_bytes(const mrh_scene* scene, uint64_t* bytes);
+
+/* Renders one frame: d_image (device float[height][width][3], bottom row
+ * first) receives the per-pixel mean radiance.  Synchronous; stats may be NULL. */
+int32_t mrh_render(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                   const mrh_render_params* params, float* d_image, mrh_render_stats* stats);
+
+/* Device bytes mrh_render needs for a frame of this size */
+int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic workloads (host only; no device needed).                   */
+/* ------------------------------------------------------------------ */
+/* Random triangles: centroids U[0,1)^3, corners offset by U(-0.01,0.01)^3;
+ * rays (7 planes) with origins U[0,1)^3 and directions uniform on S^2,
+ * tmax = FLT_MAX.  Either output may be NULL. */
+int32_t mrh_gen_random_tris(uint32_t n, uint32_t seed, float* tris);
+int32_t mrh_gen_random_rays(uint32_t n, uint32_t seed, float* rays);
+
+/* Scene generators fill caller buffers sized by a first call with NULL
+ * buffers (counts written to the out-parameters). */
+typedef struct {
+    float*    tris;
+    uint32_t* prim_key;
+    float*    lambert;
+    float*    metal;
+    float*    dielectric;
+    float*    emissive;
+    uint32_t  n, n_lambert, n_metal, n_dielectric, n_emissive;
+    mrh_camera_desc camera;
+} mrh_scene_buffers;
+
+/* Closed box of 12 triangles (walls albedo 0.75/0.25), icosphere of
+ * 20*4^subdiv triangles (radius 0.3, centred) and a 2-triangle ceiling
+ * light of radiance 15. */
+int32_t mrh_gen_cornell(uint32_t subdiv, mrh_scene_buffers* out);
+/* grid x grid displaced UV spheres (stacks x slices each), materials drawn
+ * from {Lambert, GGX conductor, dielectric} with 5% emissive spheres. */
+int32_t mrh_gen_sphere_grid(uint32_t grid, uint32_t stacks, uint32_t slices, uint32_t seed,
+                            mrh_scene_buffers* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

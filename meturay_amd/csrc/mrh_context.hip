@@ -1,0 +1,199 @@
+// mrh_context.hip — context lifetime, error strings, scratch slots and the
+// hipEvent kernel profiler.
+#include "mrh_internal.hpp"
+
+#include <cstring>
+
+namespace mrh {
+
+int32_t fail(mrh_context* ctx, int32_t code, const char* fmt, ...)
+{
+    if (ctx) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int32_t dmalloc(mrh_context* ctx, void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(ctx, e == hipErrorOutOfMemory ? MRH_ERR_OOM : MRH_ERR_HIP,
+                    "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return MRH_OK;
+}
+
+int32_t scratch(mrh_context* ctx, MrhSlot slot, size_t bytes, void** out)
+{
+    if (bytes > ctx->slot_bytes[slot]) {
+        if (ctx->slot_ptr[slot]) {
+            MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+            MRH_HIP_CHECK(ctx, hipFree(ctx->slot_ptr[slot]));
+            ctx->slot_ptr[slot] = nullptr;
+            ctx->slot_bytes[slot] = 0;
+        }
+        size_t want = bytes + bytes / 8 + 4096;
+        MRH_TRY(dmalloc(ctx, &ctx->slot_ptr[slot], want));
+        ctx->slot_bytes[slot] = want;
+    }
+    *out = ctx->slot_ptr[slot];
+    return MRH_OK;
+}
+
+hipEvent_t prof_begin(mrh_context* ctx)
+{
+    hipEvent_t e = nullptr;
+    if (!ctx->free_events.empty()) {
+        e = ctx->free_events.back();
+        ctx->free_events.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+        return nullptr;
+    }
+    hipEventRecord(e, ctx->stream);
+    return e;
+}
+
+void prof_end(mrh_context* ctx, const char* name, hipEvent_t a)
+{
+    hipEvent_t b = prof_begin(ctx);
+    if (!b) return;
+    ctx->pending.push_back({name, a, b});
+}
+
+int32_t prof_collect(mrh_context* ctx)
+{
+    if (ctx->pending.empty()) return MRH_OK;
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& p : ctx->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto& t = ctx->totals[p.name];
+            t.first += ms;
+            t.second += 1;
+        }
+        ctx->free_events.push_back(p.a);
+        ctx->free_events.push_back(p.b);
+    }
+    ctx->pending.clear();
+    return MRH_OK;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t count)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int32_t fill_u32(mrh_context* ctx, uint32_t* p, uint32_t value, size_t count)
+{
+    if (count == 0) return MRH_OK;
+    return launch(ctx, "fill_u32", dim3(stride_grid(ctx, count, 256)), dim3(256), 0, k_fill_u32, p,
+                  value, count);
+}
+
+}  // namespace mrh
+
+using namespace mrh;
+
+extern "C" {
+
+int32_t mrh_context_create(int32_t device, mrh_context** out)
+{
+    if (!out) return MRH_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return MRH_ERR_HIP;
+    }
+    if (device < 0 || device >= count) return MRH_ERR_INVALID;
+    mrh_context* ctx = new mrh_context();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete ctx; return MRH_ERR_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MRH_ERR_HIP;
+    }
+    *out = ctx;
+    return MRH_OK;
+}
+
+int32_t mrh_context_destroy(mrh_context* ctx)
+{
+    if (!ctx) return MRH_ERR_INVALID;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (int i = 0; i < SLOT_COUNT; i++)
+        if (ctx->slot_ptr[i]) hipFree(ctx->slot_ptr[i]);
+    for (auto& p : ctx->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto e : ctx->free_events) hipEventDestroy(e);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MRH_OK;
+}
+
+const char* mrh_last_error(const mrh_context* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int32_t mrh_sync(mrh_context* ctx)
+{
+    if (!ctx) return MRH_ERR_INVALID;
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return MRH_OK;
+}
+
+int32_t mrh_stream(mrh_context* ctx, void** stream)
+{
+    if (!ctx || !stream) return MRH_ERR_INVALID;
+    *stream = (void*)ctx->stream;
+    return MRH_OK;
+}
+
+int32_t mrh_memcpy(mrh_context* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx || (bytes && (!dst || !src))) return MRH_ERR_INVALID;
+    if (bytes == 0) return MRH_OK;
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return MRH_OK;
+}
+
+int32_t mrh_profile_enable(mrh_context* ctx, int32_t on)
+{
+    if (!ctx) return MRH_ERR_INVALID;
+    ctx->profile = on != 0;
+    return MRH_OK;
+}
+
+int32_t mrh_profile_reset(mrh_context* ctx)
+{
+    if (!ctx) return MRH_ERR_INVALID;
+    MRH_TRY(prof_collect(ctx));
+    ctx->totals.clear();
+    return MRH_OK;
+}
+
+int32_t mrh_profile_get(mrh_context* ctx, const char* kernel, double* total_ms, uint64_t* launches)
+{
+    if (!ctx || !kernel) return MRH_ERR_INVALID;
+    MRH_TRY(prof_collect(ctx));
+    auto it = ctx->totals.find(kernel);
+    if (total_ms) *total_ms = (it == ctx->totals.end()) ? 0.0 : it->second.first;
+    if (launches) *launches = (it == ctx->totals.end()) ? 0 : it->second.second;
+    return MRH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,441 @@
+// mrh_lbvh.hip — Morton codes and the one-pass LBVH builder.
+//
+// Build = 4 kernels + the onesweep sort:
+//   k_tri_bounds   : triangle AABBs + extent (order-preserving integer
+//                    atomicMin/Max, exact and order independent)
+//   k_morton       : quantised centroids -> 30- or 63-bit codes
+//   radix sort     : (code, primitive) ascending, stable
+//   k_lbvh_emit    : one thread per sorted leaf walks up (Apetrei 2014): the
+//                    parent of a range is chosen by the reference's rule, the
+//                    first child to reach a parent stops, the second
+//                    continues, so hierarchy, AABB refit and the traversal
+//                    records are written in a single pass with no relaunch
+//                    loop and no host round trip.
+//   k_lbvh_depth   : deepest leaf, bounds the traversal stack.
+#include "mrh_bvh.hpp"
+
+namespace mrh {
+namespace {
+
+__device__ __forceinline__ uint32_t enc_ordered(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float dec_ordered(uint32_t e)
+{
+    uint32_t b = (e & 0x80000000u) ? (e & 0x7FFFFFFFu) : ~e;
+    return __uint_as_float(b);
+}
+
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p)
+{
+    return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// meta[2..4] = encoded extent min, meta[5..7] = encoded extent max
+__global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ tris, uint32_t n,
+                                                    float* __restrict__ aabb, uint32_t* __restrict__ meta)
+{
+    float mn[3] = {INFINITY, INFINITY, INFINITY};
+    float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* t = tris + (size_t)i * 9;
+        float b[6];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            b[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
+            b[3 + a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
+            mn[a] = fminf(mn[a], b[a]);
+            mx[a] = fmaxf(mx[a], b[3 + a]);
+        }
+        float* o = aabb + (size_t)i * 6;
+#pragma unroll
+        for (int a = 0; a < 6; a++) o[a] = b[a];
+    }
+    // wave reduction then one atomic per wave and axis
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off, 64));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (mn[a] != INFINITY) atomicMin(&meta[2 + a], enc_ordered(mn[a]));
+            if (mx[a] != -INFINITY) atomicMax(&meta[5 + a], enc_ordered(mx[a]));
+        }
+    }
+}
+
+// LinearBVH.hpp:143-165 with IEEE division and the clamp to 2^b - 1;
+// delta = float(max span * 2^-b) as FindMortonDelta (LinearBVH.hpp:400-414).
+template <class CodeT>
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ aabb, uint32_t n,
+                                                const uint32_t* __restrict__ meta, uint32_t bpd,
+                                                CodeT* __restrict__ codes)
+{
+    float emin[3], emax[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        emin[a] = dec_ordered(meta[2 + a]);
+        emax[a] = dec_ordered(meta[5 + a]);
+    }
+    const double res = 1.0 / (double)(1u << bpd);
+    double best = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float span = emax[a] - emin[a];
+        double dd = (double)span * res;
+        best = (dd > best) ? dd : best;
+    }
+    const float delta = (float)best;
+    const uint32_t qmax = (1u << bpd) - 1u;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* b = aabb + (size_t)i * 6;
+        uint32_t q[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float span = b[3 + a] - b[a];
+            float c = b[a] + span * 0.5f;
+            float rel = c - emin[a];
+            uint32_t v = 0;
+            if (delta > 0.0f) {
+                float f = floorf(rel / delta);
+                v = (f >= (float)qmax) ? qmax : (uint32_t)f;
+            }
+            q[a] = v;
+        }
+        if (sizeof(CodeT) == 4)
+            codes[i] = (CodeT)(expand10(q[0]) | (expand10(q[1]) << 1) | (expand10(q[2]) << 2));
+        else
+            codes[i] = (CodeT)(expand21(q[0]) | (expand21(q[1]) << 1) | (expand21(q[2]) << 2));
+    }
+}
+
+// LinearBVH.hpp:30-39 + the (code, index) tie-break for equal codes.
+template <class CodeT>
+__device__ __forceinline__ uint32_t delta_at(const CodeT* __restrict__ c, uint32_t i)
+{
+    CodeT x = c[i] ^ c[i + 1];
+    if (sizeof(CodeT) == 4)
+        return x ? (uint32_t)__clz((uint32_t)x) : 32u + (uint32_t)__clz(i ^ (i + 1u));
+    return x ? (uint32_t)__clzll((unsigned long long)x) : 64u + (uint32_t)__clz(i ^ (i + 1u));
+}
+
+__device__ __forceinline__ uint32_t child_code(uint32_t node, uint32_t n)
+{
+    return (node < n) ? (node | LEAF_BIT) : (node - n);
+}
+
+template <class CodeT>
+__global__ __launch_bounds__(256) void k_lbvh_emit(
+    uint32_t n, const CodeT* __restrict__ scodes, const uint32_t* __restrict__ sidx,
+    const float* __restrict__ leaf_aabb, const float* __restrict__ tris,
+    const uint32_t* __restrict__ prim_key, uint32_t* left, uint32_t* right, uint32_t* parent,
+    float* node_aabb, uint32_t* ranges, uint32_t* counters, float4* __restrict__ tnodes,
+    float4* __restrict__ leaves, uint32_t* meta)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t prim = sidx[i];
+        float box[6];
+#pragma unroll
+        for (int a = 0; a < 6; a++) box[a] = leaf_aabb[(size_t)prim * 6 + a];
+        // leaf records
+        {
+            const float* t = tris + (size_t)prim * 9;
+            const uint32_t key = prim_key ? prim_key[prim] : 0u;
+            leaves[3 * (size_t)i + 0] = make_float4(t[0], t[1], t[2], __uint_as_float(prim));
+            leaves[3 * (size_t)i + 1] = make_float4(t[3], t[4], t[5], __uint_as_float(key));
+            leaves[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+        }
+#pragma unroll
+        for (int a = 0; a < 6; a++) st_agent((uint32_t*)&node_aabb[(size_t)i * 6 + a], __float_as_uint(box[a]));
+
+        uint32_t node = i, l = i, r = i;
+        for (;;) {
+            if (l == 0 && r == n - 1) {
+                st_agent(&parent[node], 0xFFFFFFFFu);
+                meta[0] = node;
+                break;
+            }
+            // ChooseParentAndUpdate (LinearBVH.hpp:43-100)
+            const uint32_t ld = (l == 0) ? 0u : delta_at(scodes, l - 1);
+            const uint32_t rd = (r == n - 1) ? 0u : delta_at(scodes, r);
+            uint32_t p;
+            if (rd > ld) {
+                p = r;
+                st_agent(&left[p], node);
+                st_agent(&ranges[2 * (size_t)p + 0], l);
+            } else {
+                p = l - 1;
+                st_agent(&right[p], node);
+                st_agent(&ranges[2 * (size_t)p + 1], r);
+            }
+            st_agent(&parent[node], n + p);
+            __threadfence();
+            const uint32_t old = atomicAdd(&counters[p], 1u);
+            if (old == 0u) break;   // sibling not done yet: it will continue
+            __threadfence();
+            // second arrival: both children and both range ends are published
+            const uint32_t lc = ld_agent(&left[p]);
+            const uint32_t rc = ld_agent(&right[p]);
+            const uint32_t sib = (lc == node) ? rc : lc;
+            if (sib >= 2 * n - 1) {   // never taken on a correct build; reported to the host
+                atomicOr(&meta[8], 1u);
+                break;
+            }
+            float sbox[6];
+#pragma unroll
+            for (int a = 0; a < 6; a++) sbox[a] = __uint_as_float(ld_agent((const uint32_t*)&node_aabb[(size_t)sib * 6 + a]));
+            const float* lb = (lc == node) ? box : sbox;
+            const float* rb = (lc == node) ? sbox : box;
+            float4* tn = tnodes + 4 * (size_t)p;
+            tn[0] = make_float4(lb[0], lb[1], lb[2], lb[3]);
+            tn[1] = make_float4(lb[4], lb[5], rb[0], rb[1]);
+            tn[2] = make_float4(rb[2], rb[3], rb[4], rb[5]);
+            tn[3] = make_float4(__uint_as_float(child_code(lc, n)), __uint_as_float(child_code(rc, n)),
+                                0.0f, 0.0f);
+            float ub[6];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                ub[a] = fminf(lb[a], rb[a]);
+                ub[3 + a] = fmaxf(lb[3 + a], rb[3 + a]);
+            }
+            node = n + p;
+#pragma unroll
+            for (int a = 0; a < 6; a++) {
+                box[a] = ub[a];
+                st_agent((uint32_t*)&node_aabb[(size_t)node * 6 + a], __float_as_uint(ub[a]));
+            }
+            l = ld_agent(&ranges[2 * (size_t)p + 0]);
+            r = ld_agent(&ranges[2 * (size_t)p + 1]);
+            if (l > r || r >= n) {
+                atomicOr(&meta[8], 2u);
+                break;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lbvh_depth(uint32_t n, const uint32_t* __restrict__ parent,
+                                                    uint32_t* __restrict__ meta)
+{
+    uint32_t best = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t d = 0, x = i;
+        while (parent[x] != 0xFFFFFFFFu && d < 4096u) {
+            x = parent[x];
+            d++;
+        }
+        best = max(best, d);
+    }
+    for (int off = 32; off > 0; off >>= 1) best = max(best, (uint32_t)__shfl_xor((int)best, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&meta[1], best);
+}
+
+template <class T>
+int32_t alloc_into(mrh_lbvh* b, T** p, size_t count)
+{
+    void* q = nullptr;
+    MRH_TRY(dmalloc(b->ctx, &q, count * sizeof(T)));
+    b->allocs.push_back(q);
+    *p = (T*)q;
+    return MRH_OK;
+}
+
+}  // namespace
+
+// Morton codes of n triangles into `codes` (extent + AABBs in scratch).
+int32_t morton_codes(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t bits, void* codes,
+                     float* aabb_out, uint32_t* meta_out)
+{
+    if (n == 0) return MRH_OK;
+    float* aabb = aabb_out;
+    uint32_t* meta = meta_out;
+    if (!aabb) {
+        void* p = nullptr;
+        MRH_TRY(scratch(ctx, SLOT_MORTON_AABB, (size_t)n * 24, &p));
+        aabb = (float*)p;
+    }
+    if (!meta) {
+        void* p = nullptr;
+        MRH_TRY(scratch(ctx, SLOT_MORTON_EXT, 64, &p));
+        meta = (uint32_t*)p;
+    }
+    // meta[0..1] = root/depth, [2..4] = enc(+inf), [5..7] = enc(-inf)
+    MRH_TRY(fill_u32(ctx, meta, 0u, 2));
+    MRH_TRY(fill_u32(ctx, meta + 8, 0u, 8));
+    MRH_TRY(fill_u32(ctx, meta + 2, 0xFFFFFFFFu, 3));
+    MRH_TRY(fill_u32(ctx, meta + 5, 0u, 3));
+    MRH_TRY(launch(ctx, "tri_bounds", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_tri_bounds,
+                   d_tris, n, aabb, meta));
+    if (bits == 30)
+        return launch(ctx, "morton", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_morton<uint32_t>,
+                      (const float*)aabb, n, (const uint32_t*)meta, 10u, (uint32_t*)codes);
+    return launch(ctx, "morton", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_morton<uint64_t>,
+                  (const float*)aabb, n, (const uint32_t*)meta, 21u, (uint64_t*)codes);
+}
+
+int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src)
+{
+    mrh_context* ctx = b->ctx;
+    const uint32_t n = b->n;
+    if (d_tris_src && d_tris_src != b->tris)
+        MRH_HIP_CHECK(ctx, hipMemcpyAsync(b->tris, d_tris_src, (size_t)n * 36,
+                                          hipMemcpyDeviceToDevice, ctx->stream));
+    MRH_TRY(morton_codes(ctx, b->tris, n, b->bits, b->codes, b->leaf_aabb, b->meta));
+    SortArgs sa;
+    sa.keys_in = b->codes;
+    sa.keys_out = b->sorted_codes;
+    sa.vals_out = b->sorted_idx;
+    sa.n_max = n;
+    sa.key_bytes = (b->bits == 30) ? 4 : 8;
+    sa.begin_bit = 0;
+    sa.end_bit = b->bits;
+    MRH_TRY(radix_sort(ctx, sa));
+    if (n > 1) MRH_TRY(fill_u32(ctx, b->counters, 0u, n - 1));
+    const uint32_t grid = stride_grid(ctx, n, 256);
+    if (b->bits == 30)
+        MRH_TRY(launch(ctx, "lbvh_emit", dim3(grid), dim3(256), 0, k_lbvh_emit<uint32_t>, n,
+                       (const uint32_t*)b->sorted_codes, (const uint32_t*)b->sorted_idx,
+                       (const float*)b->leaf_aabb, (const float*)b->tris,
+                       (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
+                       b->ranges, b->counters, b->tnodes, b->leaves, b->meta));
+    else
+        MRH_TRY(launch(ctx, "lbvh_emit", dim3(grid), dim3(256), 0, k_lbvh_emit<uint64_t>, n,
+                       (const uint64_t*)b->sorted_codes, (const uint32_t*)b->sorted_idx,
+                       (const float*)b->leaf_aabb, (const float*)b->tris,
+                       (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
+                       b->ranges, b->counters, b->tnodes, b->leaves, b->meta));
+    MRH_TRY(launch(ctx, "lbvh_depth", dim3(grid), dim3(256), 0, k_lbvh_depth, n,
+                   (const uint32_t*)b->parent, b->meta));
+    uint32_t host_meta[9] = {};
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(host_meta, b->meta, sizeof(host_meta), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    b->root = host_meta[0];
+    b->max_depth = host_meta[1];
+    if (host_meta[8])
+        return fail(ctx, MRH_ERR_HIP, "lbvh build: inconsistent hierarchy (flags 0x%x)", host_meta[8]);
+    if (b->root >= 2 * n - 1)
+        return fail(ctx, MRH_ERR_HIP, "lbvh build produced no root (root = %u, n = %u)", b->root, n);
+    return MRH_OK;
+}
+
+int32_t lbvh_create(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t bits,
+                    const uint32_t* d_prim_key, mrh_lbvh** out)
+{
+    *out = nullptr;
+    if (n == 0) return fail(ctx, MRH_ERR_INVALID, "mrh_lbvh_build: n = 0");
+    if (bits != 30 && bits != 63) return fail(ctx, MRH_ERR_INVALID, "mrh_lbvh_build: bits must be 30 or 63");
+    if (n >= (1u << 30)) return fail(ctx, MRH_ERR_INVALID, "mrh_lbvh_build: n too large");
+    mrh_lbvh* b = new mrh_lbvh();
+    b->ctx = ctx;
+    b->n = n;
+    b->bits = bits;
+    const size_t ni = (n > 1) ? n - 1 : 1;
+    const size_t cb = (bits == 30) ? 4 : 8;
+    int32_t rc = MRH_OK;
+    do {
+        if ((rc = alloc_into(b, &b->tris, (size_t)n * 9)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->leaf_aabb, (size_t)n * 6)) != MRH_OK) break;
+        if ((rc = alloc_into(b, (char**)&b->codes, (size_t)n * cb)) != MRH_OK) break;
+        if ((rc = alloc_into(b, (char**)&b->sorted_codes, (size_t)n * cb)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->sorted_idx, n)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->left, ni)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->right, ni)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->parent, 2 * (size_t)n - 1)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->node_aabb, (2 * (size_t)n - 1) * 6)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->ranges, 2 * ni)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->counters, ni)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->tnodes, 4 * ni)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->leaves, 3 * (size_t)n)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->meta, 16)) != MRH_OK) break;
+        if (d_prim_key) {
+            if ((rc = alloc_into(b, &b->prim_key, n)) != MRH_OK) break;
+            if (hipMemcpyAsync(b->prim_key, d_prim_key, (size_t)n * 4, hipMemcpyDeviceToDevice,
+                               ctx->stream) != hipSuccess) {
+                rc = fail(ctx, MRH_ERR_HIP, "prim key copy failed");
+                break;
+            }
+        }
+        rc = lbvh_build_into(b, d_tris);
+    } while (0);
+    if (rc != MRH_OK) {
+        hipStreamSynchronize(ctx->stream);
+        for (void* p : b->allocs) hipFree(p);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return MRH_OK;
+}
+
+void lbvh_free(mrh_lbvh* b)
+{
+    if (!b) return;
+    if (b->ctx) hipStreamSynchronize(b->ctx->stream);
+    for (void* p : b->allocs) hipFree(p);
+    delete b;
+}
+
+}  // namespace mrh
+
+using namespace mrh;
+
+extern "C" {
+
+int32_t mrh_morton_codes(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t bits, void* d_codes)
+{
+    if (!ctx) return MRH_ERR_INVALID;
+    if (bits != 30 && bits != 63) return fail(ctx, MRH_ERR_INVALID, "mrh_morton_codes: bits must be 30 or 63");
+    if (n && (!d_tris || !d_codes)) return fail(ctx, MRH_ERR_INVALID, "mrh_morton_codes: null pointer");
+    return morton_codes(ctx, d_tris, n, bits, d_codes, nullptr, nullptr);
+}
+
+int32_t mrh_lbvh_build(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t bits, mrh_lbvh** out)
+{
+    if (!ctx || !out) return MRH_ERR_INVALID;
+    if (!d_tris) return fail(ctx, MRH_ERR_INVALID, "mrh_lbvh_build: null triangles");
+    return lbvh_create(ctx, d_tris, n, bits, nullptr, out);
+}
+
+int32_t mrh_lbvh_rebuild(mrh_context* ctx, mrh_lbvh* bvh, const float* d_tris)
+{
+    if (!ctx || !bvh || bvh->ctx != ctx) return MRH_ERR_INVALID;
+    return lbvh_build_into(bvh, d_tris);
+}
+
+int32_t mrh_lbvh_view_get(const mrh_lbvh* b, mrh_lbvh_view* v)
+{
+    if (!b || !v) return MRH_ERR_INVALID;
+    v->n = b->n;
+    v->bits = b->bits;
+    v->root = b->root;
+    v->max_depth = b->max_depth;
+    v->d_sorted_codes = b->sorted_codes;
+    v->d_sorted_idx = b->sorted_idx;
+    v->d_left = b->left;
+    v->d_right = b->right;
+    v->d_parent = b->parent;
+    v->d_node_aabb = b->node_aabb;
+    return MRH_OK;
+}
+
+int32_t mrh_lbvh_destroy(mrh_lbvh* bvh)
+{
+    if (!bvh) return MRH_ERR_INVALID;
+    lbvh_free(bvh);
+    return MRH_OK;
+}
+
+}  // extern "C"

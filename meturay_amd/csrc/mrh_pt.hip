@@ -1,0 +1,759 @@
+// mrh_pt.hip — the plain wavefront path tracer (PathTracer::Render,
+// TracerCUDA/PathTracer.cu:144-291) over resident scenes.
+//
+// Per bounce, with every count kept on the device (no host round trip, so
+// the loop can be captured or replayed without synchronising):
+//   trace_pt   : live rays [0, n) -> leaf, t, barycentrics, HitKey
+//   partition  : one onesweep pass, stable descending by the key's batch
+//                byte -> ray order + batch offsets
+//   shade      : one launch over all partitions; rays are visited in
+//                partition order so a wave nearly always runs one material;
+//                emitters add Le*T to the image, Lambert / GGX conductor /
+//                dielectric sample the next direction, Russian roulette runs
+//                after rr_start, survivors are appended (wave-aggregated
+//                atomics) into the other path buffer, which becomes the next
+//                bounce's contiguous live range.
+// Paths own a PCG32 stream keyed by (seed, global sample id), so results do
+// not depend on the order rays are processed in.
+#include "mrh_bvh.hpp"
+
+#include <cfloat>
+#include <cstring>
+
+struct mrh_scene {
+    mrh_context* ctx = nullptr;
+    mrh_lbvh* bvh = nullptr;
+    float* lambert = nullptr;
+    float* metal = nullptr;
+    float* dielectric = nullptr;
+    float* emissive = nullptr;
+    uint32_t n_lambert = 0, n_metal = 0, n_dielectric = 0, n_emissive = 0;
+    uint64_t bytes = 0;
+    std::vector<void*> allocs;
+};
+
+namespace mrh {
+namespace {
+
+constexpr float INV_PI = 0.31830988618379067154f;
+constexpr float PI_F = 3.14159265358979323846f;
+
+struct Cam {
+    V3 pos, bl, right, up;
+    float pw, ph, near_d, far_d;
+};
+
+struct PathBuf {
+    float *ox, *oy, *oz, *dx, *dy, *dz;
+    float *tr, *tg, *tb;
+    uint32_t* sample;
+    uint32_t* dm;        // depth (bits 0..7) | medium << 8
+    uint64_t* rng;
+};
+
+struct HitBuf {
+    uint32_t* leaf;
+    float *t, *ba, *bb;
+    uint32_t* key;
+};
+
+struct Materials {
+    const float* lambert;
+    const float* metal;
+    const float* dielectric;
+    const float* emissive;
+};
+
+// GPUCameraPinhole.cuh:140-170, evaluated on the host with the same float
+// operations as the oracle.
+Cam make_camera(const mrh_camera_desc& c, uint32_t w, uint32_t h)
+{
+    float fov_y = (float)(2.0 * atan(tan((double)c.fov_x * 0.5) * (double)h / (double)w));
+    V3 p = v3(c.position[0], c.position[1], c.position[2]);
+    V3 g = v3(c.gaze[0], c.gaze[1], c.gaze[2]);
+    V3 u = v3(c.up[0], c.up[1], c.up[2]);
+    float wh = tanf(c.fov_x * 0.5f) * c.near_plane;
+    float hh = tanf(fov_y * 0.5f) * c.near_plane;
+    V3 gd = g - p;
+    V3 right = normalize(cross(gd, u));
+    u = normalize(cross(right, gd));
+    gd = normalize(cross(u, right));
+    Cam cam;
+    cam.pos = p;
+    cam.bl = ((p - scale(right, wh)) - scale(u, hh)) + scale(gd, c.near_plane);
+    cam.right = right;
+    cam.up = u;
+    cam.pw = wh * 2.0f;
+    cam.ph = hh * 2.0f;
+    cam.near_d = c.near_plane;
+    cam.far_d = c.far_plane;
+    return cam;
+}
+
+// GenerationKernels.cuh:29-108 + GPUCameraPinhole.cuh:209-240
+__global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h, uint32_t S,
+                                                uint32_t seed, uint32_t n_total, PathBuf out)
+{
+    const uint32_t spp = S * S;
+    const float dxp = cam.pw / (float)(w * S);
+    const float dyp = cam.ph / (float)(h * S);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_total; i += gridDim.x * blockDim.x) {
+        const uint32_t pix = i / spp, s = i - pix * spp;
+        const uint32_t px = pix % w, py = pix / w, sx = s % S, sy = s / S;
+        Rng rng{path_rng_init(seed, i), pcg32_inc(i)};
+        const float jx = rng.uniform();
+        const float jy = rng.uniform();
+        float sdx = (float)(px * S + sx) * dxp;
+        float sdy = (float)(py * S + sy) * dyp;
+        sdx += jx * dxp;
+        sdy += jy * dyp;
+        const V3 sp = cam.bl + (scale(cam.right, sdx) + scale(cam.up, sdy));
+        const V3 d = normalize(sp - cam.pos);
+        out.ox[i] = cam.pos.x; out.oy[i] = cam.pos.y; out.oz[i] = cam.pos.z;
+        out.dx[i] = d.x; out.dy[i] = d.y; out.dz[i] = d.z;
+        out.tr[i] = 1.0f; out.tg[i] = 1.0f; out.tb[i] = 1.0f;
+        out.sample[i] = i;
+        out.dm[i] = 1u;
+        out.rng[i] = rng.state;
+    }
+}
+
+// counts layout: [0, 64) live rays per bounce, [64, 66) u64 node visits,
+// [66, 68) u64 triangle tests
+constexpr uint32_t COUNTS_WORDS = 128;
+
+template <int STACK, bool STATS>
+__global__ __launch_bounds__(256) void k_trace_pt(const float4* __restrict__ tnodes,
+                                                  const float4* __restrict__ leaves, uint32_t root, uint32_t n_leaves,
+                                                  PathBuf in, uint32_t* __restrict__ counts,
+                                                  uint32_t bounce, float tmin, float tmax, HitBuf hit)
+{
+    const uint32_t n = counts[bounce];
+    uint64_t visits = 0, tests = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const V3 o = v3(in.ox[i], in.oy[i], in.oz[i]);
+        const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
+        const TraceResult r = traverse<false, STATS, STACK>(tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+        hit.leaf[i] = r.leaf;
+        hit.t[i] = r.t;
+        hit.ba[i] = r.ba;
+        hit.bb[i] = r.bb;
+        hit.key[i] = (r.leaf == NO_HIT) ? 0u : __float_as_uint(leaves[3 * (size_t)r.leaf + 1].w);
+        if (STATS) {
+            visits += r.visits;
+            tests += r.tests;
+        }
+    }
+    if (STATS) {
+        for (int off = 32; off > 0; off >>= 1) {
+            visits += __shfl_xor(visits, off, 64);
+            tests += __shfl_xor(tests, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd((unsigned long long*)(counts + 64), (unsigned long long)visits);
+            atomicAdd((unsigned long long*)(counts + 66), (unsigned long long)tests);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ BxDFs
+// TracerFunctions.cuh:8-30
+__device__ __forceinline__ float fresnel_dielectric(float cos_in, float ior_in, float ior_out)
+{
+    float sin_in = sqrtf(fmaxf(0.0f, 1.0f - cos_in * cos_in));
+    float sin_out = ior_in / ior_out * sin_in;
+    if (sin_out >= 1.0f) return 1.0f;
+    float cos_out = sqrtf(fmaxf(0.0f, 1.0f - sin_out * sin_out));
+    float par = ((ior_out * cos_in - ior_in * cos_out) / (ior_out * cos_in + ior_in * cos_out));
+    par = par * par;
+    float perp = ((ior_in * cos_in - ior_out * cos_out) / (ior_in * cos_in + ior_out * cos_out));
+    perp = perp * perp;
+    return (par + perp) * 0.5f;
+}
+
+// TracerFunctions.cuh:33-79, one channel
+__device__ __forceinline__ float fresnel_conductor(float cos_in, float eta, float k)
+{
+    float cos2 = cos_in * cos_in;
+    float sin2 = fmaxf(0.0f, 1.0f - cos2);
+    float sin2_2 = sin2 * sin2;
+    float sin2_4 = sin2_2 * sin2_2;
+    float eta2 = eta * eta;
+    float k2 = k * k;
+    float diff = eta2 - k2 - sin2;
+    float a2b2 = diff * diff;
+    a2b2 += 4.0f * eta2 * k2;
+    a2b2 = sqrtf(fmaxf(a2b2, 0.0f));
+    float a = 0.5f * (a2b2 + diff);
+    a = sqrtf(fmaxf(a, 0.0f));
+    float pt1 = a2b2 + cos2;
+    float pt2 = a * (2.0f * cos_in);
+    float perp = (pt1 - pt2) / (pt1 + pt2);
+    float qt1 = a2b2 * cos2 + sin2_4;
+    float qt2 = pt2 * sin2_2;
+    float par = perp * (qt1 - qt2) / (qt1 + qt2);
+    return (par + perp) * 0.5f;
+}
+
+// TracerFunctions.cuh:81-90
+__device__ __forceinline__ float dggx(float ndh, float alpha)
+{
+    float a2 = alpha * alpha;
+    float denom = ndh * ndh * (a2 - 1.0f) + 1.0f;
+    denom = denom * denom;
+    denom *= PI_F;
+    return a2 / denom;
+}
+
+// TracerFunctions.cuh:92-116
+__device__ __forceinline__ float lambda_smith(V3 v, float alpha)
+{
+    V3 s = mul(v, v);
+    float a2 = alpha * alpha;
+    float inner = a2 * (s.x + s.y) / s.z;
+    float l = sqrtf(1.0f + inner) - 1.0f;
+    l *= 0.5f;
+    return l;
+}
+__device__ __forceinline__ float gsmith_single(V3 v, float alpha) { return 1.0f / (1.0f + lambda_smith(v, alpha)); }
+__device__ __forceinline__ float gsmith_corr(V3 wo, V3 wi, float alpha)
+{
+    return 1.0f / (lambda_smith(wo, alpha) + lambda_smith(wi, alpha) + 1.0f);
+}
+
+// TracerFunctions.cuh:186-196
+__device__ __forceinline__ float vndf_pdf(V3 v, V3 h, float alpha)
+{
+    float vdh = fmaxf(0.0f, dot(h, v));
+    float ndh = fmaxf(0.0f, h.z);
+    float ndv = fmaxf(0.0f, v.z);
+    float dd = dggx(ndh, alpha);
+    float g1 = gsmith_single(v, alpha);
+    return (ndv == 0.0f) ? 0.0f : (vdh * dd * g1 / ndv);
+}
+
+// TracerFunctions.cuh:198-258 (Heitz 2018)
+__device__ __forceinline__ V3 vndf_sample(V3 v, float a, float xi0, float xi1, float& pdf)
+{
+    V3 vh = normalize(v3(a * v.x, a * v.y, v.z));
+    float lensq = 0.0f;
+    lensq += vh.x * vh.x;
+    lensq += vh.y * vh.y;
+    V3 t1v = (lensq > 0.0f) ? divs(v3(-vh.y, vh.x, 0.0f), sqrtf(lensq)) : v3(1.0f, 0.0f, 0.0f);
+    V3 t2v = cross(vh, t1v);
+    float r = sqrtf(xi0);
+    float sp, cp;
+    sincos_2pi(xi1, sp, cp);
+    float t1 = r * cp;
+    float t2 = r * sp;
+    float s = 0.5f * (1.0f + vh.z);
+    t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
+    float val = 1.0f - t1 * t1 - t2 * t2;
+    float sq = sqrtf(fmaxf(0.0f, val));
+    V3 nh = (scale(t1v, t1) + scale(t2v, t2)) + scale(vh, sq);
+    V3 nm = v3(a * nh.x, a * nh.y, fmaxf(0.0f, nh.z));
+    float len2 = dot(nm, nm);
+    if (len2 < 1.0e-5f) nm = v3(0.0f, 0.0f, 1.0f);
+    else nm = scale(nm, 1.0f / sqrtf(len2));
+    pdf = vndf_pdf(v, nm, a);
+    return nm;
+}
+
+struct Surface {
+    V3 pos, geo_n;
+    Q4 w2t;
+    bool back;
+};
+
+// GPUPrimitiveTriangle.h:626-670 for flat-shaded two-sided triangles (the
+// face frame conj(RotationBetweenZAxis(n)) at every corner).
+__device__ __forceinline__ Surface make_surface(V3 p0, V3 p1, V3 p2, float ba, float bb, V3 dir)
+{
+    Surface s;
+    float c = 1.0f - ba - bb;
+    s.pos = (scale(p0, ba) + scale(p1, bb)) + scale(p2, c);
+    V3 n = normalize(cross(p1 - p0, p2 - p0));
+    s.w2t = conj(rot_between_z(n));
+    s.back = dot(n, dir) > 0.0f;
+    if (s.back) {
+        n = -n;
+        s.w2t = qmul(qmake(0.0f, 1.0f, 0.0f, 0.0f), s.w2t);
+    }
+    s.geo_n = n;
+    return s;
+}
+
+struct BsdfOut {
+    V3 o, d, refl;
+    float pdf;
+    bool specular;
+    uint32_t medium;
+};
+
+// SimpleMaterialsKC.cuh:16-60
+__device__ __forceinline__ void lambert_sample(const Surface& s, V3 albedo, Rng& rng, BsdfOut& out)
+{
+    const V3 normal = rotate(conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
+    const float xi0 = rng.uniform();
+    const float xi1 = rng.uniform();
+    const float c0 = sqrtf(fmaxf(0.0f, 1.0f - xi0 * xi0));
+    float sn, cs;
+    sincos_2pi(xi1, sn, cs);
+    V3 dir = normalize(v3(c0 * cs, c0 * sn, xi0));
+    dir = rotate(rot_between_z(normal), dir);
+    const float ndl = fabsf(dot(normal, dir));
+    out.o = nudge(s.pos, s.geo_n);
+    out.d = dir;
+    out.refl = scale(scale(albedo, ndl), INV_PI);
+    out.pdf = INV_PI * 0.5f;
+    out.specular = false;
+}
+
+// MetalGlassMaterialsKC.cuh:24-150
+__device__ __forceinline__ void metal_sample(const Surface& s, V3 wi, const float* __restrict__ m,
+                                             Rng& rng, BsdfOut& out)
+{
+    const float rough = m[0];
+    const float alpha = rough * rough;
+    const V3 V = rotate(s.w2t, wi);
+    const float xi0 = rng.uniform();
+    const float xi1 = rng.uniform();
+    float pdf;
+    const V3 H = vndf_sample(V, alpha, xi0, xi1, pdf);
+    const float vdh = fmaxf(0.0f, dot(V, H));
+    V3 L = scale(H, 2.0f * vdh) - V;
+    pdf = (vdh == 0.0f) ? 0.0f : pdf / (4.0f * vdh);
+    const float ldh = fmaxf(0.0f, dot(L, H));
+    const float ndh = fmaxf(0.0f, H.z);
+    const float ndv = fmaxf(0.0f, V.z);
+    V3 F = v3(fresnel_conductor(vdh, m[1], m[4]), fresnel_conductor(vdh, m[2], m[5]),
+              fresnel_conductor(vdh, m[3], m[6]));
+    F = mul(F, v3(m[7], m[8], m[9]));
+    const float D = dggx(ndh, alpha);
+    float G = gsmith_corr(V, L, alpha);
+    G = (ldh == 0.0f) ? 0.0f : G;
+    G = (vdh == 0.0f) ? 0.0f : G;
+    V3 spec = v3(((D * F.x) * G) * 0.25f, ((D * F.y) * G) * 0.25f, ((D * F.z) * G) * 0.25f);
+    spec = divs(spec, ndv);
+    if (ndv == 0.0f) spec = v3(0.0f, 0.0f, 0.0f);
+    if (isinf(D) || isnan(D)) {
+        if (rough == 0.0f) L = scale(v3(0.0f, 0.0f, 1.0f), 2.0f * V.z) - V;
+        spec = scale(F, G);
+        const float gs = gsmith_single(V, alpha);
+        spec = (gs != 0.0f) ? divs(spec, gs) : v3(0.0f, 0.0f, 0.0f);
+        pdf = 1.0f;
+    }
+    out.d = rotate(conj(s.w2t), L);
+    out.o = nudge(s.pos, s.geo_n);
+    out.refl = spec;
+    out.pdf = pdf;
+    out.specular = (1.0f - rough) >= 0.99f;
+}
+
+// SimpleMaterialsKC.cuh:172-290; medium 0 = vacuum, k+1 = inside dielectric k
+__device__ __forceinline__ void dielectric_sample(const Surface& s, V3 wi, const float* __restrict__ m,
+                                                  uint32_t mat_id, uint32_t medium,
+                                                  const float* __restrict__ diel, Rng& rng, BsdfOut& out)
+{
+    const V3 albedo = v3(m[0], m[1], m[2]);
+    const float ior = m[3];
+    const V3 normal = rotate(conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
+    const bool entering = !s.back;
+    const float from = (medium == 0u) ? 1.0f : diel[(size_t)(medium - 1u) * 4 + 3];
+    const float to = entering ? ior : 1.0f;
+    const float f = fresnel_dielectric(fabsf(dot(wi, normal)), from, to);
+    const float xi = rng.uniform();
+    out.specular = true;
+    if (xi < f) {
+        const float k = 2.0f * dot(wi, normal);
+        out.d = scale(normal, k) - wi;
+        out.o = nudge(s.pos, s.geo_n);
+        out.pdf = f;
+        out.medium = medium;
+        out.refl = scale(albedo, f);
+    } else {
+        out.medium = entering ? (mat_id + 1u) : 0u;
+        out.pdf = 1.0f - f;
+        const float ratio = from / to;
+        const float cos_in = dot(normal, wi);
+        const float sin_in2 = fmaxf(0.0f, 1.0f - cos_in * cos_in);
+        const float sin_out2 = ratio * ratio * sin_in2;
+        const float cos_out = sqrtf(fmaxf(0.0f, 1.0f - sin_out2));
+        if (sin_out2 >= 1.0f) {
+            out.pdf = 0.0f;
+            out.refl = v3(0.0f, 0.0f, 0.0f);
+            out.d = wi;
+            out.o = s.pos;
+            return;
+        }
+        const float kk = ratio * cos_in - cos_out;
+        out.d = scale(-wi, ratio) + scale(normal, kk);
+        out.o = nudge(s.pos, -s.geo_n);
+        out.refl = scale(scale(albedo, 1.0f), 1.0f - f);
+    }
+}
+
+// PathTracerKC.cuh:337-426 (light hit) and 430-648 (surface work)
+__global__ __launch_bounds__(256) void k_shade(
+    PathBuf in, PathBuf out, HitBuf hit, const uint32_t* __restrict__ order, uint32_t* counts,
+    uint32_t bounce, uint32_t last, uint32_t rr_start, uint32_t spp, Materials mats,
+    const float4* __restrict__ leaves, float* __restrict__ image)
+{
+    const uint32_t n = counts[bounce];
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lanes_below = (1ull << lane) - 1ull;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t j = base + threadIdx.x;
+        bool alive = false;
+        V3 no = v3(0.0f, 0.0f, 0.0f), nd = no, nT = no;
+        uint32_t sample = 0, ndm = 0;
+        uint64_t nrng = 0;
+        if (j < n) {
+            const uint32_t rid = order[j];
+            const uint32_t key = hit.key[rid];
+            const uint32_t batch = key >> 24, id = key & 0xFFFFFFu;
+            if (batch != MRH_BATCH_NULL) {
+                const V3 T = v3(in.tr[rid], in.tg[rid], in.tb[rid]);
+                sample = in.sample[rid];
+                if (batch == MRH_BATCH_EMISSIVE) {
+                    const float* le = mats.emissive + (size_t)id * 3;
+                    float* px = image + (size_t)(sample / spp) * 3;
+                    atomicAdd(px + 0, le[0] * T.x);
+                    atomicAdd(px + 1, le[1] * T.y);
+                    atomicAdd(px + 2, le[2] * T.z);
+                } else if (!last) {
+                    const uint32_t li = hit.leaf[rid];
+                    const float4 l0 = leaves[3 * (size_t)li + 0];
+                    const float4 l1 = leaves[3 * (size_t)li + 1];
+                    const float4 l2 = leaves[3 * (size_t)li + 2];
+                    const V3 d = v3(in.dx[rid], in.dy[rid], in.dz[rid]);
+                    const Surface s = make_surface(v3(l0.x, l0.y, l0.z), v3(l1.x, l1.y, l1.z),
+                                                   v3(l2.x, l2.y, l2.z), hit.ba[rid], hit.bb[rid], d);
+                    const V3 wi = -normalize(d);
+                    Rng rng{in.rng[rid], pcg32_inc(sample)};
+                    const uint32_t dm = in.dm[rid];
+                    const uint32_t depth = dm & 0xFFu, medium = dm >> 8;
+                    BsdfOut o;
+                    o.medium = medium;
+                    if (batch == MRH_BATCH_LAMBERT) {
+                        const float* a = mats.lambert + (size_t)id * 3;
+                        lambert_sample(s, v3(a[0], a[1], a[2]), rng, o);
+                    } else if (batch == MRH_BATCH_METAL) {
+                        metal_sample(s, wi, mats.metal + (size_t)id * 10, rng, o);
+                    } else {
+                        dielectric_sample(s, wi, mats.dielectric + (size_t)id * 4, id, medium,
+                                          mats.dielectric, rng, o);
+                    }
+                    V3 t2 = mul(T, o.refl);
+                    t2 = (o.pdf == 0.0f) ? v3(0.0f, 0.0f, 0.0f) : divs(t2, o.pdf);
+                    const float avg = dot(t2, v3(0.333f, 0.333f, 0.333f));
+                    bool terminate = false;
+                    if (!o.specular && depth > rr_start) {
+                        // TracerFunctions.cuh:321-330
+                        const float pr = avg < 0.005f ? 0.005f : (avg > 1.0f ? 1.0f : avg);
+                        if (rng.uniform() >= pr) terminate = true;
+                        else t2 = scale(t2, 1.0f / pr);
+                    }
+                    alive = !((terminate && !o.specular) || is_zero(t2));
+                    no = o.o;
+                    nd = o.d;
+                    nT = t2;
+                    ndm = ((depth + 1u) & 0xFFu) | (o.medium << 8);
+                    nrng = rng.state;
+                }
+            }
+        }
+        // wave-aggregated append into the next bounce's live range
+        const uint64_t m = __ballot(alive);
+        if (m) {
+            const uint32_t leader = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+            uint32_t slot0 = 0;
+            if (lane == leader) slot0 = atomicAdd(&counts[bounce + 1], (uint32_t)__popcll(m));
+            slot0 = __shfl(slot0, (int)leader, 64);
+            if (alive) {
+                const uint32_t k = slot0 + (uint32_t)__popcll(m & lanes_below);
+                out.ox[k] = no.x; out.oy[k] = no.y; out.oz[k] = no.z;
+                out.dx[k] = nd.x; out.dy[k] = nd.y; out.dz[k] = nd.z;
+                out.tr[k] = nT.x; out.tg[k] = nT.y; out.tb[k] = nT.z;
+                out.sample[k] = sample;
+                out.dm[k] = ndm;
+                out.rng[k] = nrng;
+            }
+        }
+    }
+}
+
+__global__ void k_scale(float* __restrict__ p, size_t count, float s)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = p[i] * s;
+}
+
+constexpr uint32_t MAX_BOUNCES = 62;
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t frame_bytes(uint64_t cap)
+{
+    // 2 path buffers (9 floats + sample + dm + u64 rng = 13 words),
+    // hits (4 words) + key + order, plus counts/offsets
+    size_t per = 0;
+    per += 2 * (11 * align_up(cap * 4) + align_up(cap * 8));
+    per += 6 * align_up(cap * 4);
+    per += 4096;
+    return per;
+}
+
+int32_t carve_frame(mrh_context* ctx, uint64_t cap, PathBuf* bufs, HitBuf* hit, uint32_t** order,
+                    uint32_t** counts, uint32_t** offsets)
+{
+    void* base = nullptr;
+    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(cap), &base));
+    char* p = (char*)base;
+    auto take = [&](size_t bytes) {
+        char* r = p;
+        p += align_up(bytes);
+        return r;
+    };
+    for (int b = 0; b < 2; b++) {
+        PathBuf& q = bufs[b];
+        q.ox = (float*)take(cap * 4); q.oy = (float*)take(cap * 4); q.oz = (float*)take(cap * 4);
+        q.dx = (float*)take(cap * 4); q.dy = (float*)take(cap * 4); q.dz = (float*)take(cap * 4);
+        q.tr = (float*)take(cap * 4); q.tg = (float*)take(cap * 4); q.tb = (float*)take(cap * 4);
+        q.sample = (uint32_t*)take(cap * 4);
+        q.dm = (uint32_t*)take(cap * 4);
+        q.rng = (uint64_t*)take(cap * 8);
+    }
+    hit->leaf = (uint32_t*)take(cap * 4);
+    hit->t = (float*)take(cap * 4);
+    hit->ba = (float*)take(cap * 4);
+    hit->bb = (float*)take(cap * 4);
+    hit->key = (uint32_t*)take(cap * 4);
+    *order = (uint32_t*)take(cap * 4);
+    *counts = (uint32_t*)take(COUNTS_WORDS * 4);
+    *offsets = (uint32_t*)take(260 * 4);
+    return MRH_OK;
+}
+
+template <class T>
+int32_t scene_upload(mrh_scene* s, T** dst, const T* src, size_t count)
+{
+    *dst = nullptr;
+    size_t bytes = count * sizeof(T);
+    void* p = nullptr;
+    MRH_TRY(dmalloc(s->ctx, &p, bytes ? bytes : 16));
+    s->allocs.push_back(p);
+    s->bytes += bytes;
+    if (bytes) MRH_HIP_CHECK(s->ctx, hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    *dst = (T*)p;
+    return MRH_OK;
+}
+
+void scene_free(mrh_scene* s)
+{
+    if (!s) return;
+    if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+    if (s->bvh) lbvh_free(s->bvh);
+    for (void* p : s->allocs) hipFree(p);
+    delete s;
+}
+
+int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
+{
+    if (d->n == 0 || !d->tris || !d->prim_key)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: empty scene");
+    // validate keys against the material tables (SceneError on bad ids)
+    for (uint32_t i = 0; i < d->n; i++) {
+        uint32_t b = d->prim_key[i] >> 24, id = d->prim_key[i] & 0xFFFFFFu;
+        uint32_t lim = (b == MRH_BATCH_LAMBERT) ? d->n_lambert
+                     : (b == MRH_BATCH_METAL) ? d->n_metal
+                     : (b == MRH_BATCH_DIELECTRIC) ? d->n_dielectric
+                     : (b == MRH_BATCH_EMISSIVE) ? d->n_emissive : 0u;
+        if (b == MRH_BATCH_NULL || b > MRH_BATCH_EMISSIVE || id >= lim)
+            return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: primitive %u has invalid key 0x%08x", i,
+                        d->prim_key[i]);
+    }
+    mrh_scene* s = new mrh_scene();
+    s->ctx = ctx;
+    s->n_lambert = d->n_lambert;
+    s->n_metal = d->n_metal;
+    s->n_dielectric = d->n_dielectric;
+    s->n_emissive = d->n_emissive;
+    float* dtris = nullptr;
+    uint32_t* dkeys = nullptr;
+    int32_t rc = MRH_OK;
+    do {
+        if ((rc = scene_upload(s, &s->lambert, d->lambert, (size_t)d->n_lambert * 3)) != MRH_OK) break;
+        if ((rc = scene_upload(s, &s->metal, d->metal, (size_t)d->n_metal * 10)) != MRH_OK) break;
+        if ((rc = scene_upload(s, &s->dielectric, d->dielectric, (size_t)d->n_dielectric * 4)) != MRH_OK) break;
+        if ((rc = scene_upload(s, &s->emissive, d->emissive, (size_t)d->n_emissive * 3)) != MRH_OK) break;
+        if ((rc = scene_upload(s, &dtris, d->tris, (size_t)d->n * 9)) != MRH_OK) break;
+        if ((rc = scene_upload(s, &dkeys, d->prim_key, (size_t)d->n)) != MRH_OK) break;
+        uint32_t bits = d->morton_bits ? d->morton_bits : 30u;
+        if ((rc = lbvh_create(ctx, dtris, d->n, bits, dkeys, &s->bvh)) != MRH_OK) break;
+        s->bytes += (size_t)d->n * (36 + 24 + 8 + 8 + 4 + 12 + 8 + 48 + 64 + 4);
+    } while (0);
+    if (rc != MRH_OK) {
+        scene_free(s);
+        return rc;
+    }
+    *out = s;
+    return MRH_OK;
+}
+
+int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const mrh_render_params* p,
+               float* d_image, mrh_render_stats* stats)
+{
+    if (p->nee) return fail(ctx, MRH_ERR_UNSUPPORTED, "NextEventEstimation is not implemented");
+    if (p->width == 0 || p->height == 0 || p->samples == 0)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render: empty frame");
+    if (p->max_depth == 0 || p->max_depth > MAX_BOUNCES)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render: max_depth must be in [1, %u]", MAX_BOUNCES);
+    const uint64_t spp = (uint64_t)p->samples * p->samples;
+    const uint64_t total = (uint64_t)p->width * p->height * spp;
+    if (total >= (1ull << 30)) return fail(ctx, MRH_ERR_INVALID, "mrh_render: %llu paths exceed 2^30",
+                                           (unsigned long long)total);
+    const mrh_lbvh* b = s->bvh;
+    if (b->max_depth > 128)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "tree depth %u exceeds the traversal stack", b->max_depth);
+
+    PathBuf bufs[2];
+    HitBuf hit;
+    uint32_t *order = nullptr, *counts = nullptr, *offsets = nullptr;
+    MRH_TRY(carve_frame(ctx, total, bufs, &hit, &order, &counts, &offsets));
+    const size_t img_count = (size_t)p->width * p->height * 3;
+    MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
+    MRH_TRY(fill_u32(ctx, counts, 0u, COUNTS_WORDS));
+    MRH_TRY(fill_u32(ctx, counts, (uint32_t)total, 1));
+
+    const Cam cam = make_camera(*cd, p->width, p->height);
+    const Materials mats{s->lambert, s->metal, s->dielectric, s->emissive};
+    const uint32_t root = root_code(b);
+    const uint32_t grid = stride_grid(ctx, total, 256);
+
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MRH_HIP_CHECK(ctx, hipEventCreate(&e0));
+    MRH_HIP_CHECK(ctx, hipEventCreate(&e1));
+    MRH_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
+    int32_t rc = launch(ctx, "raygen", dim3(grid), dim3(256), 0, k_raygen, cam, p->width, p->height,
+                        p->samples, p->seed, (uint32_t)total, bufs[0]);
+    for (uint32_t bounce = 0; rc == MRH_OK && bounce < p->max_depth; bounce++) {
+        const PathBuf& in = bufs[bounce & 1];
+        const PathBuf& outb = bufs[(bounce + 1) & 1];
+        const float tmin = (bounce == 0) ? cam.near_d : 0.0f;
+        const float tmax = (bounce == 0) ? cam.far_d : FLT_MAX;
+        const float4* tn = (const float4*)b->tnodes;
+        const float4* lv = (const float4*)b->leaves;
+        if (p->count_traversal) {
+            if (b->max_depth <= 64)
+                rc = launch(ctx, "trace_pt_stats", dim3(grid), dim3(256), 0, k_trace_pt<64, true>, tn, lv,
+                            root, b->n, in, counts, bounce, tmin, tmax, hit);
+            else
+                rc = launch(ctx, "trace_pt_stats", dim3(grid), dim3(256), 0, k_trace_pt<128, true>, tn, lv,
+                            root, b->n, in, counts, bounce, tmin, tmax, hit);
+        } else if (b->max_depth <= 64) {
+            rc = launch(ctx, "trace_pt", dim3(grid), dim3(256), 0, k_trace_pt<64, false>, tn, lv, root, b->n,
+                        in, counts, bounce, tmin, tmax, hit);
+        } else {
+            rc = launch(ctx, "trace_pt", dim3(grid), dim3(256), 0, k_trace_pt<128, false>, tn, lv, root, b->n,
+                        in, counts, bounce, tmin, tmax, hit);
+        }
+        if (rc != MRH_OK) break;
+        SortArgs sa;
+        sa.keys_in = hit.key;
+        sa.vals_out = order;
+        sa.n_max = (uint32_t)total;
+        sa.n_dev = counts + bounce;
+        sa.key_bytes = 4;
+        sa.begin_bit = 24;
+        sa.end_bit = 32;
+        sa.descending = true;
+        sa.part_offsets = offsets;
+        sa.part_begin = 24;
+        if ((rc = radix_sort(ctx, sa)) != MRH_OK) break;
+        rc = launch(ctx, "shade", dim3(grid), dim3(256), 0, k_shade, in, outb, hit, (const uint32_t*)order,
+                    counts, bounce, (uint32_t)(bounce + 1 == p->max_depth), p->rr_start, (uint32_t)spp,
+                    mats, (const float4*)b->leaves, d_image);
+    }
+    if (rc == MRH_OK)
+        rc = launch(ctx, "finalize", dim3(stride_grid(ctx, img_count, 256)), dim3(256), 0, k_scale, d_image,
+                    img_count, 1.0f / (float)spp);
+    if (rc == MRH_OK) {
+        hipEventRecord(e1, ctx->stream);
+        uint32_t hc[68];
+        hipError_t e = hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "render: %s", hipGetErrorString(e));
+        if (rc == MRH_OK && stats) {
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            memset(stats, 0, sizeof(*stats));
+            stats->render_ms = ms;
+            stats->bounces = p->max_depth;
+            for (uint32_t i = 0; i < p->max_depth; i++) {
+                stats->segments += hc[i];
+                if (i < 32) stats->bounce_rays[i] = hc[i];
+            }
+            stats->node_visits = (uint64_t)hc[64] | ((uint64_t)hc[65] << 32);
+            stats->tri_tests = (uint64_t)hc[66] | ((uint64_t)hc[67] << 32);
+        }
+    } else {
+        hipStreamSynchronize(ctx->stream);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+}  // namespace
+}  // namespace mrh
+
+using namespace mrh;
+
+extern "C" {
+
+int32_t mrh_scene_create(mrh_context* ctx, const mrh_scene_desc* desc, mrh_scene** out)
+{
+    if (!ctx || !desc || !out) return MRH_ERR_INVALID;
+    *out = nullptr;
+    return scene_create(ctx, desc, out);
+}
+
+int32_t mrh_scene_destroy(mrh_scene* scene)
+{
+    if (!scene) return MRH_ERR_INVALID;
+    scene_free(scene);
+    return MRH_OK;
+}
+
+int32_t mrh_scene_bvh(const mrh_scene* scene, const mrh_lbvh** out)
+{
+    if (!scene || !out) return MRH_ERR_INVALID;
+    *out = scene->bvh;
+    return MRH_OK;
+}
+
+int32_t mrh_scene_bytes(const mrh_scene* scene, uint64_t* bytes)
+{
+    if (!scene || !bytes) return MRH_ERR_INVALID;
+    *bytes = scene->bytes;
+    return MRH_OK;
+}
+
+int32_t mrh_render(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                   const mrh_render_params* params, float* d_image, mrh_render_stats* stats)
+{
+    if (!ctx || !scene || !cam || !params || !d_image) return MRH_ERR_INVALID;
+    if (scene->ctx != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_render: scene belongs to another context");
+    return render(ctx, scene, cam, params, d_image, stats);
+}
+
+int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes)
+{
+    if (!params || !bytes) return MRH_ERR_INVALID;
+    const uint64_t total = (uint64_t)params->width * params->height * params->samples * params->samples;
+    *bytes = frame_bytes(total) + total * 16;   // + sort scratch
+    return MRH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,344 @@
+"""ctypes binding of libmrh.so, the C-ABI in include/mrh.h.
+
+The library must have been built (``python -m meturay_amd.build``); loading
+fails loudly otherwise — there is no fallback path.  Device buffers are
+passed as raw pointers; the helpers below accept torch CUDA tensors (torch is
+only the device-memory allocator here) or integers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libmrh.so")
+HEADER = os.path.join(ROOT, "include", "mrh.h")
+
+OK, ERR_INVALID, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_BUDGET = range(6)
+BATCH_NULL, BATCH_LAMBERT, BATCH_METAL, BATCH_DIELECTRIC, BATCH_EMISSIVE = range(5)
+NO_HIT = 0xFFFFFFFF
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_I32 = C.c_int32
+_F = C.c_float
+
+
+def key(batch: int, idx: int) -> int:
+    """HitKey = batch:8 | id:24 (RayLib/HitStructs.h:42-77)."""
+    return ((batch & 0xFF) << 24) | (idx & 0xFFFFFF)
+
+
+class MrhError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mrh error {code}: {msg}")
+        self.code = code
+
+
+class LbvhView(C.Structure):
+    _fields_ = [("n", _U32), ("bits", _U32), ("root", _U32), ("max_depth", _U32),
+                ("d_sorted_codes", _P), ("d_sorted_idx", _P), ("d_left", _P), ("d_right", _P),
+                ("d_parent", _P), ("d_node_aabb", _P)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("position", _F * 3), ("gaze", _F * 3), ("up", _F * 3), ("fov_x", _F),
+                ("near_plane", _F), ("far_plane", _F)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("samples", _U32), ("max_depth", _U32),
+                ("rr_start", _U32), ("nee", _U32), ("seed", _U32), ("count_traversal", _U32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("segments", C.c_uint64), ("bounce_rays", _U32 * 32), ("bounces", _U32),
+                ("render_ms", C.c_double), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
+                ("lambert", _P), ("n_lambert", _U32), ("metal", _P), ("n_metal", _U32),
+                ("dielectric", _P), ("n_dielectric", _U32), ("emissive", _P),
+                ("n_emissive", _U32), ("morton_bits", _U32)]
+
+
+class SceneBuffers(C.Structure):
+    _fields_ = [("tris", _P), ("prim_key", _P), ("lambert", _P), ("metal", _P),
+                ("dielectric", _P), ("emissive", _P), ("n", _U32), ("n_lambert", _U32),
+                ("n_metal", _U32), ("n_dielectric", _U32), ("n_emissive", _U32),
+                ("camera", CameraDesc)]
+
+
+_SIGS = {
+    "mrh_context_create": (_I32, [_I32, C.POINTER(_P)]),
+    "mrh_context_destroy": (_I32, [_P]),
+    "mrh_last_error": (C.c_char_p, [_P]),
+    "mrh_sync": (_I32, [_P]),
+    "mrh_stream": (_I32, [_P, C.POINTER(_P)]),
+    "mrh_memcpy": (_I32, [_P, _P, _P, C.c_size_t]),
+    "mrh_profile_enable": (_I32, [_P, _I32]),
+    "mrh_profile_reset": (_I32, [_P]),
+    "mrh_profile_get": (_I32, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "mrh_morton_codes": (_I32, [_P, _P, _U32, _U32, _P]),
+    "mrh_sort_pairs": (_I32, [_P, _P, _P, _P, _P, _U32, _U32, _U32, _U32, _I32]),
+    "mrh_partition_by_key": (_I32, [_P, _P, _U32, _U32, _U32, _U32, _P, _P]),
+    "mrh_lbvh_build": (_I32, [_P, _P, _U32, _U32, C.POINTER(_P)]),
+    "mrh_lbvh_view_get": (_I32, [_P, C.POINTER(LbvhView)]),
+    "mrh_lbvh_destroy": (_I32, [_P]),
+    "mrh_lbvh_rebuild": (_I32, [_P, _P, _P]),
+    "mrh_trace_closest": (_I32, [_P, _P, _P, _U32, _U32, _F, _P, _P, _P, _P]),
+    "mrh_trace_any": (_I32, [_P, _P, _P, _U32, _U32, _F, _P]),
+    "mrh_scene_create": (_I32, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
+    "mrh_scene_destroy": (_I32, [_P]),
+    "mrh_scene_bvh": (_I32, [_P, C.POINTER(_P)]),
+    "mrh_scene_bytes": (_I32, [_P, C.POINTER(C.c_uint64)]),
+    "mrh_render": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams), _P,
+                          C.POINTER(RenderStats)]),
+    "mrh_render_bytes": (_I32, [C.POINTER(RenderParams), C.POINTER(C.c_uint64)]),
+    "mrh_gen_random_tris": (_I32, [_U32, _U32, _P]),
+    "mrh_gen_random_rays": (_I32, [_U32, _U32, _P]),
+    "mrh_gen_cornell": (_I32, [_U32, C.POINTER(SceneBuffers)]),
+    "mrh_gen_sphere_grid": (_I32, [_U32, _U32, _U32, _U32, C.POINTER(SceneBuffers)]),
+}
+
+_lib = None
+
+
+def header_symbols() -> list[str]:
+    """Every function the public header declares."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\S.*?\b(mrh_[a-z0-9_]+)\s*\(", text, flags=re.M)))
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m meturay_amd.build` "
+                "(there is no non-HIP fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(x) -> int | None:
+    """Device/host address of a torch tensor, numpy array or int."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):
+        return x.ctypes.data
+    raise TypeError(type(x))
+
+
+class Context:
+    """Owns an mrh_context (device, stream, scratch, kernel timers)."""
+
+    def __init__(self, device: int = 0):
+        self.h = _P()
+        rc = lib().mrh_context_create(device, C.byref(self.h))
+        if rc != OK:
+            raise MrhError(rc, f"mrh_context_create(device={device}) failed")
+        self.device = device
+
+    def check(self, rc: int) -> None:
+        if rc != OK:
+            raise MrhError(rc, lib().mrh_last_error(self.h).decode())
+
+    def close(self) -> None:
+        if self.h:
+            lib().mrh_context_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self) -> None:
+        self.check(lib().mrh_sync(self.h))
+
+    def memcpy(self, dst, src, nbytes: int) -> None:
+        self.check(lib().mrh_memcpy(self.h, ptr(dst), ptr(src), nbytes))
+
+    def profile(self, on: bool = True) -> None:
+        self.check(lib().mrh_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self) -> None:
+        self.check(lib().mrh_profile_reset(self.h))
+
+    def profile_get(self, kernel: str) -> tuple[float, int]:
+        ms = C.c_double(0)
+        n = C.c_uint64(0)
+        self.check(lib().mrh_profile_get(self.h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+class Lbvh:
+    """Device LBVH (mrh_lbvh)."""
+
+    def __init__(self, ctx: Context, d_tris, n: int, bits: int = 30, handle=None):
+        self.ctx = ctx
+        self.owned = handle is None
+        if handle is None:
+            self.h = _P()
+            ctx.check(lib().mrh_lbvh_build(ctx.h, ptr(d_tris), n, bits, C.byref(self.h)))
+        else:
+            self.h = handle
+
+    def view(self) -> LbvhView:
+        v = LbvhView()
+        self.ctx.check(lib().mrh_lbvh_view_get(self.h, C.byref(v)))
+        return v
+
+    def rebuild(self, d_tris) -> None:
+        self.ctx.check(lib().mrh_lbvh_rebuild(self.ctx.h, self.h, ptr(d_tris)))
+
+    def close(self) -> None:
+        if self.owned and self.h:
+            lib().mrh_lbvh_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scene_buffers(gen, *args):
+    """Runs a generator twice (size query, fill) and returns numpy arrays."""
+    import numpy as np
+
+    b = SceneBuffers()
+    rc = gen(*args, C.byref(b))
+    if rc != OK:
+        raise MrhError(rc, "scene generator failed")
+    out = {
+        "tris": np.zeros((b.n, 9), np.float32),
+        "prim_key": np.zeros(b.n, np.uint32),
+        "lambert": np.zeros((max(b.n_lambert, 1), 3), np.float32),
+        "metal": np.zeros((max(b.n_metal, 1), 10), np.float32),
+        "dielectric": np.zeros((max(b.n_dielectric, 1), 4), np.float32),
+        "emissive": np.zeros((max(b.n_emissive, 1), 3), np.float32),
+    }
+    b.tris = out["tris"].ctypes.data
+    b.prim_key = out["prim_key"].ctypes.data
+    b.lambert = out["lambert"].ctypes.data
+    b.metal = out["metal"].ctypes.data
+    b.dielectric = out["dielectric"].ctypes.data
+    b.emissive = out["emissive"].ctypes.data
+    rc = gen(*args, C.byref(b))
+    if rc != OK:
+        raise MrhError(rc, "scene generator failed")
+    out["lambert"] = out["lambert"][: b.n_lambert]
+    out["metal"] = out["metal"][: b.n_metal]
+    out["dielectric"] = out["dielectric"][: b.n_dielectric]
+    out["emissive"] = out["emissive"][: b.n_emissive]
+    cam = b.camera
+    out["camera"] = {
+        "position": list(cam.position), "gaze": list(cam.gaze), "up": list(cam.up),
+        "fov_x": cam.fov_x, "near": cam.near_plane, "far": cam.far_plane,
+    }
+    return out
+
+
+def gen_cornell(subdiv: int = 4):
+    return scene_buffers(lib().mrh_gen_cornell, subdiv)
+
+
+def gen_sphere_grid(grid: int = 64, stacks: int = 16, slices: int = 16, seed: int = 2):
+    return scene_buffers(lib().mrh_gen_sphere_grid, grid, stacks, slices, seed)
+
+
+def gen_random_tris(n: int, seed: int = 1):
+    import numpy as np
+
+    t = np.zeros((n, 9), np.float32)
+    rc = lib().mrh_gen_random_tris(n, seed, t.ctypes.data)
+    if rc != OK:
+        raise MrhError(rc, "mrh_gen_random_tris failed")
+    return t
+
+
+def gen_random_rays(n: int, seed: int = 1):
+    import numpy as np
+
+    r = np.zeros((7, n), np.float32)
+    rc = lib().mrh_gen_random_rays(n, seed, r.ctypes.data)
+    if rc != OK:
+        raise MrhError(rc, "mrh_gen_random_rays failed")
+    return r
+
+
+def camera_desc(cam: dict) -> CameraDesc:
+    c = CameraDesc()
+    c.position[:] = cam["position"]
+    c.gaze[:] = cam["gaze"]
+    c.up[:] = cam["up"]
+    c.fov_x = cam["fov_x"]
+    c.near_plane = cam["near"]
+    c.far_plane = cam["far"]
+    return c
+
+
+class Scene:
+    """Resident scene (mrh_scene): triangles, HitKeys, materials and LBVH."""
+
+    def __init__(self, ctx: Context, scene: dict, morton_bits: int = 30):
+        import numpy as np
+
+        self.ctx = ctx
+        self._keep = {k: np.ascontiguousarray(v) for k, v in scene.items()
+                      if isinstance(v, np.ndarray)}
+        d = SceneDesc()
+        k = self._keep
+        d.tris = k["tris"].ctypes.data
+        d.prim_key = k["prim_key"].ctypes.data
+        d.n = k["tris"].shape[0]
+        for name in ("lambert", "metal", "dielectric", "emissive"):
+            a = k.get(name)
+            setattr(d, name, a.ctypes.data if a is not None and a.size else None)
+            setattr(d, "n_" + name, a.shape[0] if a is not None else 0)
+        d.morton_bits = morton_bits
+        self.h = _P()
+        ctx.check(lib().mrh_scene_create(ctx.h, C.byref(d), C.byref(self.h)))
+        self.camera = scene.get("camera")
+
+    def bvh(self) -> Lbvh:
+        h = _P()
+        self.ctx.check(lib().mrh_scene_bvh(self.h, C.byref(h)))
+        return Lbvh(self.ctx, None, 0, handle=h)
+
+    def render(self, d_image, width, height, samples, max_depth, rr_start, seed=0, camera=None,
+               nee=0, count_traversal=False) -> RenderStats:
+        cam = camera_desc(camera or self.camera)
+        p = RenderParams(width, height, samples, max_depth, rr_start, nee, seed,
+                         1 if count_traversal else 0)
+        st = RenderStats()
+        self.ctx.check(lib().mrh_render(self.ctx.h, self.h, C.byref(cam), C.byref(p),
+                                        ptr(d_image), C.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self.h:
+            lib().mrh_scene_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,578 @@
+/*
+ * om_pt.c — oracle: camera rays, BxDFs, surface generation and the plain
+ * path tracer, restated from the reference's device code.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Paths are traced depth-first, one sample at a time; the wavefront order of
+ * the GPU path does not change any path's arithmetic because every path owns
+ * its PCG32 stream, keyed by (seed, global sample id).  Per-path arithmetic is
+ * FP32 in the same operation order as the HIP kernels; pixel accumulation is
+ * FP64.
+ */
+#include "oracle.h"
+#include "om.h"
+
+#include <float.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OM_PI_F     3.14159265358979323846f
+#define OM_INV_PI_F 0.31830988618379067154f
+
+/* ---------------------------- helpers -------------------------------- */
+
+void om_sincos_2pi_f(float u, float* s, float* c) { om_sincos_2pi(u, s, c); }
+
+/* RayLib/HemiDistribution.h:32-51 (HemiUniformCDF); cos/sin of 2*pi*xi1
+ * evaluated by om_sincos_2pi.  pdf = (1/pi) * 0.5. */
+static om_v3 hemi_uniform(float xi0, float xi1, float* pdf)
+{
+    float c0 = sqrtf(fmaxf(0.0f, 1.0f - xi0 * xi0));
+    float s, c;
+    om_sincos_2pi(xi1, &s, &c);
+    *pdf = OM_INV_PI_F * 0.5f;
+    return v3(c0 * c, c0 * s, xi0);
+}
+
+void om_hemi_uniform(float xi0, float xi1, float dir[3], float* pdf)
+{
+    om_v3 d = hemi_uniform(xi0, xi1, pdf);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+
+/* RayLib/Ray.hpp:503-545 (Ray<float>::Nudge, Ray Tracing Gems I ch. 6),
+ * curvature offset 0. */
+static om_v3 nudge(om_v3 p, om_v3 n)
+{
+    const float origin = 1.0f / 32.0f;
+    const float float_scale = 1.0f / 65536.0f;
+    const float int_scale = 256.0f;
+    float pv[3] = {p.x, p.y, p.z};
+    float nv[3] = {n.x, n.y, n.z};
+    float out[3];
+    for (int i = 0; i < 3; i++) {
+        int32_t ofi = (int32_t)(int_scale * nv[i]);
+        int32_t pi;
+        memcpy(&pi, &pv[i], 4);
+        pi += (pv[i] < 0.0f) ? -ofi : ofi;
+        float pf;
+        memcpy(&pf, &pi, 4);
+        out[i] = (fabsf(pv[i]) < origin) ? (pv[i] + float_scale * nv[i]) : pf;
+    }
+    return v3(out[0], out[1], out[2]);
+}
+
+void om_nudge(const float p[3], const float n[3], float out[3])
+{
+    om_v3 r = nudge(v3(p[0], p[1], p[2]), v3(n[0], n[1], n[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+void om_rot_between_z(const float n[3], float q[4])
+{
+    om_q r = q_rot_between_z(v3(n[0], n[1], n[2]));
+    q[0] = r.w; q[1] = r.x; q[2] = r.y; q[3] = r.z;
+}
+
+void om_quat_rotate(const float q[4], const float v[3], float out[3])
+{
+    om_v3 r = q_rotate(q_make(q[0], q[1], q[2], q[3]), v3(v[0], v[1], v[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* TracerFunctions.cuh:8-30 */
+float om_fresnel_dielectric(float cos_in, float ior_in, float ior_out)
+{
+    float sin_in = sqrtf(fmaxf(0.0f, 1.0f - cos_in * cos_in));
+    float sin_out = ior_in / ior_out * sin_in;
+    if (sin_out >= 1.0f) return 1.0f;
+    float cos_out = sqrtf(fmaxf(0.0f, 1.0f - sin_out * sin_out));
+    float par = ((ior_out * cos_in - ior_in * cos_out) / (ior_out * cos_in + ior_in * cos_out));
+    par = par * par;
+    float perp = ((ior_in * cos_in - ior_out * cos_out) / (ior_in * cos_in + ior_out * cos_out));
+    perp = perp * perp;
+    return (par + perp) * 0.5f;
+}
+
+/* TracerFunctions.cuh:33-79 (per colour channel) */
+static float fresnel_conductor1(float cos_in, float eta, float k)
+{
+    float cos2 = cos_in * cos_in;
+    float sin2 = fmaxf(0.0f, 1.0f - cos2);
+    float sin2_2 = sin2 * sin2;
+    float sin2_4 = sin2_2 * sin2_2;
+    float eta2 = eta * eta;
+    float k2 = k * k;
+    float diff = eta2 - k2 - sin2;
+    float a2b2 = diff * diff;
+    a2b2 += 4.0f * eta2 * k2;
+    a2b2 = sqrtf(fmaxf(a2b2, 0.0f));
+    float a = 0.5f * (a2b2 + diff);
+    a = sqrtf(fmaxf(a, 0.0f));
+    float perp_t1 = a2b2 + cos2;
+    float perp_t2 = a * (2.0f * cos_in);
+    float perp = (perp_t1 - perp_t2) / (perp_t1 + perp_t2);
+    float par_t1 = a2b2 * cos2 + sin2_4;
+    float par_t2 = perp_t2 * sin2_2;
+    float par = perp * (par_t1 - par_t2) / (par_t1 + par_t2);
+    return (par + perp) * 0.5f;
+}
+
+void om_fresnel_conductor(float cos_in, const float eta[3], const float k[3], float out[3])
+{
+    for (int i = 0; i < 3; i++) out[i] = fresnel_conductor1(cos_in, eta[i], k[i]);
+}
+
+/* TracerFunctions.cuh:81-90 */
+float om_dggx(float ndh, float alpha)
+{
+    float a2 = alpha * alpha;
+    float denom = ndh * ndh * (a2 - 1.0f) + 1.0f;
+    denom = denom * denom;
+    denom *= OM_PI_F;
+    return a2 / denom;
+}
+
+/* TracerFunctions.cuh:92-101 */
+static float lambda_smith(om_v3 v, float alpha)
+{
+    om_v3 s = v3_mul(v, v);
+    float a2 = alpha * alpha;
+    float inner = a2 * (s.x + s.y) / s.z;
+    float l = sqrtf(1.0f + inner) - 1.0f;
+    l *= 0.5f;
+    return l;
+}
+
+static float gsmith_single(om_v3 v, float alpha) { return 1.0f / (1.0f + lambda_smith(v, alpha)); }
+
+static float gsmith_corr(om_v3 wo, om_v3 wi, float alpha)
+{
+    return 1.0f / (lambda_smith(wo, alpha) + lambda_smith(wi, alpha) + 1.0f);
+}
+
+float om_gsmith_single(const float v[3], float alpha) { return gsmith_single(v3(v[0], v[1], v[2]), alpha); }
+float om_gsmith_correlated(const float wo[3], const float wi[3], float alpha)
+{
+    return gsmith_corr(v3(wo[0], wo[1], wo[2]), v3(wi[0], wi[1], wi[2]), alpha);
+}
+
+/* TracerFunctions.cuh:186-196 */
+static float vndf_pdf(om_v3 v, om_v3 h, float alpha)
+{
+    float vdh = fmaxf(0.0f, v3_dot(h, v));
+    float ndh = fmaxf(0.0f, h.z);
+    float ndv = fmaxf(0.0f, v.z);
+    float d = om_dggx(ndh, alpha);
+    float g1 = gsmith_single(v, alpha);
+    return (ndv == 0.0f) ? 0.0f : (vdh * d * g1 / ndv);
+}
+
+float om_vndf_pdf(const float v[3], const float h[3], float alpha)
+{
+    return vndf_pdf(v3(v[0], v[1], v[2]), v3(h[0], h[1], h[2]), alpha);
+}
+
+/* TracerFunctions.cuh:198-258 (Heitz 2018, isotropic) */
+static om_v3 vndf_sample(om_v3 v, float a, float xi0, float xi1, float* pdf)
+{
+    om_v3 vh = v3_normalize(v3(a * v.x, a * v.y, v.z));
+    float lensq = 0.0f;
+    lensq += vh.x * vh.x;
+    lensq += vh.y * vh.y;
+    om_v3 t1v = (lensq > 0.0f) ? v3_divs(v3(-vh.y, vh.x, 0.0f), sqrtf(lensq)) : v3(1.0f, 0.0f, 0.0f);
+    om_v3 t2v = v3_cross(vh, t1v);
+    float r = sqrtf(xi0);
+    float sp, cp;
+    om_sincos_2pi(xi1, &sp, &cp);
+    float t1 = r * cp;
+    float t2 = r * sp;
+    float s = 0.5f * (1.0f + vh.z);
+    t2 = (1.0f - s) * sqrtf(1.0f - t1 * t1) + s * t2;
+    float val = 1.0f - t1 * t1 - t2 * t2;
+    float sq = sqrtf(fmaxf(0.0f, val));
+    om_v3 nh = v3_add(v3_add(v3_scale(t1v, t1), v3_scale(t2v, t2)), v3_scale(vh, sq));
+    om_v3 nm = v3(a * nh.x, a * nh.y, fmaxf(0.0f, nh.z));
+    float len2 = v3_dot(nm, nm);
+    if (len2 < 1.0e-5f) nm = v3(0.0f, 0.0f, 1.0f);
+    else nm = v3_scale(nm, 1.0f / sqrtf(len2));
+    *pdf = vndf_pdf(v, nm, a);
+    return nm;
+}
+
+void om_vndf_sample(const float v[3], float alpha, float xi0, float xi1, float h[3], float* pdf)
+{
+    om_v3 r = vndf_sample(v3(v[0], v[1], v[2]), alpha, xi0, xi1, pdf);
+    h[0] = r.x; h[1] = r.y; h[2] = r.z;
+}
+
+/* ---------------------------- camera --------------------------------- */
+
+/* GPUCameraPinhole.cuh:140-170 */
+void om_camera_setup(const float pos[3], const float gaze[3], const float up[3],
+                     float fov_x, float near_d, float far_d, uint32_t w, uint32_t h,
+                     om_camera* cam)
+{
+    float fov_y = (float)(2.0 * atan(tan((double)fov_x * 0.5) * (double)h / (double)w));
+    om_v3 p = v3(pos[0], pos[1], pos[2]);
+    om_v3 g = v3(gaze[0], gaze[1], gaze[2]);
+    om_v3 u = v3(up[0], up[1], up[2]);
+    float wh = tanf(fov_x * 0.5f) * near_d;
+    float hh = tanf(fov_y * 0.5f) * near_d;
+    om_v3 gd = v3_sub(g, p);
+    om_v3 right = v3_normalize(v3_cross(gd, u));
+    u = v3_normalize(v3_cross(right, gd));
+    gd = v3_normalize(v3_cross(u, right));
+    om_v3 bl = v3_add(v3_sub(v3_sub(p, v3_scale(right, wh)), v3_scale(u, hh)), v3_scale(gd, near_d));
+    cam->position[0] = p.x; cam->position[1] = p.y; cam->position[2] = p.z;
+    cam->bottom_left[0] = bl.x; cam->bottom_left[1] = bl.y; cam->bottom_left[2] = bl.z;
+    cam->right[0] = right.x; cam->right[1] = right.y; cam->right[2] = right.z;
+    cam->up[0] = u.x; cam->up[1] = u.y; cam->up[2] = u.z;
+    cam->plane_size[0] = wh * 2.0f;
+    cam->plane_size[1] = hh * 2.0f;
+    cam->near_far[0] = near_d;
+    cam->near_far[1] = far_d;
+}
+
+/* GenerationKernels.cuh:29-108 + GPUCameraPinhole.cuh:209-240: stratified
+ * pinhole sample.  Sample ids run pixel-major: sample_id =
+ * (y*W + x)*S^2 + (sy*S + sx). */
+static void camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t S,
+                       uint32_t sample_id, om_rng* rng, om_v3* o, om_v3* d)
+{
+    uint32_t spp = S * S;
+    uint32_t pix = sample_id / spp;
+    uint32_t s = sample_id - pix * spp;
+    uint32_t px = pix % w, py = pix / w;
+    uint32_t sx = s % S, sy = s / S;
+    float dx = cam->plane_size[0] / (float)(w * S);
+    float dy = cam->plane_size[1] / (float)(h * S);
+    float jx = om_uniform(rng);
+    float jy = om_uniform(rng);
+    float sdx = (float)(px * S + sx) * dx;
+    float sdy = (float)(py * S + sy) * dy;
+    sdx += jx * dx;
+    sdy += jy * dy;
+    om_v3 bl = v3(cam->bottom_left[0], cam->bottom_left[1], cam->bottom_left[2]);
+    om_v3 r = v3(cam->right[0], cam->right[1], cam->right[2]);
+    om_v3 u = v3(cam->up[0], cam->up[1], cam->up[2]);
+    om_v3 pos = v3(cam->position[0], cam->position[1], cam->position[2]);
+    om_v3 sp = v3_add(bl, v3_add(v3_scale(r, sdx), v3_scale(u, sdy)));
+    *d = v3_normalize(v3_sub(sp, pos));
+    *o = pos;
+}
+
+void om_camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t samples,
+                   uint32_t seed, uint32_t sample_id, float o[3], float d[3],
+                   uint64_t* rng_state)
+{
+    om_rng rng;
+    om_path_rng(&rng, seed, sample_id);
+    om_v3 oo, dd;
+    camera_ray(cam, w, h, samples, sample_id, &rng, &oo, &dd);
+    o[0] = oo.x; o[1] = oo.y; o[2] = oo.z;
+    d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
+    *rng_state = rng.state;
+}
+
+/* ---------------------------- path tracer ---------------------------- */
+
+typedef struct {
+    om_v3 pos;
+    om_v3 geo_n;      /* geometric normal, flipped to the ray side */
+    om_q  w2t;        /* world -> tangent rotation */
+    int   back;
+} om_surface;
+
+/* GPUPrimitiveTriangle.h:626-670 (GenBasicSurface) for flat-shaded,
+ * two-sided triangles: every corner carries the face frame
+ * conj(RotationBetweenZAxis(n)), so the barycentric SLERP is that frame.
+ * Back-side hits flip the geometric normal and pre-multiply the frame by the
+ * 180-degree rotation about tangent X (TANGENT_ROT = (0,1,0,0)). */
+static om_surface make_surface(const float* tri, float ba, float bb, om_v3 ray_dir)
+{
+    om_surface s;
+    om_v3 p0 = v3(tri[0], tri[1], tri[2]);
+    om_v3 p1 = v3(tri[3], tri[4], tri[5]);
+    om_v3 p2 = v3(tri[6], tri[7], tri[8]);
+    float c = 1.0f - ba - bb;
+    s.pos = v3_add(v3_add(v3_scale(p0, ba), v3_scale(p1, bb)), v3_scale(p2, c));
+    om_v3 n = v3_normalize(v3_cross(v3_sub(p1, p0), v3_sub(p2, p0)));
+    s.w2t = q_conj(q_rot_between_z(n));
+    s.back = v3_dot(n, ray_dir) > 0.0f;
+    if (s.back) {
+        n = v3_neg(n);
+        s.w2t = q_mul(q_make(0.0f, 1.0f, 0.0f, 0.0f), s.w2t);
+    }
+    s.geo_n = n;
+    return s;
+}
+
+typedef struct {
+    om_v3 o, d;
+    om_v3 refl;
+    float pdf;
+    int   specular;
+    uint32_t medium;
+} om_bsdf_out;
+
+/* SimpleMaterialsKC.cuh:16-60 (LambertConstFuncs::Sample) */
+static void lambert_sample(const om_surface* s, om_v3 albedo, om_rng* rng, om_bsdf_out* out)
+{
+    om_v3 normal = q_rotate(q_conj(s->w2t), v3(0.0f, 0.0f, 1.0f));
+    float xi0 = om_uniform(rng);
+    float xi1 = om_uniform(rng);
+    float pdf;
+    om_v3 dir = v3_normalize(hemi_uniform(xi0, xi1, &pdf));
+    om_q q = q_rot_between_z(normal);
+    dir = q_rotate(q, dir);
+    float ndl = fabsf(v3_dot(normal, dir));
+    out->o = nudge(s->pos, s->geo_n);
+    out->d = dir;
+    out->refl = v3_scale(v3_scale(albedo, ndl), OM_INV_PI_F);
+    out->pdf = pdf;
+    out->specular = 0;
+}
+
+/* MetalGlassMaterialsKC.cuh:24-150 (MetalMatFuncs::Sample) */
+static void metal_sample(const om_surface* s, om_v3 wi, const float* m, om_rng* rng, om_bsdf_out* out)
+{
+    float rough = m[0];
+    float alpha = rough * rough;
+    om_v3 V = q_rotate(s->w2t, wi);
+    float xi0 = om_uniform(rng);
+    float xi1 = om_uniform(rng);
+    float pdf;
+    om_v3 H = vndf_sample(V, alpha, xi0, xi1, &pdf);
+    float vdh = fmaxf(0.0f, v3_dot(V, H));
+    om_v3 L = v3_sub(v3_scale(H, 2.0f * vdh), V);
+    pdf = (vdh == 0.0f) ? 0.0f : pdf / (4.0f * vdh);
+    float ldh = fmaxf(0.0f, v3_dot(L, H));
+    float ndh = fmaxf(0.0f, H.z);
+    float ndv = fmaxf(0.0f, V.z);
+    om_v3 F = v3(fresnel_conductor1(vdh, m[1], m[4]),
+                 fresnel_conductor1(vdh, m[2], m[5]),
+                 fresnel_conductor1(vdh, m[3], m[6]));
+    F = v3_mul(F, v3(m[7], m[8], m[9]));
+    float D = om_dggx(ndh, alpha);
+    float G = gsmith_corr(V, L, alpha);
+    G = (ldh == 0.0f) ? 0.0f : G;
+    G = (vdh == 0.0f) ? 0.0f : G;
+    /* D * F * G * 0.25 / NdV, evaluated left to right per channel */
+    om_v3 spec = v3(((D * F.x) * G) * 0.25f, ((D * F.y) * G) * 0.25f, ((D * F.z) * G) * 0.25f);
+    spec = v3_divs(spec, ndv);
+    if (ndv == 0.0f) spec = v3(0.0f, 0.0f, 0.0f);
+    if (isinf(D) || isnan(D)) {
+        if (rough == 0.0f) L = v3_sub(v3_scale(v3(0.0f, 0.0f, 1.0f), 2.0f * V.z), V);
+        spec = v3_scale(F, G);
+        float gs = gsmith_single(V, alpha);
+        if (gs != 0.0f) spec = v3_divs(spec, gs);
+        else spec = v3(0.0f, 0.0f, 0.0f);
+        pdf = 1.0f;
+    }
+    out->d = q_rotate(q_conj(s->w2t), L);
+    out->o = nudge(s->pos, s->geo_n);
+    out->refl = spec;
+    out->pdf = pdf;
+    /* MetalMatFuncs::Specularity = 1 - roughness; specular at >= 0.99
+     * (TracerConstants.h:6) */
+    out->specular = (1.0f - rough) >= 0.99f;
+}
+
+/* SimpleMaterialsKC.cuh:172-290 (RefractMatFuncs::Sample).  Media: 0 is the
+ * vacuum (IOR 1); medium k+1 is the interior of dielectric material k. */
+static void dielectric_sample(const om_surface* s, om_v3 wi, const float* m, uint32_t mat_id,
+                              uint32_t medium, const float* dielectric, om_rng* rng,
+                              om_bsdf_out* out)
+{
+    om_v3 albedo = v3(m[0], m[1], m[2]);
+    float ior = m[3];
+    om_v3 normal = q_rotate(q_conj(s->w2t), v3(0.0f, 0.0f, 1.0f));
+    int entering = !s->back;
+    float from = (medium == 0u) ? 1.0f : dielectric[(size_t)(medium - 1u) * 4 + 3];
+    float to = entering ? ior : 1.0f;
+    float f = om_fresnel_dielectric(fabsf(v3_dot(wi, normal)), from, to);
+    float xi = om_uniform(rng);
+    out->specular = 1;
+    if (xi < f) {
+        float k = 2.0f * v3_dot(wi, normal);
+        out->d = v3_sub(v3_scale(normal, k), wi);
+        out->o = nudge(s->pos, s->geo_n);
+        out->pdf = f;
+        out->medium = medium;
+        out->refl = v3_scale(albedo, f);
+    } else {
+        out->medium = entering ? (mat_id + 1u) : 0u;
+        out->pdf = 1.0f - f;
+        float ratio = from / to;
+        float cos_in = v3_dot(normal, wi);
+        float sin_in2 = fmaxf(0.0f, 1.0f - cos_in * cos_in);
+        float sin_out2 = ratio * ratio * sin_in2;
+        float cos_out = sqrtf(fmaxf(0.0f, 1.0f - sin_out2));
+        if (sin_out2 >= 1.0f) {
+            out->pdf = 0.0f;
+            out->refl = v3(0.0f, 0.0f, 0.0f);
+            out->d = wi;
+            out->o = s->pos;
+            return;
+        }
+        float kk = ratio * cos_in - cos_out;
+        out->d = v3_add(v3_scale(v3_neg(wi), ratio), v3_scale(normal, kk));
+        out->o = nudge(s->pos, v3_neg(s->geo_n));
+        out->refl = v3_scale(v3_scale(albedo, 1.0f), 1.0f - f);
+    }
+}
+
+typedef struct {
+    const om_scene* s;
+    const om_camera* cam;
+    const om_render_params* p;
+    double* img;
+    uint32_t tid;
+    uint64_t segments;
+} om_job;
+
+static void trace_path(const om_scene* sc, const om_camera* cam, const om_render_params* p,
+                       uint32_t sample_id, double* pixel, uint64_t* segments)
+{
+    om_rng rng;
+    om_path_rng(&rng, p->seed, sample_id);
+    om_v3 o, d;
+    camera_ray(cam, p->width, p->height, p->samples, sample_id, &rng, &o, &d);
+    om_v3 T = v3(1.0f, 1.0f, 1.0f);
+    uint32_t depth = 1, medium = 0;
+    float tmin = cam->near_far[0], tmax = cam->near_far[1];
+
+    for (uint32_t bounce = 0; bounce < p->max_depth; bounce++) {
+        (*segments)++;
+        float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+        float b[2];
+        uint32_t prim = om_trace_one(sc->tris, sc->n, sc->sorted_idx, sc->left, sc->right,
+                                     sc->node_aabb, sc->root, oo, dd, tmin, &tmax, b, 0, NULL, NULL);
+        if (prim == OM_INVALID) return;
+        uint32_t key = sc->prim_key[prim];
+        uint32_t batch = key >> 24, id = key & 0xFFFFFFu;
+        if (batch == OM_BATCH_EMISSIVE) {
+            /* PathTracerKC.cuh:337-426: Le * throughput, MIS weight 1 */
+            const float* le = sc->emissive + (size_t)id * 3;
+            pixel[0] += (double)(le[0] * T.x);
+            pixel[1] += (double)(le[1] * T.y);
+            pixel[2] += (double)(le[2] * T.z);
+            return;
+        }
+        if (batch == OM_BATCH_NULL) return;
+        om_surface s = make_surface(sc->tris + (size_t)prim * 9, b[0], b[1], d);
+        om_v3 wi = v3_neg(v3_normalize(d));
+        om_bsdf_out out;
+        out.medium = medium;
+        if (batch == OM_BATCH_LAMBERT) {
+            const float* a = sc->lambert + (size_t)id * 3;
+            lambert_sample(&s, v3(a[0], a[1], a[2]), &rng, &out);
+        } else if (batch == OM_BATCH_METAL) {
+            metal_sample(&s, wi, sc->metal + (size_t)id * 10, &rng, &out);
+        } else {
+            dielectric_sample(&s, wi, sc->dielectric + (size_t)id * 4, id, medium, sc->dielectric,
+                              &rng, &out);
+        }
+        /* PathTracerKC.cuh:600-647 */
+        om_v3 nT = v3_mul(T, out.refl);
+        nT = (out.pdf == 0.0f) ? v3(0.0f, 0.0f, 0.0f) : v3_divs(nT, out.pdf);
+        float avg = v3_dot(nT, v3(0.333f, 0.333f, 0.333f));
+        int terminate = 0;
+        if (!out.specular && depth > p->rr_start) {
+            /* TracerFunctions.cuh:321-330 */
+            float pr = avg < 0.005f ? 0.005f : (avg > 1.0f ? 1.0f : avg);
+            if (om_uniform(&rng) >= pr) terminate = 1;
+            else nT = v3_scale(nT, 1.0f / pr);
+        }
+        if ((terminate && !out.specular) || v3_is_zero(nT)) return;
+        T = nT;
+        o = out.o;
+        d = out.d;
+        medium = out.medium;
+        depth++;
+        tmin = 0.0f;
+        tmax = FLT_MAX;
+    }
+}
+
+static void* render_worker(void* arg)
+{
+    om_job* job = (om_job*)arg;
+    const om_render_params* p = job->p;
+    uint32_t spp = p->samples * p->samples;
+    uint32_t step = p->row_step ? p->row_step : 1u;
+    uint32_t nthreads = p->threads ? p->threads : 1u;
+    uint32_t row_index = 0;
+    for (uint32_t y = p->row_offset; y < p->height; y += step, row_index++) {
+        if (row_index % nthreads != job->tid) continue;
+        for (uint32_t x = 0; x < p->width; x++) {
+            uint32_t pix = y * p->width + x;
+            double* px = job->img + (size_t)pix * 3;
+            for (uint32_t s = 0; s < spp; s++)
+                trace_path(job->s, job->cam, p, pix * spp + s, px, &job->segments);
+        }
+    }
+    return NULL;
+}
+
+uint64_t om_render(const om_scene* s, const om_camera* cam, const om_render_params* p, double* img)
+{
+    uint32_t nthreads = p->threads ? p->threads : 1u;
+    om_job* jobs = (om_job*)calloc(nthreads, sizeof(om_job));
+    pthread_t* th = (pthread_t*)calloc(nthreads, sizeof(pthread_t));
+    for (uint32_t t = 0; t < nthreads; t++) {
+        jobs[t].s = s; jobs[t].cam = cam; jobs[t].p = p; jobs[t].img = img; jobs[t].tid = t;
+        if (nthreads > 1) pthread_create(&th[t], NULL, render_worker, &jobs[t]);
+    }
+    if (nthreads == 1) render_worker(&jobs[0]);
+    uint64_t seg = 0;
+    for (uint32_t t = 0; t < nthreads; t++) {
+        if (nthreads > 1) pthread_join(th[t], NULL);
+        seg += jobs[t].segments;
+    }
+    free(jobs);
+    free(th);
+    return seg;
+}
+
+/* ---------------------------- exported helpers ----------------------- */
+
+/* RayLib/Ray.hpp:304-309: 2 (wi.n) n - wi */
+void om_reflect(const float wi[3], const float n[3], float out[3])
+{
+    om_v3 w = v3(wi[0], wi[1], wi[2]), nn = v3(n[0], n[1], n[2]);
+    om_v3 r = v3_sub(v3_scale(nn, 2.0f * v3_dot(w, nn)), w);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+/* RayLib/Ray.hpp:322-355 (wi points away from the surface, n on its side) */
+int om_refract(const float wi[3], const float n[3], float from, float to, float out[3])
+{
+    om_v3 w = v3(wi[0], wi[1], wi[2]), nn = v3(n[0], n[1], n[2]);
+    float ratio = from / to;
+    float cos_in = v3_dot(nn, w);
+    float sin_in2 = fmaxf(0.0f, 1.0f - cos_in * cos_in);
+    float sin_out2 = ratio * ratio * sin_in2;
+    float cos_out = sqrtf(fmaxf(0.0f, 1.0f - sin_out2));
+    if (sin_out2 >= 1.0f) return 0;
+    om_v3 r = v3_add(v3_scale(v3_neg(w), ratio), v3_scale(nn, ratio * cos_in - cos_out));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+    return 1;
+}
+
+/* RayLib/Triangle.h:203-209 */
+void om_triangle_normal(const float tri[9], float out[3])
+{
+    om_v3 p0 = v3(tri[0], tri[1], tri[2]), p1 = v3(tri[3], tri[4], tri[5]), p2 = v3(tri[6], tri[7], tri[8]);
+    om_v3 n = v3_normalize(v3_cross(v3_sub(p1, p0), v3_sub(p2, p0)));
+    out[0] = n.x; out[1] = n.y; out[2] = n.z;
+}
+
+/* RayLib/AABB.hpp:79-82 */
+void om_aabb_centroid(const float box[6], float out[3])
+{
+    for (int a = 0; a < 3; a++) out[a] = box[a] + (box[3 + a] - box[a]) * 0.5f;
+}

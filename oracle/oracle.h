@@ -1,0 +1,168 @@
+/*
+ * oracle.h — CPU restatement of MRay's wavefront path-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg are the only callers.  The product never links this.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference checkout).  Layouts:
+ *   tris   : float[n][9]  (v0.xyz, v1.xyz, v2.xyz), primitive order
+ *   aabb   : float[n][6]  (min.xyz, max.xyz)
+ *   rays   : 7 planes of float[pitch]: ox, oy, oz, dx, dy, dz, tmax
+ *   hits   : prim u32 (0xFFFFFFFF = miss), t f32, bary a/b f32 (2 planes)
+ *   LBVH   : leaves are nodes [0,n), internal node p is node n+p and its
+ *            split lies between sorted leaves p and p+1;
+ *            left/right[p] and parent[] hold global node indices,
+ *            parent[root] = 0xFFFFFFFF.
+ */
+#ifndef MRH_ORACLE_H
+#define MRH_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OM_INVALID 0xFFFFFFFFu
+
+/* ---- Morton codes (RayLib/MortonCode.h:43-158, LinearBVH.hpp:119-166,400-414) */
+uint32_t om_morton_expand10(uint32_t v);
+uint64_t om_morton_expand21(uint32_t v);
+uint32_t om_morton_compose30(uint32_t x, uint32_t y, uint32_t z);
+uint64_t om_morton_compose63(uint32_t x, uint32_t y, uint32_t z);
+void     om_tri_aabbs(const float* tris, uint32_t n, float* aabbs);
+void     om_extent(const float* aabbs, uint32_t n, float ext[6]);
+float    om_morton_delta(const float ext[6], uint32_t bits_per_dim);
+void     om_quantise(const float aabb[6], const float ext[6], float delta,
+                     uint32_t bits_per_dim, uint32_t q[3]);
+/* bits = 30 writes uint32 codes, bits = 63 writes uint64 codes */
+void     om_morton_codes(const float* tris, uint32_t n, uint32_t bits, void* codes);
+
+/* ---- Sorting / partition (RayMemory.cu:187-312, LinearBVH.hpp:489-492) ---- */
+/* stable ascending sort of (code, index); writes sorted codes and indices */
+void om_sort_pairs_u32(const uint32_t* keys, uint32_t n, uint32_t* sorted_keys, uint32_t* sorted_idx);
+void om_sort_pairs_u64(const uint64_t* keys, uint32_t n, uint64_t* sorted_keys, uint32_t* sorted_idx);
+/* Stable DESCENDING sort by key bits [begin_bit, end_bit); order[] receives
+ * the input positions.  offsets[] (2^(end-begin) + 1 entries) receives, for
+ * every digit d, the first sorted position holding digit d; entries are in
+ * descending digit order, so offsets[0] is digit max .. ; offsets[last] = n. */
+void om_partition_desc(const uint32_t* keys, uint32_t n, uint32_t begin_bit, uint32_t end_bit,
+                       uint32_t* order, uint32_t* offsets);
+
+/* ---- LBVH (LinearBVH.hpp:30-100, 183-396, 423-596) ---- */
+/* Returns 0 on success.  sorted_codes may be NULL.  Arrays sized for n >= 1:
+ * sorted_idx[n], left[n-1], right[n-1], parent[2n-1], node_aabb[(2n-1)*6]. */
+int om_lbvh_build(const float* tris, uint32_t n, uint32_t bits,
+                  void* sorted_codes, uint32_t* sorted_idx,
+                  uint32_t* left, uint32_t* right, uint32_t* parent,
+                  float* node_aabb, uint32_t* root);
+
+/* ---- Ray casting (Ray.hpp:143-283, GPUAcceleratorBVHKC.cuh:229-346,
+ *      GPUAcceleratorCommonKC.cuh:33-78) ---- */
+int  om_intersect_triangle(const float o[3], const float d[3], const float tri[9],
+                           float* t, float bary[3]);
+int  om_intersect_aabb(const float o[3], const float inv_d[3], const float box[6],
+                       float tmin, float tmax, float* t_enter);
+/* stats (optional, may be NULL): 2 u32 per ray = internal-node visits, triangle tests */
+void om_trace_closest(const float* tris, uint32_t n, const uint32_t* sorted_idx,
+                      const uint32_t* left, const uint32_t* right, const float* node_aabb,
+                      uint32_t root, const float* rays, uint32_t nrays, uint32_t pitch,
+                      float tmin, uint32_t* hit_prim, float* hit_t, float* hit_bary,
+                      uint32_t* stats);
+/* any-hit: hit_prim receives 1 if any triangle with t in [tmin, tmax) exists, else 0 */
+void om_trace_any(const float* tris, uint32_t n, const uint32_t* sorted_idx,
+                  const uint32_t* left, const uint32_t* right, const float* node_aabb,
+                  uint32_t root, const float* rays, uint32_t nrays, uint32_t pitch,
+                  float tmin, uint32_t* hit_any);
+void om_trace_brute(const float* tris, uint32_t n, const float* rays, uint32_t nrays,
+                    uint32_t pitch, float tmin, uint32_t* hit_prim, float* hit_t,
+                    float* hit_bary);
+
+/* ---- Sampling helpers (RayLib/HemiDistribution.h, Ray.hpp:503-560,
+ *      TracerFunctions.cuh) ---- */
+void  om_sincos_2pi_f(float u, float* s, float* c);
+void  om_reflect(const float wi[3], const float n[3], float out[3]);
+int   om_refract(const float wi[3], const float n[3], float from, float to, float out[3]);
+void  om_triangle_normal(const float tri[9], float out[3]);
+void  om_aabb_centroid(const float box[6], float out[3]);
+void  om_hemi_uniform(float xi0, float xi1, float dir[3], float* pdf);
+void  om_nudge(const float p[3], const float n[3], float out[3]);
+void  om_rot_between_z(const float n[3], float q[4]);
+void  om_quat_rotate(const float q[4], const float v[3], float out[3]);
+float om_fresnel_dielectric(float cos_in, float ior_in, float ior_out);
+void  om_fresnel_conductor(float cos_in, const float eta[3], const float k[3], float out[3]);
+float om_dggx(float ndh, float alpha);
+float om_gsmith_single(const float v[3], float alpha);
+float om_gsmith_correlated(const float wo[3], const float wi[3], float alpha);
+float om_vndf_pdf(const float v[3], const float h[3], float alpha);
+void  om_vndf_sample(const float v[3], float alpha, float xi0, float xi1, float h[3], float* pdf);
+
+/* ---- Scene + path tracer (PathTracer.cu:144-291, PathTracerKC.cuh:337-648,
+ *      SimpleMaterialsKC.cuh:16-303, MetalGlassMaterialsKC.cuh:11-222,
+ *      GenerationKernels.cuh:29-108, GPUCameraPinhole.cuh:140-240) ---- */
+
+/* Work batches of the 32-bit HitKey (batch:8 | id:24, RayLib/HitStructs.h:42-77) */
+#define OM_BATCH_NULL       0u
+#define OM_BATCH_LAMBERT    1u
+#define OM_BATCH_METAL      2u
+#define OM_BATCH_DIELECTRIC 3u
+#define OM_BATCH_EMISSIVE   4u
+
+typedef struct {
+    const float*    tris;          /* n x 9 */
+    const uint32_t* prim_key;      /* n keys */
+    uint32_t        n;
+    const float*    lambert;       /* albedo rgb per id */
+    const float*    metal;         /* per id: roughness, eta rgb, k rgb, specular rgb (10) */
+    const float*    dielectric;    /* per id: albedo rgb, ior (4) */
+    const float*    emissive;      /* radiance rgb per id */
+    /* LBVH over tris */
+    const uint32_t* sorted_idx;
+    const uint32_t* left;
+    const uint32_t* right;
+    const float*    node_aabb;
+    uint32_t        root;
+} om_scene;
+
+typedef struct {
+    float position[3];
+    float bottom_left[3];
+    float right[3];
+    float up[3];
+    float plane_size[2];
+    float near_far[2];
+} om_camera;
+
+/* GPUCameraPinhole.cuh:140-170 */
+void om_camera_setup(const float pos[3], const float gaze[3], const float up[3],
+                     float fov_x, float near_d, float far_d, uint32_t w, uint32_t h,
+                     om_camera* cam);
+
+typedef struct {
+    uint32_t width, height;
+    uint32_t samples;        /* strata per axis: spp = samples^2 */
+    uint32_t max_depth;
+    uint32_t rr_start;
+    uint32_t seed;
+    uint32_t row_step;       /* render rows y with y % row_step == row_offset */
+    uint32_t row_offset;
+    uint32_t threads;
+} om_render_params;
+
+/* Renders the selected rows; img is double[h*w*3] (zeroed by the caller),
+ * accumulating the per-sample FP32 contributions in FP64.  Returns the number
+ * of ray segments traced. */
+uint64_t om_render(const om_scene* s, const om_camera* cam, const om_render_params* p,
+                   double* img);
+
+/* Camera ray of one sample (for raygen parity tests): writes o, d, and the
+ * path RNG state after the two jitter draws. */
+void om_camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t samples,
+                   uint32_t seed, uint32_t sample_id, float o[3], float d[3],
+                   uint64_t* rng_state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

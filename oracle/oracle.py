@@ -1,0 +1,305 @@
+"""ctypes bindings of the CPU oracle (liboracle.so) and of the reference's
+host-compiled helpers (_ref/libmrhref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by bench.py's cpu_baseline leg, always as the checker or the baseline,
+never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libmrhref.so")
+
+INVALID = 0xFFFFFFFF
+BATCH_NULL, BATCH_LAMBERT, BATCH_METAL, BATCH_DIELECTRIC, BATCH_EMISSIVE = 0, 1, 2, 3, 4
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_F = C.c_float
+
+
+def build(reference: bool = False) -> None:
+    """Compile liboracle.so (and _ref/libmrhref.so when asked and the
+    reference checkout exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if reference and os.path.isdir("/root/reference/Source"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_P)
+
+
+class OmCamera(C.Structure):
+    _fields_ = [("position", _F * 3), ("bottom_left", _F * 3), ("right", _F * 3),
+                ("up", _F * 3), ("plane_size", _F * 2), ("near_far", _F * 2)]
+
+
+class OmScene(C.Structure):
+    _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
+                ("lambert", _P), ("metal", _P), ("dielectric", _P), ("emissive", _P),
+                ("sorted_idx", _P), ("left", _P), ("right", _P), ("node_aabb", _P),
+                ("root", _U32)]
+
+
+class OmRenderParams(C.Structure):
+    _fields_ = [("width", _U32), ("height", _U32), ("samples", _U32), ("max_depth", _U32),
+                ("rr_start", _U32), ("seed", _U32), ("row_step", _U32), ("row_offset", _U32),
+                ("threads", _U32)]
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.om_morton_expand10.restype = _U32
+        L.om_morton_expand10.argtypes = [_U32]
+        L.om_morton_compose30.restype = _U32
+        L.om_morton_compose30.argtypes = [_U32, _U32, _U32]
+        L.om_morton_compose63.restype = C.c_uint64
+        L.om_morton_compose63.argtypes = [_U32, _U32, _U32]
+        L.om_morton_delta.restype = _F
+        L.om_morton_delta.argtypes = [_P, _U32]
+        L.om_quantise.argtypes = [_P, _P, _F, _U32, _P]
+        L.om_tri_aabbs.argtypes = [_P, _U32, _P]
+        L.om_extent.argtypes = [_P, _U32, _P]
+        L.om_morton_codes.argtypes = [_P, _U32, _U32, _P]
+        L.om_sort_pairs_u32.argtypes = [_P, _U32, _P, _P]
+        L.om_sort_pairs_u64.argtypes = [_P, _U32, _P, _P]
+        L.om_partition_desc.argtypes = [_P, _U32, _U32, _U32, _P, _P]
+        L.om_lbvh_build.restype = C.c_int
+        L.om_lbvh_build.argtypes = [_P, _U32, _U32, _P, _P, _P, _P, _P, _P, _P]
+        L.om_intersect_triangle.restype = C.c_int
+        L.om_intersect_triangle.argtypes = [_P, _P, _P, _P, _P]
+        L.om_intersect_aabb.restype = C.c_int
+        L.om_intersect_aabb.argtypes = [_P, _P, _P, _F, _F, _P]
+        L.om_trace_closest.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F,
+                                       _P, _P, _P, _P]
+        L.om_trace_any.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F, _P]
+        L.om_trace_brute.argtypes = [_P, _U32, _P, _U32, _U32, _F, _P, _P, _P]
+        L.om_sincos_2pi_f.argtypes = [_F, _P, _P]
+        L.om_hemi_uniform.argtypes = [_F, _F, _P, _P]
+        L.om_reflect.argtypes = [_P, _P, _P]
+        L.om_refract.restype = C.c_int
+        L.om_refract.argtypes = [_P, _P, _F, _F, _P]
+        L.om_triangle_normal.argtypes = [_P, _P]
+        L.om_aabb_centroid.argtypes = [_P, _P]
+        L.om_nudge.argtypes = [_P, _P, _P]
+        L.om_rot_between_z.argtypes = [_P, _P]
+        L.om_quat_rotate.argtypes = [_P, _P, _P]
+        L.om_fresnel_dielectric.restype = _F
+        L.om_fresnel_dielectric.argtypes = [_F, _F, _F]
+        L.om_fresnel_conductor.argtypes = [_F, _P, _P, _P]
+        L.om_dggx.restype = _F
+        L.om_dggx.argtypes = [_F, _F]
+        L.om_gsmith_single.restype = _F
+        L.om_gsmith_single.argtypes = [_P, _F]
+        L.om_gsmith_correlated.restype = _F
+        L.om_gsmith_correlated.argtypes = [_P, _P, _F]
+        L.om_vndf_pdf.restype = _F
+        L.om_vndf_pdf.argtypes = [_P, _P, _F]
+        L.om_vndf_sample.argtypes = [_P, _F, _F, _F, _P, _P]
+        L.om_camera_setup.argtypes = [_P, _P, _P, _F, _F, _F, _U32, _U32, C.POINTER(OmCamera)]
+        L.om_render.restype = C.c_uint64
+        L.om_render.argtypes = [C.POINTER(OmScene), C.POINTER(OmCamera),
+                                C.POINTER(OmRenderParams), _P]
+        L.om_camera_ray.argtypes = [C.POINTER(OmCamera), _U32, _U32, _U32, _U32, _U32,
+                                    _P, _P, C.POINTER(C.c_uint64)]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    """The reference's own host-compiled RayLib helpers (see ref_wrap.cpp)."""
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_PATH)
+        R.ref_compose30.restype = _U32
+        R.ref_compose30.argtypes = [_U32, _U32, _U32]
+        R.ref_compose63.restype = C.c_uint64
+        R.ref_compose63.argtypes = [_U32, _U32, _U32]
+        R.ref_decompose30.argtypes = [_U32, _P]
+        R.ref_decompose63.argtypes = [C.c_uint64, _P]
+        R.ref_intersect_triangle.restype = C.c_int
+        R.ref_intersect_triangle.argtypes = [_P, _P, _P, _P, _P]
+        R.ref_intersect_aabb.restype = C.c_int
+        R.ref_intersect_aabb.argtypes = [_P, _P, _P, _F, _F, _P]
+        R.ref_nudge.argtypes = [_P, _P, _P, _P]
+        R.ref_reflect.argtypes = [_P, _P, _P]
+        R.ref_hemi_uniform.argtypes = [_F, _F, _P, _P]
+        R.ref_hemi_cosine.argtypes = [_F, _F, _P, _P]
+        R.ref_rot_between_z.argtypes = [_P, _P]
+        R.ref_quat_rotate.argtypes = [_P, _P, _P]
+        R.ref_quat_mul.argtypes = [_P, _P, _P]
+        R.ref_triangle_normal.argtypes = [_P, _P]
+        R.ref_triangle_aabb.argtypes = [_P, _P]
+        R.ref_aabb_centroid.argtypes = [_P, _P]
+        _ref = R
+    return _ref
+
+
+# ---------------------------------------------------------------- wrappers
+def morton_codes(tris: np.ndarray, bits: int) -> np.ndarray:
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    n = tris.shape[0]
+    out = np.zeros(n, dtype=np.uint32 if bits == 30 else np.uint64)
+    lib().om_morton_codes(_ptr(tris), n, bits, _ptr(out))
+    return out
+
+
+def tri_aabbs(tris: np.ndarray) -> np.ndarray:
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    out = np.zeros((tris.shape[0], 6), dtype=np.float32)
+    lib().om_tri_aabbs(_ptr(tris), tris.shape[0], _ptr(out))
+    return out
+
+
+def sort_pairs(keys: np.ndarray):
+    keys = np.ascontiguousarray(keys)
+    n = keys.shape[0]
+    sk = np.zeros_like(keys)
+    si = np.zeros(n, dtype=np.uint32)
+    if keys.dtype == np.uint32:
+        lib().om_sort_pairs_u32(_ptr(keys), n, _ptr(sk), _ptr(si))
+    else:
+        lib().om_sort_pairs_u64(_ptr(keys), n, _ptr(sk), _ptr(si))
+    return sk, si
+
+
+def partition_desc(keys: np.ndarray, begin_bit: int, end_bit: int):
+    keys = np.ascontiguousarray(keys, dtype=np.uint32)
+    n = keys.shape[0]
+    order = np.zeros(n, dtype=np.uint32)
+    offsets = np.zeros((1 << (end_bit - begin_bit)) + 1, dtype=np.uint32)
+    lib().om_partition_desc(_ptr(keys), n, begin_bit, end_bit, _ptr(order), _ptr(offsets))
+    return order, offsets
+
+
+class LBVH:
+    """Oracle LBVH in the reference's node numbering (see oracle.h)."""
+
+    def __init__(self, tris: np.ndarray, bits: int = 30):
+        self.tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        n = self.n = self.tris.shape[0]
+        self.bits = bits
+        self.sorted_codes = np.zeros(n, dtype=np.uint32 if bits == 30 else np.uint64)
+        self.sorted_idx = np.zeros(n, dtype=np.uint32)
+        ni = max(n - 1, 1)
+        self.left = np.full(ni, INVALID, dtype=np.uint32)
+        self.right = np.full(ni, INVALID, dtype=np.uint32)
+        self.parent = np.zeros(2 * n - 1, dtype=np.uint32)
+        self.node_aabb = np.zeros((2 * n - 1, 6), dtype=np.float32)
+        root = C.c_uint32(0)
+        rc = lib().om_lbvh_build(_ptr(self.tris), n, bits, _ptr(self.sorted_codes),
+                                 _ptr(self.sorted_idx), _ptr(self.left), _ptr(self.right),
+                                 _ptr(self.parent), _ptr(self.node_aabb), C.byref(root))
+        if rc != 0:
+            raise RuntimeError(f"om_lbvh_build failed ({rc})")
+        self.root = root.value
+
+    def trace(self, rays: np.ndarray, tmin: float = 0.0, stats: bool = False):
+        """rays: float32 [7, m] planes (ox oy oz dx dy dz tmax)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        m = rays.shape[1]
+        prim = np.zeros(m, dtype=np.uint32)
+        t = np.zeros(m, dtype=np.float32)
+        bary = np.zeros((2, m), dtype=np.float32)
+        st = np.zeros((m, 2), dtype=np.uint32) if stats else None
+        lib().om_trace_closest(_ptr(self.tris), self.n, _ptr(self.sorted_idx), _ptr(self.left),
+                               _ptr(self.right), _ptr(self.node_aabb), self.root, _ptr(rays), m,
+                               m, tmin, _ptr(prim), _ptr(t), _ptr(bary), _ptr(st))
+        if stats:
+            return prim, t, bary, st
+        return prim, t, bary
+
+    def trace_any(self, rays: np.ndarray, tmin: float = 0.0):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        m = rays.shape[1]
+        out = np.zeros(m, dtype=np.uint32)
+        lib().om_trace_any(_ptr(self.tris), self.n, _ptr(self.sorted_idx), _ptr(self.left),
+                           _ptr(self.right), _ptr(self.node_aabb), self.root, _ptr(rays), m, m,
+                           tmin, _ptr(out))
+        return out
+
+
+def trace_brute(tris: np.ndarray, rays: np.ndarray, tmin: float = 0.0):
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    m = rays.shape[1]
+    prim = np.zeros(m, dtype=np.uint32)
+    t = np.zeros(m, dtype=np.float32)
+    bary = np.zeros((2, m), dtype=np.float32)
+    lib().om_trace_brute(_ptr(tris), tris.shape[0], _ptr(rays), m, m, tmin, _ptr(prim),
+                         _ptr(t), _ptr(bary))
+    return prim, t, bary
+
+
+def camera(pos, gaze, up, fov_x, near, far, w, h) -> OmCamera:
+    cam = OmCamera()
+    f3 = lambda v: np.ascontiguousarray(v, dtype=np.float32)
+    p, g, u = f3(pos), f3(gaze), f3(up)
+    lib().om_camera_setup(_ptr(p), _ptr(g), _ptr(u), fov_x, near, far, w, h, C.byref(cam))
+    return cam
+
+
+def camera_ray(cam: OmCamera, w, h, samples, seed, sample_id):
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    st = C.c_uint64(0)
+    lib().om_camera_ray(C.byref(cam), w, h, samples, seed, sample_id, _ptr(o), _ptr(d),
+                        C.byref(st))
+    return o, d, st.value
+
+
+def render(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_depth, rr_start,
+           seed, row_step=1, row_offset=0, threads=1):
+    """scene: dict with tris (n,9), prim_key (n,), lambert (.,3), metal (.,10),
+    dielectric (.,4), emissive (.,3).  Returns (image float64 [h,w,3], segments)."""
+    keep = {}
+
+    def arr(name, dtype, cols):
+        a = scene.get(name)
+        if a is None or len(a) == 0:
+            a = np.zeros((1, cols), dtype=dtype)
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep[name] = a
+        return _ptr(a)
+
+    sc = OmScene()
+    sc.tris = arr("tris", np.float32, 9)
+    sc.prim_key = arr("prim_key", np.uint32, 1)
+    sc.n = bvh.n
+    sc.lambert = arr("lambert", np.float32, 3)
+    sc.metal = arr("metal", np.float32, 10)
+    sc.dielectric = arr("dielectric", np.float32, 4)
+    sc.emissive = arr("emissive", np.float32, 3)
+    sc.sorted_idx = _ptr(bvh.sorted_idx)
+    sc.left = _ptr(bvh.left)
+    sc.right = _ptr(bvh.right)
+    sc.node_aabb = _ptr(bvh.node_aabb)
+    sc.root = bvh.root
+    p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, row_step, row_offset,
+                       threads)
+    img = np.zeros((height, width, 3), dtype=np.float64)
+    seg = lib().om_render(C.byref(sc), C.byref(cam), C.byref(p), _ptr(img))
+    return img / float(samples * samples), int(seg)

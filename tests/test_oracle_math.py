@@ -1,0 +1,381 @@
+"""Pins the oracle where no reference golden vector exists: closed forms,
+invariants, textbook algorithms and brute force on small inputs.  Each check
+is chosen so that a dropped term, a flipped sign or a swapped operand in the
+restatement fails it.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+
+def P(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ----------------------------------------------------------------- sin/cos
+def test_sincos_2pi_accuracy(oracle):
+    L = oracle.lib()
+    u = np.concatenate([np.random.default_rng(0).random(20000), [0.0, 0.25, 0.5, 0.75, 1 - 2 ** -24]])
+    s = np.zeros(1, np.float32)
+    c = np.zeros(1, np.float32)
+    err = 0.0
+    for x in u.astype(np.float32):
+        L.om_sincos_2pi_f(float(x), P(s), P(c))
+        err = max(err, abs(s[0] - math.sin(2 * math.pi * float(x))), abs(c[0] - math.cos(2 * math.pi * float(x))))
+    assert err < 2.5e-7
+
+
+# ----------------------------------------------------------------- Morton
+def _interleave(q, bits):
+    code = 0
+    for b in range(bits):
+        for a in range(3):
+            code |= ((int(q[a]) >> b) & 1) << (3 * b + a)
+    return code
+
+
+def test_morton_compose_is_bit_interleave(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(1)
+    for q in rng.integers(0, 1024, (500, 3)):
+        assert L.om_morton_compose30(*map(int, q)) == _interleave(q, 10)
+    for q in rng.integers(0, 1 << 21, (500, 3)):
+        assert L.om_morton_compose63(*map(int, q)) == _interleave(q, 21)
+
+
+def test_quantiser_closed_form(oracle):
+    """Corner leaves quantise to 0 and to the clamp 2^b - 1 (the reference
+    wraps the latter to 0 through its mask; recorded divergence)."""
+    L = oracle.lib()
+    tris = np.array([[0, 0, 0, 0, 0, 0, 0, 0, 0],
+                     [4, 2, 1, 4, 2, 1, 4, 2, 1],
+                     [1, 1, 1, 1, 1, 1, 1, 1, 1]], np.float32)
+    codes = oracle.morton_codes(tris, 30)
+    assert codes[0] == 0
+    assert codes[1] == L.om_morton_compose30(1023, 512, 256)
+    assert codes[2] == L.om_morton_compose30(256, 256, 256)
+    ext = np.array([0, 0, 0, 4, 2, 1], np.float32)
+    assert L.om_morton_delta(P(ext), 10) == np.float32(4.0 / 1024)
+    # all leaves at one point: delta 0 -> code 0
+    same = np.tile(np.array([[0.5] * 9], np.float32), (5, 1))
+    assert not oracle.morton_codes(same, 63).any()
+
+
+# ----------------------------------------------------------------- sorting
+@pytest.mark.parametrize("n", [1, 7, 1000, 50000])
+def test_sort_pairs_is_stable_sort(oracle, n):
+    rng = np.random.default_rng(n)
+    k32 = rng.integers(0, 64, n).astype(np.uint32)   # many duplicates
+    sk, si = oracle.sort_pairs(k32)
+    ref = np.argsort(k32, kind="stable")
+    assert np.array_equal(si, ref) and np.array_equal(sk, k32[ref])
+    k64 = rng.integers(0, 1 << 62, n, dtype=np.uint64)
+    sk, si = oracle.sort_pairs(k64)
+    assert np.array_equal(si, np.argsort(k64, kind="stable"))
+
+
+def test_partition_matches_reference_test_model(oracle):
+    """T_ParallelPartition.cu:19-92 model: random keys, per-partition count and
+    membership against a CPU histogram; plus stable descending order."""
+    rng = np.random.default_rng(0)
+    for it in range(5):
+        n = 75000
+        batch = rng.integers(0, 256, n).astype(np.uint32)
+        ids = rng.integers(0, 1 << 24, n).astype(np.uint32)
+        keys = (batch << 24) | ids
+        order, offsets = oracle.partition_desc(keys, 24, 32)
+        ref = np.argsort(-batch.astype(np.int64), kind="stable")
+        assert np.array_equal(order, ref)
+        counts = np.bincount(batch, minlength=256)
+        for j in range(256):
+            b = 255 - j
+            seg = order[offsets[j]:offsets[j + 1]]
+            assert len(seg) == counts[b]
+            assert np.all(batch[seg] == b)
+        assert offsets[-1] == n
+
+
+# ----------------------------------------------------------------- LBVH
+def _aug_delta(codes, i, j, bits):
+    W = 32 if bits == 30 else 64
+    x = int(codes[i]) ^ int(codes[j])
+    if x:
+        return W - x.bit_length()
+    return W + (32 - (i ^ j).bit_length())
+
+
+def _topdown(codes, bits):
+    """Radix tree by its top-down definition (Karras 2012): the split of
+    [l, r] is the last position whose augmented common prefix with l exceeds
+    the prefix of the whole range.  Internal node = n + split."""
+    n = len(codes)
+    left, right = {}, {}
+
+    def build(l, r):
+        if l == r:
+            return l
+        dn = _aug_delta(codes, l, r, bits)
+        s = l
+        for g in range(l, r):
+            if _aug_delta(codes, l, g, bits) > dn:
+                s = g
+        # the split is the largest g with delta(l, g) > delta(l, r); g = l
+        # always qualifies when the range's first two keys share a longer prefix
+        node = n + s
+        left[s] = build(l, s)
+        right[s] = build(s + 1, r)
+        return node
+
+    root = build(0, n - 1)
+    return root, left, right
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 33, 200])
+def test_lbvh_equals_topdown_radix_tree(oracle, bits, n):
+    rng = np.random.default_rng(n + bits)
+    tris = rng.uniform(0, 1, (n, 9)).astype(np.float32)
+    if n >= 8:
+        tris[n // 2] = tris[0]          # duplicate codes -> index tie-break
+        tris[n // 2 + 1] = tris[0]
+    b = oracle.LBVH(tris, bits)
+    root, left, right = _topdown(b.sorted_codes, bits)
+    assert b.root == root
+    for s in range(n - 1):
+        assert b.left[s] == left[s] and b.right[s] == right[s], s
+
+
+def _check_lbvh_invariants(b, tris):
+    n = b.n
+    aabb = np_tri_aabbs(tris)
+    # leaves carry their primitive's box
+    assert np.array_equal(b.node_aabb[:n], aabb[b.sorted_idx])
+    # parent/child consistency and split-index numbering
+    lo = np.zeros(2 * n - 1, np.int64)
+    hi = np.zeros(2 * n - 1, np.int64)
+    lo[:n] = np.arange(n)
+    hi[:n] = np.arange(n)
+    order = []
+    stack = [b.root]
+    while stack:
+        x = stack.pop()
+        order.append(x)
+        if x >= n:
+            p = x - n
+            stack += [int(b.left[p]), int(b.right[p])]
+            assert b.parent[b.left[p]] == x and b.parent[b.right[p]] == x
+    assert len(order) == 2 * n - 1                 # every node reached once
+    for x in reversed(order):
+        if x >= n:
+            p = x - n
+            l, r = int(b.left[p]), int(b.right[p])
+            assert hi[l] == p and lo[r] == p + 1    # split index numbering
+            lo[x], hi[x] = lo[l], hi[r]
+            box = b.node_aabb[x]
+            assert np.array_equal(box[:3], np.minimum(b.node_aabb[l][:3], b.node_aabb[r][:3]))
+            assert np.array_equal(box[3:], np.maximum(b.node_aabb[l][3:], b.node_aabb[r][3:]))
+    assert lo[b.root] == 0 and hi[b.root] == n - 1
+    assert b.parent[b.root] == 0xFFFFFFFF
+
+
+def np_tri_aabbs(tris):
+    t = tris.reshape(-1, 3, 3)
+    return np.concatenate([t.min(axis=1), t.max(axis=1)], axis=1).astype(np.float32)
+
+
+def test_lbvh_invariants_random_and_duplicates(oracle):
+    from meturay_amd import native as N
+    tris = N.gen_random_tris(20000, 1)
+    _check_lbvh_invariants(oracle.LBVH(tris, 30), tris)
+    dup = np.repeat(tris[:50], 40, axis=0)          # 40 copies of every triangle
+    _check_lbvh_invariants(oracle.LBVH(dup, 30), dup)
+    _check_lbvh_invariants(oracle.LBVH(dup, 63), dup)
+
+
+# ----------------------------------------------------------------- traversal
+def test_closest_hit_equals_brute_force(oracle):
+    from meturay_amd import native as N
+    tris = N.gen_random_tris(3000, 3)
+    rays = N.gen_random_rays(4000, 3)
+    b = oracle.LBVH(tris, 30)
+    p1, t1, b1 = b.trace(rays)
+    p2, t2, b2 = oracle.trace_brute(tris, rays)
+    assert np.array_equal(p1, p2)
+    assert np.array_equal(t1, t2) and np.array_equal(b1, b2)
+    anyh = b.trace_any(rays)
+    assert np.array_equal(anyh.astype(bool), p2 != 0xFFFFFFFF)
+    # tmin excludes hits closer than it
+    p3, t3, _ = b.trace(rays, tmin=0.05)
+    p4, t4, _ = oracle.trace_brute(tris, rays, tmin=0.05)
+    assert np.array_equal(p3, p4)
+
+
+# ----------------------------------------------------------------- BxDFs
+def test_fresnel_dielectric_closed_forms(oracle):
+    L = oracle.lib()
+    for n1, n2 in [(1.0, 1.5), (1.5, 1.0), (1.0, 2.4)]:
+        r0 = ((n1 - n2) / (n1 + n2)) ** 2
+        assert abs(L.om_fresnel_dielectric(1.0, n1, n2) - r0) < 1e-6
+    # total internal reflection past the critical angle
+    crit = math.sqrt(1 - (1 / 1.5) ** 2)
+    assert L.om_fresnel_dielectric(crit * 0.9, 1.5, 1.0) == 1.0
+    # grazing incidence reflects everything
+    assert abs(L.om_fresnel_dielectric(1e-6, 1.0, 1.5) - 1.0) < 1e-4
+
+
+def _conductor_closed_form(c, eta, k):
+    """pbrt-v3 FrConductor with the reference's exponents: TracerFunctions.cuh
+    names sin^2 squared "sinInSqr2" and uses it (sin^4) where pbrt has sin^2,
+    and sin^8 where pbrt has sin^4.  The restatement keeps that quirk."""
+    s2 = 1.0 - c * c
+    t0 = eta * eta - k * k - s2
+    a2b2 = math.sqrt(t0 * t0 + 4 * eta * eta * k * k)
+    a = math.sqrt(max(0.5 * (a2b2 + t0), 0.0))
+    rs = (a2b2 + c * c - 2 * a * c) / (a2b2 + c * c + 2 * a * c)
+    t3 = c * c * a2b2 + s2 ** 4
+    t4 = 2 * a * c * s2 ** 2
+    rp = rs * (t3 - t4) / (t3 + t4)
+    return 0.5 * (rs + rp)
+
+
+def test_fresnel_conductor_closed_form(oracle):
+    L = oracle.lib()
+    out = np.zeros(3, np.float32)
+    for cosv in [1.0, 0.8, 0.5, 0.2, 0.05]:
+        eta = np.array([0.143, 0.374, 1.442], np.float32)
+        k = np.array([3.983, 2.385, 1.603], np.float32)
+        L.om_fresnel_conductor(cosv, P(eta), P(k), P(out))
+        for c in range(3):
+            ref = _conductor_closed_form(cosv, float(eta[c]), float(k[c]))
+            assert abs(out[c] - ref) < 2e-5 * max(1.0, ref)
+    # k = 0 at normal incidence is the dielectric reflectance ((n-1)/(n+1))^2
+    eta = np.array([1.5, 2.0, 1.2], np.float32)
+    L.om_fresnel_conductor(1.0, P(eta), P(np.zeros(3, np.float32)), P(out))
+    for c in range(3):
+        assert abs(out[c] - ((eta[c] - 1) / (eta[c] + 1)) ** 2) < 1e-6
+
+
+def test_ggx_ndf_normalisation_and_smith(oracle):
+    L = oracle.lib()
+    th = np.linspace(0, math.pi / 2, 200001)
+    for alpha in [0.1, 0.3, 0.5]:
+        D = np.array([L.om_dggx(float(math.cos(x)), alpha) for x in th[::50]])
+        integrand = D * np.cos(th[::50]) * np.sin(th[::50]) * 2 * math.pi
+        assert abs(np.trapezoid(integrand, th[::50]) - 1.0) < 2e-3
+        for t in [0.1, 0.7, 1.2]:
+            v = np.array([math.sin(t), 0, math.cos(t)], np.float32)
+            g1 = 2.0 / (1.0 + math.sqrt(1.0 + alpha ** 2 * math.tan(t) ** 2))
+            assert abs(L.om_gsmith_single(P(v), alpha) - g1) < 1e-5
+
+
+def test_vndf_pdf_normalised_and_matches_samples(oracle):
+    L = oracle.lib()
+    alpha = 0.4
+    v = np.array([math.sin(0.8), 0.0, math.cos(0.8)], np.float32)
+    # pdf over microfacet normals integrates to one
+    th = np.linspace(0, math.pi / 2, 241)
+    ph = np.linspace(0, 2 * math.pi, 241)
+    acc = 0.0
+    for i in range(len(th) - 1):
+        t = 0.5 * (th[i] + th[i + 1])
+        for j in range(len(ph) - 1):
+            p = 0.5 * (ph[j] + ph[j + 1])
+            h = np.array([math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)], np.float32)
+            acc += L.om_vndf_pdf(P(v), P(h), alpha) * math.sin(t) * (th[1] - th[0]) * (ph[1] - ph[0])
+    assert abs(acc - 1.0) < 1e-2
+    # sampled normals follow the pdf: compare the mean of cos(theta_h)
+    rng = np.random.default_rng(3)
+    h = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    cs = []
+    for xi in rng.random((40000, 2)).astype(np.float32):
+        L.om_vndf_sample(P(v), alpha, float(xi[0]), float(xi[1]), P(h), P(pdf))
+        cs.append(h[2])
+        assert pdf[0] >= 0
+    mean_mc = float(np.mean(cs))
+    acc = 0.0
+    for i in range(len(th) - 1):
+        t = 0.5 * (th[i] + th[i + 1])
+        for j in range(0, len(ph) - 1, 4):
+            p = 0.5 * (ph[j] + ph[j + 4]) if j + 4 < len(ph) else ph[j]
+            hh = np.array([math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)], np.float32)
+            acc += L.om_vndf_pdf(P(v), P(hh), alpha) * math.cos(t) * math.sin(t) * (th[1] - th[0]) * (ph[4] - ph[0])
+    assert abs(mean_mc - acc) < 1e-2
+
+
+def test_refract_obeys_snell(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(5)
+    n = np.array([0, 0, 1], np.float32)
+    for _ in range(200):
+        t = rng.uniform(0, 1.2)
+        wi = np.array([math.sin(t), 0, math.cos(t)], np.float32)
+        out = np.zeros(3, np.float32)
+        assert L.om_refract(P(wi), P(n), 1.0, 1.5, P(out)) == 1
+        assert abs(np.linalg.norm(out) - 1) < 1e-5
+        assert out[2] < 0                                        # crossed the surface
+        assert abs(math.hypot(out[0], out[1]) - math.sin(t) / 1.5) < 1e-5
+        assert out[0] <= 1e-7                                   # tangential part flips side
+    # total internal reflection from the dense side
+    wi = np.array([math.sin(1.0), 0, math.cos(1.0)], np.float32)
+    assert L.om_refract(P(wi), P(n), 1.5, 1.0, P(np.zeros(3, np.float32))) == 0
+
+
+# ----------------------------------------------------------------- path tracer
+def _furnace_scene(inner_batch, inner_params):
+    """Camera inside a closed emissive box (radiance 1) looking at an
+    icosphere of one material: pixels on the box read exactly 1; the sphere
+    reads its directional albedo."""
+    from meturay_amd import native as N
+    sc = N.gen_cornell(3)
+    keys = sc["prim_key"].copy()
+    keys[:14] = (4 << 24) | 0                                   # walls + light emissive
+    keys[14:] = (inner_batch << 24) | 0
+    sc["prim_key"] = keys
+    sc["emissive"] = np.ones((1, 3), np.float32)
+    if inner_batch == 1:
+        sc["lambert"] = np.array([inner_params], np.float32)
+    elif inner_batch == 2:
+        sc["metal"] = np.array([inner_params], np.float32)
+    else:
+        sc["dielectric"] = np.array([inner_params], np.float32)
+    return sc
+
+
+def _render(oracle, sc, w, h, S, depth, rr):
+    bvh = oracle.LBVH(sc["tris"], 30)
+    c = sc["camera"]
+    cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], w, h)
+    return oracle.render(sc, bvh, cam, w, h, S, depth, rr, 0, threads=8)
+
+
+def test_furnace_lambert_estimator(oracle):
+    img, seg = _render(oracle, _furnace_scene(1, [0.5, 0.5, 0.5]), 24, 24, 16, 16, 1000)
+    # centre pixels see the sphere: expected albedo * 1 with uniform-hemisphere
+    # sampling weight 2 cos(theta) albedo (unbiased); border pixels see walls
+    centre = img[10:14, 10:14].mean()
+    assert abs(centre - 0.5) < 0.03
+    assert abs(img[0, 0].mean() - 1.0) < 1e-6
+
+
+def test_furnace_dielectric_conserves_energy(oracle):
+    img, _ = _render(oracle, _furnace_scene(3, [1.0, 1.0, 1.0, 1.5]), 16, 16, 4, 32, 1000)
+    # each Fresnel-chosen branch carries weight 1: every escaping path reads 1
+    assert img.max() <= 1.0 + 1e-6
+    assert img[6:10, 6:10].mean() > 0.97
+
+
+def test_furnace_conductor_bounded(oracle):
+    img, _ = _render(oracle, _furnace_scene(2, [0.3, 0.2, 0.2, 0.2, 3.0, 3.0, 3.0, 1, 1, 1]), 16, 16, 8, 16, 1000)
+    centre = img[6:10, 6:10].mean()
+    assert 0.4 < centre < 1.0
+
+
+def test_russian_roulette_is_unbiased(oracle):
+    sc = _furnace_scene(1, [0.7, 0.7, 0.7])
+    a, s1 = _render(oracle, sc, 16, 16, 16, 24, 1000)
+    b, s2 = _render(oracle, sc, 16, 16, 16, 24, 0)
+    assert s2 < s1                                             # RR terminated paths
+    assert abs(a[6:10, 6:10].mean() - b[6:10, 6:10].mean()) < 0.03

@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the wavefront path-tracing hot path on one MI355X.
+
+Workload (one "step" = one full frame): a 64x64 grid of displaced UV
+spheres (16 stacks x 16 slices, 1,966,080 triangles, seed 2) with Lambert /
+GGX conductor / dielectric materials and 5% emissive spheres, rendered at
+1920x1080 with 16 spp (4x4 strata), max depth 10, Russian roulette after
+depth 3, the ray state partitioned by material key every bounce.  The scene
+and its LBVH are resident in HBM before the timed region.
+
+Prints one JSON line on stdout.  Progress goes to stderr.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
+W, H, S, DEPTH, RR, SEED = 1920, 1080, 4, 10, 3, 0
+GRID, STACKS, SLICES, SCENE_SEED = 64, 16, 16, 2
+
+# algorithmic bytes of one closest-hit ray cast (per ray: origin + direction
+# 24 B read; leaf, t, 2 barycentrics, HitKey 20 B written), of one internal
+# node visit (one 64 B record) and of one triangle test (one 48 B leaf record)
+RAY_BYTES, NODE_BYTES, LEAF_BYTES = 44, 64, 48
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, target_s: float):
+    """The oracle (C restatement, FP32 paths, FP64 accumulation) on host
+    threads over a row subset of the same frame."""
+    from oracle import oracle as O
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    t0 = time.perf_counter()
+    bvh = O.LBVH(sc["tris"], 30)
+    build_s = time.perf_counter() - t0
+    c = sc["camera"]
+    cam = O.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], W, H)
+    # calibrate on one row per thread, then size the sample for about target_s seconds
+    step = max(1, H // threads)
+    t0 = time.perf_counter()
+    _, seg = O.render(sc, bvh, cam, W, H, S, DEPTH, RR, SEED, row_step=step, row_offset=step // 2,
+                      threads=threads)
+    dt = max(time.perf_counter() - t0, 1e-3)
+    rows_per_s = len(range(step // 2, H, step)) / dt
+    rows = int(max(4, min(H, rows_per_s * target_s)))
+    step = max(1, H // rows)
+    rows = len(range(step // 2, H, step))
+    t0 = time.perf_counter()
+    _, seg = O.render(sc, bvh, cam, W, H, S, DEPTH, RR, SEED, row_step=step, row_offset=step // 2,
+                      threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": seg / dt / 1e6,
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{rows} of 1080 rows (every {step}th) x 1920 px x 16 spp of the same frame: "
+                  f"{seg} ray segments in {dt:.2f} s; CPU LBVH build {build_s:.2f} s not included",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(requests shard by image tile or by request across GPUs; see the design notes).",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from meturay_amd import native as N
+
+    log("generating scene ...")
+    sc = N.gen_sphere_grid(GRID, STACKS, SLICES, SCENE_SEED)
+    ntri = int(sc["tris"].shape[0])
+    ctx = N.Context(0)
+    t0 = time.perf_counter()
+    scene = N.Scene(ctx, sc)
+    build_ms = (time.perf_counter() - t0) * 1e3
+    log(f"scene: {ntri} triangles, upload + LBVH build {build_ms:.1f} ms")
+    img = torch.zeros(W * H * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    for i in range(args.warmup):
+        st = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
+        log(f"warmup {i}: {st.segments} segments, {st.render_ms:.1f} ms")
+
+    ctx.profile(True)
+    ctx.profile_reset()
+    lat = []
+    segs = 0
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        st = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        segs += st.segments
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    trace_ms, trace_launches = ctx.profile_get("trace_pt")
+    shade_ms, _ = ctx.profile_get("shade")
+    sort_ms, _ = ctx.profile_get("sort_onesweep")
+    hist_ms, _ = ctx.profile_get("sort_hist")
+    ctx.profile(False)
+    log(f"timed: {args.steps} frames in {t_total:.3f} s; trace {trace_ms:.1f} ms, shade {shade_ms:.1f} ms, "
+        f"partition {sort_ms + hist_ms:.1f} ms")
+
+    # traversal work of the same frame (same seed -> same paths), counted
+    # outside the timed region
+    stc = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED, count_traversal=True)
+    assert stc.segments == st.segments
+    bytes_per_frame = (RAY_BYTES * stc.segments + NODE_BYTES * stc.node_visits
+                       + LEAF_BYTES * stc.tri_tests)
+    trace_s_per_frame = trace_ms / 1e3 / args.steps
+    achieved = bytes_per_frame / trace_s_per_frame / 1e9
+    img_np = img.cpu().numpy()
+    log(f"per ray: {stc.node_visits / stc.segments:.1f} node visits, "
+        f"{stc.tri_tests / stc.segments:.2f} triangle tests; image mean {img_np.mean():.4f}")
+
+    value = segs / t_total / 1e6
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline ...")
+        cpu = cpu_baseline(sc, args.cpu_seconds)
+        log(f"CPU: {cpu['value']:.2f} Mrays/s on {cpu['cores']} threads")
+
+    lat_sorted = np.sort(np.array(lat))
+    out = {
+        "metric": "Mrays/s path-traced at 1080p (% of HBM roofline); p50/p99 request latency",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "displaced UV-sphere grid 64x64 (16x16 tessellation), Lambert/GGX conductor/"
+                        "dielectric + 5% emissive, 1920x1080, 16 spp, max depth 10, RR after 3, "
+                        "material-key partition every bounce",
+            "triangles": ntri,
+            "resolution": "1920x1080",
+            "spp": S * S,
+            "max_depth": DEPTH,
+            "segments_per_frame": int(st.segments),
+            "bvh_build_ms": round(build_ms, 2),
+            "latency_ms": {"p50": round(float(np.percentile(lat_sorted, 50)), 3),
+                           "p99": round(float(np.percentile(lat_sorted, 99)), 3)},
+            "kernel_ms_per_frame": {"trace": round(trace_ms / args.steps, 3),
+                                    "shade": round(shade_ms / args.steps, 3),
+                                    "partition": round((sort_ms + hist_ms) / args.steps, 3)},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "trace_pt",
+            "bytes_per_frame": int(bytes_per_frame),
+            "launches_per_frame": trace_launches // max(args.steps, 1),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    scene.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

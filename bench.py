@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 512 MiB device-to-device first: the reference kernel for scaling "
+                         "FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -102,6 +105,13 @@ def main():
     log(f"scene: {ntri} triangles, upload + LBVH build {build_ms:.1f} ms")
     img = torch.zeros(W * H * 3, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
+    if args.copy_calibration:
+        a = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("copy calibration: 536870912 bytes read + written")
 
     for i in range(args.warmup):
         st = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)

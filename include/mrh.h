@@ -33,6 +33,7 @@ typedef enum {
 typedef struct mrh_context mrh_context;
 typedef struct mrh_lbvh mrh_lbvh;
 typedef struct mrh_scene mrh_scene;
+typedef struct mrh_tracer mrh_tracer;
 
 /* ------------------------------------------------------------------ */
 /* Context: device, stream, scratch memory, per-kernel timers.         */
@@ -50,6 +51,10 @@ int32_t     mrh_stream(mrh_context* ctx, void** stream);
  * (hipMemcpyDefault), then wait for it.  Utility for hosts that keep their
  * buffers outside any framework. */
 int32_t     mrh_memcpy(mrh_context* ctx, void* dst, const void* src, size_t bytes);
+/* Device memory on the context's device for hosts without an allocator of
+ * their own (DeviceMemory, TracerCUDA/DeviceMemory.cpp; no managed memory). */
+int32_t     mrh_device_alloc(mrh_context* ctx, void** out, size_t bytes);
+int32_t     mrh_device_free(mrh_context* ctx, void* ptr);
 /* Per-kernel hipEvent timing on the context stream (the reference's
  * cudaEvent timers, WFPGTracer.cu:592-596).  When enabled every launch is
  * bracketed by events; mrh_profile_get returns total ms and launches for a
@@ -221,6 +226,23 @@ int32_t mrh_render(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* ca
 
 /* Device bytes mrh_render needs for a frame of this size */
 int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes);
+
+/* Incremental tracer over a resident scene, the GPUTracerI protocol
+ * (RayLib/GPUTracerI.h:30-72): mrh_tracer_create ~ GenerateTracer +
+ * Initialize + SetOptions; mrh_tracer_generate_work ~ GenerateWork(camera)
+ * (raygen of one frame); mrh_tracer_render ~ Render(): one bounce per call,
+ * *more = 0 once MaxDepth bounces are done (PathTracer.cu:144-291);
+ * mrh_tracer_finalize ~ Finalize(): writes the mean of every finished frame
+ * to d_image (progressive accumulation; frame k uses global sample ids
+ * k*W*H*spp + i); mrh_tracer_reset_image ~ ResetImage(). */
+int32_t mrh_tracer_create(mrh_context* ctx, mrh_scene* scene, const mrh_render_params* params,
+                          mrh_tracer** out);
+int32_t mrh_tracer_generate_work(mrh_tracer* tracer, const mrh_camera_desc* cam);
+int32_t mrh_tracer_render(mrh_tracer* tracer, int32_t* more);
+int32_t mrh_tracer_finalize(mrh_tracer* tracer, float* d_image);
+int32_t mrh_tracer_reset_image(mrh_tracer* tracer);
+int32_t mrh_tracer_stats(const mrh_tracer* tracer, mrh_render_stats* out, uint32_t* frames);
+int32_t mrh_tracer_destroy(mrh_tracer* tracer);
 
 /* ------------------------------------------------------------------ */
 /* Synthetic workloads (host only; no device needed).                   */

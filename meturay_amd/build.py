@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC,
 ]
-CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+CXX_FLAGS = ["-O2", "-fPIC", "-std=c++20", "-ffp-contract=off", "-fno-fast-math", "-Wall",
              "-I", INCLUDE, "-I", CSRC]
 
 
@@ -60,6 +60,14 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the C++ host that drives the library through TracerSystemI (tests)
+    host_src = os.path.join(ROOT, "tests", "cpp", "tracer_system_host.cpp")
+    if os.path.exists(host_src):
+        exe = os.path.join(BUILD, "tracer_system_host")
+        cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-I", INCLUDE, host_src, "-o", exe, "-ldl"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
     return LIB

@@ -171,6 +171,22 @@ int32_t mrh_memcpy(mrh_context* ctx, void* dst, const void* src, size_t bytes)
     return MRH_OK;
 }
 
+int32_t mrh_device_alloc(mrh_context* ctx, void** out, size_t bytes)
+{
+    if (!ctx || !out) return MRH_ERR_INVALID;
+    MRH_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return dmalloc(ctx, out, bytes);
+}
+
+int32_t mrh_device_free(mrh_context* ctx, void* ptr)
+{
+    if (!ctx) return MRH_ERR_INVALID;
+    if (!ptr) return MRH_OK;
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    MRH_HIP_CHECK(ctx, hipFree(ptr));
+    return MRH_OK;
+}
+
 int32_t mrh_profile_enable(mrh_context* ctx, int32_t on)
 {
     if (!ctx) return MRH_ERR_INVALID;

@@ -92,7 +92,8 @@ Cam make_camera(const mrh_camera_desc& c, uint32_t w, uint32_t h)
 
 // GenerationKernels.cuh:29-108 + GPUCameraPinhole.cuh:209-240
 __global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h, uint32_t S,
-                                                uint32_t seed, uint32_t n_total, PathBuf out)
+                                                uint32_t seed, uint32_t sample_base, uint32_t n_total,
+                                                PathBuf out)
 {
     const uint32_t spp = S * S;
     const float dxp = cam.pw / (float)(w * S);
@@ -100,7 +101,8 @@ __global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h,
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_total; i += gridDim.x * blockDim.x) {
         const uint32_t pix = i / spp, s = i - pix * spp;
         const uint32_t px = pix % w, py = pix / w, sx = s % S, sy = s / S;
-        Rng rng{path_rng_init(seed, i), pcg32_inc(i)};
+        const uint32_t gid = sample_base + i;   // global sample id keys the stream
+        Rng rng{path_rng_init(seed, gid), pcg32_inc(gid)};
         const float jx = rng.uniform();
         const float jy = rng.uniform();
         float sdx = (float)(px * S + sx) * dxp;
@@ -396,8 +398,8 @@ __device__ __forceinline__ void dielectric_sample(const Surface& s, V3 wi, const
 // PathTracerKC.cuh:337-426 (light hit) and 430-648 (surface work)
 __global__ __launch_bounds__(256) void k_shade(
     PathBuf in, PathBuf out, HitBuf hit, const uint32_t* __restrict__ order, uint32_t* counts,
-    uint32_t bounce, uint32_t last, uint32_t rr_start, uint32_t spp, Materials mats,
-    const float4* __restrict__ leaves, float* __restrict__ image)
+    uint32_t bounce, uint32_t last, uint32_t rr_start, uint32_t spp, uint32_t sample_base,
+    Materials mats, const float4* __restrict__ leaves, float* __restrict__ image)
 {
     const uint32_t n = counts[bounce];
     const uint32_t lane = threadIdx.x & 63;
@@ -430,7 +432,7 @@ __global__ __launch_bounds__(256) void k_shade(
                     const Surface s = make_surface(v3(l0.x, l0.y, l0.z), v3(l1.x, l1.y, l1.z),
                                                    v3(l2.x, l2.y, l2.z), hit.ba[rid], hit.bb[rid], d);
                     const V3 wi = -normalize(d);
-                    Rng rng{in.rng[rid], pcg32_inc(sample)};
+                    Rng rng{in.rng[rid], pcg32_inc(sample_base + sample)};
                     const uint32_t dm = in.dm[rid];
                     const uint32_t depth = dm & 0xFFu, medium = dm >> 8;
                     BsdfOut o;
@@ -495,7 +497,7 @@ size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t frame_bytes(uint64_t cap)
 {
-    // 2 path buffers (9 floats + sample + dm + u64 rng = 13 words),
+    // 2 path buffers (9 floats + sample + dm = 11 words, + u64 rng),
     // hits (4 words) + key + order, plus counts/offsets
     size_t per = 0;
     per += 2 * (11 * align_up(cap * 4) + align_up(cap * 8));
@@ -504,11 +506,20 @@ size_t frame_bytes(uint64_t cap)
     return per;
 }
 
-int32_t carve_frame(mrh_context* ctx, uint64_t cap, PathBuf* bufs, HitBuf* hit, uint32_t** order,
-                    uint32_t** counts, uint32_t** offsets)
+// One frame in flight: path/hit buffers carved from an arena, the bounce
+// counter and the frame's constants.
+struct Frame {
+    PathBuf bufs[2];
+    HitBuf hit;
+    uint32_t *order = nullptr, *counts = nullptr, *offsets = nullptr;
+    Cam cam;
+    uint64_t total = 0, spp = 0;
+    uint32_t w = 0, h = 0, S = 0, max_depth = 0, rr_start = 0, seed = 0, sample_base = 0;
+    uint32_t count_traversal = 0, bounce = 0;
+};
+
+void carve_frame(void* base, uint64_t cap, Frame& f)
 {
-    void* base = nullptr;
-    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(cap), &base));
     char* p = (char*)base;
     auto take = [&](size_t bytes) {
         char* r = p;
@@ -516,7 +527,7 @@ int32_t carve_frame(mrh_context* ctx, uint64_t cap, PathBuf* bufs, HitBuf* hit, 
         return r;
     };
     for (int b = 0; b < 2; b++) {
-        PathBuf& q = bufs[b];
+        PathBuf& q = f.bufs[b];
         q.ox = (float*)take(cap * 4); q.oy = (float*)take(cap * 4); q.oz = (float*)take(cap * 4);
         q.dx = (float*)take(cap * 4); q.dy = (float*)take(cap * 4); q.dz = (float*)take(cap * 4);
         q.tr = (float*)take(cap * 4); q.tg = (float*)take(cap * 4); q.tb = (float*)take(cap * 4);
@@ -524,15 +535,14 @@ int32_t carve_frame(mrh_context* ctx, uint64_t cap, PathBuf* bufs, HitBuf* hit, 
         q.dm = (uint32_t*)take(cap * 4);
         q.rng = (uint64_t*)take(cap * 8);
     }
-    hit->leaf = (uint32_t*)take(cap * 4);
-    hit->t = (float*)take(cap * 4);
-    hit->ba = (float*)take(cap * 4);
-    hit->bb = (float*)take(cap * 4);
-    hit->key = (uint32_t*)take(cap * 4);
-    *order = (uint32_t*)take(cap * 4);
-    *counts = (uint32_t*)take(COUNTS_WORDS * 4);
-    *offsets = (uint32_t*)take(260 * 4);
-    return MRH_OK;
+    f.hit.leaf = (uint32_t*)take(cap * 4);
+    f.hit.t = (float*)take(cap * 4);
+    f.hit.ba = (float*)take(cap * 4);
+    f.hit.bb = (float*)take(cap * 4);
+    f.hit.key = (uint32_t*)take(cap * 4);
+    f.order = (uint32_t*)take(cap * 4);
+    f.counts = (uint32_t*)take(COUNTS_WORDS * 4);
+    f.offsets = (uint32_t*)take(260 * 4);
 }
 
 template <class T>
@@ -601,101 +611,139 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
     return MRH_OK;
 }
 
-int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const mrh_render_params* p,
-               float* d_image, mrh_render_stats* stats)
+int32_t validate_params(mrh_context* ctx, const mrh_scene* s, const mrh_render_params* p, uint64_t* total)
 {
     if (p->nee) return fail(ctx, MRH_ERR_UNSUPPORTED, "NextEventEstimation is not implemented");
     if (p->width == 0 || p->height == 0 || p->samples == 0)
-        return fail(ctx, MRH_ERR_INVALID, "mrh_render: empty frame");
+        return fail(ctx, MRH_ERR_INVALID, "render: empty frame");
     if (p->max_depth == 0 || p->max_depth > MAX_BOUNCES)
-        return fail(ctx, MRH_ERR_INVALID, "mrh_render: max_depth must be in [1, %u]", MAX_BOUNCES);
-    const uint64_t spp = (uint64_t)p->samples * p->samples;
-    const uint64_t total = (uint64_t)p->width * p->height * spp;
-    if (total >= (1ull << 30)) return fail(ctx, MRH_ERR_INVALID, "mrh_render: %llu paths exceed 2^30",
-                                           (unsigned long long)total);
-    const mrh_lbvh* b = s->bvh;
-    if (b->max_depth > 128)
-        return fail(ctx, MRH_ERR_UNSUPPORTED, "tree depth %u exceeds the traversal stack", b->max_depth);
+        return fail(ctx, MRH_ERR_INVALID, "render: max_depth must be in [1, %u]", MAX_BOUNCES);
+    const uint64_t t = (uint64_t)p->width * p->height * p->samples * p->samples;
+    if (t >= (1ull << 30))
+        return fail(ctx, MRH_ERR_INVALID, "render: %llu paths exceed 2^30", (unsigned long long)t);
+    if (s->bvh->max_depth > 128)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "tree depth %u exceeds the traversal stack", s->bvh->max_depth);
+    *total = t;
+    return MRH_OK;
+}
 
-    PathBuf bufs[2];
-    HitBuf hit;
-    uint32_t *order = nullptr, *counts = nullptr, *offsets = nullptr;
-    MRH_TRY(carve_frame(ctx, total, bufs, &hit, &order, &counts, &offsets));
+// GenerateWork: clear counts, write the camera rays of one frame
+int32_t frame_begin(mrh_context* ctx, const mrh_camera_desc* cd, const mrh_render_params* p,
+                    void* arena, uint32_t sample_base, Frame& f)
+{
+    f.w = p->width;
+    f.h = p->height;
+    f.S = p->samples;
+    f.spp = (uint64_t)p->samples * p->samples;
+    f.total = (uint64_t)p->width * p->height * f.spp;
+    f.max_depth = p->max_depth;
+    f.rr_start = p->rr_start;
+    f.seed = p->seed;
+    f.sample_base = sample_base;
+    f.count_traversal = p->count_traversal;
+    f.bounce = 0;
+    f.cam = make_camera(*cd, p->width, p->height);
+    carve_frame(arena, f.total, f);
+    MRH_TRY(fill_u32(ctx, f.counts, 0u, COUNTS_WORDS));
+    MRH_TRY(fill_u32(ctx, f.counts, (uint32_t)f.total, 1));
+    return launch(ctx, "raygen", dim3(stride_grid(ctx, f.total, 256)), dim3(256), 0, k_raygen, f.cam,
+                  f.w, f.h, f.S, f.seed, f.sample_base, (uint32_t)f.total, f.bufs[0]);
+}
+
+// One iteration of PathTracer::Render: trace, partition, shade
+int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* image)
+{
+    const mrh_lbvh* b = s->bvh;
+    const uint32_t bounce = f.bounce;
+    const PathBuf& in = f.bufs[bounce & 1];
+    const PathBuf& outb = f.bufs[(bounce + 1) & 1];
+    const float tmin = (bounce == 0) ? f.cam.near_d : 0.0f;
+    const float tmax = (bounce == 0) ? f.cam.far_d : FLT_MAX;
+    const uint32_t root = root_code(b);
+    const uint32_t grid = stride_grid(ctx, f.total, 256);
+    const float4* tn = (const float4*)b->tnodes;
+    const float4* lv = (const float4*)b->leaves;
+    if (f.count_traversal) {
+        if (b->max_depth <= 64)
+            MRH_TRY(launch(ctx, "trace_pt_stats", dim3(grid), dim3(256), 0, k_trace_pt<64, true>, tn, lv,
+                           root, b->n, in, f.counts, bounce, tmin, tmax, f.hit));
+        else
+            MRH_TRY(launch(ctx, "trace_pt_stats", dim3(grid), dim3(256), 0, k_trace_pt<128, true>, tn, lv,
+                           root, b->n, in, f.counts, bounce, tmin, tmax, f.hit));
+    } else if (b->max_depth <= 64) {
+        MRH_TRY(launch(ctx, "trace_pt", dim3(grid), dim3(256), 0, k_trace_pt<64, false>, tn, lv, root,
+                       b->n, in, f.counts, bounce, tmin, tmax, f.hit));
+    } else {
+        MRH_TRY(launch(ctx, "trace_pt", dim3(grid), dim3(256), 0, k_trace_pt<128, false>, tn, lv, root,
+                       b->n, in, f.counts, bounce, tmin, tmax, f.hit));
+    }
+    SortArgs sa;
+    sa.keys_in = f.hit.key;
+    sa.vals_out = f.order;
+    sa.n_max = (uint32_t)f.total;
+    sa.n_dev = f.counts + bounce;
+    sa.key_bytes = 4;
+    sa.begin_bit = 24;
+    sa.end_bit = 32;
+    sa.descending = true;
+    sa.part_offsets = f.offsets;
+    sa.part_begin = 24;
+    MRH_TRY(radix_sort(ctx, sa));
+    const Materials mats{s->lambert, s->metal, s->dielectric, s->emissive};
+    MRH_TRY(launch(ctx, "shade", dim3(grid), dim3(256), 0, k_shade, in, outb, f.hit, (const uint32_t*)f.order,
+                   f.counts, bounce, (uint32_t)(bounce + 1 == f.max_depth), f.rr_start, (uint32_t)f.spp,
+                   f.sample_base, mats, lv, image));
+    f.bounce++;
+    return MRH_OK;
+}
+
+// Waits for the frame and reads its per-bounce counts
+int32_t frame_counts(mrh_context* ctx, const Frame& f, mrh_render_stats* stats)
+{
+    uint32_t hc[68];
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(hc, f.counts, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (stats) {
+        stats->bounces = f.bounce;
+        for (uint32_t i = 0; i < f.bounce; i++) {
+            stats->segments += hc[i];
+            if (i < 32) stats->bounce_rays[i] = hc[i];
+        }
+        stats->node_visits += (uint64_t)hc[64] | ((uint64_t)hc[65] << 32);
+        stats->tri_tests += (uint64_t)hc[66] | ((uint64_t)hc[67] << 32);
+    }
+    return MRH_OK;
+}
+
+int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const mrh_render_params* p,
+               float* d_image, mrh_render_stats* stats)
+{
+    uint64_t total = 0;
+    MRH_TRY(validate_params(ctx, s, p, &total));
+    void* arena = nullptr;
+    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(total), &arena));
     const size_t img_count = (size_t)p->width * p->height * 3;
     MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
-    MRH_TRY(fill_u32(ctx, counts, 0u, COUNTS_WORDS));
-    MRH_TRY(fill_u32(ctx, counts, (uint32_t)total, 1));
-
-    const Cam cam = make_camera(*cd, p->width, p->height);
-    const Materials mats{s->lambert, s->metal, s->dielectric, s->emissive};
-    const uint32_t root = root_code(b);
-    const uint32_t grid = stride_grid(ctx, total, 256);
-
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MRH_HIP_CHECK(ctx, hipEventCreate(&e0));
     MRH_HIP_CHECK(ctx, hipEventCreate(&e1));
     MRH_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
-    int32_t rc = launch(ctx, "raygen", dim3(grid), dim3(256), 0, k_raygen, cam, p->width, p->height,
-                        p->samples, p->seed, (uint32_t)total, bufs[0]);
-    for (uint32_t bounce = 0; rc == MRH_OK && bounce < p->max_depth; bounce++) {
-        const PathBuf& in = bufs[bounce & 1];
-        const PathBuf& outb = bufs[(bounce + 1) & 1];
-        const float tmin = (bounce == 0) ? cam.near_d : 0.0f;
-        const float tmax = (bounce == 0) ? cam.far_d : FLT_MAX;
-        const float4* tn = (const float4*)b->tnodes;
-        const float4* lv = (const float4*)b->leaves;
-        if (p->count_traversal) {
-            if (b->max_depth <= 64)
-                rc = launch(ctx, "trace_pt_stats", dim3(grid), dim3(256), 0, k_trace_pt<64, true>, tn, lv,
-                            root, b->n, in, counts, bounce, tmin, tmax, hit);
-            else
-                rc = launch(ctx, "trace_pt_stats", dim3(grid), dim3(256), 0, k_trace_pt<128, true>, tn, lv,
-                            root, b->n, in, counts, bounce, tmin, tmax, hit);
-        } else if (b->max_depth <= 64) {
-            rc = launch(ctx, "trace_pt", dim3(grid), dim3(256), 0, k_trace_pt<64, false>, tn, lv, root, b->n,
-                        in, counts, bounce, tmin, tmax, hit);
-        } else {
-            rc = launch(ctx, "trace_pt", dim3(grid), dim3(256), 0, k_trace_pt<128, false>, tn, lv, root, b->n,
-                        in, counts, bounce, tmin, tmax, hit);
-        }
-        if (rc != MRH_OK) break;
-        SortArgs sa;
-        sa.keys_in = hit.key;
-        sa.vals_out = order;
-        sa.n_max = (uint32_t)total;
-        sa.n_dev = counts + bounce;
-        sa.key_bytes = 4;
-        sa.begin_bit = 24;
-        sa.end_bit = 32;
-        sa.descending = true;
-        sa.part_offsets = offsets;
-        sa.part_begin = 24;
-        if ((rc = radix_sort(ctx, sa)) != MRH_OK) break;
-        rc = launch(ctx, "shade", dim3(grid), dim3(256), 0, k_shade, in, outb, hit, (const uint32_t*)order,
-                    counts, bounce, (uint32_t)(bounce + 1 == p->max_depth), p->rr_start, (uint32_t)spp,
-                    mats, (const float4*)b->leaves, d_image);
-    }
+    Frame f;
+    int32_t rc = frame_begin(ctx, cd, p, arena, 0u, f);
+    while (rc == MRH_OK && f.bounce < f.max_depth) rc = frame_bounce(ctx, s, f, d_image);
     if (rc == MRH_OK)
         rc = launch(ctx, "finalize", dim3(stride_grid(ctx, img_count, 256)), dim3(256), 0, k_scale, d_image,
-                    img_count, 1.0f / (float)spp);
+                    img_count, 1.0f / (float)f.spp);
     if (rc == MRH_OK) {
         hipEventRecord(e1, ctx->stream);
-        uint32_t hc[68];
-        hipError_t e = hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "render: %s", hipGetErrorString(e));
+        mrh_render_stats local;
+        memset(&local, 0, sizeof(local));
+        rc = frame_counts(ctx, f, &local);
         if (rc == MRH_OK && stats) {
             float ms = 0.0f;
             hipEventElapsedTime(&ms, e0, e1);
-            memset(stats, 0, sizeof(*stats));
-            stats->render_ms = ms;
-            stats->bounces = p->max_depth;
-            for (uint32_t i = 0; i < p->max_depth; i++) {
-                stats->segments += hc[i];
-                if (i < 32) stats->bounce_rays[i] = hc[i];
-            }
-            stats->node_visits = (uint64_t)hc[64] | ((uint64_t)hc[65] << 32);
-            stats->tri_tests = (uint64_t)hc[66] | ((uint64_t)hc[67] << 32);
+            local.render_ms = ms;
+            *stats = local;
         }
     } else {
         hipStreamSynchronize(ctx->stream);
@@ -703,6 +751,12 @@ int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const 
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return rc;
+}
+
+__global__ void k_mean(const float* __restrict__ acc, float* __restrict__ out, size_t count, float s)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = acc[i] * s;
 }
 
 }  // namespace
@@ -753,6 +807,121 @@ int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes)
     if (!params || !bytes) return MRH_ERR_INVALID;
     const uint64_t total = (uint64_t)params->width * params->height * params->samples * params->samples;
     *bytes = frame_bytes(total) + total * 16;   // + sort scratch
+    return MRH_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------
+// Incremental tracer: the GPUTracerI protocol (GenerateWork, Render() until it
+// returns false, Finalize) over a resident scene, with progressive
+// accumulation across frames.
+// --------------------------------------------------------------------------
+struct mrh_tracer {
+    mrh_context* ctx = nullptr;
+    mrh_scene* scene = nullptr;
+    mrh_render_params p = {};
+    void* arena = nullptr;
+    float* accum = nullptr;
+    uint64_t total = 0;
+    uint32_t frames = 0;
+    bool active = false;
+    mrh::Frame f;
+    mrh_render_stats stats = {};
+};
+
+extern "C" {
+
+int32_t mrh_tracer_create(mrh_context* ctx, mrh_scene* scene, const mrh_render_params* params,
+                          mrh_tracer** out)
+{
+    if (!ctx || !scene || !params || !out) return MRH_ERR_INVALID;
+    *out = nullptr;
+    if (scene->ctx != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_create: scene belongs to another context");
+    uint64_t total = 0;
+    MRH_TRY(validate_params(ctx, scene, params, &total));
+    mrh_tracer* t = new mrh_tracer();
+    t->ctx = ctx;
+    t->scene = scene;
+    t->p = *params;
+    t->total = total;
+    const size_t img = (size_t)params->width * params->height * 3;
+    int32_t rc = dmalloc(ctx, &t->arena, frame_bytes(total));
+    if (rc == MRH_OK) rc = dmalloc(ctx, (void**)&t->accum, img * 4);
+    if (rc == MRH_OK) rc = fill_u32(ctx, (uint32_t*)t->accum, 0u, img);
+    if (rc != MRH_OK) {
+        hipFree(t->arena);
+        hipFree(t->accum);
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return MRH_OK;
+}
+
+int32_t mrh_tracer_generate_work(mrh_tracer* t, const mrh_camera_desc* cam)
+{
+    if (!t || !cam) return MRH_ERR_INVALID;
+    mrh_context* ctx = t->ctx;
+    if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_generate_work: the previous frame is still rendering");
+    MRH_TRY(frame_begin(ctx, cam, &t->p, t->arena, (uint32_t)(t->frames * t->total), t->f));
+    t->active = true;
+    return MRH_OK;
+}
+
+int32_t mrh_tracer_render(mrh_tracer* t, int32_t* more)
+{
+    if (!t || !more) return MRH_ERR_INVALID;
+    *more = 0;
+    if (!t->active) return MRH_OK;
+    MRH_TRY(frame_bounce(t->ctx, t->scene, t->f, t->accum));
+    if (t->f.bounce < t->f.max_depth) {
+        *more = 1;
+        return MRH_OK;
+    }
+    MRH_TRY(frame_counts(t->ctx, t->f, &t->stats));
+    t->active = false;
+    t->frames++;
+    return MRH_OK;
+}
+
+int32_t mrh_tracer_finalize(mrh_tracer* t, float* d_image)
+{
+    if (!t || !d_image) return MRH_ERR_INVALID;
+    mrh_context* ctx = t->ctx;
+    if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_finalize: frame still rendering");
+    const size_t img = (size_t)t->p.width * t->p.height * 3;
+    const float s = t->frames ? 1.0f / (float)((uint64_t)t->frames * t->f.spp) : 0.0f;
+    MRH_TRY(launch(ctx, "finalize", dim3(stride_grid(ctx, img, 256)), dim3(256), 0, k_mean,
+                   (const float*)t->accum, d_image, img, s));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return MRH_OK;
+}
+
+int32_t mrh_tracer_reset_image(mrh_tracer* t)
+{
+    if (!t) return MRH_ERR_INVALID;
+    if (t->active) return fail(t->ctx, MRH_ERR_INVALID, "mrh_tracer_reset_image: frame still rendering");
+    t->frames = 0;
+    t->stats = mrh_render_stats{};
+    return fill_u32(t->ctx, (uint32_t*)t->accum, 0u, (size_t)t->p.width * t->p.height * 3);
+}
+
+int32_t mrh_tracer_stats(const mrh_tracer* t, mrh_render_stats* out, uint32_t* frames)
+{
+    if (!t) return MRH_ERR_INVALID;
+    if (out) *out = t->stats;
+    if (frames) *frames = t->frames;
+    return MRH_OK;
+}
+
+int32_t mrh_tracer_destroy(mrh_tracer* t)
+{
+    if (!t) return MRH_ERR_INVALID;
+    hipStreamSynchronize(t->ctx->stream);
+    hipFree(t->arena);
+    hipFree(t->accum);
+    delete t;
     return MRH_OK;
 }
 

@@ -1,0 +1,298 @@
+// mrh_tracer_system.cpp — the TracerSystemI / GPUTracerI implementation over
+// the C-ABI (host code only; every GPU operation goes through mrh.h).
+#include "mrh_tracer.hpp"
+
+#include <chrono>
+#include <cstring>
+
+namespace mrh {
+
+TracerError TracerError::FromStatus(int32_t status, const char* message)
+{
+    std::string m = message ? message : "";
+    switch (status) {
+    case MRH_OK: return TracerError(OK);
+    case MRH_ERR_OOM: return TracerError(GPU_OUT_OF_MEMORY, m);
+    case MRH_ERR_UNSUPPORTED: return TracerError(UNABLE_TO_INITIALIZE_TRACER, m);
+    case MRH_ERR_BUDGET: return TracerError(GPU_OUT_OF_MEMORY, m);
+    default: return TracerError(TRACER_INTERNAL_ERROR, m);
+    }
+}
+
+namespace {
+
+class SceneMRH final : public GPUSceneI {
+public:
+    SceneMRH(mrh_scene* h, size_t prims, std::vector<mrh_camera_desc> cams)
+        : h(h), prims(prims), cams(std::move(cams)) {}
+    ~SceneMRH() override { mrh_scene_destroy(h); }
+    size_t PrimitiveCount() const override { return prims; }
+    uint32_t CameraCount() const override { return (uint32_t)cams.size(); }
+    const mrh_camera_desc& Camera(uint32_t id) const override { return cams.at(id); }
+    size_t UsedGPUMemory() const override
+    {
+        uint64_t b = 0;
+        mrh_scene_bytes(h, &b);
+        return (size_t)b;
+    }
+    mrh_scene* Handle() const override { return h; }
+
+private:
+    mrh_scene* h;
+    size_t prims;
+    std::vector<mrh_camera_desc> cams;
+};
+
+// PathTracer (TracerCUDA/PathTracer.cu) over mrh_tracer
+class PathTracerMRH final : public GPUTracerI {
+public:
+    PathTracerMRH(mrh_context* ctx, SceneMRH* scene, const TracerParameters& p)
+        : ctx(ctx), scene(scene), params(p) {}
+    ~PathTracerMRH() override { Release(); }
+
+    void AttachTracerCallbacks(TracerCallbacksI& cb) override { callbacks = &cb; }
+
+    TracerError Initialize() override
+    {
+        if (nee) return Report(TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER,
+                                           "NextEventEstimation is not implemented"));
+        Release();
+        mrh_render_params rp = {};
+        rp.width = width;
+        rp.height = height;
+        rp.samples = samples;
+        rp.max_depth = maxDepth;
+        rp.rr_start = rrStart;
+        rp.nee = 0;
+        rp.seed = params.seed;
+        int32_t rc = mrh_tracer_create(ctx, scene->Handle(), &rp, &tracer);
+        if (rc != MRH_OK) return Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+        void* img = nullptr;
+        if (mrh_device_alloc(ctx, &img, (size_t)width * height * 3 * sizeof(float)) != MRH_OK)
+            return Report(TracerError(TracerError::GPU_OUT_OF_MEMORY, "image buffer"));
+        dImage = (float*)img;
+        return TracerError::OK;
+    }
+
+    // PathTracer.h:17-33 option names; "Samples" = strata per axis
+    TracerError SetOptions(const Options& o) override
+    {
+        int64_t v;
+        bool b;
+        if (o.Has("Samples")) {
+            TracerError e = o.GetInt(v, "Samples");
+            if (e != TracerError::OK) return Report(e);
+            samples = (uint32_t)v;
+        }
+        if (o.Has("MaxDepth")) {
+            TracerError e = o.GetInt(v, "MaxDepth");
+            if (e != TracerError::OK) return Report(e);
+            maxDepth = (uint32_t)v;
+        }
+        if (o.Has("RussianRouletteStart")) {
+            TracerError e = o.GetInt(v, "RussianRouletteStart");
+            if (e != TracerError::OK) return Report(e);
+            rrStart = (uint32_t)v;
+        }
+        if (o.Has("NextEventEstimation")) {
+            TracerError e = o.GetBool(b, "NextEventEstimation");
+            if (e != TracerError::OK) return Report(e);
+            nee = b;
+        }
+        options = o;
+        return TracerError::OK;
+    }
+
+    void SetParameters(const TracerParameters& p) override { params = p; }
+    void AskOptions() override {}
+    void AskParameters() override {}
+
+    void GenerateWork(uint32_t cameraId) override
+    {
+        if (cameraId >= scene->CameraCount()) {
+            Report(TracerError(TracerError::UNABLE_TO_GENERATE_WORK, "camera id out of range"));
+            return;
+        }
+        GenerateWork(scene->Camera(cameraId));
+    }
+
+    void GenerateWork(const mrh_camera_desc& cam) override
+    {
+        if (!tracer && Initialize() != TracerError::OK) return;
+        frameStart = std::chrono::steady_clock::now();
+        int32_t rc = mrh_tracer_generate_work(tracer, &cam);
+        if (rc != MRH_OK) Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+    }
+
+    bool Render() override
+    {
+        if (!tracer) return false;
+        int32_t more = 0;
+        int32_t rc = mrh_tracer_render(tracer, &more);
+        if (rc != MRH_OK) {
+            Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+            return false;
+        }
+        return more != 0;
+    }
+
+    void Finalize() override
+    {
+        if (!tracer) return;
+        int32_t rc = mrh_tracer_finalize(tracer, dImage);
+        if (rc != MRH_OK) {
+            Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+            return;
+        }
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - frameStart).count();
+        if (!callbacks) return;
+        std::vector<float> rgb;
+        if (GetImage(rgb) != TracerError::OK) return;
+        uint32_t frames = 0;
+        mrh_tracer_stats(tracer, nullptr, &frames);
+        callbacks->SendImage(rgb.data(), width, height, frames);
+        const double paths = (double)width * height * samples * samples;
+        callbacks->SendAnalyticData(paths / (ms * 1e3), ms);
+    }
+
+    void ResizeImage(uint32_t w, uint32_t h) override
+    {
+        width = w;
+        height = h;
+        if (tracer) Initialize();
+    }
+
+    void ResetImage() override
+    {
+        if (tracer) mrh_tracer_reset_image(tracer);
+    }
+
+    size_t TotalGPUMemoryUsed() const override
+    {
+        mrh_render_params rp = {};
+        rp.width = width;
+        rp.height = height;
+        rp.samples = samples;
+        uint64_t b = 0;
+        mrh_render_bytes(&rp, &b);
+        return (size_t)b + scene->UsedGPUMemory();
+    }
+
+    TracerError GetImage(std::vector<float>& rgb) const override
+    {
+        if (!dImage) return TracerError(TracerError::NO_LOGIC_SET, "no image yet");
+        rgb.resize((size_t)width * height * 3);
+        int32_t rc = mrh_memcpy(ctx, rgb.data(), dImage, rgb.size() * sizeof(float));
+        return TracerError::FromStatus(rc, rc ? mrh_last_error(ctx) : nullptr);
+    }
+
+private:
+    void Release()
+    {
+        if (tracer) mrh_tracer_destroy(tracer);
+        tracer = nullptr;
+        if (dImage) mrh_device_free(ctx, dImage);
+        dImage = nullptr;
+    }
+
+    TracerError Report(TracerError e)
+    {
+        if (e != TracerError::OK && callbacks) callbacks->SendError(e);
+        return e;
+    }
+
+    mrh_context* ctx;
+    SceneMRH* scene;
+    TracerParameters params;
+    Options options;
+    TracerCallbacksI* callbacks = nullptr;
+    mrh_tracer* tracer = nullptr;
+    float* dImage = nullptr;
+    uint32_t width = 256, height = 256, samples = 1, maxDepth = 10, rrStart = 3;
+    bool nee = false;
+    std::chrono::steady_clock::time_point frameStart;
+};
+
+class TracerSystemMRH final : public TracerSystemI {
+public:
+    ~TracerSystemMRH() override
+    {
+        ClearScene();
+        if (ctx) mrh_context_destroy(ctx);
+    }
+
+    TracerError Initialize(int32_t device) override
+    {
+        if (ctx) return TracerError::OK;
+        int32_t rc = mrh_context_create(device, &ctx);
+        if (rc != MRH_OK) return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "no usable GPU");
+        return TracerError::OK;
+    }
+
+    void ClearScene() override { scene.reset(); }
+
+    SceneError GenerateScene(GPUSceneI*& out, const SceneDescription& d) override
+    {
+        out = nullptr;
+        if (!ctx) return SceneError(SceneError::INTERNAL_ERROR, "tracer system not initialized");
+        if (d.accelerator != "LBVH" && d.accelerator != "BasicBVH")
+            return SceneError(SceneError::ACCELERATOR_TYPE_UNKNOWN, d.accelerator);
+        const size_t n = d.hitKeys.size();
+        if (n == 0 || d.triangles.size() != n * 9) return SceneError(SceneError::EMPTY_SCENE);
+        mrh_scene_desc sd = {};
+        sd.tris = d.triangles.data();
+        sd.prim_key = d.hitKeys.data();
+        sd.n = (uint32_t)n;
+        sd.lambert = d.lambert.data();
+        sd.n_lambert = (uint32_t)(d.lambert.size() / 3);
+        sd.metal = d.metal.data();
+        sd.n_metal = (uint32_t)(d.metal.size() / 10);
+        sd.dielectric = d.dielectric.data();
+        sd.n_dielectric = (uint32_t)(d.dielectric.size() / 4);
+        sd.emissive = d.emissive.data();
+        sd.n_emissive = (uint32_t)(d.emissive.size() / 3);
+        sd.morton_bits = d.mortonBits;
+        ClearScene();
+        mrh_scene* h = nullptr;
+        int32_t rc = mrh_scene_create(ctx, &sd, &h);
+        if (rc == MRH_ERR_INVALID) return SceneError(SceneError::INVALID_HIT_KEY, mrh_last_error(ctx));
+        if (rc == MRH_ERR_OOM) return SceneError(SceneError::GPU_OUT_OF_MEMORY, mrh_last_error(ctx));
+        if (rc != MRH_OK) return SceneError(SceneError::INTERNAL_ERROR, mrh_last_error(ctx));
+        scene = std::make_unique<SceneMRH>(h, n, d.cameras);
+        out = scene.get();   // non-owning, as TracerSystemCUDA::GenerateScene
+        return SceneError::OK;
+    }
+
+    SceneError GenerateScene(GPUSceneI*& out, const std::u8string& path) override
+    {
+        out = nullptr;
+        return SceneError(SceneError::NO_LOADER, std::string(path.begin(), path.end()));
+    }
+
+    TracerError GenerateTracer(GPUTracerPtr& tracer, const TracerParameters& params, const Options& opts,
+                               const std::string& type) override
+    {
+        if (!ctx || !scene) return TracerError(TracerError::NO_LOGIC_SET, "no scene");
+        if (type != "PathTracer") return TracerError(TracerError::NO_LOGIC_FOR_TRACER, type);
+        auto t = std::make_unique<PathTracerMRH>(ctx, scene.get(), params);
+        TracerError e = t->SetOptions(opts);
+        if (e != TracerError::OK) return e;
+        tracer = std::move(t);
+        return TracerError::OK;
+    }
+
+private:
+    mrh_context* ctx = nullptr;
+    std::unique_ptr<SceneMRH> scene;
+};
+
+}  // namespace
+}  // namespace mrh
+
+extern "C" {
+
+mrh::TracerSystemI* GenerateTracerSystem() { return new mrh::TracerSystemMRH(); }
+
+void DeleteTracerSystem(mrh::TracerSystemI* system) { delete system; }
+
+}  // extern "C"

@@ -98,6 +98,15 @@ _SIGS = {
     "mrh_render": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams), _P,
                           C.POINTER(RenderStats)]),
     "mrh_render_bytes": (_I32, [C.POINTER(RenderParams), C.POINTER(C.c_uint64)]),
+    "mrh_device_alloc": (_I32, [_P, C.POINTER(_P), C.c_size_t]),
+    "mrh_device_free": (_I32, [_P, _P]),
+    "mrh_tracer_create": (_I32, [_P, _P, C.POINTER(RenderParams), C.POINTER(_P)]),
+    "mrh_tracer_generate_work": (_I32, [_P, C.POINTER(CameraDesc)]),
+    "mrh_tracer_render": (_I32, [_P, C.POINTER(_I32)]),
+    "mrh_tracer_finalize": (_I32, [_P, _P]),
+    "mrh_tracer_reset_image": (_I32, [_P]),
+    "mrh_tracer_stats": (_I32, [_P, C.POINTER(RenderStats), C.POINTER(_U32)]),
+    "mrh_tracer_destroy": (_I32, [_P]),
     "mrh_gen_random_tris": (_I32, [_U32, _U32, _P]),
     "mrh_gen_random_rays": (_I32, [_U32, _U32, _P]),
     "mrh_gen_cornell": (_I32, [_U32, C.POINTER(SceneBuffers)]),

@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 512 MiB device-to-device first: the reference kernel for scaling "
                          "FETCH_SIZE/WRITE_SIZE counters")
@@ -113,6 +114,14 @@ def main():
         del a, b
         log("copy calibration: 536870912 bytes read + written")
 
+    tune = None
+    if not args.no_autotune:
+        res = scene.autotune(img.data_ptr(), W, H, S, DEPTH, RR, SEED, reps=2)
+        tune = {"trace_lds_entries": res.best_lds, "trace_blocks_per_cu": res.best_bpc,
+                "trials": [[res.trial_lds[i], res.trial_bpc[i], round(res.trial_ms[i], 3)]
+                           for i in range(res.trials)]}
+        log(f"autotune: lds {res.best_lds}, blocks/CU {res.best_bpc}, {res.best_ms:.2f} ms; "
+            f"trials (lds, bpc, ms): {tune['trials']}")
     for i in range(args.warmup):
         st = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
         log(f"warmup {i}: {st.segments} segments, {st.render_ms:.1f} ms")
@@ -183,6 +192,7 @@ def main():
             "bvh_build_ms": round(build_ms, 2),
             "latency_ms": {"p50": round(float(np.percentile(lat_sorted, 50)), 3),
                            "p99": round(float(np.percentile(lat_sorted, 99)), 3)},
+            "autotune": tune,
             "kernel_ms_per_frame": {"trace": round(trace_ms / args.steps, 3),
                                     "shade": round(shade_ms / args.steps, 3),
                                     "partition": round((sort_ms + hist_ms) / args.steps, 3)},

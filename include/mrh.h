@@ -63,6 +63,14 @@ int32_t     mrh_profile_enable(mrh_context* ctx, int32_t on);
 int32_t     mrh_profile_reset(mrh_context* ctx);
 int32_t     mrh_profile_get(mrh_context* ctx, const char* kernel, double* total_ms,
                             uint64_t* launches);
+/* Launch-shape knobs of the context, set by the autotuner (no counterpart in
+ * the reference, whose block counts are fixed in CudaSystem.cpp:109-122):
+ *   "trace_lds_entries"   traversal stack entries kept in LDS per thread
+ *                         (0, 8, 12, 16, 24; 0 = whole stack in scratch)
+ *   "trace_blocks_per_cu" persistent traversal grid, blocks of 256 per CU
+ * mrh_tune_get reads a knob back. */
+int32_t     mrh_tune(mrh_context* ctx, const char* knob, int64_t value);
+int32_t     mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value);
 
 /* ------------------------------------------------------------------ */
 /* Morton codes.  Replaces KCGenerateAABBs + ReduceArrayGPU + FindMortonDelta
@@ -226,6 +234,21 @@ int32_t mrh_render(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* ca
 
 /* Device bytes mrh_render needs for a frame of this size */
 int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes);
+
+/* Autotuner (no counterpart in the reference): times `reps` renders of the
+ * given frame per candidate launch shape — traversal stack entries in LDS,
+ * then blocks per CU of the traversal grid — and leaves the fastest set on
+ * the context (mrh_tune).  d_image is scratch for the trial frames. */
+typedef struct {
+    uint32_t best_lds, best_bpc;
+    double   best_ms;
+    uint32_t trials;
+    uint32_t trial_lds[32], trial_bpc[32];
+    double   trial_ms[32];
+} mrh_tune_result;
+int32_t mrh_autotune(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                     const mrh_render_params* params, float* d_image, uint32_t reps,
+                     mrh_tune_result* out);
 
 /* Incremental tracer over a resident scene, the GPUTracerI protocol
  * (RayLib/GPUTracerI.h:30-72): mrh_tracer_create ~ GenerateTracer +

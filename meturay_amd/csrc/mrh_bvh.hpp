@@ -51,6 +51,28 @@ int32_t lbvh_create(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t 
 int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src);
 void lbvh_free(mrh_lbvh* b);
 
+// Persistent grid of the traversal kernels: a multiple of 8 blocks (one
+// slice per XCD) of 256 threads.
+inline uint32_t trace_grid(mrh_context* ctx, uint64_t rays)
+{
+    uint64_t want = (rays + 255) / 256;
+    uint64_t cap = (uint64_t)ctx->num_cus * ctx->trace_blocks_per_cu;
+    if (want > cap) want = cap;
+    want = (want + 7) & ~7ull;
+    return (uint32_t)(want ? want : 8);
+}
+
+// Overflow buffer for stack entries beyond the LDS part (per grid thread).
+inline int32_t trace_overflow(mrh_context* ctx, uint32_t grid, uint32_t cap, uint32_t lds, uint2** out)
+{
+    *out = nullptr;
+    if (lds == 0 || cap <= lds) return MRH_OK;
+    void* p = nullptr;
+    MRH_TRY(scratch(ctx, SLOT_TRACE_OVERFLOW, (size_t)(cap - lds) * grid * 256 * sizeof(uint2), &p));
+    *out = (uint2*)p;
+    return MRH_OK;
+}
+
 // Traversal entry code of the root
 inline uint32_t root_code(const mrh_lbvh* b) { return (b->n == 1) ? LEAF_BIT : (b->root - b->n); }
 
@@ -60,14 +82,76 @@ struct TraceResult {
     uint32_t visits, tests;
 };
 
+// Traversal stacks.  An entry is (child code, entry distance).
+//
+// PrivateStack keeps the whole stack in a private array (scratch memory,
+// which the L2 writes back to HBM).  LdsStack keeps the first L entries of
+// every thread in LDS, entry-major and thread-minor so a wave's accesses
+// spread over the banks, and spills deeper entries to the thread's own slice
+// of a global overflow buffer.
+template <int N>
+struct PrivateStack {
+    static constexpr int CAP = N;
+    uint32_t node[N];
+    float t[N];
+    __device__ __forceinline__ void put(int i, uint32_t n, float d)
+    {
+        node[i] = n;
+        t[i] = d;
+    }
+    __device__ __forceinline__ void get(int i, uint32_t& n, float& d) const
+    {
+        n = node[i];
+        d = t[i];
+    }
+};
+
+template <int L, int N, int BLOCK>
+struct LdsStack {
+    static constexpr int CAP = N;
+    uint32_t* s_node;     // [L][BLOCK]
+    float* s_t;           // [L][BLOCK]
+    uint32_t slot;        // threadIdx.x
+    uint2* overflow;      // [(N-L)][ostride]
+    size_t ostride, oslot;
+    __device__ __forceinline__ void put(int i, uint32_t n, float d)
+    {
+        if (i < L) {
+            s_node[i * BLOCK + slot] = n;
+            s_t[i * BLOCK + slot] = d;
+        } else {
+            overflow[(size_t)(i - L) * ostride + oslot] = make_uint2(n, __float_as_uint(d));
+        }
+    }
+    __device__ __forceinline__ void get(int i, uint32_t& n, float& d) const
+    {
+        if (i < L) {
+            n = s_node[i * BLOCK + slot];
+            d = s_t[i * BLOCK + slot];
+        } else {
+            const uint2 e = overflow[(size_t)(i - L) * ostride + oslot];
+            n = e.x;
+            d = __uint_as_float(e.y);
+        }
+    }
+};
+
+// Logical block index that gives each of the 8 XCDs (workgroups are handed
+// to XCDs round-robin) a contiguous slice of the ray range, so neighbouring
+// rays share an L2.  Requires gridDim.x % 8 == 0.
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    return (blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+}
+
 // Closest (ANY = false) or any (ANY = true) hit.  Visit order: both child
 // boxes are tested at the parent; the child with the smaller entry distance
 // is taken first (ties: left); the other is pushed with its entry distance
 // and dropped on pop when that distance exceeds the current tmax (the same
 // outcome as re-testing its box).  Accepts t in [tmin, tmax); the first hit
-// found at a given t wins.  STACK must be >= the tree depth.
-template <bool ANY, bool STATS, int STACK>
-__device__ __forceinline__ TraceResult traverse(const float4* __restrict__ tnodes,
+// found at a given t wins.  Stack::CAP must be >= the tree depth.
+template <bool ANY, bool STATS, class Stack>
+__device__ __forceinline__ TraceResult traverse(Stack& st, const float4* __restrict__ tnodes,
                                                 const float4* __restrict__ leaves, uint32_t root,
                                                 uint32_t n_leaves, V3 o, V3 d, float tmin,
                                                 float tmax)
@@ -78,8 +162,6 @@ __device__ __forceinline__ TraceResult traverse(const float4* __restrict__ tnode
     r.ba = r.bb = 0.0f;
     r.visits = r.tests = 0;
     const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    uint32_t st_node[STACK];
-    float st_t[STACK];
     int sp = 0;
     uint32_t next = root;
     for (;;) {
@@ -89,8 +171,11 @@ __device__ __forceinline__ TraceResult traverse(const float4* __restrict__ tnode
                 bool found = false;
                 while (sp > 0) {
                     --sp;
-                    if (!(st_t[sp] > r.t)) {
-                        next = st_node[sp];
+                    uint32_t nn;
+                    float et;
+                    st.get(sp, nn, et);
+                    if (!(et > r.t)) {
+                        next = nn;
                         found = true;
                         break;
                     }
@@ -134,15 +219,15 @@ __device__ __forceinline__ TraceResult traverse(const float4* __restrict__ tnode
         const bool h1 = intersect_box(o, inv, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, tmin, r.t, e1);
         const uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
         if (h0 && h1) {
-            // the host only launches with STACK >= tree depth; the guard keeps
-            // a malformed tree from writing past the private array
-            const bool room = sp < STACK;
+            // the host only launches with CAP >= tree depth; the guard keeps a
+            // malformed tree from writing past the stack
+            const bool room = sp < Stack::CAP;
             if (e1 < e0) {
                 next = c1;
-                if (room) { st_node[sp] = c0; st_t[sp] = e0; }
+                if (room) st.put(sp, c0, e0);
             } else {
                 next = c0;
-                if (room) { st_node[sp] = c1; st_t[sp] = e1; }
+                if (room) st.put(sp, c1, e1);
             }
             sp += room ? 1 : 0;
         } else if (h0) {

@@ -187,6 +187,34 @@ int32_t mrh_device_free(mrh_context* ctx, void* ptr)
     return MRH_OK;
 }
 
+int32_t mrh_tune(mrh_context* ctx, const char* knob, int64_t value)
+{
+    if (!ctx || !knob) return MRH_ERR_INVALID;
+    const std::string k = knob;
+    if (k == "trace_lds_entries") {
+        if (value != 0 && value != 8 && value != 12 && value != 16 && value != 24)
+            return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24");
+        ctx->trace_lds_entries = (uint32_t)value;
+        return MRH_OK;
+    }
+    if (k == "trace_blocks_per_cu") {
+        if (value < 1 || value > 64) return fail(ctx, MRH_ERR_INVALID, "trace_blocks_per_cu must be in [1, 64]");
+        ctx->trace_blocks_per_cu = (uint32_t)value;
+        return MRH_OK;
+    }
+    return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
+}
+
+int32_t mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value)
+{
+    if (!ctx || !knob || !value) return MRH_ERR_INVALID;
+    const std::string k = knob;
+    if (k == "trace_lds_entries") *value = ctx->trace_lds_entries;
+    else if (k == "trace_blocks_per_cu") *value = ctx->trace_blocks_per_cu;
+    else return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
+    return MRH_OK;
+}
+
 int32_t mrh_profile_enable(mrh_context* ctx, int32_t on)
 {
     if (!ctx) return MRH_ERR_INVALID;

@@ -26,6 +26,7 @@ enum MrhSlot : int {
     SLOT_MORTON_EXT,
     SLOT_PART_KEYS,
     SLOT_FRAME,
+    SLOT_TRACE_OVERFLOW,
     SLOT_COUNT
 };
 
@@ -45,6 +46,11 @@ struct mrh_context {
     std::vector<MrhEventPair> pending;
     std::vector<hipEvent_t> free_events;
     std::map<std::string, std::pair<double, uint64_t>> totals;
+    // tuning knobs (mrh_tune): traversal stack entries kept in LDS per
+    // thread (0 = whole stack in private scratch) and blocks per CU of the
+    // persistent trace grid
+    uint32_t trace_lds_entries = 16;
+    uint32_t trace_blocks_per_cu = 8;
 };
 
 namespace mrh {

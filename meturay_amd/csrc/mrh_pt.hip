@@ -124,18 +124,31 @@ __global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h,
 // [66, 68) u64 triangle tests
 constexpr uint32_t COUNTS_WORDS = 128;
 
-template <int STACK, bool STATS>
+template <int CAP, bool STATS, int L>
 __global__ __launch_bounds__(256) void k_trace_pt(const float4* __restrict__ tnodes,
                                                   const float4* __restrict__ leaves, uint32_t root, uint32_t n_leaves,
                                                   PathBuf in, uint32_t* __restrict__ counts,
-                                                  uint32_t bounce, float tmin, float tmax, HitBuf hit)
+                                                  uint32_t bounce, float tmin, float tmax, HitBuf hit,
+                                                  uint2* __restrict__ overflow)
 {
+    __shared__ uint32_t s_node[L > 0 ? L * 256 : 1];
+    __shared__ float s_t[L > 0 ? L * 256 : 1];
     const uint32_t n = counts[bounce];
+    const uint32_t lb = xcd_block();
+    const size_t ostride = (size_t)gridDim.x * blockDim.x;
+    const size_t oslot = (size_t)lb * blockDim.x + threadIdx.x;
     uint64_t visits = 0, tests = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = lb * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const V3 o = v3(in.ox[i], in.oy[i], in.oz[i]);
         const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
-        const TraceResult r = traverse<false, STATS, STACK>(tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+        TraceResult r;
+        if constexpr (L > 0) {
+            LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
+            r = traverse<false, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+        } else {
+            PrivateStack<CAP> st;
+            r = traverse<false, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+        }
         hit.leaf[i] = r.leaf;
         hit.t[i] = r.t;
         hit.ba[i] = r.ba;
@@ -156,6 +169,31 @@ __global__ __launch_bounds__(256) void k_trace_pt(const float4* __restrict__ tno
             atomicAdd((unsigned long long*)(counts + 66), (unsigned long long)tests);
         }
     }
+}
+
+template <int CAP, bool STATS>
+int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, const PathBuf& in,
+                        uint32_t* counts, uint32_t bounce, float tmin, float tmax, const HitBuf& hit)
+{
+    const uint32_t root = root_code(b);
+    const uint32_t lds = ctx->trace_lds_entries;
+    uint2* ovf = nullptr;
+    MRH_TRY(trace_overflow(ctx, grid, CAP, lds, &ovf));
+    const char* name = STATS ? "trace_pt_stats" : "trace_pt";
+    const float4* tn = (const float4*)b->tnodes;
+    const float4* lv = (const float4*)b->leaves;
+#define MRH_TRACE_PT_L(LV)                                                                         \
+    return launch(ctx, name, dim3(grid), dim3(256), 0, k_trace_pt<CAP, STATS, LV>, tn, lv, root, b->n, \
+                  in, counts, bounce, tmin, tmax, hit, ovf)
+    switch (lds) {
+    case 0: MRH_TRACE_PT_L(0);
+    case 8: MRH_TRACE_PT_L(8);
+    case 12: MRH_TRACE_PT_L(12);
+    case 16: MRH_TRACE_PT_L(16);
+    case 24: MRH_TRACE_PT_L(24);
+    default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24");
+    }
+#undef MRH_TRACE_PT_L
 }
 
 // ------------------------------------------------------------------ BxDFs
@@ -659,23 +697,15 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     const PathBuf& outb = f.bufs[(bounce + 1) & 1];
     const float tmin = (bounce == 0) ? f.cam.near_d : 0.0f;
     const float tmax = (bounce == 0) ? f.cam.far_d : FLT_MAX;
-    const uint32_t root = root_code(b);
     const uint32_t grid = stride_grid(ctx, f.total, 256);
-    const float4* tn = (const float4*)b->tnodes;
+    const uint32_t tgrid = trace_grid(ctx, f.total);
     const float4* lv = (const float4*)b->leaves;
     if (f.count_traversal) {
-        if (b->max_depth <= 64)
-            MRH_TRY(launch(ctx, "trace_pt_stats", dim3(grid), dim3(256), 0, k_trace_pt<64, true>, tn, lv,
-                           root, b->n, in, f.counts, bounce, tmin, tmax, f.hit));
-        else
-            MRH_TRY(launch(ctx, "trace_pt_stats", dim3(grid), dim3(256), 0, k_trace_pt<128, true>, tn, lv,
-                           root, b->n, in, f.counts, bounce, tmin, tmax, f.hit));
-    } else if (b->max_depth <= 64) {
-        MRH_TRY(launch(ctx, "trace_pt", dim3(grid), dim3(256), 0, k_trace_pt<64, false>, tn, lv, root,
-                       b->n, in, f.counts, bounce, tmin, tmax, f.hit));
+        if (b->max_depth <= 64) MRH_TRY((launch_trace_pt<64, true>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit)));
+        else MRH_TRY((launch_trace_pt<128, true>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit)));
     } else {
-        MRH_TRY(launch(ctx, "trace_pt", dim3(grid), dim3(256), 0, k_trace_pt<128, false>, tn, lv, root,
-                       b->n, in, f.counts, bounce, tmin, tmax, f.hit));
+        if (b->max_depth <= 64) MRH_TRY((launch_trace_pt<64, false>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit)));
+        else MRH_TRY((launch_trace_pt<128, false>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit)));
     }
     SortArgs sa;
     sa.keys_in = f.hit.key;

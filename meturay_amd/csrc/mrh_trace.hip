@@ -1,26 +1,39 @@
 // mrh_trace.hip — closest-hit and any-hit ray casting over the LBVH.
 //
-// One thread per ray, SoA ray planes read coalesced, SoA hit planes written
-// coalesced.  The traversal keeps a private (scratch) stack sized from the
-// tree depth measured at build time.
+// One thread per ray in a persistent, XCD-sliced grid; SoA ray planes read
+// coalesced, SoA hit planes written coalesced.  The traversal stack keeps
+// its top entries in LDS (see LdsStack) with a global overflow tail.
 #include "mrh_bvh.hpp"
 
 namespace mrh {
 namespace {
 
-template <bool ANY, bool STATS, int STACK>
+template <bool ANY, bool STATS, int CAP, int L>
 __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ tnodes,
-                                               const float4* __restrict__ leaves, uint32_t root, uint32_t n_leaves,
-                                               const float* __restrict__ rays, uint32_t n,
-                                               uint32_t pitch, float tmin, uint32_t* __restrict__ out_prim,
-                                               float* __restrict__ out_t, float* __restrict__ out_bary,
-                                               uint32_t* __restrict__ stats)
+                                               const float4* __restrict__ leaves, uint32_t root,
+                                               uint32_t n_leaves, const float* __restrict__ rays,
+                                               uint32_t n, uint32_t pitch, float tmin,
+                                               uint32_t* __restrict__ out_prim, float* __restrict__ out_t,
+                                               float* __restrict__ out_bary, uint32_t* __restrict__ stats,
+                                               uint2* __restrict__ overflow)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    __shared__ uint32_t s_node[L > 0 ? L * 256 : 1];
+    __shared__ float s_t[L > 0 ? L * 256 : 1];
+    const uint32_t lb = xcd_block();
+    const size_t ostride = (size_t)gridDim.x * blockDim.x;
+    const size_t oslot = (size_t)lb * blockDim.x + threadIdx.x;
+    for (uint32_t i = lb * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const V3 o = v3(rays[i], rays[(size_t)pitch + i], rays[2 * (size_t)pitch + i]);
         const V3 d = v3(rays[3 * (size_t)pitch + i], rays[4 * (size_t)pitch + i], rays[5 * (size_t)pitch + i]);
         const float tmax = rays[6 * (size_t)pitch + i];
-        TraceResult r = traverse<ANY, STATS, STACK>(tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+        TraceResult r;
+        if constexpr (L > 0) {
+            LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
+            r = traverse<ANY, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+        } else {
+            PrivateStack<CAP> st;
+            r = traverse<ANY, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+        }
         if (ANY) {
             out_prim[i] = (r.leaf != NO_HIT) ? 1u : 0u;
         } else {
@@ -38,19 +51,36 @@ __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ tnodes
     }
 }
 
+template <bool ANY, bool STATS, int CAP>
+int32_t trace_cap(mrh_context* ctx, const mrh_lbvh* b, const float* rays, uint32_t n, uint32_t pitch,
+                  float tmin, uint32_t* prim, float* t, float* bary, uint32_t* stats)
+{
+    const uint32_t root = root_code(b);
+    const uint32_t grid = trace_grid(ctx, n);
+    const char* name = ANY ? "trace_any" : (STATS ? "trace_closest_stats" : "trace_closest");
+    const uint32_t lds = ctx->trace_lds_entries;
+    uint2* ovf = nullptr;
+    MRH_TRY(trace_overflow(ctx, grid, CAP, lds, &ovf));
+#define MRH_TRACE_L(LV)                                                                                  \
+    return launch(ctx, name, dim3(grid), dim3(256), 0, k_trace<ANY, STATS, CAP, LV>, b->tnodes, b->leaves, \
+                  root, b->n, rays, n, pitch, tmin, prim, t, bary, stats, ovf)
+    switch (lds) {
+    case 0: MRH_TRACE_L(0);
+    case 8: MRH_TRACE_L(8);
+    case 12: MRH_TRACE_L(12);
+    case 16: MRH_TRACE_L(16);
+    case 24: MRH_TRACE_L(24);
+    default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24");
+    }
+#undef MRH_TRACE_L
+}
+
 template <bool ANY, bool STATS>
 int32_t trace_launch(mrh_context* ctx, const mrh_lbvh* b, const float* rays, uint32_t n, uint32_t pitch,
                      float tmin, uint32_t* prim, float* t, float* bary, uint32_t* stats)
 {
-    const uint32_t root = root_code(b);
-    const dim3 grid(stride_grid(ctx, n, 256)), block(256);
-    const char* name = ANY ? "trace_any" : (STATS ? "trace_closest_stats" : "trace_closest");
-    if (b->max_depth <= 64)
-        return launch(ctx, name, grid, block, 0, k_trace<ANY, STATS, 64>, b->tnodes, b->leaves, root, b->n, rays,
-                      n, pitch, tmin, prim, t, bary, stats);
-    if (b->max_depth <= 128)
-        return launch(ctx, name, grid, block, 0, k_trace<ANY, STATS, 128>, b->tnodes, b->leaves, root, b->n,
-                      rays, n, pitch, tmin, prim, t, bary, stats);
+    if (b->max_depth <= 64) return trace_cap<ANY, STATS, 64>(ctx, b, rays, n, pitch, tmin, prim, t, bary, stats);
+    if (b->max_depth <= 128) return trace_cap<ANY, STATS, 128>(ctx, b, rays, n, pitch, tmin, prim, t, bary, stats);
     return fail(ctx, MRH_ERR_UNSUPPORTED, "trace: tree depth %u exceeds the traversal stack (128)",
                 b->max_depth);
 }

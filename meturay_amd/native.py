@@ -58,6 +58,11 @@ class RenderStats(C.Structure):
                 ("render_ms", C.c_double), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("best_lds", _U32), ("best_bpc", _U32), ("best_ms", C.c_double), ("trials", _U32),
+                ("trial_lds", _U32 * 32), ("trial_bpc", _U32 * 32), ("trial_ms", C.c_double * 32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
                 ("lambert", _P), ("n_lambert", _U32), ("metal", _P), ("n_metal", _U32),
@@ -99,6 +104,10 @@ _SIGS = {
                           C.POINTER(RenderStats)]),
     "mrh_render_bytes": (_I32, [C.POINTER(RenderParams), C.POINTER(C.c_uint64)]),
     "mrh_device_alloc": (_I32, [_P, C.POINTER(_P), C.c_size_t]),
+    "mrh_tune": (_I32, [_P, C.c_char_p, C.c_int64]),
+    "mrh_tune_get": (_I32, [_P, C.c_char_p, C.POINTER(C.c_int64)]),
+    "mrh_autotune": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams), _P, _U32,
+                            C.POINTER(TuneResult)]),
     "mrh_device_free": (_I32, [_P, _P]),
     "mrh_tracer_create": (_I32, [_P, _P, C.POINTER(RenderParams), C.POINTER(_P)]),
     "mrh_tracer_generate_work": (_I32, [_P, C.POINTER(CameraDesc)]),
@@ -181,6 +190,14 @@ class Context:
 
     def memcpy(self, dst, src, nbytes: int) -> None:
         self.check(lib().mrh_memcpy(self.h, ptr(dst), ptr(src), nbytes))
+
+    def tune(self, knob: str, value: int) -> None:
+        self.check(lib().mrh_tune(self.h, knob.encode(), int(value)))
+
+    def tune_get(self, knob: str) -> int:
+        v = C.c_int64(0)
+        self.check(lib().mrh_tune_get(self.h, knob.encode(), C.byref(v)))
+        return v.value
 
     def profile(self, on: bool = True) -> None:
         self.check(lib().mrh_profile_enable(self.h, 1 if on else 0))
@@ -340,6 +357,15 @@ class Scene:
         self.ctx.check(lib().mrh_render(self.ctx.h, self.h, C.byref(cam), C.byref(p),
                                         ptr(d_image), C.byref(st)))
         return st
+
+    def autotune(self, d_image, width, height, samples, max_depth, rr_start, seed=0, reps=2,
+                 camera=None) -> TuneResult:
+        cam = camera_desc(camera or self.camera)
+        p = RenderParams(width, height, samples, max_depth, rr_start, 0, seed, 0)
+        res = TuneResult()
+        self.ctx.check(lib().mrh_autotune(self.ctx.h, self.h, C.byref(cam), C.byref(p), ptr(d_image),
+                                          reps, C.byref(res)))
+        return res
 
     def close(self) -> None:
         if self.h:

@@ -235,6 +235,81 @@ int32_t mrh_render(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* ca
 /* Device bytes mrh_render needs for a frame of this size */
 int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes);
 
+/* ------------------------------------------------------------------ */
+/* Path-guiding sparse voxel octree.  Replaces the WFPG tracer's SVO
+ * (AnisoSVOctreeCPU::Constrcut AnisoSVO.cu:657-1188, KCAccumulateRadianceToLeaf
+ * AnisoSVO.cu:225-263, NormalizeAndFilterRadiance AnisoSVO.cu:1415-1483) and
+ * its radiance-field sampling (KCGenAndSampleDistributionProduct,
+ * WFPGTracerKC.cuh:1562-1766) with the structure of this library: leaves are
+ * the occupied cells of a 2^depth grid (depth <= 10) over a cube, every node
+ * holds 8x8 FP32 radiance bins over the equal-area concentric octahedral map.
+ * Codes are 30-bit Morton codes of cells at `depth` (cube = lo.xyz, size).  */
+/* ------------------------------------------------------------------ */
+typedef struct mrh_svo mrh_svo;
+typedef struct {
+    uint32_t depth, nodes;
+    uint32_t level_count[11];      /* nodes per level, root = level 0 */
+    uint32_t level_offset[12];     /* first node of each level */
+    float    cube[4];
+    const uint32_t* d_codes;       /* [nodes] level code (cell code >> 3*(depth-level)) */
+    const uint32_t* d_first_child; /* [nodes] 0xFFFFFFFF at leaves */
+    const uint32_t* d_child_mask;  /* [nodes] 8-bit occupancy */
+    const uint32_t* d_parent;      /* [nodes] 0xFFFFFFFF at the root */
+    const float*    d_sum;         /* [nodes][64] after filter: per-bin means before blur */
+    const uint32_t* d_count;       /* [nodes][64] samples per bin */
+    const float*    d_radiance;    /* [nodes][64] filtered radiance (3x3 blur + 1e-4 floor) */
+    const uint32_t* d_node_count;  /* [nodes] samples in the subtree (after filter) */
+} mrh_svo_view;
+
+int32_t mrh_svo_create(mrh_context* ctx, mrh_svo** out);
+int32_t mrh_svo_destroy(mrh_svo* svo);
+/* d_points: 3 planes of float[pitch] */
+int32_t mrh_svo_codes(mrh_context* ctx, const float* d_points, uint32_t n, uint32_t pitch,
+                      const float cube[4], uint32_t depth, uint32_t* d_codes);
+/* Builds the octree over the cells of d_codes and clears its radiance (synchronous). */
+int32_t mrh_svo_build(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, uint32_t n,
+                      const float cube[4], uint32_t depth);
+/* Adds value i to bin d_bins[i] of the deepest node holding d_codes[i]
+ * (codes 0xFFFFFFFF are skipped). */
+int32_t mrh_svo_splat(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, const uint32_t* d_bins,
+                      const float* d_values, uint32_t n);
+int32_t mrh_svo_filter(mrh_context* ctx, mrh_svo* svo);
+/* Deepest existing node on each code's root-to-cell path; with min_count > 0
+ * the deepest whose subtree holds at least min_count samples (after filter). */
+int32_t mrh_svo_lookup(mrh_context* ctx, const mrh_svo* svo, const uint32_t* d_codes, uint32_t n,
+                       uint32_t min_count, uint32_t* d_nodes);
+/* Product sampling: bin ~ radiance x max(cos(n, bin centre), 0), then a
+ * uniform point of the bin; d_normals, d_xi (3 planes), d_dirs 3 planes of
+ * float[n]; d_pdf solid-angle pdf (0 when the product vanishes). */
+int32_t mrh_svo_sample(mrh_context* ctx, const mrh_svo* svo, const uint32_t* d_nodes,
+                       const float* d_normals, const float* d_xi, uint32_t n, float* d_dirs,
+                       float* d_pdf);
+int32_t mrh_svo_pdf(mrh_context* ctx, const mrh_svo* svo, const uint32_t* d_nodes,
+                    const float* d_normals, const float* d_dirs, uint32_t n, float* d_pdf);
+int32_t mrh_svo_view_get(const mrh_svo* svo, mrh_svo_view* out);
+
+/* Guided path tracing: replaces WFPGTracer::Render / Finalize
+ * (WFPGTracer.cu:1590-1790) and the WFPG path work (WFPGTracerKC.cuh:202-520,
+ * options WFPGTracer.h:25-96).  `iterations` frames of samples^2 spp each;
+ * every frame but the last records its path vertices (PathNode.cuh:152-190,
+ * PPGTracerKC.cuh:605-620); after it the octree is rebuilt over the vertices'
+ * cells (depth svo_depth, cube = scene bounds padded by one voxel), their
+ * incident radiance is splatted and the tree filtered; the next frame draws
+ * each non-specular bounce from guide (probability guide_ratio) or BxDF and
+ * weights it by the mixture pdf.  d_image = mean of all frames.  `svo` may be
+ * NULL; otherwise it keeps the last guide for inspection.  Deterministic:
+ * splat sums are fixed point, so equal inputs give equal guides and paths. */
+typedef struct {
+    uint32_t iterations;     /* frames, 1..64 (1 = plain path tracing) */
+    uint32_t svo_depth;      /* "OctreeLevel", 1..10 */
+    float    guide_ratio;    /* "MISRatio": probability of sampling the guide */
+    uint32_t min_samples;    /* guide lookup stops above nodes with fewer samples ("BinRayCount") */
+} mrh_guide_params;
+
+int32_t mrh_render_guided(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                          const mrh_render_params* params, const mrh_guide_params* guide, mrh_svo* svo,
+                          float* d_image, mrh_render_stats* stats);
+
 /* Autotuner (no counterpart in the reference): times `reps` renders of the
  * given frame per candidate launch shape — traversal stack entries in LDS,
  * then blocks per CU of the traversal grid — and leaves the fastest set on

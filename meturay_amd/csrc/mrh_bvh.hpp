@@ -34,6 +34,7 @@ struct mrh_lbvh {
     float4* leaves = nullptr;       // [n][3]
     uint32_t* prim_key = nullptr;   // [n] (optional HitKeys, primitive order)
     uint32_t* meta = nullptr;       // [0] root, [1] max depth, [2..7] extent encodings
+    float lo[3] = {}, hi[3] = {};   // scene bounds (host copy)
     std::vector<void*> allocs;
 };
 

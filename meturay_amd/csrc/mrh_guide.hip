@@ -1,0 +1,646 @@
+// mrh_guide.hip — the path-guiding SVO: build from surface-point voxel
+// codes, radiance splat, filter, lookup and product sampling.
+//
+// Build (replaces AnisoSVOctreeCPU::Constrcut's sort / RLE / per-level
+// passes, AnisoSVO.cu:657-1188, with the structure config 4 asks for):
+//   sort the leaf codes (onesweep), then for each level from the leaves up:
+//   unique (flag + scan + scatter) and parent links; then child masks and
+//   first-child indices with atomics.  Levels are stored root first.
+// Splat (KCAccumulateRadianceToLeaf, AnisoSVO.cu:225-263): atomic add of a
+//   radiance sample into one of the leaf's 8x8 direction bins + a count.
+// Filter (NormalizeAndFilterRadiance, AnisoSVO.cu:1415-1483): sums and counts
+//   are pushed up the tree, bins become means, bins without samples take their
+//   parent's value top-down, then a 3x3 box blur over the bin grid and a
+//   1e-4 floor.
+#include "mrh_guide.hpp"
+#include "mrh_bvh.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+struct mrh_svo {
+    mrh_context* ctx = nullptr;
+    uint32_t depth = 0, nodes = 0, cap = 0;
+    uint32_t lvl_count[11] = {}, lvl_off[12] = {};
+    float cube[4] = {0, 0, 0, 1};   // lo xyz, size
+    uint32_t* codes = nullptr;       // [cap] level code of each node
+    uint32_t* first_child = nullptr; // [cap]
+    uint32_t* child_mask = nullptr;  // [cap]
+    uint32_t* parent = nullptr;      // [cap]
+    float* sum = nullptr;            // [cap][64] sums, then means (filter)
+    unsigned long long* acc = nullptr; // [cap][64] fixed-point splat sums
+    uint32_t* cnt = nullptr;         // [cap][64]
+    float* radiance = nullptr;       // [cap][64]
+    uint32_t* node_count = nullptr;  // [cap] samples in the subtree
+    mrh::V3* centers = nullptr;      // [64]
+};
+
+namespace mrh {
+namespace {
+
+// ---------------------------------------------------------------- scan
+constexpr uint32_t SCAN_ITEMS = 4, SCAN_BLOCK = 256, SCAN_TILE = SCAN_ITEMS * SCAN_BLOCK;
+
+__global__ __launch_bounds__(256) void k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                    uint32_t* __restrict__ tile_sums, uint32_t n)
+{
+    __shared__ uint32_t s_tmp[8];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS], t = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        t += v[k];
+    }
+    // exclusive scan of the per-thread totals over the block
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = t;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= (uint32_t)off) incl += y;
+    }
+    if (lane == 63) s_tmp[wave] = incl;
+    __syncthreads();
+    uint32_t wp = 0;
+    for (uint32_t w = 0; w < wave; w++) wp += s_tmp[w];
+    uint32_t run = incl - t + wp;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (threadIdx.x == SCAN_BLOCK - 1) tile_sums[blockIdx.x] = run;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restrict__ tile_pre,
+                                                  uint32_t n)
+{
+    const uint32_t add = tile_pre[blockIdx.x];
+    const uint32_t base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+    for (uint32_t k = 0; k < SCAN_ITEMS; k++)
+        if (base + k < n) out[base + k] += add;
+}
+
+// exclusive scan in -> out (out may alias in); tmp holds the tile sums
+int32_t scan_exclusive(mrh_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n)
+{
+    if (n == 0) return MRH_OK;
+    const uint32_t tiles = div_up(n, SCAN_TILE);
+    void* p = nullptr;
+    MRH_TRY(dmalloc(ctx, &p, (size_t)(tiles + 1) * 4 * 2));
+    uint32_t* sums = (uint32_t*)p;
+    uint32_t* pre = sums + tiles + 1;
+    int32_t rc = launch(ctx, "scan_tiles", dim3(tiles), dim3(SCAN_BLOCK), 0, k_scan_tiles, in, out, sums, n);
+    if (rc == MRH_OK && tiles > 1) {
+        rc = scan_exclusive(ctx, sums, pre, tiles);
+        if (rc == MRH_OK)
+            rc = launch(ctx, "scan_add", dim3(tiles), dim3(SCAN_BLOCK), 0, k_scan_add, out, (const uint32_t*)pre, n);
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(p);
+    return rc;
+}
+
+// ---------------------------------------------------------------- unique
+// flags[i] = 1 where (in[i] >> shift) starts a new run
+__global__ void k_flag_runs(const uint32_t* __restrict__ in, uint32_t n, uint32_t shift, uint32_t* __restrict__ flags)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        flags[i] = (i == 0 || (in[i] >> shift) != (in[i - 1] >> shift)) ? 1u : 0u;
+}
+
+// out[pos[i]] = in[i] >> shift where flags[i]; seg[i] = pos[i] + flags[i] - 1
+__global__ void k_compact_runs(const uint32_t* __restrict__ in, const uint32_t* __restrict__ flags,
+                               const uint32_t* __restrict__ pos, uint32_t n, uint32_t shift,
+                               uint32_t* __restrict__ out, uint32_t* __restrict__ seg)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (flags[i]) out[pos[i]] = in[i] >> shift;
+        if (seg) seg[i] = pos[i] + flags[i] - 1u;
+    }
+}
+
+int32_t unique_runs(mrh_context* ctx, const uint32_t* in, uint32_t n, uint32_t shift, uint32_t* out,
+                    uint32_t* seg, uint32_t* flags, uint32_t* pos, uint32_t* count)
+{
+    *count = 0;
+    if (n == 0) return MRH_OK;
+    const uint32_t g = stride_grid(ctx, n, 256);
+    MRH_TRY(launch(ctx, "svo_flag", dim3(g), dim3(256), 0, k_flag_runs, in, n, shift, flags));
+    MRH_TRY(scan_exclusive(ctx, flags, pos, n));
+    MRH_TRY(launch(ctx, "svo_compact", dim3(g), dim3(256), 0, k_compact_runs, in, (const uint32_t*)flags,
+                   (const uint32_t*)pos, n, shift, out, seg));
+    uint32_t last[2];
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(&last[1], flags + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *count = last[0] + last[1];
+    return MRH_OK;
+}
+
+// out[pos[i]] = in[i] where the record is kept
+__global__ void k_flag_records(const uint32_t* __restrict__ codes, uint32_t n, uint32_t* __restrict__ flags)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        flags[i] = (codes[i] != SVO_NO_NODE) ? 1u : 0u;
+}
+
+__global__ void k_scatter_records(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ bins,
+                                  const float* __restrict__ rad, const uint32_t* __restrict__ pos, uint32_t n,
+                                  uint32_t* __restrict__ oc, uint32_t* __restrict__ ob, float* __restrict__ orad)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = codes[i];
+        if (c == SVO_NO_NODE) continue;
+        const uint32_t k = pos[i];
+        oc[k] = c;
+        ob[k] = bins[i];
+        orad[k] = rad[i];
+    }
+}
+
+// ---------------------------------------------------------------- build
+__global__ void k_svo_codes(const float* __restrict__ pts, uint32_t n, uint32_t pitch, float lox, float loy,
+                            float loz, float inv_cell, uint32_t depth, uint32_t* __restrict__ codes)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        codes[i] = svo_code(v3(pts[i], pts[(size_t)pitch + i], pts[2 * (size_t)pitch + i]), lox, loy, loz,
+                            inv_cell, depth);
+}
+
+// node records of one level: code, parent (global), child links into parent
+__global__ void k_svo_link(const uint32_t* __restrict__ lvl_codes, const uint32_t* __restrict__ parent_local,
+                           uint32_t count, uint32_t off, uint32_t parent_off, uint32_t* __restrict__ codes,
+                           uint32_t* __restrict__ parent, uint32_t* __restrict__ first_child,
+                           uint32_t* __restrict__ child_mask)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t g = off + i;
+        const uint32_t c = lvl_codes[i];
+        codes[g] = c;
+        first_child[g] = SVO_NO_NODE;
+        child_mask[g] = 0u;
+        if (parent_local) {
+            const uint32_t p = parent_off + parent_local[i];
+            parent[g] = p;
+            atomicOr(&child_mask[p], 1u << (c & 7u));
+            atomicMin(&first_child[p], g);
+        } else {
+            parent[g] = SVO_NO_NODE;
+        }
+    }
+}
+
+__global__ void k_svo_centers(V3* __restrict__ c)
+{
+    const uint32_t b = threadIdx.x;
+    if (b >= GUIDE_BINS) return;
+    const uint32_t ix = b % GUIDE_RES, iy = b / GUIDE_RES;
+    c[b] = octa_to_dir(((float)ix + 0.5f) / (float)GUIDE_RES, ((float)iy + 0.5f) / (float)GUIDE_RES);
+}
+
+// ---------------------------------------------------------------- splat / filter
+__global__ void k_svo_splat(SvoView s, const uint32_t* __restrict__ codes, const uint32_t* __restrict__ bins,
+                            const float* __restrict__ values, uint32_t n, unsigned long long* __restrict__ acc,
+                            uint32_t* __restrict__ cnt)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = codes[i];
+        if (c == SVO_NO_NODE) continue;
+        const uint32_t node = svo_lookup(s, c);
+        const uint32_t b = bins[i] & (GUIDE_BINS - 1u);
+        atomicAdd(&acc[(size_t)node * GUIDE_BINS + b], (unsigned long long)guide_fix(values[i]));
+        atomicAdd(&cnt[(size_t)node * GUIDE_BINS + b], 1u);
+    }
+}
+
+__global__ void k_svo_fix(size_t count, const unsigned long long* __restrict__ acc, float* __restrict__ sum)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        sum[i] = (float)acc[i] * (1.0f / GUIDE_FIX_ONE);
+}
+
+// interior sums = sums of children (one level, children already final)
+__global__ void k_svo_up(uint32_t off, uint32_t count, const uint32_t* __restrict__ first_child,
+                         const uint32_t* __restrict__ child_mask, float* __restrict__ sum, uint32_t* __restrict__ cnt)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count * GUIDE_BINS; t += gridDim.x * blockDim.x) {
+        const uint32_t node = off + t / GUIDE_BINS, b = t % GUIDE_BINS;
+        const uint32_t m = child_mask[node];
+        if (!m) continue;
+        const uint32_t fc = first_child[node];
+        const uint32_t k = (uint32_t)__popc(m);
+        float s = sum[(size_t)node * GUIDE_BINS + b];   // samples splatted here directly, if any
+        uint32_t c = cnt[(size_t)node * GUIDE_BINS + b];
+        for (uint32_t j = 0; j < k; j++) {
+            s += sum[(size_t)(fc + j) * GUIDE_BINS + b];
+            c += cnt[(size_t)(fc + j) * GUIDE_BINS + b];
+        }
+        sum[(size_t)node * GUIDE_BINS + b] = s;
+        cnt[(size_t)node * GUIDE_BINS + b] = c;
+    }
+}
+
+// samples under each node (counts already summed up the tree)
+__global__ void k_svo_node_count(uint32_t nodes, const uint32_t* __restrict__ cnt, uint32_t* __restrict__ out)
+{
+    for (uint32_t node = blockIdx.x * blockDim.x + threadIdx.x; node < nodes; node += gridDim.x * blockDim.x) {
+        uint32_t c = 0;
+        for (uint32_t b = 0; b < GUIDE_BINS; b++) c += cnt[(size_t)node * GUIDE_BINS + b];
+        out[node] = c;
+    }
+}
+
+// bins become means; a bin without samples takes its parent's value (root: 0)
+__global__ void k_svo_down(uint32_t off, uint32_t count, const uint32_t* __restrict__ parent,
+                           const uint32_t* __restrict__ cnt, float* __restrict__ mean)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count * GUIDE_BINS; t += gridDim.x * blockDim.x) {
+        const uint32_t node = off + t / GUIDE_BINS, b = t % GUIDE_BINS;
+        const size_t k = (size_t)node * GUIDE_BINS + b;
+        const uint32_t c = cnt[k];
+        if (c) {
+            mean[k] = mean[k] / (float)c;
+        } else {
+            const uint32_t p = parent[node];
+            mean[k] = (p == SVO_NO_NODE) ? 0.0f : mean[(size_t)p * GUIDE_BINS + b];
+        }
+    }
+}
+
+// 3x3 box blur over the bin grid (clamped at the borders) + floor
+__global__ void k_svo_blur(uint32_t nodes, const float* __restrict__ mean, float* __restrict__ rad)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nodes * GUIDE_BINS; t += gridDim.x * blockDim.x) {
+        const uint32_t node = t / GUIDE_BINS, b = t % GUIDE_BINS;
+        const int ix = (int)(b % GUIDE_RES), iy = (int)(b / GUIDE_RES);
+        float s = 0.0f;
+        int k = 0;
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int x = ix + dx, y = iy + dy;
+                if (x < 0 || y < 0 || x >= (int)GUIDE_RES || y >= (int)GUIDE_RES) continue;
+                s += mean[(size_t)node * GUIDE_BINS + y * GUIDE_RES + x];
+                k++;
+            }
+        rad[t] = s / (float)k + GUIDE_EPS;
+    }
+}
+
+}  // namespace
+
+SvoView svo_view(const mrh_svo* s)
+{
+    SvoView v;
+    v.depth = s->depth;
+    for (int i = 0; i < 12; i++) v.lvl_off[i] = s->lvl_off[i];
+    v.first_child = s->first_child;
+    v.child_mask = s->child_mask;
+    v.radiance = s->radiance;
+    v.node_count = s->node_count;
+    v.lox = s->cube[0];
+    v.loy = s->cube[1];
+    v.loz = s->cube[2];
+    v.inv_cell = (float)(1u << s->depth) / s->cube[3];
+    v.centers = s->centers;
+    return v;
+}
+
+static int32_t svo_reserve(mrh_svo* s, uint32_t nodes)
+{
+    mrh_context* ctx = s->ctx;
+    if (nodes <= s->cap) return MRH_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    void* ptrs[] = {s->codes, s->first_child, s->child_mask, s->parent, s->sum, s->acc, s->cnt, s->radiance,
+                    s->node_count};
+    for (void* p : ptrs) (void)hipFree(p);
+    s->codes = s->first_child = s->child_mask = s->parent = s->node_count = nullptr;
+    s->sum = s->radiance = nullptr;
+    s->acc = nullptr;
+    s->cnt = nullptr;
+    s->cap = 0;
+    const uint32_t cap = nodes + nodes / 4 + 64;
+    MRH_TRY(dmalloc(ctx, (void**)&s->codes, (size_t)cap * 4));
+    MRH_TRY(dmalloc(ctx, (void**)&s->first_child, (size_t)cap * 4));
+    MRH_TRY(dmalloc(ctx, (void**)&s->child_mask, (size_t)cap * 4));
+    MRH_TRY(dmalloc(ctx, (void**)&s->parent, (size_t)cap * 4));
+    MRH_TRY(dmalloc(ctx, (void**)&s->sum, (size_t)cap * GUIDE_BINS * 4));
+    MRH_TRY(dmalloc(ctx, (void**)&s->acc, (size_t)cap * GUIDE_BINS * 8));
+    MRH_TRY(dmalloc(ctx, (void**)&s->cnt, (size_t)cap * GUIDE_BINS * 4));
+    MRH_TRY(dmalloc(ctx, (void**)&s->radiance, (size_t)cap * GUIDE_BINS * 4));
+    MRH_TRY(dmalloc(ctx, (void**)&s->node_count, (size_t)cap * 4));
+    s->cap = cap;
+    return MRH_OK;
+}
+
+int32_t svo_codes(mrh_context* ctx, const float* d_points, uint32_t n, uint32_t pitch, const float cube[4],
+                  uint32_t depth, uint32_t* d_codes)
+{
+    const float inv_cell = (float)(1u << depth) / cube[3];
+    return launch(ctx, "svo_codes", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_svo_codes, d_points, n, pitch,
+                  cube[0], cube[1], cube[2], inv_cell, depth, d_codes);
+}
+
+int32_t svo_build(mrh_context* ctx, mrh_svo* s, const uint32_t* d_codes, uint32_t n, const float cube[4],
+                  uint32_t depth)
+{
+    if (depth < 1 || depth > 10) return fail(ctx, MRH_ERR_INVALID, "svo depth must be in [1, 10]");
+    if (n == 0) return fail(ctx, MRH_ERR_INVALID, "svo build: no points");
+    if (!(cube[3] > 0.0f)) return fail(ctx, MRH_ERR_INVALID, "svo build: empty cube");
+    for (int i = 0; i < 4; i++) s->cube[i] = cube[i];
+    s->depth = depth;
+    // scratch: sorted codes, values, flags, pos, per-level codes and parent ids
+    std::vector<void*> tmp;
+    auto take = [&](size_t bytes, void** out) -> int32_t {
+        MRH_TRY(dmalloc(ctx, out, bytes));
+        tmp.push_back(*out);
+        return MRH_OK;
+    };
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(ctx->stream);
+        for (void* p : tmp) (void)hipFree(p);
+    };
+    int32_t rc = MRH_OK;
+    uint32_t *sorted = nullptr, *vals = nullptr, *flags = nullptr, *pos = nullptr;
+    do {
+        if ((rc = take((size_t)n * 4, (void**)&sorted)) != MRH_OK) break;
+        if ((rc = take((size_t)n * 4, (void**)&vals)) != MRH_OK) break;
+        if ((rc = take((size_t)n * 4, (void**)&flags)) != MRH_OK) break;
+        if ((rc = take((size_t)n * 4, (void**)&pos)) != MRH_OK) break;
+        SortArgs sa;
+        sa.keys_in = d_codes;
+        sa.keys_out = sorted;
+        sa.vals_out = vals;
+        sa.n_max = n;
+        sa.key_bytes = 4;
+        sa.begin_bit = 0;
+        sa.end_bit = 3 * depth;
+        if ((rc = radix_sort(ctx, sa)) != MRH_OK) break;
+        // leaves, then each level up: codes and the parent ids of the level below
+        uint32_t* lvl_codes[11] = {};
+        uint32_t* lvl_parent[11] = {};   // parent_local of level l's nodes (index into level l-1)
+        uint32_t count = 0;
+        void* p = nullptr;
+        if ((rc = take((size_t)n * 4, &p)) != MRH_OK) break;
+        lvl_codes[depth] = (uint32_t*)p;
+        if ((rc = unique_runs(ctx, sorted, n, 0, lvl_codes[depth], nullptr, flags, pos, &count)) != MRH_OK) break;
+        s->lvl_count[depth] = count;
+        for (int l = (int)depth - 1; l >= 0 && rc == MRH_OK; l--) {
+            const uint32_t below = s->lvl_count[l + 1];
+            if ((rc = take((size_t)below * 4, &p)) != MRH_OK) break;
+            lvl_codes[l] = (uint32_t*)p;
+            if ((rc = take((size_t)below * 4, &p)) != MRH_OK) break;
+            lvl_parent[l + 1] = (uint32_t*)p;
+            rc = unique_runs(ctx, lvl_codes[l + 1], below, 3, lvl_codes[l], lvl_parent[l + 1], flags, pos, &count);
+            s->lvl_count[l] = count;
+        }
+        if (rc != MRH_OK) break;
+        if (s->lvl_count[0] != 1) {
+            rc = fail(ctx, MRH_ERR_HIP, "svo build: %u roots", s->lvl_count[0]);
+            break;
+        }
+        s->lvl_off[0] = 0;
+        for (uint32_t l = 0; l <= depth; l++) s->lvl_off[l + 1] = s->lvl_off[l] + s->lvl_count[l];
+        for (uint32_t l = depth + 1; l < 11; l++) s->lvl_off[l + 1] = s->lvl_off[l];
+        s->nodes = s->lvl_off[depth + 1];
+        if ((rc = svo_reserve(s, s->nodes)) != MRH_OK) break;
+        for (uint32_t l = 0; l <= depth && rc == MRH_OK; l++) {
+            const uint32_t c = s->lvl_count[l];
+            rc = launch(ctx, "svo_link", dim3(stride_grid(ctx, c, 256)), dim3(256), 0, k_svo_link,
+                        (const uint32_t*)lvl_codes[l], (const uint32_t*)lvl_parent[l], c, s->lvl_off[l],
+                        l ? s->lvl_off[l - 1] : 0u, s->codes, s->parent, s->first_child, s->child_mask);
+        }
+        if (rc != MRH_OK) break;
+        if (!s->centers) {
+            if ((rc = dmalloc(ctx, (void**)&s->centers, GUIDE_BINS * sizeof(V3))) != MRH_OK) break;
+        }
+        if ((rc = launch(ctx, "svo_centers", dim3(1), dim3(64), 0, k_svo_centers, s->centers)) != MRH_OK) break;
+        // fresh radiance
+        if ((rc = fill_u32(ctx, (uint32_t*)s->sum, 0u, (size_t)s->nodes * GUIDE_BINS)) != MRH_OK) break;
+        if ((rc = fill_u32(ctx, (uint32_t*)s->acc, 0u, (size_t)s->nodes * GUIDE_BINS * 2)) != MRH_OK) break;
+        if ((rc = fill_u32(ctx, s->cnt, 0u, (size_t)s->nodes * GUIDE_BINS)) != MRH_OK) break;
+        uint32_t eps_bits;
+        std::memcpy(&eps_bits, &GUIDE_EPS, 4);
+        rc = fill_u32(ctx, (uint32_t*)s->radiance, eps_bits, (size_t)s->nodes * GUIDE_BINS);
+        if (rc == MRH_OK) rc = fill_u32(ctx, s->node_count, 0u, s->nodes);
+    } while (0);
+    cleanup();
+    return rc;
+}
+
+int32_t svo_splat(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, const uint32_t* bins, const float* values,
+                  uint32_t n)
+{
+    if (!s->nodes) return fail(ctx, MRH_ERR_INVALID, "svo splat: empty svo");
+    if (n == 0) return MRH_OK;
+    return launch(ctx, "svo_splat", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_svo_splat, svo_view(s), codes,
+                  bins, values, n, s->acc, s->cnt);
+}
+
+int32_t svo_filter(mrh_context* ctx, mrh_svo* s)
+{
+    if (!s->nodes) return fail(ctx, MRH_ERR_INVALID, "svo filter: empty svo");
+    const size_t bins = (size_t)s->nodes * GUIDE_BINS;
+    MRH_TRY(launch(ctx, "svo_fix", dim3(stride_grid(ctx, bins, 256)), dim3(256), 0, k_svo_fix, bins,
+                   (const unsigned long long*)s->acc, s->sum));
+    for (int l = (int)s->depth - 1; l >= 0; l--) {
+        const uint32_t c = s->lvl_count[l];
+        MRH_TRY(launch(ctx, "svo_up", dim3(stride_grid(ctx, (uint64_t)c * GUIDE_BINS, 256)), dim3(256), 0, k_svo_up,
+                       s->lvl_off[l], c, (const uint32_t*)s->first_child, (const uint32_t*)s->child_mask, s->sum,
+                       s->cnt));
+    }
+    MRH_TRY(launch(ctx, "svo_node_count", dim3(stride_grid(ctx, s->nodes, 256)), dim3(256), 0, k_svo_node_count,
+                   s->nodes, (const uint32_t*)s->cnt, s->node_count));
+    for (uint32_t l = 0; l <= s->depth; l++) {
+        const uint32_t c = s->lvl_count[l];
+        MRH_TRY(launch(ctx, "svo_down", dim3(stride_grid(ctx, (uint64_t)c * GUIDE_BINS, 256)), dim3(256), 0,
+                       k_svo_down, s->lvl_off[l], c, (const uint32_t*)s->parent, (const uint32_t*)s->cnt, s->sum));
+    }
+    return launch(ctx, "svo_blur", dim3(stride_grid(ctx, (uint64_t)s->nodes * GUIDE_BINS, 256)), dim3(256), 0,
+                  k_svo_blur, s->nodes, (const float*)s->sum, s->radiance);
+}
+
+int32_t svo_compact_records(mrh_context* ctx, const uint32_t* codes, const uint32_t* bins, const float* rad,
+                            uint32_t n, uint32_t* out_codes, uint32_t* out_bins, float* out_rad, uint32_t* count)
+{
+    *count = 0;
+    if (n == 0) return MRH_OK;
+    void* p = nullptr;
+    MRH_TRY(dmalloc(ctx, &p, (size_t)n * 8));
+    uint32_t* flags = (uint32_t*)p;
+    uint32_t* pos = flags + n;
+    const uint32_t g = stride_grid(ctx, n, 256);
+    int32_t rc = launch(ctx, "svo_rec_flag", dim3(g), dim3(256), 0, k_flag_records, codes, n, flags);
+    if (rc == MRH_OK) rc = scan_exclusive(ctx, flags, pos, n);
+    if (rc == MRH_OK)
+        rc = launch(ctx, "svo_rec_scatter", dim3(g), dim3(256), 0, k_scatter_records, codes, bins, rad,
+                    (const uint32_t*)pos, n, out_codes, out_bins, out_rad);
+    uint32_t last[2] = {0, 0};
+    if (rc == MRH_OK) {
+        (void)hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
+        (void)hipMemcpyAsync(&last[1], flags + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(p);
+    if (rc != MRH_OK) return rc;
+    if (e != hipSuccess) return fail(ctx, MRH_ERR_HIP, "svo record compaction: %s", hipGetErrorString(e));
+    *count = last[0] + last[1];
+    return MRH_OK;
+}
+
+mrh_context* svo_ctx(const mrh_svo* s) { return s->ctx; }
+
+void svo_free(mrh_svo* s)
+{
+    if (!s) return;
+    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+    void* ptrs[] = {s->codes, s->first_child, s->child_mask, s->parent,  s->sum,
+                    s->acc,   s->cnt,         s->radiance,   s->centers, s->node_count};
+    for (void* p : ptrs) (void)hipFree(p);
+    delete s;
+}
+
+namespace {
+
+__global__ void k_svo_lookup(SvoView s, const uint32_t* __restrict__ codes, uint32_t n, uint32_t min_count,
+                             uint32_t* __restrict__ out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = min_count ? svo_lookup_guide(s, codes[i], min_count) : svo_lookup(s, codes[i]);
+}
+
+__global__ void k_svo_sample(SvoView s, const uint32_t* __restrict__ nodes, const float* __restrict__ nrm,
+                             const float* __restrict__ xi, uint32_t n, float* __restrict__ dirs,
+                             float* __restrict__ pdf)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const V3 nn = v3(nrm[i], nrm[(size_t)n + i], nrm[2 * (size_t)n + i]);
+        float pd;
+        const V3 d = guide_sample(s, nodes[i], nn, guide_total(s, nodes[i], nn), xi[i], xi[(size_t)n + i],
+                                  xi[2 * (size_t)n + i], pd);
+        dirs[i] = d.x;
+        dirs[(size_t)n + i] = d.y;
+        dirs[2 * (size_t)n + i] = d.z;
+        pdf[i] = pd;
+    }
+}
+
+__global__ void k_svo_pdf(SvoView s, const uint32_t* __restrict__ nodes, const float* __restrict__ nrm,
+                          const float* __restrict__ dirs, uint32_t n, float* __restrict__ pdf)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const V3 nn = v3(nrm[i], nrm[(size_t)n + i], nrm[2 * (size_t)n + i]);
+        const V3 d = v3(dirs[i], dirs[(size_t)n + i], dirs[2 * (size_t)n + i]);
+        pdf[i] = guide_pdf(s, nodes[i], nn, guide_total(s, nodes[i], nn), d);
+    }
+}
+
+}  // namespace
+}  // namespace mrh
+
+using namespace mrh;
+
+extern "C" {
+
+int32_t mrh_svo_create(mrh_context* ctx, mrh_svo** out)
+{
+    if (!ctx || !out) return MRH_ERR_INVALID;
+    mrh_svo* s = new mrh_svo();
+    s->ctx = ctx;
+    *out = s;
+    return MRH_OK;
+}
+
+int32_t mrh_svo_destroy(mrh_svo* svo)
+{
+    if (!svo) return MRH_ERR_INVALID;
+    svo_free(svo);
+    return MRH_OK;
+}
+
+int32_t mrh_svo_codes(mrh_context* ctx, const float* d_points, uint32_t n, uint32_t pitch, const float cube[4],
+                      uint32_t depth, uint32_t* d_codes)
+{
+    if (!ctx || !cube) return MRH_ERR_INVALID;
+    if (n == 0) return MRH_OK;
+    if (!d_points || !d_codes || pitch < n) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_codes: bad arguments");
+    if (depth < 1 || depth > 10) return fail(ctx, MRH_ERR_INVALID, "svo depth must be in [1, 10]");
+    return svo_codes(ctx, d_points, n, pitch, cube, depth, d_codes);
+}
+
+int32_t mrh_svo_build(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, uint32_t n, const float cube[4],
+                      uint32_t depth)
+{
+    if (!ctx || !svo || !cube || svo->ctx != ctx) return MRH_ERR_INVALID;
+    if (n && !d_codes) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_build: null codes");
+    return svo_build(ctx, svo, d_codes, n, cube, depth);
+}
+
+int32_t mrh_svo_splat(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, const uint32_t* d_bins,
+                      const float* d_values, uint32_t n)
+{
+    if (!ctx || !svo || svo->ctx != ctx) return MRH_ERR_INVALID;
+    if (n && (!d_codes || !d_bins || !d_values)) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_splat: null input");
+    return svo_splat(ctx, svo, d_codes, d_bins, d_values, n);
+}
+
+int32_t mrh_svo_filter(mrh_context* ctx, mrh_svo* svo)
+{
+    if (!ctx || !svo || svo->ctx != ctx) return MRH_ERR_INVALID;
+    return svo_filter(ctx, svo);
+}
+
+int32_t mrh_svo_lookup(mrh_context* ctx, const mrh_svo* svo, const uint32_t* d_codes, uint32_t n, uint32_t min_count,
+                       uint32_t* d_nodes)
+{
+    if (!ctx || !svo) return MRH_ERR_INVALID;
+    if (n == 0) return MRH_OK;
+    if (!svo->nodes || !d_codes || !d_nodes) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_lookup: bad arguments");
+    return launch(ctx, "svo_lookup", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_svo_lookup, svo_view(svo),
+                  d_codes, n, min_count, d_nodes);
+}
+
+int32_t mrh_svo_sample(mrh_context* ctx, const mrh_svo* svo, const uint32_t* d_nodes, const float* d_normals,
+                       const float* d_xi, uint32_t n, float* d_dirs, float* d_pdf)
+{
+    if (!ctx || !svo) return MRH_ERR_INVALID;
+    if (n == 0) return MRH_OK;
+    if (!svo->nodes || !d_nodes || !d_normals || !d_xi || !d_dirs || !d_pdf)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_sample: bad arguments");
+    return launch(ctx, "svo_sample", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_svo_sample, svo_view(svo),
+                  d_nodes, d_normals, d_xi, n, d_dirs, d_pdf);
+}
+
+int32_t mrh_svo_pdf(mrh_context* ctx, const mrh_svo* svo, const uint32_t* d_nodes, const float* d_normals,
+                    const float* d_dirs, uint32_t n, float* d_pdf)
+{
+    if (!ctx || !svo) return MRH_ERR_INVALID;
+    if (n == 0) return MRH_OK;
+    if (!svo->nodes || !d_nodes || !d_normals || !d_dirs || !d_pdf)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_pdf: bad arguments");
+    return launch(ctx, "svo_pdf", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_svo_pdf, svo_view(svo), d_nodes,
+                  d_normals, d_dirs, n, d_pdf);
+}
+
+int32_t mrh_svo_view_get(const mrh_svo* s, mrh_svo_view* v)
+{
+    if (!s || !v) return MRH_ERR_INVALID;
+    v->depth = s->depth;
+    v->nodes = s->nodes;
+    for (int i = 0; i < 11; i++) v->level_count[i] = s->lvl_count[i];
+    for (int i = 0; i < 12; i++) v->level_offset[i] = s->lvl_off[i];
+    for (int i = 0; i < 4; i++) v->cube[i] = s->cube[i];
+    v->d_codes = s->codes;
+    v->d_first_child = s->first_child;
+    v->d_child_mask = s->child_mask;
+    v->d_parent = s->parent;
+    v->d_sum = s->sum;
+    v->d_count = s->cnt;
+    v->d_radiance = s->radiance;
+    v->d_node_count = s->node_count;
+    return MRH_OK;
+}
+
+}  // extern "C"

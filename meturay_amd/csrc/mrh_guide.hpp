@@ -1,0 +1,227 @@
+// mrh_guide.hpp — sparse voxel octree (SVO) radiance cache for path guiding:
+// device-side lookup and product sampling shared by the guide kernels and
+// the guided shade kernel.
+//
+// Directions are binned on an 8x8 grid of the equal-area concentric
+// octahedral map (Clarberg 2008), the parameterisation the reference's
+// guided tracer uses for its radiance fields
+// (RayLib/CoordinateConversion.h:112-180): uniform (u,v) maps to uniform
+// solid angle, so every bin spans 4*pi/64 steradians.
+#pragma once
+
+#include "mrh_internal.hpp"
+
+namespace mrh {
+
+constexpr uint32_t GUIDE_RES = 8;                  // bins per axis
+constexpr uint32_t GUIDE_BINS = GUIDE_RES * GUIDE_RES;
+constexpr uint32_t SVO_NO_NODE = 0xFFFFFFFFu;
+constexpr float GUIDE_EPS = 1.0e-4f;               // radiance floor of a bin
+
+// Splatted radiance is summed in 48.16 fixed point so that the sums, and every
+// guide built from them, do not depend on the order of the atomic adds.
+// Samples are clamped to [0, 32767].
+constexpr float GUIDE_FIX_ONE = 65536.0f;
+constexpr float GUIDE_FIX_MAX = 32767.0f;
+MRH_HD uint64_t guide_fix(float v)
+{
+    return (v > 0.0f) ? (uint64_t)(fminf(v, GUIDE_FIX_MAX) * GUIDE_FIX_ONE) : 0ull;
+}
+
+// (u,v) in [0,1]^2 -> unit direction, z up
+MRH_HD V3 octa_to_dir(float u, float v)
+{
+    const float a = 2.0f * u - 1.0f, b = 2.0f * v - 1.0f;
+    const float ua = fabsf(a), va = fabsf(b);
+    const float d = 1.0f - (ua + va);
+    const float r = 1.0f - fabsf(d);
+    const float phin = (r == 0.0f) ? 0.0f : ((va - ua) / r + 1.0f) * 0.125f;   // phi / (2 pi)
+    const float z = copysignf(1.0f - r * r, d);
+    float s, c;
+    sincos_2pi(phin, s, c);
+    const float k = r * sqrtf(fmaxf(0.0f, 2.0f - r * r));
+    return v3(copysignf(k * c, a), copysignf(k * s, b), z);
+}
+
+// atan(t) for t in [0, 1]: two half-angle reductions,
+// atan(t) = 2 atan(t / (1 + sqrt(1 + t^2))), bring t below tan(pi/16),
+// where the Taylor series to t^13 is exact to about 2e-12.  Written out so
+// that host and device round identically.
+MRH_HD float atan_unit(float t)
+{
+    t = t / (1.0f + sqrtf(1.0f + t * t));
+    t = t / (1.0f + sqrtf(1.0f + t * t));
+    const float t2 = t * t;
+    float p = 0.07692307692307693f;                  //  1/13
+    p = p * t2 - 0.09090909090909091f;               // -1/11
+    p = p * t2 + 0.1111111111111111f;                //  1/9
+    p = p * t2 - 0.14285714285714285f;               // -1/7
+    p = p * t2 + 0.2f;                               //  1/5
+    p = p * t2 - 0.3333333333333333f;                // -1/3
+    p = p * t2 + 1.0f;
+    return 4.0f * (t * p);
+}
+
+// atan2(y, x) / (pi / 2) for y, x >= 0: the angle as a fraction of a quadrant
+MRH_HD float quadrant_angle(float y, float x)
+{
+    if (y == 0.0f && x == 0.0f) return 0.0f;
+    if (y <= x) return atan_unit(y / x) * 0.63661977236758134f;
+    return 1.0f - atan_unit(x / y) * 0.63661977236758134f;
+}
+
+// unit direction -> (u,v)
+MRH_HD void dir_to_octa(V3 w, float& u, float& v)
+{
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - fabsf(w.z)));
+    const float t = quadrant_angle(fabsf(w.y), fabsf(w.x));   // 2 phi / pi
+    float b = r * t, a = r - b;
+    if (w.z < 0.0f) {
+        const float a2 = 1.0f - b, b2 = 1.0f - a;
+        a = a2;
+        b = b2;
+    }
+    u = (copysignf(a, w.x) + 1.0f) * 0.5f;
+    v = (copysignf(b, w.y) + 1.0f) * 0.5f;
+}
+
+MRH_HD uint32_t dir_to_bin(V3 w)
+{
+    float u, v;
+    dir_to_octa(w, u, v);
+    uint32_t ix = (uint32_t)(u * (float)GUIDE_RES), iy = (uint32_t)(v * (float)GUIDE_RES);
+    ix = ix > GUIDE_RES - 1 ? GUIDE_RES - 1 : ix;
+    iy = iy > GUIDE_RES - 1 ? GUIDE_RES - 1 : iy;
+    return iy * GUIDE_RES + ix;
+}
+
+// Voxel code of a point: 30-bit Morton code of its cell at depth `depth`
+// (<= 10) of the cube [lo, lo + size)^3; points outside clamp to the border.
+MRH_HD uint32_t svo_code(V3 p, float lox, float loy, float loz, float inv_cell, uint32_t depth)
+{
+    const uint32_t qmax = (1u << depth) - 1u;
+    const float f[3] = {(p.x - lox) * inv_cell, (p.y - loy) * inv_cell, (p.z - loz) * inv_cell};
+    uint32_t q[3];
+    for (int a = 0; a < 3; a++) {
+        float g = floorf(f[a]);
+        q[a] = (g <= 0.0f) ? 0u : (g >= (float)qmax ? qmax : (uint32_t)g);
+    }
+    return expand10(q[0]) | (expand10(q[1]) << 1) | (expand10(q[2]) << 2);
+}
+
+// Read-only view of a built SVO for device code.  Level l holds the unique
+// prefixes code >> 3(depth - l); node records of all levels are
+// concatenated (level offsets in lvl_off).
+struct SvoView {
+    uint32_t depth;
+    uint32_t lvl_off[12];            // first global node of each level, [depth+1] = total
+    const uint32_t* first_child;     // [nodes] global index of the first child (SVO_NO_NODE at leaves)
+    const uint32_t* child_mask;      // [nodes] 8-bit child occupancy
+    const float* radiance;           // [nodes][64] filtered radiance
+    const uint32_t* node_count;      // [nodes] samples in the subtree (after filter)
+    float lox, loy, loz, inv_cell;   // cube of the leaf grid
+    const V3* centers;               // [64] bin centre directions
+};
+
+// Deepest existing node on the path from the root to `code`'s leaf cell.
+__device__ __forceinline__ uint32_t svo_lookup(const SvoView& s, uint32_t code)
+{
+    uint32_t node = 0;   // root
+    for (uint32_t l = 0; l < s.depth; l++) {
+        const uint32_t m = s.child_mask[node];
+        const uint32_t oct = (code >> (3u * (s.depth - 1u - l))) & 7u;
+        if (!((m >> oct) & 1u)) break;
+        node = s.first_child[node] + (uint32_t)__popc(m & ((1u << oct) - 1u));
+    }
+    return node;
+}
+
+// Guide lookup: the deepest node on the path whose data rests on at least
+// min_count samples (the role of the reference's collapse of sparse bins to
+// their ancestors, AnisoSVO.cu:1485-1529, BinRayCount); the root when no child
+// qualifies.
+__device__ __forceinline__ uint32_t svo_lookup_guide(const SvoView& s, uint32_t code, uint32_t min_count)
+{
+    uint32_t node = 0;
+    for (uint32_t l = 0; l < s.depth; l++) {
+        const uint32_t m = s.child_mask[node];
+        const uint32_t oct = (code >> (3u * (s.depth - 1u - l))) & 7u;
+        if (!((m >> oct) & 1u)) break;
+        const uint32_t child = s.first_child[node] + (uint32_t)__popc(m & ((1u << oct) - 1u));
+        if (s.node_count[child] < min_count) break;
+        node = child;
+    }
+    return node;
+}
+
+// Product of a node's filtered radiance and the clamped cosine around n,
+// evaluated at the bin centres.
+__device__ __forceinline__ float guide_weight(const SvoView& s, const float* __restrict__ rad, uint32_t b, V3 n)
+{
+    return rad[b] * fmaxf(dot(n, s.centers[b]), 0.0f);
+}
+
+__device__ __forceinline__ float guide_total(const SvoView& s, uint32_t node, V3 n)
+{
+    const float* rad = s.radiance + (size_t)node * GUIDE_BINS;
+    float t = 0.0f;
+    for (uint32_t b = 0; b < GUIDE_BINS; b++) t += guide_weight(s, rad, b, n);
+    return t;
+}
+
+// Solid-angle pdf of one bin's share of the product (equal-area bins)
+constexpr float GUIDE_BIN_SR_INV = (float)GUIDE_BINS * 0.07957747154594767f;   // 64 / (4 pi)
+
+// Samples a bin with probability proportional to its product weight, then a
+// uniform point of the bin in the map.  pdf = 0 when the product vanishes.
+// `total` is guide_total(s, node, n).
+__device__ __forceinline__ V3 guide_sample(const SvoView& s, uint32_t node, V3 n, float total, float xi0, float xi1,
+                                           float xi2, float& pdf)
+{
+    const float* rad = s.radiance + (size_t)node * GUIDE_BINS;
+    if (!(total > 0.0f)) {
+        pdf = 0.0f;
+        return v3(0.0f, 0.0f, 1.0f);
+    }
+    const float target = xi0 * total;
+    float acc = 0.0f, wp = 0.0f;
+    uint32_t pick = 0;
+    for (uint32_t b = 0; b < GUIDE_BINS; b++) {
+        const float w = guide_weight(s, rad, b, n);
+        if (w > 0.0f) {
+            pick = b;
+            wp = w;
+            acc += w;
+            if (acc > target) break;
+        }
+    }
+    const float u = ((float)(pick % GUIDE_RES) + xi1) / (float)GUIDE_RES;
+    const float v = ((float)(pick / GUIDE_RES) + xi2) / (float)GUIDE_RES;
+    pdf = wp / total * GUIDE_BIN_SR_INV;
+    return octa_to_dir(u, v);
+}
+
+__device__ __forceinline__ float guide_pdf(const SvoView& s, uint32_t node, V3 n, float total, V3 d)
+{
+    if (!(total > 0.0f)) return 0.0f;
+    const float w = guide_weight(s, s.radiance + (size_t)node * GUIDE_BINS, dir_to_bin(d), n);
+    return w / total * GUIDE_BIN_SR_INV;
+}
+
+// host-side handles shared by the guide and path tracer units
+SvoView svo_view(const mrh_svo* s);
+int32_t svo_codes(mrh_context* ctx, const float* d_points, uint32_t n, uint32_t pitch, const float cube[4],
+                  uint32_t depth, uint32_t* d_codes);
+int32_t svo_build(mrh_context* ctx, mrh_svo* s, const uint32_t* d_codes, uint32_t n, const float cube[4],
+                  uint32_t depth);
+int32_t svo_splat(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, const uint32_t* bins, const float* values,
+                  uint32_t n);
+int32_t svo_filter(mrh_context* ctx, mrh_svo* s);
+// Order-preserving compaction of the learning records whose code is not
+// SVO_NO_NODE; *count receives the number kept (synchronous).
+int32_t svo_compact_records(mrh_context* ctx, const uint32_t* codes, const uint32_t* bins, const float* rad,
+                            uint32_t n, uint32_t* out_codes, uint32_t* out_bins, float* out_rad, uint32_t* count);
+void svo_free(mrh_svo* s);
+mrh_context* svo_ctx(const mrh_svo* s);
+
+}  // namespace mrh

@@ -14,6 +14,8 @@
 //   k_lbvh_depth   : deepest leaf, bounds the traversal stack.
 #include "mrh_bvh.hpp"
 
+#include <cstring>
+
 namespace mrh {
 namespace {
 
@@ -324,6 +326,13 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src)
     MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     b->root = host_meta[0];
     b->max_depth = host_meta[1];
+    for (int a = 0; a < 3; a++) {
+        const uint32_t e0 = host_meta[2 + a], e1 = host_meta[5 + a];
+        uint32_t u0 = (e0 & 0x80000000u) ? (e0 & 0x7FFFFFFFu) : ~e0;
+        uint32_t u1 = (e1 & 0x80000000u) ? (e1 & 0x7FFFFFFFu) : ~e1;
+        std::memcpy(&b->lo[a], &u0, 4);
+        std::memcpy(&b->hi[a], &u1, 4);
+    }
     if (host_meta[8])
         return fail(ctx, MRH_ERR_HIP, "lbvh build: inconsistent hierarchy (flags 0x%x)", host_meta[8]);
     if (b->root >= 2 * n - 1)

@@ -16,6 +16,7 @@
 // Paths own a PCG32 stream keyed by (seed, global sample id), so results do
 // not depend on the order rays are processed in.
 #include "mrh_bvh.hpp"
+#include "mrh_guide.hpp"
 
 #include <cfloat>
 #include <cstring>
@@ -433,11 +434,79 @@ __device__ __forceinline__ void dielectric_sample(const Surface& s, V3 wi, const
     }
 }
 
-// PathTracerKC.cuh:337-426 (light hit) and 430-648 (surface work)
+// Lambert f*cos and the pdf of its (uniform hemisphere) sampler
+// (SimpleMaterialsKC.cuh:80-101 Evaluate; sampling pdf 1/(2 pi) on the
+// normal's side).  The reference's LambertC::Pdf returns the cosine pdf, which
+// does not match its uniform sampler; the mixture below needs the sampler's
+// own pdf, so that is what this returns.
+__device__ __forceinline__ V3 lambert_eval(const Surface& s, V3 albedo, V3 dir, float& pdf)
+{
+    const V3 normal = rotate(conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
+    const float ndl = dot(normal, dir);
+    pdf = (ndl > 0.0f) ? INV_PI * 0.5f : 0.0f;
+    return scale(scale(albedo, fmaxf(ndl, 0.0f)), INV_PI);
+}
+
+// MetalMatFuncs::Evaluate / Pdf (MetalGlassMaterialsKC.cuh:152-220)
+__device__ __forceinline__ V3 metal_eval(const Surface& s, V3 wi, const float* __restrict__ m, V3 dir, float& pdf)
+{
+    const float rough = m[0];
+    const float alpha = rough * rough;
+    const V3 L = rotate(s.w2t, dir);
+    const V3 V = rotate(s.w2t, wi);
+    const V3 H = normalize(L + V);
+    const float ndh = fmaxf(0.0f, H.z);
+    const float vdh = fmaxf(0.0f, dot(V, H));
+    const float ldh = fmaxf(0.0f, dot(L, H));
+    const float ndv = fmaxf(0.0f, V.z);
+    float pspec = vndf_pdf(V, H, alpha);
+    float D = dggx(ndh, alpha);
+    pspec = (isnan(D) || isinf(D)) ? 0.0f : pspec;
+    pdf = (vdh == 0.0f) ? 0.0f : pspec / (4.0f * vdh);
+    D = (isnan(D) || isinf(D)) ? 0.0f : D;
+    float G = gsmith_corr(V, L, alpha);
+    G = (ldh == 0.0f) ? 0.0f : G;
+    G = (vdh == 0.0f) ? 0.0f : G;
+    V3 F = v3(fresnel_conductor(vdh, m[1], m[4]), fresnel_conductor(vdh, m[2], m[5]),
+              fresnel_conductor(vdh, m[3], m[6]));
+    F = mul(F, v3(m[7], m[8], m[9]));
+    V3 spec = v3(((D * F.x) * G) * 0.25f, ((D * F.y) * G) * 0.25f, ((D * F.z) * G) * 0.25f);
+    spec = divs(spec, ndv);
+    return (ndv == 0.0f) ? v3(0.0f, 0.0f, 0.0f) : spec;
+}
+
+// Utility::RGBToLuminance (RayLib/ColorConversion.h:15-22, its coefficients)
+__device__ __forceinline__ float luminance(V3 c) { return (0.2120f * c.x + 0.7150f * c.y) + 0.0722f * c.z; }
+
+// Path-vertex records of the learning pass, indexed [sample * max_depth + bounce]
+struct RecBuf {
+    uint32_t* code;    // voxel code of the vertex, SVO_NO_NODE = no record
+    uint32_t* bin;     // direction bin of the continuing ray
+    float* tr;         // throughput leaving the vertex (PathNode.cuh:116 radFactor), R
+    float* tg;         // G
+    float* tb;         // B
+    float* rad;        // luminance of the radiance arriving along that ray
+    uint32_t max_depth;
+};
+
+struct GuideArgs {
+    RecBuf rec;
+    SvoView svo;        // valid when GUIDE
+    float lox, loy, loz, inv_cell;   // leaf grid of the records
+    uint32_t depth;
+    float ratio;        // probability of sampling the guide
+    uint32_t min_samples;   // guide lookup threshold
+};
+
+// PathTracerKC.cuh:337-426 (light hit) and 430-648 (surface work); with
+// LEARN the WFPG path-node bookkeeping (WFPGTracerKC.cuh:202-520) and with
+// GUIDE the one-sample MIS of guide and BxDF (WFPGTracerKC.cuh:1480-1553,
+// pdf = r p_guide + (1 - r) p_bxdf, weight = f cos / pdf).
+template <bool LEARN, bool GUIDE>
 __global__ __launch_bounds__(256) void k_shade(
     PathBuf in, PathBuf out, HitBuf hit, const uint32_t* __restrict__ order, uint32_t* counts,
     uint32_t bounce, uint32_t last, uint32_t rr_start, uint32_t spp, uint32_t sample_base,
-    Materials mats, const float4* __restrict__ leaves, float* __restrict__ image)
+    Materials mats, const float4* __restrict__ leaves, float* __restrict__ image, GuideArgs ga)
 {
     const uint32_t n = counts[bounce];
     const uint32_t lane = threadIdx.x & 63;
@@ -461,6 +530,20 @@ __global__ __launch_bounds__(256) void k_shade(
                     atomicAdd(px + 0, le[0] * T.x);
                     atomicAdd(px + 1, le[1] * T.y);
                     atomicAdd(px + 2, le[2] * T.z);
+                    if (LEARN) {
+                        // radiance arriving along each earlier vertex's outgoing ray:
+                        // path contribution over that vertex's throughput
+                        // (PathNode.cuh:152-190, per channel, then luminance)
+                        const V3 c = mul(v3(le[0], le[1], le[2]), T);
+                        const size_t r0 = (size_t)sample * ga.rec.max_depth;
+                        for (uint32_t k = 0; k < bounce; k++) {
+                            if (ga.rec.code[r0 + k] == SVO_NO_NODE) continue;
+                            const float fr = ga.rec.tr[r0 + k], fg = ga.rec.tg[r0 + k], fb = ga.rec.tb[r0 + k];
+                            const V3 L = v3(fr == 0.0f ? 0.0f : c.x / fr, fg == 0.0f ? 0.0f : c.y / fg,
+                                            fb == 0.0f ? 0.0f : c.z / fb);
+                            ga.rec.rad[r0 + k] = luminance(L);
+                        }
+                    }
                 } else if (!last) {
                     const uint32_t li = hit.leaf[rid];
                     const float4 l0 = leaves[3 * (size_t)li + 0];
@@ -475,7 +558,53 @@ __global__ __launch_bounds__(256) void k_shade(
                     const uint32_t depth = dm & 0xFFu, medium = dm >> 8;
                     BsdfOut o;
                     o.medium = medium;
-                    if (batch == MRH_BATCH_LAMBERT) {
+                    bool guided = false;
+                    uint32_t gnode = 0;
+                    float gtotal = 0.0f;
+                    V3 ns = v3(0.0f, 0.0f, 1.0f);
+                    if (GUIDE && (batch == MRH_BATCH_LAMBERT ||
+                                  (batch == MRH_BATCH_METAL && (1.0f - mats.metal[(size_t)id * 10]) < 0.99f))) {
+                        ns = rotate(conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
+                        gnode = svo_lookup_guide(ga.svo,
+                                                 svo_code(s.pos, ga.svo.lox, ga.svo.loy, ga.svo.loz,
+                                                          ga.svo.inv_cell, ga.svo.depth),
+                                                 ga.min_samples);
+                        gtotal = guide_total(ga.svo, gnode, ns);
+                        guided = gtotal > 0.0f;
+                    }
+                    if (guided) {
+                        const float xi = rng.uniform();
+                        V3 dir;
+                        float pg = 0.0f, pb = 0.0f;
+                        V3 fcos;
+                        if (xi < ga.ratio) {
+                            const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+                            dir = guide_sample(ga.svo, gnode, ns, gtotal, u0, u1, u2, pg);
+                            if (batch == MRH_BATCH_LAMBERT) {
+                                const float* a = mats.lambert + (size_t)id * 3;
+                                fcos = lambert_eval(s, v3(a[0], a[1], a[2]), dir, pb);
+                            } else {
+                                fcos = metal_eval(s, wi, mats.metal + (size_t)id * 10, dir, pb);
+                            }
+                        } else {
+                            if (batch == MRH_BATCH_LAMBERT) {
+                                const float* a = mats.lambert + (size_t)id * 3;
+                                lambert_sample(s, v3(a[0], a[1], a[2]), rng, o);
+                            } else {
+                                metal_sample(s, wi, mats.metal + (size_t)id * 10, rng, o);
+                            }
+                            dir = o.d;
+                            fcos = o.refl;
+                            pb = o.pdf;
+                            pg = guide_pdf(ga.svo, gnode, ns, gtotal, dir);
+                        }
+                        o.d = dir;
+                        o.o = nudge(s.pos, s.geo_n);
+                        o.refl = fcos;
+                        o.pdf = ga.ratio * pg + (1.0f - ga.ratio) * pb;
+                        o.specular = false;
+                        o.medium = medium;
+                    } else if (batch == MRH_BATCH_LAMBERT) {
                         const float* a = mats.lambert + (size_t)id * 3;
                         lambert_sample(s, v3(a[0], a[1], a[2]), rng, o);
                     } else if (batch == MRH_BATCH_METAL) {
@@ -495,6 +624,16 @@ __global__ __launch_bounds__(256) void k_shade(
                         else t2 = scale(t2, 1.0f / pr);
                     }
                     alive = !((terminate && !o.specular) || is_zero(t2));
+                    if (LEARN && alive) {
+                        // path node of this vertex (PPGTracerKC.cuh:605-620)
+                        const size_t r = (size_t)sample * ga.rec.max_depth + bounce;
+                        ga.rec.code[r] = svo_code(s.pos, ga.lox, ga.loy, ga.loz, ga.inv_cell, ga.depth);
+                        ga.rec.bin[r] = dir_to_bin(normalize(o.d));
+                        ga.rec.tr[r] = t2.x;
+                        ga.rec.tg[r] = t2.y;
+                        ga.rec.tb[r] = t2.z;
+                        ga.rec.rad[r] = 0.0f;
+                    }
                     no = o.o;
                     nd = o.d;
                     nT = t2;
@@ -554,6 +693,8 @@ struct Frame {
     uint64_t total = 0, spp = 0;
     uint32_t w = 0, h = 0, S = 0, max_depth = 0, rr_start = 0, seed = 0, sample_base = 0;
     uint32_t count_traversal = 0, bounce = 0;
+    uint32_t learn = 0, guide = 0;   // record path vertices / sample the guide
+    GuideArgs ga = {};
 };
 
 void carve_frame(void* base, uint64_t cap, Frame& f)
@@ -680,6 +821,7 @@ int32_t frame_begin(mrh_context* ctx, const mrh_camera_desc* cd, const mrh_rende
     f.sample_base = sample_base;
     f.count_traversal = p->count_traversal;
     f.bounce = 0;
+    f.learn = f.guide = 0;
     f.cam = make_camera(*cd, p->width, p->height);
     carve_frame(arena, f.total, f);
     MRH_TRY(fill_u32(ctx, f.counts, 0u, COUNTS_WORDS));
@@ -720,9 +862,15 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     sa.part_begin = 24;
     MRH_TRY(radix_sort(ctx, sa));
     const Materials mats{s->lambert, s->metal, s->dielectric, s->emissive};
-    MRH_TRY(launch(ctx, "shade", dim3(grid), dim3(256), 0, k_shade, in, outb, f.hit, (const uint32_t*)f.order,
-                   f.counts, bounce, (uint32_t)(bounce + 1 == f.max_depth), f.rr_start, (uint32_t)f.spp,
-                   f.sample_base, mats, lv, image));
+    auto shade = [&](auto kernel) {
+        return launch(ctx, "shade", dim3(grid), dim3(256), 0, kernel, in, outb, f.hit, (const uint32_t*)f.order,
+                      f.counts, bounce, (uint32_t)(bounce + 1 == f.max_depth), f.rr_start, (uint32_t)f.spp,
+                      f.sample_base, mats, lv, image, f.ga);
+    };
+    if (f.learn && f.guide) MRH_TRY(shade(k_shade<true, true>));
+    else if (f.learn) MRH_TRY(shade(k_shade<true, false>));
+    else if (f.guide) MRH_TRY(shade(k_shade<false, true>));
+    else MRH_TRY(shade(k_shade<false, false>));
     f.bounce++;
     return MRH_OK;
 }
@@ -783,6 +931,125 @@ int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const 
     return rc;
 }
 
+// WFPGTracer::Render / Finalize (WFPGTracer.cu:1590-1790) over the structure
+// of this library: `iterations` frames of the same view, each with
+// samples^2 spp and its own sample range.  Every frame but the last records
+// its path vertices; after it the octree is rebuilt over the recorded
+// vertices' cells, their radiance splatted and filtered, and the next frame
+// samples directions from the one-sample mixture of that guide and the BxDF.
+// The image is the mean over all frames.
+int32_t render_guided(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const mrh_render_params* p,
+                      const mrh_guide_params* gp, mrh_svo* svo, float* d_image, mrh_render_stats* stats)
+{
+    uint64_t total = 0;
+    MRH_TRY(validate_params(ctx, s, p, &total));
+    if (gp->iterations == 0 || gp->iterations > 64)
+        return fail(ctx, MRH_ERR_INVALID, "render_guided: iterations must be in [1, 64]");
+    if (gp->svo_depth == 0 || gp->svo_depth > 10)
+        return fail(ctx, MRH_ERR_INVALID, "render_guided: svo_depth must be in [1, 10]");
+    if (!(gp->guide_ratio >= 0.0f && gp->guide_ratio <= 1.0f))
+        return fail(ctx, MRH_ERR_INVALID, "render_guided: guide_ratio must be in [0, 1]");
+    if ((uint64_t)gp->iterations * total >= (1ull << 32))
+        return fail(ctx, MRH_ERR_INVALID, "render_guided: iterations x paths exceed 2^32 samples");
+    const uint64_t n_rec = total * p->max_depth;
+    if (n_rec >= (1ull << 32))
+        return fail(ctx, MRH_ERR_INVALID, "render_guided: %llu path vertices exceed 2^32",
+                    (unsigned long long)n_rec);
+    void* arena = nullptr;
+    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(total), &arena));
+    const size_t img_count = (size_t)p->width * p->height * 3;
+    MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
+
+    // cube of the voxel grid: scene bounds padded by one voxel on each side
+    const mrh_lbvh* b = s->bvh;
+    float span = 0.0f;
+    for (int a = 0; a < 3; a++) span = fmaxf(span, b->hi[a] - b->lo[a]);
+    if (!(span > 0.0f)) span = 1.0f;
+    const float cells = (float)(1u << gp->svo_depth);
+    const float pad = span / (cells - 2.0f > 1.0f ? cells - 2.0f : 1.0f);
+    const float cube[4] = {b->lo[0] - pad, b->lo[1] - pad, b->lo[2] - pad, span + 2.0f * pad};
+
+    mrh_svo* own = nullptr;
+    if (!svo) {
+        MRH_TRY(mrh_svo_create(ctx, &own));
+        svo = own;
+    }
+    // records [n_rec] x (code, bin, tr, tg, tb, rad) + compacted (code, bin, rad)
+    void* recmem = nullptr;
+    int32_t rc = MRH_OK;
+    const bool learns = gp->iterations > 1;
+    if (learns) rc = dmalloc(ctx, &recmem, (size_t)n_rec * 36);
+    RecBuf rec = {};
+    uint32_t *c_codes = nullptr, *c_bins = nullptr;
+    float* c_rad = nullptr;
+    if (recmem) {
+        char* q = (char*)recmem;
+        rec.code = (uint32_t*)q; q += n_rec * 4;
+        rec.bin = (uint32_t*)q; q += n_rec * 4;
+        rec.tr = (float*)q; q += n_rec * 4;
+        rec.tg = (float*)q; q += n_rec * 4;
+        rec.tb = (float*)q; q += n_rec * 4;
+        rec.rad = (float*)q; q += n_rec * 4;
+        c_codes = (uint32_t*)q; q += n_rec * 4;
+        c_bins = (uint32_t*)q; q += n_rec * 4;
+        c_rad = (float*)q;
+        rec.max_depth = p->max_depth;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == MRH_OK && hipEventCreate(&e0) != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "event create");
+    if (rc == MRH_OK && hipEventCreate(&e1) != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "event create");
+    if (rc == MRH_OK) (void)hipEventRecord(e0, ctx->stream);
+    mrh_render_stats local;
+    memset(&local, 0, sizeof(local));
+    bool have_guide = false;
+    for (uint32_t it = 0; it < gp->iterations && rc == MRH_OK; it++) {
+        const bool learn = learns && it + 1 < gp->iterations;
+        if (learn) rc = fill_u32(ctx, rec.code, SVO_NO_NODE, n_rec);
+        Frame f;
+        if (rc == MRH_OK) rc = frame_begin(ctx, cd, p, arena, (uint32_t)(it * total), f);
+        if (rc != MRH_OK) break;
+        f.learn = learn;
+        f.guide = have_guide;
+        f.ga.rec = rec;
+        f.ga.lox = cube[0];
+        f.ga.loy = cube[1];
+        f.ga.loz = cube[2];
+        f.ga.inv_cell = cells / cube[3];
+        f.ga.depth = gp->svo_depth;
+        f.ga.ratio = gp->guide_ratio;
+        f.ga.min_samples = gp->min_samples;
+        if (have_guide) f.ga.svo = svo_view(svo);
+        while (rc == MRH_OK && f.bounce < f.max_depth) rc = frame_bounce(ctx, s, f, d_image);
+        if (rc == MRH_OK) rc = frame_counts(ctx, f, &local);
+        if (rc != MRH_OK || !learn) continue;
+        uint32_t kept = 0;
+        rc = svo_compact_records(ctx, rec.code, rec.bin, rec.rad, (uint32_t)n_rec, c_codes, c_bins, c_rad, &kept);
+        if (rc != MRH_OK || kept == 0) continue;   // nothing learned: keep the previous guide
+        rc = svo_build(ctx, svo, c_codes, kept, cube, gp->svo_depth);
+        if (rc == MRH_OK) rc = svo_splat(ctx, svo, c_codes, c_bins, c_rad, kept);
+        if (rc == MRH_OK) rc = svo_filter(ctx, svo);
+        if (rc == MRH_OK) have_guide = true;
+    }
+    if (rc == MRH_OK)
+        rc = launch(ctx, "finalize", dim3(stride_grid(ctx, img_count, 256)), dim3(256), 0, k_scale, d_image,
+                    img_count, 1.0f / ((float)((uint64_t)p->samples * p->samples) * (float)gp->iterations));
+    if (rc == MRH_OK) {
+        (void)hipEventRecord(e1, ctx->stream);
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "render_guided: %s", hipGetErrorString(e));
+        float ms = 0.0f;
+        if (rc == MRH_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+        local.render_ms = ms;
+        if (rc == MRH_OK && stats) *stats = local;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(recmem);
+    if (own) svo_free(own);
+    return rc;
+}
+
 __global__ void k_mean(const float* __restrict__ acc, float* __restrict__ out, size_t count, float s)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
@@ -830,6 +1097,17 @@ int32_t mrh_render(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* ca
     if (!ctx || !scene || !cam || !params || !d_image) return MRH_ERR_INVALID;
     if (scene->ctx != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_render: scene belongs to another context");
     return render(ctx, scene, cam, params, d_image, stats);
+}
+
+int32_t mrh_render_guided(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                          const mrh_render_params* params, const mrh_guide_params* guide, mrh_svo* svo,
+                          float* d_image, mrh_render_stats* stats)
+{
+    if (!ctx || !scene || !cam || !params || !guide || !d_image) return MRH_ERR_INVALID;
+    if (scene->ctx != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_render_guided: scene belongs to another context");
+    if (svo && svo_ctx(svo) != ctx)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render_guided: svo belongs to another context");
+    return render_guided(ctx, scene, cam, params, guide, svo, d_image, stats);
 }
 
 int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes)

@@ -63,6 +63,18 @@ class TuneResult(C.Structure):
                 ("trial_lds", _U32 * 32), ("trial_bpc", _U32 * 32), ("trial_ms", C.c_double * 32)]
 
 
+class SvoView(C.Structure):
+    _fields_ = [("depth", _U32), ("nodes", _U32), ("level_count", _U32 * 11),
+                ("level_offset", _U32 * 12), ("cube", _F * 4), ("d_codes", _P),
+                ("d_first_child", _P), ("d_child_mask", _P), ("d_parent", _P), ("d_sum", _P),
+                ("d_count", _P), ("d_radiance", _P), ("d_node_count", _P)]
+
+
+class GuideParams(C.Structure):
+    _fields_ = [("iterations", _U32), ("svo_depth", _U32), ("guide_ratio", _F),
+                ("min_samples", _U32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
                 ("lambert", _P), ("n_lambert", _U32), ("metal", _P), ("n_metal", _U32),
@@ -116,6 +128,18 @@ _SIGS = {
     "mrh_tracer_reset_image": (_I32, [_P]),
     "mrh_tracer_stats": (_I32, [_P, C.POINTER(RenderStats), C.POINTER(_U32)]),
     "mrh_tracer_destroy": (_I32, [_P]),
+    "mrh_svo_create": (_I32, [_P, C.POINTER(_P)]),
+    "mrh_svo_destroy": (_I32, [_P]),
+    "mrh_svo_codes": (_I32, [_P, _P, _U32, _U32, _P, _U32, _P]),
+    "mrh_svo_build": (_I32, [_P, _P, _P, _U32, _P, _U32]),
+    "mrh_svo_splat": (_I32, [_P, _P, _P, _P, _P, _U32]),
+    "mrh_svo_filter": (_I32, [_P, _P]),
+    "mrh_svo_lookup": (_I32, [_P, _P, _P, _U32, _U32, _P]),
+    "mrh_svo_sample": (_I32, [_P, _P, _P, _P, _P, _U32, _P, _P]),
+    "mrh_svo_pdf": (_I32, [_P, _P, _P, _P, _P, _U32, _P]),
+    "mrh_svo_view_get": (_I32, [_P, C.POINTER(SvoView)]),
+    "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
+                                 C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
     "mrh_gen_random_tris": (_I32, [_U32, _U32, _P]),
     "mrh_gen_random_rays": (_I32, [_U32, _U32, _P]),
     "mrh_gen_cornell": (_I32, [_U32, C.POINTER(SceneBuffers)]),
@@ -358,6 +382,19 @@ class Scene:
                                         ptr(d_image), C.byref(st)))
         return st
 
+    def render_guided(self, d_image, width, height, samples, max_depth, rr_start, seed=0,
+                      iterations=4, svo_depth=10, guide_ratio=0.5, min_samples=512, svo=None,
+                      camera=None) -> RenderStats:
+        """Guided render (mrh_render_guided); `svo` (an Svo) keeps the last guide."""
+        cam = camera_desc(camera or self.camera)
+        p = RenderParams(width, height, samples, max_depth, rr_start, 0, seed, 0)
+        g = GuideParams(iterations, svo_depth, guide_ratio, min_samples)
+        st = RenderStats()
+        self.ctx.check(lib().mrh_render_guided(self.ctx.h, self.h, C.byref(cam), C.byref(p), C.byref(g),
+                                               svo.h if svo is not None else None, ptr(d_image),
+                                               C.byref(st)))
+        return st
+
     def autotune(self, d_image, width, height, samples, max_depth, rr_start, seed=0, reps=2,
                  camera=None) -> TuneResult:
         cam = camera_desc(camera or self.camera)
@@ -370,6 +407,56 @@ class Scene:
     def close(self) -> None:
         if self.h:
             lib().mrh_scene_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Svo:
+    """Guide octree (mrh_svo) on the context's device."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = _P()
+        ctx.check(lib().mrh_svo_create(ctx.h, C.byref(self.h)))
+
+    def codes(self, d_points, n: int, pitch: int, cube, depth: int, d_codes) -> None:
+        c = (_F * 4)(*[float(x) for x in cube])
+        self.ctx.check(lib().mrh_svo_codes(self.ctx.h, ptr(d_points), n, pitch, c, depth, ptr(d_codes)))
+
+    def build(self, d_codes, n: int, cube, depth: int) -> None:
+        c = (_F * 4)(*[float(x) for x in cube])
+        self.ctx.check(lib().mrh_svo_build(self.ctx.h, self.h, ptr(d_codes), n, c, depth))
+
+    def splat(self, d_codes, d_bins, d_values, n: int) -> None:
+        self.ctx.check(lib().mrh_svo_splat(self.ctx.h, self.h, ptr(d_codes), ptr(d_bins), ptr(d_values), n))
+
+    def filter(self) -> None:
+        self.ctx.check(lib().mrh_svo_filter(self.ctx.h, self.h))
+
+    def lookup(self, d_codes, n: int, d_nodes, min_count: int = 0) -> None:
+        self.ctx.check(lib().mrh_svo_lookup(self.ctx.h, self.h, ptr(d_codes), n, min_count, ptr(d_nodes)))
+
+    def sample(self, d_nodes, d_normals, d_xi, n: int, d_dirs, d_pdf) -> None:
+        self.ctx.check(lib().mrh_svo_sample(self.ctx.h, self.h, ptr(d_nodes), ptr(d_normals), ptr(d_xi), n,
+                                            ptr(d_dirs), ptr(d_pdf)))
+
+    def pdf(self, d_nodes, d_normals, d_dirs, n: int, d_pdf) -> None:
+        self.ctx.check(lib().mrh_svo_pdf(self.ctx.h, self.h, ptr(d_nodes), ptr(d_normals), ptr(d_dirs), n,
+                                         ptr(d_pdf)))
+
+    def view(self) -> SvoView:
+        v = SvoView()
+        self.ctx.check(lib().mrh_svo_view_get(self.h, C.byref(v)))
+        return v
+
+    def close(self) -> None:
+        if self.h:
+            lib().mrh_svo_destroy(self.h)
             self.h = _P()
 
     def __del__(self):

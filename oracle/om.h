@@ -165,6 +165,106 @@ static inline void om_sincos_2pi(float u, float* s, float* c)
     }
 }
 
+/* ---------------- guide direction map and voxel grid -------------------
+ * Equal-area concentric octahedral map (Clarberg 2008; the reference's
+ * RayLib/CoordinateConversion.h:112-180 uses the same map for its radiance
+ * fields), 8x8 bins, and 30-bit Morton voxel codes of a cube grid. */
+#define OM_GUIDE_RES  8u
+#define OM_GUIDE_BINS 64u
+#define OM_SVO_NO_NODE 0xFFFFFFFFu
+#define OM_GUIDE_EPS 1.0e-4f
+#define OM_GUIDE_FIX_ONE 65536.0f
+#define OM_GUIDE_FIX_MAX 32767.0f
+
+uint32_t om_morton_expand10(uint32_t v);
+
+/* 48.16 fixed point of a splatted radiance sample, clamped to [0, 32767] */
+static inline uint64_t om_guide_fix(float v)
+{
+    return (v > 0.0f) ? (uint64_t)(fminf(v, OM_GUIDE_FIX_MAX) * OM_GUIDE_FIX_ONE) : 0ull;
+}
+
+static inline om_v3 om_octa_to_dir_v(float u, float v)
+{
+    float a = 2.0f * u - 1.0f, b = 2.0f * v - 1.0f;
+    float ua = fabsf(a), va = fabsf(b);
+    float d = 1.0f - (ua + va);
+    float r = 1.0f - fabsf(d);
+    float phin = (r == 0.0f) ? 0.0f : ((va - ua) / r + 1.0f) * 0.125f;
+    float z = copysignf(1.0f - r * r, d);
+    float s, c;
+    om_sincos_2pi(phin, &s, &c);
+    float k = r * sqrtf(fmaxf(0.0f, 2.0f - r * r));
+    return v3(copysignf(k * c, a), copysignf(k * s, b), z);
+}
+
+/* atan(t), t in [0,1]: two half-angle steps t <- t / (1 + sqrt(1 + t^2))
+ * (each halves the angle) bring t below tan(pi/16); Taylor series to t^13. */
+static inline float om_atan_unit_f(float t)
+{
+    t = t / (1.0f + sqrtf(1.0f + t * t));
+    t = t / (1.0f + sqrtf(1.0f + t * t));
+    float t2 = t * t;
+    float p = 0.07692307692307693f;
+    p = p * t2 - 0.09090909090909091f;
+    p = p * t2 + 0.1111111111111111f;
+    p = p * t2 - 0.14285714285714285f;
+    p = p * t2 + 0.2f;
+    p = p * t2 - 0.3333333333333333f;
+    p = p * t2 + 1.0f;
+    return 4.0f * (t * p);
+}
+
+/* atan2(y, x) / (pi/2) for x, y >= 0 */
+static inline float om_quadrant_angle(float y, float x)
+{
+    if (y == 0.0f && x == 0.0f) return 0.0f;
+    if (y <= x) return om_atan_unit_f(y / x) * 0.63661977236758134f;
+    return 1.0f - om_atan_unit_f(x / y) * 0.63661977236758134f;
+}
+
+static inline void om_dir_to_octa_v(om_v3 w, float* u, float* v)
+{
+    float r = sqrtf(fmaxf(0.0f, 1.0f - fabsf(w.z)));
+    float t = om_quadrant_angle(fabsf(w.y), fabsf(w.x));
+    float b = r * t, a = r - b;
+    if (w.z < 0.0f) {
+        float a2 = 1.0f - b, b2 = 1.0f - a;
+        a = a2;
+        b = b2;
+    }
+    *u = (copysignf(a, w.x) + 1.0f) * 0.5f;
+    *v = (copysignf(b, w.y) + 1.0f) * 0.5f;
+}
+
+static inline uint32_t om_dir_to_bin_v(om_v3 w)
+{
+    float u, v;
+    om_dir_to_octa_v(w, &u, &v);
+    uint32_t ix = (uint32_t)(u * (float)OM_GUIDE_RES), iy = (uint32_t)(v * (float)OM_GUIDE_RES);
+    ix = ix > OM_GUIDE_RES - 1 ? OM_GUIDE_RES - 1 : ix;
+    iy = iy > OM_GUIDE_RES - 1 ? OM_GUIDE_RES - 1 : iy;
+    return iy * OM_GUIDE_RES + ix;
+}
+
+/* Morton code of the cell holding p at `depth` of the grid with origin lo and
+ * inv_cell cells per unit; points outside clamp to the border cells. */
+static inline uint32_t om_svo_code_v(om_v3 p, float lox, float loy, float loz, float inv_cell,
+                                     uint32_t depth)
+{
+    uint32_t qmax = (1u << depth) - 1u;
+    float f[3] = {(p.x - lox) * inv_cell, (p.y - loy) * inv_cell, (p.z - loz) * inv_cell};
+    uint32_t q[3];
+    for (int a = 0; a < 3; a++) {
+        float g = floorf(f[a]);
+        q[a] = (g <= 0.0f) ? 0u : (g >= (float)qmax ? qmax : (uint32_t)g);
+    }
+    return om_morton_expand10(q[0]) | (om_morton_expand10(q[1]) << 1) | (om_morton_expand10(q[2]) << 2);
+}
+
+/* RayLib/ColorConversion.h:15-22 (its coefficients, as written there) */
+static inline float om_luminance(om_v3 c) { return (0.2120f * c.x + 0.7150f * c.y) + 0.0722f * c.z; }
+
 static inline uint32_t om_f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 static inline float om_u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 
@@ -175,5 +275,12 @@ uint32_t om_trace_one(const float* tris, uint32_t n, const uint32_t* sorted_idx,
                       uint32_t root, const float o[3], const float d[3], float tmin,
                       float* tmax_io, float bary_out[2], int any_hit,
                       uint32_t* visits, uint32_t* tests);
+
+/* guide evaluation shared by the guided path tracer (om_guide.c) */
+struct om_svo;
+float    om_guide_total_v(const struct om_svo* s, uint32_t node, om_v3 n);
+om_v3    om_guide_sample_v(const struct om_svo* s, uint32_t node, om_v3 n, float total, float xi0, float xi1,
+                           float xi2, float* pdf);
+float    om_guide_pdf_v(const struct om_svo* s, uint32_t node, om_v3 n, float total, om_v3 d);
 
 #endif

@@ -425,6 +425,77 @@ static void dielectric_sample(const om_surface* s, om_v3 wi, const float* m, uin
     }
 }
 
+/* Lambert f*cos and its sampler's own pdf (uniform hemisphere, 1/(2 pi) on
+ * the normal's side); SimpleMaterialsKC.cuh:80-101 for the evaluation. */
+static om_v3 lambert_eval(const om_surface* s, om_v3 albedo, om_v3 dir, float* pdf)
+{
+    om_v3 normal = q_rotate(q_conj(s->w2t), v3(0.0f, 0.0f, 1.0f));
+    float ndl = v3_dot(normal, dir);
+    *pdf = (ndl > 0.0f) ? OM_INV_PI_F * 0.5f : 0.0f;
+    return v3_scale(v3_scale(albedo, fmaxf(ndl, 0.0f)), OM_INV_PI_F);
+}
+
+/* MetalMatFuncs::Evaluate / Pdf (MetalGlassMaterialsKC.cuh:152-220) */
+static om_v3 metal_eval(const om_surface* s, om_v3 wi, const float* m, om_v3 dir, float* pdf)
+{
+    float rough = m[0];
+    float alpha = rough * rough;
+    om_v3 L = q_rotate(s->w2t, dir);
+    om_v3 V = q_rotate(s->w2t, wi);
+    om_v3 H = v3_normalize(v3_add(L, V));
+    float ndh = fmaxf(0.0f, H.z);
+    float vdh = fmaxf(0.0f, v3_dot(V, H));
+    float ldh = fmaxf(0.0f, v3_dot(L, H));
+    float ndv = fmaxf(0.0f, V.z);
+    float pspec = vndf_pdf(V, H, alpha);
+    float D = om_dggx(ndh, alpha);
+    pspec = (isnan(D) || isinf(D)) ? 0.0f : pspec;
+    *pdf = (vdh == 0.0f) ? 0.0f : pspec / (4.0f * vdh);
+    D = (isnan(D) || isinf(D)) ? 0.0f : D;
+    float G = gsmith_corr(V, L, alpha);
+    G = (ldh == 0.0f) ? 0.0f : G;
+    G = (vdh == 0.0f) ? 0.0f : G;
+    om_v3 F = v3(fresnel_conductor1(vdh, m[1], m[4]),
+                 fresnel_conductor1(vdh, m[2], m[5]),
+                 fresnel_conductor1(vdh, m[3], m[6]));
+    F = v3_mul(F, v3(m[7], m[8], m[9]));
+    om_v3 spec = v3(((D * F.x) * G) * 0.25f, ((D * F.y) * G) * 0.25f, ((D * F.z) * G) * 0.25f);
+    spec = v3_divs(spec, ndv);
+    return (ndv == 0.0f) ? v3(0.0f, 0.0f, 0.0f) : spec;
+}
+
+/* Guide state of one frame of the guided tracer */
+typedef struct {
+    const om_svo* svo;       /* guide to sample, NULL = plain path tracing */
+    float ratio;             /* probability of sampling the guide */
+    uint32_t min_samples;    /* guide lookup threshold */
+    int learn;               /* record path vertices */
+    float lox, loy, loz, inv_cell;
+    uint32_t depth;
+} om_guide_frame;
+
+/* path-vertex records of one thread, appended as paths finish */
+typedef struct {
+    uint32_t* code;
+    uint32_t* bin;
+    float* rad;
+    size_t n, cap;
+} om_rec_list;
+
+static void rec_push(om_rec_list* r, uint32_t code, uint32_t bin, float rad)
+{
+    if (r->n == r->cap) {
+        r->cap = r->cap ? r->cap * 2 : 4096;
+        r->code = (uint32_t*)realloc(r->code, r->cap * 4);
+        r->bin = (uint32_t*)realloc(r->bin, r->cap * 4);
+        r->rad = (float*)realloc(r->rad, r->cap * 4);
+    }
+    r->code[r->n] = code;
+    r->bin[r->n] = bin;
+    r->rad[r->n] = rad;
+    r->n++;
+}
+
 typedef struct {
     const om_scene* s;
     const om_camera* cam;
@@ -432,18 +503,32 @@ typedef struct {
     double* img;
     uint32_t tid;
     uint64_t segments;
+    uint32_t sample_base;
+    const om_guide_frame* g;
+    om_rec_list rec;
 } om_job;
 
+/* One path, depth first.  With g->svo set, non-specular Lambert / metal
+ * vertices draw from the one-sample mixture of guide and BxDF
+ * (WFPGTracerKC.cuh:1480-1553: pdf = r p_guide + (1 - r) p_bxdf); with
+ * g->learn, every continuing vertex leaves a record (cell, direction bin,
+ * throughput) whose radiance the emitter hit fills in (PathNode.cuh:152-190). */
 static void trace_path(const om_scene* sc, const om_camera* cam, const om_render_params* p,
-                       uint32_t sample_id, double* pixel, uint64_t* segments)
+                       uint32_t sample_base, uint32_t sample_id, const om_guide_frame* g,
+                       double* pixel, uint64_t* segments, om_rec_list* recs)
 {
     om_rng rng;
-    om_path_rng(&rng, p->seed, sample_id);
+    om_path_rng(&rng, p->seed, sample_base + sample_id);
     om_v3 o, d;
     camera_ray(cam, p->width, p->height, p->samples, sample_id, &rng, &o, &d);
     om_v3 T = v3(1.0f, 1.0f, 1.0f);
     uint32_t depth = 1, medium = 0;
     float tmin = cam->near_far[0], tmax = cam->near_far[1];
+    const int learn = g && g->learn;
+    uint32_t rc[64], rb[64];
+    om_v3 rt[64];
+    float rr[64];
+    uint32_t nrec = 0;
 
     for (uint32_t bounce = 0; bounce < p->max_depth; bounce++) {
         (*segments)++;
@@ -451,7 +536,7 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         float b[2];
         uint32_t prim = om_trace_one(sc->tris, sc->n, sc->sorted_idx, sc->left, sc->right,
                                      sc->node_aabb, sc->root, oo, dd, tmin, &tmax, b, 0, NULL, NULL);
-        if (prim == OM_INVALID) return;
+        if (prim == OM_INVALID) break;
         uint32_t key = sc->prim_key[prim];
         uint32_t batch = key >> 24, id = key & 0xFFFFFFu;
         if (batch == OM_BATCH_EMISSIVE) {
@@ -460,14 +545,69 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
             pixel[0] += (double)(le[0] * T.x);
             pixel[1] += (double)(le[1] * T.y);
             pixel[2] += (double)(le[2] * T.z);
-            return;
+            if (learn) {
+                om_v3 c = v3_mul(v3(le[0], le[1], le[2]), T);
+                for (uint32_t k = 0; k < nrec; k++) {
+                    om_v3 f = rt[k];
+                    om_v3 L = v3(f.x == 0.0f ? 0.0f : c.x / f.x, f.y == 0.0f ? 0.0f : c.y / f.y,
+                                 f.z == 0.0f ? 0.0f : c.z / f.z);
+                    rr[k] = om_luminance(L);
+                }
+            }
+            break;
         }
-        if (batch == OM_BATCH_NULL) return;
+        if (batch == OM_BATCH_NULL) break;
+        if (bounce + 1 == p->max_depth) break;   /* the last segment only collects emission */
         om_surface s = make_surface(sc->tris + (size_t)prim * 9, b[0], b[1], d);
         om_v3 wi = v3_neg(v3_normalize(d));
         om_bsdf_out out;
         out.medium = medium;
-        if (batch == OM_BATCH_LAMBERT) {
+        int guided = 0;
+        uint32_t gnode = 0;
+        float gtotal = 0.0f;
+        om_v3 ns = v3(0.0f, 0.0f, 1.0f);
+        if (g && g->svo && (batch == OM_BATCH_LAMBERT ||
+                            (batch == OM_BATCH_METAL && (1.0f - sc->metal[(size_t)id * 10]) < 0.99f))) {
+            const om_svo* sv = g->svo;
+            ns = q_rotate(q_conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
+            float inv_cell = (float)(1u << sv->depth) / sv->cube[3];
+            gnode = om_svo_lookup_guide(sv, om_svo_code_v(s.pos, sv->cube[0], sv->cube[1], sv->cube[2],
+                                                          inv_cell, sv->depth), g->min_samples);
+            gtotal = om_guide_total_v(sv, gnode, ns);
+            guided = gtotal > 0.0f;
+        }
+        if (guided) {
+            float xi = om_uniform(&rng);
+            om_v3 dir, fcos;
+            float pg = 0.0f, pb = 0.0f;
+            if (xi < g->ratio) {
+                float u0 = om_uniform(&rng), u1 = om_uniform(&rng), u2 = om_uniform(&rng);
+                dir = om_guide_sample_v(g->svo, gnode, ns, gtotal, u0, u1, u2, &pg);
+                if (batch == OM_BATCH_LAMBERT) {
+                    const float* a = sc->lambert + (size_t)id * 3;
+                    fcos = lambert_eval(&s, v3(a[0], a[1], a[2]), dir, &pb);
+                } else {
+                    fcos = metal_eval(&s, wi, sc->metal + (size_t)id * 10, dir, &pb);
+                }
+            } else {
+                if (batch == OM_BATCH_LAMBERT) {
+                    const float* a = sc->lambert + (size_t)id * 3;
+                    lambert_sample(&s, v3(a[0], a[1], a[2]), &rng, &out);
+                } else {
+                    metal_sample(&s, wi, sc->metal + (size_t)id * 10, &rng, &out);
+                }
+                dir = out.d;
+                fcos = out.refl;
+                pb = out.pdf;
+                pg = om_guide_pdf_v(g->svo, gnode, ns, gtotal, dir);
+            }
+            out.d = dir;
+            out.o = nudge(s.pos, s.geo_n);
+            out.refl = fcos;
+            out.pdf = g->ratio * pg + (1.0f - g->ratio) * pb;
+            out.specular = 0;
+            out.medium = medium;
+        } else if (batch == OM_BATCH_LAMBERT) {
             const float* a = sc->lambert + (size_t)id * 3;
             lambert_sample(&s, v3(a[0], a[1], a[2]), &rng, &out);
         } else if (batch == OM_BATCH_METAL) {
@@ -487,7 +627,14 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
             if (om_uniform(&rng) >= pr) terminate = 1;
             else nT = v3_scale(nT, 1.0f / pr);
         }
-        if ((terminate && !out.specular) || v3_is_zero(nT)) return;
+        if ((terminate && !out.specular) || v3_is_zero(nT)) break;
+        if (learn) {
+            rc[nrec] = om_svo_code_v(s.pos, g->lox, g->loy, g->loz, g->inv_cell, g->depth);
+            rb[nrec] = om_dir_to_bin_v(v3_normalize(out.d));
+            rt[nrec] = nT;
+            rr[nrec] = 0.0f;
+            nrec++;
+        }
         T = nT;
         o = out.o;
         d = out.d;
@@ -496,6 +643,7 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         tmin = 0.0f;
         tmax = FLT_MAX;
     }
+    for (uint32_t k = 0; k < nrec; k++) rec_push(recs, rc[k], rb[k], rr[k]);
 }
 
 static void* render_worker(void* arg)
@@ -512,19 +660,24 @@ static void* render_worker(void* arg)
             uint32_t pix = y * p->width + x;
             double* px = job->img + (size_t)pix * 3;
             for (uint32_t s = 0; s < spp; s++)
-                trace_path(job->s, job->cam, p, pix * spp + s, px, &job->segments);
+                trace_path(job->s, job->cam, p, job->sample_base, pix * spp + s, job->g, px, &job->segments,
+                           &job->rec);
         }
     }
     return NULL;
 }
 
-uint64_t om_render(const om_scene* s, const om_camera* cam, const om_render_params* p, double* img)
+/* one frame over the selected rows; merges the threads' records into *recs */
+static uint64_t render_frame(const om_scene* s, const om_camera* cam, const om_render_params* p,
+                             uint32_t sample_base, const om_guide_frame* g, double* img, om_rec_list* recs)
 {
     uint32_t nthreads = p->threads ? p->threads : 1u;
     om_job* jobs = (om_job*)calloc(nthreads, sizeof(om_job));
     pthread_t* th = (pthread_t*)calloc(nthreads, sizeof(pthread_t));
     for (uint32_t t = 0; t < nthreads; t++) {
         jobs[t].s = s; jobs[t].cam = cam; jobs[t].p = p; jobs[t].img = img; jobs[t].tid = t;
+        jobs[t].sample_base = sample_base;
+        jobs[t].g = g;
         if (nthreads > 1) pthread_create(&th[t], NULL, render_worker, &jobs[t]);
     }
     if (nthreads == 1) render_worker(&jobs[0]);
@@ -532,9 +685,87 @@ uint64_t om_render(const om_scene* s, const om_camera* cam, const om_render_para
     for (uint32_t t = 0; t < nthreads; t++) {
         if (nthreads > 1) pthread_join(th[t], NULL);
         seg += jobs[t].segments;
+        om_rec_list* r = &jobs[t].rec;
+        for (size_t i = 0; i < r->n; i++)
+            if (recs) rec_push(recs, r->code[i], r->bin[i], r->rad[i]);
+        free(r->code);
+        free(r->bin);
+        free(r->rad);
     }
     free(jobs);
     free(th);
+    return seg;
+}
+
+uint64_t om_render(const om_scene* s, const om_camera* cam, const om_render_params* p, double* img)
+{
+    return render_frame(s, cam, p, 0u, NULL, img, NULL);
+}
+
+/* WFPGTracer::Render / Finalize (WFPGTracer.cu:1590-1790) restated over the
+ * structure of the HIP path: `iterations` frames of samples^2 spp, frame i
+ * keyed by sample ids i*W*H*spp + local id; every frame but the last records
+ * its vertices; after it the octree is rebuilt over those cells (cube = scene
+ * bounds padded by one voxel), splatted and filtered, and guides the next
+ * frame.  Renders every row (the guide depends on all paths).  img receives
+ * the sum of all frames' contributions. */
+uint64_t om_render_guided(const om_scene* s, const om_camera* cam, const om_render_params* p,
+                          const om_guide_params* gp, double* img, om_svo* last)
+{
+    om_render_params q = *p;
+    q.row_step = 1;
+    q.row_offset = 0;
+    /* scene bounds: min / max over all triangle vertices */
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < s->n; i++)
+        for (int v = 0; v < 3; v++)
+            for (int a = 0; a < 3; a++) {
+                float x = s->tris[(size_t)i * 9 + 3 * v + a];
+                lo[a] = fminf(lo[a], x);
+                hi[a] = fmaxf(hi[a], x);
+            }
+    float span = 0.0f;
+    for (int a = 0; a < 3; a++) span = fmaxf(span, hi[a] - lo[a]);
+    if (!(span > 0.0f)) span = 1.0f;
+    float cells = (float)(1u << gp->svo_depth);
+    float pad = span / (cells - 2.0f > 1.0f ? cells - 2.0f : 1.0f);
+    float cube[4] = {lo[0] - pad, lo[1] - pad, lo[2] - pad, span + 2.0f * pad};
+    uint32_t total = p->width * p->height * p->samples * p->samples;
+
+    om_svo svo;
+    memset(&svo, 0, sizeof(svo));
+    int have = 0;
+    uint64_t seg = 0;
+    for (uint32_t it = 0; it < gp->iterations; it++) {
+        om_guide_frame g;
+        memset(&g, 0, sizeof(g));
+        g.svo = have ? &svo : NULL;
+        g.ratio = gp->guide_ratio;
+        g.min_samples = gp->min_samples;
+        g.learn = gp->iterations > 1 && it + 1 < gp->iterations;
+        g.lox = cube[0];
+        g.loy = cube[1];
+        g.loz = cube[2];
+        g.inv_cell = cells / cube[3];
+        g.depth = gp->svo_depth;
+        om_rec_list recs;
+        memset(&recs, 0, sizeof(recs));
+        seg += render_frame(s, cam, &q, it * total, &g, img, &recs);
+        if (g.learn && recs.n > 0) {
+            om_svo next;
+            om_svo_build(&next, recs.code, (uint32_t)recs.n, cube, gp->svo_depth);
+            om_svo_splat(&next, recs.code, recs.bin, recs.rad, (uint32_t)recs.n);
+            om_svo_filter(&next);
+            om_svo_free(&svo);
+            svo = next;
+            have = 1;
+        }
+        free(recs.code);
+        free(recs.bin);
+        free(recs.rad);
+    }
+    if (last) *last = svo;
+    else om_svo_free(&svo);
     return seg;
 }
 

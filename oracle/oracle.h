@@ -156,6 +156,47 @@ typedef struct {
 uint64_t om_render(const om_scene* s, const om_camera* cam, const om_render_params* p,
                    double* img);
 
+/* ---- Path guiding (AnisoSVO.cu:225-263, 657-1188, 1415-1483;
+ *      WFPGTracerKC.cuh:202-520, 1480-1766; WFPGTracer.cu:1590-1790) ---- */
+typedef struct om_svo {
+    uint32_t depth, nodes;
+    uint32_t level_count[11], level_offset[12];
+    float    cube[4];                 /* lo xyz, size */
+    uint32_t *codes, *first_child, *child_mask, *parent;
+    uint64_t *acc;                    /* [nodes][64] 48.16 fixed-point splat sums */
+    uint32_t *count;                  /* [nodes][64] */
+    float    *sum;                    /* [nodes][64] means before the blur (after filter) */
+    float    *radiance;               /* [nodes][64] filtered */
+    uint32_t *node_count;             /* [nodes] samples in the subtree (after filter) */
+    float    centers[64][3];
+} om_svo;
+/* 0 on success; codes are 30-bit cell codes at `depth` (1..10) */
+int      om_svo_build(om_svo* s, const uint32_t* codes, uint32_t n, const float cube[4], uint32_t depth);
+void     om_svo_free(om_svo* s);
+uint32_t om_svo_lookup(const om_svo* s, uint32_t code);
+uint32_t om_svo_lookup_guide(const om_svo* s, uint32_t code, uint32_t min_count);
+void     om_svo_splat(om_svo* s, const uint32_t* codes, const uint32_t* bins, const float* values,
+                      uint32_t n);
+void     om_svo_filter(om_svo* s);
+/* pts / dirs / normals / xi: n x 3 floats; uv: n x 2 */
+void     om_svo_codes(const float* pts, uint32_t n, const float cube[4], uint32_t depth, uint32_t* codes);
+void     om_dir_to_bins(const float* dirs, uint32_t n, uint32_t* bins);
+void     om_octa_to_dirs(const float* uv, uint32_t n, float* dirs);
+void     om_dir_to_octas(const float* dirs, uint32_t n, float* uv);
+float    om_atan_unit(float t);
+void     om_guide_sample(const om_svo* s, const uint32_t* nodes, const float* normals, const float* xi,
+                         uint32_t n, float* dirs, float* pdf);
+void     om_guide_pdf(const om_svo* s, const uint32_t* nodes, const float* normals, const float* dirs,
+                      uint32_t n, float* pdf);
+typedef struct {
+    uint32_t iterations, svo_depth;
+    float    guide_ratio;
+    uint32_t min_samples;             /* guide lookup stops above sparser nodes */
+} om_guide_params;
+/* All rows (row_step / row_offset ignored); img = sum over all frames; *last
+ * (may be NULL) receives the final guide (free with om_svo_free). */
+uint64_t om_render_guided(const om_scene* s, const om_camera* cam, const om_render_params* p,
+                          const om_guide_params* g, double* img, om_svo* last);
 /* Camera ray of one sample (for raygen parity tests): writes o, d, and the
  * path RNG state after the two jitter draws. */
 void om_camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t samples,

@@ -58,6 +58,18 @@ class OmRenderParams(C.Structure):
                 ("threads", _U32)]
 
 
+class OmSvo(C.Structure):
+    _fields_ = [("depth", _U32), ("nodes", _U32), ("level_count", _U32 * 11),
+                ("level_offset", _U32 * 12), ("cube", _F * 4),
+                ("codes", _P), ("first_child", _P), ("child_mask", _P), ("parent", _P),
+                ("acc", _P), ("count", _P), ("sum", _P), ("radiance", _P), ("node_count", _P),
+                ("centers", _F * 192)]
+
+
+class OmGuideParams(C.Structure):
+    _fields_ = [("iterations", _U32), ("svo_depth", _U32), ("guide_ratio", _F), ("min_samples", _U32)]
+
+
 _lib = None
 _ref = None
 
@@ -121,6 +133,27 @@ def lib():
                                 C.POINTER(OmRenderParams), _P]
         L.om_camera_ray.argtypes = [C.POINTER(OmCamera), _U32, _U32, _U32, _U32, _U32,
                                     _P, _P, C.POINTER(C.c_uint64)]
+        L.om_svo_build.restype = C.c_int
+        L.om_svo_build.argtypes = [C.POINTER(OmSvo), _P, _U32, _P, _U32]
+        L.om_svo_free.argtypes = [C.POINTER(OmSvo)]
+        L.om_svo_lookup.restype = _U32
+        L.om_svo_lookup.argtypes = [C.POINTER(OmSvo), _U32]
+        L.om_svo_lookup_guide.restype = _U32
+        L.om_svo_lookup_guide.argtypes = [C.POINTER(OmSvo), _U32, _U32]
+        L.om_svo_splat.argtypes = [C.POINTER(OmSvo), _P, _P, _P, _U32]
+        L.om_svo_filter.argtypes = [C.POINTER(OmSvo)]
+        L.om_svo_codes.argtypes = [_P, _U32, _P, _U32, _P]
+        L.om_dir_to_bins.argtypes = [_P, _U32, _P]
+        L.om_octa_to_dirs.argtypes = [_P, _U32, _P]
+        L.om_dir_to_octas.argtypes = [_P, _U32, _P]
+        L.om_atan_unit.restype = _F
+        L.om_atan_unit.argtypes = [_F]
+        L.om_guide_sample.argtypes = [C.POINTER(OmSvo), _P, _P, _P, _U32, _P, _P]
+        L.om_guide_pdf.argtypes = [C.POINTER(OmSvo), _P, _P, _P, _U32, _P]
+        L.om_render_guided.restype = C.c_uint64
+        L.om_render_guided.argtypes = [C.POINTER(OmScene), C.POINTER(OmCamera),
+                                       C.POINTER(OmRenderParams), C.POINTER(OmGuideParams), _P,
+                                       C.POINTER(OmSvo)]
         _lib = L
     return _lib
 
@@ -271,11 +304,7 @@ def camera_ray(cam: OmCamera, w, h, samples, seed, sample_id):
     return o, d, st.value
 
 
-def render(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_depth, rr_start,
-           seed, row_step=1, row_offset=0, threads=1):
-    """scene: dict with tris (n,9), prim_key (n,), lambert (.,3), metal (.,10),
-    dielectric (.,4), emissive (.,3).  Returns (image float64 [h,w,3], segments)."""
-    keep = {}
+def _scene_struct(scene: dict, bvh: LBVH, keep: dict) -> OmScene:
 
     def arr(name, dtype, cols):
         a = scene.get(name)
@@ -298,8 +327,159 @@ def render(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_de
     sc.right = _ptr(bvh.right)
     sc.node_aabb = _ptr(bvh.node_aabb)
     sc.root = bvh.root
+    return sc
+
+
+def render(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_depth, rr_start,
+           seed, row_step=1, row_offset=0, threads=1):
+    """scene: dict with tris (n,9), prim_key (n,), lambert (.,3), metal (.,10),
+    dielectric (.,4), emissive (.,3).  Returns (image float64 [h,w,3], segments)."""
+    keep = {}
+    sc = _scene_struct(scene, bvh, keep)
     p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, row_step, row_offset,
                        threads)
     img = np.zeros((height, width, 3), dtype=np.float64)
     seg = lib().om_render(C.byref(sc), C.byref(cam), C.byref(p), _ptr(img))
     return img / float(samples * samples), int(seg)
+
+
+class Svo:
+    """The oracle's guide octree (om_svo), with its arrays copied out."""
+
+    def __init__(self, s: OmSvo):
+        self._s = s
+        self.depth = s.depth
+        self.nodes = s.nodes
+        self.cube = np.array(s.cube[:], dtype=np.float32)
+        self.level_count = np.array(s.level_count[:], dtype=np.uint32)
+        self.level_offset = np.array(s.level_offset[:], dtype=np.uint32)
+        self.refresh()
+
+    def refresh(self):
+        n = self.nodes
+
+        def get(ptr, ctype, count, dtype):
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,)).astype(dtype).copy()
+
+        self.codes = get(self._s.codes, C.c_uint32, n, np.uint32)
+        self.first_child = get(self._s.first_child, C.c_uint32, n, np.uint32)
+        self.child_mask = get(self._s.child_mask, C.c_uint32, n, np.uint32)
+        self.parent = get(self._s.parent, C.c_uint32, n, np.uint32)
+        self.acc = get(self._s.acc, C.c_uint64, n * 64, np.uint64).reshape(n, 64)
+        self.count = get(self._s.count, C.c_uint32, n * 64, np.uint32).reshape(n, 64)
+        self.sum = get(self._s.sum, C.c_float, n * 64, np.float32).reshape(n, 64)
+        self.radiance = get(self._s.radiance, C.c_float, n * 64, np.float32).reshape(n, 64)
+        self.node_count = get(self._s.node_count, C.c_uint32, n, np.uint32)
+        self.centers = np.array(self._s.centers[:], dtype=np.float32).reshape(64, 3)
+
+    @classmethod
+    def build(cls, codes: np.ndarray, cube, depth: int) -> "Svo":
+        codes = np.ascontiguousarray(codes, dtype=np.uint32)
+        cube = np.ascontiguousarray(cube, dtype=np.float32)
+        s = OmSvo()
+        rc = lib().om_svo_build(C.byref(s), _ptr(codes), len(codes), _ptr(cube), depth)
+        if rc != 0:
+            raise ValueError(f"om_svo_build failed ({rc})")
+        return cls(s)
+
+    def lookup(self, codes: np.ndarray) -> np.ndarray:
+        return np.array([lib().om_svo_lookup(C.byref(self._s), int(c)) for c in codes], dtype=np.uint32)
+
+    def lookup_guide(self, codes: np.ndarray, min_count: int) -> np.ndarray:
+        return np.array([lib().om_svo_lookup_guide(C.byref(self._s), int(c), min_count) for c in codes],
+                        dtype=np.uint32)
+
+    def splat(self, codes, bins, values):
+        codes = np.ascontiguousarray(codes, dtype=np.uint32)
+        bins = np.ascontiguousarray(bins, dtype=np.uint32)
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        lib().om_svo_splat(C.byref(self._s), _ptr(codes), _ptr(bins), _ptr(values), len(codes))
+        self.refresh()
+
+    def filter(self):
+        lib().om_svo_filter(C.byref(self._s))
+        self.refresh()
+
+    def sample(self, nodes, normals, xi):
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        normals = np.ascontiguousarray(normals, dtype=np.float32)
+        xi = np.ascontiguousarray(xi, dtype=np.float32)
+        n = len(nodes)
+        dirs = np.zeros((n, 3), dtype=np.float32)
+        pdf = np.zeros(n, dtype=np.float32)
+        lib().om_guide_sample(C.byref(self._s), _ptr(nodes), _ptr(normals), _ptr(xi), n, _ptr(dirs), _ptr(pdf))
+        return dirs, pdf
+
+    def pdf(self, nodes, normals, dirs):
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        normals = np.ascontiguousarray(normals, dtype=np.float32)
+        dirs = np.ascontiguousarray(dirs, dtype=np.float32)
+        n = len(nodes)
+        pdf = np.zeros(n, dtype=np.float32)
+        lib().om_guide_pdf(C.byref(self._s), _ptr(nodes), _ptr(normals), _ptr(dirs), n, _ptr(pdf))
+        return pdf
+
+    def close(self):
+        if self._s is not None:
+            lib().om_svo_free(C.byref(self._s))
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def svo_codes(pts: np.ndarray, cube, depth: int) -> np.ndarray:
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    cube = np.ascontiguousarray(cube, dtype=np.float32)
+    out = np.zeros(len(pts), dtype=np.uint32)
+    lib().om_svo_codes(_ptr(pts), len(pts), _ptr(cube), depth, _ptr(out))
+    return out
+
+
+def dir_to_bins(dirs: np.ndarray) -> np.ndarray:
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32)
+    out = np.zeros(len(dirs), dtype=np.uint32)
+    lib().om_dir_to_bins(_ptr(dirs), len(dirs), _ptr(out))
+    return out
+
+
+def octa_to_dirs(uv: np.ndarray) -> np.ndarray:
+    uv = np.ascontiguousarray(uv, dtype=np.float32)
+    out = np.zeros((len(uv), 3), dtype=np.float32)
+    lib().om_octa_to_dirs(_ptr(uv), len(uv), _ptr(out))
+    return out
+
+
+def dir_to_octas(dirs: np.ndarray) -> np.ndarray:
+    dirs = np.ascontiguousarray(dirs, dtype=np.float32)
+    out = np.zeros((len(dirs), 2), dtype=np.float32)
+    lib().om_dir_to_octas(_ptr(dirs), len(dirs), _ptr(out))
+    return out
+
+
+def atan_unit(t: float) -> float:
+    return float(lib().om_atan_unit(t))
+
+
+def render_guided(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_depth, rr_start,
+                  seed, iterations, svo_depth, guide_ratio=0.5, min_samples=0, threads=1, keep_svo=False):
+    """Guided render of every row.  Returns (image float64 [h,w,3] = mean over
+    all frames, segments, Svo of the last guide or None)."""
+    keep = {}
+    sc = _scene_struct(scene, bvh, keep)
+    p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, 1, 0, threads)
+    g = OmGuideParams(iterations, svo_depth, guide_ratio, min_samples)
+    img = np.zeros((height, width, 3), dtype=np.float64)
+    last = OmSvo()
+    seg = lib().om_render_guided(C.byref(sc), C.byref(cam), C.byref(p), C.byref(g), _ptr(img),
+                                 C.byref(last))
+    svo = None
+    if last.nodes:
+        if keep_svo:
+            svo = Svo(last)
+        else:
+            lib().om_svo_free(C.byref(last))
+    return img / float(samples * samples * iterations), int(seg), svo

@@ -74,6 +74,151 @@ def cpu_baseline(sc, target_s: float):
     }
 
 
+def guided_leg(N, ctx, scene, img, torch):
+    """The same scene through the guided tracer: 4 learn-then-render
+    iterations of 16 spp (64 spp in all), max depth 12, octree depth 10,
+    50/50 guide/BxDF mixture.  One warm-up render, one timed."""
+    it, S, depth, svo_depth = 4, 4, 12, 10
+    svo = N.Svo(ctx)
+    scene.render_guided(img.data_ptr(), W, H, S, depth, RR, SEED + 1, iterations=it, svo_depth=svo_depth,
+                        guide_ratio=0.5, min_samples=512, svo=svo)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = scene.render_guided(img.data_ptr(), W, H, S, depth, RR, SEED + 1, iterations=it,
+                             svo_depth=svo_depth, guide_ratio=0.5, min_samples=512, svo=svo)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    v = svo.view()
+    svo.close()
+    log(f"guided: {st.segments} segments in {dt * 1e3:.1f} ms, octree {v.nodes} nodes")
+    return {"workload": f"same scene, 1920x1080, {it} learn-then-render iterations x {S * S} spp "
+                        f"({it * S * S} spp), max depth {depth}, octree depth {svo_depth}, 50/50 mixture",
+            "Mrays_per_s": round(st.segments / dt / 1e6, 3), "ms": round(dt * 1e3, 3),
+            "segments": int(st.segments), "octree_nodes": int(v.nodes)}
+
+
+SERVE_RES = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
+SERVE_SPP = [1, 4, 16, 64]
+
+
+def serving_workload(n_req: int, rate: float, seed: int = 3):
+    """The request stream: Poisson arrivals, 8 sphere-grid scenes of
+    log-uniform size in [10K, 4M] triangles, resolution, spp and guiding
+    drawn per request."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(4e6), 8))
+    grids = [int(max(5, round(np.sqrt(x / 512.0)))) for x in sizes]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for i in range(n_req):
+        w, h = SERVE_RES[rng.integers(0, 4)]
+        spp = SERVE_SPP[rng.integers(0, 4)]
+        guided = bool(rng.random() < 0.5)
+        it = min(4, spp) if guided else 1
+        S = int(round(np.sqrt(spp // it)))
+        reqs.append({"t": float(arrivals[i]), "scene": int(rng.integers(0, 8)), "w": w, "h": h, "S": S,
+                     "it": it, "spp": spp, "seed": i})
+    return grids, reqs
+
+
+def serving_leg(N, ctx, torch, n_req: int, rate: float):
+    """C5: serial one-request-at-a-time launches vs the serving engine, the
+    latter both saturated (all requests queued at once) and in real time
+    with the Poisson arrivals.  Latency = completion - arrival."""
+    import time as _t
+    grids, reqs = serving_workload(n_req, rate)
+    scenes = []
+    for k, g in enumerate(grids):
+        sc = N.gen_sphere_grid(g, 16, 16, 100 + k)
+        scenes.append(N.Scene(ctx, sc))
+    tris = [g * g * 512 for g in grids]
+    log(f"serving: {n_req} requests, scenes {tris} triangles")
+    depth, rr = DEPTH, RR
+
+    def image(r):
+        return torch.empty(r["w"] * r["h"] * 3, dtype=torch.float32, device="cuda")
+
+    # serial: each request alone, back to back
+    service = np.zeros(n_req)
+    seg_serial = 0
+    buf = torch.empty(1920 * 1080 * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    t_all = _t.perf_counter()
+    for i, r in enumerate(reqs):
+        t0 = _t.perf_counter()
+        sc = scenes[r["scene"]]
+        if r["it"] > 1:
+            st = sc.render_guided(buf.data_ptr(), r["w"], r["h"], r["S"], depth, rr, r["seed"],
+                                  iterations=r["it"], svo_depth=10, guide_ratio=0.5, min_samples=512)
+        else:
+            st = sc.render(buf.data_ptr(), r["w"], r["h"], r["S"], depth, rr, r["seed"])
+        service[i] = _t.perf_counter() - t0
+        seg_serial += st.segments
+    serial_s = _t.perf_counter() - t_all
+    # FIFO single-server queue of the serial launches under the arrivals
+    done = 0.0
+    lat_serial = np.zeros(n_req)
+    for i, r in enumerate(reqs):
+        done = max(done, r["t"]) + service[i]
+        lat_serial[i] = done - r["t"]
+    log(f"serial: {seg_serial / serial_s / 1e6:.1f} Mrays/s over {serial_s:.1f} s")
+
+    def run_engine(realtime: bool):
+        srv = N.Server(ctx, hbm_budget=200 << 30, wave_paths=1 << 24, max_slots=64, use_graphs=True)
+        imgs = {}
+        lat = np.zeros(n_req)
+        segs = 0
+        n_done = 0
+        i = 0
+        t0 = _t.monotonic()
+        while n_done < n_req:
+            now = _t.monotonic() - t0
+            while i < n_req and (not realtime or reqs[i]["t"] <= now):
+                r = reqs[i]
+                imgs[i] = image(r)
+                srv.submit(scenes[r["scene"]], imgs[i], r["w"], r["h"], r["S"], depth, rr, r["seed"],
+                           iterations=r["it"], svo_depth=10, guide_ratio=0.5, min_samples=512, tag=i)
+                i += 1
+            a, q = srv.step()
+            for d in srv.poll():
+                k = d["tag"]
+                if d["status"] != 0:
+                    raise RuntimeError(f"request {k} failed with status {d['status']}")
+                lat[k] = d["done_s"] - (t0 + (reqs[k]["t"] if realtime else 0.0))
+                segs += d["segments"]
+                del imgs[k]
+                n_done += 1
+            if a == 0 and q == 0 and i < n_req and realtime:
+                _t.sleep(max(0.0, reqs[i]["t"] - (_t.monotonic() - t0)))
+        wall = _t.monotonic() - t0
+        st = srv.stats()
+        srv.close()
+        return segs, wall, lat, st.steps
+
+    segs_sat, wall_sat, _, steps_sat = run_engine(False)
+    log(f"engine saturated: {segs_sat / wall_sat / 1e6:.1f} Mrays/s over {wall_sat:.1f} s, {steps_sat} steps")
+    segs_rt, wall_rt, lat_rt, _ = run_engine(True)
+    log(f"engine real time: p50 {np.percentile(lat_rt, 50) * 1e3:.1f} ms, "
+        f"p99 {np.percentile(lat_rt, 99) * 1e3:.1f} ms over {wall_rt:.1f} s")
+    assert segs_sat == seg_serial == segs_rt
+    for sc in scenes:
+        sc.close()
+    return {
+        "workload": f"{n_req} requests, Poisson {rate:g}/s (seed 3), 8 sphere-grid scenes {tris} tris, "
+                    "res in {256^2, 512^2, 1024x768, 1920x1080}, spp in {1,4,16,64}, guiding p=0.5 "
+                    "(4 learn-then-render iterations), max depth 10, 200 GB budget",
+        "p50_ms": round(float(np.percentile(lat_rt, 50)) * 1e3, 2),
+        "p99_ms": round(float(np.percentile(lat_rt, 99)) * 1e3, 2),
+        "aggregate_Mrays_s_realtime": round(segs_rt / wall_rt / 1e6, 1),
+        "saturated_Mrays_s": round(segs_sat / wall_sat / 1e6, 1),
+        "serial_Mrays_s": round(seg_serial / serial_s / 1e6, 1),
+        "speedup_vs_serial": round((segs_sat / wall_sat) / (seg_serial / serial_s), 3),
+        "serial_p50_ms": round(float(np.percentile(lat_serial, 50)) * 1e3, 2),
+        "serial_p99_ms": round(float(np.percentile(lat_serial, 99)) * 1e3, 2),
+        "segments": int(seg_serial),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +227,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided-tracer leg")
+    ap.add_argument("--requests", type=int, default=2000, help="serving leg requests (0 = skip)")
+    ap.add_argument("--rate", type=float, default=40.0, help="serving leg arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 512 MiB device-to-device first: the reference kernel for scaling "
                          "FETCH_SIZE/WRITE_SIZE counters")
@@ -160,6 +308,12 @@ def main():
         f"{stc.tri_tests / stc.segments:.2f} triangle tests; image mean {img_np.mean():.4f}")
 
     value = segs / t_total / 1e6
+    guided = None
+    if not args.no_guided:
+        guided = guided_leg(N, ctx, scene, img, torch)
+    serving = None
+    if args.requests > 0:
+        serving = serving_leg(N, ctx, torch, args.requests, args.rate)
     cpu = None
     if not args.no_cpu:
         log("CPU baseline ...")
@@ -190,9 +344,13 @@ def main():
             "max_depth": DEPTH,
             "segments_per_frame": int(st.segments),
             "bvh_build_ms": round(build_ms, 2),
-            "latency_ms": {"p50": round(float(np.percentile(lat_sorted, 50)), 3),
-                           "p99": round(float(np.percentile(lat_sorted, 99)), 3)},
+            "latency_ms": {"p50": serving["p50_ms"], "p99": serving["p99_ms"], "of": "serving requests"}
+                          if serving else
+                          {"p50": round(float(np.percentile(lat_sorted, 50)), 3),
+                           "p99": round(float(np.percentile(lat_sorted, 99)), 3), "of": "frames"},
+            "serving": serving,
             "autotune": tune,
+            "guided": guided,
             "kernel_ms_per_frame": {"trace": round(trace_ms / args.steps, 3),
                                     "shade": round(shade_ms / args.steps, 3),
                                     "partition": round((sort_ms + hist_ms) / args.steps, 3)},

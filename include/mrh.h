@@ -340,6 +340,13 @@ int32_t mrh_tracer_render(mrh_tracer* tracer, int32_t* more);
 int32_t mrh_tracer_finalize(mrh_tracer* tracer, float* d_image);
 int32_t mrh_tracer_reset_image(mrh_tracer* tracer);
 int32_t mrh_tracer_stats(const mrh_tracer* tracer, mrh_render_stats* out, uint32_t* frames);
+/* Turns the tracer into the guided tracer (WFPGTracer, WFPGTracer.h:25-96,
+ * WFPGTracer.cu:1590-1790): from the next frame on, frame k (counted from
+ * this call) records its path vertices while k + 1 < guide->iterations and
+ * the octree is rebuilt from them when the frame ends; every frame after
+ * the first samples from the mixture of the latest guide and the BxDF.
+ * `iterations` frames reproduce mrh_render_guided.  svo may be NULL. */
+int32_t mrh_tracer_set_guide(mrh_tracer* tracer, const mrh_guide_params* guide, mrh_svo* svo);
 int32_t mrh_tracer_destroy(mrh_tracer* tracer);
 
 /* ------------------------------------------------------------------ */
@@ -350,6 +357,64 @@ int32_t mrh_tracer_destroy(mrh_tracer* tracer);
  * tmax = FLT_MAX.  Either output may be NULL. */
 int32_t mrh_gen_random_tris(uint32_t n, uint32_t seed, float* tris);
 int32_t mrh_gen_random_rays(uint32_t n, uint32_t seed, float* rays);
+
+/* ------------------------------------------------------------------ */
+/* Serving engine (no counterpart in the reference, whose TracerThread
+ * renders one scene with one tracer at a time, TracerThread.cpp,
+ * PathTracer.cu:144-291).  Requests for resident scenes are queued,
+ * admitted into <= 64 slots while the HBM budget allows, and traced
+ * together: one wavefront per step over a shared queue of paths from every
+ * admitted request (continuous batching), with the per-step launches
+ * replayed from hipGraphs.  Each request's paths are the ones mrh_render /
+ * mrh_render_guided trace for it (same streams, same arithmetic), so its
+ * image matches a solo render up to the order of the per-pixel float adds.
+ * The server works on a private context (mrh_server_context) on the same
+ * device; the scenes and images of its requests must stay alive until the
+ * request completes.                                                   */
+/* ------------------------------------------------------------------ */
+typedef struct mrh_server mrh_server;
+
+typedef struct {
+    uint64_t hbm_budget;     /* device bytes admitted requests may hold (image, learn records, guide) */
+    uint32_t wave_paths;     /* queue capacity: paths traced per step */
+    uint32_t max_slots;      /* concurrent requests, 1..64 */
+    uint32_t use_graphs;     /* replay each step's launches from a captured hipGraph */
+    uint32_t reserved;
+} mrh_server_params;
+
+typedef struct {
+    mrh_scene*        scene;
+    mrh_camera_desc   camera;
+    mrh_render_params params;   /* samples = strata per axis of each iteration; nee = 0 */
+    mrh_guide_params  guide;    /* iterations <= 1: plain path tracing */
+    float*            d_image;  /* width*height*3 floats, written when the request completes */
+    uint64_t          user_tag;
+} mrh_request;
+
+typedef struct {
+    uint64_t ticket, user_tag;
+    int32_t  status;            /* MRH_OK or the error that ended the request */
+    uint32_t reserved;
+    uint64_t segments;          /* rays traced for the request */
+    double   submit_s, admit_s, done_s;   /* CLOCK_MONOTONIC seconds */
+} mrh_request_done;
+
+typedef struct {
+    uint64_t steps, segments, budget_used;
+    uint32_t active, queued, live_paths, reserved;
+} mrh_server_info;
+
+int32_t mrh_server_create(mrh_context* ctx, const mrh_server_params* params, mrh_server** out);
+int32_t mrh_server_context(mrh_server* server, mrh_context** out);
+/* Queues a request (MRH_ERR_BUDGET if it can never fit the budget). */
+int32_t mrh_server_submit(mrh_server* server, const mrh_request* request, uint64_t* ticket);
+/* Admits what fits, traces one wavefront step, finishes what ended
+ * (synchronous).  active / queued may be NULL. */
+int32_t mrh_server_step(mrh_server* server, uint32_t* active, uint32_t* queued);
+/* Moves up to max completed requests into out (oldest first). */
+int32_t mrh_server_poll(mrh_server* server, mrh_request_done* out, uint32_t max, uint32_t* n);
+int32_t mrh_server_stats(const mrh_server* server, mrh_server_info* out);
+int32_t mrh_server_destroy(mrh_server* server);
 
 /* Scene generators fill caller buffers sized by a first call with NULL
  * buffers (counts written to the out-parameters). */

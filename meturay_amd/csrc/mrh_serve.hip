@@ -1,0 +1,764 @@
+// mrh_serve.hip — the serving engine: render requests of different scenes,
+// resolutions, sample counts and guiding settings share one set of
+// persistent, compacting wavefront kernels (continuous batching).
+//
+// The reference renders one scene per TracerSystem and drives one tracer's
+// Render() loop per frame (TracerThread / TracerSystemCUDA,
+// PathTracer.cu:144-291); it has no request loop.  This engine keeps the
+// same per-path semantics (a request's paths are exactly the paths
+// mrh_render / mrh_render_guided trace for it: same PCG32 keys, same
+// arithmetic) and changes only how work reaches the GPU:
+//   * a request occupies one of <= 64 slots; a device slot table holds its
+//     scene (BVH, materials), camera, frame constants, image and guide;
+//   * every step is one wavefront over the shared queue: camera rays of newly
+//     injected samples are appended behind the surviving rays of all
+//     requests, then trace (per-ray scene), partition by material batch,
+//     shade (per-ray request constants), survivors compacted into the other
+//     queue; per-slot live and traced counts come from LDS histograms;
+//   * the host admits queued requests into free slots while the HBM budget
+//     allows, injects new samples smallest-remaining-work first, and
+//     finishes a request (or runs its guide's learn pass) when its last path
+//     ends;
+//   * the launch sequence of a step is captured once per queue parity in a
+//     hipGraph and replayed (all per-step values live in device memory).
+// The engine works on a private context (own stream and scratch), so the
+// graphs' baked-in buffers cannot be moved by other work.
+#include "mrh_path.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+
+namespace mrh {
+namespace {
+
+constexpr uint32_t SERVE_MAX_SLOTS = 64;
+constexpr uint32_t RES_LIVE = 1, RES_SEG = 1 + SERVE_MAX_SLOTS, RES_WORDS = 1 + 2 * SERVE_MAX_SLOTS;
+
+// Device view of one admitted request
+struct SlotDev {
+    const float4* tnodes;
+    uint32_t root, n_leaves;
+    ShadeReq q;
+    GuideArgs ga;
+    Cam cam;
+    uint32_t w, h, S, seed;
+    uint32_t learn, guide, pad0, pad1;
+};
+
+struct Inject {
+    uint32_t slot, first, count, offset;   // offset: first injected index of the chunk
+};
+
+struct StepHdr {
+    uint32_t n_live;    // surviving rays at the head of the queue
+    uint32_t n_total;   // n_live + injected
+    uint32_t n_chunks, pad;
+    Inject chunks[SERVE_MAX_SLOTS];
+};
+
+// Camera rays of the injected chunks, behind the live rays
+__global__ __launch_bounds__(256) void k_serve_raygen(const StepHdr* __restrict__ hdr,
+                                                      const SlotDev* __restrict__ slots, PathBuf out,
+                                                      uint32_t* __restrict__ out_slot)
+{
+    const uint32_t n_live = hdr->n_live, n_inj = hdr->n_total - hdr->n_live, nc = hdr->n_chunks;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_inj; t += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = nc;   // last chunk with offset <= t
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (hdr->chunks[mid].offset <= t) lo = mid;
+            else hi = mid;
+        }
+        const Inject c = hdr->chunks[lo];
+        const SlotDev& sd = slots[c.slot];
+        const uint32_t i = c.first + (t - c.offset);
+        V3 o, d;
+        uint64_t st;
+        camera_ray(sd.cam, sd.w, sd.h, sd.S, sd.seed, sd.q.sample_base + i, i, o, d, st);
+        const uint32_t k = n_live + t;
+        out.ox[k] = o.x; out.oy[k] = o.y; out.oz[k] = o.z;
+        out.dx[k] = d.x; out.dy[k] = d.y; out.dz[k] = d.z;
+        out.tr[k] = 1.0f; out.tg[k] = 1.0f; out.tb[k] = 1.0f;
+        out.sample[k] = i;
+        out.dm[k] = 1u;
+        out.rng[k] = st;
+        out_slot[k] = c.slot;
+    }
+}
+
+// Closest hit of every queued ray in its own request's scene; camera rays
+// (depth 1) are clipped to the request's near/far planes.
+template <int CAP, int L>
+__global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__ hdr,
+                                                     const SlotDev* __restrict__ slots, PathBuf in,
+                                                     const uint32_t* __restrict__ in_slot, HitBuf hit,
+                                                     uint2* __restrict__ overflow)
+{
+    __shared__ uint32_t s_node[L > 0 ? L * 256 : 1];
+    __shared__ float s_t[L > 0 ? L * 256 : 1];
+    const uint32_t n = hdr->n_total;
+    const uint32_t lb = xcd_block();
+    const size_t ostride = (size_t)gridDim.x * blockDim.x;
+    const size_t oslot = (size_t)lb * blockDim.x + threadIdx.x;
+    for (uint32_t i = lb * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const SlotDev& sd = slots[in_slot[i]];
+        const V3 o = v3(in.ox[i], in.oy[i], in.oz[i]);
+        const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
+        const bool cam_ray = (in.dm[i] & 0xFFu) == 1u;
+        const float tmin = cam_ray ? sd.cam.near_d : 0.0f;
+        const float tmax = cam_ray ? sd.cam.far_d : FLT_MAX;
+        const float4* leaves = sd.q.leaves;
+        TraceResult r;
+        if constexpr (L > 0) {
+            LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
+            r = traverse<false, false>(st, sd.tnodes, leaves, sd.root, sd.n_leaves, o, d, tmin, tmax);
+        } else {
+            PrivateStack<CAP> st;
+            r = traverse<false, false>(st, sd.tnodes, leaves, sd.root, sd.n_leaves, o, d, tmin, tmax);
+        }
+        hit.leaf[i] = r.leaf;
+        hit.t[i] = r.t;
+        hit.ba[i] = r.ba;
+        hit.bb[i] = r.bb;
+        hit.key[i] = (r.leaf == NO_HIT) ? 0u : __float_as_uint(leaves[3 * (size_t)r.leaf + 1].w);
+    }
+}
+
+// Shade every queued ray with its request's constants (shade_ray), append
+// the survivors to the other queue; res[0] = survivors, res[RES_LIVE + s] =
+// survivors of slot s, res[RES_SEG + s] = rays of slot s traced this step.
+__global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__ hdr,
+                                                     const SlotDev* __restrict__ slots, PathBuf in,
+                                                     const uint32_t* __restrict__ in_slot, PathBuf out,
+                                                     uint32_t* __restrict__ out_slot, HitBuf hit,
+                                                     const uint32_t* __restrict__ order, uint32_t* res)
+{
+    __shared__ uint32_t s_live[SERVE_MAX_SLOTS], s_seg[SERVE_MAX_SLOTS];
+    if (threadIdx.x < SERVE_MAX_SLOTS) {
+        s_live[threadIdx.x] = 0;
+        s_seg[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const uint32_t n = hdr->n_total;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        const uint32_t j = base + threadIdx.x;
+        bool alive = false;
+        RayNext nx;
+        uint32_t sample = 0, slot = 0;
+        if (j < n) {
+            const uint32_t rid = order[j];
+            slot = in_slot[rid];
+            const SlotDev& sd = slots[slot];
+            sample = in.sample[rid];
+            alive = shade_ray<true, true>(in, hit, rid, sd.q, sd.ga, sd.learn != 0u, sd.guide != 0u, nx);
+            atomicAdd(&s_seg[slot], 1u);
+            if (alive) atomicAdd(&s_live[slot], 1u);
+        }
+        append_ray(alive, nx, sample, out, &res[0], out_slot, slot);
+    }
+    __syncthreads();
+    if (threadIdx.x < SERVE_MAX_SLOTS) {
+        if (s_live[threadIdx.x]) atomicAdd(&res[RES_LIVE + threadIdx.x], s_live[threadIdx.x]);
+        if (s_seg[threadIdx.x]) atomicAdd(&res[RES_SEG + threadIdx.x], s_seg[threadIdx.x]);
+    }
+}
+
+__global__ void k_serve_scale(float* __restrict__ p, size_t count, float s)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] *= s;
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+}  // namespace mrh
+
+using namespace mrh;
+
+// Host state of one request
+struct ServeReq {
+    uint64_t ticket = 0;
+    mrh_request r = {};
+    uint32_t iterations = 1, it = 0, S = 1;
+    uint64_t total = 0;        // paths per iteration
+    uint64_t injected = 0;     // of the current iteration
+    uint64_t live = 0;         // paths in flight
+    uint64_t segments = 0;
+    uint64_t charge = 0;       // budget bytes
+    int slot = -1;
+    bool learning = false, have_guide = false;
+    void* recmem = nullptr;
+    RecBuf rec = {};
+    uint32_t *c_codes = nullptr, *c_bins = nullptr;
+    float* c_rad = nullptr;
+    mrh_svo* svo = nullptr;
+    float cube[4] = {0, 0, 0, 1};
+    double t_submit = 0, t_admit = 0;
+};
+
+struct mrh_server {
+    mrh_context* user = nullptr;   // errors are mirrored here
+    mrh_context* ctx = nullptr;    // private context: stream, scratch, graphs
+    mrh_server_params p = {};
+    uint32_t cap = 0;
+    PathBuf bufs[2] = {};
+    uint32_t* slot_of[2] = {};
+    HitBuf hit = {};
+    uint32_t* order = nullptr;
+    uint32_t* d_res = nullptr;
+    StepHdr* d_hdr = nullptr;
+    SlotDev* d_slots = nullptr;
+    std::vector<void*> allocs;
+    // pinned host mirrors
+    StepHdr* h_hdr = nullptr;
+    SlotDev* h_slots = nullptr;
+    uint32_t* h_res = nullptr;
+    uint32_t parity = 0;           // queue holding the live rays
+    uint32_t n_live = 0;
+    uint64_t used = 0;             // budget bytes held by admitted requests
+    uint64_t next_ticket = 1;
+    uint64_t steps = 0, segments = 0;
+    std::deque<ServeReq*> queue;
+    ServeReq* slots[SERVE_MAX_SLOTS] = {};
+    std::vector<mrh_request_done> done;
+    // graphs per parity, with what they were captured against
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    void* graph_scratch[2][SLOT_COUNT] = {};
+    uint32_t graph_cap128[2] = {}, graph_lds[2] = {}, graph_bpc[2] = {};
+    bool cap128 = false;
+};
+
+namespace {
+
+int32_t mirror_error(mrh_server* s, int32_t rc)
+{
+    if (rc != MRH_OK && s->user) s->user->err = s->ctx->err;
+    return rc;
+}
+
+// device bytes a request holds while admitted: its image, learn records and
+// an estimate of its guide (nodes <= min(records, 16 * 4^depth), 1300 B each)
+uint64_t request_charge(const ServeReq* q)
+{
+    uint64_t b = (uint64_t)q->r.params.width * q->r.params.height * 12;
+    if (q->iterations > 1) {
+        const uint64_t rec = q->total * q->r.params.max_depth;
+        uint64_t nodes = std::min<uint64_t>(rec, 16ull << (2 * q->r.guide.svo_depth));
+        b += rec * 36 + nodes * 1300;
+    }
+    return b;
+}
+
+void release(mrh_server* s, ServeReq* q)
+{
+    if (q->svo) svo_free(q->svo);
+    q->svo = nullptr;
+    if (q->recmem) {
+        (void)hipStreamSynchronize(s->ctx->stream);
+        (void)hipFree(q->recmem);
+    }
+    q->recmem = nullptr;
+    if (q->slot >= 0) s->slots[q->slot] = nullptr;
+    s->used -= q->charge;
+    delete q;
+}
+
+SlotDev& host_slot(mrh_server* s, const ServeReq* q) { return s->h_slots[q->slot]; }
+
+int32_t upload_slot(mrh_server* s, const ServeReq* q)
+{
+    MRH_HIP_CHECK(s->ctx, hipMemcpyAsync(s->d_slots + q->slot, s->h_slots + q->slot, sizeof(SlotDev),
+                                         hipMemcpyHostToDevice, s->ctx->stream));
+    // the pinned entry may be rewritten before the next step's wait
+    MRH_HIP_CHECK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+    return MRH_OK;
+}
+
+// slot descriptor for the request's current iteration
+void fill_slot(mrh_server* s, ServeReq* q)
+{
+    SlotDev& d = host_slot(s, q);
+    const mrh_scene* sc = q->r.scene;
+    const mrh_lbvh* b = sc->bvh;
+    memset(&d, 0, sizeof(d));
+    d.tnodes = (const float4*)b->tnodes;
+    d.root = root_code(b);
+    d.n_leaves = b->n;
+    d.q.mats = Materials{sc->lambert, sc->metal, sc->dielectric, sc->emissive};
+    d.q.leaves = (const float4*)b->leaves;
+    d.q.image = q->r.d_image;
+    d.q.spp = q->S * q->S;
+    d.q.sample_base = (uint32_t)(q->it * q->total);
+    d.q.rr_start = q->r.params.rr_start;
+    d.q.max_depth = q->r.params.max_depth;
+    d.cam = make_camera(q->r.camera, q->r.params.width, q->r.params.height);
+    d.w = q->r.params.width;
+    d.h = q->r.params.height;
+    d.S = q->S;
+    d.seed = q->r.params.seed;
+    d.learn = q->learning ? 1u : 0u;
+    d.guide = q->have_guide ? 1u : 0u;
+    d.ga.rec = q->rec;
+    d.ga.lox = q->cube[0];
+    d.ga.loy = q->cube[1];
+    d.ga.loz = q->cube[2];
+    d.ga.inv_cell = (float)(1u << q->r.guide.svo_depth) / q->cube[3];
+    d.ga.depth = q->r.guide.svo_depth;
+    d.ga.ratio = q->r.guide.guide_ratio;
+    d.ga.min_samples = q->r.guide.min_samples;
+    if (q->have_guide) d.ga.svo = svo_view(q->svo);
+}
+
+int32_t admit(mrh_server* s, ServeReq* q, int slot)
+{
+    mrh_context* ctx = s->ctx;
+    q->slot = slot;
+    s->slots[slot] = q;
+    s->used += q->charge;
+    q->t_admit = now_s();
+    const size_t img = (size_t)q->r.params.width * q->r.params.height * 3;
+    MRH_TRY(fill_u32(ctx, (uint32_t*)q->r.d_image, 0u, img));
+    if (q->iterations > 1) {
+        const mrh_lbvh* b = q->r.scene->bvh;
+        float span = 0.0f;
+        for (int a = 0; a < 3; a++) span = fmaxf(span, b->hi[a] - b->lo[a]);
+        if (!(span > 0.0f)) span = 1.0f;
+        const float cells = (float)(1u << q->r.guide.svo_depth);
+        const float pad = span / (cells - 2.0f > 1.0f ? cells - 2.0f : 1.0f);
+        q->cube[0] = b->lo[0] - pad;
+        q->cube[1] = b->lo[1] - pad;
+        q->cube[2] = b->lo[2] - pad;
+        q->cube[3] = span + 2.0f * pad;
+        const uint64_t n_rec = q->total * q->r.params.max_depth;
+        MRH_TRY(dmalloc(ctx, &q->recmem, (size_t)n_rec * 36));
+        char* m = (char*)q->recmem;
+        q->rec.code = (uint32_t*)m; m += n_rec * 4;
+        q->rec.bin = (uint32_t*)m; m += n_rec * 4;
+        q->rec.tr = (float*)m; m += n_rec * 4;
+        q->rec.tg = (float*)m; m += n_rec * 4;
+        q->rec.tb = (float*)m; m += n_rec * 4;
+        q->rec.rad = (float*)m; m += n_rec * 4;
+        q->c_codes = (uint32_t*)m; m += n_rec * 4;
+        q->c_bins = (uint32_t*)m; m += n_rec * 4;
+        q->c_rad = (float*)m;
+        q->rec.max_depth = q->r.params.max_depth;
+        MRH_TRY(fill_u32(ctx, q->rec.code, SVO_NO_NODE, n_rec));
+        MRH_TRY(mrh_svo_create(ctx, &q->svo));
+        q->learning = true;
+    }
+    fill_slot(s, q);
+    return upload_slot(s, q);
+}
+
+// All paths of the request's current iteration have ended
+int32_t iteration_done(mrh_server* s, ServeReq* q)
+{
+    mrh_context* ctx = s->ctx;
+    if (q->learning) {
+        const uint32_t n_rec = (uint32_t)(q->total * q->r.params.max_depth);
+        uint32_t kept = 0;
+        MRH_TRY(svo_compact_records(ctx, q->rec.code, q->rec.bin, q->rec.rad, n_rec, q->c_codes, q->c_bins,
+                                    q->c_rad, &kept));
+        if (kept) {
+            MRH_TRY(svo_build(ctx, q->svo, q->c_codes, kept, q->cube, q->r.guide.svo_depth));
+            MRH_TRY(svo_splat(ctx, q->svo, q->c_codes, q->c_bins, q->c_rad, kept));
+            MRH_TRY(svo_filter(ctx, q->svo));
+            q->have_guide = true;
+        }
+    }
+    q->it++;
+    if (q->it < q->iterations) {
+        q->injected = 0;
+        q->learning = q->it + 1 < q->iterations;
+        if (q->learning) MRH_TRY(fill_u32(ctx, q->rec.code, SVO_NO_NODE, q->total * q->r.params.max_depth));
+        fill_slot(s, q);
+        return upload_slot(s, q);
+    }
+    // finished: the image is the mean over all samples of all iterations
+    const size_t img = (size_t)q->r.params.width * q->r.params.height * 3;
+    MRH_TRY(launch(ctx, "serve_finalize", dim3(stride_grid(ctx, img, 256)), dim3(256), 0, k_serve_scale,
+                   q->r.d_image, img, 1.0f / ((float)(q->S * q->S) * (float)q->iterations)));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    mrh_request_done d = {};
+    d.ticket = q->ticket;
+    d.user_tag = q->r.user_tag;
+    d.status = MRH_OK;
+    d.segments = q->segments;
+    d.submit_s = q->t_submit;
+    d.admit_s = q->t_admit;
+    d.done_s = now_s();
+    s->done.push_back(d);
+    release(s, q);
+    return MRH_OK;
+}
+
+// The launches of one step over queue parity `par` (captured or eager)
+int32_t step_launches(mrh_server* s, uint32_t par)
+{
+    mrh_context* ctx = s->ctx;
+    const PathBuf& in = s->bufs[par];
+    const PathBuf& out = s->bufs[par ^ 1];
+    const uint32_t* in_slot = s->slot_of[par];
+    uint32_t* out_slot = s->slot_of[par ^ 1];
+    MRH_TRY(fill_u32(ctx, s->d_res, 0u, RES_WORDS));
+    MRH_TRY(launch(ctx, "serve_raygen", dim3(stride_grid(ctx, s->cap, 256)), dim3(256), 0, k_serve_raygen,
+                   (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, s->slot_of[par]));
+    const uint32_t tgrid = trace_grid(ctx, s->cap);
+    const uint32_t lds = ctx->trace_lds_entries;
+    const int cap = s->cap128 ? 128 : 64;
+    uint2* ovf = nullptr;
+    MRH_TRY(trace_overflow(ctx, tgrid, cap, lds, &ovf));
+#define MRH_SERVE_TRACE(C, LV)                                                                                \
+    MRH_TRY(launch(ctx, "serve_trace", dim3(tgrid), dim3(256), 0, k_serve_trace<C, LV>, (const StepHdr*)s->d_hdr, \
+                   (const SlotDev*)s->d_slots, in, in_slot, s->hit, ovf))
+#define MRH_SERVE_TRACE_L(C)                                                     \
+    switch (lds) {                                                               \
+    case 0: MRH_SERVE_TRACE(C, 0); break;                                        \
+    case 8: MRH_SERVE_TRACE(C, 8); break;                                        \
+    case 12: MRH_SERVE_TRACE(C, 12); break;                                      \
+    case 16: MRH_SERVE_TRACE(C, 16); break;                                      \
+    case 24: MRH_SERVE_TRACE(C, 24); break;                                      \
+    default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24"); \
+    }
+    if (cap == 128) {
+        MRH_SERVE_TRACE_L(128)
+    } else {
+        MRH_SERVE_TRACE_L(64)
+    }
+#undef MRH_SERVE_TRACE_L
+#undef MRH_SERVE_TRACE
+    SortArgs sa;
+    sa.keys_in = s->hit.key;
+    sa.vals_out = s->order;
+    sa.n_max = s->cap;
+    sa.n_dev = &s->d_hdr->n_total;
+    sa.key_bytes = 4;
+    sa.begin_bit = 24;
+    sa.end_bit = 32;
+    sa.descending = true;
+    MRH_TRY(radix_sort(ctx, sa));
+    return launch(ctx, "serve_shade", dim3(stride_grid(ctx, s->cap, 256)), dim3(256), 0, k_serve_shade,
+                  (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, in_slot, out, out_slot, s->hit,
+                  (const uint32_t*)s->order, s->d_res);
+}
+
+bool graph_valid(mrh_server* s, uint32_t par)
+{
+    if (!s->exec[par]) return false;
+    for (int k = 0; k < SLOT_COUNT; k++)
+        if (s->graph_scratch[par][k] != s->ctx->slot_ptr[k]) return false;
+    return s->graph_cap128[par] == (uint32_t)s->cap128 && s->graph_lds[par] == s->ctx->trace_lds_entries &&
+           s->graph_bpc[par] == s->ctx->trace_blocks_per_cu;
+}
+
+int32_t run_step(mrh_server* s, uint32_t par)
+{
+    mrh_context* ctx = s->ctx;
+    if (!s->p.use_graphs) return step_launches(s, par);
+    if (!graph_valid(s, par)) {
+        if (s->exec[par]) {
+            (void)hipGraphExecDestroy(s->exec[par]);
+            s->exec[par] = nullptr;
+        }
+        // one eager step first so every scratch buffer exists at its final size
+        MRH_TRY(step_launches(s, par));
+        MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        const bool prof = ctx->profile;
+        ctx->profile = false;
+        hipGraph_t g = nullptr;
+        MRH_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int32_t rc = step_launches(s, par);
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        ctx->profile = prof;
+        if (rc != MRH_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(ctx, MRH_ERR_HIP, "step capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&s->exec[par], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(ctx, MRH_ERR_HIP, "step graph instantiate: %s", hipGetErrorString(e));
+        for (int k = 0; k < SLOT_COUNT; k++) s->graph_scratch[par][k] = ctx->slot_ptr[k];
+        s->graph_cap128[par] = s->cap128;
+        s->graph_lds[par] = ctx->trace_lds_entries;
+        s->graph_bpc[par] = ctx->trace_blocks_per_cu;
+        // the eager step above already did this step's work
+        return MRH_OK;
+    }
+    MRH_HIP_CHECK(ctx, hipGraphLaunch(s->exec[par], ctx->stream));
+    return MRH_OK;
+}
+
+int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
+{
+    mrh_context* ctx = s->ctx;
+    // admission, first come first served, while slots and budget allow
+    while (!s->queue.empty()) {
+        ServeReq* q = s->queue.front();
+        int slot = -1;
+        uint32_t n_active = 0;
+        for (uint32_t k = 0; k < s->p.max_slots; k++) {
+            if (s->slots[k]) n_active++;
+            else if (slot < 0) slot = (int)k;
+        }
+        if (slot < 0 || (n_active > 0 && s->used + q->charge > s->p.hbm_budget)) break;
+        s->queue.pop_front();
+        int32_t rc = admit(s, q, slot);
+        if (rc != MRH_OK) {
+            mrh_request_done d = {};
+            d.ticket = q->ticket;
+            d.user_tag = q->r.user_tag;
+            d.status = rc;
+            d.submit_s = q->t_submit;
+            d.done_s = now_s();
+            s->done.push_back(d);
+            release(s, q);
+            return rc;
+        }
+    }
+    // injection, smallest remaining work first
+    std::vector<ServeReq*> act;
+    for (uint32_t k = 0; k < s->p.max_slots; k++)
+        if (s->slots[k] && s->slots[k]->injected < s->slots[k]->total) act.push_back(s->slots[k]);
+    std::sort(act.begin(), act.end(), [](const ServeReq* a, const ServeReq* b) {
+        const uint64_t ra = (a->total - a->injected) + (uint64_t)(a->iterations - a->it - 1) * a->total;
+        const uint64_t rb = (b->total - b->injected) + (uint64_t)(b->iterations - b->it - 1) * b->total;
+        return ra != rb ? ra < rb : a->ticket < b->ticket;
+    });
+    StepHdr& h = *s->h_hdr;
+    h.n_live = s->n_live;
+    h.n_chunks = 0;
+    uint64_t room = s->cap - s->n_live, inj = 0;
+    for (ServeReq* q : act) {
+        if (!room) break;
+        const uint64_t k = std::min<uint64_t>(room, q->total - q->injected);
+        Inject& c = h.chunks[h.n_chunks++];
+        c.slot = (uint32_t)q->slot;
+        c.first = (uint32_t)q->injected;
+        c.count = (uint32_t)k;
+        c.offset = (uint32_t)inj;
+        q->injected += k;
+        q->live += k;
+        inj += k;
+        room -= k;
+    }
+    h.n_total = s->n_live + (uint32_t)inj;
+    if (h.n_total > 0) {
+        MRH_HIP_CHECK(ctx, hipMemcpyAsync(s->d_hdr, s->h_hdr, sizeof(StepHdr), hipMemcpyHostToDevice, ctx->stream));
+        MRH_TRY(run_step(s, s->parity));
+        MRH_HIP_CHECK(ctx, hipMemcpyAsync(s->h_res, s->d_res, RES_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
+        MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        s->steps++;
+        s->segments += h.n_total;
+        s->n_live = s->h_res[0];
+        s->parity ^= 1u;
+        for (uint32_t k = 0; k < s->p.max_slots; k++) {
+            ServeReq* q = s->slots[k];
+            if (!q) continue;
+            q->live = s->h_res[RES_LIVE + k];
+            q->segments += s->h_res[RES_SEG + k];
+        }
+    }
+    // iterations / requests whose paths have all ended
+    for (uint32_t k = 0; k < s->p.max_slots; k++) {
+        ServeReq* q = s->slots[k];
+        if (q && q->injected == q->total && q->live == 0) MRH_TRY(iteration_done(s, q));
+    }
+    uint32_t a = 0;
+    for (uint32_t k = 0; k < s->p.max_slots; k++) a += s->slots[k] ? 1u : 0u;
+    if (active) *active = a;
+    if (queued) *queued = (uint32_t)s->queue.size();
+    return MRH_OK;
+}
+
+void server_free(mrh_server* s)
+{
+    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+    for (uint32_t k = 0; k < SERVE_MAX_SLOTS; k++)
+        if (s->slots[k]) release(s, s->slots[k]);
+    for (ServeReq* q : s->queue) delete q;
+    for (int par = 0; par < 2; par++)
+        if (s->exec[par]) (void)hipGraphExecDestroy(s->exec[par]);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->h_hdr) (void)hipHostFree(s->h_hdr);
+    if (s->h_slots) (void)hipHostFree(s->h_slots);
+    if (s->h_res) (void)hipHostFree(s->h_res);
+    if (s->ctx) mrh_context_destroy(s->ctx);
+    delete s;
+}
+
+int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server** out)
+{
+    if (p->max_slots == 0 || p->max_slots > SERVE_MAX_SLOTS)
+        return fail(user, MRH_ERR_INVALID, "server: max_slots must be in [1, %u]", SERVE_MAX_SLOTS);
+    if (p->wave_paths < 1024 || p->wave_paths >= (1u << 30))
+        return fail(user, MRH_ERR_INVALID, "server: wave_paths must be in [1024, 2^30)");
+    mrh_server* s = new mrh_server();
+    s->user = user;
+    s->p = *p;
+    s->cap = p->wave_paths;
+    int32_t rc = mrh_context_create(user->device, &s->ctx);
+    if (rc != MRH_OK) {
+        delete s;
+        return fail(user, rc, "server: context creation failed");
+    }
+    mrh_context* ctx = s->ctx;
+    ctx->trace_lds_entries = user->trace_lds_entries;
+    ctx->trace_blocks_per_cu = user->trace_blocks_per_cu;
+    auto take = [&](size_t bytes) -> void* {
+        void* q = nullptr;
+        if (rc == MRH_OK) rc = dmalloc(ctx, &q, bytes);
+        if (q) s->allocs.push_back(q);
+        return q;
+    };
+    const size_t n = s->cap;
+    for (int b = 0; b < 2; b++) {
+        PathBuf& q = s->bufs[b];
+        q.ox = (float*)take(n * 4); q.oy = (float*)take(n * 4); q.oz = (float*)take(n * 4);
+        q.dx = (float*)take(n * 4); q.dy = (float*)take(n * 4); q.dz = (float*)take(n * 4);
+        q.tr = (float*)take(n * 4); q.tg = (float*)take(n * 4); q.tb = (float*)take(n * 4);
+        q.sample = (uint32_t*)take(n * 4);
+        q.dm = (uint32_t*)take(n * 4);
+        q.rng = (uint64_t*)take(n * 8);
+        s->slot_of[b] = (uint32_t*)take(n * 4);
+    }
+    s->hit.leaf = (uint32_t*)take(n * 4);
+    s->hit.t = (float*)take(n * 4);
+    s->hit.ba = (float*)take(n * 4);
+    s->hit.bb = (float*)take(n * 4);
+    s->hit.key = (uint32_t*)take(n * 4);
+    s->order = (uint32_t*)take(n * 4);
+    s->d_res = (uint32_t*)take(RES_WORDS * 4);
+    s->d_hdr = (StepHdr*)take(sizeof(StepHdr));
+    s->d_slots = (SlotDev*)take(sizeof(SlotDev) * SERVE_MAX_SLOTS);
+    if (rc == MRH_OK && hipHostMalloc((void**)&s->h_hdr, sizeof(StepHdr), hipHostMallocDefault) != hipSuccess)
+        rc = fail(ctx, MRH_ERR_OOM, "server: pinned header");
+    if (rc == MRH_OK &&
+        hipHostMalloc((void**)&s->h_slots, sizeof(SlotDev) * SERVE_MAX_SLOTS, hipHostMallocDefault) != hipSuccess)
+        rc = fail(ctx, MRH_ERR_OOM, "server: pinned slot table");
+    if (rc == MRH_OK && hipHostMalloc((void**)&s->h_res, RES_WORDS * 4, hipHostMallocDefault) != hipSuccess)
+        rc = fail(ctx, MRH_ERR_OOM, "server: pinned results");
+    if (rc == MRH_OK) {
+        memset(s->h_hdr, 0, sizeof(StepHdr));
+        memset(s->h_slots, 0, sizeof(SlotDev) * SERVE_MAX_SLOTS);
+        rc = fill_u32(ctx, (uint32_t*)s->d_slots, 0u, sizeof(SlotDev) * SERVE_MAX_SLOTS / 4);
+    }
+    if (rc == MRH_OK) rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? MRH_OK : MRH_ERR_HIP;
+    if (rc != MRH_OK) {
+        if (s->user != nullptr) mirror_error(s, rc);
+        server_free(s);
+        return rc;
+    }
+    *out = s;
+    return MRH_OK;
+}
+
+int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
+{
+    mrh_context* ctx = s->ctx;
+    if (!r->scene || !r->d_image) return fail(ctx, MRH_ERR_INVALID, "submit: scene and d_image are required");
+    const mrh_render_params& p = r->params;
+    if (p.width == 0 || p.height == 0 || p.samples == 0)
+        return fail(ctx, MRH_ERR_INVALID, "submit: empty frame");
+    if (p.max_depth == 0 || p.max_depth > 62) return fail(ctx, MRH_ERR_INVALID, "submit: max_depth must be in [1, 62]");
+    if (p.nee) return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: next-event estimation is not implemented");
+    if (r->scene->bvh->max_depth > 128)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: tree depth %u exceeds the traversal stack",
+                    r->scene->bvh->max_depth);
+    ServeReq* q = new ServeReq();
+    q->r = *r;
+    q->S = p.samples;
+    q->iterations = r->guide.iterations > 1 ? r->guide.iterations : 1u;
+    q->total = (uint64_t)p.width * p.height * p.samples * p.samples;
+    if (q->iterations > 1) {
+        if (q->iterations > 64 || r->guide.svo_depth == 0 || r->guide.svo_depth > 10 ||
+            !(r->guide.guide_ratio >= 0.0f && r->guide.guide_ratio <= 1.0f)) {
+            delete q;
+            return fail(ctx, MRH_ERR_INVALID, "submit: bad guide parameters");
+        }
+    }
+    if (q->total * q->iterations >= (1ull << 32) || q->total * p.max_depth >= (1ull << 31)) {
+        delete q;
+        return fail(ctx, MRH_ERR_INVALID, "submit: request too large");
+    }
+    q->charge = request_charge(q);
+    if (q->charge > s->p.hbm_budget) {
+        delete q;
+        return fail(ctx, MRH_ERR_BUDGET, "submit: request needs %llu bytes, budget is %llu",
+                    (unsigned long long)q->charge, (unsigned long long)s->p.hbm_budget);
+    }
+    if (r->scene->bvh->max_depth > 64) s->cap128 = true;
+    q->ticket = s->next_ticket++;
+    q->t_submit = now_s();
+    s->queue.push_back(q);
+    if (ticket) *ticket = q->ticket;
+    return MRH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mrh_server_create(mrh_context* ctx, const mrh_server_params* params, mrh_server** out)
+{
+    if (!ctx || !params || !out) return MRH_ERR_INVALID;
+    *out = nullptr;
+    return server_create(ctx, params, out);
+}
+
+int32_t mrh_server_context(mrh_server* server, mrh_context** out)
+{
+    if (!server || !out) return MRH_ERR_INVALID;
+    *out = server->ctx;
+    return MRH_OK;
+}
+
+int32_t mrh_server_submit(mrh_server* server, const mrh_request* request, uint64_t* ticket)
+{
+    if (!server || !request) return MRH_ERR_INVALID;
+    return mirror_error(server, server_submit(server, request, ticket));
+}
+
+int32_t mrh_server_step(mrh_server* server, uint32_t* active, uint32_t* queued)
+{
+    if (!server) return MRH_ERR_INVALID;
+    return mirror_error(server, server_step(server, active, queued));
+}
+
+int32_t mrh_server_poll(mrh_server* server, mrh_request_done* out, uint32_t max, uint32_t* n)
+{
+    if (!server || !n || (max && !out)) return MRH_ERR_INVALID;
+    const uint32_t k = (uint32_t)std::min<size_t>(max, server->done.size());
+    for (uint32_t i = 0; i < k; i++) out[i] = server->done[i];
+    server->done.erase(server->done.begin(), server->done.begin() + k);
+    *n = k;
+    return MRH_OK;
+}
+
+int32_t mrh_server_stats(const mrh_server* server, mrh_server_info* out)
+{
+    if (!server || !out) return MRH_ERR_INVALID;
+    memset(out, 0, sizeof(*out));
+    out->steps = server->steps;
+    out->segments = server->segments;
+    out->budget_used = server->used;
+    out->queued = (uint32_t)server->queue.size();
+    for (uint32_t k = 0; k < SERVE_MAX_SLOTS; k++) out->active += server->slots[k] ? 1u : 0u;
+    out->live_paths = server->n_live;
+    return MRH_OK;
+}
+
+int32_t mrh_server_destroy(mrh_server* server)
+{
+    if (!server) return MRH_ERR_INVALID;
+    server_free(server);
+    return MRH_OK;
+}
+
+}  // extern "C"

@@ -43,11 +43,12 @@ private:
     std::vector<mrh_camera_desc> cams;
 };
 
-// PathTracer (TracerCUDA/PathTracer.cu) over mrh_tracer
+// PathTracer (TracerCUDA/PathTracer.cu) and, with `guided`, WFPGTracer
+// (TracerCUDA/WFPGTracer.cu) over mrh_tracer
 class PathTracerMRH final : public GPUTracerI {
 public:
-    PathTracerMRH(mrh_context* ctx, SceneMRH* scene, const TracerParameters& p)
-        : ctx(ctx), scene(scene), params(p) {}
+    PathTracerMRH(mrh_context* ctx, SceneMRH* scene, const TracerParameters& p, bool guided)
+        : ctx(ctx), scene(scene), params(p), guided(guided) {}
     ~PathTracerMRH() override { Release(); }
 
     void AttachTracerCallbacks(TracerCallbacksI& cb) override { callbacks = &cb; }
@@ -67,6 +68,15 @@ public:
         rp.seed = params.seed;
         int32_t rc = mrh_tracer_create(ctx, scene->Handle(), &rp, &tracer);
         if (rc != MRH_OK) return Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+        if (guided) {
+            mrh_guide_params gp = {};
+            gp.iterations = learnIterations;
+            gp.svo_depth = octreeLevel;
+            gp.guide_ratio = (float)misRatio;
+            gp.min_samples = binRayCount;
+            rc = mrh_tracer_set_guide(tracer, &gp, nullptr);
+            if (rc != MRH_OK) return Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+        }
         void* img = nullptr;
         if (mrh_device_alloc(ctx, &img, (size_t)width * height * 3 * sizeof(float)) != MRH_OK)
             return Report(TracerError(TracerError::GPU_OUT_OF_MEMORY, "image buffer"));
@@ -74,11 +84,36 @@ public:
         return TracerError::OK;
     }
 
-    // PathTracer.h:17-33 option names; "Samples" = strata per axis
+    // PathTracer.h:17-33 option names; "Samples" = strata per axis.  The
+    // guided tracer adds WFPGTracer.h:25-96's "OctreeLevel", "BinRayCount"
+    // (guide lookups stop above octree nodes with fewer samples) and
+    // "BXDF-GuideMISRatio", plus "LearnIterations": the number of frames
+    // that learn (the octree of frame k guides frame k + 1).
     TracerError SetOptions(const Options& o) override
     {
         int64_t v;
         bool b;
+        double dv;
+        if (o.Has("OctreeLevel")) {
+            TracerError e = o.GetInt(v, "OctreeLevel");
+            if (e != TracerError::OK) return Report(e);
+            octreeLevel = (uint32_t)v;
+        }
+        if (o.Has("BinRayCount")) {
+            TracerError e = o.GetInt(v, "BinRayCount");
+            if (e != TracerError::OK) return Report(e);
+            binRayCount = (uint32_t)v;
+        }
+        if (o.Has("LearnIterations")) {
+            TracerError e = o.GetInt(v, "LearnIterations");
+            if (e != TracerError::OK) return Report(e);
+            learnIterations = (uint32_t)v;
+        }
+        if (o.Has("BXDF-GuideMISRatio")) {
+            TracerError e = o.GetDouble(dv, "BXDF-GuideMISRatio");
+            if (e != TracerError::OK) return Report(e);
+            misRatio = dv;
+        }
         if (o.Has("Samples")) {
             TracerError e = o.GetInt(v, "Samples");
             if (e != TracerError::OK) return Report(e);
@@ -210,6 +245,9 @@ private:
     float* dImage = nullptr;
     uint32_t width = 256, height = 256, samples = 1, maxDepth = 10, rrStart = 3;
     bool nee = false;
+    bool guided = false;
+    uint32_t octreeLevel = 10, binRayCount = 512, learnIterations = 4;
+    double misRatio = 0.5;
     std::chrono::steady_clock::time_point frameStart;
 };
 
@@ -273,8 +311,8 @@ public:
                                const std::string& type) override
     {
         if (!ctx || !scene) return TracerError(TracerError::NO_LOGIC_SET, "no scene");
-        if (type != "PathTracer") return TracerError(TracerError::NO_LOGIC_FOR_TRACER, type);
-        auto t = std::make_unique<PathTracerMRH>(ctx, scene.get(), params);
+        if (type != "PathTracer" && type != "WFPGTracer") return TracerError(TracerError::NO_LOGIC_FOR_TRACER, type);
+        auto t = std::make_unique<PathTracerMRH>(ctx, scene.get(), params, type == "WFPGTracer");
         TracerError e = t->SetOptions(opts);
         if (e != TracerError::OK) return e;
         tracer = std::move(t);

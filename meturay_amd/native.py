@@ -75,6 +75,27 @@ class GuideParams(C.Structure):
                 ("min_samples", _U32)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("wave_paths", _U32), ("max_slots", _U32),
+                ("use_graphs", _U32), ("reserved", _U32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", _P), ("camera", CameraDesc), ("params", RenderParams), ("guide", GuideParams),
+                ("d_image", _P), ("user_tag", C.c_uint64)]
+
+
+class RequestDone(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("user_tag", C.c_uint64), ("status", _I32), ("reserved", _U32),
+                ("segments", C.c_uint64), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("segments", C.c_uint64), ("budget_used", C.c_uint64),
+                ("active", _U32), ("queued", _U32), ("live_paths", _U32), ("reserved", _U32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
                 ("lambert", _P), ("n_lambert", _U32), ("metal", _P), ("n_metal", _U32),
@@ -128,6 +149,7 @@ _SIGS = {
     "mrh_tracer_reset_image": (_I32, [_P]),
     "mrh_tracer_stats": (_I32, [_P, C.POINTER(RenderStats), C.POINTER(_U32)]),
     "mrh_tracer_destroy": (_I32, [_P]),
+    "mrh_tracer_set_guide": (_I32, [_P, C.POINTER(GuideParams), _P]),
     "mrh_svo_create": (_I32, [_P, C.POINTER(_P)]),
     "mrh_svo_destroy": (_I32, [_P]),
     "mrh_svo_codes": (_I32, [_P, _P, _U32, _U32, _P, _U32, _P]),
@@ -140,6 +162,13 @@ _SIGS = {
     "mrh_svo_view_get": (_I32, [_P, C.POINTER(SvoView)]),
     "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                  C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
+    "mrh_server_create": (_I32, [_P, C.POINTER(ServerParams), C.POINTER(_P)]),
+    "mrh_server_context": (_I32, [_P, C.POINTER(_P)]),
+    "mrh_server_submit": (_I32, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "mrh_server_step": (_I32, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
+    "mrh_server_poll": (_I32, [_P, C.POINTER(RequestDone), _U32, C.POINTER(_U32)]),
+    "mrh_server_stats": (_I32, [_P, C.POINTER(ServerInfo)]),
+    "mrh_server_destroy": (_I32, [_P]),
     "mrh_gen_random_tris": (_I32, [_U32, _U32, _P]),
     "mrh_gen_random_rays": (_I32, [_U32, _U32, _P]),
     "mrh_gen_cornell": (_I32, [_U32, C.POINTER(SceneBuffers)]),
@@ -457,6 +486,81 @@ class Svo:
     def close(self) -> None:
         if self.h:
             lib().mrh_svo_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _BorrowedContext(Context):
+    """A context owned by someone else (the server's private context)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        self.device = None
+
+    def close(self) -> None:
+        self.h = _P()
+
+
+class Server:
+    """Serving engine (mrh_server): queued render requests traced together."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 200 << 30, wave_paths: int = 1 << 24,
+                 max_slots: int = 64, use_graphs: bool = True):
+        self.ctx = ctx
+        self.h = _P()
+        p = ServerParams(hbm_budget, wave_paths, max_slots, 1 if use_graphs else 0, 0)
+        ctx.check(lib().mrh_server_create(ctx.h, C.byref(p), C.byref(self.h)))
+        inner = _P()
+        ctx.check(lib().mrh_server_context(self.h, C.byref(inner)))
+        self.inner = _BorrowedContext(inner)
+
+    def submit(self, scene: Scene, d_image, width, height, samples, max_depth, rr_start, seed=0,
+               camera=None, iterations=1, svo_depth=10, guide_ratio=0.5, min_samples=512, tag=0) -> int:
+        r = Request()
+        r.scene = scene.h
+        r.camera = camera_desc(camera or scene.camera)
+        r.params = RenderParams(width, height, samples, max_depth, rr_start, 0, seed, 0)
+        r.guide = GuideParams(iterations, svo_depth, guide_ratio, min_samples)
+        r.d_image = ptr(d_image)
+        r.user_tag = tag
+        t = C.c_uint64()
+        self.ctx.check(lib().mrh_server_submit(self.h, C.byref(r), C.byref(t)))
+        return t.value
+
+    def step(self) -> tuple[int, int]:
+        a, q = _U32(), _U32()
+        self.ctx.check(lib().mrh_server_step(self.h, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def poll(self, max_items: int = 256) -> list[dict]:
+        buf = (RequestDone * max_items)()
+        n = _U32()
+        self.ctx.check(lib().mrh_server_poll(self.h, buf, max_items, C.byref(n)))
+        return [{"ticket": d.ticket, "tag": d.user_tag, "status": d.status, "segments": d.segments,
+                 "submit_s": d.submit_s, "admit_s": d.admit_s, "done_s": d.done_s} for d in buf[:n.value]]
+
+    def stats(self) -> ServerInfo:
+        st = ServerInfo()
+        self.ctx.check(lib().mrh_server_stats(self.h, C.byref(st)))
+        return st
+
+    def run(self) -> list[dict]:
+        """Steps until nothing is active or queued; returns the completions."""
+        done = []
+        while True:
+            a, q = self.step()
+            done += self.poll()
+            if a == 0 and q == 0:
+                return done + self.poll()
+
+    def close(self) -> None:
+        if self.h:
+            lib().mrh_server_destroy(self.h)
             self.h = _P()
 
     def __del__(self):

@@ -235,3 +235,42 @@ def test_guided_render_one_iteration_equals_plain(ctx):
     with pytest.raises(N.MrhError):
         scene.render_guided(b.data_ptr(), 32, 32, 2, 6, 2, 4, guide_ratio=1.5)
     scene.close()
+
+
+def test_incremental_guided_tracer_matches_render_guided(ctx):
+    import ctypes as C
+    sc = N.gen_cornell(2)
+    scene = N.Scene(ctx, sc)
+    w, h, S, depth, rr, seed = 32, 24, 2, 6, 2, 7
+    g = N.GuideParams(3, 6, 0.5, 16)
+    p = N.RenderParams(w, h, S, depth, rr, 0, seed, 0)
+    t = C.c_void_p()
+    ctx.check(N.lib().mrh_tracer_create(ctx.h, scene.h, C.byref(p), C.byref(t)))
+    svo_t = N.Svo(ctx)
+    ctx.check(N.lib().mrh_tracer_set_guide(t, C.byref(g), svo_t.h))
+    cam = N.camera_desc(sc["camera"])
+    img = torch.zeros(h * w * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        ctx.check(N.lib().mrh_tracer_generate_work(t, C.byref(cam)))
+        more = C.c_int32(1)
+        while more.value:
+            ctx.check(N.lib().mrh_tracer_render(t, C.byref(more)))
+    ctx.check(N.lib().mrh_tracer_finalize(t, img.data_ptr()))
+    st = N.RenderStats()
+    frames = C.c_uint32()
+    ctx.check(N.lib().mrh_tracer_stats(t, C.byref(st), C.byref(frames)))
+    assert frames.value == 3
+    ref = torch.zeros_like(img)
+    svo_r = N.Svo(ctx)
+    st_r = scene.render_guided(ref.data_ptr(), w, h, S, depth, rr, seed, iterations=3, svo_depth=6,
+                               guide_ratio=0.5, min_samples=16, svo=svo_r)
+    assert st.segments == st_r.segments
+    assert torch.allclose(img, ref, rtol=1e-5, atol=1e-6)
+    a, b = gpu_view(ctx, svo_t), gpu_view(ctx, svo_r)
+    for k in ("codes", "count", "radiance"):
+        assert np.array_equal(a[k], b[k]), k
+    ctx.check(N.lib().mrh_tracer_destroy(t))
+    svo_t.close()
+    svo_r.close()
+    scene.close()

@@ -29,10 +29,28 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak
 W, H, S, DEPTH, RR, SEED = 1920, 1080, 4, 10, 3, 0
 GRID, STACKS, SLICES, SCENE_SEED = 64, 16, 16, 2
 
-# algorithmic bytes of one closest-hit ray cast (per ray: origin + direction
-# 24 B read; leaf, t, 2 barycentrics, HitKey 20 B written), of one internal
-# node visit (one 64 B record) and of one triangle test (one 48 B leaf record)
-RAY_BYTES, NODE_BYTES, LEAF_BYTES = 44, 64, 48
+# Algorithmic bytes (the per-unit figures of the measurement definition):
+# a traced segment streams 44 B through the trace kernel (ray 28 B in, hit
+# 16 B out) and 212 B through the whole bounce (trace 44 + key/partition 32 +
+# shade 128 + compaction 8); the tree costs 32 B per node visit and 36 B per
+# triangle test of the specified traversal (nearer child first, ties left) on
+# the binary LBVH, whatever tree the kernel actually walks.
+TRACE_SEG_BYTES, FRAME_SEG_BYTES, NODE_BYTES, TRI_BYTES = 44, 212, 32, 36
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def recorded_traffic(kernel: str, width: int, lds: int):
+    """HBM bytes per launch of `kernel` measured by the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs recorded in profiles/ for the same tree width and LDS
+    stack depth, or None."""
+    import glob
+    import re
+    pat = re.compile(r"^bench-traffic: (\S+) width=(\d+) lds=(\d+) bytes_per_launch=(\d+)", re.M)
+    for f in sorted(glob.glob(os.path.join(PROFILES, "*.md"))):
+        for m in pat.finditer(open(f).read()):
+            if m.group(1) == kernel and (int(m.group(2)), int(m.group(3))) == (width, lds):
+                return int(m.group(4)), os.path.basename(f)
+    return None
 
 
 def log(*a):
@@ -227,6 +245,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--knobs", default=None,
+                    help="fixed launch shape width,lds,blocks_per_cu instead of autotuning (e.g. 4,8,8)")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-tracer leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving leg requests (0 = skip)")
     ap.add_argument("--rate", type=float, default=40.0, help="serving leg arrivals per second")
@@ -263,13 +283,20 @@ def main():
         log("copy calibration: 536870912 bytes read + written")
 
     tune = None
-    if not args.no_autotune:
+    if args.knobs:
+        w_, l_, b_ = (int(x) for x in args.knobs.split(","))
+        ctx.tune("trace_width", w_)
+        ctx.tune("trace_lds_entries", l_)
+        ctx.tune("trace_blocks_per_cu", b_)
+        tune = {"trace_width": w_, "trace_lds_entries": l_, "trace_blocks_per_cu": b_, "fixed": True}
+    elif not args.no_autotune:
         res = scene.autotune(img.data_ptr(), W, H, S, DEPTH, RR, SEED, reps=2)
-        tune = {"trace_lds_entries": res.best_lds, "trace_blocks_per_cu": res.best_bpc,
-                "trials": [[res.trial_lds[i], res.trial_bpc[i], round(res.trial_ms[i], 3)]
+        tune = {"trace_width": res.best_width, "trace_lds_entries": res.best_lds,
+                "trace_blocks_per_cu": res.best_bpc,
+                "trials": [[res.trial_width[i], res.trial_lds[i], res.trial_bpc[i], round(res.trial_ms[i], 3)]
                            for i in range(res.trials)]}
-        log(f"autotune: lds {res.best_lds}, blocks/CU {res.best_bpc}, {res.best_ms:.2f} ms; "
-            f"trials (lds, bpc, ms): {tune['trials']}")
+        log(f"autotune: width {res.best_width}, lds {res.best_lds}, blocks/CU {res.best_bpc}, "
+            f"{res.best_ms:.2f} ms; trials (width, lds, bpc, ms): {tune['trials']}")
     for i in range(args.warmup):
         st = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
         log(f"warmup {i}: {st.segments} segments, {st.render_ms:.1f} ms")
@@ -297,15 +324,30 @@ def main():
 
     # traversal work of the same frame (same seed -> same paths), counted
     # outside the timed region
+    width = ctx.tune_get("trace_width")
+    lds = ctx.tune_get("trace_lds_entries")
+    bpc = ctx.tune_get("trace_blocks_per_cu")
+    stw = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED, count_traversal=True)
+    # node visits / triangle tests of the specified binary traversal: the
+    # binary kernel's counters equal the oracle's per ray
+    ctx.tune("trace_width", 2)
     stc = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED, count_traversal=True)
-    assert stc.segments == st.segments
-    bytes_per_frame = (RAY_BYTES * stc.segments + NODE_BYTES * stc.node_visits
-                       + LEAF_BYTES * stc.tri_tests)
+    ctx.tune("trace_width", width)
+    # the two trees may resolve an exact-t tie differently
+    assert abs(stc.segments - st.segments) <= 1e-6 * st.segments + 8
+    tree_bytes = NODE_BYTES * stc.node_visits + TRI_BYTES * stc.tri_tests
+    trace_bytes = TRACE_SEG_BYTES * st.segments + tree_bytes
+    launches = trace_launches // max(args.steps, 1)
     trace_s_per_frame = trace_ms / 1e3 / args.steps
-    achieved = bytes_per_frame / trace_s_per_frame / 1e9
+    achieved = trace_bytes / trace_s_per_frame / 1e9
+    frame_s = t_total / args.steps
+    frame_with = (FRAME_SEG_BYTES * st.segments + tree_bytes) / frame_s / 1e9
+    frame_without = FRAME_SEG_BYTES * st.segments / frame_s / 1e9
+    traffic = recorded_traffic("trace_pt", width, lds)
     img_np = img.cpu().numpy()
-    log(f"per ray: {stc.node_visits / stc.segments:.1f} node visits, "
-        f"{stc.tri_tests / stc.segments:.2f} triangle tests; image mean {img_np.mean():.4f}")
+    log(f"per ray: {stc.node_visits / stc.segments:.1f} binary node visits, "
+        f"{stc.tri_tests / stc.segments:.2f} triangle tests ({stw.node_visits / stw.segments:.1f} visits of "
+        f"the {width}-wide tree walked); image mean {img_np.mean():.4f}")
 
     value = segs / t_total / 1e6
     guided = None
@@ -361,10 +403,19 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic[0] if traffic else None,
+            "traffic_source": traffic[1] if traffic else None,
             "kernel": "trace_pt",
-            "bytes_per_frame": int(bytes_per_frame),
-            "launches_per_frame": trace_launches // max(args.steps, 1),
+            "algorithmic_bytes_per_frame": int(trace_bytes),
+            "algorithmic_bytes_per_launch": int(trace_bytes / max(launches, 1)),
+            "launches_per_frame": launches,
+            "kernel_ms_per_launch": round(trace_ms / max(trace_launches, 1), 4),
+            "tree_width": width,
+            "binary_node_visits_per_ray": round(stc.node_visits / stc.segments, 3),
+            "binary_tri_tests_per_ray": round(stc.tri_tests / stc.segments, 3),
+            "walked_node_visits_per_ray": round(stw.node_visits / stw.segments, 3),
+            "frame_frac_with_bvh": round(frame_with / HBM_PEAK_GBS, 4),
+            "frame_frac_without_bvh": round(frame_without / HBM_PEAK_GBS, 4),
         },
         "cpu_baseline": cpu,
     }

@@ -68,6 +68,8 @@ int32_t     mrh_profile_get(mrh_context* ctx, const char* kernel, double* total_
  *   "trace_lds_entries"   traversal stack entries kept in LDS per thread
  *                         (0, 8, 12, 16, 24; 0 = whole stack in scratch)
  *   "trace_blocks_per_cu" persistent traversal grid, blocks of 256 per CU
+ *   "trace_width"         traversal tree: 2 = the binary LBVH, 4 = the 4-wide
+ *                         tree collapsed from it (trees deeper than 84 use 2)
  * mrh_tune_get reads a knob back. */
 int32_t     mrh_tune(mrh_context* ctx, const char* knob, int64_t value);
 int32_t     mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value);
@@ -131,6 +133,14 @@ typedef struct {
     const uint32_t* d_right;      /* [n-1] */
     const uint32_t* d_parent;     /* [2n-1]; root holds 0xFFFFFFFF */
     const float*    d_node_aabb;  /* [(2n-1)][6] min.xyz max.xyz */
+    /* 4-wide traversal tree collapsed from the LBVH: node p (an internal
+     * node at even depth) = float[32]: min x[4], min y[4], min z[4],
+     * max x[4], max y[4], max z[4], child code[4] (0x80000000 | sorted
+     * leaf, internal index, 0xFFFFFFFF empty), pad[4]; slots are the
+     * children with internal children replaced by their two children. */
+    const float*    d_nodes4;     /* [n-1][32]; odd-depth entries unused */
+    uint32_t        root4;        /* code of the 4-wide root */
+    uint32_t        reserved;
 } mrh_lbvh_view;
 
 /* d_tris is copied; the caller may free it after the call returns.
@@ -311,15 +321,18 @@ int32_t mrh_render_guided(mrh_context* ctx, mrh_scene* scene, const mrh_camera_d
                           float* d_image, mrh_render_stats* stats);
 
 /* Autotuner (no counterpart in the reference): times `reps` renders of the
- * given frame per candidate launch shape — traversal stack entries in LDS,
- * then blocks per CU of the traversal grid — and leaves the fastest set on
- * the context (mrh_tune).  d_image is scratch for the trial frames. */
+ * given frame per candidate launch shape — traversal tree width, then
+ * traversal stack entries in LDS, then blocks per CU of the traversal grid —
+ * and leaves the fastest set on the context (mrh_tune).  d_image is scratch
+ * for the trial frames. */
 typedef struct {
     uint32_t best_lds, best_bpc;
     double   best_ms;
     uint32_t trials;
     uint32_t trial_lds[32], trial_bpc[32];
     double   trial_ms[32];
+    uint32_t best_width;
+    uint32_t trial_width[32];
 } mrh_tune_result;
 int32_t mrh_autotune(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
                      const mrh_render_params* params, float* d_image, uint32_t reps,

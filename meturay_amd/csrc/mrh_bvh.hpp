@@ -35,6 +35,8 @@ struct mrh_lbvh {
     uint32_t* prim_key = nullptr;   // [n] (optional HitKeys, primitive order)
     uint32_t* meta = nullptr;       // [0] root, [1] max depth, [2..7] extent encodings
     float lo[3] = {}, hi[3] = {};   // scene bounds (host copy)
+    float4* nodes4 = nullptr;       // [n-1][8] 4-wide traversal nodes (even-depth entries used)
+    uint32_t root4 = 0;             // traversal code of the 4-wide root
     std::vector<void*> allocs;
 };
 
@@ -76,6 +78,33 @@ inline int32_t trace_overflow(mrh_context* ctx, uint32_t grid, uint32_t cap, uin
 
 // Traversal entry code of the root
 inline uint32_t root_code(const mrh_lbvh* b) { return (b->n == 1) ? LEAF_BIT : (b->root - b->n); }
+
+// Traversal tree chosen for a launch: the binary LBVH (width 2) or the
+// 4-wide tree collapsed from it (width 4; its stack needs
+// 3 * ceil(depth / 2) + 1 entries, so trees deeper than 84 use width 2).
+struct TreeSel {
+    const float4* nodes;
+    uint32_t root, width, cap;
+};
+
+inline int32_t select_tree(mrh_context* ctx, const mrh_lbvh* b, TreeSel& t)
+{
+    uint32_t w = ctx->trace_width;
+    if (w == 4 && b->max_depth > 84) w = 2;
+    t.width = w;
+    if (w == 4) {
+        t.nodes = b->nodes4;
+        t.root = b->root4;
+        t.cap = (b->max_depth <= 42) ? 64 : 128;
+        return MRH_OK;
+    }
+    if (b->max_depth > 128)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "tree depth %u exceeds the traversal stack (128)", b->max_depth);
+    t.nodes = b->tnodes;
+    t.root = root_code(b);
+    t.cap = (b->max_depth <= 64) ? 64 : 128;
+    return MRH_OK;
+}
 
 struct TraceResult {
     uint32_t leaf;   // sorted leaf index or NO_HIT
@@ -239,6 +268,128 @@ __device__ __forceinline__ TraceResult traverse(Stack& st, const float4* __restr
             next = NO_HIT;
         }
     }
+}
+
+// Closest / any hit over the 4-wide tree.  Node layout (8 float4, 128 B):
+// min x[4], min y[4], min z[4], max x[4], max y[4], max z[4], child code[4]
+// (LEAF_BIT | leaf, internal index, or NO_HIT for an empty slot), pad.  The
+// slots whose boxes the ray enters are ordered by (entry distance, slot)
+// with a 5-comparator network; the nearest is taken next and the others are
+// pushed farthest first, culled on pop once their entry distance exceeds
+// tmax.  Acceptance as traverse().
+template <bool ANY, bool STATS, class Stack>
+__device__ __forceinline__ TraceResult traverse4(Stack& st, const float4* __restrict__ nodes4,
+                                                 const float4* __restrict__ leaves, uint32_t root,
+                                                 uint32_t n_leaves, V3 o, V3 d, float tmin, float tmax)
+{
+    TraceResult r;
+    r.leaf = NO_HIT;
+    r.t = tmax;
+    r.ba = r.bb = 0.0f;
+    r.visits = r.tests = 0;
+    const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int sp = 0;
+    uint32_t next = root;
+    for (;;) {
+        for (;;) {
+            if (next == NO_HIT) {
+                bool found = false;
+                while (sp > 0) {
+                    --sp;
+                    uint32_t nn;
+                    float et;
+                    st.get(sp, nn, et);
+                    if (!(et > r.t)) {
+                        next = nn;
+                        found = true;
+                        break;
+                    }
+                }
+                if (!found) return r;
+            }
+            if (next & LEAF_BIT) {
+                const uint32_t li = next & ~LEAF_BIT;
+                if (li >= n_leaves) {   // unreachable on a well-formed tree
+                    next = NO_HIT;
+                    continue;
+                }
+                const float4 a = leaves[3 * li + 0];
+                const float4 b = leaves[3 * li + 1];
+                const float4 c = leaves[3 * li + 2];
+                float t, ba, bb;
+                if (STATS) r.tests++;
+                if (intersect_tri(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, ba, bb) &&
+                    t >= tmin && t < r.t) {
+                    r.t = t;
+                    r.leaf = li;
+                    r.ba = ba;
+                    r.bb = bb;
+                    if (ANY) return r;
+                }
+                next = NO_HIT;
+                continue;
+            }
+            if (next + 1u >= n_leaves) {   // internal index out of range: unreachable
+                next = NO_HIT;
+                continue;
+            }
+            break;
+        }
+        if (STATS) r.visits++;
+        const float4* nd = nodes4 + 8 * (size_t)next;
+        const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5], cc = nd[6];
+        uint32_t c0 = __float_as_uint(cc.x), c1 = __float_as_uint(cc.y), c2 = __float_as_uint(cc.z),
+                 c3 = __float_as_uint(cc.w);
+        float k0, k1, k2, k3;
+        bool h0 = intersect_box(o, inv, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tmin, r.t, k0) && c0 != NO_HIT;
+        bool h1 = intersect_box(o, inv, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tmin, r.t, k1) && c1 != NO_HIT;
+        bool h2 = intersect_box(o, inv, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tmin, r.t, k2) && c2 != NO_HIT;
+        bool h3 = intersect_box(o, inv, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tmin, r.t, k3) && c3 != NO_HIT;
+        const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+        if (cnt == 0) {
+            next = NO_HIT;
+            continue;
+        }
+        k0 = h0 ? k0 : INFINITY;
+        k1 = h1 ? k1 : INFINITY;
+        k2 = h2 ? k2 : INFINITY;
+        k3 = h3 ? k3 : INFINITY;
+        uint32_t s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+        // (key, slot) lexicographic compare-exchange: the order is unique
+        auto cx = [](float& ka, uint32_t& sa, uint32_t& ca, float& kb, uint32_t& sb, uint32_t& cb) {
+            const bool sw = (ka > kb) || (ka == kb && sa > sb);
+            const float tk = sw ? kb : ka;
+            kb = sw ? ka : kb;
+            ka = tk;
+            const uint32_t ts = sw ? sb : sa;
+            sb = sw ? sa : sb;
+            sa = ts;
+            const uint32_t tc = sw ? cb : ca;
+            cb = sw ? ca : cb;
+            ca = tc;
+        };
+        cx(k0, s0, c0, k1, s1, c1);
+        cx(k2, s2, c2, k3, s3, c3);
+        cx(k0, s0, c0, k2, s2, c2);
+        cx(k1, s1, c1, k3, s3, c3);
+        cx(k1, s1, c1, k2, s2, c2);
+        // push farthest first; the guard keeps a malformed tree from
+        // writing past the stack (the host sizes CAP for the tree depth)
+        if (cnt >= 4 && sp < Stack::CAP) st.put(sp++, c3, k3);
+        if (cnt >= 3 && sp < Stack::CAP) st.put(sp++, c2, k2);
+        if (cnt >= 2 && sp < Stack::CAP) st.put(sp++, c1, k1);
+        next = c0;
+    }
+}
+
+// Either traversal, by tree width
+template <int W, bool ANY, bool STATS, class Stack>
+__device__ __forceinline__ TraceResult traverse_w(Stack& st, const float4* __restrict__ nodes,
+                                                  const float4* __restrict__ leaves, uint32_t root,
+                                                  uint32_t n_leaves, V3 o, V3 d, float tmin, float tmax)
+{
+    if constexpr (W == 4) return traverse4<ANY, STATS>(st, nodes, leaves, root, n_leaves, o, d, tmin, tmax);
+    else return traverse<ANY, STATS>(st, nodes, leaves, root, n_leaves, o, d, tmin, tmax);
 }
 
 }  // namespace mrh

@@ -202,6 +202,11 @@ int32_t mrh_tune(mrh_context* ctx, const char* knob, int64_t value)
         ctx->trace_blocks_per_cu = (uint32_t)value;
         return MRH_OK;
     }
+    if (k == "trace_width") {
+        if (value != 2 && value != 4) return fail(ctx, MRH_ERR_INVALID, "trace_width must be 2 or 4");
+        ctx->trace_width = (uint32_t)value;
+        return MRH_OK;
+    }
     return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
 }
 
@@ -211,6 +216,7 @@ int32_t mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value)
     const std::string k = knob;
     if (k == "trace_lds_entries") *value = ctx->trace_lds_entries;
     else if (k == "trace_blocks_per_cu") *value = ctx->trace_blocks_per_cu;
+    else if (k == "trace_width") *value = ctx->trace_width;
     else return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
     return MRH_OK;
 }

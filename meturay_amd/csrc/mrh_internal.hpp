@@ -51,6 +51,8 @@ struct mrh_context {
     // persistent trace grid
     uint32_t trace_lds_entries = 16;
     uint32_t trace_blocks_per_cu = 8;
+    // traversal tree: 2 = binary LBVH, 4 = 4-wide tree collapsed from it
+    uint32_t trace_width = 4;
 };
 
 namespace mrh {

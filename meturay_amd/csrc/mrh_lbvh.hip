@@ -244,6 +244,54 @@ __global__ __launch_bounds__(256) void k_lbvh_depth(uint32_t n, const uint32_t* 
     if ((threadIdx.x & 63) == 0) atomicMax(&meta[1], best);
 }
 
+// 4-wide traversal nodes: every internal node at even depth (root = 0)
+// gathers its children, an internal child replaced by that child's two
+// children, from the binary nodes' child boxes (layout in traverse4).
+__global__ __launch_bounds__(256) void k_bvh4_collapse(uint32_t n, const float4* __restrict__ tnodes,
+                                                       const uint32_t* __restrict__ parent,
+                                                       float4* __restrict__ nodes4)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p + 1 < n; p += gridDim.x * blockDim.x) {
+        uint32_t d = 0, x = n + p;
+        while (parent[x] != 0xFFFFFFFFu && d < 4096u) {
+            x = parent[x];
+            d++;
+        }
+        if (d & 1u) continue;
+        float* q = (float*)(nodes4 + 8 * (size_t)p);
+        uint32_t k = 0;
+        const float4* tn = tnodes + 4 * (size_t)p;
+        const float4 a0 = tn[0], a1 = tn[1], a2 = tn[2], a3 = tn[3];
+        const float box[2][6] = {{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, {a1.z, a1.w, a2.x, a2.y, a2.z, a2.w}};
+        const uint32_t code[2] = {__float_as_uint(a3.x), __float_as_uint(a3.y)};
+        for (int c = 0; c < 2; c++) {
+            if (code[c] & LEAF_BIT) {
+                for (int a = 0; a < 6; a++) q[a * 4 + k] = box[c][a];
+                q[24 + k] = __uint_as_float(code[c]);
+                k++;
+            } else {
+                const float4* tq = tnodes + 4 * (size_t)code[c];
+                const float4 b0 = tq[0], b1 = tq[1], b2 = tq[2], b3 = tq[3];
+                const float gb[2][6] = {{b0.x, b0.y, b0.z, b0.w, b1.x, b1.y}, {b1.z, b1.w, b2.x, b2.y, b2.z, b2.w}};
+                const uint32_t gc[2] = {__float_as_uint(b3.x), __float_as_uint(b3.y)};
+                for (int g = 0; g < 2; g++) {
+                    for (int a = 0; a < 6; a++) q[a * 4 + k] = gb[g][a];
+                    q[24 + k] = __uint_as_float(gc[g]);
+                    k++;
+                }
+            }
+        }
+        for (; k < 4; k++) {
+            for (int a = 0; a < 3; a++) {
+                q[a * 4 + k] = INFINITY;
+                q[(a + 3) * 4 + k] = -INFINITY;
+            }
+            q[24 + k] = __uint_as_float(0xFFFFFFFFu);
+        }
+        for (int j = 28; j < 32; j++) q[j] = 0.0f;
+    }
+}
+
 template <class T>
 int32_t alloc_into(mrh_lbvh* b, T** p, size_t count)
 {
@@ -320,6 +368,9 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src)
                        b->ranges, b->counters, b->tnodes, b->leaves, b->meta));
     MRH_TRY(launch(ctx, "lbvh_depth", dim3(grid), dim3(256), 0, k_lbvh_depth, n,
                    (const uint32_t*)b->parent, b->meta));
+    if (n > 1)
+        MRH_TRY(launch(ctx, "bvh4_collapse", dim3(grid), dim3(256), 0, k_bvh4_collapse, n,
+                       (const float4*)b->tnodes, (const uint32_t*)b->parent, b->nodes4));
     uint32_t host_meta[9] = {};
     MRH_HIP_CHECK(ctx, hipMemcpyAsync(host_meta, b->meta, sizeof(host_meta), hipMemcpyDeviceToHost,
                                       ctx->stream));
@@ -337,6 +388,7 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src)
         return fail(ctx, MRH_ERR_HIP, "lbvh build: inconsistent hierarchy (flags 0x%x)", host_meta[8]);
     if (b->root >= 2 * n - 1)
         return fail(ctx, MRH_ERR_HIP, "lbvh build produced no root (root = %u, n = %u)", b->root, n);
+    b->root4 = root_code(b);
     return MRH_OK;
 }
 
@@ -367,6 +419,7 @@ int32_t lbvh_create(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t 
         if ((rc = alloc_into(b, &b->ranges, 2 * ni)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->counters, ni)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->tnodes, 4 * ni)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->nodes4, 8 * ni)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->leaves, 3 * (size_t)n)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->meta, 16)) != MRH_OK) break;
         if (d_prim_key) {
@@ -437,6 +490,9 @@ int32_t mrh_lbvh_view_get(const mrh_lbvh* b, mrh_lbvh_view* v)
     v->d_right = b->right;
     v->d_parent = b->parent;
     v->d_node_aabb = b->node_aabb;
+    v->d_nodes4 = (const float*)b->nodes4;
+    v->root4 = b->root4;
+    v->reserved = 0;
     return MRH_OK;
 }
 

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h,
 // [66, 68) u64 triangle tests
 constexpr uint32_t COUNTS_WORDS = 128;
 
-template <int CAP, bool STATS, int L>
+template <int W, int CAP, bool STATS, int L>
 __global__ __launch_bounds__(256) void k_trace_pt(const float4* __restrict__ tnodes,
                                                   const float4* __restrict__ leaves, uint32_t root, uint32_t n_leaves,
                                                   PathBuf in, uint32_t* __restrict__ counts,
@@ -65,10 +65,10 @@ __global__ __launch_bounds__(256) void k_trace_pt(const float4* __restrict__ tno
         TraceResult r;
         if constexpr (L > 0) {
             LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
-            r = traverse<false, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+            r = traverse_w<W, false, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
         } else {
             PrivateStack<CAP> st;
-            r = traverse<false, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+            r = traverse_w<W, false, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
         }
         hit.leaf[i] = r.leaf;
         hit.t[i] = r.t;
@@ -92,19 +92,20 @@ __global__ __launch_bounds__(256) void k_trace_pt(const float4* __restrict__ tno
     }
 }
 
-template <int CAP, bool STATS>
-int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, const PathBuf& in,
-                        uint32_t* counts, uint32_t bounce, float tmin, float tmax, const HitBuf& hit)
+template <int W, int CAP, bool STATS>
+int32_t launch_trace_pt_w(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, const TreeSel& ts,
+                          const PathBuf& in, uint32_t* counts, uint32_t bounce, float tmin, float tmax,
+                          const HitBuf& hit)
 {
-    const uint32_t root = root_code(b);
+    const uint32_t root = ts.root;
     const uint32_t lds = ctx->trace_lds_entries;
     uint2* ovf = nullptr;
     MRH_TRY(trace_overflow(ctx, grid, CAP, lds, &ovf));
     const char* name = STATS ? "trace_pt_stats" : "trace_pt";
-    const float4* tn = (const float4*)b->tnodes;
+    const float4* tn = ts.nodes;
     const float4* lv = (const float4*)b->leaves;
-#define MRH_TRACE_PT_L(LV)                                                                         \
-    return launch(ctx, name, dim3(grid), dim3(256), 0, k_trace_pt<CAP, STATS, LV>, tn, lv, root, b->n, \
+#define MRH_TRACE_PT_L(LV)                                                                            \
+    return launch(ctx, name, dim3(grid), dim3(256), 0, k_trace_pt<W, CAP, STATS, LV>, tn, lv, root, b->n, \
                   in, counts, bounce, tmin, tmax, hit, ovf)
     switch (lds) {
     case 0: MRH_TRACE_PT_L(0);
@@ -115,6 +116,20 @@ int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, cons
     default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24");
     }
 #undef MRH_TRACE_PT_L
+}
+
+template <bool STATS>
+int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, const PathBuf& in, uint32_t* counts,
+                        uint32_t bounce, float tmin, float tmax, const HitBuf& hit)
+{
+    TreeSel ts;
+    MRH_TRY(select_tree(ctx, b, ts));
+    if (ts.width == 4) {
+        if (ts.cap == 64) return launch_trace_pt_w<4, 64, STATS>(ctx, grid, b, ts, in, counts, bounce, tmin, tmax, hit);
+        return launch_trace_pt_w<4, 128, STATS>(ctx, grid, b, ts, in, counts, bounce, tmin, tmax, hit);
+    }
+    if (ts.cap == 64) return launch_trace_pt_w<2, 64, STATS>(ctx, grid, b, ts, in, counts, bounce, tmin, tmax, hit);
+    return launch_trace_pt_w<2, 128, STATS>(ctx, grid, b, ts, in, counts, bounce, tmin, tmax, hit);
 }
 
 // One launch over all partitions of a bounce (see shade_ray)
@@ -319,11 +334,9 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     const uint32_t tgrid = trace_grid(ctx, f.total);
     const float4* lv = (const float4*)b->leaves;
     if (f.count_traversal) {
-        if (b->max_depth <= 64) MRH_TRY((launch_trace_pt<64, true>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit)));
-        else MRH_TRY((launch_trace_pt<128, true>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit)));
+        MRH_TRY(launch_trace_pt<true>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit));
     } else {
-        if (b->max_depth <= 64) MRH_TRY((launch_trace_pt<64, false>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit)));
-        else MRH_TRY((launch_trace_pt<128, false>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit)));
+        MRH_TRY(launch_trace_pt<false>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit));
     }
     SortArgs sa;
     sa.keys_in = f.hit.key;

@@ -38,8 +38,9 @@ constexpr uint32_t RES_LIVE = 1, RES_SEG = 1 + SERVE_MAX_SLOTS, RES_WORDS = 1 + 
 
 // Device view of one admitted request
 struct SlotDev {
-    const float4* tnodes;
-    uint32_t root, n_leaves;
+    const float4* tnodes;   // binary traversal nodes
+    const float4* nodes4;   // 4-wide traversal nodes
+    uint32_t root, root4, n_leaves, pad2;
     ShadeReq q;
     GuideArgs ga;
     Cam cam;
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(256) void k_serve_raygen(const StepHdr* __restrict_
 
 // Closest hit of every queued ray in its own request's scene; camera rays
 // (depth 1) are clipped to the request's near/far planes.
-template <int CAP, int L>
+template <int W, int CAP, int L>
 __global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__ hdr,
                                                      const SlotDev* __restrict__ slots, PathBuf in,
                                                      const uint32_t* __restrict__ in_slot, HitBuf hit,
@@ -110,13 +111,15 @@ __global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__
         const float tmin = cam_ray ? sd.cam.near_d : 0.0f;
         const float tmax = cam_ray ? sd.cam.far_d : FLT_MAX;
         const float4* leaves = sd.q.leaves;
+        const float4* nodes = (W == 4) ? sd.nodes4 : sd.tnodes;
+        const uint32_t root = (W == 4) ? sd.root4 : sd.root;
         TraceResult r;
         if constexpr (L > 0) {
             LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
-            r = traverse<false, false>(st, sd.tnodes, leaves, sd.root, sd.n_leaves, o, d, tmin, tmax);
+            r = traverse_w<W, false, false>(st, nodes, leaves, root, sd.n_leaves, o, d, tmin, tmax);
         } else {
             PrivateStack<CAP> st;
-            r = traverse<false, false>(st, sd.tnodes, leaves, sd.root, sd.n_leaves, o, d, tmin, tmax);
+            r = traverse_w<W, false, false>(st, nodes, leaves, root, sd.n_leaves, o, d, tmin, tmax);
         }
         hit.leaf[i] = r.leaf;
         hit.t[i] = r.t;
@@ -230,8 +233,8 @@ struct mrh_server {
     // graphs per parity, with what they were captured against
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     void* graph_scratch[2][SLOT_COUNT] = {};
-    uint32_t graph_cap128[2] = {}, graph_lds[2] = {}, graph_bpc[2] = {};
-    bool cap128 = false;
+    uint32_t graph_cap[2] = {}, graph_w[2] = {}, graph_lds[2] = {}, graph_bpc[2] = {};
+    uint32_t max_tree_depth = 0;    // deepest LBVH among submitted requests
 };
 
 namespace {
@@ -288,7 +291,9 @@ void fill_slot(mrh_server* s, ServeReq* q)
     const mrh_lbvh* b = sc->bvh;
     memset(&d, 0, sizeof(d));
     d.tnodes = (const float4*)b->tnodes;
+    d.nodes4 = (const float4*)b->nodes4;
     d.root = root_code(b);
+    d.root4 = b->root4;
     d.n_leaves = b->n;
     d.q.mats = Materials{sc->lambert, sc->metal, sc->dielectric, sc->emissive};
     d.q.leaves = (const float4*)b->leaves;
@@ -398,6 +403,15 @@ int32_t iteration_done(mrh_server* s, ServeReq* q)
     return MRH_OK;
 }
 
+// Tree width and stack capacity that fit every submitted scene (see
+// select_tree: 4-wide needs 3 * ceil(depth / 2) + 1 stack entries)
+void tree_shape(const mrh_server* s, uint32_t& w, uint32_t& cap)
+{
+    const uint32_t d = s->max_tree_depth;
+    w = (s->ctx->trace_width == 4 && d <= 84) ? 4u : 2u;
+    cap = (w == 4) ? (d <= 42 ? 64u : 128u) : (d <= 64 ? 64u : 128u);
+}
+
 // The launches of one step over queue parity `par` (captured or eager)
 int32_t step_launches(mrh_server* s, uint32_t par)
 {
@@ -411,25 +425,30 @@ int32_t step_launches(mrh_server* s, uint32_t par)
                    (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, s->slot_of[par]));
     const uint32_t tgrid = trace_grid(ctx, s->cap);
     const uint32_t lds = ctx->trace_lds_entries;
-    const int cap = s->cap128 ? 128 : 64;
+    uint32_t w, cap;
+    tree_shape(s, w, cap);
     uint2* ovf = nullptr;
     MRH_TRY(trace_overflow(ctx, tgrid, cap, lds, &ovf));
-#define MRH_SERVE_TRACE(C, LV)                                                                                \
-    MRH_TRY(launch(ctx, "serve_trace", dim3(tgrid), dim3(256), 0, k_serve_trace<C, LV>, (const StepHdr*)s->d_hdr, \
-                   (const SlotDev*)s->d_slots, in, in_slot, s->hit, ovf))
-#define MRH_SERVE_TRACE_L(C)                                                     \
+#define MRH_SERVE_TRACE(W, C, LV)                                                                             \
+    MRH_TRY(launch(ctx, "serve_trace", dim3(tgrid), dim3(256), 0, k_serve_trace<W, C, LV>,                     \
+                   (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, in_slot, s->hit, ovf))
+#define MRH_SERVE_TRACE_L(W, C)                                                  \
     switch (lds) {                                                               \
-    case 0: MRH_SERVE_TRACE(C, 0); break;                                        \
-    case 8: MRH_SERVE_TRACE(C, 8); break;                                        \
-    case 12: MRH_SERVE_TRACE(C, 12); break;                                      \
-    case 16: MRH_SERVE_TRACE(C, 16); break;                                      \
-    case 24: MRH_SERVE_TRACE(C, 24); break;                                      \
+    case 0: MRH_SERVE_TRACE(W, C, 0); break;                                     \
+    case 8: MRH_SERVE_TRACE(W, C, 8); break;                                     \
+    case 12: MRH_SERVE_TRACE(W, C, 12); break;                                   \
+    case 16: MRH_SERVE_TRACE(W, C, 16); break;                                   \
+    case 24: MRH_SERVE_TRACE(W, C, 24); break;                                   \
     default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24"); \
     }
-    if (cap == 128) {
-        MRH_SERVE_TRACE_L(128)
+    if (w == 4 && cap == 64) {
+        MRH_SERVE_TRACE_L(4, 64)
+    } else if (w == 4) {
+        MRH_SERVE_TRACE_L(4, 128)
+    } else if (cap == 64) {
+        MRH_SERVE_TRACE_L(2, 64)
     } else {
-        MRH_SERVE_TRACE_L(64)
+        MRH_SERVE_TRACE_L(2, 128)
     }
 #undef MRH_SERVE_TRACE_L
 #undef MRH_SERVE_TRACE
@@ -453,7 +472,9 @@ bool graph_valid(mrh_server* s, uint32_t par)
     if (!s->exec[par]) return false;
     for (int k = 0; k < SLOT_COUNT; k++)
         if (s->graph_scratch[par][k] != s->ctx->slot_ptr[k]) return false;
-    return s->graph_cap128[par] == (uint32_t)s->cap128 && s->graph_lds[par] == s->ctx->trace_lds_entries &&
+    uint32_t w, cap;
+    tree_shape(s, w, cap);
+    return s->graph_cap[par] == cap && s->graph_w[par] == w && s->graph_lds[par] == s->ctx->trace_lds_entries &&
            s->graph_bpc[par] == s->ctx->trace_blocks_per_cu;
 }
 
@@ -485,7 +506,7 @@ int32_t run_step(mrh_server* s, uint32_t par)
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail(ctx, MRH_ERR_HIP, "step graph instantiate: %s", hipGetErrorString(e));
         for (int k = 0; k < SLOT_COUNT; k++) s->graph_scratch[par][k] = ctx->slot_ptr[k];
-        s->graph_cap128[par] = s->cap128;
+        tree_shape(s, s->graph_w[par], s->graph_cap[par]);
         s->graph_lds[par] = ctx->trace_lds_entries;
         s->graph_bpc[par] = ctx->trace_blocks_per_cu;
         // the eager step above already did this step's work
@@ -611,6 +632,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
     mrh_context* ctx = s->ctx;
     ctx->trace_lds_entries = user->trace_lds_entries;
     ctx->trace_blocks_per_cu = user->trace_blocks_per_cu;
+    ctx->trace_width = user->trace_width;
     auto take = [&](size_t bytes) -> void* {
         void* q = nullptr;
         if (rc == MRH_OK) rc = dmalloc(ctx, &q, bytes);
@@ -693,7 +715,7 @@ int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
         return fail(ctx, MRH_ERR_BUDGET, "submit: request needs %llu bytes, budget is %llu",
                     (unsigned long long)q->charge, (unsigned long long)s->p.hbm_budget);
     }
-    if (r->scene->bvh->max_depth > 64) s->cap128 = true;
+    s->max_tree_depth = std::max(s->max_tree_depth, r->scene->bvh->max_depth);
     q->ticket = s->next_ticket++;
     q->t_submit = now_s();
     s->queue.push_back(q);

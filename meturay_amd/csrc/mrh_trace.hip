@@ -8,7 +8,7 @@
 namespace mrh {
 namespace {
 
-template <bool ANY, bool STATS, int CAP, int L>
+template <int W, bool ANY, bool STATS, int CAP, int L>
 __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ tnodes,
                                                const float4* __restrict__ leaves, uint32_t root,
                                                uint32_t n_leaves, const float* __restrict__ rays,
@@ -29,10 +29,10 @@ __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ tnodes
         TraceResult r;
         if constexpr (L > 0) {
             LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
-            r = traverse<ANY, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+            r = traverse_w<W, ANY, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
         } else {
             PrivateStack<CAP> st;
-            r = traverse<ANY, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
+            r = traverse_w<W, ANY, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
         }
         if (ANY) {
             out_prim[i] = (r.leaf != NO_HIT) ? 1u : 0u;
@@ -51,19 +51,18 @@ __global__ __launch_bounds__(256) void k_trace(const float4* __restrict__ tnodes
     }
 }
 
-template <bool ANY, bool STATS, int CAP>
-int32_t trace_cap(mrh_context* ctx, const mrh_lbvh* b, const float* rays, uint32_t n, uint32_t pitch,
-                  float tmin, uint32_t* prim, float* t, float* bary, uint32_t* stats)
+template <int W, bool ANY, bool STATS, int CAP>
+int32_t trace_cap(mrh_context* ctx, const mrh_lbvh* b, const TreeSel& ts, const float* rays, uint32_t n,
+                  uint32_t pitch, float tmin, uint32_t* prim, float* t, float* bary, uint32_t* stats)
 {
-    const uint32_t root = root_code(b);
     const uint32_t grid = trace_grid(ctx, n);
     const char* name = ANY ? "trace_any" : (STATS ? "trace_closest_stats" : "trace_closest");
     const uint32_t lds = ctx->trace_lds_entries;
     uint2* ovf = nullptr;
     MRH_TRY(trace_overflow(ctx, grid, CAP, lds, &ovf));
-#define MRH_TRACE_L(LV)                                                                                  \
-    return launch(ctx, name, dim3(grid), dim3(256), 0, k_trace<ANY, STATS, CAP, LV>, b->tnodes, b->leaves, \
-                  root, b->n, rays, n, pitch, tmin, prim, t, bary, stats, ovf)
+#define MRH_TRACE_L(LV)                                                                                     \
+    return launch(ctx, name, dim3(grid), dim3(256), 0, k_trace<W, ANY, STATS, CAP, LV>, ts.nodes, b->leaves, \
+                  ts.root, b->n, rays, n, pitch, tmin, prim, t, bary, stats, ovf)
     switch (lds) {
     case 0: MRH_TRACE_L(0);
     case 8: MRH_TRACE_L(8);
@@ -79,10 +78,14 @@ template <bool ANY, bool STATS>
 int32_t trace_launch(mrh_context* ctx, const mrh_lbvh* b, const float* rays, uint32_t n, uint32_t pitch,
                      float tmin, uint32_t* prim, float* t, float* bary, uint32_t* stats)
 {
-    if (b->max_depth <= 64) return trace_cap<ANY, STATS, 64>(ctx, b, rays, n, pitch, tmin, prim, t, bary, stats);
-    if (b->max_depth <= 128) return trace_cap<ANY, STATS, 128>(ctx, b, rays, n, pitch, tmin, prim, t, bary, stats);
-    return fail(ctx, MRH_ERR_UNSUPPORTED, "trace: tree depth %u exceeds the traversal stack (128)",
-                b->max_depth);
+    TreeSel ts;
+    MRH_TRY(select_tree(ctx, b, ts));
+    if (ts.width == 4) {
+        if (ts.cap == 64) return trace_cap<4, ANY, STATS, 64>(ctx, b, ts, rays, n, pitch, tmin, prim, t, bary, stats);
+        return trace_cap<4, ANY, STATS, 128>(ctx, b, ts, rays, n, pitch, tmin, prim, t, bary, stats);
+    }
+    if (ts.cap == 64) return trace_cap<2, ANY, STATS, 64>(ctx, b, ts, rays, n, pitch, tmin, prim, t, bary, stats);
+    return trace_cap<2, ANY, STATS, 128>(ctx, b, ts, rays, n, pitch, tmin, prim, t, bary, stats);
 }
 
 }  // namespace

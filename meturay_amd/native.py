@@ -40,7 +40,8 @@ class MrhError(RuntimeError):
 class LbvhView(C.Structure):
     _fields_ = [("n", _U32), ("bits", _U32), ("root", _U32), ("max_depth", _U32),
                 ("d_sorted_codes", _P), ("d_sorted_idx", _P), ("d_left", _P), ("d_right", _P),
-                ("d_parent", _P), ("d_node_aabb", _P)]
+                ("d_parent", _P), ("d_node_aabb", _P), ("d_nodes4", _P), ("root4", _U32),
+                ("reserved", _U32)]
 
 
 class CameraDesc(C.Structure):
@@ -60,7 +61,8 @@ class RenderStats(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("best_lds", _U32), ("best_bpc", _U32), ("best_ms", C.c_double), ("trials", _U32),
-                ("trial_lds", _U32 * 32), ("trial_bpc", _U32 * 32), ("trial_ms", C.c_double * 32)]
+                ("trial_lds", _U32 * 32), ("trial_bpc", _U32 * 32), ("trial_ms", C.c_double * 32),
+                ("best_width", _U32), ("trial_width", _U32 * 32)]
 
 
 class SvoView(C.Structure):

@@ -276,6 +276,10 @@ uint32_t om_trace_one(const float* tris, uint32_t n, const uint32_t* sorted_idx,
                       float* tmax_io, float bary_out[2], int any_hit,
                       uint32_t* visits, uint32_t* tests);
 
+uint32_t om_trace4_one(const float* tris, uint32_t n, const uint32_t* sorted_idx, const float* nodes4,
+                       uint32_t root4, const float o[3], const float d[3], float tmin, float* tmax_io,
+                       float bary_out[2], int any_hit, uint32_t* visits, uint32_t* tests);
+
 /* guide evaluation shared by the guided path tracer (om_guide.c) */
 struct om_svo;
 float    om_guide_total_v(const struct om_svo* s, uint32_t node, om_v3 n);

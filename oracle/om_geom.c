@@ -458,6 +458,179 @@ void om_trace_any(const float* tris, uint32_t n, const uint32_t* sorted_idx,
     }
 }
 
+/* ------------------------------------------------------------------------
+ * 4-wide traversal tree collapsed from the LBVH.  Every internal node at even
+ * depth (root = 0) becomes a 4-wide node whose children are its binary
+ * children, each internal child replaced by that child's two children:
+ * slots [c0 or c0.left, c0.right, c1 or c1.left, c1.right], compacted in
+ * that order, empty slots last.  Layout per node (32 floats): min x[4],
+ * min y[4], min z[4], max x[4], max y[4], max z[4], child code[4] (uint
+ * bits: LEAF_BIT | sorted leaf, internal index, or 0xFFFFFFFF = empty),
+ * 4 pad.  Nodes are indexed by their binary internal index (odd-depth
+ * entries are unused). */
+#define OM_LEAF_BIT 0x80000000u
+
+static uint32_t code_of(uint32_t g, uint32_t n) { return g < n ? (OM_LEAF_BIT | g) : (g - n); }
+
+uint32_t om_bvh4_build(uint32_t n, const uint32_t* left, const uint32_t* right, const float* node_aabb,
+                       uint32_t root, float* nodes4)
+{
+    if (n <= 1) return OM_LEAF_BIT;
+    /* depth of every internal node, top-down from the root */
+    uint8_t* depth = (uint8_t*)calloc(n - 1, 1);
+    uint32_t* todo = (uint32_t*)malloc((size_t)n * 4);
+    int top = 0;
+    todo[top++] = root - n;
+    depth[root - n] = 0;
+    while (top) {
+        uint32_t p = todo[--top];
+        uint32_t cs[2] = {left[p], right[p]};
+        for (int k = 0; k < 2; k++)
+            if (cs[k] >= n) {
+                depth[cs[k] - n] = (uint8_t)(depth[p] + 1);
+                todo[top++] = cs[k] - n;
+            }
+    }
+    for (uint32_t p = 0; p + 1 < n; p++) {
+        if (depth[p] & 1u) continue;
+        float* q = nodes4 + (size_t)p * 32;
+        uint32_t code[4];
+        int k = 0;
+        uint32_t cs[2] = {left[p], right[p]};
+        for (int c = 0; c < 2; c++) {
+            uint32_t kids[2];
+            int nk = 0;
+            if (cs[c] < n) kids[nk++] = cs[c];
+            else { kids[nk++] = left[cs[c] - n]; kids[nk++] = right[cs[c] - n]; }
+            for (int j = 0; j < nk; j++, k++) {
+                const float* b = node_aabb + (size_t)kids[j] * 6;
+                for (int a = 0; a < 6; a++) q[a * 4 + k] = b[a];
+                code[k] = code_of(kids[j], n);
+            }
+        }
+        for (; k < 4; k++) {
+            for (int a = 0; a < 3; a++) { q[a * 4 + k] = INFINITY; q[(a + 3) * 4 + k] = -INFINITY; }
+            code[k] = OM_INVALID;
+        }
+        for (int j = 0; j < 4; j++) q[24 + j] = om_u2f(code[j]);
+        for (int j = 0; j < 4; j++) q[28 + j] = 0.0f;
+    }
+    free(depth);
+    free(todo);
+    return root - n;
+}
+
+/* Closest / any hit over the 4-wide tree.  At a node the occupied slots
+ * whose boxes the ray enters (within [tmin, current tmax]) are ordered by
+ * (entry distance, slot); the nearest is taken, the others pushed farthest
+ * first with their entry distance and culled on pop when it exceeds tmax.
+ * Acceptance as om_trace_one. */
+uint32_t om_trace4_one(const float* tris, uint32_t n, const uint32_t* sorted_idx, const float* nodes4,
+                       uint32_t root4, const float o[3], const float d[3], float tmin, float* tmax_io,
+                       float bary_out[2], int any_hit, uint32_t* visits, uint32_t* tests)
+{
+    uint32_t hit = OM_INVALID;
+    float tmax = *tmax_io;
+    float inv_d[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    uint32_t nv = 0, nt = 0;
+    om_stack_entry stack[256];
+    int sp = 0;
+    uint32_t next = root4;
+    if (n == 0) goto done4;
+    for (;;) {
+        for (;;) {
+            if (next == OM_INVALID) {
+                for (;;) {
+                    if (sp == 0) goto done4;
+                    om_stack_entry e = stack[--sp];
+                    if (!(e.t > tmax)) { next = e.node; break; }
+                }
+            }
+            if (next & OM_LEAF_BIT) {
+                uint32_t prim = sorted_idx[next & ~OM_LEAF_BIT];
+                float t, b[3];
+                nt++;
+                if (om_intersect_triangle(o, d, tris + (size_t)prim * 9, &t, b) && t >= tmin && t < tmax) {
+                    tmax = t;
+                    hit = prim;
+                    bary_out[0] = b[0];
+                    bary_out[1] = b[1];
+                    if (any_hit) goto done4;
+                }
+                next = OM_INVALID;
+                continue;
+            }
+            break;
+        }
+        nv++;
+        const float* q = nodes4 + (size_t)next * 32;
+        float key[4];
+        uint32_t slot[4], code[4];
+        int cnt = 0;
+        for (int k = 0; k < 4; k++) {
+            uint32_t c = om_f2u(q[24 + k]);
+            float box[6] = {q[k], q[4 + k], q[8 + k], q[12 + k], q[16 + k], q[20 + k]};
+            float e;
+            int h = (c != OM_INVALID) && om_intersect_aabb(o, inv_d, box, tmin, tmax, &e);
+            key[k] = h ? e : INFINITY;
+            slot[k] = (uint32_t)k;
+            code[k] = c;
+            cnt += h;
+        }
+        /* sorting network (0,1)(2,3)(0,2)(1,3)(1,2) on (key, slot) */
+        static const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+        for (int s = 0; s < 5; s++) {
+            int a = net[s][0], b = net[s][1];
+            if (key[a] > key[b] || (key[a] == key[b] && slot[a] > slot[b])) {
+                float tk = key[a]; key[a] = key[b]; key[b] = tk;
+                uint32_t ts = slot[a]; slot[a] = slot[b]; slot[b] = ts;
+                uint32_t tc = code[a]; code[a] = code[b]; code[b] = tc;
+            }
+        }
+        if (cnt == 0) { next = OM_INVALID; continue; }
+        for (int k = cnt - 1; k >= 1; k--) { stack[sp].node = code[k]; stack[sp].t = key[k]; sp++; }
+        next = code[0];
+    }
+done4:
+    *tmax_io = tmax;
+    if (visits) *visits = nv;
+    if (tests) *tests = nt;
+    return hit;
+}
+
+void om_trace4_closest(const float* tris, uint32_t n, const uint32_t* sorted_idx, const float* nodes4,
+                       uint32_t root4, const float* rays, uint32_t nrays, uint32_t pitch, float tmin,
+                       uint32_t* hit_prim, float* hit_t, float* hit_bary, uint32_t* stats)
+{
+    for (uint32_t i = 0; i < nrays; i++) {
+        float o[3] = {rays[i], rays[pitch + i], rays[2 * (size_t)pitch + i]};
+        float d[3] = {rays[3 * (size_t)pitch + i], rays[4 * (size_t)pitch + i], rays[5 * (size_t)pitch + i]};
+        float tmax = rays[6 * (size_t)pitch + i];
+        float b[2] = {0.0f, 0.0f};
+        uint32_t nv, nt;
+        uint32_t h = om_trace4_one(tris, n, sorted_idx, nodes4, root4, o, d, tmin, &tmax, b, 0, &nv, &nt);
+        hit_prim[i] = h;
+        hit_t[i] = (h == OM_INVALID) ? 0.0f : tmax;
+        hit_bary[i] = (h == OM_INVALID) ? 0.0f : b[0];
+        hit_bary[(size_t)nrays + i] = (h == OM_INVALID) ? 0.0f : b[1];
+        if (stats) { stats[2 * (size_t)i] = nv; stats[2 * (size_t)i + 1] = nt; }
+    }
+}
+
+void om_trace4_any(const float* tris, uint32_t n, const uint32_t* sorted_idx, const float* nodes4,
+                   uint32_t root4, const float* rays, uint32_t nrays, uint32_t pitch, float tmin,
+                   uint32_t* hit_any)
+{
+    for (uint32_t i = 0; i < nrays; i++) {
+        float o[3] = {rays[i], rays[pitch + i], rays[2 * (size_t)pitch + i]};
+        float d[3] = {rays[3 * (size_t)pitch + i], rays[4 * (size_t)pitch + i], rays[5 * (size_t)pitch + i]};
+        float tmax = rays[6 * (size_t)pitch + i];
+        float b[2];
+        uint32_t h = om_trace4_one(tris, n, sorted_idx, nodes4, root4, o, d, tmin, &tmax, b, 1, NULL, NULL);
+        hit_any[i] = (h != OM_INVALID) ? 1u : 0u;
+    }
+}
+
 /* Brute force over all primitives in index order (the semantics of
  * KCIntersectLinear, GPUAcceleratorLinearKC.cuh:24-108). */
 void om_trace_brute(const float* tris, uint32_t n, const float* rays, uint32_t nrays,

@@ -534,8 +534,11 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         (*segments)++;
         float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
         float b[2];
-        uint32_t prim = om_trace_one(sc->tris, sc->n, sc->sorted_idx, sc->left, sc->right,
-                                     sc->node_aabb, sc->root, oo, dd, tmin, &tmax, b, 0, NULL, NULL);
+        uint32_t prim = sc->nodes4
+                            ? om_trace4_one(sc->tris, sc->n, sc->sorted_idx, sc->nodes4, sc->root4, oo, dd, tmin,
+                                            &tmax, b, 0, NULL, NULL)
+                            : om_trace_one(sc->tris, sc->n, sc->sorted_idx, sc->left, sc->right,
+                                           sc->node_aabb, sc->root, oo, dd, tmin, &tmax, b, 0, NULL, NULL);
         if (prim == OM_INVALID) break;
         uint32_t key = sc->prim_key[prim];
         uint32_t batch = key >> 24, id = key & 0xFFFFFFu;

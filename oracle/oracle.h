@@ -75,6 +75,16 @@ void om_trace_any(const float* tris, uint32_t n, const uint32_t* sorted_idx,
                   const uint32_t* left, const uint32_t* right, const float* node_aabb,
                   uint32_t root, const float* rays, uint32_t nrays, uint32_t pitch,
                   float tmin, uint32_t* hit_any);
+/* 4-wide traversal tree collapsed from the LBVH (om_geom.c); nodes4 is
+ * float[n-1][32]; returns the root code. */
+uint32_t om_bvh4_build(uint32_t n, const uint32_t* left, const uint32_t* right, const float* node_aabb,
+                       uint32_t root, float* nodes4);
+void om_trace4_closest(const float* tris, uint32_t n, const uint32_t* sorted_idx, const float* nodes4,
+                       uint32_t root4, const float* rays, uint32_t nrays, uint32_t pitch, float tmin,
+                       uint32_t* hit_prim, float* hit_t, float* hit_bary, uint32_t* stats);
+void om_trace4_any(const float* tris, uint32_t n, const uint32_t* sorted_idx, const float* nodes4,
+                   uint32_t root4, const float* rays, uint32_t nrays, uint32_t pitch, float tmin,
+                   uint32_t* hit_any);
 void om_trace_brute(const float* tris, uint32_t n, const float* rays, uint32_t nrays,
                     uint32_t pitch, float tmin, uint32_t* hit_prim, float* hit_t,
                     float* hit_bary);
@@ -123,6 +133,9 @@ typedef struct {
     const uint32_t* right;
     const float*    node_aabb;
     uint32_t        root;
+    /* 4-wide traversal tree (om_bvh4_build); NULL = walk the binary LBVH */
+    const float*    nodes4;
+    uint32_t        root4;
 } om_scene;
 
 typedef struct {

@@ -49,7 +49,7 @@ class OmScene(C.Structure):
     _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
                 ("lambert", _P), ("metal", _P), ("dielectric", _P), ("emissive", _P),
                 ("sorted_idx", _P), ("left", _P), ("right", _P), ("node_aabb", _P),
-                ("root", _U32)]
+                ("root", _U32), ("nodes4", _P), ("root4", _U32)]
 
 
 class OmRenderParams(C.Structure):
@@ -105,6 +105,10 @@ def lib():
                                        _P, _P, _P, _P]
         L.om_trace_any.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F, _P]
         L.om_trace_brute.argtypes = [_P, _U32, _P, _U32, _U32, _F, _P, _P, _P]
+        L.om_bvh4_build.restype = _U32
+        L.om_bvh4_build.argtypes = [_U32, _P, _P, _P, _U32, _P]
+        L.om_trace4_closest.argtypes = [_P, _U32, _P, _P, _U32, _P, _U32, _U32, _F, _P, _P, _P, _P]
+        L.om_trace4_any.argtypes = [_P, _U32, _P, _P, _U32, _P, _U32, _U32, _F, _P]
         L.om_sincos_2pi_f.argtypes = [_F, _P, _P]
         L.om_hemi_uniform.argtypes = [_F, _F, _P, _P]
         L.om_reflect.argtypes = [_P, _P, _P]
@@ -249,29 +253,50 @@ class LBVH:
         if rc != 0:
             raise RuntimeError(f"om_lbvh_build failed ({rc})")
         self.root = root.value
+        self._nodes4 = None
+        self.root4 = None
 
-    def trace(self, rays: np.ndarray, tmin: float = 0.0, stats: bool = False):
-        """rays: float32 [7, m] planes (ox oy oz dx dy dz tmax)."""
+    def nodes4(self) -> np.ndarray:
+        """The 4-wide traversal tree collapsed from this LBVH (float32 [n-1, 32])."""
+        if self._nodes4 is None:
+            self._nodes4 = np.zeros((max(self.n - 1, 1), 32), dtype=np.float32)
+            self.root4 = int(lib().om_bvh4_build(self.n, _ptr(self.left), _ptr(self.right),
+                                                 _ptr(self.node_aabb), self.root, _ptr(self._nodes4)))
+        return self._nodes4
+
+    def trace(self, rays: np.ndarray, tmin: float = 0.0, stats: bool = False, width: int = 4):
+        """rays: float32 [7, m] planes (ox oy oz dx dy dz tmax); width 2 walks
+        the binary LBVH, 4 the collapsed 4-wide tree."""
         rays = np.ascontiguousarray(rays, dtype=np.float32)
         m = rays.shape[1]
         prim = np.zeros(m, dtype=np.uint32)
         t = np.zeros(m, dtype=np.float32)
         bary = np.zeros((2, m), dtype=np.float32)
         st = np.zeros((m, 2), dtype=np.uint32) if stats else None
-        lib().om_trace_closest(_ptr(self.tris), self.n, _ptr(self.sorted_idx), _ptr(self.left),
-                               _ptr(self.right), _ptr(self.node_aabb), self.root, _ptr(rays), m,
-                               m, tmin, _ptr(prim), _ptr(t), _ptr(bary), _ptr(st))
+        if width == 4:
+            nodes4 = self.nodes4()
+            lib().om_trace4_closest(_ptr(self.tris), self.n, _ptr(self.sorted_idx), _ptr(nodes4), self.root4,
+                                    _ptr(rays), m, m, tmin, _ptr(prim), _ptr(t), _ptr(bary), _ptr(st))
+        else:
+            lib().om_trace_closest(_ptr(self.tris), self.n, _ptr(self.sorted_idx), _ptr(self.left),
+                                   _ptr(self.right), _ptr(self.node_aabb), self.root, _ptr(rays), m,
+                                   m, tmin, _ptr(prim), _ptr(t), _ptr(bary), _ptr(st))
         if stats:
             return prim, t, bary, st
         return prim, t, bary
 
-    def trace_any(self, rays: np.ndarray, tmin: float = 0.0):
+    def trace_any(self, rays: np.ndarray, tmin: float = 0.0, width: int = 4):
         rays = np.ascontiguousarray(rays, dtype=np.float32)
         m = rays.shape[1]
         out = np.zeros(m, dtype=np.uint32)
-        lib().om_trace_any(_ptr(self.tris), self.n, _ptr(self.sorted_idx), _ptr(self.left),
-                           _ptr(self.right), _ptr(self.node_aabb), self.root, _ptr(rays), m, m,
-                           tmin, _ptr(out))
+        if width == 4:
+            nodes4 = self.nodes4()
+            lib().om_trace4_any(_ptr(self.tris), self.n, _ptr(self.sorted_idx), _ptr(nodes4), self.root4,
+                                _ptr(rays), m, m, tmin, _ptr(out))
+        else:
+            lib().om_trace_any(_ptr(self.tris), self.n, _ptr(self.sorted_idx), _ptr(self.left),
+                               _ptr(self.right), _ptr(self.node_aabb), self.root, _ptr(rays), m, m,
+                               tmin, _ptr(out))
         return out
 
 
@@ -304,7 +329,7 @@ def camera_ray(cam: OmCamera, w, h, samples, seed, sample_id):
     return o, d, st.value
 
 
-def _scene_struct(scene: dict, bvh: LBVH, keep: dict) -> OmScene:
+def _scene_struct(scene: dict, bvh: LBVH, keep: dict, width: int = 4) -> OmScene:
 
     def arr(name, dtype, cols):
         a = scene.get(name)
@@ -327,15 +352,19 @@ def _scene_struct(scene: dict, bvh: LBVH, keep: dict) -> OmScene:
     sc.right = _ptr(bvh.right)
     sc.node_aabb = _ptr(bvh.node_aabb)
     sc.root = bvh.root
+    if width == 4:
+        sc.nodes4 = _ptr(bvh.nodes4())
+        sc.root4 = bvh.root4
     return sc
 
 
 def render(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_depth, rr_start,
-           seed, row_step=1, row_offset=0, threads=1):
+           seed, row_step=1, row_offset=0, threads=1, tree_width=4):
     """scene: dict with tris (n,9), prim_key (n,), lambert (.,3), metal (.,10),
-    dielectric (.,4), emissive (.,3).  Returns (image float64 [h,w,3], segments)."""
+    dielectric (.,4), emissive (.,3); tree_width = traversal tree (2 or 4).
+    Returns (image float64 [h,w,3], segments)."""
     keep = {}
-    sc = _scene_struct(scene, bvh, keep)
+    sc = _scene_struct(scene, bvh, keep, tree_width)
     p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, row_step, row_offset,
                        threads)
     img = np.zeros((height, width, 3), dtype=np.float64)
@@ -465,11 +494,12 @@ def atan_unit(t: float) -> float:
 
 
 def render_guided(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_depth, rr_start,
-                  seed, iterations, svo_depth, guide_ratio=0.5, min_samples=0, threads=1, keep_svo=False):
+                  seed, iterations, svo_depth, guide_ratio=0.5, min_samples=0, threads=1, keep_svo=False,
+                  tree_width=4):
     """Guided render of every row.  Returns (image float64 [h,w,3] = mean over
     all frames, segments, Svo of the last guide or None)."""
     keep = {}
-    sc = _scene_struct(scene, bvh, keep)
+    sc = _scene_struct(scene, bvh, keep, tree_width)
     p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, 1, 0, threads)
     g = OmGuideParams(iterations, svo_depth, guide_ratio, min_samples)
     img = np.zeros((height, width, 3), dtype=np.float64)

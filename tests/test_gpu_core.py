@@ -223,13 +223,41 @@ def _gpu_trace(ctx, b, rays, tmin=0.0, stats=False):
     return out
 
 
-def test_trace_closest_and_any_match_oracle(ctx, oracle):
+@pytest.fixture(params=[2, 4])
+def width(request, ctx):
+    """Runs the test with the binary LBVH (2) and the 4-wide tree (4)."""
+    old = ctx.tune_get("trace_width")
+    ctx.tune("trace_width", request.param)
+    yield request.param
+    ctx.tune("trace_width", old)
+
+
+@pytest.mark.parametrize("n", [2, 3, 17, 1000, 65537])
+def test_bvh4_nodes_bit_exact(ctx, oracle, n):
+    tris = N.gen_random_tris(n, 51 + n)
+    b, _ = _gpu_lbvh(ctx, tris, 30)
+    o = oracle.LBVH(tris, 30)
+    ref = o.nodes4()
+    v = b.view()
+    assert v.root4 == o.root4
+    got = fetch(ctx, v.d_nodes4, (n - 1) * 32, np.float32).reshape(n - 1, 32)
+    # the nodes reachable from the root (even-depth binary nodes)
+    live, todo = [], [o.root4]
+    while todo:
+        p = todo.pop()
+        live.append(p)
+        todo += [int(c) for c in ref[p, 24:28].view(np.uint32) if c < 0x80000000]
+    live = np.array(sorted(live))
+    assert got[live].tobytes() == ref[live].tobytes()
+
+
+def test_trace_closest_and_any_match_oracle(ctx, oracle, width):
     tris = N.gen_random_tris(100_000, 4)
     rays = N.gen_random_rays(200_000, 4)
     b, _ = _gpu_lbvh(ctx, tris, 30)
     o = oracle.LBVH(tris, 30)
     p, t, bary, st = _gpu_trace(ctx, b, rays, stats=True)
-    rp, rt, rb, rst = o.trace(rays, stats=True)
+    rp, rt, rb, rst = o.trace(rays, stats=True, width=width)
     assert np.array_equal(p, rp)
     assert t.tobytes() == rt.tobytes()
     assert bary.tobytes() == rb.tobytes()
@@ -248,7 +276,7 @@ def test_trace_closest_and_any_match_oracle(ctx, oracle):
     assert np.mean(bp == p[:3000]) >= 0.9999
 
 
-def test_trace_c2_full_size(ctx, oracle):
+def test_trace_c2_full_size(ctx, oracle, width):
     """1M triangles x 4M rays: GPU hits vs the oracle on a 1% sample and vs
     brute force on a small sample; t within 1e-5 relative."""
     tris = N.gen_random_tris(1_000_000, 1)
@@ -258,7 +286,7 @@ def test_trace_c2_full_size(ctx, oracle):
     o = oracle.LBVH(tris, 30)
     idx = np.random.default_rng(1).choice(rays.shape[1], 40_000, replace=False)
     sub = np.ascontiguousarray(rays[:, idx])
-    rp, rt, _ = o.trace(sub)
+    rp, rt, _ = o.trace(sub, width=width)
     assert np.array_equal(p[idx], rp)
     assert np.array_equal(t[idx], rt)
     sub2 = np.ascontiguousarray(rays[:, idx[:150]])
@@ -268,12 +296,12 @@ def test_trace_c2_full_size(ctx, oracle):
     assert np.all(np.abs(bt[hit] - t[idx[:150]][hit]) <= 1e-5 * np.abs(bt[hit]))
 
 
-def test_trace_tmin_and_errors(ctx, oracle):
+def test_trace_tmin_and_errors(ctx, oracle, width):
     tris = N.gen_random_tris(5000, 9)
     rays = N.gen_random_rays(20000, 9)
     b, _ = _gpu_lbvh(ctx, tris, 63)
     p, t, _ = _gpu_trace(ctx, b, rays, tmin=0.03)
-    rp, rt, _ = oracle.LBVH(tris, 63).trace(rays, tmin=0.03)
+    rp, rt, _ = oracle.LBVH(tris, 63).trace(rays, tmin=0.03, width=width)
     assert np.array_equal(p, rp) and np.array_equal(t, rt)
     with pytest.raises(N.MrhError):
         N.Lbvh(ctx, dev(tris).data_ptr(), 5000, 31)
@@ -289,7 +317,8 @@ def _render_pair(ctx, oracle, sc, w, h, S, depth, rr, seed=0):
     ob = oracle.LBVH(sc["tris"], 30)
     c = sc["camera"]
     cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], w, h)
-    ref, seg = oracle.render(sc, ob, cam, w, h, S, depth, rr, seed, threads=8)
+    ref, seg = oracle.render(sc, ob, cam, w, h, S, depth, rr, seed, threads=8,
+                             tree_width=ctx.tune_get("trace_width"))
     return g, st, ref, seg
 
 
@@ -309,7 +338,7 @@ def test_render_c1_multisample(ctx, oracle):
     assert relmse(g, ref) <= 1e-4
 
 
-def test_render_sphere_grid_all_materials(ctx, oracle):
+def test_render_sphere_grid_all_materials(ctx, oracle, width):
     sc = N.gen_sphere_grid(6, 8, 8, 2)
     kinds = set((sc["prim_key"] >> 24).tolist())
     assert {1, 2, 3}.issubset(kinds)

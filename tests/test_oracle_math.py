@@ -195,21 +195,80 @@ def test_lbvh_invariants_random_and_duplicates(oracle):
 
 
 # ----------------------------------------------------------------- traversal
-def test_closest_hit_equals_brute_force(oracle):
+@pytest.mark.parametrize("width", [2, 4])
+def test_closest_hit_equals_brute_force(oracle, width):
     from meturay_amd import native as N
     tris = N.gen_random_tris(3000, 3)
     rays = N.gen_random_rays(4000, 3)
     b = oracle.LBVH(tris, 30)
-    p1, t1, b1 = b.trace(rays)
+    p1, t1, b1 = b.trace(rays, width=width)
     p2, t2, b2 = oracle.trace_brute(tris, rays)
     assert np.array_equal(p1, p2)
     assert np.array_equal(t1, t2) and np.array_equal(b1, b2)
-    anyh = b.trace_any(rays)
+    anyh = b.trace_any(rays, width=width)
     assert np.array_equal(anyh.astype(bool), p2 != 0xFFFFFFFF)
     # tmin excludes hits closer than it
-    p3, t3, _ = b.trace(rays, tmin=0.05)
+    p3, t3, _ = b.trace(rays, tmin=0.05, width=width)
     p4, t4, _ = oracle.trace_brute(tris, rays, tmin=0.05)
     assert np.array_equal(p3, p4)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 1000, 20000])
+def test_bvh4_collapse_structure(oracle, n):
+    """Every 4-wide node is an even-depth binary node whose slots hold its
+    children, internal children replaced by their two children, with the
+    binary tree's boxes; every leaf is reachable exactly once."""
+    from meturay_amd import native as N
+    tris = N.gen_random_tris(n, 7)
+    b = oracle.LBVH(tris, 30)
+    q = b.nodes4()
+    LEAF = 0x80000000
+    if n == 1:
+        assert b.root4 == LEAF
+        return
+    assert b.root4 == b.root - n
+    seen = []
+    todo = [b.root4]
+    depth4 = {b.root4: 0}
+    while todo:
+        p = todo.pop()
+        codes = q[p, 24:28].view(np.uint32)
+        expect = []
+        for c in (b.left[p], b.right[p]):
+            expect += [c] if c < n else [b.left[c - n], b.right[c - n]]
+        occupied = [int(c) for c in codes if c != 0xFFFFFFFF]
+        assert occupied == [int(g) | LEAF if g < n else int(g) - n for g in expect]
+        assert all(c == 0xFFFFFFFF for c in codes[len(expect):])
+        for k, g in enumerate(expect):
+            box = q[p, [k, 4 + k, 8 + k, 12 + k, 16 + k, 20 + k]]
+            assert np.array_equal(box, b.node_aabb[g])
+            if g < n:
+                seen.append(int(g))
+            else:
+                todo.append(int(g) - n)
+                depth4[int(g) - n] = depth4[p] + 1
+    assert sorted(seen) == list(range(n))
+    if n >= 1000:
+        # halves the depth: deepest 4-wide level <= ceil(binary depth / 2)
+        d2 = 0
+        par = b.parent
+        for leaf in range(0, n, max(1, n // 500)):
+            d, g = 0, leaf
+            while par[g] != 0xFFFFFFFF:
+                g = par[g]
+                d += 1
+            d2 = max(d2, d)
+        assert max(depth4.values()) + 1 <= (d2 + 1) // 2 + 1
+
+
+def test_bvh4_visits_fewer_nodes(oracle):
+    from meturay_amd import native as N
+    tris = N.gen_random_tris(20000, 5)
+    rays = N.gen_random_rays(5000, 5)
+    b = oracle.LBVH(tris, 30)
+    _, _, _, s2 = b.trace(rays, stats=True, width=2)
+    _, _, _, s4 = b.trace(rays, stats=True, width=4)
+    assert s4[:, 0].sum() < 0.7 * s2[:, 0].sum()
 
 
 # ----------------------------------------------------------------- BxDFs

@@ -137,6 +137,7 @@ int32_t mrh_context_destroy(mrh_context* ctx)
     if (!ctx) return MRH_ERR_INVALID;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    if (ctx->guide_svo) mrh_svo_destroy(ctx->guide_svo);
     for (int i = 0; i < SLOT_COUNT; i++)
         if (ctx->slot_ptr[i]) hipFree(ctx->slot_ptr[i]);
     for (auto& p : ctx->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }

@@ -33,7 +33,13 @@ struct mrh_svo {
     uint32_t* cnt = nullptr;         // [cap][64]
     float* radiance = nullptr;       // [cap][64]
     uint32_t* node_count = nullptr;  // [cap] samples in the subtree
-    mrh::V3* centers = nullptr;      // [64]
+    uint4* pack = nullptr;           // [cap] first child, child mask, subtree samples (lookup records)
+    // grow-only build scratch: sort/flag/scan buffers, per-level arrays, counter
+    void* work_a = nullptr;
+    size_t work_a_bytes = 0;
+    void* work_b = nullptr;
+    size_t work_b_bytes = 0;
+    uint32_t* counter = nullptr;     // [16]
 };
 
 namespace mrh {
@@ -83,24 +89,31 @@ __global__ __launch_bounds__(256) void k_scan_add(uint32_t* __restrict__ out, co
         if (base + k < n) out[base + k] += add;
 }
 
-// exclusive scan in -> out (out may alias in); tmp holds the tile sums
-int32_t scan_exclusive(mrh_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n)
+// words of scratch scan_exclusive needs for n elements
+size_t scan_tmp_words(uint32_t n)
+{
+    size_t w = 0;
+    while (n > 1) {
+        const uint32_t tiles = div_up(n, SCAN_TILE);
+        w += 2 * ((size_t)tiles + 1);
+        n = tiles;
+    }
+    return w + 2;
+}
+
+// exclusive scan in -> out (out may alias in); tmp: scan_tmp_words(n) words
+int32_t scan_exclusive(mrh_context* ctx, const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* tmp)
 {
     if (n == 0) return MRH_OK;
     const uint32_t tiles = div_up(n, SCAN_TILE);
-    void* p = nullptr;
-    MRH_TRY(dmalloc(ctx, &p, (size_t)(tiles + 1) * 4 * 2));
-    uint32_t* sums = (uint32_t*)p;
+    uint32_t* sums = tmp;
     uint32_t* pre = sums + tiles + 1;
-    int32_t rc = launch(ctx, "scan_tiles", dim3(tiles), dim3(SCAN_BLOCK), 0, k_scan_tiles, in, out, sums, n);
-    if (rc == MRH_OK && tiles > 1) {
-        rc = scan_exclusive(ctx, sums, pre, tiles);
-        if (rc == MRH_OK)
-            rc = launch(ctx, "scan_add", dim3(tiles), dim3(SCAN_BLOCK), 0, k_scan_add, out, (const uint32_t*)pre, n);
+    MRH_TRY(launch(ctx, "scan_tiles", dim3(tiles), dim3(SCAN_BLOCK), 0, k_scan_tiles, in, out, sums, n));
+    if (tiles > 1) {
+        MRH_TRY(scan_exclusive(ctx, sums, pre, tiles, pre + tiles + 1));
+        MRH_TRY(launch(ctx, "scan_add", dim3(tiles), dim3(SCAN_BLOCK), 0, k_scan_add, out, (const uint32_t*)pre, n));
     }
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(p);
-    return rc;
+    return MRH_OK;
 }
 
 // ---------------------------------------------------------------- unique
@@ -122,42 +135,90 @@ __global__ void k_compact_runs(const uint32_t* __restrict__ in, const uint32_t* 
     }
 }
 
+// distinct values of in >> shift (in sorted) into out; seg[i] = index of
+// in[i]'s run; no host round trip
 int32_t unique_runs(mrh_context* ctx, const uint32_t* in, uint32_t n, uint32_t shift, uint32_t* out,
-                    uint32_t* seg, uint32_t* flags, uint32_t* pos, uint32_t* count)
+                    uint32_t* seg, uint32_t* flags, uint32_t* pos, uint32_t* tmp)
 {
-    *count = 0;
     if (n == 0) return MRH_OK;
     const uint32_t g = stride_grid(ctx, n, 256);
     MRH_TRY(launch(ctx, "svo_flag", dim3(g), dim3(256), 0, k_flag_runs, in, n, shift, flags));
-    MRH_TRY(scan_exclusive(ctx, flags, pos, n));
-    MRH_TRY(launch(ctx, "svo_compact", dim3(g), dim3(256), 0, k_compact_runs, in, (const uint32_t*)flags,
-                   (const uint32_t*)pos, n, shift, out, seg));
-    uint32_t last[2];
-    MRH_HIP_CHECK(ctx, hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-    MRH_HIP_CHECK(ctx, hipMemcpyAsync(&last[1], flags + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
-    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    *count = last[0] + last[1];
-    return MRH_OK;
+    MRH_TRY(scan_exclusive(ctx, flags, pos, n, tmp));
+    return launch(ctx, "svo_compact", dim3(g), dim3(256), 0, k_compact_runs, in, (const uint32_t*)flags,
+                  (const uint32_t*)pos, n, shift, out, seg);
 }
 
-// out[pos[i]] = in[i] where the record is kept
-__global__ void k_flag_records(const uint32_t* __restrict__ codes, uint32_t n, uint32_t* __restrict__ flags)
+// Histogram of the coarsest level at which each sorted code starts a new
+// prefix (level l's node count = sum of bins 0..l); codes < 2^(3 depth).
+__global__ __launch_bounds__(256) void k_level_hist(const uint32_t* __restrict__ sorted, uint32_t n, uint32_t depth,
+                                                    uint32_t* __restrict__ hist)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        flags[i] = (codes[i] != SVO_NO_NODE) ? 1u : 0u;
-}
-
-__global__ void k_scatter_records(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ bins,
-                                  const float* __restrict__ rad, const uint32_t* __restrict__ pos, uint32_t n,
-                                  uint32_t* __restrict__ oc, uint32_t* __restrict__ ob, float* __restrict__ orad)
-{
+    __shared__ uint32_t h[12];
+    if (threadIdx.x < 12) h[threadIdx.x] = 0;
+    __syncthreads();
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t c = codes[i];
-        if (c == SVO_NO_NODE) continue;
-        const uint32_t k = pos[i];
-        oc[k] = c;
-        ob[k] = bins[i];
-        orad[k] = rad[i];
+        uint32_t m = 0;
+        if (i > 0) {
+            const uint32_t x = sorted[i] ^ sorted[i - 1];
+            m = x ? depth - (31u - (uint32_t)__clz(x)) / 3u : 12u;
+        }
+        if (m <= depth) atomicAdd(&h[m], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 12 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// Appends the kept records (code != SVO_NO_NODE) at *ctr.  Each block
+// takes chunks of 4096 records, ranks the kept ones with a block scan and
+// claims their output range with one atomic per chunk.  The order of the
+// output depends on scheduling; everything built from it (sorted cells,
+// fixed-point sums) does not.
+constexpr uint32_t APPEND_ITEMS = 16, APPEND_CHUNK = APPEND_ITEMS * 256;
+
+__global__ __launch_bounds__(256) void k_append_records(const uint32_t* __restrict__ codes,
+                                                        const uint32_t* __restrict__ bins,
+                                                        const float* __restrict__ rad, uint32_t n,
+                                                        uint32_t* __restrict__ oc, uint32_t* __restrict__ ob,
+                                                        float* __restrict__ orad, uint32_t* ctr)
+{
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t c0 = blockIdx.x * APPEND_CHUNK; c0 < n; c0 += gridDim.x * APPEND_CHUNK) {
+        uint32_t code[APPEND_ITEMS];
+        uint32_t t = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < APPEND_ITEMS; k++) {
+            const uint32_t i = c0 + k * 256 + threadIdx.x;
+            code[k] = (i < n) ? codes[i] : SVO_NO_NODE;
+            t += (code[k] != SVO_NO_NODE) ? 1u : 0u;
+        }
+        // exclusive scan of the per-thread counts over the block
+        uint32_t incl = t;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off) incl += y;
+        }
+        if (lane == 63) s_wave[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < 4; w++) {
+            before += (w < wave) ? s_wave[w] : 0u;
+            total += s_wave[w];
+        }
+        if (threadIdx.x == 0) s_base = total ? atomicAdd(ctr, total) : 0u;
+        __syncthreads();
+        uint32_t k_out = s_base + before + incl - t;
+#pragma unroll
+        for (uint32_t k = 0; k < APPEND_ITEMS; k++) {
+            if (code[k] == SVO_NO_NODE) continue;
+            const uint32_t i = c0 + k * 256 + threadIdx.x;
+            oc[k_out] = code[k];
+            ob[k_out] = bins[i];
+            orad[k_out] = rad[i];
+            k_out++;
+        }
+        __syncthreads();   // s_wave / s_base are reused by the next chunk
     }
 }
 
@@ -193,12 +254,13 @@ __global__ void k_svo_link(const uint32_t* __restrict__ lvl_codes, const uint32_
     }
 }
 
-__global__ void k_svo_centers(V3* __restrict__ c)
+// lookup records: first child, child mask, samples in the subtree
+__global__ void k_svo_pack(uint32_t nodes, const uint32_t* __restrict__ first_child,
+                           const uint32_t* __restrict__ child_mask, const uint32_t* __restrict__ node_count,
+                           uint4* __restrict__ pack)
 {
-    const uint32_t b = threadIdx.x;
-    if (b >= GUIDE_BINS) return;
-    const uint32_t ix = b % GUIDE_RES, iy = b / GUIDE_RES;
-    c[b] = octa_to_dir(((float)ix + 0.5f) / (float)GUIDE_RES, ((float)iy + 0.5f) / (float)GUIDE_RES);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nodes; i += gridDim.x * blockDim.x)
+        pack[i] = make_uint4(first_child[i], child_mask[i], node_count ? node_count[i] : 0u, 0u);
 }
 
 // ---------------------------------------------------------------- splat / filter
@@ -296,15 +358,12 @@ SvoView svo_view(const mrh_svo* s)
     SvoView v;
     v.depth = s->depth;
     for (int i = 0; i < 12; i++) v.lvl_off[i] = s->lvl_off[i];
-    v.first_child = s->first_child;
-    v.child_mask = s->child_mask;
+    v.pack = s->pack;
     v.radiance = s->radiance;
-    v.node_count = s->node_count;
     v.lox = s->cube[0];
     v.loy = s->cube[1];
     v.loz = s->cube[2];
     v.inv_cell = (float)(1u << s->depth) / s->cube[3];
-    v.centers = s->centers;
     return v;
 }
 
@@ -314,9 +373,10 @@ static int32_t svo_reserve(mrh_svo* s, uint32_t nodes)
     if (nodes <= s->cap) return MRH_OK;
     (void)hipStreamSynchronize(ctx->stream);
     void* ptrs[] = {s->codes, s->first_child, s->child_mask, s->parent, s->sum, s->acc, s->cnt, s->radiance,
-                    s->node_count};
+                    s->node_count, s->pack};
     for (void* p : ptrs) (void)hipFree(p);
     s->codes = s->first_child = s->child_mask = s->parent = s->node_count = nullptr;
+    s->pack = nullptr;
     s->sum = s->radiance = nullptr;
     s->acc = nullptr;
     s->cnt = nullptr;
@@ -331,6 +391,7 @@ static int32_t svo_reserve(mrh_svo* s, uint32_t nodes)
     MRH_TRY(dmalloc(ctx, (void**)&s->cnt, (size_t)cap * GUIDE_BINS * 4));
     MRH_TRY(dmalloc(ctx, (void**)&s->radiance, (size_t)cap * GUIDE_BINS * 4));
     MRH_TRY(dmalloc(ctx, (void**)&s->node_count, (size_t)cap * 4));
+    MRH_TRY(dmalloc(ctx, (void**)&s->pack, (size_t)cap * 16));
     s->cap = cap;
     return MRH_OK;
 }
@@ -343,6 +404,26 @@ int32_t svo_codes(mrh_context* ctx, const float* d_points, uint32_t n, uint32_t 
                   cube[0], cube[1], cube[2], inv_cell, depth, d_codes);
 }
 
+// grow-only device buffer
+static int32_t grow(mrh_context* ctx, void** p, size_t* have, size_t want)
+{
+    if (want <= *have) return MRH_OK;
+    if (*p) {
+        MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        (void)hipFree(*p);
+        *p = nullptr;
+        *have = 0;
+    }
+    want += want / 8 + 4096;
+    MRH_TRY(dmalloc(ctx, p, want));
+    *have = want;
+    return MRH_OK;
+}
+
+// Sort the cells, count every level's distinct prefixes in one pass (one
+// host round trip), then for each level from the leaves up: unique prefixes
+// and the parent index of the level below; then links, lookup records and
+// fresh radiance.
 int32_t svo_build(mrh_context* ctx, mrh_svo* s, const uint32_t* d_codes, uint32_t n, const float cube[4],
                   uint32_t depth)
 {
@@ -351,83 +432,73 @@ int32_t svo_build(mrh_context* ctx, mrh_svo* s, const uint32_t* d_codes, uint32_
     if (!(cube[3] > 0.0f)) return fail(ctx, MRH_ERR_INVALID, "svo build: empty cube");
     for (int i = 0; i < 4; i++) s->cube[i] = cube[i];
     s->depth = depth;
-    // scratch: sorted codes, values, flags, pos, per-level codes and parent ids
-    std::vector<void*> tmp;
-    auto take = [&](size_t bytes, void** out) -> int32_t {
-        MRH_TRY(dmalloc(ctx, out, bytes));
-        tmp.push_back(*out);
-        return MRH_OK;
-    };
-    auto cleanup = [&]() {
-        (void)hipStreamSynchronize(ctx->stream);
-        for (void* p : tmp) (void)hipFree(p);
-    };
-    int32_t rc = MRH_OK;
-    uint32_t *sorted = nullptr, *vals = nullptr, *flags = nullptr, *pos = nullptr;
-    do {
-        if ((rc = take((size_t)n * 4, (void**)&sorted)) != MRH_OK) break;
-        if ((rc = take((size_t)n * 4, (void**)&vals)) != MRH_OK) break;
-        if ((rc = take((size_t)n * 4, (void**)&flags)) != MRH_OK) break;
-        if ((rc = take((size_t)n * 4, (void**)&pos)) != MRH_OK) break;
-        SortArgs sa;
-        sa.keys_in = d_codes;
-        sa.keys_out = sorted;
-        sa.vals_out = vals;
-        sa.n_max = n;
-        sa.key_bytes = 4;
-        sa.begin_bit = 0;
-        sa.end_bit = 3 * depth;
-        if ((rc = radix_sort(ctx, sa)) != MRH_OK) break;
-        // leaves, then each level up: codes and the parent ids of the level below
-        uint32_t* lvl_codes[11] = {};
-        uint32_t* lvl_parent[11] = {};   // parent_local of level l's nodes (index into level l-1)
-        uint32_t count = 0;
-        void* p = nullptr;
-        if ((rc = take((size_t)n * 4, &p)) != MRH_OK) break;
-        lvl_codes[depth] = (uint32_t*)p;
-        if ((rc = unique_runs(ctx, sorted, n, 0, lvl_codes[depth], nullptr, flags, pos, &count)) != MRH_OK) break;
-        s->lvl_count[depth] = count;
-        for (int l = (int)depth - 1; l >= 0 && rc == MRH_OK; l--) {
-            const uint32_t below = s->lvl_count[l + 1];
-            if ((rc = take((size_t)below * 4, &p)) != MRH_OK) break;
-            lvl_codes[l] = (uint32_t*)p;
-            if ((rc = take((size_t)below * 4, &p)) != MRH_OK) break;
-            lvl_parent[l + 1] = (uint32_t*)p;
-            rc = unique_runs(ctx, lvl_codes[l + 1], below, 3, lvl_codes[l], lvl_parent[l + 1], flags, pos, &count);
-            s->lvl_count[l] = count;
-        }
-        if (rc != MRH_OK) break;
-        if (s->lvl_count[0] != 1) {
-            rc = fail(ctx, MRH_ERR_HIP, "svo build: %u roots", s->lvl_count[0]);
-            break;
-        }
-        s->lvl_off[0] = 0;
-        for (uint32_t l = 0; l <= depth; l++) s->lvl_off[l + 1] = s->lvl_off[l] + s->lvl_count[l];
-        for (uint32_t l = depth + 1; l < 11; l++) s->lvl_off[l + 1] = s->lvl_off[l];
-        s->nodes = s->lvl_off[depth + 1];
-        if ((rc = svo_reserve(s, s->nodes)) != MRH_OK) break;
-        for (uint32_t l = 0; l <= depth && rc == MRH_OK; l++) {
-            const uint32_t c = s->lvl_count[l];
-            rc = launch(ctx, "svo_link", dim3(stride_grid(ctx, c, 256)), dim3(256), 0, k_svo_link,
-                        (const uint32_t*)lvl_codes[l], (const uint32_t*)lvl_parent[l], c, s->lvl_off[l],
-                        l ? s->lvl_off[l - 1] : 0u, s->codes, s->parent, s->first_child, s->child_mask);
-        }
-        if (rc != MRH_OK) break;
-        if (!s->centers) {
-            if ((rc = dmalloc(ctx, (void**)&s->centers, GUIDE_BINS * sizeof(V3))) != MRH_OK) break;
-        }
-        if ((rc = launch(ctx, "svo_centers", dim3(1), dim3(64), 0, k_svo_centers, s->centers)) != MRH_OK) break;
-        // fresh radiance
-        if ((rc = fill_u32(ctx, (uint32_t*)s->sum, 0u, (size_t)s->nodes * GUIDE_BINS)) != MRH_OK) break;
-        if ((rc = fill_u32(ctx, (uint32_t*)s->acc, 0u, (size_t)s->nodes * GUIDE_BINS * 2)) != MRH_OK) break;
-        if ((rc = fill_u32(ctx, s->cnt, 0u, (size_t)s->nodes * GUIDE_BINS)) != MRH_OK) break;
-        uint32_t eps_bits;
-        std::memcpy(&eps_bits, &GUIDE_EPS, 4);
-        rc = fill_u32(ctx, (uint32_t*)s->radiance, eps_bits, (size_t)s->nodes * GUIDE_BINS);
-        if (rc == MRH_OK) rc = fill_u32(ctx, s->node_count, 0u, s->nodes);
-    } while (0);
-    cleanup();
-    return rc;
+    // work_a: sorted codes, sort values / flags, scan positions, scan scratch
+    const size_t tmp_words = scan_tmp_words(n);
+    MRH_TRY(grow(ctx, &s->work_a, &s->work_a_bytes, ((size_t)n * 3 + tmp_words) * 4));
+    if (!s->counter) MRH_TRY(dmalloc(ctx, (void**)&s->counter, 64));
+    uint32_t* sorted = (uint32_t*)s->work_a;
+    uint32_t* flags = sorted + n;   // also the sort's value output
+    uint32_t* pos = flags + n;
+    uint32_t* tmp = pos + n;
+    SortArgs sa;
+    sa.keys_in = d_codes;
+    sa.keys_out = sorted;
+    sa.vals_out = flags;
+    sa.n_max = n;
+    sa.key_bytes = 4;
+    sa.begin_bit = 0;
+    sa.end_bit = 3 * depth;
+    MRH_TRY(radix_sort(ctx, sa));
+    MRH_TRY(fill_u32(ctx, s->counter, 0u, 16));
+    MRH_TRY(launch(ctx, "svo_level_hist", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_level_hist,
+                   (const uint32_t*)sorted, n, depth, s->counter));
+    uint32_t hist[12] = {};
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(hist, s->counter, sizeof(hist), hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t run = 0;
+    for (uint32_t l = 0; l <= depth; l++) {
+        run += hist[l];
+        s->lvl_count[l] = run;
+    }
+    for (uint32_t l = depth + 1; l < 11; l++) s->lvl_count[l] = 0;
+    if (s->lvl_count[0] != 1) return fail(ctx, MRH_ERR_HIP, "svo build: %u roots", s->lvl_count[0]);
+    s->lvl_off[0] = 0;
+    for (uint32_t l = 0; l <= depth; l++) s->lvl_off[l + 1] = s->lvl_off[l] + s->lvl_count[l];
+    for (uint32_t l = depth + 1; l < 11; l++) s->lvl_off[l + 1] = s->lvl_off[l];
+    s->nodes = s->lvl_off[depth + 1];
+    // work_b: codes of every level and parent indices of every level but the root's
+    MRH_TRY(grow(ctx, &s->work_b, &s->work_b_bytes, (size_t)s->nodes * 2 * 4));
+    uint32_t* lvl_codes[11] = {};
+    uint32_t* lvl_parent[11] = {};   // parent_local of level l's nodes (index into level l-1)
+    uint32_t* q = (uint32_t*)s->work_b;
+    for (uint32_t l = 0; l <= depth; l++) {
+        lvl_codes[l] = q;
+        q += s->lvl_count[l];
+        lvl_parent[l] = l ? q : nullptr;
+        q += l ? s->lvl_count[l] : 0;
+    }
+    MRH_TRY(unique_runs(ctx, sorted, n, 0, lvl_codes[depth], nullptr, flags, pos, tmp));
+    for (int l = (int)depth - 1; l >= 0; l--)
+        MRH_TRY(unique_runs(ctx, lvl_codes[l + 1], s->lvl_count[l + 1], 3, lvl_codes[l], lvl_parent[l + 1], flags,
+                            pos, tmp));
+    MRH_TRY(svo_reserve(s, s->nodes));
+    for (uint32_t l = 0; l <= depth; l++) {
+        const uint32_t c = s->lvl_count[l];
+        MRH_TRY(launch(ctx, "svo_link", dim3(stride_grid(ctx, c, 256)), dim3(256), 0, k_svo_link,
+                       (const uint32_t*)lvl_codes[l], (const uint32_t*)lvl_parent[l], c, s->lvl_off[l],
+                       l ? s->lvl_off[l - 1] : 0u, s->codes, s->parent, s->first_child, s->child_mask));
+    }
+    MRH_TRY(launch(ctx, "svo_pack", dim3(stride_grid(ctx, s->nodes, 256)), dim3(256), 0, k_svo_pack, s->nodes,
+                   (const uint32_t*)s->first_child, (const uint32_t*)s->child_mask, (const uint32_t*)nullptr,
+                   s->pack));
+    // fresh radiance
+    MRH_TRY(fill_u32(ctx, (uint32_t*)s->sum, 0u, (size_t)s->nodes * GUIDE_BINS));
+    MRH_TRY(fill_u32(ctx, (uint32_t*)s->acc, 0u, (size_t)s->nodes * GUIDE_BINS * 2));
+    MRH_TRY(fill_u32(ctx, s->cnt, 0u, (size_t)s->nodes * GUIDE_BINS));
+    uint32_t eps_bits;
+    std::memcpy(&eps_bits, &GUIDE_EPS, 4);
+    MRH_TRY(fill_u32(ctx, (uint32_t*)s->radiance, eps_bits, (size_t)s->nodes * GUIDE_BINS));
+    return fill_u32(ctx, s->node_count, 0u, s->nodes);
 }
 
 int32_t svo_splat(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, const uint32_t* bins, const float* values,
@@ -453,6 +524,9 @@ int32_t svo_filter(mrh_context* ctx, mrh_svo* s)
     }
     MRH_TRY(launch(ctx, "svo_node_count", dim3(stride_grid(ctx, s->nodes, 256)), dim3(256), 0, k_svo_node_count,
                    s->nodes, (const uint32_t*)s->cnt, s->node_count));
+    MRH_TRY(launch(ctx, "svo_pack", dim3(stride_grid(ctx, s->nodes, 256)), dim3(256), 0, k_svo_pack, s->nodes,
+                   (const uint32_t*)s->first_child, (const uint32_t*)s->child_mask, (const uint32_t*)s->node_count,
+                   s->pack));
     for (uint32_t l = 0; l <= s->depth; l++) {
         const uint32_t c = s->lvl_count[l];
         MRH_TRY(launch(ctx, "svo_down", dim3(stride_grid(ctx, (uint64_t)c * GUIDE_BINS, 256)), dim3(256), 0,
@@ -462,31 +536,20 @@ int32_t svo_filter(mrh_context* ctx, mrh_svo* s)
                   k_svo_blur, s->nodes, (const float*)s->sum, s->radiance);
 }
 
-int32_t svo_compact_records(mrh_context* ctx, const uint32_t* codes, const uint32_t* bins, const float* rad,
-                            uint32_t n, uint32_t* out_codes, uint32_t* out_bins, float* out_rad, uint32_t* count)
+int32_t svo_compact_records(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, const uint32_t* bins,
+                            const float* rad, uint32_t n, uint32_t* out_codes, uint32_t* out_bins, float* out_rad,
+                            uint32_t* count)
 {
     *count = 0;
     if (n == 0) return MRH_OK;
-    void* p = nullptr;
-    MRH_TRY(dmalloc(ctx, &p, (size_t)n * 8));
-    uint32_t* flags = (uint32_t*)p;
-    uint32_t* pos = flags + n;
-    const uint32_t g = stride_grid(ctx, n, 256);
-    int32_t rc = launch(ctx, "svo_rec_flag", dim3(g), dim3(256), 0, k_flag_records, codes, n, flags);
-    if (rc == MRH_OK) rc = scan_exclusive(ctx, flags, pos, n);
-    if (rc == MRH_OK)
-        rc = launch(ctx, "svo_rec_scatter", dim3(g), dim3(256), 0, k_scatter_records, codes, bins, rad,
-                    (const uint32_t*)pos, n, out_codes, out_bins, out_rad);
-    uint32_t last[2] = {0, 0};
-    if (rc == MRH_OK) {
-        (void)hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
-        (void)hipMemcpyAsync(&last[1], flags + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
-    }
-    const hipError_t e = hipStreamSynchronize(ctx->stream);
-    (void)hipFree(p);
-    if (rc != MRH_OK) return rc;
-    if (e != hipSuccess) return fail(ctx, MRH_ERR_HIP, "svo record compaction: %s", hipGetErrorString(e));
-    *count = last[0] + last[1];
+    if (!s->counter) MRH_TRY(dmalloc(ctx, (void**)&s->counter, 64));
+    MRH_TRY(fill_u32(ctx, s->counter + 12, 0u, 1));
+    uint32_t grid = div_up(n, APPEND_CHUNK);
+    if (grid > (uint32_t)ctx->num_cus * 8u) grid = (uint32_t)ctx->num_cus * 8u;
+    MRH_TRY(launch(ctx, "svo_rec_append", dim3(grid), dim3(256), 0, k_append_records, codes, bins, rad, n, out_codes,
+                   out_bins, out_rad, s->counter + 12));
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(count, s->counter + 12, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return MRH_OK;
 }
 
@@ -496,8 +559,9 @@ void svo_free(mrh_svo* s)
 {
     if (!s) return;
     if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
-    void* ptrs[] = {s->codes, s->first_child, s->child_mask, s->parent,  s->sum,
-                    s->acc,   s->cnt,         s->radiance,   s->centers, s->node_count};
+    void* ptrs[] = {s->codes, s->first_child, s->child_mask, s->parent, s->sum,    s->acc,
+                    s->cnt,   s->radiance,    s->pack,       s->node_count, s->work_a, s->work_b,
+                    s->counter};
     for (void* p : ptrs) (void)hipFree(p);
     delete s;
 }
@@ -518,8 +582,8 @@ __global__ void k_svo_sample(SvoView s, const uint32_t* __restrict__ nodes, cons
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const V3 nn = v3(nrm[i], nrm[(size_t)n + i], nrm[2 * (size_t)n + i]);
         float pd;
-        const V3 d = guide_sample(s, nodes[i], nn, guide_total(s, nodes[i], nn), xi[i], xi[(size_t)n + i],
-                                  xi[2 * (size_t)n + i], pd);
+        const GuideBins g = guide_bins(s, nodes[i]);
+        const V3 d = guide_sample(g, nn, guide_total(g, nn), xi[i], xi[(size_t)n + i], xi[2 * (size_t)n + i], pd);
         dirs[i] = d.x;
         dirs[(size_t)n + i] = d.y;
         dirs[2 * (size_t)n + i] = d.z;
@@ -533,7 +597,7 @@ __global__ void k_svo_pdf(SvoView s, const uint32_t* __restrict__ nodes, const f
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const V3 nn = v3(nrm[i], nrm[(size_t)n + i], nrm[2 * (size_t)n + i]);
         const V3 d = v3(dirs[i], dirs[(size_t)n + i], dirs[2 * (size_t)n + i]);
-        pdf[i] = guide_pdf(s, nodes[i], nn, guide_total(s, nodes[i], nn), d);
+        pdf[i] = guide_pdf(s, nodes[i], nn, guide_total(guide_bins(s, nodes[i]), nn), d);
     }
 }
 

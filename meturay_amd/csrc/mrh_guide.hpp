@@ -95,6 +95,77 @@ MRH_HD uint32_t dir_to_bin(V3 w)
     return iy * GUIDE_RES + ix;
 }
 
+// Directions of the 64 bin centres, octa_to_dir((ix + 1/2) / 8, (iy + 1/2) / 8)
+// for bin iy * 8 + ix, as the exact float32 values that function returns (the
+// tests recompute them).  Written out so that the unrolled product loops use
+// them as immediate operands instead of loading them.
+constexpr float GUIDE_CENTERS[GUIDE_BINS][3] = {
+    {-0x1.f7efc0p-3f, -0x1.f7efc0p-3f, -0x1.ep-1f},
+    {-0x1.03323cp-2f, -0x1.38e0a2p-1f, -0x1.8p-1f},
+    {-0x1.dca414p-3f, -0x1.bcb64cp-1f, -0x1.cp-2f},
+    {-0x1.8f8b84p-3f, -0x1.f6297cp-1f, 0x0.0p+0f},
+    {0x1.8f8b84p-3f, -0x1.f6297cp-1f, 0x0.0p+0f},
+    {0x1.dca414p-3f, -0x1.bcb64cp-1f, -0x1.cp-2f},
+    {0x1.03323cp-2f, -0x1.38e0a2p-1f, -0x1.8p-1f},
+    {0x1.f7efc0p-3f, -0x1.f7efc0p-3f, -0x1.ep-1f},
+    {-0x1.38e0a2p-1f, -0x1.03323cp-2f, -0x1.8p-1f},
+    {-0x1.458d48p-1f, -0x1.458d48p-1f, -0x1.cp-2f},
+    {-0x1.1c73b4p-1f, -0x1.a9b662p-1f, 0x0.0p+0f},
+    {-0x1.dca414p-3f, -0x1.bcb64cp-1f, 0x1.cp-2f},
+    {0x1.dca414p-3f, -0x1.bcb64cp-1f, 0x1.cp-2f},
+    {0x1.1c73b4p-1f, -0x1.a9b662p-1f, 0x0.0p+0f},
+    {0x1.458d48p-1f, -0x1.458d48p-1f, -0x1.cp-2f},
+    {0x1.38e0a2p-1f, -0x1.03323cp-2f, -0x1.8p-1f},
+    {-0x1.bcb64cp-1f, -0x1.dca414p-3f, -0x1.cp-2f},
+    {-0x1.a9b662p-1f, -0x1.1c73b4p-1f, 0x0.0p+0f},
+    {-0x1.458d48p-1f, -0x1.458d48p-1f, 0x1.cp-2f},
+    {-0x1.03323cp-2f, -0x1.38e0a2p-1f, 0x1.8p-1f},
+    {0x1.03323cp-2f, -0x1.38e0a2p-1f, 0x1.8p-1f},
+    {0x1.458d48p-1f, -0x1.458d48p-1f, 0x1.cp-2f},
+    {0x1.a9b662p-1f, -0x1.1c73b4p-1f, 0x0.0p+0f},
+    {0x1.bcb64cp-1f, -0x1.dca414p-3f, -0x1.cp-2f},
+    {-0x1.f6297cp-1f, -0x1.8f8b84p-3f, 0x0.0p+0f},
+    {-0x1.bcb64cp-1f, -0x1.dca414p-3f, 0x1.cp-2f},
+    {-0x1.38e0a2p-1f, -0x1.03323cp-2f, 0x1.8p-1f},
+    {-0x1.f7efc0p-3f, -0x1.f7efc0p-3f, 0x1.ep-1f},
+    {0x1.f7efc0p-3f, -0x1.f7efc0p-3f, 0x1.ep-1f},
+    {0x1.38e0a2p-1f, -0x1.03323cp-2f, 0x1.8p-1f},
+    {0x1.bcb64cp-1f, -0x1.dca414p-3f, 0x1.cp-2f},
+    {0x1.f6297cp-1f, -0x1.8f8b84p-3f, 0x0.0p+0f},
+    {-0x1.f6297cp-1f, 0x1.8f8b84p-3f, 0x0.0p+0f},
+    {-0x1.bcb64cp-1f, 0x1.dca414p-3f, 0x1.cp-2f},
+    {-0x1.38e0a2p-1f, 0x1.03323cp-2f, 0x1.8p-1f},
+    {-0x1.f7efc0p-3f, 0x1.f7efc0p-3f, 0x1.ep-1f},
+    {0x1.f7efc0p-3f, 0x1.f7efc0p-3f, 0x1.ep-1f},
+    {0x1.38e0a2p-1f, 0x1.03323cp-2f, 0x1.8p-1f},
+    {0x1.bcb64cp-1f, 0x1.dca414p-3f, 0x1.cp-2f},
+    {0x1.f6297cp-1f, 0x1.8f8b84p-3f, 0x0.0p+0f},
+    {-0x1.bcb64cp-1f, 0x1.dca414p-3f, -0x1.cp-2f},
+    {-0x1.a9b662p-1f, 0x1.1c73b4p-1f, 0x0.0p+0f},
+    {-0x1.458d48p-1f, 0x1.458d48p-1f, 0x1.cp-2f},
+    {-0x1.03323cp-2f, 0x1.38e0a2p-1f, 0x1.8p-1f},
+    {0x1.03323cp-2f, 0x1.38e0a2p-1f, 0x1.8p-1f},
+    {0x1.458d48p-1f, 0x1.458d48p-1f, 0x1.cp-2f},
+    {0x1.a9b662p-1f, 0x1.1c73b4p-1f, 0x0.0p+0f},
+    {0x1.bcb64cp-1f, 0x1.dca414p-3f, -0x1.cp-2f},
+    {-0x1.38e0a2p-1f, 0x1.03323cp-2f, -0x1.8p-1f},
+    {-0x1.458d48p-1f, 0x1.458d48p-1f, -0x1.cp-2f},
+    {-0x1.1c73b4p-1f, 0x1.a9b662p-1f, 0x0.0p+0f},
+    {-0x1.dca414p-3f, 0x1.bcb64cp-1f, 0x1.cp-2f},
+    {0x1.dca414p-3f, 0x1.bcb64cp-1f, 0x1.cp-2f},
+    {0x1.1c73b4p-1f, 0x1.a9b662p-1f, 0x0.0p+0f},
+    {0x1.458d48p-1f, 0x1.458d48p-1f, -0x1.cp-2f},
+    {0x1.38e0a2p-1f, 0x1.03323cp-2f, -0x1.8p-1f},
+    {-0x1.f7efc0p-3f, 0x1.f7efc0p-3f, -0x1.ep-1f},
+    {-0x1.03323cp-2f, 0x1.38e0a2p-1f, -0x1.8p-1f},
+    {-0x1.dca414p-3f, 0x1.bcb64cp-1f, -0x1.cp-2f},
+    {-0x1.8f8b84p-3f, 0x1.f6297cp-1f, 0x0.0p+0f},
+    {0x1.8f8b84p-3f, 0x1.f6297cp-1f, 0x0.0p+0f},
+    {0x1.dca414p-3f, 0x1.bcb64cp-1f, -0x1.cp-2f},
+    {0x1.03323cp-2f, 0x1.38e0a2p-1f, -0x1.8p-1f},
+    {0x1.f7efc0p-3f, 0x1.f7efc0p-3f, -0x1.ep-1f},
+};
+
 // Voxel code of a point: 30-bit Morton code of its cell at depth `depth`
 // (<= 10) of the cube [lo, lo + size)^3; points outside clamp to the border.
 MRH_HD uint32_t svo_code(V3 p, float lox, float loy, float loz, float inv_cell, uint32_t depth)
@@ -115,70 +186,105 @@ MRH_HD uint32_t svo_code(V3 p, float lox, float loy, float loz, float inv_cell, 
 struct SvoView {
     uint32_t depth;
     uint32_t lvl_off[12];            // first global node of each level, [depth+1] = total
-    const uint32_t* first_child;     // [nodes] global index of the first child (SVO_NO_NODE at leaves)
-    const uint32_t* child_mask;      // [nodes] 8-bit child occupancy
+    const uint4* pack;               // [nodes] first child, child mask, samples in the subtree, 0
     const float* radiance;           // [nodes][64] filtered radiance
-    const uint32_t* node_count;      // [nodes] samples in the subtree (after filter)
     float lox, loy, loz, inv_cell;   // cube of the leaf grid
-    const V3* centers;               // [64] bin centre directions
 };
 
-// Deepest existing node on the path from the root to `code`'s leaf cell.
-__device__ __forceinline__ uint32_t svo_lookup(const SvoView& s, uint32_t code)
-{
-    uint32_t node = 0;   // root
-    for (uint32_t l = 0; l < s.depth; l++) {
-        const uint32_t m = s.child_mask[node];
-        const uint32_t oct = (code >> (3u * (s.depth - 1u - l))) & 7u;
-        if (!((m >> oct) & 1u)) break;
-        node = s.first_child[node] + (uint32_t)__popc(m & ((1u << oct) - 1u));
-    }
-    return node;
-}
-
-// Guide lookup: the deepest node on the path whose data rests on at least
-// min_count samples (the role of the reference's collapse of sparse bins to
-// their ancestors, AnisoSVO.cu:1485-1529, BinRayCount); the root when no child
-// qualifies.
+// Deepest node on the path from the root to `code`'s leaf cell whose
+// subtree holds >= min_count samples (min_count = 0: the deepest existing
+// node).  With min_count > 0 this is the guide lookup: the role of the
+// reference's collapse of sparse bins to their ancestors
+// (AnisoSVO.cu:1485-1529, BinRayCount); the root when no child qualifies.
+// One 16-byte record per level.
 __device__ __forceinline__ uint32_t svo_lookup_guide(const SvoView& s, uint32_t code, uint32_t min_count)
 {
-    uint32_t node = 0;
+    uint32_t node = 0;   // root
+    uint4 rec = s.pack[0];
     for (uint32_t l = 0; l < s.depth; l++) {
-        const uint32_t m = s.child_mask[node];
+        const uint32_t m = rec.y;
         const uint32_t oct = (code >> (3u * (s.depth - 1u - l))) & 7u;
         if (!((m >> oct) & 1u)) break;
-        const uint32_t child = s.first_child[node] + (uint32_t)__popc(m & ((1u << oct) - 1u));
-        if (s.node_count[child] < min_count) break;
+        const uint32_t child = rec.x + (uint32_t)__popc(m & ((1u << oct) - 1u));
+        const uint4 crec = s.pack[child];
+        if (crec.z < min_count) break;
         node = child;
+        rec = crec;
     }
     return node;
 }
 
-// Product of a node's filtered radiance and the clamped cosine around n,
-// evaluated at the bin centres.
-__device__ __forceinline__ float guide_weight(const SvoView& s, const float* __restrict__ rad, uint32_t b, V3 n)
+__device__ __forceinline__ uint32_t svo_lookup(const SvoView& s, uint32_t code) { return svo_lookup_guide(s, code, 0u); }
+
+// A node's 64 filtered radiances, held in registers
+struct GuideBins {
+    float4 r[GUIDE_BINS / 4];
+};
+
+__device__ __forceinline__ GuideBins guide_bins(const SvoView& s, uint32_t node)
 {
-    return rad[b] * fmaxf(dot(n, s.centers[b]), 0.0f);
+    const float4* p = (const float4*)(s.radiance + (size_t)node * GUIDE_BINS);
+    GuideBins g;
+#pragma unroll
+    for (int k = 0; k < (int)GUIDE_BINS / 4; k++) g.r[k] = p[k];
+    return g;
 }
 
-__device__ __forceinline__ float guide_total(const SvoView& s, uint32_t node, V3 n)
+template <int B>
+__device__ __forceinline__ float bin_radiance(const GuideBins& g)
 {
-    const float* rad = s.radiance + (size_t)node * GUIDE_BINS;
+    const float4 v = g.r[B >> 2];
+    return (B & 3) == 0 ? v.x : ((B & 3) == 1 ? v.y : ((B & 3) == 2 ? v.z : v.w));
+}
+
+// Product of bin B's filtered radiance and the clamped cosine of its centre
+// around n
+template <int B>
+__device__ __forceinline__ float guide_weight(const GuideBins& g, V3 n)
+{
+    return bin_radiance<B>(g) * fmaxf(dot(n, v3(GUIDE_CENTERS[B][0], GUIDE_CENTERS[B][1], GUIDE_CENTERS[B][2])), 0.0f);
+}
+
+template <int B>
+__device__ __forceinline__ void guide_total_acc(const GuideBins& g, V3 n, float& t)
+{
+    t += guide_weight<B>(g, n);
+    if constexpr (B + 1 < (int)GUIDE_BINS) guide_total_acc<B + 1>(g, n, t);
+}
+
+// Sum of the product weights over the 64 bins, in bin order
+__device__ __forceinline__ float guide_total(const GuideBins& g, V3 n)
+{
     float t = 0.0f;
-    for (uint32_t b = 0; b < GUIDE_BINS; b++) t += guide_weight(s, rad, b, n);
+    guide_total_acc<0>(g, n, t);
     return t;
 }
 
 // Solid-angle pdf of one bin's share of the product (equal-area bins)
 constexpr float GUIDE_BIN_SR_INV = (float)GUIDE_BINS * 0.07957747154594767f;   // 64 / (4 pi)
 
+// running-sum scan: the first bin (of positive weight) where the sum
+// exceeds target, else the last bin of positive weight
+template <int B>
+__device__ __forceinline__ void guide_pick(const GuideBins& g, V3 n, float target, bool& found, float& acc,
+                                           uint32_t& pick, float& wp)
+{
+    const float w = guide_weight<B>(g, n);
+    if (!found && w > 0.0f) {
+        pick = B;
+        wp = w;
+        acc += w;
+        found = acc > target;
+    }
+    if constexpr (B + 1 < (int)GUIDE_BINS) guide_pick<B + 1>(g, n, target, found, acc, pick, wp);
+}
+
 // Samples a bin with probability proportional to its product weight, then a
 // uniform point of the bin in the map.  pdf = 0 when the product vanishes.
-// `total` is guide_total(s, node, n).
-__device__ __forceinline__ V3 guide_sample(const SvoView& s, uint32_t node, V3 n, float total, float xi0, float xi1,
-                                           float xi2, float& pdf)
+// `total` is guide_total(g, n).
+__device__ __forceinline__ V3 guide_sample(const GuideBins& g, V3 n, float total, float xi0, float xi1, float xi2,
+                                           float& pdf)
 {
-    const float* rad = s.radiance + (size_t)node * GUIDE_BINS;
     if (!(total > 0.0f)) {
         pdf = 0.0f;
         return v3(0.0f, 0.0f, 1.0f);
@@ -186,25 +292,21 @@ __device__ __forceinline__ V3 guide_sample(const SvoView& s, uint32_t node, V3 n
     const float target = xi0 * total;
     float acc = 0.0f, wp = 0.0f;
     uint32_t pick = 0;
-    for (uint32_t b = 0; b < GUIDE_BINS; b++) {
-        const float w = guide_weight(s, rad, b, n);
-        if (w > 0.0f) {
-            pick = b;
-            wp = w;
-            acc += w;
-            if (acc > target) break;
-        }
-    }
+    bool found = false;
+    guide_pick<0>(g, n, target, found, acc, pick, wp);
     const float u = ((float)(pick % GUIDE_RES) + xi1) / (float)GUIDE_RES;
     const float v = ((float)(pick / GUIDE_RES) + xi2) / (float)GUIDE_RES;
     pdf = wp / total * GUIDE_BIN_SR_INV;
     return octa_to_dir(u, v);
 }
 
+// pdf of direction d under the node's product distribution
 __device__ __forceinline__ float guide_pdf(const SvoView& s, uint32_t node, V3 n, float total, V3 d)
 {
     if (!(total > 0.0f)) return 0.0f;
-    const float w = guide_weight(s, s.radiance + (size_t)node * GUIDE_BINS, dir_to_bin(d), n);
+    const uint32_t b = dir_to_bin(d);
+    const float w = s.radiance[(size_t)node * GUIDE_BINS + b] *
+                    fmaxf(dot(n, v3(GUIDE_CENTERS[b][0], GUIDE_CENTERS[b][1], GUIDE_CENTERS[b][2])), 0.0f);
     return w / total * GUIDE_BIN_SR_INV;
 }
 
@@ -217,10 +319,11 @@ int32_t svo_build(mrh_context* ctx, mrh_svo* s, const uint32_t* d_codes, uint32_
 int32_t svo_splat(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, const uint32_t* bins, const float* values,
                   uint32_t n);
 int32_t svo_filter(mrh_context* ctx, mrh_svo* s);
-// Order-preserving compaction of the learning records whose code is not
-// SVO_NO_NODE; *count receives the number kept (synchronous).
-int32_t svo_compact_records(mrh_context* ctx, const uint32_t* codes, const uint32_t* bins, const float* rad,
-                            uint32_t n, uint32_t* out_codes, uint32_t* out_bins, float* out_rad, uint32_t* count);
+// Compaction of the learning records whose code is not SVO_NO_NODE (in no
+// particular order); *count receives the number kept (synchronous).
+int32_t svo_compact_records(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, const uint32_t* bins,
+                            const float* rad, uint32_t n, uint32_t* out_codes, uint32_t* out_bins, float* out_rad,
+                            uint32_t* count);
 void svo_free(mrh_svo* s);
 mrh_context* svo_ctx(const mrh_svo* s);
 

@@ -27,6 +27,7 @@ enum MrhSlot : int {
     SLOT_PART_KEYS,
     SLOT_FRAME,
     SLOT_TRACE_OVERFLOW,
+    SLOT_GUIDE_REC,
     SLOT_COUNT
 };
 
@@ -53,6 +54,8 @@ struct mrh_context {
     uint32_t trace_blocks_per_cu = 8;
     // traversal tree: 2 = binary LBVH, 4 = 4-wide tree collapsed from it
     uint32_t trace_width = 4;
+    // octree reused by guided renders that do not bring their own
+    struct mrh_svo* guide_svo = nullptr;
 };
 
 namespace mrh {

@@ -480,6 +480,7 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
     bool guided = false;
     uint32_t gnode = 0;
     float gtotal = 0.0f;
+    GuideBins gbins;
     V3 ns = v3(0.0f, 0.0f, 1.0f);
     if (GUIDE && guide &&
         (batch == MRH_BATCH_LAMBERT || (batch == MRH_BATCH_METAL && (1.0f - mats.metal[(size_t)id * 10]) < 0.99f))) {
@@ -487,7 +488,8 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
         gnode = svo_lookup_guide(ga.svo,
                                  svo_code(s.pos, ga.svo.lox, ga.svo.loy, ga.svo.loz, ga.svo.inv_cell, ga.svo.depth),
                                  ga.min_samples);
-        gtotal = guide_total(ga.svo, gnode, ns);
+        gbins = guide_bins(ga.svo, gnode);
+        gtotal = guide_total(gbins, ns);
         guided = gtotal > 0.0f;
     }
     if (GUIDE && guided) {
@@ -497,7 +499,7 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
         V3 fcos;
         if (xi < ga.ratio) {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-            dir = guide_sample(ga.svo, gnode, ns, gtotal, u0, u1, u2, pg);
+            dir = guide_sample(gbins, ns, gtotal, u0, u1, u2, pg);
             if (batch == MRH_BATCH_LAMBERT) {
                 const float* a = mats.lambert + (size_t)id * 3;
                 fcos = lambert_eval(s, v3(a[0], a[1], a[2]), dir, pb);

@@ -438,7 +438,7 @@ struct GuideState {
     uint32_t *c_codes = nullptr, *c_bins = nullptr;
     float* c_rad = nullptr;
     mrh_svo* svo = nullptr;
-    bool own_svo = false, have_guide = false;
+    bool own_svo = false, own_records = false, have_guide = false;
 };
 
 int32_t guide_validate(mrh_context* ctx, const mrh_guide_params* gp, uint64_t total, uint32_t max_depth)
@@ -459,16 +459,18 @@ int32_t guide_validate(mrh_context* ctx, const mrh_guide_params* gp, uint64_t to
 
 void guide_free(GuideState& g)
 {
-    if (g.recmem) (void)hipFree(g.recmem);
+    if (g.recmem && g.own_records) (void)hipFree(g.recmem);
     g.recmem = nullptr;
     if (g.own_svo && g.svo) svo_free(g.svo);
     g.svo = nullptr;
 }
 
 // Cube = scene bounds padded by one voxel; records for frames of `total`
-// paths; `svo` (may be null) receives the guides.
+// paths; `svo` (may be null) receives the guides.  With `persistent` the
+// state owns its records and octree (the incremental tracer keeps them
+// across calls); otherwise they live in the context's reusable scratch.
 int32_t guide_init(mrh_context* ctx, const mrh_scene* s, uint64_t total, uint32_t max_depth,
-                   const mrh_guide_params* gp, mrh_svo* svo, GuideState& g)
+                   const mrh_guide_params* gp, mrh_svo* svo, bool persistent, GuideState& g)
 {
     g.gp = *gp;
     const mrh_lbvh* b = s->bvh;
@@ -483,13 +485,21 @@ int32_t guide_init(mrh_context* ctx, const mrh_scene* s, uint64_t total, uint32_
     g.cube[3] = span + 2.0f * pad;
     if (svo) {
         g.svo = svo;
-    } else {
+    } else if (persistent) {
         MRH_TRY(mrh_svo_create(ctx, &g.svo));
         g.own_svo = true;
+    } else {
+        if (!ctx->guide_svo) MRH_TRY(mrh_svo_create(ctx, &ctx->guide_svo));
+        g.svo = ctx->guide_svo;
     }
     if (gp->iterations > 1) {
         g.n_rec = total * max_depth;
-        MRH_TRY(dmalloc(ctx, &g.recmem, (size_t)g.n_rec * 36));
+        if (persistent) {
+            MRH_TRY(dmalloc(ctx, &g.recmem, (size_t)g.n_rec * 36));
+            g.own_records = true;
+        } else {
+            MRH_TRY(scratch(ctx, SLOT_GUIDE_REC, (size_t)g.n_rec * 36, &g.recmem));
+        }
         char* q = (char*)g.recmem;
         g.rec.code = (uint32_t*)q; q += g.n_rec * 4;
         g.rec.bin = (uint32_t*)q; q += g.n_rec * 4;
@@ -530,7 +540,7 @@ int32_t guide_frame(mrh_context* ctx, GuideState& g, uint32_t index, Frame& f)
 int32_t guide_learn(mrh_context* ctx, GuideState& g)
 {
     uint32_t kept = 0;
-    MRH_TRY(svo_compact_records(ctx, g.rec.code, g.rec.bin, g.rec.rad, (uint32_t)g.n_rec, g.c_codes, g.c_bins,
+    MRH_TRY(svo_compact_records(ctx, g.svo, g.rec.code, g.rec.bin, g.rec.rad, (uint32_t)g.n_rec, g.c_codes, g.c_bins,
                                 g.c_rad, &kept));
     if (kept == 0) return MRH_OK;   // nothing learned: keep the previous guide
     MRH_TRY(svo_build(ctx, g.svo, g.c_codes, kept, g.cube, g.gp.svo_depth));
@@ -558,7 +568,7 @@ int32_t render_guided(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd,
     const size_t img_count = (size_t)p->width * p->height * 3;
     MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
     GuideState g;
-    int32_t rc = guide_init(ctx, s, total, p->max_depth, gp, svo, g);
+    int32_t rc = guide_init(ctx, s, total, p->max_depth, gp, svo, false, g);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == MRH_OK && hipEventCreate(&e0) != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "event create");
     if (rc == MRH_OK && hipEventCreate(&e1) != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "event create");
@@ -766,7 +776,7 @@ int32_t mrh_tracer_set_guide(mrh_tracer* t, const mrh_guide_params* guide, mrh_s
     guide_free(t->g);
     t->g = GuideState{};
     t->guided = false;
-    MRH_TRY(guide_init(ctx, t->scene, t->total, t->p.max_depth, guide, svo, t->g));
+    MRH_TRY(guide_init(ctx, t->scene, t->total, t->p.max_depth, guide, svo, true, t->g));
     t->guided = true;
     t->guide_frames = 0;
     return MRH_OK;

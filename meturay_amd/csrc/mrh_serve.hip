@@ -27,8 +27,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <mutex>
+#include <thread>
 
 namespace mrh {
 namespace {
@@ -132,6 +135,7 @@ __global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__
 // Shade every queued ray with its request's constants (shade_ray), append
 // the survivors to the other queue; res[0] = survivors, res[RES_LIVE + s] =
 // survivors of slot s, res[RES_SEG + s] = rays of slot s traced this step.
+template <bool G>
 __global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__ hdr,
                                                      const SlotDev* __restrict__ slots, PathBuf in,
                                                      const uint32_t* __restrict__ in_slot, PathBuf out,
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__
             slot = in_slot[rid];
             const SlotDev& sd = slots[slot];
             sample = in.sample[rid];
-            alive = shade_ray<true, true>(in, hit, rid, sd.q, sd.ga, sd.learn != 0u, sd.guide != 0u, nx);
+            alive = shade_ray<G, G>(in, hit, rid, sd.q, sd.ga, G && sd.learn != 0u, G && sd.guide != 0u, nx);
             atomicAdd(&s_seg[slot], 1u);
             if (alive) atomicAdd(&s_live[slot], 1u);
         }
@@ -196,7 +200,11 @@ struct ServeReq {
     uint64_t charge = 0;       // budget bytes
     int slot = -1;
     bool learning = false, have_guide = false;
+    bool in_learn = false;     // its learn pass is queued or running on the learner
+    bool learned_ok = true;    // result of the last learn pass
+    uint32_t kept = 0;
     void* recmem = nullptr;
+    size_t recmem_bytes = 0;
     RecBuf rec = {};
     uint32_t *c_codes = nullptr, *c_bins = nullptr;
     float* c_rad = nullptr;
@@ -230,11 +238,26 @@ struct mrh_server {
     std::deque<ServeReq*> queue;
     ServeReq* slots[SERVE_MAX_SLOTS] = {};
     std::vector<mrh_request_done> done;
-    // graphs per parity, with what they were captured against
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
-    void* graph_scratch[2][SLOT_COUNT] = {};
-    uint32_t graph_cap[2] = {}, graph_w[2] = {}, graph_lds[2] = {}, graph_bpc[2] = {};
+    // graphs per shade variant (plain / learn-or-guide) and queue parity,
+    // with what they were captured against
+    hipGraphExec_t exec[2][2] = {};
+    void* graph_scratch[2][2][SLOT_COUNT] = {};
+    uint32_t graph_cap[2][2] = {}, graph_w[2][2] = {}, graph_lds[2][2] = {}, graph_bpc[2][2] = {};
     uint32_t max_tree_depth = 0;    // deepest LBVH among submitted requests
+    // learner: guide learn passes run on their own context and host thread,
+    // overlapping the wavefront steps of other requests
+    mrh_context* lctx = nullptr;
+    std::thread learner;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<ServeReq*> learn_q;   // waiting for the learner
+    std::vector<ServeReq*> learned;  // finished, waiting for the engine thread
+    bool stop = false;
+    std::string learn_err;
+    // reusable octrees (on lctx) and record buffers
+    std::vector<mrh_svo*> free_svos;
+    std::vector<std::pair<size_t, void*>> free_recs;
+    size_t pooled_bytes = 0;
 };
 
 namespace {
@@ -258,13 +281,21 @@ uint64_t request_charge(const ServeReq* q)
     return b;
 }
 
+// keeps up to this many bytes of released record buffers for reuse
+constexpr size_t REC_POOL_BYTES = 64ull << 30;
+
 void release(mrh_server* s, ServeReq* q)
 {
-    if (q->svo) svo_free(q->svo);
+    if (q->svo) s->free_svos.push_back(q->svo);
     q->svo = nullptr;
     if (q->recmem) {
         (void)hipStreamSynchronize(s->ctx->stream);
-        (void)hipFree(q->recmem);
+        if (s->pooled_bytes + q->recmem_bytes <= REC_POOL_BYTES) {
+            s->free_recs.emplace_back(q->recmem_bytes, q->recmem);
+            s->pooled_bytes += q->recmem_bytes;
+        } else {
+            (void)hipFree(q->recmem);
+        }
     }
     q->recmem = nullptr;
     if (q->slot >= 0) s->slots[q->slot] = nullptr;
@@ -341,7 +372,21 @@ int32_t admit(mrh_server* s, ServeReq* q, int slot)
         q->cube[2] = b->lo[2] - pad;
         q->cube[3] = span + 2.0f * pad;
         const uint64_t n_rec = q->total * q->r.params.max_depth;
-        MRH_TRY(dmalloc(ctx, &q->recmem, (size_t)n_rec * 36));
+        const size_t need = (size_t)n_rec * 36;
+        // smallest pooled buffer that fits, else a new one
+        int best = -1;
+        for (size_t k = 0; k < s->free_recs.size(); k++)
+            if (s->free_recs[k].first >= need && (best < 0 || s->free_recs[k].first < s->free_recs[best].first))
+                best = (int)k;
+        if (best >= 0) {
+            q->recmem_bytes = s->free_recs[best].first;
+            q->recmem = s->free_recs[best].second;
+            s->pooled_bytes -= q->recmem_bytes;
+            s->free_recs.erase(s->free_recs.begin() + best);
+        } else {
+            MRH_TRY(dmalloc(ctx, &q->recmem, need));
+            q->recmem_bytes = need;
+        }
         char* m = (char*)q->recmem;
         q->rec.code = (uint32_t*)m; m += n_rec * 4;
         q->rec.bin = (uint32_t*)m; m += n_rec * 4;
@@ -354,9 +399,70 @@ int32_t admit(mrh_server* s, ServeReq* q, int slot)
         q->c_rad = (float*)m;
         q->rec.max_depth = q->r.params.max_depth;
         MRH_TRY(fill_u32(ctx, q->rec.code, SVO_NO_NODE, n_rec));
-        MRH_TRY(mrh_svo_create(ctx, &q->svo));
+        if (!s->free_svos.empty()) {
+            q->svo = s->free_svos.back();
+            s->free_svos.pop_back();
+        } else {
+            MRH_TRY(mrh_svo_create(s->lctx, &q->svo));
+        }
         q->learning = true;
     }
+    fill_slot(s, q);
+    return upload_slot(s, q);
+}
+
+// The learn pass of one request on the learner's context (learner thread):
+// compact the records, rebuild, splat and filter the octree, reset the
+// records for the next learning frame; returns when the device is done.
+int32_t learn_pass(mrh_server* s, ServeReq* q)
+{
+    mrh_context* lc = s->lctx;
+    const uint32_t n_rec = (uint32_t)(q->total * q->r.params.max_depth);
+    uint32_t kept = 0;
+    MRH_TRY(svo_compact_records(lc, q->svo, q->rec.code, q->rec.bin, q->rec.rad, n_rec, q->c_codes, q->c_bins,
+                                q->c_rad, &kept));
+    if (kept) {
+        MRH_TRY(svo_build(lc, q->svo, q->c_codes, kept, q->cube, q->r.guide.svo_depth));
+        MRH_TRY(svo_splat(lc, q->svo, q->c_codes, q->c_bins, q->c_rad, kept));
+        MRH_TRY(svo_filter(lc, q->svo));
+    }
+    MRH_TRY(fill_u32(lc, q->rec.code, SVO_NO_NODE, n_rec));
+    MRH_HIP_CHECK(lc, hipStreamSynchronize(lc->stream));
+    q->kept = kept;
+    return MRH_OK;
+}
+
+void learner_main(mrh_server* s)
+{
+    (void)hipSetDevice(s->lctx->device);
+    for (;;) {
+        ServeReq* q = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || !s->learn_q.empty(); });
+            if (s->stop && s->learn_q.empty()) return;
+            q = s->learn_q.front();
+            s->learn_q.pop_front();
+        }
+        const int32_t rc = learn_pass(s, q);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            q->learned_ok = rc == MRH_OK;
+            if (rc != MRH_OK) s->learn_err = s->lctx->err;
+            s->learned.push_back(q);
+        }
+        s->cv.notify_all();
+    }
+}
+
+int32_t finish_request(mrh_server* s, ServeReq* q);
+
+// The request's next frame: guide from the last learn pass, fresh sample range
+int32_t next_iteration(mrh_server* s, ServeReq* q)
+{
+    q->it++;
+    q->injected = 0;
+    q->learning = q->it + 1 < q->iterations;
     fill_slot(s, q);
     return upload_slot(s, q);
 }
@@ -364,27 +470,24 @@ int32_t admit(mrh_server* s, ServeReq* q, int slot)
 // All paths of the request's current iteration have ended
 int32_t iteration_done(mrh_server* s, ServeReq* q)
 {
-    mrh_context* ctx = s->ctx;
     if (q->learning) {
-        const uint32_t n_rec = (uint32_t)(q->total * q->r.params.max_depth);
-        uint32_t kept = 0;
-        MRH_TRY(svo_compact_records(ctx, q->rec.code, q->rec.bin, q->rec.rad, n_rec, q->c_codes, q->c_bins,
-                                    q->c_rad, &kept));
-        if (kept) {
-            MRH_TRY(svo_build(ctx, q->svo, q->c_codes, kept, q->cube, q->r.guide.svo_depth));
-            MRH_TRY(svo_splat(ctx, q->svo, q->c_codes, q->c_bins, q->c_rad, kept));
-            MRH_TRY(svo_filter(ctx, q->svo));
-            q->have_guide = true;
+        // hand the records to the learner; injection waits until it is done
+        q->in_learn = true;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->learn_q.push_back(q);
         }
+        s->cv.notify_one();
+        return MRH_OK;
     }
+    if (q->it + 1 < q->iterations) return next_iteration(s, q);
+    return finish_request(s, q);
+}
+
+int32_t finish_request(mrh_server* s, ServeReq* q)
+{
+    mrh_context* ctx = s->ctx;
     q->it++;
-    if (q->it < q->iterations) {
-        q->injected = 0;
-        q->learning = q->it + 1 < q->iterations;
-        if (q->learning) MRH_TRY(fill_u32(ctx, q->rec.code, SVO_NO_NODE, q->total * q->r.params.max_depth));
-        fill_slot(s, q);
-        return upload_slot(s, q);
-    }
     // finished: the image is the mean over all samples of all iterations
     const size_t img = (size_t)q->r.params.width * q->r.params.height * 3;
     MRH_TRY(launch(ctx, "serve_finalize", dim3(stride_grid(ctx, img, 256)), dim3(256), 0, k_serve_scale,
@@ -412,8 +515,9 @@ void tree_shape(const mrh_server* s, uint32_t& w, uint32_t& cap)
     cap = (w == 4) ? (d <= 42 ? 64u : 128u) : (d <= 64 ? 64u : 128u);
 }
 
-// The launches of one step over queue parity `par` (captured or eager)
-int32_t step_launches(mrh_server* s, uint32_t par)
+// The launches of one step over queue parity `par` (captured or eager);
+// g = some admitted request learns or samples a guide
+int32_t step_launches(mrh_server* s, uint32_t par, bool g)
 {
     mrh_context* ctx = s->ctx;
     const PathBuf& in = s->bufs[par];
@@ -462,57 +566,62 @@ int32_t step_launches(mrh_server* s, uint32_t par)
     sa.end_bit = 32;
     sa.descending = true;
     MRH_TRY(radix_sort(ctx, sa));
-    return launch(ctx, "serve_shade", dim3(stride_grid(ctx, s->cap, 256)), dim3(256), 0, k_serve_shade,
+    if (g)
+        return launch(ctx, "serve_shade", dim3(stride_grid(ctx, s->cap, 256)), dim3(256), 0, k_serve_shade<true>,
+                      (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, in_slot, out, out_slot, s->hit,
+                      (const uint32_t*)s->order, s->d_res);
+    return launch(ctx, "serve_shade", dim3(stride_grid(ctx, s->cap, 256)), dim3(256), 0, k_serve_shade<false>,
                   (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, in_slot, out, out_slot, s->hit,
                   (const uint32_t*)s->order, s->d_res);
 }
 
-bool graph_valid(mrh_server* s, uint32_t par)
+bool graph_valid(mrh_server* s, uint32_t v, uint32_t par)
 {
-    if (!s->exec[par]) return false;
+    if (!s->exec[v][par]) return false;
     for (int k = 0; k < SLOT_COUNT; k++)
-        if (s->graph_scratch[par][k] != s->ctx->slot_ptr[k]) return false;
+        if (s->graph_scratch[v][par][k] != s->ctx->slot_ptr[k]) return false;
     uint32_t w, cap;
     tree_shape(s, w, cap);
-    return s->graph_cap[par] == cap && s->graph_w[par] == w && s->graph_lds[par] == s->ctx->trace_lds_entries &&
-           s->graph_bpc[par] == s->ctx->trace_blocks_per_cu;
+    return s->graph_cap[v][par] == cap && s->graph_w[v][par] == w &&
+           s->graph_lds[v][par] == s->ctx->trace_lds_entries && s->graph_bpc[v][par] == s->ctx->trace_blocks_per_cu;
 }
 
-int32_t run_step(mrh_server* s, uint32_t par)
+int32_t run_step(mrh_server* s, uint32_t par, bool g)
 {
     mrh_context* ctx = s->ctx;
-    if (!s->p.use_graphs) return step_launches(s, par);
-    if (!graph_valid(s, par)) {
-        if (s->exec[par]) {
-            (void)hipGraphExecDestroy(s->exec[par]);
-            s->exec[par] = nullptr;
+    if (!s->p.use_graphs) return step_launches(s, par, g);
+    const uint32_t v = g ? 1u : 0u;
+    if (!graph_valid(s, v, par)) {
+        if (s->exec[v][par]) {
+            (void)hipGraphExecDestroy(s->exec[v][par]);
+            s->exec[v][par] = nullptr;
         }
         // one eager step first so every scratch buffer exists at its final size
-        MRH_TRY(step_launches(s, par));
+        MRH_TRY(step_launches(s, par, g));
         MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         const bool prof = ctx->profile;
         ctx->profile = false;
-        hipGraph_t g = nullptr;
+        hipGraph_t graph = nullptr;
         MRH_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int32_t rc = step_launches(s, par);
-        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        int32_t rc = step_launches(s, par, g);
+        hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
         ctx->profile = prof;
         if (rc != MRH_OK) {
-            if (g) (void)hipGraphDestroy(g);
+            if (graph) (void)hipGraphDestroy(graph);
             return rc;
         }
         if (e != hipSuccess) return fail(ctx, MRH_ERR_HIP, "step capture: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&s->exec[par], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
+        e = hipGraphInstantiate(&s->exec[v][par], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return fail(ctx, MRH_ERR_HIP, "step graph instantiate: %s", hipGetErrorString(e));
-        for (int k = 0; k < SLOT_COUNT; k++) s->graph_scratch[par][k] = ctx->slot_ptr[k];
-        tree_shape(s, s->graph_w[par], s->graph_cap[par]);
-        s->graph_lds[par] = ctx->trace_lds_entries;
-        s->graph_bpc[par] = ctx->trace_blocks_per_cu;
+        for (int k = 0; k < SLOT_COUNT; k++) s->graph_scratch[v][par][k] = ctx->slot_ptr[k];
+        tree_shape(s, s->graph_w[v][par], s->graph_cap[v][par]);
+        s->graph_lds[v][par] = ctx->trace_lds_entries;
+        s->graph_bpc[v][par] = ctx->trace_blocks_per_cu;
         // the eager step above already did this step's work
         return MRH_OK;
     }
-    MRH_HIP_CHECK(ctx, hipGraphLaunch(s->exec[par], ctx->stream));
+    MRH_HIP_CHECK(ctx, hipGraphLaunch(s->exec[v][par], ctx->stream));
     return MRH_OK;
 }
 
@@ -543,6 +652,20 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
             return rc;
         }
     }
+    // requests whose learn pass finished: next frame, guided
+    std::vector<ServeReq*> ready;
+    std::string lerr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        ready.swap(s->learned);
+        lerr = s->learn_err;
+    }
+    for (ServeReq* q : ready) {
+        q->in_learn = false;
+        if (!q->learned_ok) return fail(ctx, MRH_ERR_HIP, "guide learn pass failed: %s", lerr.c_str());
+        if (q->kept) q->have_guide = true;
+        MRH_TRY(next_iteration(s, q));
+    }
     // injection, smallest remaining work first
     std::vector<ServeReq*> act;
     for (uint32_t k = 0; k < s->p.max_slots; k++)
@@ -572,7 +695,10 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
     h.n_total = s->n_live + (uint32_t)inj;
     if (h.n_total > 0) {
         MRH_HIP_CHECK(ctx, hipMemcpyAsync(s->d_hdr, s->h_hdr, sizeof(StepHdr), hipMemcpyHostToDevice, ctx->stream));
-        MRH_TRY(run_step(s, s->parity));
+        bool g = false;
+        for (uint32_t k = 0; k < s->p.max_slots; k++)
+            if (s->slots[k] && (s->slots[k]->learning || s->slots[k]->have_guide)) g = true;
+        MRH_TRY(run_step(s, s->parity, g));
         MRH_HIP_CHECK(ctx, hipMemcpyAsync(s->h_res, s->d_res, RES_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
         MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         s->steps++;
@@ -589,10 +715,15 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
     // iterations / requests whose paths have all ended
     for (uint32_t k = 0; k < s->p.max_slots; k++) {
         ServeReq* q = s->slots[k];
-        if (q && q->injected == q->total && q->live == 0) MRH_TRY(iteration_done(s, q));
+        if (q && !q->in_learn && q->injected == q->total && q->live == 0) MRH_TRY(iteration_done(s, q));
     }
     uint32_t a = 0;
     for (uint32_t k = 0; k < s->p.max_slots; k++) a += s->slots[k] ? 1u : 0u;
+    // nothing to trace while learn passes run: wait for one instead of spinning
+    if (h.n_total == 0 && a > 0) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv.wait_for(lk, std::chrono::milliseconds(2), [&] { return !s->learned.empty(); });
+    }
     if (active) *active = a;
     if (queued) *queued = (uint32_t)s->queue.size();
     return MRH_OK;
@@ -600,17 +731,29 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
 
 void server_free(mrh_server* s)
 {
+    if (s->learner.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        s->learner.join();
+    }
     if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
     for (uint32_t k = 0; k < SERVE_MAX_SLOTS; k++)
         if (s->slots[k]) release(s, s->slots[k]);
     for (ServeReq* q : s->queue) delete q;
-    for (int par = 0; par < 2; par++)
-        if (s->exec[par]) (void)hipGraphExecDestroy(s->exec[par]);
+    for (int v = 0; v < 2; v++)
+        for (int par = 0; par < 2; par++)
+            if (s->exec[v][par]) (void)hipGraphExecDestroy(s->exec[v][par]);
+    for (mrh_svo* v : s->free_svos) svo_free(v);
+    for (auto& r : s->free_recs) (void)hipFree(r.second);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->h_hdr) (void)hipHostFree(s->h_hdr);
     if (s->h_slots) (void)hipHostFree(s->h_slots);
     if (s->h_res) (void)hipHostFree(s->h_res);
     if (s->ctx) mrh_context_destroy(s->ctx);
+    if (s->lctx) mrh_context_destroy(s->lctx);
     delete s;
 }
 
@@ -628,6 +771,12 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
     if (rc != MRH_OK) {
         delete s;
         return fail(user, rc, "server: context creation failed");
+    }
+    rc = mrh_context_create(user->device, &s->lctx);
+    if (rc != MRH_OK) {
+        mrh_context_destroy(s->ctx);
+        delete s;
+        return fail(user, rc, "server: learner context creation failed");
     }
     mrh_context* ctx = s->ctx;
     ctx->trace_lds_entries = user->trace_lds_entries;
@@ -672,6 +821,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
         rc = fill_u32(ctx, (uint32_t*)s->d_slots, 0u, sizeof(SlotDev) * SERVE_MAX_SLOTS / 4);
     }
     if (rc == MRH_OK) rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? MRH_OK : MRH_ERR_HIP;
+    if (rc == MRH_OK) s->learner = std::thread(learner_main, s);
     if (rc != MRH_OK) {
         if (s->user != nullptr) mirror_error(s, rc);
         server_free(s);

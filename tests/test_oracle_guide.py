@@ -251,3 +251,16 @@ def test_guided_render_single_iteration_is_plain(oracle):
     b, sb, _ = oracle.render_guided(sc, bvh, cam, 16, 16, 2, 5, 2, 3, 1, 5)
     assert sa == sb
     assert np.array_equal(a, b)
+
+
+def test_bin_centre_table_is_the_map(oracle):
+    """The kernels' bin-centre table is exactly octa_to_dir of the centres."""
+    import os
+    import re
+    src = open(os.path.join(os.path.dirname(__file__), "..", "meturay_amd", "csrc", "mrh_guide.hpp")).read()
+    body = src[src.index("GUIDE_CENTERS[GUIDE_BINS][3] = {"):]
+    body = body[:body.index("};")]
+    vals = [float.fromhex(t) for t in re.findall(r"([-+]?0x[0-9a-fA-F.]+p[-+]?\d+)f", body)]
+    table = np.array(vals, dtype=np.float32).reshape(64, 3)
+    uv = np.array([[(b % 8 + 0.5) / 8, (b // 8 + 0.5) / 8] for b in range(64)], dtype=np.float32)
+    assert table.tobytes() == oracle.octa_to_dirs(uv).tobytes()

@@ -216,9 +216,13 @@ typedef struct {
     uint32_t samples;        /* strata per axis ("Samples"), spp = samples^2 */
     uint32_t max_depth;      /* "MaxDepth" */
     uint32_t rr_start;       /* "RussianRouletteStart" */
-    uint32_t nee;            /* "NextEventEstimation": only 0 is implemented */
+    uint32_t nee;            /* "NextEventEstimation": 1 = one light sample per
+                                non-specular vertex (uniform light choice over the
+                                emissive triangles, uniform point on it) */
     uint32_t seed;
     uint32_t count_traversal; /* 1: also count node visits / triangle tests (slower) */
+    uint32_t direct_light_mis; /* "DirectLightMIS": 1 = power-heuristic MIS of the
+                                NEE and BxDF strategies (used with nee = 1 only) */
 } mrh_render_params;
 
 typedef struct {

@@ -19,6 +19,8 @@ struct mrh_scene {
     float* dielectric = nullptr;
     float* emissive = nullptr;
     uint32_t n_lambert = 0, n_metal = 0, n_dielectric = 0, n_emissive = 0;
+    uint32_t* lights = nullptr;     // leaf index of every emissive triangle, in primitive order
+    uint32_t n_lights = 0;
     uint64_t bytes = 0;
     std::vector<void*> allocs;
 };
@@ -34,12 +36,22 @@ struct Cam {
     float pw, ph, near_d, far_d;
 };
 
+// Ray word dm: depth (bits 0..7; 1 = camera ray), the ray type (RayAuxPath
+// type: bit 8 = SPECULAR_PATH_RAY, bit 9 = NEE_RAY, neither = camera or
+// PATH_RAY) and the medium (bits 10..31; 0 = vacuum, k + 1 = inside
+// dielectric k).
+constexpr uint32_t DM_SPEC = 1u << 8;
+constexpr uint32_t DM_NEE = 1u << 9;
+constexpr uint32_t DM_MEDIUM_SHIFT = 10;
+constexpr uint32_t MAX_DIELECTRICS = (1u << 22) - 1u;
+
 struct PathBuf {
     float *ox, *oy, *oz, *dx, *dy, *dz;
     float *tr, *tg, *tb;
     uint32_t* sample;
-    uint32_t* dm;        // depth (bits 0..7) | medium << 8
-    uint64_t* rng;
+    uint32_t* dm;        // depth | type | medium (above)
+    uint64_t* rng;       // PCG32 state; a NEE ray holds its light's leaf instead
+    float* pdf;          // BxDF pdf of a path ray's direction (RayAuxPath prevPDF), with MIS only
 };
 
 struct HitBuf {
@@ -414,6 +426,10 @@ struct ShadeReq {
     const float4* leaves;
     float* image;            // accumulation, [pixels][3]
     uint32_t spp, sample_base, rr_start, max_depth;
+    const uint32_t* lights;  // leaf index of every emissive triangle, in primitive order
+    uint32_t n_lights;
+    uint32_t nee;            // next-event estimation ("NextEventEstimation")
+    uint32_t mis;            // power-heuristic MIS of NEE and BxDF rays ("DirectLightMIS"), with nee only
 };
 
 // Next-bounce state of a surviving path
@@ -421,45 +437,147 @@ struct RayNext {
     V3 o, d, T;
     uint32_t dm;
     uint64_t rng;
+    float pdf;
 };
+
+// TracerFunctions.cuh:302-309 (power 2, one sample per strategy)
+__device__ __forceinline__ float power_heuristic(float pdf0, float pdf1)
+{
+    const float w0 = 1.0f * pdf0;
+    const float w1 = 1.0f * pdf1;
+    return (w0 * w0) / (w0 * w0 + w1 * w1);
+}
+
+// TriFunctions::Area (GPUPrimitiveTriangle.h:321-340)
+__device__ __forceinline__ float tri_area(V3 p0, V3 p1, V3 p2)
+{
+    const V3 c = cross(p1 - p0, p2 - p0);
+    return sqrtf(dot(c, c)) * 0.5f;
+}
+
+// Solid-angle pdf of reaching a point of light triangle (p0, p1, p2) at
+// distance t along unit direction d: GPULight<PGroup>::Pdf, hit form
+// (GPULightPrimitive.cuh:245-256, PositionPdfFromHit = 1 / area) — the
+// conditional of GPULightSamplerUniform::Pdf (GPULightSamplerUniform.cuh:166-192).
+__device__ __forceinline__ float light_pdf_hit(V3 p0, V3 p1, V3 p2, float t, V3 d)
+{
+    const float pdf = 1.0f / tri_area(p0, p1, p2);
+    const V3 n = normalize(cross(p1 - p0, p2 - p0));
+    const float ndl = fabsf(dot(n, -d));
+    return pdf * t * t / ndl;
+}
+
+// GPULightSamplerUniform::SampleLight (GPULightSamplerUniform.cuh:82-130)
+// over the scene's emissive triangles, with GPULight<PGroup>::Sample
+// (GPULightPrimitive.cuh:139-166) and the triangle's uniform point
+// (GPUPrimitiveTriangle.h:65-110, Osada 2002).  Draws three numbers: light
+// index, then the two barycentric numbers.  Returns the light's leaf, the
+// unit direction from p, the squared distance and the pdf (solid angle x
+// 1 / light count).  A primitive-backed light is traced to its first hit
+// (tMax = FLT_MAX), so no distance is returned.
+__device__ __forceinline__ uint32_t sample_light(const ShadeReq& q, V3 p, Rng& rng, V3& dir, float& dist2,
+                                                 float& pdf)
+{
+    float r1 = rng.uniform();
+    r1 *= (float)q.n_lights;
+    uint32_t idx = (uint32_t)r1;
+    if (idx == q.n_lights) idx--;
+    const uint32_t leaf = q.lights[idx];
+    const float4 l0 = q.leaves[3 * (size_t)leaf + 0];
+    const float4 l1 = q.leaves[3 * (size_t)leaf + 1];
+    const float4 l2 = q.leaves[3 * (size_t)leaf + 2];
+    const V3 p0 = v3(l0.x, l0.y, l0.z), p1 = v3(l1.x, l1.y, l1.z), p2 = v3(l2.x, l2.y, l2.z);
+    const float xi0 = rng.uniform();
+    const float xi1 = rng.uniform();
+    const float s1 = sqrtf(xi0);
+    const float s2 = xi1;
+    const float a = 1.0f - s1;
+    const float b = (1.0f - s2) * s1;
+    const float c = s1 * s2;
+    pdf = 1.0f / tri_area(p0, p1, p2);
+    const V3 n = normalize(cross(p1 - p0, p2 - p0));
+    const V3 lp = (scale(p0, a) + scale(p1, b)) + scale(p2, c);
+    dir = lp - p;
+    dist2 = dot(dir, dir);
+    const float dist = sqrtf(dist2);
+    dir = scale(dir, 1.0f / dist);
+    const float ndl = fabsf(dot(n, -dir));
+    pdf *= dist2 / ndl;
+    pdf *= 1.0f / (float)q.n_lights;
+    return leaf;
+}
+
+__device__ __forceinline__ void add_radiance(const ShadeReq& q, uint32_t sample, V3 c)
+{
+    float* px = q.image + (size_t)(sample / q.spp) * 3;
+    atomicAdd(px + 0, c.x);
+    atomicAdd(px + 1, c.y);
+    atomicAdd(px + 2, c.z);
+}
 
 // PathTracerKC.cuh:337-426 (light hit) and 430-648 (surface work) for ray
 // rid; with learn the WFPG path-node bookkeeping (WFPGTracerKC.cuh:202-520)
 // and with guide the one-sample MIS of guide and BxDF
 // (WFPGTracerKC.cuh:1480-1553, pdf = r p_guide + (1 - r) p_bxdf,
-// weight = f cos / pdf).  LEARN / GUIDE compile the two parts in; learn /
-// guide switch them per request.  The ray's depth is dm & 0xFF (1 = camera
-// ray); at depth max_depth only emission is collected.  Returns true when the
-// path continues, with its next state in nx.
-template <bool LEARN, bool GUIDE, class Q, class G>
+// weight = f cos / pdf).  LEARN / GUIDE / NEE compile the three parts in;
+// learn / guide / q.nee switch them per request (NEE is not combined with
+// LEARN or GUIDE).  The ray's depth is dm & 0xFF (1 = camera ray); at depth
+// max_depth only emission is collected.  Returns true when the path
+// continues, with its next state in nx; with NEE, has_nee says whether a
+// next-event ray was spawned, in nee (traced with the next bounce).
+template <bool LEARN, bool GUIDE, bool NEE, class Q, class G>
 __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, uint32_t rid, const Q& q,
-                                          const G& ga, bool learn, bool guide, RayNext& nx)
+                                          const G& ga, bool learn, bool guide, RayNext& nx, RayNext& nee,
+                                          bool& has_nee)
 {
+    has_nee = false;
     const uint32_t key = hit.key[rid];
     const uint32_t batch = key >> 24, id = key & 0xFFFFFFu;
     if (batch == MRH_BATCH_NULL) return false;
     const V3 T = v3(in.tr[rid], in.tg[rid], in.tb[rid]);
     const uint32_t sample = in.sample[rid];
     const uint32_t dm = in.dm[rid];
-    const uint32_t depth = dm & 0xFFu, medium = dm >> 8;
+    const uint32_t depth = dm & 0xFFu, medium = dm >> DM_MEDIUM_SHIFT;
     const Materials& mats = q.mats;
+    if (NEE && (dm & DM_NEE)) {
+        // a NEE ray counts only where it hits the light it was sent to
+        // (isCorrectNEERay); any other hit, or a miss, means occluded
+        if (batch == MRH_BATCH_EMISSIVE && hit.leaf[rid] == (uint32_t)in.rng[rid]) {
+            const float* le = mats.emissive + (size_t)id * 3;
+            add_radiance(q, sample, v3(le[0] * T.x, le[1] * T.y, le[2] * T.z));
+        }
+        return false;
+    }
     if (batch == MRH_BATCH_EMISSIVE) {
         const float* le = mats.emissive + (size_t)id * 3;
-        float* px = q.image + (size_t)(sample / q.spp) * 3;
-        atomicAdd(px + 0, le[0] * T.x);
-        atomicAdd(px + 1, le[1] * T.y);
-        atomicAdd(px + 2, le[2] * T.z);
+        V3 c = v3(le[0] * T.x, le[1] * T.y, le[2] * T.z);
+        if (NEE && q.nee && depth > 1u && !(dm & DM_SPEC)) {
+            // a path ray that left a non-specular vertex, which already sent
+            // a NEE ray: without MIS the light was counted by that ray, with
+            // MIS this hit takes the BxDF strategy's power-heuristic weight
+            if (!q.mis) return false;
+            const uint32_t li = hit.leaf[rid];
+            const float4 l0 = q.leaves[3 * (size_t)li + 0];
+            const float4 l1 = q.leaves[3 * (size_t)li + 1];
+            const float4 l2 = q.leaves[3 * (size_t)li + 2];
+            const V3 d = normalize(v3(in.dx[rid], in.dy[rid], in.dz[rid]));
+            const float pc = light_pdf_hit(v3(l0.x, l0.y, l0.z), v3(l1.x, l1.y, l1.z), v3(l2.x, l2.y, l2.z),
+                                           hit.t[rid], d);
+            const float pm = 1.0f / (float)q.n_lights;
+            c = scale(c, power_heuristic(in.pdf[rid], pc * pm));
+        }
+        add_radiance(q, sample, c);
         if (LEARN && learn) {
             // radiance arriving along each earlier vertex's outgoing ray:
             // path contribution over that vertex's throughput
             // (PathNode.cuh:152-190, per channel, then luminance)
-            const V3 c = mul(v3(le[0], le[1], le[2]), T);
+            const V3 cl = mul(v3(le[0], le[1], le[2]), T);
             const size_t r0 = (size_t)sample * ga.rec.max_depth;
             for (uint32_t k = 0; k + 1 < depth; k++) {
                 if (ga.rec.code[r0 + k] == SVO_NO_NODE) continue;
                 const float fr = ga.rec.tr[r0 + k], fg = ga.rec.tg[r0 + k], fb = ga.rec.tb[r0 + k];
-                const V3 L = v3(fr == 0.0f ? 0.0f : c.x / fr, fg == 0.0f ? 0.0f : c.y / fg,
-                                fb == 0.0f ? 0.0f : c.z / fb);
+                const V3 L = v3(fr == 0.0f ? 0.0f : cl.x / fr, fg == 0.0f ? 0.0f : cl.y / fg,
+                                fb == 0.0f ? 0.0f : cl.z / fb);
                 ga.rec.rad[r0 + k] = luminance(L);
             }
         }
@@ -475,6 +593,38 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
                                    hit.bb[rid], d);
     const V3 wi = -normalize(d);
     Rng rng{in.rng[rid], pcg32_inc(q.sample_base + sample)};
+    if (NEE && q.nee && q.n_lights &&
+        (batch == MRH_BATCH_LAMBERT || (batch == MRH_BATCH_METAL && (1.0f - mats.metal[(size_t)id * 10]) < 0.99f))) {
+        // PathTracerKC.cuh:493-575: one light sample per non-specular vertex
+        V3 ldir;
+        float dist2, pl;
+        const uint32_t lleaf = sample_light(q, s.pos, rng, ldir, dist2, pl);
+        float pb;
+        V3 f;
+        if (batch == MRH_BATCH_LAMBERT) {
+            const float* a = mats.lambert + (size_t)id * 3;
+            f = lambert_eval(s, v3(a[0], a[1], a[2]), ldir, pb);
+        } else {
+            f = metal_eval(s, wi, mats.metal + (size_t)id * 10, ldir, pb);
+        }
+        float pn = pl;
+        if (q.mis) {
+            pn /= power_heuristic(pl, pb);
+            if (isnan(pn)) pn = 0.0f;
+        }
+        V3 nf = mul(T, f);
+        nf = (pn == 0.0f) ? v3(0.0f, 0.0f, 0.0f) : divs(nf, pn);
+        // a light point on the vertex itself has no direction: no ray
+        if (dist2 > 0.0f && !is_zero(nf)) {
+            nee.o = nudge(s.pos, s.geo_n);
+            nee.d = ldir;
+            nee.T = nf;
+            nee.dm = depth | DM_NEE;
+            nee.rng = lleaf;
+            nee.pdf = 0.0f;
+            has_nee = true;
+        }
+    }
     BsdfOut o;
     o.medium = medium;
     bool guided = false;
@@ -556,15 +706,16 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
     nx.o = o.o;
     nx.d = o.d;
     nx.T = t2;
-    nx.dm = ((depth + 1u) & 0xFFu) | (o.medium << 8);
+    nx.dm = ((depth + 1u) & 0xFFu) | (o.specular ? DM_SPEC : 0u) | (o.medium << DM_MEDIUM_SHIFT);
     nx.rng = rng.state;
+    nx.pdf = o.pdf;
     return true;
 }
 
 // Appends the surviving rays of a wave into out at the counter *ctr
-// (one atomic per wave); returns the slot of this lane (undefined if !alive).
+// (one atomic per wave); with_pdf also stores the ray's BxDF pdf.
 __device__ __forceinline__ void append_ray(bool alive, const RayNext& nx, uint32_t sample, PathBuf& out,
-                                           uint32_t* ctr, uint32_t* slot_out, uint32_t slot)
+                                           uint32_t* ctr, uint32_t* slot_out, uint32_t slot, bool with_pdf)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t m = __ballot(alive);
@@ -581,6 +732,7 @@ __device__ __forceinline__ void append_ray(bool alive, const RayNext& nx, uint32
         out.sample[k] = sample;
         out.dm[k] = nx.dm;
         out.rng[k] = nx.rng;
+        if (with_pdf) out.pdf[k] = nx.pdf;
         if (slot_out) slot_out[k] = slot;
     }
 }

@@ -132,23 +132,26 @@ int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, cons
     return launch_trace_pt_w<2, 128, STATS>(ctx, grid, b, ts, in, counts, bounce, tmin, tmax, hit);
 }
 
-// One launch over all partitions of a bounce (see shade_ray)
-template <bool LEARN, bool GUIDE>
+// One launch over all partitions of a bounce (see shade_ray); with NEE a
+// path appends its next-event ray beside its continuing ray.
+template <bool LEARN, bool GUIDE, bool NEE>
 __global__ __launch_bounds__(256) void k_shade(PathBuf in, PathBuf out, HitBuf hit, const uint32_t* __restrict__ order,
                                                uint32_t* counts, uint32_t bounce, ShadeReq q, GuideArgs ga)
 {
     const uint32_t n = counts[bounce];
+    const bool with_pdf = NEE && q.mis;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t j = base + threadIdx.x;
-        bool alive = false;
-        RayNext nx;
+        bool alive = false, has_nee = false;
+        RayNext nx, ne;
         uint32_t sample = 0;
         if (j < n) {
             const uint32_t rid = order[j];
             sample = in.sample[rid];
-            alive = shade_ray<LEARN, GUIDE>(in, hit, rid, q, ga, true, true, nx);
+            alive = shade_ray<LEARN, GUIDE, NEE>(in, hit, rid, q, ga, true, true, nx, ne, has_nee);
         }
-        append_ray(alive, nx, sample, out, &counts[bounce + 1], nullptr, 0u);
+        if (NEE) append_ray(has_nee, ne, sample, out, &counts[bounce + 1], nullptr, 0u, false);
+        append_ray(alive, nx, sample, out, &counts[bounce + 1], nullptr, 0u, with_pdf);
     }
 }
 
@@ -163,12 +166,16 @@ constexpr uint32_t MAX_BOUNCES = 62;
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Live rays a frame of `total` paths can hold: every path has one ray in
+// flight, and with NEE a second (its next-event ray).
+uint64_t frame_cap(uint64_t total, uint32_t nee) { return nee ? 2 * total : total; }
+
 size_t frame_bytes(uint64_t cap)
 {
-    // 2 path buffers (9 floats + sample + dm = 11 words, + u64 rng),
+    // 2 path buffers (9 floats + sample + dm + pdf = 12 words, + u64 rng),
     // hits (4 words) + key + order, plus counts/offsets
     size_t per = 0;
-    per += 2 * (11 * align_up(cap * 4) + align_up(cap * 8));
+    per += 2 * (12 * align_up(cap * 4) + align_up(cap * 8));
     per += 6 * align_up(cap * 4);
     per += 4096;
     return per;
@@ -181,8 +188,9 @@ struct Frame {
     HitBuf hit;
     uint32_t *order = nullptr, *counts = nullptr, *offsets = nullptr;
     Cam cam;
-    uint64_t total = 0, spp = 0;
+    uint64_t total = 0, cap = 0, spp = 0;
     uint32_t w = 0, h = 0, S = 0, max_depth = 0, rr_start = 0, seed = 0, sample_base = 0;
+    uint32_t nee = 0, mis = 0;
     uint32_t count_traversal = 0, bounce = 0;
     uint32_t learn = 0, guide = 0;   // record path vertices / sample the guide
     GuideArgs ga = {};
@@ -204,6 +212,7 @@ void carve_frame(void* base, uint64_t cap, Frame& f)
         q.sample = (uint32_t*)take(cap * 4);
         q.dm = (uint32_t*)take(cap * 4);
         q.rng = (uint64_t*)take(cap * 8);
+        q.pdf = (float*)take(cap * 4);
     }
     f.hit.leaf = (uint32_t*)take(cap * 4);
     f.hit.t = (float*)take(cap * 4);
@@ -213,6 +222,19 @@ void carve_frame(void* base, uint64_t cap, Frame& f)
     f.order = (uint32_t*)take(cap * 4);
     f.counts = (uint32_t*)take(COUNTS_WORDS * 4);
     f.offsets = (uint32_t*)take(260 * 4);
+}
+
+// Light list of the scene (the primitive-backed lights of
+// CPULightGroup::InitializeGroup, GPULightPrimitive.hpp:42-110: one light per
+// emissive triangle, in primitive order): rank[p] is the light index of
+// primitive p (~0 for other primitives); lights[rank] = its sorted leaf.
+__global__ void k_light_table(const float4* __restrict__ leaves, uint32_t n, const uint32_t* __restrict__ rank,
+                              uint32_t* __restrict__ lights)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t r = rank[__float_as_uint(leaves[3 * (size_t)i].w)];
+        if (r != 0xFFFFFFFFu) lights[r] = i;
+    }
 }
 
 template <class T>
@@ -253,8 +275,15 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
             return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: primitive %u has invalid key 0x%08x", i,
                         d->prim_key[i]);
     }
+    if (d->n_dielectric > MAX_DIELECTRICS)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: at most %u dielectric materials", MAX_DIELECTRICS);
+    std::vector<uint32_t> rank(d->n);
+    uint32_t n_lights = 0;
+    for (uint32_t i = 0; i < d->n; i++)
+        rank[i] = ((d->prim_key[i] >> 24) == MRH_BATCH_EMISSIVE) ? n_lights++ : 0xFFFFFFFFu;
     mrh_scene* s = new mrh_scene();
     s->ctx = ctx;
+    s->n_lights = n_lights;
     s->n_lambert = d->n_lambert;
     s->n_metal = d->n_metal;
     s->n_dielectric = d->n_dielectric;
@@ -272,6 +301,26 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
         uint32_t bits = d->morton_bits ? d->morton_bits : 30u;
         if ((rc = lbvh_create(ctx, dtris, d->n, bits, dkeys, &s->bvh)) != MRH_OK) break;
         s->bytes += (size_t)d->n * (36 + 24 + 8 + 8 + 4 + 12 + 8 + 48 + 64 + 4);
+        if (n_lights) {
+            uint32_t* drank = nullptr;
+            if ((rc = dmalloc(ctx, (void**)&drank, (size_t)d->n * 4)) != MRH_OK) break;
+            void* lp = nullptr;
+            if ((rc = dmalloc(ctx, &lp, (size_t)n_lights * 4)) != MRH_OK) {
+                hipFree(drank);
+                break;
+            }
+            s->allocs.push_back(lp);
+            s->bytes += (size_t)n_lights * 4;
+            s->lights = (uint32_t*)lp;
+            if (hipMemcpy(drank, rank.data(), (size_t)d->n * 4, hipMemcpyHostToDevice) != hipSuccess)
+                rc = fail(ctx, MRH_ERR_HIP, "mrh_scene_create: light rank upload failed");
+            if (rc == MRH_OK)
+                rc = launch(ctx, "light_table", dim3(stride_grid(ctx, d->n, 256)), dim3(256), 0, k_light_table,
+                            (const float4*)s->bvh->leaves, d->n, (const uint32_t*)drank, s->lights);
+            if (rc == MRH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+                rc = fail(ctx, MRH_ERR_HIP, "mrh_scene_create: light table failed");
+            hipFree(drank);
+        }
     } while (0);
     if (rc != MRH_OK) {
         scene_free(s);
@@ -283,7 +332,8 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
 
 int32_t validate_params(mrh_context* ctx, const mrh_scene* s, const mrh_render_params* p, uint64_t* total)
 {
-    if (p->nee) return fail(ctx, MRH_ERR_UNSUPPORTED, "NextEventEstimation is not implemented");
+    if (p->nee > 1 || p->direct_light_mis > 1)
+        return fail(ctx, MRH_ERR_INVALID, "render: nee and direct_light_mis are 0 or 1");
     if (p->width == 0 || p->height == 0 || p->samples == 0)
         return fail(ctx, MRH_ERR_INVALID, "render: empty frame");
     if (p->max_depth == 0 || p->max_depth > MAX_BOUNCES)
@@ -311,10 +361,13 @@ int32_t frame_begin(mrh_context* ctx, const mrh_camera_desc* cd, const mrh_rende
     f.seed = p->seed;
     f.sample_base = sample_base;
     f.count_traversal = p->count_traversal;
+    f.nee = p->nee;
+    f.mis = p->nee && p->direct_light_mis;
+    f.cap = frame_cap(f.total, f.nee);
     f.bounce = 0;
     f.learn = f.guide = 0;
     f.cam = make_camera(*cd, p->width, p->height);
-    carve_frame(arena, f.total, f);
+    carve_frame(arena, f.cap, f);
     MRH_TRY(fill_u32(ctx, f.counts, 0u, COUNTS_WORDS));
     MRH_TRY(fill_u32(ctx, f.counts, (uint32_t)f.total, 1));
     return launch(ctx, "raygen", dim3(stride_grid(ctx, f.total, 256)), dim3(256), 0, k_raygen, f.cam,
@@ -330,8 +383,8 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     const PathBuf& outb = f.bufs[(bounce + 1) & 1];
     const float tmin = (bounce == 0) ? f.cam.near_d : 0.0f;
     const float tmax = (bounce == 0) ? f.cam.far_d : FLT_MAX;
-    const uint32_t grid = stride_grid(ctx, f.total, 256);
-    const uint32_t tgrid = trace_grid(ctx, f.total);
+    const uint32_t grid = stride_grid(ctx, f.cap, 256);
+    const uint32_t tgrid = trace_grid(ctx, f.cap);
     const float4* lv = (const float4*)b->leaves;
     if (f.count_traversal) {
         MRH_TRY(launch_trace_pt<true>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit));
@@ -341,7 +394,7 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     SortArgs sa;
     sa.keys_in = f.hit.key;
     sa.vals_out = f.order;
-    sa.n_max = (uint32_t)f.total;
+    sa.n_max = (uint32_t)f.cap;
     sa.n_dev = f.counts + bounce;
     sa.key_bytes = 4;
     sa.begin_bit = 24;
@@ -358,14 +411,19 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     q.sample_base = f.sample_base;
     q.rr_start = f.rr_start;
     q.max_depth = f.max_depth;
+    q.lights = s->lights;
+    q.n_lights = s->n_lights;
+    q.nee = f.nee;
+    q.mis = f.mis;
     auto shade = [&](auto kernel) {
         return launch(ctx, "shade", dim3(grid), dim3(256), 0, kernel, in, outb, f.hit, (const uint32_t*)f.order,
                       f.counts, bounce, q, f.ga);
     };
-    if (f.learn && f.guide) MRH_TRY(shade(k_shade<true, true>));
-    else if (f.learn) MRH_TRY(shade(k_shade<true, false>));
-    else if (f.guide) MRH_TRY(shade(k_shade<false, true>));
-    else MRH_TRY(shade(k_shade<false, false>));
+    if (f.learn && f.guide) MRH_TRY(shade(k_shade<true, true, false>));
+    else if (f.learn) MRH_TRY(shade(k_shade<true, false, false>));
+    else if (f.guide) MRH_TRY(shade(k_shade<false, true, false>));
+    else if (f.nee) MRH_TRY(shade(k_shade<false, false, true>));
+    else MRH_TRY(shade(k_shade<false, false, false>));
     f.bounce++;
     return MRH_OK;
 }
@@ -394,7 +452,7 @@ int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const 
     uint64_t total = 0;
     MRH_TRY(validate_params(ctx, s, p, &total));
     void* arena = nullptr;
-    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(total), &arena));
+    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(frame_cap(total, p->nee)), &arena));
     const size_t img_count = (size_t)p->width * p->height * 3;
     MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -563,8 +621,9 @@ int32_t render_guided(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd,
     uint64_t total = 0;
     MRH_TRY(validate_params(ctx, s, p, &total));
     MRH_TRY(guide_validate(ctx, gp, total, p->max_depth));
+    if (p->nee) return fail(ctx, MRH_ERR_UNSUPPORTED, "render_guided: next-event estimation with guiding is not implemented");
     void* arena = nullptr;
-    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(total), &arena));
+    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(frame_cap(total, p->nee)), &arena));
     const size_t img_count = (size_t)p->width * p->height * 3;
     MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
     GuideState g;
@@ -666,7 +725,8 @@ int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes)
 {
     if (!params || !bytes) return MRH_ERR_INVALID;
     const uint64_t total = (uint64_t)params->width * params->height * params->samples * params->samples;
-    *bytes = frame_bytes(total) + total * 16;   // + sort scratch
+    const uint64_t cap = frame_cap(total, params->nee);
+    *bytes = frame_bytes(cap) + cap * 16;   // + sort scratch
     return MRH_OK;
 }
 
@@ -709,7 +769,7 @@ int32_t mrh_tracer_create(mrh_context* ctx, mrh_scene* scene, const mrh_render_p
     t->p = *params;
     t->total = total;
     const size_t img = (size_t)params->width * params->height * 3;
-    int32_t rc = dmalloc(ctx, &t->arena, frame_bytes(total));
+    int32_t rc = dmalloc(ctx, &t->arena, frame_bytes(frame_cap(total, params->nee)));
     if (rc == MRH_OK) rc = dmalloc(ctx, (void**)&t->accum, img * 4);
     if (rc == MRH_OK) rc = fill_u32(ctx, (uint32_t*)t->accum, 0u, img);
     if (rc != MRH_OK) {
@@ -773,6 +833,8 @@ int32_t mrh_tracer_set_guide(mrh_tracer* t, const mrh_guide_params* guide, mrh_s
     if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_set_guide: frame still rendering");
     if (svo && svo_ctx(svo) != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_set_guide: svo of another context");
     MRH_TRY(guide_validate(ctx, guide, t->total, t->p.max_depth));
+    if (t->p.nee)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "mrh_tracer_set_guide: next-event estimation with guiding is not implemented");
     guide_free(t->g);
     t->g = GuideState{};
     t->guided = false;

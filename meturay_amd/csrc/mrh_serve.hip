@@ -110,7 +110,7 @@ __global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__
         const SlotDev& sd = slots[in_slot[i]];
         const V3 o = v3(in.ox[i], in.oy[i], in.oz[i]);
         const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
-        const bool cam_ray = (in.dm[i] & 0xFFu) == 1u;
+        const bool cam_ray = (in.dm[i] & (0xFFu | DM_NEE)) == 1u;
         const float tmin = cam_ray ? sd.cam.near_d : 0.0f;
         const float tmax = cam_ray ? sd.cam.far_d : FLT_MAX;
         const float4* leaves = sd.q.leaves;
@@ -151,19 +151,20 @@ __global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__
     const uint32_t n = hdr->n_total;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t j = base + threadIdx.x;
-        bool alive = false;
-        RayNext nx;
+        bool alive = false, has_nee = false;
+        RayNext nx, ne;
         uint32_t sample = 0, slot = 0;
         if (j < n) {
             const uint32_t rid = order[j];
             slot = in_slot[rid];
             const SlotDev& sd = slots[slot];
             sample = in.sample[rid];
-            alive = shade_ray<G, G>(in, hit, rid, sd.q, sd.ga, G && sd.learn != 0u, G && sd.guide != 0u, nx);
+            alive = shade_ray<G, G, false>(in, hit, rid, sd.q, sd.ga, G && sd.learn != 0u, G && sd.guide != 0u, nx,
+                                           ne, has_nee);
             atomicAdd(&s_seg[slot], 1u);
             if (alive) atomicAdd(&s_live[slot], 1u);
         }
-        append_ray(alive, nx, sample, out, &res[0], out_slot, slot);
+        append_ray(alive, nx, sample, out, &res[0], out_slot, slot, false);
     }
     __syncthreads();
     if (threadIdx.x < SERVE_MAX_SLOTS) {
@@ -797,6 +798,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
         q.sample = (uint32_t*)take(n * 4);
         q.dm = (uint32_t*)take(n * 4);
         q.rng = (uint64_t*)take(n * 8);
+        q.pdf = nullptr;
         s->slot_of[b] = (uint32_t*)take(n * 4);
     }
     s->hit.leaf = (uint32_t*)take(n * 4);

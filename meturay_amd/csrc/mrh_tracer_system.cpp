@@ -47,16 +47,19 @@ private:
 // (TracerCUDA/WFPGTracer.cu) over mrh_tracer
 class PathTracerMRH final : public GPUTracerI {
 public:
+    // "NextEventEstimation" defaults to on, as in PathTracer.h:26-33; the
+    // guided tracer defaults it to off (NEE is not combined with guiding).
     PathTracerMRH(mrh_context* ctx, SceneMRH* scene, const TracerParameters& p, bool guided)
-        : ctx(ctx), scene(scene), params(p), guided(guided) {}
+        : ctx(ctx), scene(scene), params(p), nee(!guided), guided(guided) {}
     ~PathTracerMRH() override { Release(); }
 
     void AttachTracerCallbacks(TracerCallbacksI& cb) override { callbacks = &cb; }
 
     TracerError Initialize() override
     {
-        if (nee) return Report(TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER,
-                                           "NextEventEstimation is not implemented"));
+        if (nee && guided)
+            return Report(TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER,
+                                      "NextEventEstimation with guiding is not implemented"));
         Release();
         mrh_render_params rp = {};
         rp.width = width;
@@ -64,7 +67,8 @@ public:
         rp.samples = samples;
         rp.max_depth = maxDepth;
         rp.rr_start = rrStart;
-        rp.nee = 0;
+        rp.nee = nee ? 1u : 0u;
+        rp.direct_light_mis = directLightMIS ? 1u : 0u;
         rp.seed = params.seed;
         int32_t rc = mrh_tracer_create(ctx, scene->Handle(), &rp, &tracer);
         if (rc != MRH_OK) return Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
@@ -133,6 +137,20 @@ public:
             TracerError e = o.GetBool(b, "NextEventEstimation");
             if (e != TracerError::OK) return Report(e);
             nee = b;
+        }
+        if (o.Has("DirectLightMIS")) {
+            TracerError e = o.GetBool(b, "DirectLightMIS");
+            if (e != TracerError::OK) return Report(e);
+            directLightMIS = b;
+        }
+        if (o.Has("NEESampler")) {
+            // LightSamplerCommon::StringToLightSamplerType (Tracers.cpp:10-24):
+            // "Uniform" is the one light sampler
+            std::string name;
+            TracerError e = o.GetString(name, "NEESampler");
+            if (e != TracerError::OK) return Report(e);
+            if (name != "Uniform")
+                return Report(TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "unknown NEESampler " + name));
         }
         options = o;
         return TracerError::OK;
@@ -244,8 +262,9 @@ private:
     mrh_tracer* tracer = nullptr;
     float* dImage = nullptr;
     uint32_t width = 256, height = 256, samples = 1, maxDepth = 10, rrStart = 3;
-    bool nee = false;
+    bool nee = true;
     bool guided = false;
+    bool directLightMIS = false;
     uint32_t octreeLevel = 10, binRayCount = 512, learnIterations = 4;
     double misRatio = 0.5;
     std::chrono::steady_clock::time_point frameStart;

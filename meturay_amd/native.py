@@ -10,6 +10,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import re
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -51,7 +52,8 @@ class CameraDesc(C.Structure):
 
 class RenderParams(C.Structure):
     _fields_ = [("width", _U32), ("height", _U32), ("samples", _U32), ("max_depth", _U32),
-                ("rr_start", _U32), ("nee", _U32), ("seed", _U32), ("count_traversal", _U32)]
+                ("rr_start", _U32), ("nee", _U32), ("seed", _U32), ("count_traversal", _U32),
+                ("direct_light_mis", _U32)]
 
 
 class RenderStats(C.Structure):
@@ -220,10 +222,15 @@ class Context:
 
     def __init__(self, device: int = 0):
         self.h = _P()
+        self._children = weakref.WeakSet()
         rc = lib().mrh_context_create(device, C.byref(self.h))
         if rc != OK:
             raise MrhError(rc, f"mrh_context_create(device={device}) failed")
         self.device = device
+
+    def adopt(self, obj) -> None:
+        """Objects made on this context are closed before it is destroyed."""
+        self._children.add(obj)
 
     def check(self, rc: int) -> None:
         if rc != OK:
@@ -231,6 +238,8 @@ class Context:
 
     def close(self) -> None:
         if self.h:
+            for child in list(self._children):
+                child.close()
             lib().mrh_context_destroy(self.h)
             self.h = _P()
 
@@ -272,6 +281,7 @@ class Lbvh:
 
     def __init__(self, ctx: Context, d_tris, n: int, bits: int = 30, handle=None):
         self.ctx = ctx
+        ctx.adopt(self)
         self.owned = handle is None
         if handle is None:
             self.h = _P()
@@ -288,7 +298,7 @@ class Lbvh:
         self.ctx.check(lib().mrh_lbvh_rebuild(self.ctx.h, self.h, ptr(d_tris)))
 
     def close(self) -> None:
-        if self.owned and self.h:
+        if getattr(self, "owned", False) and getattr(self, "h", None):
             lib().mrh_lbvh_destroy(self.h)
             self.h = _P()
 
@@ -382,6 +392,7 @@ class Scene:
         import numpy as np
 
         self.ctx = ctx
+        ctx.adopt(self)
         self._keep = {k: np.ascontiguousarray(v) for k, v in scene.items()
                       if isinstance(v, np.ndarray)}
         d = SceneDesc()
@@ -404,10 +415,11 @@ class Scene:
         return Lbvh(self.ctx, None, 0, handle=h)
 
     def render(self, d_image, width, height, samples, max_depth, rr_start, seed=0, camera=None,
-               nee=0, count_traversal=False) -> RenderStats:
+               nee=0, count_traversal=False, mis=0) -> RenderStats:
+        """One frame (mrh_render); nee / mis = "NextEventEstimation" / "DirectLightMIS"."""
         cam = camera_desc(camera or self.camera)
         p = RenderParams(width, height, samples, max_depth, rr_start, nee, seed,
-                         1 if count_traversal else 0)
+                         1 if count_traversal else 0, mis)
         st = RenderStats()
         self.ctx.check(lib().mrh_render(self.ctx.h, self.h, C.byref(cam), C.byref(p),
                                         ptr(d_image), C.byref(st)))
@@ -436,7 +448,7 @@ class Scene:
         return res
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):
             lib().mrh_scene_destroy(self.h)
             self.h = _P()
 
@@ -452,6 +464,7 @@ class Svo:
 
     def __init__(self, ctx: Context):
         self.ctx = ctx
+        ctx.adopt(self)
         self.h = _P()
         ctx.check(lib().mrh_svo_create(ctx.h, C.byref(self.h)))
 
@@ -486,7 +499,7 @@ class Svo:
         return v
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):
             lib().mrh_svo_destroy(self.h)
             self.h = _P()
 
@@ -502,6 +515,7 @@ class _BorrowedContext(Context):
 
     def __init__(self, handle):
         self.h = handle
+        self._children = weakref.WeakSet()
         self.device = None
 
     def close(self) -> None:
@@ -514,6 +528,7 @@ class Server:
     def __init__(self, ctx: Context, hbm_budget: int = 200 << 30, wave_paths: int = 1 << 24,
                  max_slots: int = 64, use_graphs: bool = True):
         self.ctx = ctx
+        ctx.adopt(self)
         self.h = _P()
         p = ServerParams(hbm_budget, wave_paths, max_slots, 1 if use_graphs else 0, 0)
         ctx.check(lib().mrh_server_create(ctx.h, C.byref(p), C.byref(self.h)))
@@ -561,7 +576,7 @@ class Server:
                 return done + self.poll()
 
     def close(self) -> None:
-        if self.h:
+        if getattr(self, "h", None):
             lib().mrh_server_destroy(self.h)
             self.h = _P()
 

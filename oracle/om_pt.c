@@ -464,6 +464,104 @@ static om_v3 metal_eval(const om_surface* s, om_v3 wi, const float* m, om_v3 dir
     return (ndv == 0.0f) ? v3(0.0f, 0.0f, 0.0f) : spec;
 }
 
+/* ------------------------- next-event estimation ---------------------- */
+
+/* closest hit of one ray over the scene's traversal tree */
+static uint32_t closest_hit(const om_scene* sc, om_v3 o, om_v3 d, float tmin, float* tmax, float b[2])
+{
+    float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    return sc->nodes4 ? om_trace4_one(sc->tris, sc->n, sc->sorted_idx, sc->nodes4, sc->root4, oo, dd, tmin, tmax, b,
+                                      0, NULL, NULL)
+                      : om_trace_one(sc->tris, sc->n, sc->sorted_idx, sc->left, sc->right, sc->node_aabb, sc->root,
+                                     oo, dd, tmin, tmax, b, 0, NULL, NULL);
+}
+
+/* TracerFunctions.cuh:302-309: power heuristic, exponent 2, one sample each */
+static float power_heuristic(float pdf0, float pdf1)
+{
+    float w0 = 1.0f * pdf0;
+    float w1 = 1.0f * pdf1;
+    return (w0 * w0) / (w0 * w0 + w1 * w1);
+}
+
+/* TriFunctions::Area (GPUPrimitiveTriangle.h:321-340) */
+static float tri_area(om_v3 p0, om_v3 p1, om_v3 p2)
+{
+    om_v3 c = v3_cross(v3_sub(p1, p0), v3_sub(p2, p0));
+    return sqrtf(v3_dot(c, c)) * 0.5f;
+}
+
+/* GPULight<PGroup>::Pdf, hit form (GPULightPrimitive.cuh:245-256):
+ * (1 / area) * t^2 / |n . -d| for a unit direction d reaching the light at t */
+static float light_pdf_hit(const float* tri, float t, om_v3 d)
+{
+    om_v3 p0 = v3(tri[0], tri[1], tri[2]), p1 = v3(tri[3], tri[4], tri[5]), p2 = v3(tri[6], tri[7], tri[8]);
+    float pdf = 1.0f / tri_area(p0, p1, p2);
+    om_v3 n = v3_normalize(v3_cross(v3_sub(p1, p0), v3_sub(p2, p0)));
+    float ndl = fabsf(v3_dot(n, v3_neg(d)));
+    return pdf * t * t / ndl;
+}
+
+/* GPULightSamplerUniform::SampleLight (GPULightSamplerUniform.cuh:82-130):
+ * uniform light index, then GPULight<PGroup>::Sample
+ * (GPULightPrimitive.cuh:139-166) with the triangle's uniform point
+ * (GPUPrimitiveTriangle.h:65-110, Osada 2002: a = 1 - sqrt(x0),
+ * b = (1 - x1) sqrt(x0), c = x1 sqrt(x0)).  Returns the light's primitive;
+ * *pdf = solid-angle pdf x 1 / light count. */
+static uint32_t sample_light_xi(const om_scene* sc, om_v3 p, float r1, float xi0, float xi1, om_v3* dir,
+                                float* dist2, float* pdf)
+{
+    r1 *= (float)sc->n_lights;
+    uint32_t idx = (uint32_t)r1;
+    if (idx == sc->n_lights) idx--;
+    uint32_t prim = sc->lights[idx];
+    const float* t = sc->tris + (size_t)prim * 9;
+    om_v3 p0 = v3(t[0], t[1], t[2]), p1 = v3(t[3], t[4], t[5]), p2 = v3(t[6], t[7], t[8]);
+    float s1 = sqrtf(xi0);
+    float s2 = xi1;
+    float a = 1.0f - s1;
+    float b = (1.0f - s2) * s1;
+    float c = s1 * s2;
+    float pd = 1.0f / tri_area(p0, p1, p2);
+    om_v3 n = v3_normalize(v3_cross(v3_sub(p1, p0), v3_sub(p2, p0)));
+    om_v3 lp = v3_add(v3_add(v3_scale(p0, a), v3_scale(p1, b)), v3_scale(p2, c));
+    om_v3 dv = v3_sub(lp, p);
+    float d2 = v3_dot(dv, dv);
+    float dist = sqrtf(d2);
+    dv = v3_scale(dv, 1.0f / dist);
+    float ndl = fabsf(v3_dot(n, v3_neg(dv)));
+    pd *= d2 / ndl;
+    pd *= 1.0f / (float)sc->n_lights;
+    *dir = dv;
+    *dist2 = d2;
+    *pdf = pd;
+    return prim;
+}
+
+static uint32_t sample_light(const om_scene* sc, om_v3 p, om_rng* rng, om_v3* dir, float* dist2, float* pdf)
+{
+    float r1 = om_uniform(rng);
+    float xi0 = om_uniform(rng);
+    float xi1 = om_uniform(rng);
+    return sample_light_xi(sc, p, r1, xi0, xi1, dir, dist2, pdf);
+}
+
+uint32_t om_sample_light(const om_scene* sc, const float p[3], const float xi[3], float dir[3], float* dist2,
+                         float* pdf)
+{
+    om_v3 d;
+    uint32_t prim = sample_light_xi(sc, v3(p[0], p[1], p[2]), xi[0], xi[1], xi[2], &d, dist2, pdf);
+    dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+    return prim;
+}
+
+float om_light_pdf_hit(const float tri[9], float t, const float d[3])
+{
+    return light_pdf_hit(tri, t, v3(d[0], d[1], d[2]));
+}
+
+float om_power_heuristic(float pdf0, float pdf1) { return power_heuristic(pdf0, pdf1); }
+
 /* Guide state of one frame of the guided tracer */
 typedef struct {
     const om_svo* svo;       /* guide to sample, NULL = plain path tracing */
@@ -525,6 +623,9 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
     uint32_t depth = 1, medium = 0;
     float tmin = cam->near_far[0], tmax = cam->near_far[1];
     const int learn = g && g->learn;
+    const int nee = p->nee != 0, mis = p->nee && p->mis;
+    int spec_ray = 0;          /* the ray left a specular vertex (SPECULAR_PATH_RAY) */
+    float prev_pdf = 0.0f;     /* BxDF pdf of the ray's direction (prevPDF) */
     uint32_t rc[64], rb[64];
     om_v3 rt[64];
     float rr[64];
@@ -543,11 +644,21 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         uint32_t key = sc->prim_key[prim];
         uint32_t batch = key >> 24, id = key & 0xFFFFFFu;
         if (batch == OM_BATCH_EMISSIVE) {
-            /* PathTracerKC.cuh:337-426: Le * throughput, MIS weight 1 */
+            /* PathTracerKC.cuh:337-426: Le * throughput.  With NEE, a path
+             * ray that left a non-specular vertex counts only with MIS, and
+             * then with the BxDF strategy's power-heuristic weight; camera
+             * and specular rays count in full. */
             const float* le = sc->emissive + (size_t)id * 3;
-            pixel[0] += (double)(le[0] * T.x);
-            pixel[1] += (double)(le[1] * T.y);
-            pixel[2] += (double)(le[2] * T.z);
+            om_v3 c = v3(le[0] * T.x, le[1] * T.y, le[2] * T.z);
+            if (nee && depth > 1 && !spec_ray) {
+                if (!mis) break;
+                float pc = light_pdf_hit(sc->tris + (size_t)prim * 9, tmax, v3_normalize(d));
+                float pm = 1.0f / (float)sc->n_lights;
+                c = v3_scale(c, power_heuristic(prev_pdf, pc * pm));
+            }
+            pixel[0] += (double)c.x;
+            pixel[1] += (double)c.y;
+            pixel[2] += (double)c.z;
             if (learn) {
                 om_v3 c = v3_mul(v3(le[0], le[1], le[2]), T);
                 for (uint32_t k = 0; k < nrec; k++) {
@@ -563,6 +674,40 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         if (bounce + 1 == p->max_depth) break;   /* the last segment only collects emission */
         om_surface s = make_surface(sc->tris + (size_t)prim * 9, b[0], b[1], d);
         om_v3 wi = v3_neg(v3_normalize(d));
+        if (nee && sc->n_lights &&
+            (batch == OM_BATCH_LAMBERT || (batch == OM_BATCH_METAL && (1.0f - sc->metal[(size_t)id * 10]) < 0.99f))) {
+            /* PathTracerKC.cuh:493-575: one light sample per non-specular
+             * vertex, traced as a closest-hit ray; it counts only where it
+             * hits the light it was sent to (isCorrectNEERay) */
+            om_v3 ldir;
+            float dist2, pl, pb;
+            uint32_t lprim = sample_light(sc, s.pos, &rng, &ldir, &dist2, &pl);
+            om_v3 f;
+            if (batch == OM_BATCH_LAMBERT) {
+                const float* a = sc->lambert + (size_t)id * 3;
+                f = lambert_eval(&s, v3(a[0], a[1], a[2]), ldir, &pb);
+            } else {
+                f = metal_eval(&s, wi, sc->metal + (size_t)id * 10, ldir, &pb);
+            }
+            float pn = pl;
+            if (mis) {
+                pn /= power_heuristic(pl, pb);
+                if (isnan(pn)) pn = 0.0f;
+            }
+            om_v3 nf = v3_mul(T, f);
+            nf = (pn == 0.0f) ? v3(0.0f, 0.0f, 0.0f) : v3_divs(nf, pn);
+            if (dist2 > 0.0f && !v3_is_zero(nf)) {
+                (*segments)++;
+                float lt = FLT_MAX, lb[2];
+                uint32_t hp = closest_hit(sc, nudge(s.pos, s.geo_n), ldir, 0.0f, &lt, lb);
+                if (hp == lprim) {
+                    const float* le = sc->emissive + (size_t)(sc->prim_key[hp] & 0xFFFFFFu) * 3;
+                    pixel[0] += (double)(le[0] * nf.x);
+                    pixel[1] += (double)(le[1] * nf.y);
+                    pixel[2] += (double)(le[2] * nf.z);
+                }
+            }
+        }
         om_bsdf_out out;
         out.medium = medium;
         int guided = 0;
@@ -642,6 +787,8 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         o = out.o;
         d = out.d;
         medium = out.medium;
+        spec_ray = out.specular;
+        prev_pdf = out.pdf;
         depth++;
         tmin = 0.0f;
         tmax = FLT_MAX;
@@ -718,6 +865,8 @@ uint64_t om_render_guided(const om_scene* s, const om_camera* cam, const om_rend
     om_render_params q = *p;
     q.row_step = 1;
     q.row_offset = 0;
+    q.nee = 0;   /* guided frames trace no next-event rays */
+    q.mis = 0;
     /* scene bounds: min / max over all triangle vertices */
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (uint32_t i = 0; i < s->n; i++)

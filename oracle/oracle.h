@@ -136,6 +136,10 @@ typedef struct {
     /* 4-wide traversal tree (om_bvh4_build); NULL = walk the binary LBVH */
     const float*    nodes4;
     uint32_t        root4;
+    /* light list: primitive index of every emissive triangle, in primitive
+     * order (one primitive-backed light per triangle) */
+    const uint32_t* lights;
+    uint32_t        n_lights;
 } om_scene;
 
 typedef struct {
@@ -161,7 +165,20 @@ typedef struct {
     uint32_t row_step;       /* render rows y with y % row_step == row_offset */
     uint32_t row_offset;
     uint32_t threads;
+    uint32_t nee;            /* next-event estimation ("NextEventEstimation") */
+    uint32_t mis;            /* power-heuristic MIS of NEE and BxDF ("DirectLightMIS"), with nee only */
 } om_render_params;
+
+/* Next-event estimation pieces (GPULightSamplerUniform.cuh:82-130,
+ * GPULightPrimitive.cuh:139-166 / 245-256, TracerFunctions.cuh:302-309):
+ * light sample from xi = (light choice, two barycentric numbers), returning
+ * the light's primitive, unit direction, squared distance and pdf (solid
+ * angle / light count); the solid-angle pdf of hitting triangle tri at
+ * distance t along unit d; the power-2 heuristic weight of pdf0. */
+uint32_t om_sample_light(const om_scene* sc, const float p[3], const float xi[3], float dir[3], float* dist2,
+                         float* pdf);
+float    om_light_pdf_hit(const float tri[9], float t, const float d[3]);
+float    om_power_heuristic(float pdf0, float pdf1);
 
 /* Renders the selected rows; img is double[h*w*3] (zeroed by the caller),
  * accumulating the per-sample FP32 contributions in FP64.  Returns the number

@@ -49,13 +49,13 @@ class OmScene(C.Structure):
     _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
                 ("lambert", _P), ("metal", _P), ("dielectric", _P), ("emissive", _P),
                 ("sorted_idx", _P), ("left", _P), ("right", _P), ("node_aabb", _P),
-                ("root", _U32), ("nodes4", _P), ("root4", _U32)]
+                ("root", _U32), ("nodes4", _P), ("root4", _U32), ("lights", _P), ("n_lights", _U32)]
 
 
 class OmRenderParams(C.Structure):
     _fields_ = [("width", _U32), ("height", _U32), ("samples", _U32), ("max_depth", _U32),
                 ("rr_start", _U32), ("seed", _U32), ("row_step", _U32), ("row_offset", _U32),
-                ("threads", _U32)]
+                ("threads", _U32), ("nee", _U32), ("mis", _U32)]
 
 
 class OmSvo(C.Structure):
@@ -154,6 +154,12 @@ def lib():
         L.om_atan_unit.argtypes = [_F]
         L.om_guide_sample.argtypes = [C.POINTER(OmSvo), _P, _P, _P, _U32, _P, _P]
         L.om_guide_pdf.argtypes = [C.POINTER(OmSvo), _P, _P, _P, _U32, _P]
+        L.om_sample_light.restype = _U32
+        L.om_sample_light.argtypes = [C.POINTER(OmScene), _P, _P, _P, C.POINTER(_F), C.POINTER(_F)]
+        L.om_light_pdf_hit.restype = _F
+        L.om_light_pdf_hit.argtypes = [_P, _F, _P]
+        L.om_power_heuristic.restype = _F
+        L.om_power_heuristic.argtypes = [_F, _F]
         L.om_render_guided.restype = C.c_uint64
         L.om_render_guided.argtypes = [C.POINTER(OmScene), C.POINTER(OmCamera),
                                        C.POINTER(OmRenderParams), C.POINTER(OmGuideParams), _P,
@@ -355,21 +361,52 @@ def _scene_struct(scene: dict, bvh: LBVH, keep: dict, width: int = 4) -> OmScene
     if width == 4:
         sc.nodes4 = _ptr(bvh.nodes4())
         sc.root4 = bvh.root4
+    keys = np.ascontiguousarray(scene["prim_key"], dtype=np.uint32)
+    lights = np.nonzero((keys >> 24) == 4)[0].astype(np.uint32)
+    keep["lights"] = lights if len(lights) else np.zeros(1, np.uint32)
+    sc.lights = _ptr(keep["lights"])
+    sc.n_lights = len(lights)
     return sc
 
 
 def render(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_depth, rr_start,
-           seed, row_step=1, row_offset=0, threads=1, tree_width=4):
+           seed, row_step=1, row_offset=0, threads=1, tree_width=4, nee=0, mis=0):
     """scene: dict with tris (n,9), prim_key (n,), lambert (.,3), metal (.,10),
-    dielectric (.,4), emissive (.,3); tree_width = traversal tree (2 or 4).
+    dielectric (.,4), emissive (.,3); tree_width = traversal tree (2 or 4);
+    nee / mis = next-event estimation / its MIS with the BxDF.
     Returns (image float64 [h,w,3], segments)."""
     keep = {}
     sc = _scene_struct(scene, bvh, keep, tree_width)
     p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, row_step, row_offset,
-                       threads)
+                       threads, nee, mis)
     img = np.zeros((height, width, 3), dtype=np.float64)
     seg = lib().om_render(C.byref(sc), C.byref(cam), C.byref(p), _ptr(img))
     return img / float(samples * samples), int(seg)
+
+
+def sample_light(scene: dict, bvh: "LBVH", p, xi):
+    """Light sample of the next-event estimator from point p with numbers xi
+    (light choice, two barycentric numbers): (light primitive, unit direction,
+    squared distance, pdf = solid-angle pdf / light count)."""
+    keep = {}
+    sc = _scene_struct(scene, bvh, keep, 2)
+    pp = np.ascontiguousarray(p, np.float32)
+    xx = np.ascontiguousarray(xi, np.float32)
+    d = np.zeros(3, np.float32)
+    d2, pdf = _F(0), _F(0)
+    prim = lib().om_sample_light(C.byref(sc), _ptr(pp), _ptr(xx), _ptr(d), C.byref(d2), C.byref(pdf))
+    return int(prim), d, float(d2.value), float(pdf.value)
+
+
+def light_pdf_hit(tri, t, d) -> float:
+    """Solid-angle pdf of reaching triangle tri (9 floats) at distance t along unit d."""
+    tt = np.ascontiguousarray(tri, np.float32).reshape(9)
+    dd = np.ascontiguousarray(d, np.float32)
+    return float(lib().om_light_pdf_hit(_ptr(tt), float(t), _ptr(dd)))
+
+
+def power_heuristic(pdf0, pdf1) -> float:
+    return float(lib().om_power_heuristic(float(pdf0), float(pdf1)))
 
 
 class Svo:
@@ -500,7 +537,7 @@ def render_guided(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples,
     all frames, segments, Svo of the last guide or None)."""
     keep = {}
     sc = _scene_struct(scene, bvh, keep, tree_width)
-    p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, 1, 0, threads)
+    p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, 1, 0, threads, 0, 0)
     g = OmGuideParams(iterations, svo_depth, guide_ratio, min_samples)
     img = np.zeros((height, width, 3), dtype=np.float64)
     last = OmSvo()

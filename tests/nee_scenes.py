@@ -1,0 +1,77 @@
+"""Small scenes with known answers for next-event estimation tests.
+
+Scenes are dicts in the layout mrh_scene_desc takes (tris, prim_key,
+material tables, camera), shared by the oracle (CPU) and GPU tests."""
+import math
+
+import numpy as np
+
+from meturay_amd import native as N
+
+
+def quad(cx, cy, z, hx, hy):
+    """Two triangles of the rectangle [cx-hx, cx+hx] x [cy-hy, cy+hy] at height z."""
+    p = [(cx - hx, cy - hy, z), (cx + hx, cy - hy, z), (cx + hx, cy + hy, z), (cx - hx, cy + hy, z)]
+    return [sum((p[0], p[1], p[2]), ()), sum((p[0], p[2], p[3]), ())]
+
+
+def square_lights_scene(lights, rho=0.8, floor_half=3.0, mirror=False):
+    """Lambert floor (albedo rho) at z = 0 under square lights given as
+    (height, half size, radiance); a narrow camera halfway up to the lowest
+    light looks straight down at the floor's centre.  With mirror the floor
+    is a smooth conductor (roughness 0) instead."""
+    tris = quad(0.0, 0.0, 0.0, floor_half, floor_half)
+    keys = [((2 if mirror else 1) << 24) | 0] * 2
+    em = []
+    for k, (z, half, le) in enumerate(lights):
+        tris += quad(0.0, 0.0, z, half, half)
+        keys += [(4 << 24) | k] * 2
+        em.append([le, le, le])
+    zc = 0.5 * min(z for z, _, _ in lights)
+    return {
+        "tris": np.array(tris, np.float32),
+        "prim_key": np.array(keys, np.uint32),
+        "lambert": np.array([[rho, rho, rho]], np.float32),
+        "metal": np.array([[0.0, 0.2, 0.4, 1.4, 3.5, 2.4, 1.8, 1.0, 1.0, 1.0]], np.float32),
+        "dielectric": np.zeros((0, 4), np.float32),
+        "emissive": np.array(em, np.float32),
+        "camera": {"position": [0.0, 0.0, zc], "gaze": [0.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0],
+                   "fov_x": 0.02, "near": 1e-3, "far": 100.0},
+    }
+
+
+def view_factor_square(half, height):
+    """Form factor from a differential area to a parallel, centred square of
+    half size `half` at distance `height`: four corner rectangles of the
+    textbook differential-to-parallel-rectangle factor."""
+    a = half / height
+    s = math.sqrt(1.0 + a * a)
+    corner = (a / s) * math.atan(a / s) * 2.0 / (2.0 * math.pi)
+    return 4.0 * corner
+
+
+def furnace(light_radiance=1.0):
+    """Camera inside a closed emissive box looking at a Lambert icosphere
+    (albedo 0.5); every wall has radiance 1 except the light patch."""
+    sc = N.gen_cornell(3)
+    keys = sc["prim_key"].copy()
+    keys[:12] = (4 << 24) | 0
+    keys[12:14] = (4 << 24) | 1
+    keys[14:] = (1 << 24) | 0
+    sc["prim_key"] = keys
+    sc["emissive"] = np.array([[1, 1, 1], [light_radiance] * 3], np.float32)
+    sc["lambert"] = np.array([[0.5, 0.5, 0.5]], np.float32)
+    return sc
+
+
+def lit_spheres():
+    """4x4 sphere grid (Lambert, GGX conductors of several roughnesses,
+    dielectrics) with spheres 0 and 5 turned into lights."""
+    sc = N.gen_sphere_grid(4, 8, 8, 5)
+    keys = sc["prim_key"].copy()
+    per = len(keys) // 16
+    keys[0:per] = (4 << 24) | 0
+    keys[5 * per:6 * per] = (4 << 24) | 1
+    sc["prim_key"] = keys
+    sc["emissive"] = np.array([[4.0, 4.0, 4.0], [1.0, 2.0, 3.0]], np.float32)
+    return sc

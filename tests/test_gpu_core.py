@@ -352,7 +352,7 @@ def test_render_errors(ctx):
     scene = N.Scene(ctx, sc)
     img = torch.zeros(16 * 16 * 3, dtype=torch.float32, device="cuda")
     with pytest.raises(N.MrhError):
-        scene.render(img.data_ptr(), 16, 16, 1, 8, 3, 0, nee=1)
+        scene.render(img.data_ptr(), 16, 16, 1, 8, 3, 0, nee=2)
     with pytest.raises(N.MrhError):
         scene.render(img.data_ptr(), 16, 16, 1, 0, 3, 0)
     bad = dict(sc)

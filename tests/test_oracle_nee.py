@@ -1,0 +1,199 @@
+"""Next-event estimation in the oracle, pinned by closed forms and brute force.
+
+- the light sampler's pdf equals the hit-form light pdf of the same point;
+- the hit-form pdf integrates to one over the triangle's solid angle
+  (uniform sphere directions, numpy ray-triangle tests);
+- sampled light points are uniform over the triangle, lights are chosen
+  uniformly;
+- a Lambert floor under a square light renders rho * Le * F (F the
+  textbook differential-to-parallel-square form factor) with NEE off, NEE on
+  and NEE + MIS; NEE lowers the variance;
+- a light hidden behind another light adds nothing (a NEE ray counts only
+  where it hits the light it was sent to);
+- the emissive furnace renders 0.5 on its sphere in every mode;
+- specular vertices send no NEE ray and their light hits count in full:
+  over a mirror every mode renders the same image, bit for bit;
+- NEE, NEE + MIS and plain path tracing agree on the Cornell box and on the
+  lit sphere grid (conductors, dielectrics: specular vertices send no NEE
+  ray and their light hits count in full).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from meturay_amd import native as N
+import nee_scenes as S
+
+
+def render_modes(oracle, sc, w, h, spp_axis, depth, rr, seed, modes):
+    bvh = oracle.LBVH(sc["tris"], 30)
+    c = sc["camera"]
+    cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], w, h)
+    out = {}
+    for nee, mis in modes:
+        img, seg = oracle.render(sc, bvh, cam, w, h, spp_axis, depth, rr, seed, threads=8, nee=nee, mis=mis)
+        out[(nee, mis)] = (img, seg)
+    return out
+
+
+def mean_sigma(img):
+    v = img.mean(axis=2).ravel()
+    return float(v.mean()), float(v.std(ddof=1) / math.sqrt(v.size))
+
+
+MODES = [(0, 0), (1, 0), (1, 1)]
+
+
+# ------------------------------------------------------------ light sampler
+def test_light_sample_pdf_matches_hit_pdf(oracle):
+    sc = N.gen_cornell(2)
+    bvh = oracle.LBVH(sc["tris"], 30)
+    n_lights = int(np.sum((sc["prim_key"] >> 24) == 4))
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        p = rng.uniform(-0.9, 0.9, 3).astype(np.float32)
+        xi = rng.random(3).astype(np.float32)
+        prim, d, d2, pdf = oracle.sample_light(sc, bvh, p, xi)
+        assert (sc["prim_key"][prim] >> 24) == 4
+        assert abs(np.linalg.norm(d) - 1.0) < 1e-6
+        ph = oracle.light_pdf_hit(sc["tris"][prim], math.sqrt(d2), d) / n_lights
+        assert pdf == pytest.approx(ph, rel=2e-5)
+        # the sampled point lies on the light's triangle
+        q = p.astype(np.float64) + d.astype(np.float64) * math.sqrt(d2)
+        t = sc["tris"][prim].reshape(3, 3).astype(np.float64)
+        n = np.cross(t[1] - t[0], t[2] - t[0])
+        assert abs(np.dot(q - t[0], n / np.linalg.norm(n))) < 1e-5
+
+
+def ray_tri_hits(o, d, tri):
+    """Moller-Trumbore over many directions (float64): (hit mask, t)."""
+    t = tri.reshape(3, 3).astype(np.float64)
+    e1, e2 = t[1] - t[0], t[2] - t[0]
+    pv = np.cross(d, e2)
+    det = pv @ e1
+    inv = 1.0 / det
+    s = o - t[0]
+    u = (pv @ s) * inv
+    qv = np.cross(s, e1)
+    v = (d @ qv) * inv
+    tt = (qv @ e2) * inv
+    hit = (u >= 0) & (v >= 0) & (u + v <= 1) & (tt > 0)
+    return hit, tt
+
+
+def test_light_pdf_integrates_to_one(oracle):
+    tri = np.array([-0.5, -0.4, 1.0, 0.7, -0.3, 1.2, 0.1, 0.6, 0.9], np.float32)
+    o = np.array([0.05, 0.02, 0.0])
+    rng = np.random.default_rng(5)
+    n = 400000
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    hit, t = ray_tri_hits(o, d, tri)
+    total = 0.0
+    for k in np.nonzero(hit)[0]:
+        total += oracle.light_pdf_hit(tri, float(t[k]), d[k].astype(np.float32))
+    integral = total * 4.0 * math.pi / n
+    assert abs(integral - 1.0) < 0.02, integral
+
+
+def test_light_points_uniform_and_choice_uniform(oracle):
+    sc = S.square_lights_scene([(1.0, 0.5, 1.0), (2.0, 0.5, 1.0)])
+    bvh = oracle.LBVH(sc["tris"], 30)
+    rng = np.random.default_rng(7)
+    p = np.array([0.1, 0.2, 0.0], np.float32)
+    counts = np.zeros(4, int)
+    sub = np.zeros(4, int)
+    n = 20000
+    for _ in range(n):
+        xi = rng.random(3).astype(np.float32)
+        prim, d, d2, _ = oracle.sample_light(sc, bvh, p, xi)
+        counts[prim - 2] += 1
+        t = sc["tris"][prim].reshape(3, 3).astype(np.float64)
+        q = p + d.astype(np.float64) * math.sqrt(d2)
+        # barycentrics of q: the four midpoint sub-triangles each hold 1/4
+        m = np.array([t[1] - t[0], t[2] - t[0]]).T
+        b1, b2 = np.linalg.lstsq(m, q - t[0], rcond=None)[0]
+        b0 = 1 - b1 - b2
+        assert min(b0, b1, b2) > -1e-4
+        sub[0 if b0 > 0.5 else 1 if b1 > 0.5 else 2 if b2 > 0.5 else 3] += 1
+    expect = n / 4
+    assert np.all(np.abs(counts - expect) < 5 * math.sqrt(expect * 0.75)), counts
+    assert np.all(np.abs(sub - expect) < 5 * math.sqrt(expect * 0.75)), sub
+    # the top of [0, 1) picks the last light
+    prim, _, _, _ = oracle.sample_light(sc, bvh, p, np.array([0.99999994, 0.5, 0.5], np.float32))
+    assert prim == 5
+
+
+def test_power_heuristic(oracle):
+    assert oracle.power_heuristic(1.0, 1.0) == 0.5
+    assert oracle.power_heuristic(3.0, 1.0) == pytest.approx(0.9)
+    assert oracle.power_heuristic(2.0, 0.0) == 1.0
+    assert oracle.power_heuristic(0.0, 2.0) == 0.0
+
+
+# ------------------------------------------------------------ rendering
+def test_square_light_matches_view_factor(oracle):
+    rho, le, half, height = 0.8, 2.0, 0.5, 1.0
+    sc = S.square_lights_scene([(height, half, le)], rho)
+    expect = rho * le * S.view_factor_square(half, height)
+    res = render_modes(oracle, sc, 32, 32, 4, 2, 10, 1, MODES)
+    sig = {}
+    for mode, (img, seg) in res.items():
+        m, s = mean_sigma(img)
+        sig[mode] = s
+        assert abs(m - expect) < 5 * s + 1e-3 * expect, (mode, m, expect, s)
+    # one light sample per vertex beats hoping the BxDF finds the light
+    assert sig[(1, 0)] < 0.5 * sig[(0, 0)]
+    assert sig[(1, 1)] < 0.5 * sig[(0, 0)]
+    # NEE rays are traced segments too
+    assert res[(1, 0)][1] > res[(0, 0)][1]
+
+
+def test_hidden_light_adds_nothing(oracle):
+    # light B (radiance 5) sits right above light A (radiance 2): from the
+    # floor's centre B is entirely behind A, so only A's view factor counts
+    rho = 0.8
+    sc = S.square_lights_scene([(1.0, 0.5, 2.0), (2.0, 0.5, 5.0)], rho)
+    expect = rho * 2.0 * S.view_factor_square(0.5, 1.0)
+    res = render_modes(oracle, sc, 32, 32, 4, 2, 10, 2, [(1, 0), (1, 1)])
+    for mode, (img, _) in res.items():
+        m, s = mean_sigma(img)
+        assert abs(m - expect) < 5 * s + 1e-3 * expect, (mode, m, expect, s)
+
+
+def test_specular_vertices_send_no_nee(oracle):
+    sc = S.square_lights_scene([(1.0, 0.5, 2.0)], mirror=True)
+    res = render_modes(oracle, sc, 16, 16, 2, 4, 10, 3, MODES)
+    base, seg = res[(0, 0)]
+    assert base.mean() > 0.5
+    for mode, (img, s) in res.items():
+        assert s == seg
+        assert np.array_equal(img, base), mode
+
+
+def test_furnace_with_nee(oracle):
+    sc = S.furnace(1.0)
+    res = render_modes(oracle, sc, 24, 24, 8, 8, 1000, 9, [(1, 0), (1, 1)])
+    for mode, (img, _) in res.items():
+        centre = img[10:14, 10:14]
+        m, s = mean_sigma(centre)
+        assert abs(m - 0.5) < 5 * s + 1e-3, (mode, m, s)
+        assert abs(img[0, 0].mean() - 1.0) < 1e-6          # camera rays see the walls in full
+
+
+@pytest.mark.parametrize("scene", ["cornell", "lit_spheres"])
+def test_modes_agree(oracle, scene):
+    if scene == "cornell":
+        sc, w, h, spp, depth, rr = N.gen_cornell(2), 32, 32, 4, 6, 3
+    else:
+        sc, w, h, spp, depth, rr = S.lit_spheres(), 48, 32, 4, 8, 3
+    res = render_modes(oracle, sc, w, h, spp, depth, rr, 4, MODES)
+    level = res[(0, 0)][0].mean()
+    assert level > 0.01
+    for a in MODES:
+        for b in MODES:
+            if a < b:
+                # per-pixel differences: the image's own structure cancels
+                m, s = mean_sigma(res[a][0] - res[b][0])
+                assert abs(m) < 5 * s + 1e-3 * level, (a, b, m, s, level)

@@ -393,7 +393,8 @@ typedef struct mrh_server mrh_server;
 
 typedef struct {
     uint64_t hbm_budget;     /* device bytes admitted requests may hold (image, learn records, guide) */
-    uint32_t wave_paths;     /* queue capacity: paths traced per step */
+    uint32_t wave_paths;     /* paths in flight per step; each may also have a
+                                next-event ray in flight (queues hold 2x rays) */
     uint32_t max_slots;      /* concurrent requests, 1..64 */
     uint32_t use_graphs;     /* replay each step's launches from a captured hipGraph */
     uint32_t reserved;
@@ -402,7 +403,8 @@ typedef struct {
 typedef struct {
     mrh_scene*        scene;
     mrh_camera_desc   camera;
-    mrh_render_params params;   /* samples = strata per axis of each iteration; nee = 0 */
+    mrh_render_params params;   /* samples = strata per axis of each iteration; nee only
+                                   without guiding */
     mrh_guide_params  guide;    /* iterations <= 1: plain path tracing */
     float*            d_image;  /* width*height*3 floats, written when the request completes */
     uint64_t          user_tag;
@@ -418,7 +420,9 @@ typedef struct {
 
 typedef struct {
     uint64_t steps, segments, budget_used;
-    uint32_t active, queued, live_paths, reserved;
+    uint32_t active, queued;
+    uint32_t live_paths;        /* rays in flight: paths and their next-event rays */
+    uint32_t reserved;
 } mrh_server_info;
 
 int32_t mrh_server_create(mrh_context* ctx, const mrh_server_params* params, mrh_server** out);

@@ -37,7 +37,11 @@ namespace mrh {
 namespace {
 
 constexpr uint32_t SERVE_MAX_SLOTS = 64;
-constexpr uint32_t RES_LIVE = 1, RES_SEG = 1 + SERVE_MAX_SLOTS, RES_WORDS = 1 + 2 * SERVE_MAX_SLOTS;
+// step results: [0] rays appended to the other queue, [RES_LIVE + s] of
+// them slot s's, [RES_SEG + s] rays of slot s traced, [RES_PATHS] path rays
+// among the appended (the rest are next-event rays)
+constexpr uint32_t RES_LIVE = 1, RES_SEG = 1 + SERVE_MAX_SLOTS, RES_PATHS = 1 + 2 * SERVE_MAX_SLOTS,
+                   RES_WORDS = 2 + 2 * SERVE_MAX_SLOTS;
 
 // Device view of one admitted request
 struct SlotDev {
@@ -133,20 +137,21 @@ __global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__
 }
 
 // Shade every queued ray with its request's constants (shade_ray), append
-// the survivors to the other queue; res[0] = survivors, res[RES_LIVE + s] =
-// survivors of slot s, res[RES_SEG + s] = rays of slot s traced this step.
-template <bool G>
+// the survivors (and, with NEE, the next-event rays) to the other queue;
+// counts go to res (RES_* above).
+template <bool G, bool NEE>
 __global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__ hdr,
                                                      const SlotDev* __restrict__ slots, PathBuf in,
                                                      const uint32_t* __restrict__ in_slot, PathBuf out,
                                                      uint32_t* __restrict__ out_slot, HitBuf hit,
                                                      const uint32_t* __restrict__ order, uint32_t* res)
 {
-    __shared__ uint32_t s_live[SERVE_MAX_SLOTS], s_seg[SERVE_MAX_SLOTS];
+    __shared__ uint32_t s_live[SERVE_MAX_SLOTS], s_seg[SERVE_MAX_SLOTS], s_paths;
     if (threadIdx.x < SERVE_MAX_SLOTS) {
         s_live[threadIdx.x] = 0;
         s_seg[threadIdx.x] = 0;
     }
+    if (threadIdx.x == 0) s_paths = 0;
     __syncthreads();
     const uint32_t n = hdr->n_total;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -159,18 +164,23 @@ __global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__
             slot = in_slot[rid];
             const SlotDev& sd = slots[slot];
             sample = in.sample[rid];
-            alive = shade_ray<G, G, false>(in, hit, rid, sd.q, sd.ga, G && sd.learn != 0u, G && sd.guide != 0u, nx,
-                                           ne, has_nee);
+            alive = shade_ray<G, G, NEE>(in, hit, rid, sd.q, sd.ga, G && sd.learn != 0u, G && sd.guide != 0u, nx,
+                                         ne, has_nee);
             atomicAdd(&s_seg[slot], 1u);
-            if (alive) atomicAdd(&s_live[slot], 1u);
+            const uint32_t made = (alive ? 1u : 0u) + (has_nee ? 1u : 0u);
+            if (made) atomicAdd(&s_live[slot], made);
         }
-        append_ray(alive, nx, sample, out, &res[0], out_slot, slot, false);
+        if (NEE) append_ray(has_nee, ne, sample, out, &res[0], out_slot, slot, false);
+        append_ray(alive, nx, sample, out, &res[0], out_slot, slot, NEE);
+        const uint64_t m = __ballot(alive);
+        if ((threadIdx.x & 63) == 0 && m) atomicAdd(&s_paths, (uint32_t)__popcll(m));
     }
     __syncthreads();
     if (threadIdx.x < SERVE_MAX_SLOTS) {
         if (s_live[threadIdx.x]) atomicAdd(&res[RES_LIVE + threadIdx.x], s_live[threadIdx.x]);
         if (s_seg[threadIdx.x]) atomicAdd(&res[RES_SEG + threadIdx.x], s_seg[threadIdx.x]);
     }
+    if (threadIdx.x == 0 && s_paths) atomicAdd(&res[RES_PATHS], s_paths);
 }
 
 __global__ void k_serve_scale(float* __restrict__ p, size_t count, float s)
@@ -196,7 +206,7 @@ struct ServeReq {
     uint32_t iterations = 1, it = 0, S = 1;
     uint64_t total = 0;        // paths per iteration
     uint64_t injected = 0;     // of the current iteration
-    uint64_t live = 0;         // paths in flight
+    uint64_t live = 0;         // rays in flight (paths and their next-event rays)
     uint64_t segments = 0;
     uint64_t charge = 0;       // budget bytes
     int slot = -1;
@@ -218,7 +228,8 @@ struct mrh_server {
     mrh_context* user = nullptr;   // errors are mirrored here
     mrh_context* ctx = nullptr;    // private context: stream, scratch, graphs
     mrh_server_params p = {};
-    uint32_t cap = 0;
+    uint32_t cap = 0;              // paths in flight (wave_paths)
+    uint32_t buf_cap = 0;          // rays a queue holds: a path and its next-event ray
     PathBuf bufs[2] = {};
     uint32_t* slot_of[2] = {};
     HitBuf hit = {};
@@ -232,18 +243,19 @@ struct mrh_server {
     SlotDev* h_slots = nullptr;
     uint32_t* h_res = nullptr;
     uint32_t parity = 0;           // queue holding the live rays
-    uint32_t n_live = 0;
+    uint32_t n_live = 0;           // rays in that queue
+    uint32_t n_paths = 0;          // path rays among them
     uint64_t used = 0;             // budget bytes held by admitted requests
     uint64_t next_ticket = 1;
     uint64_t steps = 0, segments = 0;
     std::deque<ServeReq*> queue;
     ServeReq* slots[SERVE_MAX_SLOTS] = {};
     std::vector<mrh_request_done> done;
-    // graphs per shade variant (plain / learn-or-guide) and queue parity,
-    // with what they were captured against
-    hipGraphExec_t exec[2][2] = {};
-    void* graph_scratch[2][2][SLOT_COUNT] = {};
-    uint32_t graph_cap[2][2] = {}, graph_w[2][2] = {}, graph_lds[2][2] = {}, graph_bpc[2][2] = {};
+    // graphs per shade variant (bit 0: learn-or-guide, bit 1: NEE) and queue
+    // parity, with what they were captured against
+    hipGraphExec_t exec[4][2] = {};
+    void* graph_scratch[4][2][SLOT_COUNT] = {};
+    uint32_t graph_cap[4][2] = {}, graph_w[4][2] = {}, graph_lds[4][2] = {}, graph_bpc[4][2] = {};
     uint32_t max_tree_depth = 0;    // deepest LBVH among submitted requests
     // learner: guide learn passes run on their own context and host thread,
     // overlapping the wavefront steps of other requests
@@ -334,6 +346,10 @@ void fill_slot(mrh_server* s, ServeReq* q)
     d.q.sample_base = (uint32_t)(q->it * q->total);
     d.q.rr_start = q->r.params.rr_start;
     d.q.max_depth = q->r.params.max_depth;
+    d.q.lights = sc->lights;
+    d.q.n_lights = sc->n_lights;
+    d.q.nee = q->r.params.nee;
+    d.q.mis = q->r.params.nee && q->r.params.direct_light_mis;
     d.cam = make_camera(q->r.camera, q->r.params.width, q->r.params.height);
     d.w = q->r.params.width;
     d.h = q->r.params.height;
@@ -517,8 +533,9 @@ void tree_shape(const mrh_server* s, uint32_t& w, uint32_t& cap)
 }
 
 // The launches of one step over queue parity `par` (captured or eager);
-// g = some admitted request learns or samples a guide
-int32_t step_launches(mrh_server* s, uint32_t par, bool g)
+// variant bit 0: some admitted request learns or samples a guide, bit 1:
+// some admitted request sends next-event rays
+int32_t step_launches(mrh_server* s, uint32_t par, uint32_t v)
 {
     mrh_context* ctx = s->ctx;
     const PathBuf& in = s->bufs[par];
@@ -528,7 +545,7 @@ int32_t step_launches(mrh_server* s, uint32_t par, bool g)
     MRH_TRY(fill_u32(ctx, s->d_res, 0u, RES_WORDS));
     MRH_TRY(launch(ctx, "serve_raygen", dim3(stride_grid(ctx, s->cap, 256)), dim3(256), 0, k_serve_raygen,
                    (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, s->slot_of[par]));
-    const uint32_t tgrid = trace_grid(ctx, s->cap);
+    const uint32_t tgrid = trace_grid(ctx, s->buf_cap);
     const uint32_t lds = ctx->trace_lds_entries;
     uint32_t w, cap;
     tree_shape(s, w, cap);
@@ -560,20 +577,24 @@ int32_t step_launches(mrh_server* s, uint32_t par, bool g)
     SortArgs sa;
     sa.keys_in = s->hit.key;
     sa.vals_out = s->order;
-    sa.n_max = s->cap;
+    sa.n_max = s->buf_cap;
     sa.n_dev = &s->d_hdr->n_total;
     sa.key_bytes = 4;
     sa.begin_bit = 24;
     sa.end_bit = 32;
     sa.descending = true;
     MRH_TRY(radix_sort(ctx, sa));
-    if (g)
-        return launch(ctx, "serve_shade", dim3(stride_grid(ctx, s->cap, 256)), dim3(256), 0, k_serve_shade<true>,
+    auto shade = [&](auto kernel) {
+        return launch(ctx, "serve_shade", dim3(stride_grid(ctx, s->buf_cap, 256)), dim3(256), 0, kernel,
                       (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, in_slot, out, out_slot, s->hit,
                       (const uint32_t*)s->order, s->d_res);
-    return launch(ctx, "serve_shade", dim3(stride_grid(ctx, s->cap, 256)), dim3(256), 0, k_serve_shade<false>,
-                  (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, in_slot, out, out_slot, s->hit,
-                  (const uint32_t*)s->order, s->d_res);
+    };
+    switch (v) {
+    case 0: return shade(k_serve_shade<false, false>);
+    case 1: return shade(k_serve_shade<true, false>);
+    case 2: return shade(k_serve_shade<false, true>);
+    default: return shade(k_serve_shade<true, true>);
+    }
 }
 
 bool graph_valid(mrh_server* s, uint32_t v, uint32_t par)
@@ -587,24 +608,23 @@ bool graph_valid(mrh_server* s, uint32_t v, uint32_t par)
            s->graph_lds[v][par] == s->ctx->trace_lds_entries && s->graph_bpc[v][par] == s->ctx->trace_blocks_per_cu;
 }
 
-int32_t run_step(mrh_server* s, uint32_t par, bool g)
+int32_t run_step(mrh_server* s, uint32_t par, uint32_t v)
 {
     mrh_context* ctx = s->ctx;
-    if (!s->p.use_graphs) return step_launches(s, par, g);
-    const uint32_t v = g ? 1u : 0u;
+    if (!s->p.use_graphs) return step_launches(s, par, v);
     if (!graph_valid(s, v, par)) {
         if (s->exec[v][par]) {
             (void)hipGraphExecDestroy(s->exec[v][par]);
             s->exec[v][par] = nullptr;
         }
         // one eager step first so every scratch buffer exists at its final size
-        MRH_TRY(step_launches(s, par, g));
+        MRH_TRY(step_launches(s, par, v));
         MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         const bool prof = ctx->profile;
         ctx->profile = false;
         hipGraph_t graph = nullptr;
         MRH_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        int32_t rc = step_launches(s, par, g);
+        int32_t rc = step_launches(s, par, v);
         hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
         ctx->profile = prof;
         if (rc != MRH_OK) {
@@ -679,7 +699,9 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
     StepHdr& h = *s->h_hdr;
     h.n_live = s->n_live;
     h.n_chunks = 0;
-    uint64_t room = s->cap - s->n_live, inj = 0;
+    // at most cap paths in flight: with a next-event ray each the other
+    // queue still holds every ray a step appends
+    uint64_t room = s->cap - s->n_paths, inj = 0;
     for (ServeReq* q : act) {
         if (!room) break;
         const uint64_t k = std::min<uint64_t>(room, q->total - q->injected);
@@ -696,15 +718,20 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
     h.n_total = s->n_live + (uint32_t)inj;
     if (h.n_total > 0) {
         MRH_HIP_CHECK(ctx, hipMemcpyAsync(s->d_hdr, s->h_hdr, sizeof(StepHdr), hipMemcpyHostToDevice, ctx->stream));
-        bool g = false;
-        for (uint32_t k = 0; k < s->p.max_slots; k++)
-            if (s->slots[k] && (s->slots[k]->learning || s->slots[k]->have_guide)) g = true;
-        MRH_TRY(run_step(s, s->parity, g));
+        uint32_t v = 0;
+        for (uint32_t k = 0; k < s->p.max_slots; k++) {
+            const ServeReq* q = s->slots[k];
+            if (!q) continue;
+            if (q->learning || q->have_guide) v |= 1u;
+            if (q->r.params.nee) v |= 2u;
+        }
+        MRH_TRY(run_step(s, s->parity, v));
         MRH_HIP_CHECK(ctx, hipMemcpyAsync(s->h_res, s->d_res, RES_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
         MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         s->steps++;
         s->segments += h.n_total;
         s->n_live = s->h_res[0];
+        s->n_paths = s->h_res[RES_PATHS];
         s->parity ^= 1u;
         for (uint32_t k = 0; k < s->p.max_slots; k++) {
             ServeReq* q = s->slots[k];
@@ -744,7 +771,7 @@ void server_free(mrh_server* s)
     for (uint32_t k = 0; k < SERVE_MAX_SLOTS; k++)
         if (s->slots[k]) release(s, s->slots[k]);
     for (ServeReq* q : s->queue) delete q;
-    for (int v = 0; v < 2; v++)
+    for (int v = 0; v < 4; v++)
         for (int par = 0; par < 2; par++)
             if (s->exec[v][par]) (void)hipGraphExecDestroy(s->exec[v][par]);
     for (mrh_svo* v : s->free_svos) svo_free(v);
@@ -768,6 +795,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
     s->user = user;
     s->p = *p;
     s->cap = p->wave_paths;
+    s->buf_cap = 2 * p->wave_paths;
     int32_t rc = mrh_context_create(user->device, &s->ctx);
     if (rc != MRH_OK) {
         delete s;
@@ -789,7 +817,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
         if (q) s->allocs.push_back(q);
         return q;
     };
-    const size_t n = s->cap;
+    const size_t n = s->buf_cap;
     for (int b = 0; b < 2; b++) {
         PathBuf& q = s->bufs[b];
         q.ox = (float*)take(n * 4); q.oy = (float*)take(n * 4); q.oz = (float*)take(n * 4);
@@ -798,7 +826,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
         q.sample = (uint32_t*)take(n * 4);
         q.dm = (uint32_t*)take(n * 4);
         q.rng = (uint64_t*)take(n * 8);
-        q.pdf = nullptr;
+        q.pdf = (float*)take(n * 4);
         s->slot_of[b] = (uint32_t*)take(n * 4);
     }
     s->hit.leaf = (uint32_t*)take(n * 4);
@@ -841,7 +869,10 @@ int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
     if (p.width == 0 || p.height == 0 || p.samples == 0)
         return fail(ctx, MRH_ERR_INVALID, "submit: empty frame");
     if (p.max_depth == 0 || p.max_depth > 62) return fail(ctx, MRH_ERR_INVALID, "submit: max_depth must be in [1, 62]");
-    if (p.nee) return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: next-event estimation is not implemented");
+    if (p.nee > 1 || p.direct_light_mis > 1)
+        return fail(ctx, MRH_ERR_INVALID, "submit: nee and direct_light_mis are 0 or 1");
+    if (p.nee && r->guide.iterations > 1)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: next-event estimation with guiding is not implemented");
     if (r->scene->bvh->max_depth > 128)
         return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: tree depth %u exceeds the traversal stack",
                     r->scene->bvh->max_depth);

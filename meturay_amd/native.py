@@ -537,11 +537,12 @@ class Server:
         self.inner = _BorrowedContext(inner)
 
     def submit(self, scene: Scene, d_image, width, height, samples, max_depth, rr_start, seed=0,
-               camera=None, iterations=1, svo_depth=10, guide_ratio=0.5, min_samples=512, tag=0) -> int:
+               camera=None, iterations=1, svo_depth=10, guide_ratio=0.5, min_samples=512, tag=0,
+               nee=0, mis=0) -> int:
         r = Request()
         r.scene = scene.h
         r.camera = camera_desc(camera or scene.camera)
-        r.params = RenderParams(width, height, samples, max_depth, rr_start, 0, seed, 0)
+        r.params = RenderParams(width, height, samples, max_depth, rr_start, nee, seed, 0, mis)
         r.guide = GuideParams(iterations, svo_depth, guide_ratio, min_samples)
         r.d_image = ptr(d_image)
         r.user_tag = tag

@@ -11,48 +11,51 @@ import numpy as np
 import pytest
 
 from meturay_amd import native as N
+import nee_scenes as S
 
 pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-# (scene, width, height, strata, max depth, rr, seed, iterations)
+# (scene, width, height, strata, max depth, rr, seed, iterations, nee, mis)
 MIX = [
-    ("cornell", 48, 32, 2, 6, 2, 1, 1),
-    ("spheres", 64, 48, 1, 8, 3, 2, 1),
-    ("cornell", 40, 40, 2, 6, 2, 3, 3),
-    ("spheres", 32, 24, 2, 10, 3, 4, 3),
-    ("cornell", 16, 16, 1, 4, 1, 5, 1),
-    ("spheres", 80, 60, 1, 5, 2, 6, 2),
+    ("cornell", 48, 32, 2, 6, 2, 1, 1, 0, 0),
+    ("spheres", 64, 48, 1, 8, 3, 2, 1, 0, 0),
+    ("cornell", 40, 40, 2, 6, 2, 3, 3, 0, 0),
+    ("lit", 32, 24, 2, 10, 3, 4, 3, 0, 0),
+    ("cornell", 16, 16, 1, 4, 1, 5, 1, 1, 0),
+    ("spheres", 80, 60, 1, 5, 2, 6, 2, 0, 0),
+    ("lit", 48, 40, 1, 8, 3, 7, 1, 1, 1),
+    ("cornell", 24, 24, 2, 6, 3, 8, 1, 1, 1),
 ]
 
 
 @pytest.fixture(scope="module")
 def scenes(ctx):
-    sc = {"cornell": N.gen_cornell(2), "spheres": N.gen_sphere_grid(4, 8, 8, 5)}
+    sc = {"cornell": N.gen_cornell(2), "spheres": N.gen_sphere_grid(4, 8, 8, 5), "lit": S.lit_spheres()}
     res = {k: (v, N.Scene(ctx, v)) for k, v in sc.items()}
     yield res
     for _, s in res.values():
         s.close()
 
 
-def solo(ctx, scene, w, h, S, depth, rr, seed, it):
+def solo(ctx, scene, w, h, spp_axis, depth, rr, seed, it, nee, mis):
     img = torch.zeros(w * h * 3, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     if it > 1:
-        st = scene.render_guided(img.data_ptr(), w, h, S, depth, rr, seed, iterations=it, svo_depth=6,
+        st = scene.render_guided(img.data_ptr(), w, h, spp_axis, depth, rr, seed, iterations=it, svo_depth=6,
                                  guide_ratio=0.5, min_samples=16)
     else:
-        st = scene.render(img.data_ptr(), w, h, S, depth, rr, seed)
+        st = scene.render(img.data_ptr(), w, h, spp_axis, depth, rr, seed, nee=nee, mis=mis)
     return img.cpu().numpy(), st.segments
 
 
 def submit_mix(server, scenes, imgs):
     tickets = {}
-    for k, (name, w, h, S, depth, rr, seed, it) in enumerate(MIX):
+    for k, (name, w, h, spp_axis, depth, rr, seed, it, nee, mis) in enumerate(MIX):
         scene = scenes[name][1]
-        tickets[server.submit(scene, imgs[k], w, h, S, depth, rr, seed, iterations=it, svo_depth=6,
-                              guide_ratio=0.5, min_samples=16, tag=k)] = k
+        tickets[server.submit(scene, imgs[k], w, h, spp_axis, depth, rr, seed, iterations=it, svo_depth=6,
+                              guide_ratio=0.5, min_samples=16, tag=k, nee=nee, mis=mis)] = k
     return tickets
 
 
@@ -120,5 +123,9 @@ def test_server_errors(ctx, scenes):
         server.submit(scene, img, 16, 16, 1, 0, 1, 0)             # max_depth 0
     with pytest.raises(N.MrhError):
         server.submit(scene, img, 16, 16, 1, 4, 1, 0, iterations=2, svo_depth=11)
+    with pytest.raises(N.MrhError):
+        server.submit(scene, img, 16, 16, 1, 4, 1, 0, iterations=2, svo_depth=6, nee=1)   # NEE + guiding
+    with pytest.raises(N.MrhError):
+        server.submit(scene, img, 16, 16, 1, 4, 1, 0, nee=2)
     assert server.step() == (0, 0)                                # idle step is a no-op
     server.close()

@@ -115,6 +115,22 @@ def guided_leg(N, ctx, scene, img, torch):
             "segments": int(st.segments), "octree_nodes": int(v.nodes)}
 
 
+def nee_leg(N, ctx, scene, img, torch):
+    """The same frame with next-event estimation and direct-light MIS: one
+    light sample per non-specular vertex, traced with the next bounce.  One
+    warm-up render, one timed; segments count path and next-event rays."""
+    scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED, nee=1, mis=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED, nee=1, mis=1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"nee+mis: {st.segments} segments in {dt * 1e3:.1f} ms")
+    return {"workload": f"same frame, 1920x1080, {S * S} spp, max depth {DEPTH}, NEE + power-heuristic MIS",
+            "Mrays_per_s": round(st.segments / dt / 1e6, 3), "ms": round(dt * 1e3, 3),
+            "segments": int(st.segments)}
+
+
 SERVE_RES = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
 SERVE_SPP = [1, 4, 16, 64]
 
@@ -248,6 +264,7 @@ def main():
     ap.add_argument("--knobs", default=None,
                     help="fixed launch shape width,lds,blocks_per_cu instead of autotuning (e.g. 4,8,8)")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-tracer leg")
+    ap.add_argument("--no-nee", action="store_true", help="skip the next-event-estimation leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving leg requests (0 = skip)")
     ap.add_argument("--rate", type=float, default=40.0, help="serving leg arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -353,6 +370,9 @@ def main():
     guided = None
     if not args.no_guided:
         guided = guided_leg(N, ctx, scene, img, torch)
+    nee = None
+    if not args.no_nee:
+        nee = nee_leg(N, ctx, scene, img, torch)
     serving = None
     if args.requests > 0:
         serving = serving_leg(N, ctx, torch, args.requests, args.rate)
@@ -393,6 +413,7 @@ def main():
             "serving": serving,
             "autotune": tune,
             "guided": guided,
+            "nee": nee,
             "kernel_ms_per_frame": {"trace": round(trace_ms / args.steps, 3),
                                     "shade": round(shade_ms / args.steps, 3),
                                     "partition": round((sort_ms + hist_ms) / args.steps, 3)},

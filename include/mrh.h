@@ -250,6 +250,54 @@ int32_t mrh_render(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* ca
 int32_t mrh_render_bytes(const mrh_render_params* params, uint64_t* bytes);
 
 /* ------------------------------------------------------------------ */
+/* Reconstruction filters (GPUReconFilterI and its Box / Tent / Gaussian /
+ * Mitchell implementations, GPUReconFilter.h:59-260, GPUReconFilterKC.cuh,
+ * options "type" / "radius" / "alpha" / "b" / "c", TracerSystemCUDA.cpp:
+ * 109-140).  A sample at image coordinate c (pixel units, pixel (x, y)
+ * covers [x, x+1) x [y, y+1)) reaches the pixels of the conservative window
+ * of FilterRadiusToPixelWH(radius) whose centres lie within `radius` of c
+ * (radius 0: its own pixel only), with weight filter(centre - c).  A pixel
+ * holds sum(w * value) and sum(w); the image is their ratio (0 where the
+ * weights sum to 0).  Within a pixel, contributions are summed in a fixed
+ * order (neighbour rows, then columns, then sample order), so results are
+ * reproducible.                                                          */
+/* ------------------------------------------------------------------ */
+typedef enum {
+    MRH_FILTER_BOX = 0,       /* "Box": 1 inside |dx|, |dy| < radius (1 everywhere at radius 0) */
+    MRH_FILTER_TENT = 1,      /* "Tent": (1 - |dx|/r)(1 - |dy|/r), clamped at 0 */
+    MRH_FILTER_GAUSSIAN = 2,  /* "Gaussian": (e^(-a dx^2) - e^(-a r^2))+ (same in y) */
+    MRH_FILTER_MITCHELL = 3   /* "Mitchell": Mitchell-Netravali (b, c) of 2|dx|/r times of 2|dy|/r */
+} mrh_filter_type;
+
+typedef struct {
+    uint32_t type;            /* mrh_filter_type */
+    float    radius;          /* pixels, >= 0; Tent, Gaussian and Mitchell need > 0 */
+    float    alpha;           /* Gaussian "alpha" */
+    float    b, c;            /* Mitchell "b", "c" */
+} mrh_filter_desc;
+
+/* GPUReconFilterI::FilterToImg: n samples (values float[n][3], image
+ * coordinates float[n][2]) are filtered into a w x h image, accumulating
+ * (+=) d_weighted (float[h][w][3], sum of w * value) and d_weights
+ * (float[h][w], sum of w).  Samples outside [0, w) x [0, h) are ignored.
+ * Synchronous. */
+int32_t mrh_filter_samples(mrh_context* ctx, const mrh_filter_desc* filter, const float* d_values,
+                           const float* d_coords, uint32_t n, uint32_t width, uint32_t height,
+                           float* d_weighted, float* d_weights);
+
+/* d_image = d_weighted / d_weights per pixel (0 where the weight is 0) */
+int32_t mrh_filter_resolve(mrh_context* ctx, const float* d_weighted, const float* d_weights,
+                           uint32_t width, uint32_t height, float* d_image);
+
+/* mrh_render with a reconstruction filter (RayTracer::Finalize ->
+ * FilterToImg, RayTracer.cu:81-95): every sample's radiance and image
+ * coordinate are kept and filtered into d_image.  With Box radius 0 the
+ * image equals mrh_render's per-pixel mean. */
+int32_t mrh_render_filtered(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                            const mrh_render_params* params, const mrh_filter_desc* filter, float* d_image,
+                            mrh_render_stats* stats);
+
+/* ------------------------------------------------------------------ */
 /* Path-guiding sparse voxel octree.  Replaces the WFPG tracer's SVO
  * (AnisoSVOctreeCPU::Constrcut AnisoSVO.cu:657-1188, KCAccumulateRadianceToLeaf
  * AnisoSVO.cu:225-263, NormalizeAndFilterRadiance AnisoSVO.cu:1415-1483) and
@@ -364,6 +412,10 @@ int32_t mrh_tracer_stats(const mrh_tracer* tracer, mrh_render_stats* out, uint32
  * the first samples from the mixture of the latest guide and the BxDF.
  * `iterations` frames reproduce mrh_render_guided.  svo may be NULL. */
 int32_t mrh_tracer_set_guide(mrh_tracer* tracer, const mrh_guide_params* guide, mrh_svo* svo);
+/* GPUTracerI::AttachReconFilter (RayLib/GPUTracerI.h:69): frames from the
+ * next GenerateWork on are filtered (filter NULL: per-pixel mean again);
+ * resets the accumulated image. */
+int32_t mrh_tracer_set_filter(mrh_tracer* tracer, const mrh_filter_desc* filter);
 int32_t mrh_tracer_destroy(mrh_tracer* tracer);
 
 /* ------------------------------------------------------------------ */

@@ -98,6 +98,16 @@ public:
     TracerError GetInt(int64_t& out, const std::string& name) const { return GetT(out, name); }
     TracerError GetDouble(double& out, const std::string& name) const { return GetT(out, name); }
     TracerError GetString(std::string& out, const std::string& name) const { return GetT(out, name); }
+    // Options::GetFloat: a number given as an integer reads as well
+    TracerError GetNumber(double& out, const std::string& name) const
+    {
+        int64_t i = 0;
+        if (GetT(i, name) == TracerError::OK) {
+            out = (double)i;
+            return TracerError::OK;
+        }
+        return GetT(out, name);
+    }
     bool Has(const std::string& name) const { return values.count(name) != 0; }
 
 private:
@@ -156,10 +166,24 @@ public:
     virtual mrh_scene* Handle() const = 0;
 };
 
+// GPUReconFilterI (TracerCUDA/GPUReconFilterI.h): a reconstruction filter
+// made by TracerSystemI::GenerateReconFilter and attached to tracers
+class GPUReconFilterI {
+public:
+    static constexpr const char* RADIUS_OPTION_NAME = "radius";
+    static constexpr const char* TYPE_OPTION_NAME = "type";
+    virtual ~GPUReconFilterI() = default;
+    virtual const char* Type() const = 0;   // "Box", "Tent", "Gaussian", "Mitchell"
+    virtual const mrh_filter_desc& Desc() const = 0;
+};
+
 class GPUTracerI {
 public:
     virtual ~GPUTracerI() = default;
     virtual void AttachTracerCallbacks(TracerCallbacksI&) = 0;
+    // RayLib/GPUTracerI.h:69; frames are filtered from the next GenerateWork
+    // on (without a filter each pixel is the mean of its own samples)
+    virtual void AttachReconFilter(GPUReconFilterI* filter) = 0;
     virtual TracerError Initialize() = 0;
     virtual TracerError SetOptions(const Options&) = 0;
     virtual void SetParameters(const TracerParameters&) = 0;
@@ -190,6 +214,10 @@ public:
     // tracerType: "PathTracer"
     virtual TracerError GenerateTracer(GPUTracerPtr& tracer, const TracerParameters& params,
                                        const Options& opts, const std::string& tracerType) = 0;
+    // TracerSystemCUDA::GenerateReconFilter (TracerSystemCUDA.cpp:109-140):
+    // "type" and "radius" are required, plus "alpha" (Gaussian) or "b" and
+    // "c" (Mitchell); the system owns the filter (one at a time)
+    virtual TracerError GenerateReconFilter(GPUReconFilterI*& filter, const Options& filterOptions) = 0;
 };
 
 }  // namespace mrh

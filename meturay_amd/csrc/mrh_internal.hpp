@@ -28,6 +28,8 @@ enum MrhSlot : int {
     SLOT_FRAME,
     SLOT_TRACE_OVERFLOW,
     SLOT_GUIDE_REC,
+    SLOT_FILTER,
+    SLOT_SAMPLES,
     SLOT_COUNT
 };
 

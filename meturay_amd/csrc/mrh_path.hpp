@@ -398,9 +398,12 @@ struct GuideArgs {
 // Camera ray of sample i of a frame (GenerationKernels.cuh:29-108 +
 // GPUCameraPinhole.cuh:209-240): stratum (sx, sy) of pixel (px, py), jittered
 // by the first two draws of the path's stream, keyed by the global sample id
-// gid.
+// gid.  coord (may be null) receives the sample's image coordinate
+// ((stratum + jitter) / strata * resolution, GPUCameraPinhole.cuh:236-237
+// and GenerationKernels.cuh:77-78), kept inside its own pixel.
 __device__ __forceinline__ void camera_ray(const Cam& cam, uint32_t w, uint32_t h, uint32_t S, uint32_t seed,
-                                           uint32_t gid, uint32_t i, V3& o, V3& d, uint64_t& rng_state)
+                                           uint32_t gid, uint32_t i, V3& o, V3& d, uint64_t& rng_state,
+                                           float2* coord = nullptr)
 {
     const uint32_t spp = S * S;
     const float dxp = cam.pw / (float)(w * S);
@@ -418,6 +421,16 @@ __device__ __forceinline__ void camera_ray(const Cam& cam, uint32_t w, uint32_t 
     d = normalize(sp - cam.pos);
     o = cam.pos;
     rng_state = rng.state;
+    if (coord) {
+        float cx = ((float)(px * S + sx) + jx) / (float)(w * S) * (float)w;
+        float cy = ((float)(py * S + sy) + jy) / (float)(h * S) * (float)h;
+        // rounding may land on a neighbour's edge: keep [px, px + 1)
+        const float ex = __int_as_float(__float_as_int((float)(px + 1)) - 1);
+        const float ey = __int_as_float(__float_as_int((float)(py + 1)) - 1);
+        cx = fminf(fmaxf(cx, (float)px), ex);
+        cy = fminf(fmaxf(cy, (float)py), ey);
+        *coord = make_float2(cx, cy);
+    }
 }
 
 // Per-request constants of the shade step

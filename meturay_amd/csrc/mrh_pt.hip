@@ -15,6 +15,7 @@
 //                bounce's contiguous live range.
 // Paths own a PCG32 stream keyed by (seed, global sample id), so results do
 // not depend on the order rays are processed in.
+#include "mrh_filter.hpp"
 #include "mrh_path.hpp"
 
 #include <cfloat>
@@ -24,14 +25,16 @@ namespace mrh {
 namespace {
 
 // GenerationKernels.cuh:29-108 + GPUCameraPinhole.cuh:209-240
+// (coords, when set, receives every sample's image coordinate)
 __global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h, uint32_t S,
                                                 uint32_t seed, uint32_t sample_base, uint32_t n_total,
-                                                PathBuf out)
+                                                PathBuf out, float2* __restrict__ coords)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_total; i += gridDim.x * blockDim.x) {
         V3 o, d;
         uint64_t st;
-        camera_ray(cam, w, h, S, seed, sample_base + i, i, o, d, st);   // global sample id keys the stream
+        // the global sample id keys the stream
+        camera_ray(cam, w, h, S, seed, sample_base + i, i, o, d, st, coords ? coords + i : nullptr);
         out.ox[i] = o.x; out.oy[i] = o.y; out.oz[i] = o.z;
         out.dx[i] = d.x; out.dy[i] = d.y; out.dz[i] = d.z;
         out.tr[i] = 1.0f; out.tg[i] = 1.0f; out.tb[i] = 1.0f;
@@ -191,6 +194,7 @@ struct Frame {
     uint64_t total = 0, cap = 0, spp = 0;
     uint32_t w = 0, h = 0, S = 0, max_depth = 0, rr_start = 0, seed = 0, sample_base = 0;
     uint32_t nee = 0, mis = 0;
+    uint32_t per_sample = 0;   // radiance goes to a per-sample buffer (filtered frames)
     uint32_t count_traversal = 0, bounce = 0;
     uint32_t learn = 0, guide = 0;   // record path vertices / sample the guide
     GuideArgs ga = {};
@@ -347,9 +351,11 @@ int32_t validate_params(mrh_context* ctx, const mrh_scene* s, const mrh_render_p
     return MRH_OK;
 }
 
-// GenerateWork: clear counts, write the camera rays of one frame
+// GenerateWork: clear counts, write the camera rays of one frame; with
+// coords the frame keeps every sample's radiance apart (the image pointer
+// of frame_bounce is then a per-sample buffer) and its image coordinate.
 int32_t frame_begin(mrh_context* ctx, const mrh_camera_desc* cd, const mrh_render_params* p,
-                    void* arena, uint32_t sample_base, Frame& f)
+                    void* arena, uint32_t sample_base, Frame& f, float2* coords = nullptr)
 {
     f.w = p->width;
     f.h = p->height;
@@ -364,6 +370,7 @@ int32_t frame_begin(mrh_context* ctx, const mrh_camera_desc* cd, const mrh_rende
     f.nee = p->nee;
     f.mis = p->nee && p->direct_light_mis;
     f.cap = frame_cap(f.total, f.nee);
+    f.per_sample = coords != nullptr;
     f.bounce = 0;
     f.learn = f.guide = 0;
     f.cam = make_camera(*cd, p->width, p->height);
@@ -371,7 +378,7 @@ int32_t frame_begin(mrh_context* ctx, const mrh_camera_desc* cd, const mrh_rende
     MRH_TRY(fill_u32(ctx, f.counts, 0u, COUNTS_WORDS));
     MRH_TRY(fill_u32(ctx, f.counts, (uint32_t)f.total, 1));
     return launch(ctx, "raygen", dim3(stride_grid(ctx, f.total, 256)), dim3(256), 0, k_raygen, f.cam,
-                  f.w, f.h, f.S, f.seed, f.sample_base, (uint32_t)f.total, f.bufs[0]);
+                  f.w, f.h, f.S, f.seed, f.sample_base, (uint32_t)f.total, f.bufs[0], coords);
 }
 
 // One iteration of PathTracer::Render: trace, partition, shade
@@ -407,7 +414,7 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     q.mats = Materials{s->lambert, s->metal, s->dielectric, s->emissive};
     q.leaves = lv;
     q.image = image;
-    q.spp = (uint32_t)f.spp;
+    q.spp = f.per_sample ? 1u : (uint32_t)f.spp;
     q.sample_base = f.sample_base;
     q.rr_start = f.rr_start;
     q.max_depth = f.max_depth;
@@ -446,25 +453,73 @@ int32_t frame_counts(mrh_context* ctx, const Frame& f, mrh_render_stats* stats)
     return MRH_OK;
 }
 
+// Per-sample buffers of a filtered frame: radiance, image coordinates and
+// the filter's two sums
+struct SampleBufs {
+    float* values = nullptr;     // [samples][3]
+    float2* coords = nullptr;    // [samples]
+    float* weighted = nullptr;   // [pixels][3], sum of w * value
+    float* weights = nullptr;    // [pixels], sum of w
+};
+
+size_t sample_bytes(uint64_t total, uint64_t pixels)
+{
+    return align_up(total * 12) + align_up(total * 8) + align_up(pixels * 12) + align_up(pixels * 4);
+}
+
+SampleBufs carve_samples(void* base, uint64_t total, uint64_t pixels)
+{
+    SampleBufs b;
+    char* q = (char*)base;
+    b.values = (float*)q; q += align_up(total * 12);
+    b.coords = (float2*)q; q += align_up(total * 8);
+    b.weighted = (float*)q; q += align_up(pixels * 12);
+    b.weights = (float*)q;
+    return b;
+}
+
+// One frame (PathTracer::Render until done, then Finalize).  Without a
+// filter every sample adds to its pixel and the image is the per-pixel
+// mean; with one (RayTracer::Finalize -> FilterToImg, RayTracer.cu:81-95)
+// every sample keeps its radiance and image coordinate and the filter
+// gathers them into the image.
 int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const mrh_render_params* p,
-               float* d_image, mrh_render_stats* stats)
+               const mrh_filter_desc* fd, float* d_image, mrh_render_stats* stats)
 {
     uint64_t total = 0;
     MRH_TRY(validate_params(ctx, s, p, &total));
+    FilterDev filt;
+    if (fd) MRH_TRY(filter_make(ctx, fd, &filt));
     void* arena = nullptr;
     MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(frame_cap(total, p->nee)), &arena));
-    const size_t img_count = (size_t)p->width * p->height * 3;
-    MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
+    const size_t pixels = (size_t)p->width * p->height;
+    const size_t img_count = pixels * 3;
+    SampleBufs sb;
+    if (fd) {
+        void* mem = nullptr;
+        MRH_TRY(scratch(ctx, SLOT_SAMPLES, sample_bytes(total, pixels), &mem));
+        sb = carve_samples(mem, total, pixels);
+        MRH_TRY(fill_u32(ctx, (uint32_t*)sb.values, 0u, total * 3));
+        MRH_TRY(fill_u32(ctx, (uint32_t*)sb.weighted, 0u, img_count));
+        MRH_TRY(fill_u32(ctx, (uint32_t*)sb.weights, 0u, pixels));
+    } else {
+        MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MRH_HIP_CHECK(ctx, hipEventCreate(&e0));
     MRH_HIP_CHECK(ctx, hipEventCreate(&e1));
     MRH_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
     Frame f;
-    int32_t rc = frame_begin(ctx, cd, p, arena, 0u, f);
-    while (rc == MRH_OK && f.bounce < f.max_depth) rc = frame_bounce(ctx, s, f, d_image);
-    if (rc == MRH_OK)
+    int32_t rc = frame_begin(ctx, cd, p, arena, 0u, f, fd ? sb.coords : nullptr);
+    while (rc == MRH_OK && f.bounce < f.max_depth) rc = frame_bounce(ctx, s, f, fd ? sb.values : d_image);
+    if (rc == MRH_OK && fd) {
+        rc = filter_gather(ctx, filt, sb.values, sb.coords, nullptr, nullptr, (uint32_t)f.spp, p->width, p->height,
+                           sb.weighted, sb.weights);
+        if (rc == MRH_OK) rc = filter_resolve(ctx, sb.weighted, sb.weights, pixels, d_image);
+    } else if (rc == MRH_OK) {
         rc = launch(ctx, "finalize", dim3(stride_grid(ctx, img_count, 256)), dim3(256), 0, k_scale, d_image,
                     img_count, 1.0f / (float)f.spp);
+    }
     if (rc == MRH_OK) {
         hipEventRecord(e1, ctx->stream);
         mrh_render_stats local;
@@ -707,7 +762,17 @@ int32_t mrh_render(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* ca
 {
     if (!ctx || !scene || !cam || !params || !d_image) return MRH_ERR_INVALID;
     if (scene->ctx != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_render: scene belongs to another context");
-    return render(ctx, scene, cam, params, d_image, stats);
+    return render(ctx, scene, cam, params, nullptr, d_image, stats);
+}
+
+int32_t mrh_render_filtered(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                            const mrh_render_params* params, const mrh_filter_desc* filter, float* d_image,
+                            mrh_render_stats* stats)
+{
+    if (!ctx || !scene || !cam || !params || !filter || !d_image) return MRH_ERR_INVALID;
+    if (scene->ctx != ctx)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render_filtered: scene belongs to another context");
+    return render(ctx, scene, cam, params, filter, d_image, stats);
 }
 
 int32_t mrh_render_guided(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
@@ -751,6 +816,12 @@ struct mrh_tracer {
     bool guided = false;
     uint32_t guide_frames = 0;   // frames rendered since guiding was set
     mrh::GuideState g;
+    // attached reconstruction filter: per-sample buffers of the frame in
+    // flight and the accumulated filter sums
+    bool filtered = false;
+    mrh::FilterDev filt = {};
+    void* smem = nullptr;
+    mrh::SampleBufs sb;
 };
 
 extern "C" {
@@ -787,7 +858,9 @@ int32_t mrh_tracer_generate_work(mrh_tracer* t, const mrh_camera_desc* cam)
     if (!t || !cam) return MRH_ERR_INVALID;
     mrh_context* ctx = t->ctx;
     if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_generate_work: the previous frame is still rendering");
-    MRH_TRY(frame_begin(ctx, cam, &t->p, t->arena, (uint32_t)(t->frames * t->total), t->f));
+    if (t->filtered) MRH_TRY(fill_u32(ctx, (uint32_t*)t->sb.values, 0u, t->total * 3));
+    MRH_TRY(frame_begin(ctx, cam, &t->p, t->arena, (uint32_t)(t->frames * t->total), t->f,
+                        t->filtered ? t->sb.coords : nullptr));
     if (t->guided) MRH_TRY(guide_frame(ctx, t->g, t->guide_frames, t->f));
     t->active = true;
     return MRH_OK;
@@ -798,11 +871,14 @@ int32_t mrh_tracer_render(mrh_tracer* t, int32_t* more)
     if (!t || !more) return MRH_ERR_INVALID;
     *more = 0;
     if (!t->active) return MRH_OK;
-    MRH_TRY(frame_bounce(t->ctx, t->scene, t->f, t->accum));
+    MRH_TRY(frame_bounce(t->ctx, t->scene, t->f, t->filtered ? t->sb.values : t->accum));
     if (t->f.bounce < t->f.max_depth) {
         *more = 1;
         return MRH_OK;
     }
+    if (t->filtered)
+        MRH_TRY(filter_gather(t->ctx, t->filt, t->sb.values, t->sb.coords, nullptr, nullptr, (uint32_t)t->f.spp,
+                              t->p.width, t->p.height, t->sb.weighted, t->sb.weights));
     MRH_TRY(frame_counts(t->ctx, t->f, &t->stats));
     t->active = false;
     t->frames++;
@@ -819,9 +895,13 @@ int32_t mrh_tracer_finalize(mrh_tracer* t, float* d_image)
     mrh_context* ctx = t->ctx;
     if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_finalize: frame still rendering");
     const size_t img = (size_t)t->p.width * t->p.height * 3;
-    const float s = t->frames ? 1.0f / (float)((uint64_t)t->frames * t->f.spp) : 0.0f;
-    MRH_TRY(launch(ctx, "finalize", dim3(stride_grid(ctx, img, 256)), dim3(256), 0, k_mean,
-                   (const float*)t->accum, d_image, img, s));
+    if (t->filtered) {
+        MRH_TRY(filter_resolve(ctx, t->sb.weighted, t->sb.weights, img / 3, d_image));
+    } else {
+        const float s = t->frames ? 1.0f / (float)((uint64_t)t->frames * t->f.spp) : 0.0f;
+        MRH_TRY(launch(ctx, "finalize", dim3(stride_grid(ctx, img, 256)), dim3(256), 0, k_mean,
+                       (const float*)t->accum, d_image, img, s));
+    }
     MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return MRH_OK;
 }
@@ -850,7 +930,34 @@ int32_t mrh_tracer_reset_image(mrh_tracer* t)
     if (t->active) return fail(t->ctx, MRH_ERR_INVALID, "mrh_tracer_reset_image: frame still rendering");
     t->frames = 0;
     t->stats = mrh_render_stats{};
-    return fill_u32(t->ctx, (uint32_t*)t->accum, 0u, (size_t)t->p.width * t->p.height * 3);
+    const size_t pixels = (size_t)t->p.width * t->p.height;
+    if (t->filtered) {
+        MRH_TRY(fill_u32(t->ctx, (uint32_t*)t->sb.weighted, 0u, pixels * 3));
+        MRH_TRY(fill_u32(t->ctx, (uint32_t*)t->sb.weights, 0u, pixels));
+    }
+    return fill_u32(t->ctx, (uint32_t*)t->accum, 0u, pixels * 3);
+}
+
+int32_t mrh_tracer_set_filter(mrh_tracer* t, const mrh_filter_desc* filter)
+{
+    if (!t) return MRH_ERR_INVALID;
+    mrh_context* ctx = t->ctx;
+    if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_set_filter: frame still rendering");
+    FilterDev f = {};
+    if (filter) MRH_TRY(filter_make(ctx, filter, &f));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (t->smem) (void)hipFree(t->smem);
+    t->smem = nullptr;
+    t->sb = SampleBufs{};
+    t->filtered = false;
+    if (filter) {
+        const uint64_t pixels = (uint64_t)t->p.width * t->p.height;
+        MRH_TRY(dmalloc(ctx, &t->smem, sample_bytes(t->total, pixels)));
+        t->sb = carve_samples(t->smem, t->total, pixels);
+        t->filt = f;
+        t->filtered = true;
+    }
+    return mrh_tracer_reset_image(t);
 }
 
 int32_t mrh_tracer_stats(const mrh_tracer* t, mrh_render_stats* out, uint32_t* frames)
@@ -867,6 +974,7 @@ int32_t mrh_tracer_destroy(mrh_tracer* t)
     hipStreamSynchronize(t->ctx->stream);
     hipFree(t->arena);
     hipFree(t->accum);
+    if (t->smem) hipFree(t->smem);
     guide_free(t->g);
     delete t;
     return MRH_OK;

@@ -43,6 +43,18 @@ private:
     std::vector<mrh_camera_desc> cams;
 };
 
+// GPUReconFilter{Box,Tent,Gaussian,Mitchell} (TracerCUDA/GPUReconFilter*.h)
+class ReconFilterMRH final : public GPUReconFilterI {
+public:
+    ReconFilterMRH(const char* type, const mrh_filter_desc& d) : type(type), desc(d) {}
+    const char* Type() const override { return type; }
+    const mrh_filter_desc& Desc() const override { return desc; }
+
+private:
+    const char* type;
+    mrh_filter_desc desc;
+};
+
 // PathTracer (TracerCUDA/PathTracer.cu) and, with `guided`, WFPGTracer
 // (TracerCUDA/WFPGTracer.cu) over mrh_tracer
 class PathTracerMRH final : public GPUTracerI {
@@ -54,6 +66,15 @@ public:
     ~PathTracerMRH() override { Release(); }
 
     void AttachTracerCallbacks(TracerCallbacksI& cb) override { callbacks = &cb; }
+
+    void AttachReconFilter(GPUReconFilterI* f) override
+    {
+        filter = f;
+        if (tracer) {
+            int32_t rc = mrh_tracer_set_filter(tracer, f ? &f->Desc() : nullptr);
+            if (rc != MRH_OK) Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+        }
+    }
 
     TracerError Initialize() override
     {
@@ -72,6 +93,10 @@ public:
         rp.seed = params.seed;
         int32_t rc = mrh_tracer_create(ctx, scene->Handle(), &rp, &tracer);
         if (rc != MRH_OK) return Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+        if (filter) {
+            rc = mrh_tracer_set_filter(tracer, &filter->Desc());
+            if (rc != MRH_OK) return Report(TracerError::FromStatus(rc, mrh_last_error(ctx)));
+        }
         if (guided) {
             mrh_guide_params gp = {};
             gp.iterations = learnIterations;
@@ -259,6 +284,7 @@ private:
     TracerParameters params;
     Options options;
     TracerCallbacksI* callbacks = nullptr;
+    GPUReconFilterI* filter = nullptr;
     mrh_tracer* tracer = nullptr;
     float* dImage = nullptr;
     uint32_t width = 256, height = 256, samples = 1, maxDepth = 10, rrStart = 3;
@@ -326,6 +352,47 @@ public:
         return SceneError(SceneError::NO_LOADER, std::string(path.begin(), path.end()));
     }
 
+    TracerError GenerateReconFilter(GPUReconFilterI*& out, const Options& o) override
+    {
+        out = nullptr;
+        std::string type;
+        if (o.GetString(type, GPUReconFilterI::TYPE_OPTION_NAME) != TracerError::OK)
+            return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "Filter Options has to have \"type\" field");
+        double radius = 0.0;
+        if (o.GetNumber(radius, GPUReconFilterI::RADIUS_OPTION_NAME) != TracerError::OK)
+            return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER,
+                               "Filter Options has to have \"radius\" field");
+        static const char* names[] = {"Box", "Tent", "Gaussian", "Mitchell"};
+        mrh_filter_desc d = {};
+        uint32_t k = 0;
+        while (k < 4 && type != names[k]) k++;
+        if (k == 4) return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "unknown filter type " + type);
+        d.type = k;
+        d.radius = (float)radius;
+        double v = 0.0;
+        if (k == MRH_FILTER_GAUSSIAN) {
+            if (o.GetNumber(v, "alpha") != TracerError::OK)
+                return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "Gaussian Filter parameter \"alpha\" not found");
+            d.alpha = (float)v;
+        }
+        if (k == MRH_FILTER_MITCHELL) {
+            if (o.GetNumber(v, "b") != TracerError::OK)
+                return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "Mitchell Filter parameter \"b\" not found");
+            d.b = (float)v;
+            if (o.GetNumber(v, "c") != TracerError::OK)
+                return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "Mitchell Filter parameter \"c\" not found");
+            d.c = (float)v;
+        }
+        // the checks of filter_make, so a bad filter fails here
+        if (!(d.radius >= 0.0f) || d.radius > 16.0f || (k != MRH_FILTER_BOX && d.radius == 0.0f))
+            return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "bad filter radius");
+        if (k == MRH_FILTER_GAUSSIAN && !(d.alpha > 0.0f))
+            return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "Gaussian alpha must be positive");
+        reconFilter = std::make_unique<ReconFilterMRH>(names[k], d);
+        out = reconFilter.get();
+        return TracerError::OK;
+    }
+
     TracerError GenerateTracer(GPUTracerPtr& tracer, const TracerParameters& params, const Options& opts,
                                const std::string& type) override
     {
@@ -341,6 +408,7 @@ public:
 private:
     mrh_context* ctx = nullptr;
     std::unique_ptr<SceneMRH> scene;
+    std::unique_ptr<ReconFilterMRH> reconFilter;
 };
 
 }  // namespace

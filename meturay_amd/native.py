@@ -67,6 +67,18 @@ class TuneResult(C.Structure):
                 ("best_width", _U32), ("trial_width", _U32 * 32)]
 
 
+class FilterDesc(C.Structure):
+    _fields_ = [("type", _U32), ("radius", _F), ("alpha", _F), ("b", _F), ("c", _F)]
+
+
+FILTER_TYPES = {"Box": 0, "Tent": 1, "Gaussian": 2, "Mitchell": 3}
+
+
+def filter_desc(kind: str, radius: float, alpha: float = 0.0, b: float = 0.0, c: float = 0.0) -> FilterDesc:
+    """mrh_filter_desc of a "type" / "radius" (+ "alpha" or "b", "c") filter."""
+    return FilterDesc(FILTER_TYPES[kind], radius, alpha, b, c)
+
+
 class SvoView(C.Structure):
     _fields_ = [("depth", _U32), ("nodes", _U32), ("level_count", _U32 * 11),
                 ("level_offset", _U32 * 12), ("cube", _F * 4), ("d_codes", _P),
@@ -154,6 +166,11 @@ _SIGS = {
     "mrh_tracer_stats": (_I32, [_P, C.POINTER(RenderStats), C.POINTER(_U32)]),
     "mrh_tracer_destroy": (_I32, [_P]),
     "mrh_tracer_set_guide": (_I32, [_P, C.POINTER(GuideParams), _P]),
+    "mrh_tracer_set_filter": (_I32, [_P, C.POINTER(FilterDesc)]),
+    "mrh_filter_samples": (_I32, [_P, C.POINTER(FilterDesc), _P, _P, _U32, _U32, _U32, _P, _P]),
+    "mrh_filter_resolve": (_I32, [_P, _P, _P, _U32, _U32, _P]),
+    "mrh_render_filtered": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams), C.POINTER(FilterDesc),
+                                   _P, C.POINTER(RenderStats)]),
     "mrh_svo_create": (_I32, [_P, C.POINTER(_P)]),
     "mrh_svo_destroy": (_I32, [_P]),
     "mrh_svo_codes": (_I32, [_P, _P, _U32, _U32, _P, _U32, _P]),
@@ -274,6 +291,15 @@ class Context:
         n = C.c_uint64(0)
         self.check(lib().mrh_profile_get(self.h, kernel.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def filter_samples(self, filt: FilterDesc, d_values, d_coords, n: int, width: int, height: int,
+                       d_weighted, d_weights) -> None:
+        """mrh_filter_samples: accumulate n samples into the filter sums."""
+        self.check(lib().mrh_filter_samples(self.h, C.byref(filt), ptr(d_values), ptr(d_coords), n, width, height,
+                                            ptr(d_weighted), ptr(d_weights)))
+
+    def filter_resolve(self, d_weighted, d_weights, width: int, height: int, d_image) -> None:
+        self.check(lib().mrh_filter_resolve(self.h, ptr(d_weighted), ptr(d_weights), width, height, ptr(d_image)))
 
 
 class Lbvh:
@@ -423,6 +449,16 @@ class Scene:
         st = RenderStats()
         self.ctx.check(lib().mrh_render(self.ctx.h, self.h, C.byref(cam), C.byref(p),
                                         ptr(d_image), C.byref(st)))
+        return st
+
+    def render_filtered(self, d_image, width, height, samples, max_depth, rr_start, seed=0, filt=None,
+                        camera=None, nee=0, mis=0) -> RenderStats:
+        """One frame through a reconstruction filter (mrh_render_filtered)."""
+        cam = camera_desc(camera or self.camera)
+        p = RenderParams(width, height, samples, max_depth, rr_start, nee, seed, 0, mis)
+        st = RenderStats()
+        self.ctx.check(lib().mrh_render_filtered(self.ctx.h, self.h, C.byref(cam), C.byref(p), C.byref(filt),
+                                                 ptr(d_image), C.byref(st)))
         return st
 
     def render_guided(self, d_image, width, height, samples, max_depth, rr_start, seed=0,

@@ -240,7 +240,7 @@ void om_camera_setup(const float pos[3], const float gaze[3], const float up[3],
  * pinhole sample.  Sample ids run pixel-major: sample_id =
  * (y*W + x)*S^2 + (sy*S + sx). */
 static void camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t S,
-                       uint32_t sample_id, om_rng* rng, om_v3* o, om_v3* d)
+                       uint32_t sample_id, om_rng* rng, om_v3* o, om_v3* d, float* coord)
 {
     uint32_t spp = S * S;
     uint32_t pix = sample_id / spp;
@@ -262,6 +262,17 @@ static void camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t S,
     om_v3 sp = v3_add(bl, v3_add(v3_scale(r, sdx), v3_scale(u, sdy)));
     *d = v3_normalize(v3_sub(sp, pos));
     *o = pos;
+    if (coord) {
+        /* GPUCameraPinhole.cuh:236-237 + GenerationKernels.cuh:77-78: image
+         * coordinate (stratum + jitter) / strata * resolution, kept inside
+         * its own pixel */
+        float cx = ((float)(px * S + sx) + jx) / (float)(w * S) * (float)w;
+        float cy = ((float)(py * S + sy) + jy) / (float)(h * S) * (float)h;
+        cx = fminf(fmaxf(cx, (float)px), nextafterf((float)(px + 1), 0.0f));
+        cy = fminf(fmaxf(cy, (float)py), nextafterf((float)(py + 1), 0.0f));
+        coord[0] = cx;
+        coord[1] = cy;
+    }
 }
 
 void om_camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t samples,
@@ -271,7 +282,7 @@ void om_camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t sample
     om_rng rng;
     om_path_rng(&rng, seed, sample_id);
     om_v3 oo, dd;
-    camera_ray(cam, w, h, samples, sample_id, &rng, &oo, &dd);
+    camera_ray(cam, w, h, samples, sample_id, &rng, &oo, &dd, NULL);
     o[0] = oo.x; o[1] = oo.y; o[2] = oo.z;
     d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
     *rng_state = rng.state;
@@ -604,6 +615,8 @@ typedef struct {
     uint32_t sample_base;
     const om_guide_frame* g;
     om_rec_list rec;
+    float* values;     /* per-sample radiance (om_render_samples) or NULL */
+    float* coords;
 } om_job;
 
 /* One path, depth first.  With g->svo set, non-specular Lambert / metal
@@ -611,14 +624,29 @@ typedef struct {
  * (WFPGTracerKC.cuh:1480-1553: pdf = r p_guide + (1 - r) p_bxdf); with
  * g->learn, every continuing vertex leaves a record (cell, direction bin,
  * throughput) whose radiance the emitter hit fills in (PathNode.cuh:152-190). */
+/* pixel (FP64) and, when set, the sample's own FP32 sum take contribution c */
+static void contribute(double* pixel, float* sacc, om_v3 c)
+{
+    if (pixel) {
+        pixel[0] += (double)c.x;
+        pixel[1] += (double)c.y;
+        pixel[2] += (double)c.z;
+    }
+    if (sacc) {
+        sacc[0] += c.x;
+        sacc[1] += c.y;
+        sacc[2] += c.z;
+    }
+}
+
 static void trace_path(const om_scene* sc, const om_camera* cam, const om_render_params* p,
                        uint32_t sample_base, uint32_t sample_id, const om_guide_frame* g,
-                       double* pixel, uint64_t* segments, om_rec_list* recs)
+                       double* pixel, uint64_t* segments, om_rec_list* recs, float* sacc, float* coord)
 {
     om_rng rng;
     om_path_rng(&rng, p->seed, sample_base + sample_id);
     om_v3 o, d;
-    camera_ray(cam, p->width, p->height, p->samples, sample_id, &rng, &o, &d);
+    camera_ray(cam, p->width, p->height, p->samples, sample_id, &rng, &o, &d, coord);
     om_v3 T = v3(1.0f, 1.0f, 1.0f);
     uint32_t depth = 1, medium = 0;
     float tmin = cam->near_far[0], tmax = cam->near_far[1];
@@ -656,9 +684,7 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
                 float pm = 1.0f / (float)sc->n_lights;
                 c = v3_scale(c, power_heuristic(prev_pdf, pc * pm));
             }
-            pixel[0] += (double)c.x;
-            pixel[1] += (double)c.y;
-            pixel[2] += (double)c.z;
+            contribute(pixel, sacc, c);
             if (learn) {
                 om_v3 c = v3_mul(v3(le[0], le[1], le[2]), T);
                 for (uint32_t k = 0; k < nrec; k++) {
@@ -702,9 +728,7 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
                 uint32_t hp = closest_hit(sc, nudge(s.pos, s.geo_n), ldir, 0.0f, &lt, lb);
                 if (hp == lprim) {
                     const float* le = sc->emissive + (size_t)(sc->prim_key[hp] & 0xFFFFFFu) * 3;
-                    pixel[0] += (double)(le[0] * nf.x);
-                    pixel[1] += (double)(le[1] * nf.y);
-                    pixel[2] += (double)(le[2] * nf.z);
+                    contribute(pixel, sacc, v3(le[0] * nf.x, le[1] * nf.y, le[2] * nf.z));
                 }
             }
         }
@@ -808,18 +832,22 @@ static void* render_worker(void* arg)
         if (row_index % nthreads != job->tid) continue;
         for (uint32_t x = 0; x < p->width; x++) {
             uint32_t pix = y * p->width + x;
-            double* px = job->img + (size_t)pix * 3;
-            for (uint32_t s = 0; s < spp; s++)
-                trace_path(job->s, job->cam, p, job->sample_base, pix * spp + s, job->g, px, &job->segments,
-                           &job->rec);
+            double* px = job->img ? job->img + (size_t)pix * 3 : NULL;
+            for (uint32_t s = 0; s < spp; s++) {
+                size_t i = (size_t)pix * spp + s;
+                trace_path(job->s, job->cam, p, job->sample_base, (uint32_t)i, job->g, px, &job->segments,
+                           &job->rec, job->values ? job->values + 3 * i : NULL,
+                           job->coords ? job->coords + 2 * i : NULL);
+            }
         }
     }
     return NULL;
 }
 
 /* one frame over the selected rows; merges the threads' records into *recs */
-static uint64_t render_frame(const om_scene* s, const om_camera* cam, const om_render_params* p,
-                             uint32_t sample_base, const om_guide_frame* g, double* img, om_rec_list* recs)
+static uint64_t render_frame_to(const om_scene* s, const om_camera* cam, const om_render_params* p,
+                                uint32_t sample_base, const om_guide_frame* g, double* img, om_rec_list* recs,
+                                float* values, float* coords)
 {
     uint32_t nthreads = p->threads ? p->threads : 1u;
     om_job* jobs = (om_job*)calloc(nthreads, sizeof(om_job));
@@ -828,6 +856,8 @@ static uint64_t render_frame(const om_scene* s, const om_camera* cam, const om_r
         jobs[t].s = s; jobs[t].cam = cam; jobs[t].p = p; jobs[t].img = img; jobs[t].tid = t;
         jobs[t].sample_base = sample_base;
         jobs[t].g = g;
+        jobs[t].values = values;
+        jobs[t].coords = coords;
         if (nthreads > 1) pthread_create(&th[t], NULL, render_worker, &jobs[t]);
     }
     if (nthreads == 1) render_worker(&jobs[0]);
@@ -845,6 +875,23 @@ static uint64_t render_frame(const om_scene* s, const om_camera* cam, const om_r
     free(jobs);
     free(th);
     return seg;
+}
+
+static uint64_t render_frame(const om_scene* s, const om_camera* cam, const om_render_params* p,
+                             uint32_t sample_base, const om_guide_frame* g, double* img, om_rec_list* recs)
+{
+    return render_frame_to(s, cam, p, sample_base, g, img, recs, NULL, NULL);
+}
+
+uint64_t om_render_samples(const om_scene* s, const om_camera* cam, const om_render_params* p, uint32_t sample_base,
+                           float* values, float* coords)
+{
+    om_render_params q = *p;
+    q.row_step = 1;
+    q.row_offset = 0;
+    size_t n = (size_t)p->width * p->height * p->samples * p->samples;
+    memset(values, 0, n * 3 * sizeof(float));
+    return render_frame_to(s, cam, &q, sample_base, NULL, NULL, NULL, values, coords);
 }
 
 uint64_t om_render(const om_scene* s, const om_camera* cam, const om_render_params* p, double* img)

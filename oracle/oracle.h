@@ -180,11 +180,38 @@ uint32_t om_sample_light(const om_scene* sc, const float p[3], const float xi[3]
 float    om_light_pdf_hit(const float tri[9], float t, const float d[3]);
 float    om_power_heuristic(float pdf0, float pdf1);
 
+/* Reconstruction filters (om_filter.c) */
+#define OM_FILTER_BOX      0u
+#define OM_FILTER_TENT     1u
+#define OM_FILTER_GAUSSIAN 2u
+#define OM_FILTER_MITCHELL 3u
+typedef struct {
+    uint32_t type;
+    float    radius, alpha, b, c;
+} om_filter_desc;
+/* FilterRadiusToPixelWH */
+int   om_filter_pixel_wh(float radius);
+/* filter weight at pixel-centre offset (dx, dy) from the sample */
+float om_filter_weight(const om_filter_desc* d, float dx, float dy);
+/* arbitrary samples (values n x 3, coords n x 2), accumulated (+=) into
+ * weighted (w*h x 3) and weights (w*h) */
+void  om_filter_samples(const om_filter_desc* d, const float* values, const float* coords, uint32_t n,
+                        uint32_t w, uint32_t h, float* weighted, float* weights);
+/* samples grouped per pixel: pixel q owns samples q*spp .. q*spp + spp - 1 */
+void  om_filter_samples_grouped(const om_filter_desc* d, const float* values, const float* coords,
+                                uint32_t spp, uint32_t w, uint32_t h, float* weighted, float* weights);
+
 /* Renders the selected rows; img is double[h*w*3] (zeroed by the caller),
  * accumulating the per-sample FP32 contributions in FP64.  Returns the number
  * of ray segments traced. */
 uint64_t om_render(const om_scene* s, const om_camera* cam, const om_render_params* p,
                    double* img);
+/* All rows; every sample's radiance (FP32, contributions summed in path
+ * order) into values (W*H*spp x 3) and its image coordinate into coords
+ * (W*H*spp x 2), kept inside its own pixel; global sample ids start at
+ * sample_base (a tracer's later frames).  Returns the segments traced. */
+uint64_t om_render_samples(const om_scene* s, const om_camera* cam, const om_render_params* p, uint32_t sample_base,
+                           float* values, float* coords);
 
 /* ---- Path guiding (AnisoSVO.cu:225-263, 657-1188, 1415-1483;
  *      WFPGTracerKC.cuh:202-520, 1480-1766; WFPGTracer.cu:1590-1790) ---- */

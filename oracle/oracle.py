@@ -52,6 +52,17 @@ class OmScene(C.Structure):
                 ("root", _U32), ("nodes4", _P), ("root4", _U32), ("lights", _P), ("n_lights", _U32)]
 
 
+class OmFilterDesc(C.Structure):
+    _fields_ = [("type", _U32), ("radius", _F), ("alpha", _F), ("b", _F), ("c", _F)]
+
+
+FILTERS = {"Box": 0, "Tent": 1, "Gaussian": 2, "Mitchell": 3}
+
+
+def filter_desc(kind: str, radius: float, alpha: float = 0.0, b: float = 0.0, c: float = 0.0) -> OmFilterDesc:
+    return OmFilterDesc(FILTERS[kind], radius, alpha, b, c)
+
+
 class OmRenderParams(C.Structure):
     _fields_ = [("width", _U32), ("height", _U32), ("samples", _U32), ("max_depth", _U32),
                 ("rr_start", _U32), ("seed", _U32), ("row_step", _U32), ("row_offset", _U32),
@@ -160,6 +171,15 @@ def lib():
         L.om_light_pdf_hit.argtypes = [_P, _F, _P]
         L.om_power_heuristic.restype = _F
         L.om_power_heuristic.argtypes = [_F, _F]
+        L.om_filter_pixel_wh.restype = C.c_int
+        L.om_filter_pixel_wh.argtypes = [_F]
+        L.om_filter_weight.restype = _F
+        L.om_filter_weight.argtypes = [C.POINTER(OmFilterDesc), _F, _F]
+        L.om_filter_samples.argtypes = [C.POINTER(OmFilterDesc), _P, _P, _U32, _U32, _U32, _P, _P]
+        L.om_filter_samples_grouped.argtypes = [C.POINTER(OmFilterDesc), _P, _P, _U32, _U32, _U32, _P, _P]
+        L.om_render_samples.restype = C.c_uint64
+        L.om_render_samples.argtypes = [C.POINTER(OmScene), C.POINTER(OmCamera), C.POINTER(OmRenderParams),
+                                        _U32, _P, _P]
         L.om_render_guided.restype = C.c_uint64
         L.om_render_guided.argtypes = [C.POINTER(OmScene), C.POINTER(OmCamera),
                                        C.POINTER(OmRenderParams), C.POINTER(OmGuideParams), _P,
@@ -382,6 +402,60 @@ def render(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_de
     img = np.zeros((height, width, 3), dtype=np.float64)
     seg = lib().om_render(C.byref(sc), C.byref(cam), C.byref(p), _ptr(img))
     return img / float(samples * samples), int(seg)
+
+
+def filter_pixel_wh(radius: float) -> int:
+    return int(lib().om_filter_pixel_wh(float(radius)))
+
+
+def filter_weight(desc: OmFilterDesc, dx: float, dy: float) -> float:
+    return float(lib().om_filter_weight(C.byref(desc), float(dx), float(dy)))
+
+
+def filter_samples(desc: OmFilterDesc, values, coords, w: int, h: int, grouped_spp: int = 0):
+    """Filter samples (values n x 3, coords n x 2) into (weighted h x w x 3,
+    weights h x w); grouped_spp > 0: sample i lies in pixel i // grouped_spp."""
+    v = np.ascontiguousarray(values, np.float32)
+    c = np.ascontiguousarray(coords, np.float32)
+    weighted = np.zeros((h, w, 3), np.float32)
+    weights = np.zeros((h, w), np.float32)
+    if grouped_spp:
+        lib().om_filter_samples_grouped(C.byref(desc), _ptr(v), _ptr(c), grouped_spp, w, h, _ptr(weighted),
+                                        _ptr(weights))
+    else:
+        lib().om_filter_samples(C.byref(desc), _ptr(v), _ptr(c), len(v), w, h, _ptr(weighted), _ptr(weights))
+    return weighted, weights
+
+
+def filter_resolve(weighted, weights):
+    ws = weights[..., None]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        out = np.where(ws == 0, np.float32(0), weighted / np.where(ws == 0, np.float32(1), ws))
+    return out.astype(np.float32)
+
+
+def render_samples(scene: dict, bvh: "LBVH", cam: OmCamera, width, height, samples, max_depth, rr_start, seed,
+                   threads=1, tree_width=4, nee=0, mis=0, sample_base=0):
+    """Every sample's FP32 radiance and image coordinate:
+    (values [n, 3], coords [n, 2], segments), n = width * height * samples^2;
+    global sample ids start at sample_base."""
+    keep = {}
+    sc = _scene_struct(scene, bvh, keep, tree_width)
+    p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, 1, 0, threads, nee, mis)
+    n = width * height * samples * samples
+    values = np.zeros((n, 3), np.float32)
+    coords = np.zeros((n, 2), np.float32)
+    seg = lib().om_render_samples(C.byref(sc), C.byref(cam), C.byref(p), sample_base, _ptr(values), _ptr(coords))
+    return values, coords, int(seg)
+
+
+def render_filtered(scene: dict, bvh: "LBVH", cam: OmCamera, width, height, samples, max_depth, rr_start, seed,
+                    desc: OmFilterDesc, threads=1, tree_width=4, nee=0, mis=0):
+    """Filtered frame: (image float32 [h, w, 3], segments)."""
+    values, coords, seg = render_samples(scene, bvh, cam, width, height, samples, max_depth, rr_start, seed,
+                                         threads, tree_width, nee, mis)
+    weighted, weights = filter_samples(desc, values, coords, width, height, samples * samples)
+    return filter_resolve(weighted, weights), seg
 
 
 def sample_light(scene: dict, bvh: "LBVH", p, xi):

@@ -4,7 +4,8 @@
 // system, hand it a scene, create a "PathTracer", and loop
 // GenerateWork -> while (Render()) -> Finalize (RayLib/TracerThread.cpp:144-160).
 //
-//   tracer_system_host <libmrh.so> <out.bin> <width> <height> <frames>
+//   tracer_system_host <libmrh.so> <out.bin> <width> <height> <frames> [<filtered.bin>]
+// (filtered.bin: one more frame through a Tent filter of radius 1)
 //
 // Writes the final mean image (float RGB) to out.bin and prints one line per
 // check.  Exit status 0 when every check passed.
@@ -45,8 +46,9 @@ void check(bool ok, const char* what)
 
 int main(int argc, char** argv)
 {
-    if (argc != 6) {
-        std::fprintf(stderr, "usage: %s <libmrh.so> <out.bin> <width> <height> <frames>\n", argv[0]);
+    if (argc != 6 && argc != 7) {
+        std::fprintf(stderr, "usage: %s <libmrh.so> <out.bin> <width> <height> <frames> [<filtered.bin>]\n",
+                     argv[0]);
         return 2;
     }
     const uint32_t W = (uint32_t)std::atoi(argv[3]), H = (uint32_t)std::atoi(argv[4]);
@@ -124,6 +126,41 @@ int main(int argc, char** argv)
             std::fclose(out);
         }
         tracer.reset();
+
+        // reconstruction filter (TracerSystemI::GenerateReconFilter +
+        // GPUTracerI::AttachReconFilter)
+        mrh::GPUReconFilterI* filter = nullptr;
+        mrh::Options fo;
+        fo.Set("radius", 1.0);
+        check(sys->GenerateReconFilter(filter, fo) != mrh::TracerError::OK && !filter,
+              "a filter without \"type\" is refused");
+        fo.Set("type", std::string("Gaussian"));
+        check(sys->GenerateReconFilter(filter, fo) != mrh::TracerError::OK, "Gaussian without alpha is refused");
+        fo.Set("type", std::string("Lanczos"));
+        check(sys->GenerateReconFilter(filter, fo) != mrh::TracerError::OK, "unknown filter type is refused");
+        fo.Set("type", std::string("Tent"));
+        check(sys->GenerateReconFilter(filter, fo) == mrh::TracerError::OK && filter &&
+                  std::string(filter->Type()) == "Tent",
+              "GenerateReconFilter(Tent, radius 1)");
+        if (argc == 7 && filter) {
+            check(sys->GenerateTracer(tracer, params, opts, "PathTracer") == mrh::TracerError::OK && tracer,
+                  "GenerateTracer for the filtered frame");
+            Callbacks fcb;
+            tracer->AttachTracerCallbacks(fcb);
+            tracer->AttachReconFilter(filter);
+            tracer->ResizeImage(W, H);
+            check(tracer->Initialize() == mrh::TracerError::OK, "Initialize with a filter attached");
+            tracer->GenerateWork(0u);
+            while (tracer->Render()) {}
+            tracer->Finalize();
+            check(fcb.errors == 0 && fcb.image.size() == (size_t)W * H * 3, "filtered frame sent");
+            FILE* fout = std::fopen(argv[6], "wb");
+            if (fout) {
+                std::fwrite(fcb.image.data(), sizeof(float), fcb.image.size(), fout);
+                std::fclose(fout);
+            }
+            tracer.reset();
+        }
     }
     dlclose(lib);
     std::printf("%s\n", fails ? "FAILED" : "ALL OK");

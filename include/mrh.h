@@ -289,6 +289,14 @@ int32_t mrh_filter_samples(mrh_context* ctx, const mrh_filter_desc* filter, cons
 int32_t mrh_filter_resolve(mrh_context* ctx, const float* d_weighted, const float* d_weights,
                            uint32_t width, uint32_t height, float* d_image);
 
+/* Image files (host only; no device needed).  ImageIO::WriteImage with
+ * ImageType::EXR (ImageIO/ImageIOI.h:83-91) writes an OpenEXR 2.0 scanline
+ * file, uncompressed, FP32 channels R, G, B; PFM is the portable float
+ * map ("PF", little endian).  rgb: host float[height][width][3], bottom row
+ * first (the layout every image of this library has). */
+typedef enum { MRH_IMAGE_EXR = 0, MRH_IMAGE_PFM = 1 } mrh_image_type;
+int32_t mrh_image_write(const char* path, uint32_t type, uint32_t width, uint32_t height, const float* rgb);
+
 /* mrh_render with a reconstruction filter (RayTracer::Finalize ->
  * FilterToImg, RayTracer.cu:81-95): every sample's radiance and image
  * coordinate are kept and filtered into d_image.  With Box radius 0 the

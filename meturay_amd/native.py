@@ -167,6 +167,7 @@ _SIGS = {
     "mrh_tracer_destroy": (_I32, [_P]),
     "mrh_tracer_set_guide": (_I32, [_P, C.POINTER(GuideParams), _P]),
     "mrh_tracer_set_filter": (_I32, [_P, C.POINTER(FilterDesc)]),
+    "mrh_image_write": (_I32, [C.c_char_p, _U32, _U32, _U32, _P]),
     "mrh_filter_samples": (_I32, [_P, C.POINTER(FilterDesc), _P, _P, _U32, _U32, _U32, _P, _P]),
     "mrh_filter_resolve": (_I32, [_P, _P, _P, _U32, _U32, _P]),
     "mrh_render_filtered": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams), C.POINTER(FilterDesc),
@@ -398,6 +399,23 @@ def gen_random_rays(n: int, seed: int = 1):
     if rc != OK:
         raise MrhError(rc, "mrh_gen_random_rays failed")
     return r
+
+
+IMAGE_TYPES = {"exr": 0, "pfm": 1}
+
+
+def write_image(path: str, rgb, kind: str | None = None) -> None:
+    """Writes a host float32 image [height, width, 3] (bottom row first) as
+    EXR or PFM (mrh_image_write); the kind defaults to the file extension."""
+    import numpy as np
+
+    a = np.ascontiguousarray(rgb, dtype=np.float32)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise ValueError("rgb must be [height, width, 3]")
+    k = (kind or os.path.splitext(path)[1].lstrip(".")).lower()
+    rc = lib().mrh_image_write(os.fsencode(path), IMAGE_TYPES[k], a.shape[1], a.shape[0], a.ctypes.data)
+    if rc != OK:
+        raise MrhError(rc, f"mrh_image_write({path}) failed")
 
 
 def camera_desc(cam: dict) -> CameraDesc:

@@ -185,6 +185,7 @@ int32_t mrh_trace_any(mrh_context* ctx, const mrh_lbvh* bvh, const float* d_rays
 #define MRH_BATCH_METAL      2u
 #define MRH_BATCH_DIELECTRIC 3u
 #define MRH_BATCH_EMISSIVE   4u
+#define MRH_BATCH_REFLECT    5u   /* "Reflect": perfect mirror with albedo */
 #define MRH_KEY(batch, id)   (((uint32_t)(batch) << 24) | ((uint32_t)(id) & 0xFFFFFFu))
 
 typedef struct {
@@ -200,6 +201,8 @@ typedef struct {
     const float*    emissive;    /* host float[n_emissive][3] radiance */
     uint32_t        n_emissive;
     uint32_t        morton_bits; /* 30 or 63 */
+    const float*    reflect;     /* host float[n_reflect][4]: albedo rgb, roughness (mirror at 0) */
+    uint32_t        n_reflect;
 } mrh_scene_desc;
 
 typedef struct {

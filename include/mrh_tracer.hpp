@@ -151,6 +151,7 @@ struct SceneDescription {
     std::vector<float> metal;         // [..][10]
     std::vector<float> dielectric;    // [..][4]
     std::vector<float> emissive;      // [..][3]
+    std::vector<float> reflect;       // [..][4]
     std::vector<mrh_camera_desc> cameras;
     std::string accelerator = "LBVH";
     uint32_t mortonBits = 30;

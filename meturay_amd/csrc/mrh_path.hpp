@@ -18,7 +18,8 @@ struct mrh_scene {
     float* metal = nullptr;
     float* dielectric = nullptr;
     float* emissive = nullptr;
-    uint32_t n_lambert = 0, n_metal = 0, n_dielectric = 0, n_emissive = 0;
+    float* reflect = nullptr;
+    uint32_t n_lambert = 0, n_metal = 0, n_dielectric = 0, n_emissive = 0, n_reflect = 0;
     uint32_t* lights = nullptr;     // leaf index of every emissive triangle, in primitive order
     uint32_t n_lights = 0;
     uint64_t bytes = 0;
@@ -65,6 +66,7 @@ struct Materials {
     const float* metal;
     const float* dielectric;
     const float* emissive;
+    const float* reflect;
 };
 
 // GPUCameraPinhole.cuh:140-170, evaluated on the host with the same float
@@ -328,6 +330,26 @@ __device__ __forceinline__ void dielectric_sample(const Surface& s, V3 wi, const
         out.d = scale(-wi, ratio) + scale(normal, kk);
         out.o = nudge(s.pos, -s.geo_n);
         out.refl = scale(scale(albedo, 1.0f), 1.0f - f);
+    }
+}
+
+// ReflectMatFuncs::Sample (SimpleMaterialsKC.cuh:105-151): a perfect mirror
+// about the shading normal weighted by the albedo at roughness 0; any other
+// roughness reflects nothing (the path ends)
+__device__ __forceinline__ void reflect_sample(const Surface& s, V3 wi, const float* __restrict__ m, BsdfOut& out)
+{
+    const V3 normal = rotate(conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
+    out.specular = true;
+    out.pdf = 1.0f;
+    out.o = s.pos;
+    out.d = wi;
+    if (m[3] == 0.0f) {
+        const float k = 2.0f * dot(wi, normal);
+        out.d = scale(normal, k) - wi;
+        out.o = nudge(s.pos, s.geo_n);
+        out.refl = v3(m[0], m[1], m[2]);
+    } else {
+        out.refl = v3(0.0f, 0.0f, 0.0f);
     }
 }
 
@@ -692,6 +714,8 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
         lambert_sample(s, v3(a[0], a[1], a[2]), rng, o);
     } else if (batch == MRH_BATCH_METAL) {
         metal_sample(s, wi, mats.metal + (size_t)id * 10, rng, o);
+    } else if (batch == MRH_BATCH_REFLECT) {
+        reflect_sample(s, wi, mats.reflect + (size_t)id * 4, o);
     } else {
         dielectric_sample(s, wi, mats.dielectric + (size_t)id * 4, id, medium, mats.dielectric, rng, o);
     }

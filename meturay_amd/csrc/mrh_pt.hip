@@ -274,8 +274,9 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
         uint32_t lim = (b == MRH_BATCH_LAMBERT) ? d->n_lambert
                      : (b == MRH_BATCH_METAL) ? d->n_metal
                      : (b == MRH_BATCH_DIELECTRIC) ? d->n_dielectric
-                     : (b == MRH_BATCH_EMISSIVE) ? d->n_emissive : 0u;
-        if (b == MRH_BATCH_NULL || b > MRH_BATCH_EMISSIVE || id >= lim)
+                     : (b == MRH_BATCH_EMISSIVE) ? d->n_emissive
+                     : (b == MRH_BATCH_REFLECT) ? d->n_reflect : 0u;
+        if (b == MRH_BATCH_NULL || b > MRH_BATCH_REFLECT || id >= lim)
             return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: primitive %u has invalid key 0x%08x", i,
                         d->prim_key[i]);
     }
@@ -292,6 +293,7 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
     s->n_metal = d->n_metal;
     s->n_dielectric = d->n_dielectric;
     s->n_emissive = d->n_emissive;
+    s->n_reflect = d->n_reflect;
     float* dtris = nullptr;
     uint32_t* dkeys = nullptr;
     int32_t rc = MRH_OK;
@@ -300,6 +302,7 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
         if ((rc = scene_upload(s, &s->metal, d->metal, (size_t)d->n_metal * 10)) != MRH_OK) break;
         if ((rc = scene_upload(s, &s->dielectric, d->dielectric, (size_t)d->n_dielectric * 4)) != MRH_OK) break;
         if ((rc = scene_upload(s, &s->emissive, d->emissive, (size_t)d->n_emissive * 3)) != MRH_OK) break;
+        if ((rc = scene_upload(s, &s->reflect, d->reflect, (size_t)d->n_reflect * 4)) != MRH_OK) break;
         if ((rc = scene_upload(s, &dtris, d->tris, (size_t)d->n * 9)) != MRH_OK) break;
         if ((rc = scene_upload(s, &dkeys, d->prim_key, (size_t)d->n)) != MRH_OK) break;
         uint32_t bits = d->morton_bits ? d->morton_bits : 30u;
@@ -411,7 +414,7 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     sa.part_begin = 24;
     MRH_TRY(radix_sort(ctx, sa));
     ShadeReq q;
-    q.mats = Materials{s->lambert, s->metal, s->dielectric, s->emissive};
+    q.mats = Materials{s->lambert, s->metal, s->dielectric, s->emissive, s->reflect};
     q.leaves = lv;
     q.image = image;
     q.spp = f.per_sample ? 1u : (uint32_t)f.spp;

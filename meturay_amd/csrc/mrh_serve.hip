@@ -339,7 +339,7 @@ void fill_slot(mrh_server* s, ServeReq* q)
     d.root = root_code(b);
     d.root4 = b->root4;
     d.n_leaves = b->n;
-    d.q.mats = Materials{sc->lambert, sc->metal, sc->dielectric, sc->emissive};
+    d.q.mats = Materials{sc->lambert, sc->metal, sc->dielectric, sc->emissive, sc->reflect};
     d.q.leaves = (const float4*)b->leaves;
     d.q.image = q->r.d_image;
     d.q.spp = q->S * q->S;

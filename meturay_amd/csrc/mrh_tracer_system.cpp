@@ -334,6 +334,8 @@ public:
         sd.n_dielectric = (uint32_t)(d.dielectric.size() / 4);
         sd.emissive = d.emissive.data();
         sd.n_emissive = (uint32_t)(d.emissive.size() / 3);
+        sd.reflect = d.reflect.data();
+        sd.n_reflect = (uint32_t)(d.reflect.size() / 4);
         sd.morton_bits = d.mortonBits;
         ClearScene();
         mrh_scene* h = nullptr;

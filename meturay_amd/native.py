@@ -116,7 +116,7 @@ class SceneDesc(C.Structure):
     _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
                 ("lambert", _P), ("n_lambert", _U32), ("metal", _P), ("n_metal", _U32),
                 ("dielectric", _P), ("n_dielectric", _U32), ("emissive", _P),
-                ("n_emissive", _U32), ("morton_bits", _U32)]
+                ("n_emissive", _U32), ("morton_bits", _U32), ("reflect", _P), ("n_reflect", _U32)]
 
 
 class SceneBuffers(C.Structure):
@@ -444,7 +444,7 @@ class Scene:
         d.tris = k["tris"].ctypes.data
         d.prim_key = k["prim_key"].ctypes.data
         d.n = k["tris"].shape[0]
-        for name in ("lambert", "metal", "dielectric", "emissive"):
+        for name in ("lambert", "metal", "dielectric", "emissive", "reflect"):
             a = k.get(name)
             setattr(d, name, a.ctypes.data if a is not None and a.size else None)
             setattr(d, "n_" + name, a.shape[0] if a is not None else 0)

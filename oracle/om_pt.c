@@ -436,6 +436,25 @@ static void dielectric_sample(const om_surface* s, om_v3 wi, const float* m, uin
     }
 }
 
+/* SimpleMaterialsKC.cuh:105-151 (ReflectMatFuncs::Sample): mirror about the
+ * shading normal with the albedo at roughness 0, nothing otherwise */
+static void reflect_sample(const om_surface* s, om_v3 wi, const float* m, om_bsdf_out* out)
+{
+    om_v3 normal = q_rotate(q_conj(s->w2t), v3(0.0f, 0.0f, 1.0f));
+    out->specular = 1;
+    out->pdf = 1.0f;
+    out->o = s->pos;
+    out->d = wi;
+    if (m[3] == 0.0f) {
+        float k = 2.0f * v3_dot(wi, normal);
+        out->d = v3_sub(v3_scale(normal, k), wi);
+        out->o = nudge(s->pos, s->geo_n);
+        out->refl = v3(m[0], m[1], m[2]);
+    } else {
+        out->refl = v3(0.0f, 0.0f, 0.0f);
+    }
+}
+
 /* Lambert f*cos and its sampler's own pdf (uniform hemisphere, 1/(2 pi) on
  * the normal's side); SimpleMaterialsKC.cuh:80-101 for the evaluation. */
 static om_v3 lambert_eval(const om_surface* s, om_v3 albedo, om_v3 dir, float* pdf)
@@ -784,6 +803,8 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
             lambert_sample(&s, v3(a[0], a[1], a[2]), &rng, &out);
         } else if (batch == OM_BATCH_METAL) {
             metal_sample(&s, wi, sc->metal + (size_t)id * 10, &rng, &out);
+        } else if (batch == OM_BATCH_REFLECT) {
+            reflect_sample(&s, wi, sc->reflect + (size_t)id * 4, &out);
         } else {
             dielectric_sample(&s, wi, sc->dielectric + (size_t)id * 4, id, medium, sc->dielectric,
                               &rng, &out);

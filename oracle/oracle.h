@@ -118,6 +118,7 @@ void  om_vndf_sample(const float v[3], float alpha, float xi0, float xi1, float 
 #define OM_BATCH_METAL      2u
 #define OM_BATCH_DIELECTRIC 3u
 #define OM_BATCH_EMISSIVE   4u
+#define OM_BATCH_REFLECT    5u
 
 typedef struct {
     const float*    tris;          /* n x 9 */
@@ -140,6 +141,7 @@ typedef struct {
      * order (one primitive-backed light per triangle) */
     const uint32_t* lights;
     uint32_t        n_lights;
+    const float*    reflect;       /* per id: albedo rgb, roughness (4) */
 } om_scene;
 
 typedef struct {

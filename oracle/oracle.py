@@ -49,7 +49,8 @@ class OmScene(C.Structure):
     _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
                 ("lambert", _P), ("metal", _P), ("dielectric", _P), ("emissive", _P),
                 ("sorted_idx", _P), ("left", _P), ("right", _P), ("node_aabb", _P),
-                ("root", _U32), ("nodes4", _P), ("root4", _U32), ("lights", _P), ("n_lights", _U32)]
+                ("root", _U32), ("nodes4", _P), ("root4", _U32), ("lights", _P), ("n_lights", _U32),
+                ("reflect", _P)]
 
 
 class OmFilterDesc(C.Structure):
@@ -373,6 +374,7 @@ def _scene_struct(scene: dict, bvh: LBVH, keep: dict, width: int = 4) -> OmScene
     sc.metal = arr("metal", np.float32, 10)
     sc.dielectric = arr("dielectric", np.float32, 4)
     sc.emissive = arr("emissive", np.float32, 3)
+    sc.reflect = arr("reflect", np.float32, 4)
     sc.sorted_idx = _ptr(bvh.sorted_idx)
     sc.left = _ptr(bvh.left)
     sc.right = _ptr(bvh.right)

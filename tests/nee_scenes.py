@@ -15,13 +15,15 @@ def quad(cx, cy, z, hx, hy):
     return [sum((p[0], p[1], p[2]), ()), sum((p[0], p[2], p[3]), ())]
 
 
-def square_lights_scene(lights, rho=0.8, floor_half=3.0, mirror=False):
+def square_lights_scene(lights, rho=0.8, floor_half=3.0, mirror=False, reflect=None):
     """Lambert floor (albedo rho) at z = 0 under square lights given as
     (height, half size, radiance); a narrow camera halfway up to the lowest
     light looks straight down at the floor's centre.  With mirror the floor
-    is a smooth conductor (roughness 0) instead."""
+    is a smooth conductor (roughness 0) instead; with reflect = (albedo rgb,
+    roughness) a "Reflect" mirror."""
     tris = quad(0.0, 0.0, 0.0, floor_half, floor_half)
-    keys = [((2 if mirror else 1) << 24) | 0] * 2
+    batch = 5 if reflect is not None else (2 if mirror else 1)
+    keys = [(batch << 24) | 0] * 2
     em = []
     for k, (z, half, le) in enumerate(lights):
         tris += quad(0.0, 0.0, z, half, half)
@@ -35,6 +37,8 @@ def square_lights_scene(lights, rho=0.8, floor_half=3.0, mirror=False):
         "metal": np.array([[0.0, 0.2, 0.4, 1.4, 3.5, 2.4, 1.8, 1.0, 1.0, 1.0]], np.float32),
         "dielectric": np.zeros((0, 4), np.float32),
         "emissive": np.array(em, np.float32),
+        "reflect": np.array([list(reflect[0]) + [reflect[1]]] if reflect is not None else np.zeros((0, 4)),
+                            np.float32),
         "camera": {"position": [0.0, 0.0, zc], "gaze": [0.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0],
                    "fov_x": 0.02, "near": 1e-3, "far": 100.0},
     }
@@ -64,14 +68,19 @@ def furnace(light_radiance=1.0):
     return sc
 
 
-def lit_spheres():
+def lit_spheres(reflect=False):
     """4x4 sphere grid (Lambert, GGX conductors of several roughnesses,
-    dielectrics) with spheres 0 and 5 turned into lights."""
+    dielectrics) with spheres 0 and 5 turned into lights; with reflect,
+    spheres 2 and 9 become "Reflect" mirrors (one smooth, one rough)."""
     sc = N.gen_sphere_grid(4, 8, 8, 5)
     keys = sc["prim_key"].copy()
     per = len(keys) // 16
     keys[0:per] = (4 << 24) | 0
     keys[5 * per:6 * per] = (4 << 24) | 1
-    sc["prim_key"] = keys
     sc["emissive"] = np.array([[4.0, 4.0, 4.0], [1.0, 2.0, 3.0]], np.float32)
+    if reflect:
+        keys[2 * per:3 * per] = (5 << 24) | 0
+        keys[9 * per:10 * per] = (5 << 24) | 1
+        sc["reflect"] = np.array([[0.9, 0.7, 0.5, 0.0], [1.0, 1.0, 1.0, 0.3]], np.float32)
+    sc["prim_key"] = keys
     return sc

@@ -31,6 +31,7 @@ SCENES = {
     "furnace": (lambda: S.furnace(1.0), 32, 32, 2, 8, 1000),
     "hidden_light": (lambda: S.square_lights_scene([(1.0, 0.5, 2.0), (2.0, 0.5, 5.0)]), 32, 32, 2, 2, 10),
     "mirror": (lambda: S.square_lights_scene([(1.0, 0.5, 2.0)], mirror=True), 16, 16, 2, 4, 10),
+    "reflect_spheres": (lambda: S.lit_spheres(reflect=True), 96, 64, 2, 10, 3),
 }
 
 
@@ -58,6 +59,17 @@ def test_render_nee_matches_oracle(ctx, oracle, name, mis):
     assert ref.mean() > 0.01
     assert relmse(g, ref) <= 1e-4
     assert np.abs(g - ref).max() <= 1e-4 * max(1.0, ref.max())
+
+
+def test_render_reflect_without_nee(ctx, oracle):
+    g, st, ref, seg = _pair(ctx, oracle, S.lit_spheres(reflect=True), 96, 64, 2, 10, 3, 5, 0, 0)
+    assert st.segments == seg
+    assert relmse(g, ref) <= 1e-4
+    a, le = (0.9, 0.7, 0.5), 2.0
+    g, st, ref, seg = _pair(ctx, oracle, S.square_lights_scene([(1.0, 0.5, le)], reflect=(a, 0.0)), 16, 16, 2, 4,
+                            10, 3, 0, 0)
+    want = np.array([np.float32(le) * np.float32(x) for x in a], np.float32)
+    assert np.array_equal(g, np.broadcast_to(want, g.shape)) and st.segments == seg == 16 * 16 * 4 * 2
 
 
 def test_nee_adds_next_event_rays(ctx, oracle):

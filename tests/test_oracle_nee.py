@@ -172,6 +172,22 @@ def test_specular_vertices_send_no_nee(oracle):
         assert np.array_equal(img, base), mode
 
 
+def test_reflect_mirror_closed_form(oracle):
+    # "Reflect" floor (albedo a, roughness 0) under a light of radiance Le:
+    # every camera ray bounces once, straight into the light: a * Le, exactly
+    a, le = (0.9, 0.7, 0.5), 2.0
+    sc = S.square_lights_scene([(1.0, 0.5, le)], reflect=(a, 0.0))
+    res = render_modes(oracle, sc, 16, 16, 2, 4, 10, 3, MODES)
+    want = np.array([np.float32(le) * np.float32(x) for x in a])
+    for mode, (img, seg) in res.items():
+        assert np.array_equal(img, np.broadcast_to(want, img.shape).astype(np.float64)), mode
+        assert seg == 16 * 16 * 4 * 2
+    # a rough "Reflect" reflects nothing (the reference returns zero for it)
+    rough = S.square_lights_scene([(1.0, 0.5, le)], reflect=(a, 0.25))
+    img, seg = render_modes(oracle, rough, 8, 8, 1, 4, 10, 3, [(0, 0)])[(0, 0)]
+    assert img.max() == 0.0 and seg == 64
+
+
 def test_furnace_with_nee(oracle):
     sc = S.furnace(1.0)
     res = render_modes(oracle, sc, 24, 24, 8, 8, 1000, 9, [(1, 0), (1, 1)])
@@ -187,7 +203,7 @@ def test_modes_agree(oracle, scene):
     if scene == "cornell":
         sc, w, h, spp, depth, rr = N.gen_cornell(2), 32, 32, 4, 6, 3
     else:
-        sc, w, h, spp, depth, rr = S.lit_spheres(), 48, 32, 4, 8, 3
+        sc, w, h, spp, depth, rr = S.lit_spheres(reflect=True), 48, 32, 4, 8, 3
     res = render_modes(oracle, sc, w, h, spp, depth, rr, 4, MODES)
     level = res[(0, 0)][0].mean()
     assert level > 0.01

@@ -292,6 +292,32 @@ int32_t mrh_filter_samples(mrh_context* ctx, const mrh_filter_desc* filter, cons
 int32_t mrh_filter_resolve(mrh_context* ctx, const float* d_weighted, const float* d_weights,
                            uint32_t width, uint32_t height, float* d_image);
 
+/* Scene files (host only; no device needed).  Reads the reference's JSON
+ * scene format (GPUSceneJson.cpp:690-885: Cameras, Lights, Mediums,
+ * Transforms, Materials, Primitives, Surfaces, LightSurfaces, CameraSurfaces;
+ * // and slash-star comments allowed) with Wavefront OBJ meshes (paths relative
+ * to the scene file; innerIndex = mesh of the file, meshes split at o / g /
+ * usemtl) into world-space triangles, HitKeys and material tables
+ * (mrh_scene_file_desc, ready for mrh_scene_create) and Pinhole cameras
+ * (CameraSurfaces order).  Supported: Null / Primitive lights, Vacuum /
+ * Homogeneous media, Identity / Single transforms, LambertC or Lambert with
+ * a constant albedo, Reflect, Refract, Metal, inline and OBJ triangles.
+ * Anything else fails with MRH_ERR_UNSUPPORTED (malformed files with
+ * MRH_ERR_INVALID), the reason in msg; with MRH_SCENE_FILE_SUBSTITUTE it is
+ * replaced (a Lambert for other materials, a tessellated sphere for analytic
+ * ones) or skipped instead and listed in mrh_scene_file_warnings. */
+typedef struct mrh_scene_file mrh_scene_file;
+#define MRH_SCENE_FILE_STRICT     0u
+#define MRH_SCENE_FILE_SUBSTITUTE 1u
+int32_t mrh_scene_file_load(const char* path, uint32_t flags, mrh_scene_file** out, char* msg, uint32_t msg_len);
+/* pointers into the file object's arrays (valid until it is destroyed) */
+int32_t mrh_scene_file_desc(const mrh_scene_file* file, mrh_scene_desc* out);
+int32_t mrh_scene_file_cameras(const mrh_scene_file* file, mrh_camera_desc* out, uint32_t max, uint32_t* count);
+const char* mrh_scene_file_warnings(const mrh_scene_file* file);
+int32_t mrh_scene_file_destroy(mrh_scene_file* file);
+/* meshes of an OBJ file and the triangle count of the first `max` */
+int32_t mrh_obj_mesh_counts(const char* path, uint32_t* meshes, uint32_t* triangles, uint32_t max);
+
 /* Image files (host only; no device needed).  ImageIO::WriteImage with
  * ImageType::EXR (ImageIO/ImageIOI.h:83-91) writes an OpenEXR 2.0 scanline
  * file, uncompressed, FP32 channels R, G, B; PFM is the portable float

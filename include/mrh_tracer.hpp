@@ -66,12 +66,14 @@ struct TracerError {
 struct SceneError {
     enum Type {
         OK,
-        NO_LOADER,                // scene file formats are not handled by this library
+        NO_LOADER,                // a scene file format this library does not read
         EMPTY_SCENE,
         INVALID_HIT_KEY,          // a primitive names a missing material / batch
         ACCELERATOR_TYPE_UNKNOWN,
         GPU_OUT_OF_MEMORY,
         INTERNAL_ERROR,
+        SCENE_FILE_ERROR,         // unreadable or malformed scene / mesh file
+        UNSUPPORTED_FEATURE,      // a scene-file feature this library does not render
         END
     };
     SceneError(Type t = OK, std::string extra = {}) : type(t), extra(std::move(extra)) {}
@@ -80,7 +82,8 @@ struct SceneError {
     {
         static const char* const names[] = {"OK", "No scene loader for this format", "Empty scene",
                                             "Invalid hit key", "Unknown accelerator type",
-                                            "GPU out of memory", "Internal error"};
+                                            "GPU out of memory", "Internal error", "Scene file error",
+                                            "Unsupported scene feature"};
         std::string s = (type < END) ? names[type] : "Unknown error";
         if (!extra.empty()) s += ": " + extra;
         return s;
@@ -211,6 +214,8 @@ public:
     virtual TracerError Initialize(int32_t device) = 0;
     virtual void ClearScene() = 0;
     virtual SceneError GenerateScene(GPUSceneI*& scene, const SceneDescription& desc) = 0;
+    // the reference's JSON scene files (mrh_scene_file_load, strict: any
+    // feature it cannot render is an UNSUPPORTED_FEATURE error)
     virtual SceneError GenerateScene(GPUSceneI*& scene, const std::u8string& scenePath) = 0;
     // tracerType: "PathTracer"
     virtual TracerError GenerateTracer(GPUTracerPtr& tracer, const TracerParameters& params,

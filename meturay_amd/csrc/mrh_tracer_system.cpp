@@ -351,7 +351,31 @@ public:
     SceneError GenerateScene(GPUSceneI*& out, const std::u8string& path) override
     {
         out = nullptr;
-        return SceneError(SceneError::NO_LOADER, std::string(path.begin(), path.end()));
+        const std::string p(path.begin(), path.end());
+        const size_t dot = p.find_last_of('.');
+        if (dot == std::string::npos || p.substr(dot) != ".json") return SceneError(SceneError::NO_LOADER, p);
+        char msg[1024];
+        mrh_scene_file* f = nullptr;
+        const int32_t rc = mrh_scene_file_load(p.c_str(), MRH_SCENE_FILE_STRICT, &f, msg, sizeof(msg));
+        if (rc != MRH_OK)
+            return SceneError(rc == MRH_ERR_UNSUPPORTED ? SceneError::UNSUPPORTED_FEATURE : SceneError::SCENE_FILE_ERROR,
+                              msg);
+        mrh_scene_desc sd = {};
+        mrh_scene_file_desc(f, &sd);
+        SceneDescription d;
+        d.triangles.assign(sd.tris, sd.tris + (size_t)sd.n * 9);
+        d.hitKeys.assign(sd.prim_key, sd.prim_key + sd.n);
+        d.lambert.assign(sd.lambert, sd.lambert + (size_t)sd.n_lambert * 3);
+        d.metal.assign(sd.metal, sd.metal + (size_t)sd.n_metal * 10);
+        d.dielectric.assign(sd.dielectric, sd.dielectric + (size_t)sd.n_dielectric * 4);
+        d.emissive.assign(sd.emissive, sd.emissive + (size_t)sd.n_emissive * 3);
+        d.reflect.assign(sd.reflect, sd.reflect + (size_t)sd.n_reflect * 4);
+        uint32_t nc = 0;
+        mrh_scene_file_cameras(f, nullptr, 0, &nc);
+        d.cameras.resize(nc);
+        mrh_scene_file_cameras(f, d.cameras.data(), nc, &nc);
+        mrh_scene_file_destroy(f);
+        return GenerateScene(out, d);
     }
 
     TracerError GenerateReconFilter(GPUReconFilterI*& out, const Options& o) override

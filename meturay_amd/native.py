@@ -168,6 +168,12 @@ _SIGS = {
     "mrh_tracer_set_guide": (_I32, [_P, C.POINTER(GuideParams), _P]),
     "mrh_tracer_set_filter": (_I32, [_P, C.POINTER(FilterDesc)]),
     "mrh_image_write": (_I32, [C.c_char_p, _U32, _U32, _U32, _P]),
+    "mrh_scene_file_load": (_I32, [C.c_char_p, _U32, C.POINTER(_P), C.c_char_p, _U32]),
+    "mrh_scene_file_desc": (_I32, [_P, C.POINTER(SceneDesc)]),
+    "mrh_scene_file_cameras": (_I32, [_P, C.POINTER(CameraDesc), _U32, C.POINTER(_U32)]),
+    "mrh_scene_file_warnings": (C.c_char_p, [_P]),
+    "mrh_scene_file_destroy": (_I32, [_P]),
+    "mrh_obj_mesh_counts": (_I32, [C.c_char_p, C.POINTER(_U32), _P, _U32]),
     "mrh_filter_samples": (_I32, [_P, C.POINTER(FilterDesc), _P, _P, _U32, _U32, _U32, _P, _P]),
     "mrh_filter_resolve": (_I32, [_P, _P, _P, _U32, _U32, _P]),
     "mrh_render_filtered": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams), C.POINTER(FilterDesc),
@@ -399,6 +405,52 @@ def gen_random_rays(n: int, seed: int = 1):
     if rc != OK:
         raise MrhError(rc, "mrh_gen_random_rays failed")
     return r
+
+
+def load_scene_file(path: str, substitute: bool = False) -> dict:
+    """Reads a scene file (mrh_scene_file_load) into the dict that Scene
+    takes: tris, prim_key, material tables, "camera" (the first camera),
+    "cameras" (all, as dicts) and "warnings"."""
+    import math
+
+    import numpy as np
+
+    h = _P()
+    msg = C.create_string_buffer(1024)
+    rc = lib().mrh_scene_file_load(os.fsencode(path), 1 if substitute else 0, C.byref(h), msg, 1024)
+    if rc != OK:
+        raise MrhError(rc, msg.value.decode())
+    try:
+        d = SceneDesc()
+        lib().mrh_scene_file_desc(h, C.byref(d))
+
+        def take(ptr_, rows, cols, dt=np.float32):
+            if rows == 0:
+                return np.zeros((0, cols), dt)
+            a = np.ctypeslib.as_array(C.cast(ptr_, C.POINTER(C.c_float if dt == np.float32 else C.c_uint32)),
+                                      (rows * cols,))
+            return np.array(a, dt).reshape(rows, cols)
+
+        out = {
+            "tris": take(d.tris, d.n, 9),
+            "prim_key": take(d.prim_key, d.n, 1, np.uint32).reshape(-1),
+            "lambert": take(d.lambert, d.n_lambert, 3),
+            "metal": take(d.metal, d.n_metal, 10),
+            "dielectric": take(d.dielectric, d.n_dielectric, 4),
+            "emissive": take(d.emissive, d.n_emissive, 3),
+            "reflect": take(d.reflect, d.n_reflect, 4),
+        }
+        n = _U32()
+        lib().mrh_scene_file_cameras(h, None, 0, C.byref(n))
+        cams = (CameraDesc * max(n.value, 1))()
+        lib().mrh_scene_file_cameras(h, cams, n.value, C.byref(n))
+        out["cameras"] = [{"position": list(c.position), "gaze": list(c.gaze), "up": list(c.up),
+                           "fov_x": c.fov_x, "near": c.near_plane, "far": c.far_plane} for c in cams[:n.value]]
+        out["camera"] = out["cameras"][0] if out["cameras"] else None
+        out["warnings"] = lib().mrh_scene_file_warnings(h).decode()
+    finally:
+        lib().mrh_scene_file_destroy(h)
+    return out
 
 
 IMAGE_TYPES = {"exr": 0, "pfm": 1}

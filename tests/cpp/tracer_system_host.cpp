@@ -4,8 +4,10 @@
 // system, hand it a scene, create a "PathTracer", and loop
 // GenerateWork -> while (Render()) -> Finalize (RayLib/TracerThread.cpp:144-160).
 //
-//   tracer_system_host <libmrh.so> <out.bin> <width> <height> <frames> [<filtered.bin>]
-// (filtered.bin: one more frame through a Tent filter of radius 1)
+//   tracer_system_host <libmrh.so> <out.bin> <width> <height> <frames>
+//                      [<filtered.bin> [<scene.json> <scene.bin>]]
+// (filtered.bin: one more frame through a Tent filter of radius 1;
+//  scene.bin: one frame of scene.json loaded by GenerateScene(path))
 //
 // Writes the final mean image (float RGB) to out.bin and prints one line per
 // check.  Exit status 0 when every check passed.
@@ -46,8 +48,10 @@ void check(bool ok, const char* what)
 
 int main(int argc, char** argv)
 {
-    if (argc != 6 && argc != 7) {
-        std::fprintf(stderr, "usage: %s <libmrh.so> <out.bin> <width> <height> <frames> [<filtered.bin>]\n",
+    if (argc != 6 && argc != 7 && argc != 9) {
+        std::fprintf(stderr,
+                     "usage: %s <libmrh.so> <out.bin> <width> <height> <frames> [<filtered.bin> [<scene.json> "
+                     "<scene.bin>]]\n",
                      argv[0]);
         return 2;
     }
@@ -87,8 +91,11 @@ int main(int argc, char** argv)
         std::unique_ptr<mrh::TracerSystemI, DelFn> sys(gen(), del);
         check(sys->Initialize(0) == mrh::TracerError::OK, "TracerSystemI::Initialize");
         mrh::GPUSceneI* scene = nullptr;
-        check(sys->GenerateScene(scene, std::u8string(u8"scene.json")) == mrh::SceneError::NO_LOADER,
-              "scene files report NO_LOADER");
+        check(sys->GenerateScene(scene, std::u8string(u8"scene.gltf")) == mrh::SceneError::NO_LOADER,
+              "formats other than JSON scenes report NO_LOADER");
+        check(sys->GenerateScene(scene, std::u8string(u8"/nonexistent/scene.json")) ==
+                  mrh::SceneError::SCENE_FILE_ERROR,
+              "a missing scene file reports SCENE_FILE_ERROR");
         check(sys->GenerateScene(scene, desc) == mrh::SceneError::OK && scene, "GenerateScene");
         mrh::TracerParameters params;
         params.seed = 0;
@@ -142,7 +149,7 @@ int main(int argc, char** argv)
         check(sys->GenerateReconFilter(filter, fo) == mrh::TracerError::OK && filter &&
                   std::string(filter->Type()) == "Tent",
               "GenerateReconFilter(Tent, radius 1)");
-        if (argc == 7 && filter) {
+        if (argc >= 7 && filter) {
             check(sys->GenerateTracer(tracer, params, opts, "PathTracer") == mrh::TracerError::OK && tracer,
                   "GenerateTracer for the filtered frame");
             Callbacks fcb;
@@ -160,6 +167,32 @@ int main(int argc, char** argv)
                 std::fclose(fout);
             }
             tracer.reset();
+        }
+
+        // a scene file through GenerateScene(path)
+        if (argc == 9) {
+            mrh::GPUSceneI* js = nullptr;
+            const std::string sp = argv[7];
+            const mrh::SceneError se = sys->GenerateScene(js, std::u8string(sp.begin(), sp.end()));
+            check(se == mrh::SceneError::OK && js && js->CameraCount() > 0, "GenerateScene(scene.json)");
+            if (js) {
+                check(sys->GenerateTracer(tracer, params, opts, "PathTracer") == mrh::TracerError::OK && tracer,
+                      "GenerateTracer for the scene file");
+                Callbacks scb;
+                tracer->AttachTracerCallbacks(scb);
+                tracer->ResizeImage(W, H);
+                check(tracer->Initialize() == mrh::TracerError::OK, "Initialize on the scene file");
+                tracer->GenerateWork(0u);
+                while (tracer->Render()) {}
+                tracer->Finalize();
+                check(scb.errors == 0 && scb.image.size() == (size_t)W * H * 3, "scene-file frame sent");
+                FILE* sout = std::fopen(argv[8], "wb");
+                if (sout) {
+                    std::fwrite(scb.image.data(), sizeof(float), scb.image.size(), sout);
+                    std::fclose(sout);
+                }
+                tracer.reset();
+            }
         }
     }
     dlclose(lib);

@@ -70,6 +70,10 @@ int32_t     mrh_profile_get(mrh_context* ctx, const char* kernel, double* total_
  *   "trace_blocks_per_cu" persistent traversal grid, blocks of 256 per CU
  *   "trace_width"         traversal tree: 2 = the binary LBVH, 4 = the 4-wide
  *                         tree collapsed from it (trees deeper than 84 use 2)
+ *   "shade_partition"     material partition of a bounce: 0 = one global
+ *                         radix pass over the HitKey batch byte (the rays are
+ *                         shaded through the sorted order), 1 = each block
+ *                         partitions its own tile of 1024 rays in LDS
  * mrh_tune_get reads a knob back. */
 int32_t     mrh_tune(mrh_context* ctx, const char* knob, int64_t value);
 int32_t     mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value);

@@ -208,6 +208,11 @@ int32_t mrh_tune(mrh_context* ctx, const char* knob, int64_t value)
         ctx->trace_width = (uint32_t)value;
         return MRH_OK;
     }
+    if (k == "shade_partition") {
+        if (value != 0 && value != 1) return fail(ctx, MRH_ERR_INVALID, "shade_partition must be 0 or 1");
+        ctx->shade_partition = (uint32_t)value;
+        return MRH_OK;
+    }
     return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
 }
 
@@ -218,6 +223,7 @@ int32_t mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value)
     if (k == "trace_lds_entries") *value = ctx->trace_lds_entries;
     else if (k == "trace_blocks_per_cu") *value = ctx->trace_blocks_per_cu;
     else if (k == "trace_width") *value = ctx->trace_width;
+    else if (k == "shade_partition") *value = ctx->shade_partition;
     else return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
     return MRH_OK;
 }

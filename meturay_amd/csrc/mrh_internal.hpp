@@ -56,6 +56,8 @@ struct mrh_context {
     uint32_t trace_blocks_per_cu = 8;
     // traversal tree: 2 = binary LBVH, 4 = 4-wide tree collapsed from it
     uint32_t trace_width = 4;
+    // material partition of a bounce: 0 = global radix pass, 1 = per-block tiles
+    uint32_t shade_partition = 1;
     // octree reused by guided renders that do not bring their own
     struct mrh_svo* guide_svo = nullptr;
 };

@@ -774,5 +774,43 @@ __device__ __forceinline__ void append_ray(bool alive, const RayNext& nx, uint32
     }
 }
 
+// append_ray with one atomic per block of 256 instead of one per wave: the
+// four waves' survivor counts are scanned in LDS (s_wave: 4 words) and one
+// lane claims the block's range.  All rays of a bounce append to one
+// counter, so this cuts that address's atomics fourfold (a wave-level
+// append left the shade step waiting on them).  Every thread of the block
+// must call it.
+__device__ __forceinline__ void append_ray_block(bool alive, const RayNext& nx, uint32_t sample, PathBuf& out,
+                                                 uint32_t* ctr, bool with_pdf, uint32_t* s_wave,
+                                                 uint32_t* slot_out = nullptr, uint32_t slot = 0u)
+{
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(alive);
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t c0 = s_wave[0], c1 = s_wave[1], c2 = s_wave[2], c3 = s_wave[3];
+        const uint32_t total = c0 + c1 + c2 + c3;
+        const uint32_t b = total ? atomicAdd(ctr, total) : 0u;
+        s_wave[0] = b;
+        s_wave[1] = b + c0;
+        s_wave[2] = b + c0 + c1;
+        s_wave[3] = b + c0 + c1 + c2;
+    }
+    __syncthreads();
+    if (alive) {
+        const uint32_t k = s_wave[wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        out.ox[k] = nx.o.x; out.oy[k] = nx.o.y; out.oz[k] = nx.o.z;
+        out.dx[k] = nx.d.x; out.dy[k] = nx.d.y; out.dz[k] = nx.d.z;
+        out.tr[k] = nx.T.x; out.tg[k] = nx.T.y; out.tb[k] = nx.T.z;
+        out.sample[k] = sample;
+        out.dm[k] = nx.dm;
+        out.rng[k] = nx.rng;
+        if (with_pdf) out.pdf[k] = nx.pdf;
+        if (slot_out) slot_out[k] = slot;
+    }
+    __syncthreads();
+}
+
 }  // namespace
 }  // namespace mrh

@@ -141,6 +141,7 @@ template <bool LEARN, bool GUIDE, bool NEE>
 __global__ __launch_bounds__(256) void k_shade(PathBuf in, PathBuf out, HitBuf hit, const uint32_t* __restrict__ order,
                                                uint32_t* counts, uint32_t bounce, ShadeReq q, GuideArgs ga)
 {
+    __shared__ uint32_t s_wave[4];
     const uint32_t n = counts[bounce];
     const bool with_pdf = NEE && q.mis;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -153,11 +154,69 @@ __global__ __launch_bounds__(256) void k_shade(PathBuf in, PathBuf out, HitBuf h
             sample = in.sample[rid];
             alive = shade_ray<LEARN, GUIDE, NEE>(in, hit, rid, q, ga, true, true, nx, ne, has_nee);
         }
-        if (NEE) append_ray(has_nee, ne, sample, out, &counts[bounce + 1], nullptr, 0u, false);
-        append_ray(alive, nx, sample, out, &counts[bounce + 1], nullptr, 0u, with_pdf);
+        if (NEE) append_ray_block(has_nee, ne, sample, out, &counts[bounce + 1], false, s_wave);
+        append_ray_block(alive, nx, sample, out, &counts[bounce + 1], with_pdf, s_wave);
     }
 }
 
+
+// The same shade step with the partition done per block: a block takes a
+// tile of SHADE_TILE consecutive rays (coalesced key loads), counts them per
+// HitKey batch in LDS, lays the tile's ray indices out batch by batch and
+// shades them in that order, so a wave nearly always runs one material while
+// all ray-state lines of the tile are fetched by the one block, once.  No
+// global sort, no second pass over the keys.
+constexpr uint32_t SHADE_TILE = 1024;
+
+template <bool LEARN, bool GUIDE, bool NEE>
+__global__ __launch_bounds__(256) void k_shade_tiles(PathBuf in, PathBuf out, HitBuf hit, uint32_t* counts,
+                                                     uint32_t bounce, ShadeReq q, GuideArgs ga)
+{
+    __shared__ uint16_t s_perm[SHADE_TILE];
+    __shared__ uint32_t s_cnt[8], s_off[8], s_wave[4];
+    const uint32_t n = counts[bounce];
+    const bool with_pdf = NEE && q.mis;
+    for (uint32_t base = blockIdx.x * SHADE_TILE; base < n; base += gridDim.x * SHADE_TILE) {
+        if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t b[SHADE_TILE / 256];
+#pragma unroll
+        for (uint32_t k = 0; k < SHADE_TILE / 256; k++) {
+            const uint32_t j = base + threadIdx.x + 256 * k;
+            b[k] = (j < n) ? min(hit.key[j] >> 24, 7u) : 7u;
+            atomicAdd(&s_cnt[b[k]], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            // batches in descending order, as the global partition orders them
+            uint32_t acc = 0;
+            for (int c = 7; c >= 0; c--) {
+                s_off[c] = acc;
+                acc += s_cnt[c];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < SHADE_TILE / 256; k++) {
+            const uint32_t pos = atomicAdd(&s_off[b[k]], 1u);
+            s_perm[pos] = (uint16_t)(threadIdx.x + 256 * k);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (uint32_t k = 0; k < SHADE_TILE / 256; k++) {
+            const uint32_t rid = base + s_perm[threadIdx.x + 256 * k];
+            bool alive = false, has_nee = false;
+            RayNext nx, ne;
+            uint32_t sample = 0;
+            if (rid < n) {
+                sample = in.sample[rid];
+                alive = shade_ray<LEARN, GUIDE, NEE>(in, hit, rid, q, ga, true, true, nx, ne, has_nee);
+            }
+            if (NEE) append_ray_block(has_nee, ne, sample, out, &counts[bounce + 1], false, s_wave);
+            append_ray_block(alive, nx, sample, out, &counts[bounce + 1], with_pdf, s_wave);
+        }
+    }
+}
 
 __global__ void k_scale(float* __restrict__ p, size_t count, float s)
 {
@@ -401,18 +460,21 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     } else {
         MRH_TRY(launch_trace_pt<false>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit));
     }
-    SortArgs sa;
-    sa.keys_in = f.hit.key;
-    sa.vals_out = f.order;
-    sa.n_max = (uint32_t)f.cap;
-    sa.n_dev = f.counts + bounce;
-    sa.key_bytes = 4;
-    sa.begin_bit = 24;
-    sa.end_bit = 32;
-    sa.descending = true;
-    sa.part_offsets = f.offsets;
-    sa.part_begin = 24;
-    MRH_TRY(radix_sort(ctx, sa));
+    const bool tiles = ctx->shade_partition == 1;
+    if (!tiles) {
+        SortArgs sa;
+        sa.keys_in = f.hit.key;
+        sa.vals_out = f.order;
+        sa.n_max = (uint32_t)f.cap;
+        sa.n_dev = f.counts + bounce;
+        sa.key_bytes = 4;
+        sa.begin_bit = 24;
+        sa.end_bit = 32;
+        sa.descending = true;
+        sa.part_offsets = f.offsets;
+        sa.part_begin = 24;
+        MRH_TRY(radix_sort(ctx, sa));
+    }
     ShadeReq q;
     q.mats = Materials{s->lambert, s->metal, s->dielectric, s->emissive, s->reflect};
     q.leaves = lv;
@@ -429,7 +491,17 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
         return launch(ctx, "shade", dim3(grid), dim3(256), 0, kernel, in, outb, f.hit, (const uint32_t*)f.order,
                       f.counts, bounce, q, f.ga);
     };
-    if (f.learn && f.guide) MRH_TRY(shade(k_shade<true, true, false>));
+    const uint32_t tgrid_s = stride_grid(ctx, (f.cap + SHADE_TILE - 1) / SHADE_TILE * 256, 256);
+    auto shade_t = [&](auto kernel) {
+        return launch(ctx, "shade", dim3(tgrid_s), dim3(256), 0, kernel, in, outb, f.hit, f.counts, bounce, q, f.ga);
+    };
+    if (tiles) {
+        if (f.learn && f.guide) MRH_TRY(shade_t(k_shade_tiles<true, true, false>));
+        else if (f.learn) MRH_TRY(shade_t(k_shade_tiles<true, false, false>));
+        else if (f.guide) MRH_TRY(shade_t(k_shade_tiles<false, true, false>));
+        else if (f.nee) MRH_TRY(shade_t(k_shade_tiles<false, false, true>));
+        else MRH_TRY(shade_t(k_shade_tiles<false, false, false>));
+    } else if (f.learn && f.guide) MRH_TRY(shade(k_shade<true, true, false>));
     else if (f.learn) MRH_TRY(shade(k_shade<true, false, false>));
     else if (f.guide) MRH_TRY(shade(k_shade<false, true, false>));
     else if (f.nee) MRH_TRY(shade(k_shade<false, false, true>));

@@ -138,8 +138,12 @@ __global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__
 
 // Shade every queued ray with its request's constants (shade_ray), append
 // the survivors (and, with NEE, the next-event rays) to the other queue;
-// counts go to res (RES_* above).
-template <bool G, bool NEE>
+// counts go to res (RES_* above).  TILES: each block partitions its own tile
+// of SERVE_TILE rays by HitKey batch in LDS (no global sort; see
+// k_shade_tiles); otherwise rays are visited through the global sort's order.
+constexpr uint32_t SERVE_TILE = 1024;
+
+template <bool G, bool NEE, bool TILES>
 __global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__ hdr,
                                                      const SlotDev* __restrict__ slots, PathBuf in,
                                                      const uint32_t* __restrict__ in_slot, PathBuf out,
@@ -147,6 +151,8 @@ __global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__
                                                      const uint32_t* __restrict__ order, uint32_t* res)
 {
     __shared__ uint32_t s_live[SERVE_MAX_SLOTS], s_seg[SERVE_MAX_SLOTS], s_paths;
+    __shared__ uint16_t s_perm[TILES ? SERVE_TILE : 1];
+    __shared__ uint32_t s_cnt[8], s_off[8], s_wave[4];
     if (threadIdx.x < SERVE_MAX_SLOTS) {
         s_live[threadIdx.x] = 0;
         s_seg[threadIdx.x] = 0;
@@ -154,26 +160,62 @@ __global__ __launch_bounds__(256) void k_serve_shade(const StepHdr* __restrict__
     if (threadIdx.x == 0) s_paths = 0;
     __syncthreads();
     const uint32_t n = hdr->n_total;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-        const uint32_t j = base + threadIdx.x;
-        bool alive = false, has_nee = false;
-        RayNext nx, ne;
-        uint32_t sample = 0, slot = 0;
-        if (j < n) {
-            const uint32_t rid = order[j];
-            slot = in_slot[rid];
-            const SlotDev& sd = slots[slot];
-            sample = in.sample[rid];
-            alive = shade_ray<G, G, NEE>(in, hit, rid, sd.q, sd.ga, G && sd.learn != 0u, G && sd.guide != 0u, nx,
-                                         ne, has_nee);
-            atomicAdd(&s_seg[slot], 1u);
-            const uint32_t made = (alive ? 1u : 0u) + (has_nee ? 1u : 0u);
-            if (made) atomicAdd(&s_live[slot], made);
+    constexpr uint32_t STEP = TILES ? SERVE_TILE : 256;
+    for (uint32_t base = blockIdx.x * STEP; base < n; base += gridDim.x * STEP) {
+        if constexpr (TILES) {
+            if (threadIdx.x < 8) s_cnt[threadIdx.x] = 0;
+            __syncthreads();
+            uint32_t b[SERVE_TILE / 256];
+#pragma unroll
+            for (uint32_t k = 0; k < SERVE_TILE / 256; k++) {
+                const uint32_t j = base + threadIdx.x + 256 * k;
+                b[k] = (j < n) ? min(hit.key[j] >> 24, 7u) : 7u;
+                atomicAdd(&s_cnt[b[k]], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t acc = 0;
+                for (int c = 7; c >= 0; c--) {
+                    s_off[c] = acc;
+                    acc += s_cnt[c];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t k = 0; k < SERVE_TILE / 256; k++)
+                s_perm[atomicAdd(&s_off[b[k]], 1u)] = (uint16_t)(threadIdx.x + 256 * k);
+            __syncthreads();
         }
-        if (NEE) append_ray(has_nee, ne, sample, out, &res[0], out_slot, slot, false);
-        append_ray(alive, nx, sample, out, &res[0], out_slot, slot, NEE);
-        const uint64_t m = __ballot(alive);
-        if ((threadIdx.x & 63) == 0 && m) atomicAdd(&s_paths, (uint32_t)__popcll(m));
+#pragma unroll 1
+        for (uint32_t k = 0; k < STEP / 256; k++) {
+            uint32_t rid = 0;
+            bool valid;
+            if constexpr (TILES) {
+                rid = base + s_perm[threadIdx.x + 256 * k];
+                valid = rid < n;
+            } else {
+                const uint32_t j = base + threadIdx.x;
+                valid = j < n;
+                if (valid) rid = order[j];
+            }
+            bool alive = false, has_nee = false;
+            RayNext nx, ne;
+            uint32_t sample = 0, slot = 0;
+            if (valid) {
+                slot = in_slot[rid];
+                const SlotDev& sd = slots[slot];
+                sample = in.sample[rid];
+                alive = shade_ray<G, G, NEE>(in, hit, rid, sd.q, sd.ga, G && sd.learn != 0u, G && sd.guide != 0u,
+                                             nx, ne, has_nee);
+                atomicAdd(&s_seg[slot], 1u);
+                const uint32_t made = (alive ? 1u : 0u) + (has_nee ? 1u : 0u);
+                if (made) atomicAdd(&s_live[slot], made);
+            }
+            if (NEE) append_ray_block(has_nee, ne, sample, out, &res[0], false, s_wave, out_slot, slot);
+            const uint64_t m = __ballot(alive);
+            if ((threadIdx.x & 63) == 0 && m) atomicAdd(&s_paths, (uint32_t)__popcll(m));
+            append_ray_block(alive, nx, sample, out, &res[0], NEE, s_wave, out_slot, slot);
+        }
     }
     __syncthreads();
     if (threadIdx.x < SERVE_MAX_SLOTS) {
@@ -230,6 +272,7 @@ struct mrh_server {
     mrh_server_params p = {};
     uint32_t cap = 0;              // paths in flight (wave_paths)
     uint32_t buf_cap = 0;          // rays a queue holds: a path and its next-event ray
+    uint32_t partition = 1;        // material partition of a step (the shade_partition knob at creation)
     PathBuf bufs[2] = {};
     uint32_t* slot_of[2] = {};
     HitBuf hit = {};
@@ -574,26 +617,39 @@ int32_t step_launches(mrh_server* s, uint32_t par, uint32_t v)
     }
 #undef MRH_SERVE_TRACE_L
 #undef MRH_SERVE_TRACE
-    SortArgs sa;
-    sa.keys_in = s->hit.key;
-    sa.vals_out = s->order;
-    sa.n_max = s->buf_cap;
-    sa.n_dev = &s->d_hdr->n_total;
-    sa.key_bytes = 4;
-    sa.begin_bit = 24;
-    sa.end_bit = 32;
-    sa.descending = true;
-    MRH_TRY(radix_sort(ctx, sa));
+    const bool tiles = s->partition == 1;
+    if (!tiles) {
+        SortArgs sa;
+        sa.keys_in = s->hit.key;
+        sa.vals_out = s->order;
+        sa.n_max = s->buf_cap;
+        sa.n_dev = &s->d_hdr->n_total;
+        sa.key_bytes = 4;
+        sa.begin_bit = 24;
+        sa.end_bit = 32;
+        sa.descending = true;
+        MRH_TRY(radix_sort(ctx, sa));
+    }
+    const uint32_t grid = tiles ? stride_grid(ctx, (uint64_t)(s->buf_cap + SERVE_TILE - 1) / SERVE_TILE * 256, 256)
+                                : stride_grid(ctx, s->buf_cap, 256);
     auto shade = [&](auto kernel) {
-        return launch(ctx, "serve_shade", dim3(stride_grid(ctx, s->buf_cap, 256)), dim3(256), 0, kernel,
-                      (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, in_slot, out, out_slot, s->hit,
-                      (const uint32_t*)s->order, s->d_res);
+        return launch(ctx, "serve_shade", dim3(grid), dim3(256), 0, kernel, (const StepHdr*)s->d_hdr,
+                      (const SlotDev*)s->d_slots, in, in_slot, out, out_slot, s->hit, (const uint32_t*)s->order,
+                      s->d_res);
     };
+    if (tiles) {
+        switch (v) {
+        case 0: return shade(k_serve_shade<false, false, true>);
+        case 1: return shade(k_serve_shade<true, false, true>);
+        case 2: return shade(k_serve_shade<false, true, true>);
+        default: return shade(k_serve_shade<true, true, true>);
+        }
+    }
     switch (v) {
-    case 0: return shade(k_serve_shade<false, false>);
-    case 1: return shade(k_serve_shade<true, false>);
-    case 2: return shade(k_serve_shade<false, true>);
-    default: return shade(k_serve_shade<true, true>);
+    case 0: return shade(k_serve_shade<false, false, false>);
+    case 1: return shade(k_serve_shade<true, false, false>);
+    case 2: return shade(k_serve_shade<false, true, false>);
+    default: return shade(k_serve_shade<true, true, false>);
     }
 }
 
@@ -811,6 +867,8 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
     ctx->trace_lds_entries = user->trace_lds_entries;
     ctx->trace_blocks_per_cu = user->trace_blocks_per_cu;
     ctx->trace_width = user->trace_width;
+    ctx->shade_partition = user->shade_partition;
+    s->partition = user->shade_partition;
     auto take = [&](size_t bytes) -> void* {
         void* q = nullptr;
         if (rc == MRH_OK) rc = dmalloc(ctx, &q, bytes);

@@ -347,6 +347,20 @@ def test_render_sphere_grid_all_materials(ctx, oracle, width):
     assert relmse(g, ref) <= 1e-4
 
 
+@pytest.mark.parametrize("part", [0, 1])
+def test_render_partition_modes(ctx, oracle, part):
+    """The global radix partition and the per-block tiles shade the same rays."""
+    import nee_scenes as S
+    old = ctx.tune_get("shade_partition")
+    ctx.tune("shade_partition", part)
+    try:
+        g, st, ref, seg = _render_pair(ctx, oracle, S.lit_spheres(reflect=True), 96, 64, 2, 10, 3, 8)
+    finally:
+        ctx.tune("shade_partition", old)
+    assert st.segments == seg
+    assert relmse(g, ref) <= 1e-4
+
+
 def test_render_errors(ctx):
     sc = N.gen_cornell(1)
     scene = N.Scene(ctx, sc)

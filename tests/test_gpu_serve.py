@@ -85,6 +85,26 @@ def test_mix_matches_solo(ctx, scenes, graphs, wave):
     server.close()
 
 
+def test_mix_matches_solo_global_partition(ctx, scenes):
+    """The engine with the global radix partition instead of per-block tiles."""
+    ref = [solo(ctx, scenes[m[0]][1], *m[1:]) for m in MIX]
+    old = ctx.tune_get("shade_partition")
+    ctx.tune("shade_partition", 0)
+    try:
+        server = N.Server(ctx, hbm_budget=8 << 30, wave_paths=4096, max_slots=4, use_graphs=True)
+    finally:
+        ctx.tune("shade_partition", old)
+    imgs = [torch.full((m[1] * m[2] * 3,), 7.0, dtype=torch.float32, device="cuda") for m in MIX]
+    torch.cuda.synchronize()
+    submit_mix(server, scenes, imgs)
+    done = server.run()
+    for d in done:
+        k = d["tag"]
+        assert d["status"] == 0 and d["segments"] == ref[k][1]
+        assert np.allclose(imgs[k].cpu().numpy(), ref[k][0], rtol=1e-4, atol=1e-6), k
+    server.close()
+
+
 def test_budget_admission(ctx, scenes):
     # each plain 64x48 request holds its image (36 KiB): a 100 KiB budget
     # admits two at a time, the rest wait in the queue

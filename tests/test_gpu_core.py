@@ -347,16 +347,19 @@ def test_render_sphere_grid_all_materials(ctx, oracle, width):
     assert relmse(g, ref) <= 1e-4
 
 
-@pytest.mark.parametrize("part", [0, 1])
-def test_render_partition_modes(ctx, oracle, part):
-    """The global radix partition and the per-block tiles shade the same rays."""
+@pytest.mark.parametrize("part,lds", [(0, 8), (1, 8), (1, 24), (1, 0)])
+def test_render_partition_modes(ctx, oracle, part, lds):
+    """The global radix partition and the per-block tiles shade the same
+    rays, with the traversal stack in LDS (8 or 24 entries) or in scratch."""
     import nee_scenes as S
-    old = ctx.tune_get("shade_partition")
+    old = {k: ctx.tune_get(k) for k in ("shade_partition", "trace_lds_entries")}
     ctx.tune("shade_partition", part)
+    ctx.tune("trace_lds_entries", lds)
     try:
         g, st, ref, seg = _render_pair(ctx, oracle, S.lit_spheres(reflect=True), 96, 64, 2, 10, 3, 8)
     finally:
-        ctx.tune("shade_partition", old)
+        for k, v in old.items():
+            ctx.tune(k, v)
     assert st.segments == seg
     assert relmse(g, ref) <= 1e-4
 

@@ -207,7 +207,38 @@ typedef struct {
     uint32_t        morton_bits; /* 30 or 63 */
     const float*    reflect;     /* host float[n_reflect][4]: albedo rgb, roughness (mirror at 0) */
     uint32_t        n_reflect;
+    /* NULL: a single-level scene, tris in world space.  Otherwise a
+     * two-level scene: tris / prim_key are the meshes' triangles in their
+     * local spaces, and the instances place them in the world. */
+    const struct mrh_instancing_desc* instancing;
 } mrh_scene_desc;
+
+/* Two-level scenes.  Replaces the base accelerator over per-surface
+ * accelerators (GPUBaseAcceleratorBVH: KCIntersectBaseBVH,
+ * TracerCUDA/GPUAcceleratorBVHKC.cuh:513-652) and the inner traversal of
+ * a transformed surface (GPUAcceleratorBVHKC.cuh:274-345, the ray taken to
+ * local space by GPUTransformSingle::WorldToLocal = Ray::Transform(inverse),
+ * GPUTransformSingle.cuh:141-147, RayLib/Ray.hpp:461-465).  Every mesh gets
+ * its own LBVH; a top-level LBVH is built over the instances' world boxes
+ * (the mesh root box's eight corners taken to the world,
+ * GPUTransformSingle.cuh:180-197).  One trace launch walks the top tree and,
+ * at an instance, the mesh's tree with the ray in local space; the
+ * direction is not renormalised, so t is the world ray's parameter.  The
+ * world-to-local matrix is the inverse of xform computed in double and
+ * rounded to float. */
+typedef struct mrh_instancing_desc {
+    const uint32_t* mesh_first;  /* host uint32[n_meshes + 1]: mesh m owns triangles
+                                    [mesh_first[m], mesh_first[m + 1]); mesh_first[0] = 0,
+                                    mesh_first[n_meshes] = n, no mesh is empty */
+    uint32_t        n_meshes;
+    const float*    xform;       /* host float[n_instances][12]: local-to-world, row-major 3x4
+                                    (the last row of the 4x4 is 0 0 0 1) */
+    const uint32_t* inst_mesh;   /* host uint32[n_instances] */
+    const uint32_t* inst_key;    /* host uint32[n_instances] or NULL: a HitKey for all of the
+                                    instance's triangles, 0 = the mesh's own keys (the
+                                    per-surface material of a scene file) */
+    uint32_t        n_instances;
+} mrh_instancing_desc;
 
 typedef struct {
     float    position[3];
@@ -244,9 +275,20 @@ typedef struct {
 /* Uploads the scene and builds its LBVH (synchronous). */
 int32_t mrh_scene_create(mrh_context* ctx, const mrh_scene_desc* desc, mrh_scene** out);
 int32_t mrh_scene_destroy(mrh_scene* scene);
+/* The scene's LBVH (MRH_ERR_UNSUPPORTED for a two-level scene: use
+ * mrh_scene_trace_closest) */
 int32_t mrh_scene_bvh(const mrh_scene* scene, const mrh_lbvh** out);
 /* Device bytes held by the scene (TracerSystemI/GPUTracerI::TotalGPUMemoryUsed) */
 int32_t mrh_scene_bytes(const mrh_scene* scene, uint64_t* bytes);
+/* Closest hit against a scene, single- or two-level (GPUBaseAcceleratorI::Hit
+ * followed by the inner accelerators' Hit, GPUAcceleratorI.h:149-166).  Rays
+ * and outputs as mrh_trace_closest; d_prim is the index into the scene's
+ * tris, d_inst (may be NULL) the instance (0 in a single-level scene,
+ * 0xFFFFFFFF on a miss).  d_stats counts the top tree's internal nodes and
+ * the meshes' internal nodes together. */
+int32_t mrh_scene_trace_closest(mrh_context* ctx, const mrh_scene* scene, const float* d_rays, uint32_t n,
+                                uint32_t pitch, float tmin, uint32_t* d_inst, uint32_t* d_prim, float* d_t,
+                                float* d_bary, uint32_t* d_stats);
 
 /* Renders one frame: d_image (device float[height][width][3], bottom row
  * first) receives the per-pixel mean radiance.  Synchronous; stats may be NULL. */

@@ -392,4 +392,199 @@ __device__ __forceinline__ TraceResult traverse_w(Stack& st, const float4* __res
     else return traverse<ANY, STATS>(st, nodes, leaves, root, n_leaves, o, d, tmin, tmax);
 }
 
+// ------------------------------------------------------------ two-level
+
+// Instance record, 8 float4 (128 B):
+//   [0..2] world-to-local rows (m[r][0..3]), [3..5] local-to-world rows,
+//   [6] (mesh, HitKey override or 0, first leaf of the mesh in the
+//        concatenated leaf records, first node of the mesh's binary tree)
+//   [7] (first node of the mesh's 4-wide tree, 4-wide root code, binary
+//        root code, triangle count)
+constexpr uint32_t INST_F4 = 8;
+
+// Matrix<4>::operator*(Vector<4>) (RayLib/Matrix.hpp:268-289) for one row
+// of an affine matrix: 0 + m0 v0 + m1 v1 + m2 v2 + m3 w, summed left to right.
+__device__ __forceinline__ float xf_row(float4 m, V3 v, float w)
+{
+    float r = 0.0f;
+    r = r + m.x * v.x;
+    r = r + m.y * v.y;
+    r = r + m.z * v.z;
+    r = r + m.w * w;
+    return r;
+}
+__device__ __forceinline__ V3 xf_point(float4 r0, float4 r1, float4 r2, V3 p)
+{
+    return v3(xf_row(r0, p, 1.0f), xf_row(r1, p, 1.0f), xf_row(r2, p, 1.0f));
+}
+__device__ __forceinline__ V3 xf_dir(float4 r0, float4 r1, float4 r2, V3 d)
+{
+    return v3(xf_row(r0, d, 0.0f), xf_row(r1, d, 0.0f), xf_row(r2, d, 0.0f));
+}
+
+// A stack that continues another one above its entry `base` (a mesh's
+// traversal inside the top-level walk).  Entries past the parent's capacity
+// are dropped on put and read back as culled, so a malformed tree cannot
+// index past the stack (the host sizes CAP for both trees' depths).
+template <class P>
+struct OffsetStack {
+    static constexpr int CAP = P::CAP;
+    P& p;
+    int base;
+    __device__ __forceinline__ void put(int i, uint32_t n, float d)
+    {
+        if (base + i < CAP) p.put(base + i, n, d);
+    }
+    __device__ __forceinline__ void get(int i, uint32_t& n, float& d) const
+    {
+        if (base + i < CAP) {
+            p.get(base + i, n, d);
+        } else {
+            n = NO_HIT;
+            d = INFINITY;
+        }
+    }
+};
+
+// Closest / any hit over a two-level scene.  The top tree is walked 4-wide
+// as traverse4 (its leaf records hold the instance index in the first
+// corner's w); at an instance the ray goes to local space
+// (GPUTransformSingle::WorldToLocal = Ray::Transform by the inverse,
+// RayLib/Ray.hpp:461-465) and the mesh's tree (width W) is walked with the
+// current closest t as its tmax on the same stack, above the top-level
+// entries.  r.leaf is the index into the concatenated leaf records; inst
+// receives the instance.  Counts add the top tree's internal nodes to the
+// meshes' nodes and triangle tests.
+template <int W, bool ANY, bool STATS, class Stack>
+__device__ __forceinline__ TraceResult traverse_two_level(Stack& st, const float4* __restrict__ top4,
+                                                          const float4* __restrict__ top_leaves, uint32_t top_root,
+                                                          uint32_t n_inst, const float4* __restrict__ inst,
+                                                          const float4* __restrict__ mesh_nodes,
+                                                          const float4* __restrict__ leaves, V3 o, V3 d,
+                                                          float tmin, float tmax, uint32_t& hit_inst)
+{
+    TraceResult r;
+    r.leaf = NO_HIT;
+    r.t = tmax;
+    r.ba = r.bb = 0.0f;
+    r.visits = r.tests = 0;
+    hit_inst = NO_HIT;
+    const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int sp = 0;
+    uint32_t next = top_root;
+    for (;;) {
+        for (;;) {
+            if (next == NO_HIT) {
+                bool found = false;
+                while (sp > 0) {
+                    --sp;
+                    uint32_t nn;
+                    float et;
+                    st.get(sp, nn, et);
+                    if (!(et > r.t)) {
+                        next = nn;
+                        found = true;
+                        break;
+                    }
+                }
+                if (!found) return r;
+            }
+            if (next & LEAF_BIT) {
+                const uint32_t li = next & ~LEAF_BIT;
+                next = NO_HIT;
+                if (li >= n_inst) continue;   // unreachable on a well-formed tree
+                const uint32_t ii = __float_as_uint(top_leaves[3 * (size_t)li].w);
+                if (ii >= n_inst) continue;
+                const float4* rec = inst + INST_F4 * (size_t)ii;
+                const float4 w0 = rec[0], w1 = rec[1], w2 = rec[2];
+                const float4 m6 = rec[6], m7 = rec[7];
+                const V3 lo = xf_point(w0, w1, w2, o);
+                const V3 ld = xf_dir(w0, w1, w2, d);
+                const uint32_t leaf_base = __float_as_uint(m6.z);
+                const uint32_t n_tris = __float_as_uint(m7.w);
+                const float4* nodes;
+                uint32_t root;
+                if constexpr (W == 4) {
+                    nodes = mesh_nodes + 8 * (size_t)__float_as_uint(m7.x);
+                    root = __float_as_uint(m7.y);
+                } else {
+                    nodes = mesh_nodes + 4 * (size_t)__float_as_uint(m6.w);
+                    root = __float_as_uint(m7.z);
+                }
+                OffsetStack<Stack> sub{st, sp};
+                const TraceResult b = traverse_w<W, ANY, STATS>(sub, nodes, leaves + 3 * (size_t)leaf_base, root,
+                                                                n_tris, lo, ld, tmin, r.t);
+                if (STATS) {
+                    r.visits += b.visits;
+                    r.tests += b.tests;
+                }
+                if (b.leaf != NO_HIT) {
+                    r.t = b.t;
+                    r.leaf = leaf_base + b.leaf;
+                    r.ba = b.ba;
+                    r.bb = b.bb;
+                    hit_inst = ii;
+                    if (ANY) return r;
+                }
+                continue;
+            }
+            if (next + 1u >= n_inst) {   // internal index out of range: unreachable
+                next = NO_HIT;
+                continue;
+            }
+            break;
+        }
+        if (STATS) r.visits++;
+        const float4* nd = top4 + 8 * (size_t)next;
+        const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5], cc = nd[6];
+        uint32_t c0 = __float_as_uint(cc.x), c1 = __float_as_uint(cc.y), c2 = __float_as_uint(cc.z),
+                 c3 = __float_as_uint(cc.w);
+        float k0, k1, k2, k3;
+        bool h0 = intersect_box(o, inv, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tmin, r.t, k0) && c0 != NO_HIT;
+        bool h1 = intersect_box(o, inv, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tmin, r.t, k1) && c1 != NO_HIT;
+        bool h2 = intersect_box(o, inv, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tmin, r.t, k2) && c2 != NO_HIT;
+        bool h3 = intersect_box(o, inv, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tmin, r.t, k3) && c3 != NO_HIT;
+        const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+        if (cnt == 0) {
+            next = NO_HIT;
+            continue;
+        }
+        k0 = h0 ? k0 : INFINITY;
+        k1 = h1 ? k1 : INFINITY;
+        k2 = h2 ? k2 : INFINITY;
+        k3 = h3 ? k3 : INFINITY;
+        uint32_t s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+        auto cx = [](float& ka, uint32_t& sa, uint32_t& ca, float& kb, uint32_t& sb, uint32_t& cb) {
+            const bool sw = (ka > kb) || (ka == kb && sa > sb);
+            const float tk = sw ? kb : ka;
+            kb = sw ? ka : kb;
+            ka = tk;
+            const uint32_t ts = sw ? sb : sa;
+            sb = sw ? sa : sb;
+            sa = ts;
+            const uint32_t tc = sw ? cb : ca;
+            cb = sw ? ca : cb;
+            ca = tc;
+        };
+        cx(k0, s0, c0, k1, s1, c1);
+        cx(k2, s2, c2, k3, s3, c3);
+        cx(k0, s0, c0, k2, s2, c2);
+        cx(k1, s1, c1, k3, s3, c3);
+        cx(k1, s1, c1, k2, s2, c2);
+        if (cnt >= 4 && sp < Stack::CAP) st.put(sp++, c3, k3);
+        if (cnt >= 3 && sp < Stack::CAP) st.put(sp++, c2, k2);
+        if (cnt >= 2 && sp < Stack::CAP) st.put(sp++, c1, k1);
+        next = c0;
+    }
+}
+
+// Stack entries a two-level walk needs: the top tree's 4-wide need plus the
+// deepest mesh tree's need at width w.
+inline uint32_t two_level_stack_need(uint32_t top_depth, uint32_t mesh_depth, uint32_t w)
+{
+    const uint32_t top = 3 * ((top_depth + 1) / 2) + 1;
+    const uint32_t mesh = (w == 4) ? 3 * ((mesh_depth + 1) / 2) + 1 : mesh_depth + 1;
+    return top + mesh;
+}
+
 }  // namespace mrh

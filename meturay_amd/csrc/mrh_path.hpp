@@ -24,9 +24,29 @@ struct mrh_scene {
     uint32_t n_lights = 0;
     uint64_t bytes = 0;
     std::vector<void*> allocs;
+    // Two-level scenes: bvh is the top-level tree over the instances' world
+    // boxes; the meshes' trees and leaf records are concatenated.
+    bool two_level = false;
+    float4* inst = nullptr;         // [n_inst][INST_F4] instance records
+    uint32_t n_inst = 0;
+    float4* mesh_nodes2 = nullptr;  // binary trees of all meshes
+    float4* mesh_nodes4 = nullptr;  // 4-wide trees of all meshes
+    float4* leaves = nullptr;       // [n][3] leaf records, mesh after mesh (prim = scene index)
+    uint32_t n_tris = 0;
+    uint32_t mesh_depth = 0;        // deepest mesh tree
+    uint32_t* light_inst = nullptr; // instance of every light (lights hold leaf indices)
 };
 
 namespace mrh {
+
+// Builds the meshes' trees, the top-level tree, the instance records and the
+// light list of a two-level scene (mrh_instance.hip); s holds the material
+// tables already.
+int32_t two_level_build(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene* s);
+// World-to-local matrix of a row-major 3x4 local-to-world one (double
+// adjugate, rounded to float); false when singular or not finite.
+bool affine_inverse(const float* l2w, float* w2l);
+
 namespace {
 
 constexpr float INV_PI = 0.31830988618379067154f;
@@ -51,7 +71,7 @@ struct PathBuf {
     float *tr, *tg, *tb;
     uint32_t* sample;
     uint32_t* dm;        // depth | type | medium (above)
-    uint64_t* rng;       // PCG32 state; a NEE ray holds its light's leaf instead
+    uint64_t* rng;       // PCG32 state; a NEE ray holds its light's leaf (low word) and instance (high word) instead
     float* pdf;          // BxDF pdf of a path ray's direction (RayAuxPath prevPDF), with MIS only
 };
 
@@ -59,6 +79,7 @@ struct HitBuf {
     uint32_t* leaf;
     float *t, *ba, *bb;
     uint32_t* key;
+    uint32_t* inst = nullptr;   // instance of the hit (two-level scenes only)
 };
 
 struct Materials {
@@ -465,7 +486,30 @@ struct ShadeReq {
     uint32_t n_lights;
     uint32_t nee;            // next-event estimation ("NextEventEstimation")
     uint32_t mis;            // power-heuristic MIS of NEE and BxDF rays ("DirectLightMIS"), with nee only
+    const float4* inst = nullptr;          // two-level scenes: instance records
+    const uint32_t* light_inst = nullptr;  // two-level scenes: instance of every light
 };
+
+// World-space corners of leaf li (of instance ii in a two-level scene: the
+// local corners taken to the world by the instance's local-to-world matrix,
+// GPUTransformSingle::LocalToWorld, GPUTransformSingle.cuh:160-166).
+template <class Q>
+__device__ __forceinline__ void leaf_corners(const Q& q, uint32_t li, uint32_t ii, V3& p0, V3& p1, V3& p2)
+{
+    const float4 l0 = q.leaves[3 * (size_t)li + 0];
+    const float4 l1 = q.leaves[3 * (size_t)li + 1];
+    const float4 l2 = q.leaves[3 * (size_t)li + 2];
+    p0 = v3(l0.x, l0.y, l0.z);
+    p1 = v3(l1.x, l1.y, l1.z);
+    p2 = v3(l2.x, l2.y, l2.z);
+    if (q.inst) {
+        const float4* rec = q.inst + INST_F4 * (size_t)ii;
+        const float4 m0 = rec[3], m1 = rec[4], m2 = rec[5];
+        p0 = xf_point(m0, m1, m2, p0);
+        p1 = xf_point(m0, m1, m2, p1);
+        p2 = xf_point(m0, m1, m2, p2);
+    }
+}
 
 // Next-bounce state of a surviving path
 struct RayNext {
@@ -511,17 +555,16 @@ __device__ __forceinline__ float light_pdf_hit(V3 p0, V3 p1, V3 p2, float t, V3 
 // 1 / light count).  A primitive-backed light is traced to its first hit
 // (tMax = FLT_MAX), so no distance is returned.
 __device__ __forceinline__ uint32_t sample_light(const ShadeReq& q, V3 p, Rng& rng, V3& dir, float& dist2,
-                                                 float& pdf)
+                                                 float& pdf, uint32_t& inst)
 {
     float r1 = rng.uniform();
     r1 *= (float)q.n_lights;
     uint32_t idx = (uint32_t)r1;
     if (idx == q.n_lights) idx--;
     const uint32_t leaf = q.lights[idx];
-    const float4 l0 = q.leaves[3 * (size_t)leaf + 0];
-    const float4 l1 = q.leaves[3 * (size_t)leaf + 1];
-    const float4 l2 = q.leaves[3 * (size_t)leaf + 2];
-    const V3 p0 = v3(l0.x, l0.y, l0.z), p1 = v3(l1.x, l1.y, l1.z), p2 = v3(l2.x, l2.y, l2.z);
+    const uint32_t li_inst = q.light_inst ? q.light_inst[idx] : 0u;
+    V3 p0, p1, p2;
+    leaf_corners(q, leaf, li_inst, p0, p1, p2);
     const float xi0 = rng.uniform();
     const float xi1 = rng.uniform();
     const float s1 = sqrtf(xi0);
@@ -539,6 +582,7 @@ __device__ __forceinline__ uint32_t sample_light(const ShadeReq& q, V3 p, Rng& r
     const float ndl = fabsf(dot(n, -dir));
     pdf *= dist2 / ndl;
     pdf *= 1.0f / (float)q.n_lights;
+    inst = li_inst;
     return leaf;
 }
 
@@ -577,7 +621,9 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
     if (NEE && (dm & DM_NEE)) {
         // a NEE ray counts only where it hits the light it was sent to
         // (isCorrectNEERay); any other hit, or a miss, means occluded
-        if (batch == MRH_BATCH_EMISSIVE && hit.leaf[rid] == (uint32_t)in.rng[rid]) {
+        const uint64_t want = in.rng[rid];
+        if (batch == MRH_BATCH_EMISSIVE && hit.leaf[rid] == (uint32_t)want &&
+            (!hit.inst || hit.inst[rid] == (uint32_t)(want >> 32))) {
             const float* le = mats.emissive + (size_t)id * 3;
             add_radiance(q, sample, v3(le[0] * T.x, le[1] * T.y, le[2] * T.z));
         }
@@ -591,13 +637,10 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
             // a NEE ray: without MIS the light was counted by that ray, with
             // MIS this hit takes the BxDF strategy's power-heuristic weight
             if (!q.mis) return false;
-            const uint32_t li = hit.leaf[rid];
-            const float4 l0 = q.leaves[3 * (size_t)li + 0];
-            const float4 l1 = q.leaves[3 * (size_t)li + 1];
-            const float4 l2 = q.leaves[3 * (size_t)li + 2];
+            V3 p0, p1, p2;
+            leaf_corners(q, hit.leaf[rid], hit.inst ? hit.inst[rid] : 0u, p0, p1, p2);
             const V3 d = normalize(v3(in.dx[rid], in.dy[rid], in.dz[rid]));
-            const float pc = light_pdf_hit(v3(l0.x, l0.y, l0.z), v3(l1.x, l1.y, l1.z), v3(l2.x, l2.y, l2.z),
-                                           hit.t[rid], d);
+            const float pc = light_pdf_hit(p0, p1, p2, hit.t[rid], d);
             const float pm = 1.0f / (float)q.n_lights;
             c = scale(c, power_heuristic(in.pdf[rid], pc * pm));
         }
@@ -619,13 +662,10 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
         return false;
     }
     if (depth >= q.max_depth) return false;
-    const uint32_t li = hit.leaf[rid];
-    const float4 l0 = q.leaves[3 * (size_t)li + 0];
-    const float4 l1 = q.leaves[3 * (size_t)li + 1];
-    const float4 l2 = q.leaves[3 * (size_t)li + 2];
+    V3 c0, c1, c2;
+    leaf_corners(q, hit.leaf[rid], hit.inst ? hit.inst[rid] : 0u, c0, c1, c2);
     const V3 d = v3(in.dx[rid], in.dy[rid], in.dz[rid]);
-    const Surface s = make_surface(v3(l0.x, l0.y, l0.z), v3(l1.x, l1.y, l1.z), v3(l2.x, l2.y, l2.z), hit.ba[rid],
-                                   hit.bb[rid], d);
+    const Surface s = make_surface(c0, c1, c2, hit.ba[rid], hit.bb[rid], d);
     const V3 wi = -normalize(d);
     Rng rng{in.rng[rid], pcg32_inc(q.sample_base + sample)};
     if (NEE && q.nee && q.n_lights &&
@@ -633,7 +673,8 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
         // PathTracerKC.cuh:493-575: one light sample per non-specular vertex
         V3 ldir;
         float dist2, pl;
-        const uint32_t lleaf = sample_light(q, s.pos, rng, ldir, dist2, pl);
+        uint32_t linst;
+        const uint32_t lleaf = sample_light(q, s.pos, rng, ldir, dist2, pl, linst);
         float pb;
         V3 f;
         if (batch == MRH_BATCH_LAMBERT) {
@@ -655,7 +696,7 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
             nee.d = ldir;
             nee.T = nf;
             nee.dm = depth | DM_NEE;
-            nee.rng = lleaf;
+            nee.rng = (uint64_t)lleaf | ((uint64_t)linst << 32);   // the light's leaf and instance
             nee.pdf = 0.0f;
             has_nee = true;
         }

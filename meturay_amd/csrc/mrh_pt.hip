@@ -135,6 +135,110 @@ int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, cons
     return launch_trace_pt_w<2, 128, STATS>(ctx, grid, b, ts, in, counts, bounce, tmin, tmax, hit);
 }
 
+// Two-level scenes: the top tree over the instances, then the hit mesh's
+// tree in local space (traverse_two_level); writes the instance too, and the
+// instance's HitKey override where it has one.
+template <int W, int CAP, bool STATS, int L>
+__global__ __launch_bounds__(256) void k_trace_pt2(const float4* __restrict__ top4,
+                                                   const float4* __restrict__ top_leaves, uint32_t top_root,
+                                                   uint32_t n_inst, const float4* __restrict__ inst,
+                                                   const float4* __restrict__ mesh_nodes,
+                                                   const float4* __restrict__ leaves, PathBuf in,
+                                                   uint32_t* __restrict__ counts, uint32_t bounce, float tmin,
+                                                   float tmax, HitBuf hit, uint2* __restrict__ overflow)
+{
+    __shared__ uint32_t s_node[L > 0 ? L * 256 : 1];
+    __shared__ float s_t[L > 0 ? L * 256 : 1];
+    const uint32_t n = counts[bounce];
+    const uint32_t lb = xcd_block();
+    const size_t ostride = (size_t)gridDim.x * blockDim.x;
+    const size_t oslot = (size_t)lb * blockDim.x + threadIdx.x;
+    uint64_t visits = 0, tests = 0;
+    for (uint32_t i = lb * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const V3 o = v3(in.ox[i], in.oy[i], in.oz[i]);
+        const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
+        TraceResult r;
+        uint32_t ii;
+        if constexpr (L > 0) {
+            LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
+            r = traverse_two_level<W, false, STATS>(st, top4, top_leaves, top_root, n_inst, inst, mesh_nodes, leaves,
+                                                    o, d, tmin, tmax, ii);
+        } else {
+            PrivateStack<CAP> st;
+            r = traverse_two_level<W, false, STATS>(st, top4, top_leaves, top_root, n_inst, inst, mesh_nodes, leaves,
+                                                    o, d, tmin, tmax, ii);
+        }
+        hit.leaf[i] = r.leaf;
+        hit.t[i] = r.t;
+        hit.ba[i] = r.ba;
+        hit.bb[i] = r.bb;
+        hit.inst[i] = ii;
+        uint32_t key = 0u;
+        if (r.leaf != NO_HIT) {
+            const uint32_t ko = __float_as_uint(inst[INST_F4 * (size_t)ii + 6].y);
+            key = ko ? ko : __float_as_uint(leaves[3 * (size_t)r.leaf + 1].w);
+        }
+        hit.key[i] = key;
+        if (STATS) {
+            visits += r.visits;
+            tests += r.tests;
+        }
+    }
+    if (STATS) {
+        for (int off = 32; off > 0; off >>= 1) {
+            visits += __shfl_xor(visits, off, 64);
+            tests += __shfl_xor(tests, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd((unsigned long long*)(counts + 64), (unsigned long long)visits);
+            atomicAdd((unsigned long long*)(counts + 66), (unsigned long long)tests);
+        }
+    }
+}
+
+template <int W, int CAP, bool STATS>
+int32_t launch_trace_pt2_w(mrh_context* ctx, uint32_t grid, const mrh_scene* s, const PathBuf& in,
+                           uint32_t* counts, uint32_t bounce, float tmin, float tmax, const HitBuf& hit)
+{
+    const mrh_lbvh* top = s->bvh;
+    const uint32_t lds = ctx->trace_lds_entries;
+    uint2* ovf = nullptr;
+    MRH_TRY(trace_overflow(ctx, grid, CAP, lds, &ovf));
+    const char* name = STATS ? "trace_pt_stats" : "trace_pt";
+    const float4* t4 = top->nodes4;
+    const float4* tl = (const float4*)top->leaves;
+    const float4* mn = (W == 4) ? s->mesh_nodes4 : s->mesh_nodes2;
+#define MRH_TRACE_PT2_L(LV)                                                                                    \
+    return launch(ctx, name, dim3(grid), dim3(256), 0, k_trace_pt2<W, CAP, STATS, LV>, t4, tl, top->root4,      \
+                  s->n_inst, (const float4*)s->inst, mn, (const float4*)s->leaves, in, counts, bounce, tmin, tmax, \
+                  hit, ovf)
+    switch (lds) {
+    case 0: MRH_TRACE_PT2_L(0);
+    case 8: MRH_TRACE_PT2_L(8);
+    case 12: MRH_TRACE_PT2_L(12);
+    case 16: MRH_TRACE_PT2_L(16);
+    case 24: MRH_TRACE_PT2_L(24);
+    default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24");
+    }
+#undef MRH_TRACE_PT2_L
+}
+
+template <bool STATS>
+int32_t launch_trace_pt2(mrh_context* ctx, uint32_t grid, const mrh_scene* s, const PathBuf& in, uint32_t* counts,
+                         uint32_t bounce, float tmin, float tmax, const HitBuf& hit)
+{
+    const uint32_t w = (ctx->trace_width == 4) ? 4u : 2u;
+    const uint32_t need = two_level_stack_need(s->bvh->max_depth, s->mesh_depth, w);
+    if (need > 128)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "two-level trees need %u stack entries (at most 128)", need);
+    if (w == 4) {
+        if (need <= 64) return launch_trace_pt2_w<4, 64, STATS>(ctx, grid, s, in, counts, bounce, tmin, tmax, hit);
+        return launch_trace_pt2_w<4, 128, STATS>(ctx, grid, s, in, counts, bounce, tmin, tmax, hit);
+    }
+    if (need <= 64) return launch_trace_pt2_w<2, 64, STATS>(ctx, grid, s, in, counts, bounce, tmin, tmax, hit);
+    return launch_trace_pt2_w<2, 128, STATS>(ctx, grid, s, in, counts, bounce, tmin, tmax, hit);
+}
+
 // One launch over all partitions of a bounce (see shade_ray); with NEE a
 // path appends its next-event ray beside its continuing ray.
 template <bool LEARN, bool GUIDE, bool NEE>
@@ -235,10 +339,10 @@ uint64_t frame_cap(uint64_t total, uint32_t nee) { return nee ? 2 * total : tota
 size_t frame_bytes(uint64_t cap)
 {
     // 2 path buffers (9 floats + sample + dm + pdf = 12 words, + u64 rng),
-    // hits (4 words) + key + order, plus counts/offsets
+    // hits (4 words) + key + instance + order, plus counts/offsets
     size_t per = 0;
     per += 2 * (12 * align_up(cap * 4) + align_up(cap * 8));
-    per += 6 * align_up(cap * 4);
+    per += 7 * align_up(cap * 4);
     per += 4096;
     return per;
 }
@@ -282,6 +386,7 @@ void carve_frame(void* base, uint64_t cap, Frame& f)
     f.hit.ba = (float*)take(cap * 4);
     f.hit.bb = (float*)take(cap * 4);
     f.hit.key = (uint32_t*)take(cap * 4);
+    f.hit.inst = (uint32_t*)take(cap * 4);
     f.order = (uint32_t*)take(cap * 4);
     f.counts = (uint32_t*)take(COUNTS_WORDS * 4);
     f.offsets = (uint32_t*)take(260 * 4);
@@ -341,9 +446,10 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
     }
     if (d->n_dielectric > MAX_DIELECTRICS)
         return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: at most %u dielectric materials", MAX_DIELECTRICS);
-    std::vector<uint32_t> rank(d->n);
+    const bool two = d->instancing != nullptr;
+    std::vector<uint32_t> rank(two ? 0 : d->n);
     uint32_t n_lights = 0;
-    for (uint32_t i = 0; i < d->n; i++)
+    for (uint32_t i = 0; i < d->n && !two; i++)
         rank[i] = ((d->prim_key[i] >> 24) == MRH_BATCH_EMISSIVE) ? n_lights++ : 0xFFFFFFFFu;
     mrh_scene* s = new mrh_scene();
     s->ctx = ctx;
@@ -362,6 +468,10 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
         if ((rc = scene_upload(s, &s->dielectric, d->dielectric, (size_t)d->n_dielectric * 4)) != MRH_OK) break;
         if ((rc = scene_upload(s, &s->emissive, d->emissive, (size_t)d->n_emissive * 3)) != MRH_OK) break;
         if ((rc = scene_upload(s, &s->reflect, d->reflect, (size_t)d->n_reflect * 4)) != MRH_OK) break;
+        if (two) {
+            rc = two_level_build(ctx, d, s);
+            break;
+        }
         if ((rc = scene_upload(s, &dtris, d->tris, (size_t)d->n * 9)) != MRH_OK) break;
         if ((rc = scene_upload(s, &dkeys, d->prim_key, (size_t)d->n)) != MRH_OK) break;
         uint32_t bits = d->morton_bits ? d->morton_bits : 30u;
@@ -454,11 +564,15 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     const float tmax = (bounce == 0) ? f.cam.far_d : FLT_MAX;
     const uint32_t grid = stride_grid(ctx, f.cap, 256);
     const uint32_t tgrid = trace_grid(ctx, f.cap);
-    const float4* lv = (const float4*)b->leaves;
-    if (f.count_traversal) {
-        MRH_TRY(launch_trace_pt<true>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit));
+    const float4* lv = s->two_level ? (const float4*)s->leaves : (const float4*)b->leaves;
+    HitBuf hb = f.hit;
+    if (s->two_level) {
+        if (f.count_traversal) MRH_TRY(launch_trace_pt2<true>(ctx, tgrid, s, in, f.counts, bounce, tmin, tmax, hb));
+        else MRH_TRY(launch_trace_pt2<false>(ctx, tgrid, s, in, f.counts, bounce, tmin, tmax, hb));
     } else {
-        MRH_TRY(launch_trace_pt<false>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, f.hit));
+        hb.inst = nullptr;
+        if (f.count_traversal) MRH_TRY(launch_trace_pt<true>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, hb));
+        else MRH_TRY(launch_trace_pt<false>(ctx, tgrid, b, in, f.counts, bounce, tmin, tmax, hb));
     }
     const bool tiles = ctx->shade_partition == 1;
     if (!tiles) {
@@ -487,13 +601,15 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     q.n_lights = s->n_lights;
     q.nee = f.nee;
     q.mis = f.mis;
+    q.inst = s->two_level ? (const float4*)s->inst : nullptr;
+    q.light_inst = s->two_level ? s->light_inst : nullptr;
     auto shade = [&](auto kernel) {
-        return launch(ctx, "shade", dim3(grid), dim3(256), 0, kernel, in, outb, f.hit, (const uint32_t*)f.order,
+        return launch(ctx, "shade", dim3(grid), dim3(256), 0, kernel, in, outb, hb, (const uint32_t*)f.order,
                       f.counts, bounce, q, f.ga);
     };
     const uint32_t tgrid_s = stride_grid(ctx, (f.cap + SHADE_TILE - 1) / SHADE_TILE * 256, 256);
     auto shade_t = [&](auto kernel) {
-        return launch(ctx, "shade", dim3(tgrid_s), dim3(256), 0, kernel, in, outb, f.hit, f.counts, bounce, q, f.ga);
+        return launch(ctx, "shade", dim3(tgrid_s), dim3(256), 0, kernel, in, outb, hb, f.counts, bounce, q, f.ga);
     };
     if (tiles) {
         if (f.learn && f.guide) MRH_TRY(shade_t(k_shade_tiles<true, true, false>));
@@ -821,6 +937,7 @@ int32_t mrh_scene_destroy(mrh_scene* scene)
 int32_t mrh_scene_bvh(const mrh_scene* scene, const mrh_lbvh** out)
 {
     if (!scene || !out) return MRH_ERR_INVALID;
+    if (scene->two_level) return fail(scene->ctx, MRH_ERR_UNSUPPORTED, "mrh_scene_bvh: a two-level scene has no single LBVH");
     *out = scene->bvh;
     return MRH_OK;
 }

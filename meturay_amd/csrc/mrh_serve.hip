@@ -931,6 +931,8 @@ int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
         return fail(ctx, MRH_ERR_INVALID, "submit: nee and direct_light_mis are 0 or 1");
     if (p.nee && r->guide.iterations > 1)
         return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: next-event estimation with guiding is not implemented");
+    if (r->scene->two_level)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: two-level scenes are rendered by mrh_render, not the engine");
     if (r->scene->bvh->max_depth > 128)
         return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: tree depth %u exceeds the traversal stack",
                     r->scene->bvh->max_depth);

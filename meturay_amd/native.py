@@ -112,11 +112,17 @@ class ServerInfo(C.Structure):
                 ("active", _U32), ("queued", _U32), ("live_paths", _U32), ("reserved", _U32)]
 
 
+class InstancingDesc(C.Structure):
+    _fields_ = [("mesh_first", _P), ("n_meshes", _U32), ("xform", _P), ("inst_mesh", _P), ("inst_key", _P),
+                ("n_instances", _U32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("tris", _P), ("prim_key", _P), ("n", _U32),
                 ("lambert", _P), ("n_lambert", _U32), ("metal", _P), ("n_metal", _U32),
                 ("dielectric", _P), ("n_dielectric", _U32), ("emissive", _P),
-                ("n_emissive", _U32), ("morton_bits", _U32), ("reflect", _P), ("n_reflect", _U32)]
+                ("n_emissive", _U32), ("morton_bits", _U32), ("reflect", _P), ("n_reflect", _U32),
+                ("instancing", _P)]
 
 
 class SceneBuffers(C.Structure):
@@ -140,6 +146,7 @@ _SIGS = {
     "mrh_sort_pairs": (_I32, [_P, _P, _P, _P, _P, _U32, _U32, _U32, _U32, _I32]),
     "mrh_partition_by_key": (_I32, [_P, _P, _U32, _U32, _U32, _U32, _P, _P]),
     "mrh_lbvh_build": (_I32, [_P, _P, _U32, _U32, C.POINTER(_P)]),
+    "mrh_scene_trace_closest": (_I32, [_P, _P, _P, _U32, _U32, _F, _P, _P, _P, _P, _P]),
     "mrh_lbvh_view_get": (_I32, [_P, C.POINTER(LbvhView)]),
     "mrh_lbvh_destroy": (_I32, [_P]),
     "mrh_lbvh_rebuild": (_I32, [_P, _P, _P]),
@@ -501,9 +508,30 @@ class Scene:
             setattr(d, name, a.ctypes.data if a is not None and a.size else None)
             setattr(d, "n_" + name, a.shape[0] if a is not None else 0)
         d.morton_bits = morton_bits
+        inst = scene.get("instancing")
+        self.two_level = inst is not None
+        if inst is not None:
+            ik = inst.get("inst_key")
+            arrs = {"mesh_first": np.ascontiguousarray(inst["mesh_first"], np.uint32),
+                    "xform": np.ascontiguousarray(inst["xform"], np.float32).reshape(-1, 12),
+                    "inst_mesh": np.ascontiguousarray(inst["inst_mesh"], np.uint32)}
+            if ik is not None:
+                arrs["inst_key"] = np.ascontiguousarray(ik, np.uint32)
+            self._inst_keep = arrs
+            idesc = InstancingDesc(arrs["mesh_first"].ctypes.data, len(arrs["mesh_first"]) - 1,
+                                   arrs["xform"].ctypes.data, arrs["inst_mesh"].ctypes.data,
+                                   arrs["inst_key"].ctypes.data if ik is not None else None,
+                                   arrs["xform"].shape[0])
+            self._idesc = idesc
+            d.instancing = C.cast(C.pointer(idesc), _P)
         self.h = _P()
         ctx.check(lib().mrh_scene_create(ctx.h, C.byref(d), C.byref(self.h)))
         self.camera = scene.get("camera")
+
+    def trace_closest(self, d_rays, n, pitch, tmin, d_inst, d_prim, d_t, d_bary, d_stats=None):
+        """Closest hits against the whole scene (mrh_scene_trace_closest)."""
+        self.ctx.check(lib().mrh_scene_trace_closest(self.ctx.h, self.h, ptr(d_rays), n, pitch, tmin, ptr(d_inst),
+                                                     ptr(d_prim), ptr(d_t), ptr(d_bary), ptr(d_stats)))
 
     def bvh(self) -> Lbvh:
         h = _P()

@@ -655,3 +655,205 @@ void om_trace_brute(const float* tris, uint32_t n, const float* rays, uint32_t n
         hit_bary[(size_t)nrays + i] = bb[1];
     }
 }
+
+/* ------------------------------------------------------------- two-level */
+
+/* Matrix<4>::operator*(Vector<4>) (RayLib/Matrix.hpp:268-289) for one row
+ * of an affine matrix: 0 + m0 v0 + m1 v1 + m2 v2 + m3 w, left to right. */
+static float xf_row(const float* m, float x, float y, float z, float w)
+{
+    float r = 0.0f;
+    r = r + m[0] * x;
+    r = r + m[1] * y;
+    r = r + m[2] * z;
+    r = r + m[3] * w;
+    return r;
+}
+
+/* Ray::Transform (RayLib/Ray.hpp:461-465): position with w = 1, direction
+ * with w = 0, the direction not renormalised. */
+void om_xf_ray(const float m12[12], const float o[3], const float d[3], float out_o[3], float out_d[3])
+{
+    for (int r = 0; r < 3; r++) {
+        out_o[r] = xf_row(m12 + 4 * r, o[0], o[1], o[2], 1.0f);
+        out_d[r] = xf_row(m12 + 4 * r, d[0], d[1], d[2], 0.0f);
+    }
+}
+
+void om_xf_point(const float m12[12], const float p[3], float out[3])
+{
+    for (int r = 0; r < 3; r++) out[r] = xf_row(m12 + 4 * r, p[0], p[1], p[2], 1.0f);
+}
+
+/* World-to-local matrix: the inverse of the affine local-to-world matrix in
+ * double (adjugate over determinant, translation -A^-1 t), rounded to
+ * float.  Returns 0 when singular or not finite. */
+int om_affine_inverse(const float m[12], float out[12])
+{
+    const double a00 = (double)m[0], a01 = (double)m[1], a02 = (double)m[2], t0 = (double)m[3];
+    const double a10 = (double)m[4], a11 = (double)m[5], a12 = (double)m[6], t1 = (double)m[7];
+    const double a20 = (double)m[8], a21 = (double)m[9], a22 = (double)m[10], t2 = (double)m[11];
+    const double c00 = a11 * a22 - a12 * a21;
+    const double c01 = a12 * a20 - a10 * a22;
+    const double c02 = a10 * a21 - a11 * a20;
+    const double det = a00 * c00 + a01 * c01 + a02 * c02;
+    if (!(det != 0.0) || !isfinite(det)) return 0;
+    const double id = 1.0 / det;
+    double inv[3][3];
+    inv[0][0] = c00 * id;
+    inv[0][1] = (a02 * a21 - a01 * a22) * id;
+    inv[0][2] = (a01 * a12 - a02 * a11) * id;
+    inv[1][0] = c01 * id;
+    inv[1][1] = (a00 * a22 - a02 * a20) * id;
+    inv[1][2] = (a02 * a10 - a00 * a12) * id;
+    inv[2][0] = c02 * id;
+    inv[2][1] = (a01 * a20 - a00 * a21) * id;
+    inv[2][2] = (a00 * a11 - a01 * a10) * id;
+    for (int r = 0; r < 3; r++) {
+        const double tr = -(inv[r][0] * t0 + inv[r][1] * t1 + inv[r][2] * t2);
+        for (int c = 0; c < 3; c++) out[4 * r + c] = (float)inv[r][c];
+        out[4 * r + 3] = (float)tr;
+    }
+    for (int i = 0; i < 12; i++)
+        if (!isfinite(out[i])) return 0;
+    return 1;
+}
+
+/* GPUTransformSingle::LocalToWorld(AABB3f) (GPUTransformSingle.cuh:180-197):
+ * the box's eight corners (bit 0 -> x max, bit 1 -> y, bit 2 -> z) taken
+ * to the world, min / max over them. */
+void om_world_box(const float l2w[12], const float box[6], float out[6])
+{
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = 0; i < 8; i++) {
+        float x = (i & 1) ? box[3] : box[0];
+        float y = (i & 2) ? box[4] : box[1];
+        float z = (i & 4) ? box[5] : box[2];
+        for (int a = 0; a < 3; a++) {
+            float v = xf_row(l2w + 4 * a, x, y, z, 1.0f);
+            mn[a] = fminf(mn[a], v);
+            mx[a] = fmaxf(mx[a], v);
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        out[a] = mn[a];
+        out[3 + a] = mx[a];
+    }
+}
+
+/* Closest / any hit over a two-level scene: the top tree over the instance
+ * boxes walked 4-wide as om_trace4_one; at an instance the ray goes to local
+ * space by the instance's world-to-local matrix and the mesh's tree (4-wide
+ * or binary by width) is walked with the current closest t as its tmax
+ * (the reference resumes its base BVH per leaf, GPUAcceleratorBVHKC.cuh:
+ * 513-652, and intersects the leaf's accelerator with the transformed ray,
+ * GPUAcceleratorBVHKC.cuh:274-345).  Returns the scene primitive; *inst_out
+ * the instance.  Counts: top-tree internal nodes plus the meshes' nodes and
+ * triangle tests. */
+uint32_t om_trace2l_one(const om_two_level* tl, const float* tris, uint32_t width, const float o[3],
+                        const float d[3], float tmin, float* tmax_io, float bary_out[2], int any_hit,
+                        uint32_t* inst_out, uint32_t* visits, uint32_t* tests)
+{
+    uint32_t hit = OM_INVALID, hinst = OM_INVALID;
+    float tmax = *tmax_io;
+    float inv_d[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    uint32_t nv = 0, nt = 0;
+    om_stack_entry stack[128];
+    int sp = 0;
+    uint32_t next = tl->top_root4;
+    for (;;) {
+        for (;;) {
+            if (next == OM_INVALID) {
+                for (;;) {
+                    if (sp == 0) goto done2;
+                    om_stack_entry e = stack[--sp];
+                    if (!(e.t > tmax)) { next = e.node; break; }
+                }
+            }
+            if (next & OM_LEAF_BIT) {
+                uint32_t ii = tl->top_sorted_idx[next & ~OM_LEAF_BIT];
+                next = OM_INVALID;
+                float lo[3], ld[3];
+                om_xf_ray(tl->w2l + (size_t)ii * 12, o, d, lo, ld);
+                uint32_t m = tl->inst_mesh[ii];
+                uint32_t f = tl->mesh_first[m], n = tl->mesh_first[m + 1] - f;
+                float bt = tmax, bb[2] = {0.0f, 0.0f};
+                uint32_t bv = 0, bn = 0, h;
+                if (width == 4)
+                    h = om_trace4_one(tris + (size_t)f * 9, n, tl->mesh_sorted_idx + f,
+                                      tl->mesh_nodes4 + (size_t)tl->mesh_ibase[m] * 32, tl->mesh_root4[m], lo, ld,
+                                      tmin, &bt, bb, any_hit, &bv, &bn);
+                else
+                    h = om_trace_one(tris + (size_t)f * 9, n, tl->mesh_sorted_idx + f, tl->mesh_left + tl->mesh_ibase[m],
+                                     tl->mesh_right + tl->mesh_ibase[m],
+                                     tl->mesh_node_aabb + (size_t)tl->mesh_abase[m] * 6, tl->mesh_root[m], lo, ld,
+                                     tmin, &bt, bb, any_hit, &bv, &bn);
+                nv += bv;
+                nt += bn;
+                if (h != OM_INVALID) {
+                    tmax = bt;
+                    hit = f + h;
+                    hinst = ii;
+                    bary_out[0] = bb[0];
+                    bary_out[1] = bb[1];
+                    if (any_hit) goto done2;
+                }
+                continue;
+            }
+            break;
+        }
+        nv++;
+        const float* q = tl->top_nodes4 + (size_t)next * 32;
+        float key[4];
+        uint32_t slot[4], code[4];
+        int cnt = 0;
+        for (int k = 0; k < 4; k++) {
+            uint32_t c = om_f2u(q[24 + k]);
+            float box[6] = {q[k], q[4 + k], q[8 + k], q[12 + k], q[16 + k], q[20 + k]};
+            float e;
+            int h = (c != OM_INVALID) && om_intersect_aabb(o, inv_d, box, tmin, tmax, &e);
+            key[k] = h ? e : INFINITY;
+            slot[k] = (uint32_t)k;
+            code[k] = c;
+            cnt += h;
+        }
+        static const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+        for (int s = 0; s < 5; s++) {
+            int a = net[s][0], b = net[s][1];
+            if (key[a] > key[b] || (key[a] == key[b] && slot[a] > slot[b])) {
+                float tk = key[a]; key[a] = key[b]; key[b] = tk;
+                uint32_t ts = slot[a]; slot[a] = slot[b]; slot[b] = ts;
+                uint32_t tc = code[a]; code[a] = code[b]; code[b] = tc;
+            }
+        }
+        if (cnt == 0) { next = OM_INVALID; continue; }
+        for (int k = cnt - 1; k >= 1; k--) { stack[sp].node = code[k]; stack[sp].t = key[k]; sp++; }
+        next = code[0];
+    }
+done2:
+    *tmax_io = tmax;
+    if (inst_out) *inst_out = hinst;
+    if (visits) *visits = nv;
+    if (tests) *tests = nt;
+    return hit;
+}
+
+void om_trace2l_closest(const om_two_level* tl, const float* tris, uint32_t width, const float* rays, uint32_t nrays,
+                        uint32_t pitch, float tmin, uint32_t* hit_inst, uint32_t* hit_prim, float* hit_t,
+                        float* hit_bary, uint32_t* stats)
+{
+    for (uint32_t i = 0; i < nrays; i++) {
+        float o[3] = {rays[i], rays[pitch + i], rays[2 * (size_t)pitch + i]};
+        float d[3] = {rays[3 * (size_t)pitch + i], rays[4 * (size_t)pitch + i], rays[5 * (size_t)pitch + i]};
+        float tmax = rays[6 * (size_t)pitch + i];
+        float b[2] = {0.0f, 0.0f};
+        uint32_t nv, nt, ii;
+        uint32_t h = om_trace2l_one(tl, tris, width, o, d, tmin, &tmax, b, 0, &ii, &nv, &nt);
+        hit_inst[i] = (h == OM_INVALID) ? OM_INVALID : ii;
+        hit_prim[i] = h;
+        hit_t[i] = (h == OM_INVALID) ? 0.0f : tmax;
+        hit_bary[i] = (h == OM_INVALID) ? 0.0f : b[0];
+        hit_bary[(size_t)nrays + i] = (h == OM_INVALID) ? 0.0f : b[1];
+        if (stats) { stats[2 * (size_t)i] = nv; stats[2 * (size_t)i + 1] = nt; }
+    }
+}

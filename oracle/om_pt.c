@@ -496,14 +496,42 @@ static om_v3 metal_eval(const om_surface* s, om_v3 wi, const float* m, om_v3 dir
 
 /* ------------------------- next-event estimation ---------------------- */
 
-/* closest hit of one ray over the scene's traversal tree */
-static uint32_t closest_hit(const om_scene* sc, om_v3 o, om_v3 d, float tmin, float* tmax, float b[2])
+/* closest hit of one ray over the scene's traversal tree(s); *inst
+ * receives the instance (0 in a single-level scene) */
+static uint32_t closest_hit(const om_scene* sc, om_v3 o, om_v3 d, float tmin, float* tmax, float b[2],
+                            uint32_t* inst)
 {
     float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    *inst = 0;
+    if (sc->two_level)
+        return om_trace2l_one(sc->two_level, sc->tris, sc->tree_width, oo, dd, tmin, tmax, b, 0, inst, NULL, NULL);
     return sc->nodes4 ? om_trace4_one(sc->tris, sc->n, sc->sorted_idx, sc->nodes4, sc->root4, oo, dd, tmin, tmax, b,
                                       0, NULL, NULL)
                       : om_trace_one(sc->tris, sc->n, sc->sorted_idx, sc->left, sc->right, sc->node_aabb, sc->root,
                                      oo, dd, tmin, tmax, b, 0, NULL, NULL);
+}
+
+/* World-space corners of primitive prim (of instance inst in a two-level
+ * scene: the local corners taken to the world by the instance's
+ * local-to-world matrix, GPUTransformSingle::LocalToWorld,
+ * GPUTransformSingle.cuh:160-166) */
+static void prim_corners(const om_scene* sc, uint32_t prim, uint32_t inst, float out[9])
+{
+    const float* t = sc->tris + (size_t)prim * 9;
+    if (!sc->two_level) {
+        for (int k = 0; k < 9; k++) out[k] = t[k];
+        return;
+    }
+    const float* m = sc->two_level->l2w + (size_t)inst * 12;
+    for (int c = 0; c < 3; c++) om_xf_point(m, t + 3 * c, out + 3 * c);
+}
+
+/* HitKey of a hit: the instance's override where it has one */
+static uint32_t prim_key_of(const om_scene* sc, uint32_t prim, uint32_t inst)
+{
+    if (sc->two_level && sc->two_level->inst_key && sc->two_level->inst_key[inst])
+        return sc->two_level->inst_key[inst];
+    return sc->prim_key[prim];
 }
 
 /* TracerFunctions.cuh:302-309: power heuristic, exponent 2, one sample each */
@@ -539,13 +567,15 @@ static float light_pdf_hit(const float* tri, float t, om_v3 d)
  * b = (1 - x1) sqrt(x0), c = x1 sqrt(x0)).  Returns the light's primitive;
  * *pdf = solid-angle pdf x 1 / light count. */
 static uint32_t sample_light_xi(const om_scene* sc, om_v3 p, float r1, float xi0, float xi1, om_v3* dir,
-                                float* dist2, float* pdf)
+                                float* dist2, float* pdf, uint32_t* inst)
 {
     r1 *= (float)sc->n_lights;
     uint32_t idx = (uint32_t)r1;
     if (idx == sc->n_lights) idx--;
     uint32_t prim = sc->lights[idx];
-    const float* t = sc->tris + (size_t)prim * 9;
+    *inst = sc->light_inst ? sc->light_inst[idx] : 0u;
+    float t[9];
+    prim_corners(sc, prim, *inst, t);
     om_v3 p0 = v3(t[0], t[1], t[2]), p1 = v3(t[3], t[4], t[5]), p2 = v3(t[6], t[7], t[8]);
     float s1 = sqrtf(xi0);
     float s2 = xi1;
@@ -568,19 +598,21 @@ static uint32_t sample_light_xi(const om_scene* sc, om_v3 p, float r1, float xi0
     return prim;
 }
 
-static uint32_t sample_light(const om_scene* sc, om_v3 p, om_rng* rng, om_v3* dir, float* dist2, float* pdf)
+static uint32_t sample_light(const om_scene* sc, om_v3 p, om_rng* rng, om_v3* dir, float* dist2, float* pdf,
+                             uint32_t* inst)
 {
     float r1 = om_uniform(rng);
     float xi0 = om_uniform(rng);
     float xi1 = om_uniform(rng);
-    return sample_light_xi(sc, p, r1, xi0, xi1, dir, dist2, pdf);
+    return sample_light_xi(sc, p, r1, xi0, xi1, dir, dist2, pdf, inst);
 }
 
 uint32_t om_sample_light(const om_scene* sc, const float p[3], const float xi[3], float dir[3], float* dist2,
                          float* pdf)
 {
     om_v3 d;
-    uint32_t prim = sample_light_xi(sc, v3(p[0], p[1], p[2]), xi[0], xi[1], xi[2], &d, dist2, pdf);
+    uint32_t inst;
+    uint32_t prim = sample_light_xi(sc, v3(p[0], p[1], p[2]), xi[0], xi[1], xi[2], &d, dist2, pdf, &inst);
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
     return prim;
 }
@@ -682,13 +714,19 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         (*segments)++;
         float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
         float b[2];
-        uint32_t prim = sc->nodes4
+        uint32_t inst = 0;
+        uint32_t prim = sc->two_level
+                            ? om_trace2l_one(sc->two_level, sc->tris, sc->tree_width, oo, dd, tmin, &tmax, b, 0, &inst,
+                                             NULL, NULL)
+                        : sc->nodes4
                             ? om_trace4_one(sc->tris, sc->n, sc->sorted_idx, sc->nodes4, sc->root4, oo, dd, tmin,
                                             &tmax, b, 0, NULL, NULL)
                             : om_trace_one(sc->tris, sc->n, sc->sorted_idx, sc->left, sc->right,
                                            sc->node_aabb, sc->root, oo, dd, tmin, &tmax, b, 0, NULL, NULL);
         if (prim == OM_INVALID) break;
-        uint32_t key = sc->prim_key[prim];
+        float tri[9];
+        prim_corners(sc, prim, inst, tri);
+        uint32_t key = prim_key_of(sc, prim, inst);
         uint32_t batch = key >> 24, id = key & 0xFFFFFFu;
         if (batch == OM_BATCH_EMISSIVE) {
             /* PathTracerKC.cuh:337-426: Le * throughput.  With NEE, a path
@@ -699,7 +737,7 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
             om_v3 c = v3(le[0] * T.x, le[1] * T.y, le[2] * T.z);
             if (nee && depth > 1 && !spec_ray) {
                 if (!mis) break;
-                float pc = light_pdf_hit(sc->tris + (size_t)prim * 9, tmax, v3_normalize(d));
+                float pc = light_pdf_hit(tri, tmax, v3_normalize(d));
                 float pm = 1.0f / (float)sc->n_lights;
                 c = v3_scale(c, power_heuristic(prev_pdf, pc * pm));
             }
@@ -717,7 +755,7 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         }
         if (batch == OM_BATCH_NULL) break;
         if (bounce + 1 == p->max_depth) break;   /* the last segment only collects emission */
-        om_surface s = make_surface(sc->tris + (size_t)prim * 9, b[0], b[1], d);
+        om_surface s = make_surface(tri, b[0], b[1], d);
         om_v3 wi = v3_neg(v3_normalize(d));
         if (nee && sc->n_lights &&
             (batch == OM_BATCH_LAMBERT || (batch == OM_BATCH_METAL && (1.0f - sc->metal[(size_t)id * 10]) < 0.99f))) {
@@ -726,7 +764,8 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
              * hits the light it was sent to (isCorrectNEERay) */
             om_v3 ldir;
             float dist2, pl, pb;
-            uint32_t lprim = sample_light(sc, s.pos, &rng, &ldir, &dist2, &pl);
+            uint32_t linst;
+            uint32_t lprim = sample_light(sc, s.pos, &rng, &ldir, &dist2, &pl, &linst);
             om_v3 f;
             if (batch == OM_BATCH_LAMBERT) {
                 const float* a = sc->lambert + (size_t)id * 3;
@@ -744,9 +783,10 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
             if (dist2 > 0.0f && !v3_is_zero(nf)) {
                 (*segments)++;
                 float lt = FLT_MAX, lb[2];
-                uint32_t hp = closest_hit(sc, nudge(s.pos, s.geo_n), ldir, 0.0f, &lt, lb);
-                if (hp == lprim) {
-                    const float* le = sc->emissive + (size_t)(sc->prim_key[hp] & 0xFFFFFFu) * 3;
+                uint32_t hinst;
+                uint32_t hp = closest_hit(sc, nudge(s.pos, s.geo_n), ldir, 0.0f, &lt, lb, &hinst);
+                if (hp == lprim && hinst == linst) {
+                    const float* le = sc->emissive + (size_t)(prim_key_of(sc, hp, hinst) & 0xFFFFFFu) * 3;
                     contribute(pixel, sacc, v3(le[0] * nf.x, le[1] * nf.y, le[2] * nf.z));
                 }
             }

@@ -89,6 +89,45 @@ void om_trace_brute(const float* tris, uint32_t n, const float* rays, uint32_t n
                     uint32_t pitch, float tmin, uint32_t* hit_prim, float* hit_t,
                     float* hit_bary);
 
+/* ---- Two-level scenes (GPUAcceleratorBVHKC.cuh:274-345 and 513-652,
+ *      GPUTransformSingle.cuh:141-197, Ray.hpp:461-465, Matrix.hpp:268-289) ----
+ * Meshes are ranges of one triangle array in local space, each with its own
+ * LBVH; instances place a mesh by a row-major 3x4 local-to-world matrix; a
+ * top-level LBVH is built over the instances' world boxes given as the
+ * degenerate triangles (min, max, max). */
+typedef struct {
+    uint32_t        n_inst;
+    const float*    w2l;             /* [n_inst][12] world-to-local (om_affine_inverse of l2w) */
+    const float*    l2w;             /* [n_inst][12] local-to-world */
+    const uint32_t* inst_mesh;       /* [n_inst] */
+    const uint32_t* inst_key;        /* [n_inst] HitKey override (0 = the mesh's keys), or NULL */
+    const uint32_t* top_sorted_idx;  /* top tree: sorted leaf -> instance */
+    const float*    top_nodes4;
+    uint32_t        top_root4;
+    uint32_t        n_meshes;
+    const uint32_t* mesh_first;      /* [n_meshes + 1] */
+    const uint32_t* mesh_sorted_idx; /* [n]: mesh m's sorted leaves at [first[m], first[m+1]), mesh-local */
+    const float*    mesh_nodes4;     /* concatenated [sum(n_m - 1)][32] */
+    const uint32_t* mesh_left;       /* concatenated [sum(n_m - 1)], mesh-local node numbers */
+    const uint32_t* mesh_right;
+    const float*    mesh_node_aabb;  /* concatenated [sum(2 n_m - 1)][6] */
+    const uint32_t* mesh_root;       /* [n_meshes] binary root (mesh-local node number) */
+    const uint32_t* mesh_root4;      /* [n_meshes] */
+    const uint32_t* mesh_ibase;      /* [n_meshes] first internal node in nodes4 / left / right */
+    const uint32_t* mesh_abase;      /* [n_meshes] first node in node_aabb */
+} om_two_level;
+
+void     om_xf_ray(const float m12[12], const float o[3], const float d[3], float out_o[3], float out_d[3]);
+void     om_xf_point(const float m12[12], const float p[3], float out[3]);
+int      om_affine_inverse(const float l2w[12], float w2l[12]);
+void     om_world_box(const float l2w[12], const float box[6], float out[6]);
+uint32_t om_trace2l_one(const om_two_level* tl, const float* tris, uint32_t width, const float o[3],
+                        const float d[3], float tmin, float* tmax_io, float bary_out[2], int any_hit,
+                        uint32_t* inst_out, uint32_t* visits, uint32_t* tests);
+void     om_trace2l_closest(const om_two_level* tl, const float* tris, uint32_t width, const float* rays,
+                            uint32_t nrays, uint32_t pitch, float tmin, uint32_t* hit_inst, uint32_t* hit_prim,
+                            float* hit_t, float* hit_bary, uint32_t* stats);
+
 /* ---- Sampling helpers (RayLib/HemiDistribution.h, Ray.hpp:503-560,
  *      TracerFunctions.cuh) ---- */
 void  om_sincos_2pi_f(float u, float* s, float* c);
@@ -142,6 +181,12 @@ typedef struct {
     const uint32_t* lights;
     uint32_t        n_lights;
     const float*    reflect;       /* per id: albedo rgb, roughness (4) */
+    /* two-level scene (NULL: single-level): tris are the meshes' local
+     * triangles and the trees above are unused; lights are scene primitives
+     * with their instances in light_inst (instance-major order) */
+    const om_two_level* two_level;
+    uint32_t        tree_width;    /* two-level: the meshes' tree width, 4 or 2 */
+    const uint32_t* light_inst;
 } om_scene;
 
 typedef struct {

@@ -50,7 +50,15 @@ class OmScene(C.Structure):
                 ("lambert", _P), ("metal", _P), ("dielectric", _P), ("emissive", _P),
                 ("sorted_idx", _P), ("left", _P), ("right", _P), ("node_aabb", _P),
                 ("root", _U32), ("nodes4", _P), ("root4", _U32), ("lights", _P), ("n_lights", _U32),
-                ("reflect", _P)]
+                ("reflect", _P), ("two_level", _P), ("tree_width", _U32), ("light_inst", _P)]
+
+
+class OmTwoLevel(C.Structure):
+    _fields_ = [("n_inst", _U32), ("w2l", _P), ("l2w", _P), ("inst_mesh", _P), ("inst_key", _P),
+                ("top_sorted_idx", _P), ("top_nodes4", _P), ("top_root4", _U32), ("n_meshes", _U32),
+                ("mesh_first", _P), ("mesh_sorted_idx", _P), ("mesh_nodes4", _P), ("mesh_left", _P),
+                ("mesh_right", _P), ("mesh_node_aabb", _P), ("mesh_root", _P), ("mesh_root4", _P),
+                ("mesh_ibase", _P), ("mesh_abase", _P)]
 
 
 class OmFilterDesc(C.Structure):
@@ -117,6 +125,12 @@ def lib():
                                        _P, _P, _P, _P]
         L.om_trace_any.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F, _P]
         L.om_trace_brute.argtypes = [_P, _U32, _P, _U32, _U32, _F, _P, _P, _P]
+        L.om_affine_inverse.restype = C.c_int
+        L.om_affine_inverse.argtypes = [_P, _P]
+        L.om_xf_ray.argtypes = [_P, _P, _P, _P, _P]
+        L.om_xf_point.argtypes = [_P, _P, _P]
+        L.om_world_box.argtypes = [_P, _P, _P]
+        L.om_trace2l_closest.argtypes = [_P, _P, _U32, _P, _U32, _U32, _F, _P, _P, _P, _P, _P]
         L.om_bvh4_build.restype = _U32
         L.om_bvh4_build.argtypes = [_U32, _P, _P, _P, _U32, _P]
         L.om_trace4_closest.argtypes = [_P, _U32, _P, _P, _U32, _P, _U32, _U32, _F, _P, _P, _P, _P]
@@ -218,6 +232,8 @@ def ref():
         R.ref_triangle_normal.argtypes = [_P, _P]
         R.ref_triangle_aabb.argtypes = [_P, _P]
         R.ref_aabb_centroid.argtypes = [_P, _P]
+        if hasattr(R, "ref_ray_transform"):
+            R.ref_ray_transform.argtypes = [_P, _P, _P, _P, _P]
         _ref = R
     return _ref
 
@@ -327,6 +343,128 @@ class LBVH:
         return out
 
 
+def affine_inverse(l2w) -> np.ndarray:
+    """World-to-local 3x4 of a row-major local-to-world 3x4 (double adjugate,
+    rounded to float); raises on a singular matrix."""
+    m = np.ascontiguousarray(l2w, np.float32).reshape(12)
+    out = np.zeros(12, np.float32)
+    if not lib().om_affine_inverse(_ptr(m), _ptr(out)):
+        raise ValueError("singular transform")
+    return out
+
+
+def xf_ray(m12, o, d):
+    m = np.ascontiguousarray(m12, np.float32).reshape(12)
+    oo, dd = np.ascontiguousarray(o, np.float32), np.ascontiguousarray(d, np.float32)
+    ro, rd = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().om_xf_ray(_ptr(m), _ptr(oo), _ptr(dd), _ptr(ro), _ptr(rd))
+    return ro, rd
+
+
+def world_box(l2w, box) -> np.ndarray:
+    m = np.ascontiguousarray(l2w, np.float32).reshape(12)
+    b = np.ascontiguousarray(box, np.float32).reshape(6)
+    out = np.zeros(6, np.float32)
+    lib().om_world_box(_ptr(m), _ptr(b), _ptr(out))
+    return out
+
+
+class TwoLevel:
+    """Oracle two-level scene: one LBVH per mesh (local space), a top LBVH
+    over the instances' world boxes given as degenerate triangles
+    (min, max, max), world-to-local matrices by om_affine_inverse, and the
+    light list (instance-major, each mesh in primitive order).
+    scene["instancing"] = {"mesh_first": (m+1,), "xform": (k, 12),
+    "inst_mesh": (k,), "inst_key": (k,) or absent}."""
+
+    def __init__(self, scene: dict, bits: int = 30):
+        inst = scene["instancing"]
+        self.tris = np.ascontiguousarray(scene["tris"], np.float32).reshape(-1, 9)
+        self.n = self.tris.shape[0]
+        keys = np.ascontiguousarray(scene["prim_key"], np.uint32)
+        first = np.ascontiguousarray(inst["mesh_first"], np.uint32)
+        self.mesh_first = first
+        nm = len(first) - 1
+        self.l2w = np.ascontiguousarray(inst["xform"], np.float32).reshape(-1, 12)
+        ni = self.l2w.shape[0]
+        self.inst_mesh = np.ascontiguousarray(inst["inst_mesh"], np.uint32)
+        ik = inst.get("inst_key")
+        self.inst_key = None if ik is None else np.ascontiguousarray(ik, np.uint32)
+        self.w2l = np.stack([affine_inverse(self.l2w[i]) for i in range(ni)]).astype(np.float32)
+        self.meshes = [LBVH(self.tris[first[m]:first[m + 1]], bits) for m in range(nm)]
+        self.mesh_sorted_idx = np.concatenate([b.sorted_idx for b in self.meshes]).astype(np.uint32)
+        ib, ab = [], []
+        i_acc = a_acc = 0
+        for b in self.meshes:
+            ib.append(i_acc)
+            ab.append(a_acc)
+            i_acc += b.n - 1
+            a_acc += 2 * b.n - 1
+        self.mesh_ibase = np.array(ib, np.uint32)
+        self.mesh_abase = np.array(ab, np.uint32)
+        cat = lambda xs, shape: (np.concatenate(xs) if sum(len(x) for x in xs) else np.zeros(shape))
+        self.mesh_nodes4 = np.ascontiguousarray(cat([b.nodes4()[:b.n - 1] for b in self.meshes], (1, 32)),
+                                                np.float32)
+        self.mesh_left = np.ascontiguousarray(cat([b.left[:b.n - 1] for b in self.meshes], (1,)), np.uint32)
+        self.mesh_right = np.ascontiguousarray(cat([b.right[:b.n - 1] for b in self.meshes], (1,)), np.uint32)
+        self.mesh_node_aabb = np.ascontiguousarray(np.concatenate([b.node_aabb for b in self.meshes]), np.float32)
+        self.mesh_root = np.array([b.root for b in self.meshes], np.uint32)
+        self.mesh_root4 = np.array([b.root4 for b in self.meshes], np.uint32)
+        # instance boxes: the mesh's vertex extent taken to the world
+        boxes = np.zeros((ni, 9), np.float32)
+        for i in range(ni):
+            mt = self.tris[first[self.inst_mesh[i]]:first[self.inst_mesh[i] + 1]].reshape(-1, 3)
+            box = np.concatenate([mt.min(0), mt.max(0)]).astype(np.float32)
+            wb = world_box(self.l2w[i], box)
+            boxes[i] = [wb[0], wb[1], wb[2], wb[3], wb[4], wb[5], wb[3], wb[4], wb[5]]
+        self.boxes = boxes
+        self.top = LBVH(boxes, bits)
+        self.top.nodes4()
+        lights, linst = [], []
+        for i in range(ni):
+            m = self.inst_mesh[i]
+            f, e = int(first[m]), int(first[m + 1])
+            ko = 0 if self.inst_key is None else int(self.inst_key[i])
+            eff = np.full(e - f, ko, np.uint32) if ko else keys[f:e]
+            for p in np.nonzero((eff >> 24) == 4)[0]:
+                lights.append(f + int(p))
+                linst.append(i)
+        self.lights = np.array(lights, np.uint32)
+        self.light_inst = np.array(linst, np.uint32)
+        st = OmTwoLevel()
+        st.n_inst = ni
+        st.w2l, st.l2w = _ptr(self.w2l), _ptr(self.l2w)
+        st.inst_mesh = _ptr(self.inst_mesh)
+        st.inst_key = _ptr(self.inst_key)
+        st.top_sorted_idx = _ptr(self.top.sorted_idx)
+        st.top_nodes4 = _ptr(self.top.nodes4())
+        st.top_root4 = self.top.root4
+        st.n_meshes = nm
+        st.mesh_first = _ptr(self.mesh_first)
+        st.mesh_sorted_idx = _ptr(self.mesh_sorted_idx)
+        st.mesh_nodes4 = _ptr(self.mesh_nodes4)
+        st.mesh_left, st.mesh_right = _ptr(self.mesh_left), _ptr(self.mesh_right)
+        st.mesh_node_aabb = _ptr(self.mesh_node_aabb)
+        st.mesh_root, st.mesh_root4 = _ptr(self.mesh_root), _ptr(self.mesh_root4)
+        st.mesh_ibase, st.mesh_abase = _ptr(self.mesh_ibase), _ptr(self.mesh_abase)
+        self.struct = st
+
+    def trace(self, rays: np.ndarray, tmin: float = 0.0, stats: bool = False, width: int = 4):
+        """Closest hits: (inst, prim, t, bary[2, m]) (+ stats [m, 2])."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        m = rays.shape[1]
+        inst = np.zeros(m, np.uint32)
+        prim = np.zeros(m, np.uint32)
+        t = np.zeros(m, np.float32)
+        bary = np.zeros((2, m), np.float32)
+        st = np.zeros((m, 2), np.uint32) if stats else None
+        lib().om_trace2l_closest(C.byref(self.struct), _ptr(self.tris), width, _ptr(rays), m, m, tmin, _ptr(inst),
+                                 _ptr(prim), _ptr(t), _ptr(bary), _ptr(st))
+        if stats:
+            return inst, prim, t, bary, st
+        return inst, prim, t, bary
+
+
 def trace_brute(tris: np.ndarray, rays: np.ndarray, tmin: float = 0.0):
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     rays = np.ascontiguousarray(rays, dtype=np.float32)
@@ -375,6 +513,17 @@ def _scene_struct(scene: dict, bvh: LBVH, keep: dict, width: int = 4) -> OmScene
     sc.dielectric = arr("dielectric", np.float32, 4)
     sc.emissive = arr("emissive", np.float32, 3)
     sc.reflect = arr("reflect", np.float32, 4)
+    if isinstance(bvh, TwoLevel):
+        keep["two_level"] = bvh.struct
+        sc.two_level = C.cast(C.pointer(bvh.struct), _P)
+        sc.tree_width = width
+        sc.n = bvh.n
+        keep["lights"] = bvh.lights if len(bvh.lights) else np.zeros(1, np.uint32)
+        keep["light_inst"] = bvh.light_inst if len(bvh.light_inst) else np.zeros(1, np.uint32)
+        sc.lights = _ptr(keep["lights"])
+        sc.light_inst = _ptr(keep["light_inst"])
+        sc.n_lights = len(bvh.lights)
+        return sc
     sc.sorted_idx = _ptr(bvh.sorted_idx)
     sc.left = _ptr(bvh.left)
     sc.right = _ptr(bvh.right)

@@ -20,6 +20,7 @@
 #include "RayLib/HemiDistribution.h"
 #include "RayLib/Quaternion.h"
 #include "RayLib/Triangle.h"
+#include "RayLib/Matrix.h"
 
 #include <cstdint>
 
@@ -131,6 +132,22 @@ void ref_aabb_centroid(const float box[6], float out[3])
     AABB3f b(Vector3f(box[0], box[1], box[2]), Vector3f(box[3], box[4], box[5]));
     Vector3f c = b.Centroid();
     out[0] = c[0]; out[1] = c[1]; out[2] = c[2];
+}
+
+// RayLib/Ray.hpp:461-465 (Ray::Transform by a 4x4 matrix) with
+// Matrix<4>::operator*(Vector<4>) (RayLib/Matrix.hpp:268-289); m12 is the
+// row-major top 3x4 of an affine matrix, the last row 0 0 0 1.
+void ref_ray_transform(const float m12[12], const float o[3], const float d[3], float out_o[3],
+                       float out_d[3])
+{
+    Matrix4x4 m;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 4; c++) m(r, c) = m12[4 * r + c];
+    m(3, 0) = 0.0f; m(3, 1) = 0.0f; m(3, 2) = 0.0f; m(3, 3) = 1.0f;
+    RayF ray(Vector3f(d[0], d[1], d[2]), Vector3f(o[0], o[1], o[2]));
+    RayF t = ray.Transform(m);
+    out_o[0] = t.getPosition()[0]; out_o[1] = t.getPosition()[1]; out_o[2] = t.getPosition()[2];
+    out_d[0] = t.getDirection()[0]; out_d[1] = t.getDirection()[1]; out_d[2] = t.getDirection()[2];
 }
 
 }  // extern "C"

@@ -1,0 +1,148 @@
+"""Two-level scenes on the GPU, through the C-ABI, against the oracle.
+
+Ray casts (mrh_scene_trace_closest) return the oracle's instance and
+primitive ids, bit-identical t and barycentrics, and the same node / test
+counts (top tree plus mesh trees) with both mesh-tree widths.  Renders
+trace the same segments and match the oracle image within relMSE 1e-4 (the
+per-pixel sums are float atomics), with and without next-event estimation.
+"""
+import numpy as np
+import pytest
+
+import instancing_scenes as I
+from meturay_amd import native as N
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def relmse(a, b, eps=1e-2):
+    return float(np.mean((a - b) ** 2 / (b ** 2 + eps)))
+
+
+@pytest.fixture(params=[4, 2])
+def width(request, ctx):
+    old = ctx.tune_get("trace_width")
+    ctx.tune("trace_width", request.param)
+    yield request.param
+    ctx.tune("trace_width", old)
+
+
+def gpu_trace(ctx, scene, rays, tmin=0.0):
+    m = rays.shape[1]
+    dr = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).cuda()
+    inst = torch.empty(m, dtype=torch.int32, device="cuda")
+    prim = torch.empty(m, dtype=torch.int32, device="cuda")
+    t = torch.empty(m, dtype=torch.float32, device="cuda")
+    bary = torch.empty(2 * m, dtype=torch.float32, device="cuda")
+    st = torch.empty(2 * m, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    scene.trace_closest(dr, m, m, tmin, inst, prim, t, bary, st)
+    u = lambda x: x.cpu().numpy().view(np.uint32)
+    return (u(inst), u(prim), t.cpu().numpy(), bary.cpu().numpy().reshape(2, m),
+            u(st).reshape(m, 2))
+
+
+def test_scene_trace_two_level_matches_oracle(ctx, oracle, width):
+    sc = I.instanced_room(n_spheres=9)
+    scene = N.Scene(ctx, sc)
+    rays = I.rays_into(sc, 20000)
+    gi, gp, gt, gb, gs = gpu_trace(ctx, scene, rays)
+    ei, ep, et, eb, es = oracle.TwoLevel(sc).trace(rays, 0.0, stats=True, width=width)
+    assert (ep != 0xFFFFFFFF).mean() > 0.99
+    assert np.array_equal(gp, ep) and np.array_equal(gi, ei)
+    assert np.array_equal(gt, et) and np.array_equal(gb, eb)
+    assert np.array_equal(gs, es)
+    scene.close()
+
+
+def test_scene_trace_single_level_matches_oracle(ctx, oracle, width):
+    sc = N.gen_cornell(3)
+    scene = N.Scene(ctx, sc)
+    rays = I.rays_into(sc, 5000, seed=4)
+    gi, gp, gt, gb, gs = gpu_trace(ctx, scene, rays)
+    ep, et, eb, es = oracle.LBVH(sc["tris"]).trace(rays, 0.0, stats=True, width=width)
+    assert np.array_equal(gp, ep) and np.array_equal(gt, et) and np.array_equal(gb, eb)
+    assert np.array_equal(gs, es)
+    assert np.array_equal(gi, np.where(ep == 0xFFFFFFFF, 0xFFFFFFFF, 0).astype(np.uint32))
+    scene.close()
+
+
+def _render_pair(ctx, oracle, sc, w, h, S, depth, rr, seed, nee=0, mis=0):
+    scene = N.Scene(ctx, sc)
+    img = torch.zeros(h * w * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    st = scene.render(img.data_ptr(), w, h, S, depth, rr, seed, nee=nee, mis=mis)
+    g = img.cpu().numpy().reshape(h, w, 3)
+    scene.close()
+    c = sc["camera"]
+    cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], w, h)
+    ref, seg = oracle.render(sc, oracle.TwoLevel(sc), cam, w, h, S, depth, rr, seed, threads=8,
+                             tree_width=ctx.tune_get("trace_width"), nee=nee, mis=mis)
+    return g, st, ref, seg
+
+
+@pytest.mark.parametrize("nee,mis", [(0, 0), (1, 0), (1, 1)])
+def test_render_two_level_matches_oracle(ctx, oracle, width, nee, mis):
+    sc = I.instanced_room()
+    g, st, ref, seg = _render_pair(ctx, oracle, sc, 64, 48, 2, 10, 3, 5, nee, mis)
+    assert st.segments == seg
+    assert ref.mean() > 0.02
+    assert relmse(g, ref) <= 1e-4
+
+
+def test_render_identity_split_equals_single_level(ctx):
+    base = N.gen_cornell(3)
+    two = I.split_identity(base, [300, 700, 1100])
+    imgs = []
+    for sc in (base, two):
+        scene = N.Scene(ctx, sc)
+        img = torch.zeros(48 * 64 * 3, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        st = scene.render(img.data_ptr(), 64, 48, 2, 8, 3, 9, nee=1, mis=1)
+        imgs.append((img.cpu().numpy(), st.segments))
+        scene.close()
+    assert imgs[0][1] == imgs[1][1]
+    assert np.allclose(imgs[0][0], imgs[1][0], rtol=1e-5, atol=1e-6)
+
+
+def test_two_level_errors(ctx):
+    sc = I.instanced_room(n_spheres=2)
+
+    def bad(**kw):
+        s = dict(sc)
+        inst = dict(sc["instancing"])
+        inst.update(kw)
+        s["instancing"] = inst
+        with pytest.raises(N.MrhError):
+            N.Scene(ctx, s)
+
+    xf = sc["instancing"]["xform"].copy()
+    xf[1, :] = 0.0
+    bad(xform=xf)                                                    # singular
+    xf = sc["instancing"]["xform"].copy()
+    xf[2, 3] = np.inf
+    bad(xform=xf)                                                    # not finite
+    first = sc["instancing"]["mesh_first"].copy()
+    first[-1] += 1
+    bad(mesh_first=first)                                            # ranges miss the triangles
+    first = sc["instancing"]["mesh_first"].copy()
+    first[2] = first[1]
+    bad(mesh_first=first)                                            # empty mesh
+    im = sc["instancing"]["inst_mesh"].copy()
+    im[0] = 3
+    bad(inst_mesh=im)                                                # no such mesh
+    ik = sc["instancing"]["inst_key"].copy()
+    ik[1] = N.key(4, 7)
+    bad(inst_key=ik)                                                 # no such material
+    scene = N.Scene(ctx, sc)
+    with pytest.raises(N.MrhError):
+        scene.bvh()                                                  # no single LBVH
+    server = N.Server(ctx, hbm_budget=1 << 30, wave_paths=1 << 16, max_slots=4)
+    img = torch.zeros(16 * 16 * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    with pytest.raises(N.MrhError):
+        server.submit(scene, img, 16, 16, 1, 4, 1, 0)                # the engine takes single-level scenes
+    server.close()
+    scene.close()

@@ -131,6 +131,31 @@ def nee_leg(N, ctx, scene, img, torch):
             "segments": int(st.segments)}
 
 
+def instanced_leg(N, ctx, img, torch):
+    """The sphere grid as a two-level scene (16 displaced-sphere meshes,
+    4096 instances under y rotations, same camera, triangle count and
+    material mix): build time of the mesh trees and the top tree, then one
+    warm-up and one timed frame of the main workload."""
+    sc = N.gen_instanced_sphere_grid(GRID, STACKS, SLICES, SCENE_SEED, 16)
+    t0 = time.perf_counter()
+    scene = N.Scene(ctx, sc)
+    build = time.perf_counter() - t0
+    scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    scene.close()
+    n_inst = len(sc["instancing"]["inst_mesh"])
+    log(f"instanced: {n_inst} instances of 16 meshes, build {build * 1e3:.1f} ms, "
+        f"{st.segments} segments in {dt * 1e3:.1f} ms")
+    return {"workload": f"two-level sphere grid: 16 meshes x {n_inst} instances "
+                        f"({n_inst * 2 * SLICES * (STACKS - 1)} world triangles), same frame",
+            "Mrays_per_s": round(st.segments / dt / 1e6, 3), "ms": round(dt * 1e3, 3),
+            "segments": int(st.segments), "build_ms": round(build * 1e3, 2)}
+
+
 SERVE_RES = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
 SERVE_SPP = [1, 4, 16, 64]
 
@@ -265,6 +290,7 @@ def main():
                     help="fixed launch shape width,lds,blocks_per_cu instead of autotuning (e.g. 4,8,8)")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-tracer leg")
     ap.add_argument("--no-nee", action="store_true", help="skip the next-event-estimation leg")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the two-level (instanced) leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving leg requests (0 = skip)")
     ap.add_argument("--rate", type=float, default=40.0, help="serving leg arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -373,6 +399,9 @@ def main():
     nee = None
     if not args.no_nee:
         nee = nee_leg(N, ctx, scene, img, torch)
+    instanced = None
+    if not args.no_instanced:
+        instanced = instanced_leg(N, ctx, img, torch)
     serving = None
     if args.requests > 0:
         serving = serving_leg(N, ctx, torch, args.requests, args.rate)
@@ -414,6 +443,7 @@ def main():
             "autotune": tune,
             "guided": guided,
             "nee": nee,
+            "instanced": instanced,
             "kernel_ms_per_frame": {"trace": round(trace_ms / args.steps, 3),
                                     "shade": round(shade_ms / args.steps, 3),
                                     "partition": round((sort_ms + hist_ms) / args.steps, 3)},

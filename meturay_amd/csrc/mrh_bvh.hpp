@@ -578,6 +578,173 @@ __device__ __forceinline__ TraceResult traverse_two_level(Stack& st, const float
     }
 }
 
+// The same walk as traverse_two_level with 4-wide mesh trees, in one loop:
+// entering an instance swaps the ray to local space and the node / leaf
+// bases to the mesh's, and records the stack depth; when the stack drains
+// back to that depth the world ray is reloaded (world(o, d)) and the top
+// tree continues.  Mesh entries sit above the top-level ones, so the visit
+// order, culling and counts are those of the nested walk; one copy of the
+// node test keeps the register count of the single-level kernel.
+template <bool ANY, bool STATS, class Stack, class World>
+__device__ __forceinline__ TraceResult traverse_two_level4(Stack& st, const float4* __restrict__ top4,
+                                                           const float4* __restrict__ top_leaves, uint32_t top_root,
+                                                           uint32_t n_inst, const float4* __restrict__ inst,
+                                                           const float4* __restrict__ mesh_nodes,
+                                                           const float4* __restrict__ leaves, V3 o, V3 d,
+                                                           float tmin, float tmax, uint32_t& hit_inst,
+                                                           World&& world)
+{
+    TraceResult r;
+    r.leaf = NO_HIT;
+    r.t = tmax;
+    r.ba = r.bb = 0.0f;
+    r.visits = r.tests = 0;
+    hit_inst = NO_HIT;
+    V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    int sp = 0;
+    int base = -1;                  // stack depth at the current instance's entry (-1: top tree)
+    uint32_t cur = 0;               // current instance
+    uint32_t n_cur = n_inst;        // leaves of the current tree
+    const float4* nodes = top4;
+    const float4* lv = top_leaves;
+    uint32_t next = top_root;
+    bool done = false;
+    for (;;) {
+        for (;;) {
+            if (next == NO_HIT) {
+                bool found = false;
+                while (!found) {
+                    if (sp == base) {   // the instance's tree is done: back to the world ray
+                        world(o, d);
+                        inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                        base = -1;
+                        nodes = top4;
+                        lv = top_leaves;
+                        n_cur = n_inst;
+                    }
+                    if (sp == 0) break;
+                    --sp;
+                    uint32_t nn;
+                    float et;
+                    st.get(sp, nn, et);
+                    if (!(et > r.t)) {
+                        next = nn;
+                        found = true;
+                    }
+                }
+                if (!found) {
+                    done = true;
+                    break;
+                }
+            }
+            if (next & LEAF_BIT) {
+                const uint32_t li = next & ~LEAF_BIT;
+                next = NO_HIT;
+                if (li >= n_cur) continue;   // unreachable on a well-formed tree
+                if (base >= 0) {
+                    const float4 a = lv[3 * li + 0];
+                    const float4 b = lv[3 * li + 1];
+                    const float4 c = lv[3 * li + 2];
+                    float t, ba, bb;
+                    if (STATS) r.tests++;
+                    if (intersect_tri(o, d, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, ba, bb) &&
+                        t >= tmin && t < r.t) {
+                        r.t = t;
+                        r.leaf = li;     // mesh-local here; made global at the end
+                        r.ba = ba;
+                        r.bb = bb;
+                        hit_inst = cur;
+                        if (ANY) break;
+                    }
+                    continue;
+                }
+                const uint32_t ii = __float_as_uint(lv[3 * (size_t)li].w);
+                if (ii >= n_inst) continue;
+                const float4* rec = inst + INST_F4 * (size_t)ii;
+                const float4 w0 = rec[0], w1 = rec[1], w2 = rec[2];
+                const float4 m6 = rec[6], m7 = rec[7];
+                o = xf_point(w0, w1, w2, o);
+                d = xf_dir(w0, w1, w2, d);
+                inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                base = sp;
+                cur = ii;
+                n_cur = __float_as_uint(m7.w);
+                nodes = mesh_nodes + 8 * (size_t)__float_as_uint(m7.x);
+                lv = leaves + 3 * (size_t)__float_as_uint(m6.z);
+                next = __float_as_uint(m7.y);
+                continue;
+            }
+            if (next + 1u >= n_cur) {   // internal index out of range: unreachable
+                next = NO_HIT;
+                continue;
+            }
+            break;
+        }
+        if (done || (ANY && r.leaf != NO_HIT)) break;
+        if (STATS) r.visits++;
+        const float4* nd = nodes + 8 * (size_t)next;
+        const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5], cc = nd[6];
+        uint32_t c0 = __float_as_uint(cc.x), c1 = __float_as_uint(cc.y), c2 = __float_as_uint(cc.z),
+                 c3 = __float_as_uint(cc.w);
+        float k0, k1, k2, k3;
+        bool h0 = intersect_box(o, inv, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tmin, r.t, k0) && c0 != NO_HIT;
+        bool h1 = intersect_box(o, inv, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tmin, r.t, k1) && c1 != NO_HIT;
+        bool h2 = intersect_box(o, inv, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tmin, r.t, k2) && c2 != NO_HIT;
+        bool h3 = intersect_box(o, inv, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tmin, r.t, k3) && c3 != NO_HIT;
+        const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+        if (cnt == 0) {
+            next = NO_HIT;
+            continue;
+        }
+        k0 = h0 ? k0 : INFINITY;
+        k1 = h1 ? k1 : INFINITY;
+        k2 = h2 ? k2 : INFINITY;
+        k3 = h3 ? k3 : INFINITY;
+        uint32_t s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+        auto cx = [](float& ka, uint32_t& sa, uint32_t& ca, float& kb, uint32_t& sb, uint32_t& cb) {
+            const bool sw = (ka > kb) || (ka == kb && sa > sb);
+            const float tk = sw ? kb : ka;
+            kb = sw ? ka : kb;
+            ka = tk;
+            const uint32_t ts = sw ? sb : sa;
+            sb = sw ? sa : sb;
+            sa = ts;
+            const uint32_t tc = sw ? cb : ca;
+            cb = sw ? ca : cb;
+            ca = tc;
+        };
+        cx(k0, s0, c0, k1, s1, c1);
+        cx(k2, s2, c2, k3, s3, c3);
+        cx(k0, s0, c0, k2, s2, c2);
+        cx(k1, s1, c1, k3, s3, c3);
+        cx(k1, s1, c1, k2, s2, c2);
+        if (cnt >= 4 && sp < Stack::CAP) st.put(sp++, c3, k3);
+        if (cnt >= 3 && sp < Stack::CAP) st.put(sp++, c2, k2);
+        if (cnt >= 2 && sp < Stack::CAP) st.put(sp++, c1, k1);
+        next = c0;
+    }
+    // any hit: leave with the hit (the leaf is made global below)
+    if (r.leaf != NO_HIT) r.leaf += __float_as_uint(inst[INST_F4 * (size_t)hit_inst + 6].z);
+    return r;
+}
+
+// Either two-level walk, by mesh-tree width (world reloads the world ray)
+template <int W, bool ANY, bool STATS, class Stack, class World>
+__device__ __forceinline__ TraceResult two_level_w(Stack& st, const float4* __restrict__ top4,
+                                                   const float4* __restrict__ top_leaves, uint32_t top_root,
+                                                   uint32_t n_inst, const float4* __restrict__ inst,
+                                                   const float4* __restrict__ mesh_nodes,
+                                                   const float4* __restrict__ leaves, V3 o, V3 d, float tmin,
+                                                   float tmax, uint32_t& hit_inst, World&& world)
+{
+    if constexpr (W == 4)
+        return traverse_two_level4<ANY, STATS>(st, top4, top_leaves, top_root, n_inst, inst, mesh_nodes, leaves, o, d,
+                                               tmin, tmax, hit_inst, world);
+    else
+        return traverse_two_level<W, ANY, STATS>(st, top4, top_leaves, top_root, n_inst, inst, mesh_nodes, leaves, o,
+                                                 d, tmin, tmax, hit_inst);
+}
+
 // Stack entries a two-level walk needs: the top tree's 4-wide need plus the
 // deepest mesh tree's need at width w.
 inline uint32_t two_level_stack_need(uint32_t top_depth, uint32_t mesh_depth, uint32_t w)

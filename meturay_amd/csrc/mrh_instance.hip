@@ -303,8 +303,12 @@ __global__ __launch_bounds__(256) void k_scene_trace(const float4* __restrict__ 
         TraceResult r;
         uint32_t ii = 0;
         if constexpr (TWO) {
-            r = traverse_two_level<W, false, true>(st, nodes, top_leaves, root, n_top, inst, mesh_nodes, leaves, o, d,
-                                                   tmin, tmax, ii);
+            auto world = [&](V3& wo, V3& wd) {
+                wo = v3(rays[i], rays[pitch + i], rays[2 * (size_t)pitch + i]);
+                wd = v3(rays[3 * (size_t)pitch + i], rays[4 * (size_t)pitch + i], rays[5 * (size_t)pitch + i]);
+            };
+            r = two_level_w<W, false, true>(st, nodes, top_leaves, root, n_top, inst, mesh_nodes, leaves, o, d, tmin,
+                                            tmax, ii, world);
         } else {
             r = traverse_w<W, false, true>(st, nodes, leaves, root, n_top, o, d, tmin, tmax);
         }

@@ -139,7 +139,7 @@ int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, cons
 // tree in local space (traverse_two_level); writes the instance too, and the
 // instance's HitKey override where it has one.
 template <int W, int CAP, bool STATS, int L>
-__global__ __launch_bounds__(256) void k_trace_pt2(const float4* __restrict__ top4,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_pt2(const float4* __restrict__ top4,
                                                    const float4* __restrict__ top_leaves, uint32_t top_root,
                                                    uint32_t n_inst, const float4* __restrict__ inst,
                                                    const float4* __restrict__ mesh_nodes,
@@ -159,14 +159,18 @@ __global__ __launch_bounds__(256) void k_trace_pt2(const float4* __restrict__ to
         const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
         TraceResult r;
         uint32_t ii;
+        auto world = [&](V3& wo, V3& wd) {
+            wo = v3(in.ox[i], in.oy[i], in.oz[i]);
+            wd = v3(in.dx[i], in.dy[i], in.dz[i]);
+        };
         if constexpr (L > 0) {
             LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
-            r = traverse_two_level<W, false, STATS>(st, top4, top_leaves, top_root, n_inst, inst, mesh_nodes, leaves,
-                                                    o, d, tmin, tmax, ii);
+            r = two_level_w<W, false, STATS>(st, top4, top_leaves, top_root, n_inst, inst, mesh_nodes, leaves, o, d,
+                                             tmin, tmax, ii, world);
         } else {
             PrivateStack<CAP> st;
-            r = traverse_two_level<W, false, STATS>(st, top4, top_leaves, top_root, n_inst, inst, mesh_nodes, leaves,
-                                                    o, d, tmin, tmax, ii);
+            r = two_level_w<W, false, STATS>(st, top4, top_leaves, top_root, n_inst, inst, mesh_nodes, leaves, o, d,
+                                             tmin, tmax, ii, world);
         }
         hit.leaf[i] = r.leaf;
         hit.t[i] = r.t;

@@ -394,6 +394,45 @@ def gen_sphere_grid(grid: int = 64, stacks: int = 16, slices: int = 16, seed: in
     return scene_buffers(lib().mrh_gen_sphere_grid, grid, stacks, slices, seed)
 
 
+def gen_instanced_sphere_grid(grid: int = 64, stacks: int = 16, slices: int = 16, seed: int = 2,
+                              meshes: int = 16):
+    """The sphere grid as a two-level scene: the first `meshes` displaced
+    spheres of gen_sphere_grid, moved to the origin, are the meshes; every
+    grid cell is an instance of mesh (cell % meshes), placed at the cell's
+    centre under a rotation about y by a seeded angle, with the flat grid's
+    material for that cell as its key override.  Same camera, world
+    triangle count and material mix as gen_sphere_grid."""
+    import math
+
+    import numpy as np
+
+    flat = gen_sphere_grid(grid, stacks, slices, seed)
+    per = 2 * slices * (stacks - 1)
+    cells = grid * grid
+    meshes = min(meshes, cells)
+    tris = flat["tris"][: meshes * per].reshape(meshes, per, 3, 3).copy()
+    centre = [((i // grid) - (grid - 1) * 0.5, (i % grid) - (grid - 1) * 0.5) for i in range(cells)]
+    for m in range(meshes):
+        tris[m, :, :, 0] -= np.float32(centre[m][0])
+        tris[m, :, :, 2] -= np.float32(centre[m][1])
+    rng = np.random.default_rng(seed)
+    xf = np.zeros((cells, 12), np.float32)
+    for i in range(cells):
+        a = rng.uniform(0.0, 2.0 * math.pi)
+        c, s_ = math.cos(a), math.sin(a)
+        xf[i] = [c, 0.0, s_, centre[i][0], 0.0, 1.0, 0.0, 0.0, -s_, 0.0, c, centre[i][1]]
+    out = dict(flat)
+    out["tris"] = tris.reshape(-1, 9)
+    out["prim_key"] = flat["prim_key"][: meshes * per].copy()
+    out["instancing"] = {
+        "mesh_first": (np.arange(meshes + 1) * per).astype(np.uint32),
+        "xform": xf,
+        "inst_mesh": (np.arange(cells) % meshes).astype(np.uint32),
+        "inst_key": flat["prim_key"][::per][:cells].astype(np.uint32),
+    }
+    return out
+
+
 def gen_random_tris(n: int, seed: int = 1):
     import numpy as np
 

@@ -355,6 +355,12 @@ int32_t mrh_filter_resolve(mrh_context* ctx, const float* d_weighted, const floa
 typedef struct mrh_scene_file mrh_scene_file;
 #define MRH_SCENE_FILE_STRICT     0u
 #define MRH_SCENE_FILE_SUBSTITUTE 1u
+/* Instanced (may be or-ed with the above): every primitive a surface or
+ * light surface uses becomes one mesh in local space and every surface
+ * primitive an instance with the surface's transform and material key, so
+ * the scene is two-level (mrh_scene_desc.instancing) instead of baked into
+ * world-space triangles. */
+#define MRH_SCENE_FILE_INSTANCED  2u
 int32_t mrh_scene_file_load(const char* path, uint32_t flags, mrh_scene_file** out, char* msg, uint32_t msg_len);
 /* pointers into the file object's arrays (valid until it is destroyed) */
 int32_t mrh_scene_file_desc(const mrh_scene_file* file, mrh_scene_desc* out);

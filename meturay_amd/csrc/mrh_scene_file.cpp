@@ -38,6 +38,12 @@ struct mrh_scene_file {
     std::vector<float> lambert, metal, dielectric, emissive, reflect;
     std::vector<mrh_camera_desc> cameras;
     std::string warnings;
+    // instanced mode: meshes (one per primitive in first-use order) and one
+    // instance per surface primitive / light surface
+    bool instanced = false;
+    std::vector<uint32_t> mesh_first, inst_mesh, inst_key;
+    std::vector<float> xform;
+    mrh_instancing_desc inst_desc = {};
 };
 
 namespace {
@@ -151,7 +157,8 @@ struct MatRef {
 
 class Loader {
 public:
-    Loader(const std::string& path, bool relaxed, mrh_scene_file& out) : path(path), relaxed(relaxed), out(out)
+    Loader(const std::string& path, bool relaxed, bool instanced, mrh_scene_file& out)
+        : path(path), relaxed(relaxed), instanced(instanced), out(out)
     {
         const size_t slash = path.find_last_of('/');
         dir = (slash == std::string::npos) ? std::string() : path.substr(0, slash + 1);
@@ -176,11 +183,16 @@ public:
         read_light_surfaces(root);
         read_cameras(root);
         if (out.keys.empty()) throw std::runtime_error("scene has no triangles");
+        if (instanced) {
+            out.instanced = true;
+            out.mesh_first.push_back((uint32_t)out.keys.size());
+        }
     }
 
 private:
     std::string path, dir;
-    bool relaxed;
+    bool relaxed, instanced;
+    std::map<uint32_t, uint32_t> mesh_of;   // instanced mode: primitive id -> mesh
     mrh_scene_file& out;
     std::map<uint32_t, Mat4> transforms;
     std::map<uint32_t, float> media_ior;
@@ -513,6 +525,31 @@ private:
         return r;
     }
 
+    // Instanced mode: the primitive's triangles become a mesh (once, in
+    // local space, degenerate ones dropped; keys of its first use, which the
+    // instances override anyway) and the surface an instance of it.
+    void instance(uint32_t prim, const std::vector<float>& local, const Mat4& m, uint32_t key)
+    {
+        auto it = mesh_of.find(prim);
+        if (it == mesh_of.end()) {
+            const uint32_t before = (uint32_t)out.keys.size();
+            emit(local, identity(), key);
+            if (out.keys.size() == before) return;   // nothing but degenerate triangles
+            it = mesh_of.emplace(prim, (uint32_t)out.mesh_first.size()).first;
+            out.mesh_first.push_back(before);
+        }
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 4; c++) out.xform.push_back((float)m.m[4 * r + c]);
+        out.inst_mesh.push_back(it->second);
+        out.inst_key.push_back(key);
+    }
+
+    void place(uint32_t prim, const std::vector<float>& local, const Mat4& m, uint32_t key)
+    {
+        if (instanced) instance(prim, local, m, key);
+        else emit(local, m, key);
+    }
+
     void read_surfaces(const Value& root)
     {
         const Value* sec = root.get("Surfaces");
@@ -527,7 +564,7 @@ private:
                 if (mi == materials.end()) throw std::runtime_error("unknown material " + std::to_string(mats[k]));
                 auto pi = prims.find(prs[k]);
                 if (pi == prims.end()) throw std::runtime_error("unknown primitive " + std::to_string(prs[k]));
-                emit(pi->second, m, MRH_KEY(mi->second.batch, mi->second.index));
+                place(prs[k], pi->second, m, MRH_KEY(mi->second.batch, mi->second.index));
             }
         }
     }
@@ -544,7 +581,7 @@ private:
             auto pi = prims.find(li->second.primitive);
             if (pi == prims.end()) throw std::runtime_error("unknown light primitive " + std::to_string(li->second.primitive));
             const uint32_t e = add(out.emissive, {li->second.radiance[0], li->second.radiance[1], li->second.radiance[2]}, 3);
-            emit(pi->second, transform_of(node), MRH_KEY(MRH_BATCH_EMISSIVE, e));
+            place(li->second.primitive, pi->second, transform_of(node), MRH_KEY(MRH_BATCH_EMISSIVE, e));
         }
     }
 
@@ -611,7 +648,7 @@ int32_t mrh_scene_file_load(const char* path, uint32_t flags, mrh_scene_file** o
     copy_msg(msg, msg_len, "");
     mrh_scene_file* f = new mrh_scene_file();
     try {
-        Loader(path, (flags & MRH_SCENE_FILE_SUBSTITUTE) != 0, *f).run();
+        Loader(path, (flags & MRH_SCENE_FILE_SUBSTITUTE) != 0, (flags & MRH_SCENE_FILE_INSTANCED) != 0, *f).run();
     } catch (const Unsupported& e) {
         copy_msg(msg, msg_len, std::string(path) + ": " + e.what());
         delete f;
@@ -643,6 +680,16 @@ int32_t mrh_scene_file_desc(const mrh_scene_file* f, mrh_scene_desc* d)
     d->reflect = f->reflect.data();
     d->n_reflect = (uint32_t)(f->reflect.size() / 4);
     d->morton_bits = 30;
+    if (f->instanced) {
+        mrh_instancing_desc& I = const_cast<mrh_scene_file*>(f)->inst_desc;
+        I.mesh_first = f->mesh_first.data();
+        I.n_meshes = (uint32_t)f->mesh_first.size() - 1;
+        I.xform = f->xform.data();
+        I.inst_mesh = f->inst_mesh.data();
+        I.inst_key = f->inst_key.data();
+        I.n_instances = (uint32_t)f->inst_mesh.size();
+        d->instancing = &I;
+    }
     return MRH_OK;
 }
 

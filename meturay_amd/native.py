@@ -453,17 +453,19 @@ def gen_random_rays(n: int, seed: int = 1):
     return r
 
 
-def load_scene_file(path: str, substitute: bool = False) -> dict:
+def load_scene_file(path: str, substitute: bool = False, instanced: bool = False) -> dict:
     """Reads a scene file (mrh_scene_file_load) into the dict that Scene
     takes: tris, prim_key, material tables, "camera" (the first camera),
-    "cameras" (all, as dicts) and "warnings"."""
+    "cameras" (all, as dicts) and "warnings"; with instanced, the meshes and
+    "instancing" of a two-level scene."""
     import math
 
     import numpy as np
 
     h = _P()
     msg = C.create_string_buffer(1024)
-    rc = lib().mrh_scene_file_load(os.fsencode(path), 1 if substitute else 0, C.byref(h), msg, 1024)
+    flags = (1 if substitute else 0) | (2 if instanced else 0)
+    rc = lib().mrh_scene_file_load(os.fsencode(path), flags, C.byref(h), msg, 1024)
     if rc != OK:
         raise MrhError(rc, msg.value.decode())
     try:
@@ -486,6 +488,15 @@ def load_scene_file(path: str, substitute: bool = False) -> dict:
             "emissive": take(d.emissive, d.n_emissive, 3),
             "reflect": take(d.reflect, d.n_reflect, 4),
         }
+        if d.instancing:
+            idesc = C.cast(d.instancing, C.POINTER(InstancingDesc)).contents
+            ni = idesc.n_instances
+            out["instancing"] = {
+                "mesh_first": take(idesc.mesh_first, idesc.n_meshes + 1, 1, np.uint32).reshape(-1),
+                "xform": take(idesc.xform, ni, 12),
+                "inst_mesh": take(idesc.inst_mesh, ni, 1, np.uint32).reshape(-1),
+                "inst_key": take(idesc.inst_key, ni, 1, np.uint32).reshape(-1),
+            }
         n = _U32()
         lib().mrh_scene_file_cameras(h, None, 0, C.byref(n))
         cams = (CameraDesc * max(n.value, 1))()

@@ -146,3 +146,15 @@ def test_two_level_errors(ctx):
         server.submit(scene, img, 16, 16, 1, 4, 1, 0)                # the engine takes single-level scenes
     server.close()
     scene.close()
+
+
+def test_scene_file_instanced_render_matches_oracle(ctx, oracle):
+    """A scene file loaded as a two-level scene (MRH_SCENE_FILE_INSTANCED)
+    renders as the oracle's two-level restatement does."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "scenes", "room.json")
+    sc = N.load_scene_file(path, instanced=True)
+    g, st, ref, seg = _render_pair(ctx, oracle, sc, 48, 48, 2, 8, 3, 2, 1, 1)
+    assert st.segments == seg
+    assert ref.mean() > 0.01
+    assert relmse(g, ref) <= 1e-4

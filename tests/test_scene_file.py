@@ -189,3 +189,25 @@ def test_reference_scenes_load():
     assert len(door["cameras"]) == 3 and "Spherical" in door["warnings"]
     with pytest.raises(N.MrhError):
         N.load_scene_file(os.path.join(REF, "Scenes", "veachDoor.json"))
+
+
+def test_room_instanced_mode():
+    """MRH_SCENE_FILE_INSTANCED: each used primitive becomes one local-space
+    mesh and each surface primitive an instance; placing the instances
+    (float64 product) gives the world triangles and keys of the baked load."""
+    import instancing_scenes as I
+    sc = N.load_scene_file(os.path.join(HERE, "room.json"), instanced=True)
+    tris, keys, _ = expected_room()
+    inst = sc["instancing"]
+    assert inst["mesh_first"][0] == 0 and inst["mesh_first"][-1] == len(sc["tris"])
+    # primitives shared by several surfaces are stored once
+    assert len(inst["inst_mesh"]) > len(inst["mesh_first"]) - 1
+    assert len(sc["tris"]) < len(tris)
+    flat = I.flatten(sc)
+    assert np.array_equal(flat["prim_key"], keys)
+    assert np.allclose(flat["tris"], tris, atol=2e-6)
+    # one instance per surface primitive (8) and per light surface (1), in
+    # file order; primitives 4 and 5 are used twice, so 7 meshes
+    assert len(inst["inst_mesh"]) == 9 and len(inst["mesh_first"]) - 1 == 7
+    assert inst["inst_mesh"][6] == inst["inst_mesh"][5] and inst["inst_mesh"][7] == inst["inst_mesh"][4]
+    assert ((inst["inst_key"] >> 24) == 4).tolist() == [False] * 8 + [True]

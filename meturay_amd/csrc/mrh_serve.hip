@@ -46,8 +46,14 @@ constexpr uint32_t RES_LIVE = 1, RES_SEG = 1 + SERVE_MAX_SLOTS, RES_PATHS = 1 + 
 // Device view of one admitted request
 struct SlotDev {
     const float4* tnodes;   // binary traversal nodes
-    const float4* nodes4;   // 4-wide traversal nodes
-    uint32_t root, root4, n_leaves, pad2;
+    const float4* nodes4;   // 4-wide traversal nodes (two-level: the top tree's)
+    uint32_t root, root4, n_leaves, pad2;   // (two-level: n_leaves = instances)
+    // two-level scenes: top-tree leaf records, instance records, mesh trees
+    const float4* top_leaves;
+    const float4* inst;
+    const float4* mesh_nodes2;
+    const float4* mesh_nodes4;
+    uint32_t two, pad3, pad4, pad5;
     ShadeReq q;
     GuideArgs ga;
     Cam cam;
@@ -112,6 +118,7 @@ __global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__
     const size_t oslot = (size_t)lb * blockDim.x + threadIdx.x;
     for (uint32_t i = lb * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const SlotDev& sd = slots[in_slot[i]];
+        if (sd.two) continue;   // traced by k_serve_trace2
         const V3 o = v3(in.ox[i], in.oy[i], in.oz[i]);
         const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
         const bool cam_ray = (in.dm[i] & (0xFFu | DM_NEE)) == 1u;
@@ -133,6 +140,59 @@ __global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__
         hit.ba[i] = r.ba;
         hit.bb[i] = r.bb;
         hit.key[i] = (r.leaf == NO_HIT) ? 0u : __float_as_uint(leaves[3 * (size_t)r.leaf + 1].w);
+        hit.inst[i] = 0u;
+    }
+}
+
+// The rays of two-level requests (launched only when one is admitted): the
+// top tree, then the hit mesh's tree in local space (two_level_w).
+template <int W, int L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_serve_trace2(
+    const StepHdr* __restrict__ hdr, const SlotDev* __restrict__ slots, PathBuf in,
+    const uint32_t* __restrict__ in_slot, HitBuf hit, uint2* __restrict__ overflow)
+{
+    constexpr int CAP = 128;
+    __shared__ uint32_t s_node[L > 0 ? L * 256 : 1];
+    __shared__ float s_t[L > 0 ? L * 256 : 1];
+    const uint32_t n = hdr->n_total;
+    const uint32_t lb = xcd_block();
+    const size_t ostride = (size_t)gridDim.x * blockDim.x;
+    const size_t oslot = (size_t)lb * blockDim.x + threadIdx.x;
+    for (uint32_t i = lb * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const SlotDev& sd = slots[in_slot[i]];
+        if (!sd.two) continue;
+        const V3 o = v3(in.ox[i], in.oy[i], in.oz[i]);
+        const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
+        const bool cam_ray = (in.dm[i] & (0xFFu | DM_NEE)) == 1u;
+        const float tmin = cam_ray ? sd.cam.near_d : 0.0f;
+        const float tmax = cam_ray ? sd.cam.far_d : FLT_MAX;
+        const float4* mesh_nodes = (W == 4) ? sd.mesh_nodes4 : sd.mesh_nodes2;
+        auto world = [&](V3& wo, V3& wd) {
+            wo = v3(in.ox[i], in.oy[i], in.oz[i]);
+            wd = v3(in.dx[i], in.dy[i], in.dz[i]);
+        };
+        TraceResult r;
+        uint32_t ii;
+        if constexpr (L > 0) {
+            LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
+            r = two_level_w<W, false, false>(st, sd.nodes4, sd.top_leaves, sd.root4, sd.n_leaves, sd.inst, mesh_nodes,
+                                             sd.q.leaves, o, d, tmin, tmax, ii, world);
+        } else {
+            PrivateStack<CAP> st;
+            r = two_level_w<W, false, false>(st, sd.nodes4, sd.top_leaves, sd.root4, sd.n_leaves, sd.inst, mesh_nodes,
+                                             sd.q.leaves, o, d, tmin, tmax, ii, world);
+        }
+        hit.leaf[i] = r.leaf;
+        hit.t[i] = r.t;
+        hit.ba[i] = r.ba;
+        hit.bb[i] = r.bb;
+        hit.inst[i] = ii;
+        uint32_t key = 0u;
+        if (r.leaf != NO_HIT) {
+            const uint32_t ko = __float_as_uint(sd.inst[INST_F4 * (size_t)ii + 6].y);
+            key = ko ? ko : __float_as_uint(sd.q.leaves[3 * (size_t)r.leaf + 1].w);
+        }
+        hit.key[i] = key;
     }
 }
 
@@ -296,9 +356,10 @@ struct mrh_server {
     std::vector<mrh_request_done> done;
     // graphs per shade variant (bit 0: learn-or-guide, bit 1: NEE) and queue
     // parity, with what they were captured against
-    hipGraphExec_t exec[4][2] = {};
-    void* graph_scratch[4][2][SLOT_COUNT] = {};
-    uint32_t graph_cap[4][2] = {}, graph_w[4][2] = {}, graph_lds[4][2] = {}, graph_bpc[4][2] = {};
+    // (bit 2: some admitted request has a two-level scene)
+    hipGraphExec_t exec[8][2] = {};
+    void* graph_scratch[8][2][SLOT_COUNT] = {};
+    uint32_t graph_cap[8][2] = {}, graph_w[8][2] = {}, graph_lds[8][2] = {}, graph_bpc[8][2] = {};
     uint32_t max_tree_depth = 0;    // deepest LBVH among submitted requests
     // learner: guide learn passes run on their own context and host thread,
     // overlapping the wavefront steps of other requests
@@ -384,6 +445,17 @@ void fill_slot(mrh_server* s, ServeReq* q)
     d.n_leaves = b->n;
     d.q.mats = Materials{sc->lambert, sc->metal, sc->dielectric, sc->emissive, sc->reflect};
     d.q.leaves = (const float4*)b->leaves;
+    if (sc->two_level) {
+        d.two = 1u;
+        d.n_leaves = sc->n_inst;
+        d.top_leaves = (const float4*)b->leaves;
+        d.inst = (const float4*)sc->inst;
+        d.mesh_nodes2 = (const float4*)sc->mesh_nodes2;
+        d.mesh_nodes4 = (const float4*)sc->mesh_nodes4;
+        d.q.leaves = (const float4*)sc->leaves;
+        d.q.inst = (const float4*)sc->inst;
+        d.q.light_inst = sc->light_inst;
+    }
     d.q.image = q->r.d_image;
     d.q.spp = q->S * q->S;
     d.q.sample_base = (uint32_t)(q->it * q->total);
@@ -593,7 +665,7 @@ int32_t step_launches(mrh_server* s, uint32_t par, uint32_t v)
     uint32_t w, cap;
     tree_shape(s, w, cap);
     uint2* ovf = nullptr;
-    MRH_TRY(trace_overflow(ctx, tgrid, cap, lds, &ovf));
+    MRH_TRY(trace_overflow(ctx, tgrid, (v & 4u) ? 128u : cap, lds, &ovf));
 #define MRH_SERVE_TRACE(W, C, LV)                                                                             \
     MRH_TRY(launch(ctx, "serve_trace", dim3(tgrid), dim3(256), 0, k_serve_trace<W, C, LV>,                     \
                    (const StepHdr*)s->d_hdr, (const SlotDev*)s->d_slots, in, in_slot, s->hit, ovf))
@@ -617,6 +689,27 @@ int32_t step_launches(mrh_server* s, uint32_t par, uint32_t v)
     }
 #undef MRH_SERVE_TRACE_L
 #undef MRH_SERVE_TRACE
+    if (v & 4u) {
+#define MRH_SERVE_TRACE2(W, LV)                                                                                 \
+    MRH_TRY(launch(ctx, "serve_trace", dim3(tgrid), dim3(256), 0, k_serve_trace2<W, LV>, (const StepHdr*)s->d_hdr, \
+                   (const SlotDev*)s->d_slots, in, in_slot, s->hit, ovf))
+#define MRH_SERVE_TRACE2_L(W)                                                    \
+    switch (lds) {                                                               \
+    case 0: MRH_SERVE_TRACE2(W, 0); break;                                       \
+    case 8: MRH_SERVE_TRACE2(W, 8); break;                                       \
+    case 12: MRH_SERVE_TRACE2(W, 12); break;                                     \
+    case 16: MRH_SERVE_TRACE2(W, 16); break;                                     \
+    case 24: MRH_SERVE_TRACE2(W, 24); break;                                     \
+    default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24"); \
+    }
+        if (w == 4) {
+            MRH_SERVE_TRACE2_L(4)
+        } else {
+            MRH_SERVE_TRACE2_L(2)
+        }
+#undef MRH_SERVE_TRACE2_L
+#undef MRH_SERVE_TRACE2
+    }
     const bool tiles = s->partition == 1;
     if (!tiles) {
         SortArgs sa;
@@ -780,6 +873,7 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
             if (!q) continue;
             if (q->learning || q->have_guide) v |= 1u;
             if (q->r.params.nee) v |= 2u;
+            if (q->r.scene->two_level) v |= 4u;
         }
         MRH_TRY(run_step(s, s->parity, v));
         MRH_HIP_CHECK(ctx, hipMemcpyAsync(s->h_res, s->d_res, RES_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -827,7 +921,7 @@ void server_free(mrh_server* s)
     for (uint32_t k = 0; k < SERVE_MAX_SLOTS; k++)
         if (s->slots[k]) release(s, s->slots[k]);
     for (ServeReq* q : s->queue) delete q;
-    for (int v = 0; v < 4; v++)
+    for (int v = 0; v < 8; v++)
         for (int par = 0; par < 2; par++)
             if (s->exec[v][par]) (void)hipGraphExecDestroy(s->exec[v][par]);
     for (mrh_svo* v : s->free_svos) svo_free(v);
@@ -892,6 +986,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
     s->hit.ba = (float*)take(n * 4);
     s->hit.bb = (float*)take(n * 4);
     s->hit.key = (uint32_t*)take(n * 4);
+    s->hit.inst = (uint32_t*)take(n * 4);
     s->order = (uint32_t*)take(n * 4);
     s->d_res = (uint32_t*)take(RES_WORDS * 4);
     s->d_hdr = (StepHdr*)take(sizeof(StepHdr));
@@ -931,9 +1026,12 @@ int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
         return fail(ctx, MRH_ERR_INVALID, "submit: nee and direct_light_mis are 0 or 1");
     if (p.nee && r->guide.iterations > 1)
         return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: next-event estimation with guiding is not implemented");
-    if (r->scene->two_level)
-        return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: two-level scenes are rendered by mrh_render, not the engine");
-    if (r->scene->bvh->max_depth > 128)
+    if (r->scene->two_level) {
+        const uint32_t need = std::max(two_level_stack_need(r->scene->bvh->max_depth, r->scene->mesh_depth, 4),
+                                       two_level_stack_need(r->scene->bvh->max_depth, r->scene->mesh_depth, 2));
+        if (need > 128)
+            return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: two-level trees need %u stack entries (at most 128)", need);
+    } else if (r->scene->bvh->max_depth > 128)
         return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: tree depth %u exceeds the traversal stack",
                     r->scene->bvh->max_depth);
     ServeReq* q = new ServeReq();
@@ -958,7 +1056,7 @@ int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
         return fail(ctx, MRH_ERR_BUDGET, "submit: request needs %llu bytes, budget is %llu",
                     (unsigned long long)q->charge, (unsigned long long)s->p.hbm_budget);
     }
-    s->max_tree_depth = std::max(s->max_tree_depth, r->scene->bvh->max_depth);
+    if (!r->scene->two_level) s->max_tree_depth = std::max(s->max_tree_depth, r->scene->bvh->max_depth);
     q->ticket = s->next_ticket++;
     q->t_submit = now_s();
     s->queue.push_back(q);

@@ -139,13 +139,45 @@ def test_two_level_errors(ctx):
     scene = N.Scene(ctx, sc)
     with pytest.raises(N.MrhError):
         scene.bvh()                                                  # no single LBVH
-    server = N.Server(ctx, hbm_budget=1 << 30, wave_paths=1 << 16, max_slots=4)
-    img = torch.zeros(16 * 16 * 3, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
-    with pytest.raises(N.MrhError):
-        server.submit(scene, img, 16, 16, 1, 4, 1, 0)                # the engine takes single-level scenes
-    server.close()
     scene.close()
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_engine_mixes_two_level_and_single_level(ctx, graphs):
+    """Two-level and single-level requests traced together in the serving
+    engine give their solo segments and images."""
+    two = N.Scene(ctx, I.instanced_room())
+    one = N.Scene(ctx, N.gen_cornell(2))
+    # (scene, w, h, strata, depth, rr, seed, iterations, nee, mis)
+    mix = [(two, 48, 32, 2, 8, 3, 1, 1, 0, 0), (one, 40, 30, 2, 6, 2, 2, 1, 0, 0),
+           (two, 32, 32, 1, 10, 3, 3, 1, 1, 1), (two, 40, 24, 2, 8, 3, 4, 2, 0, 0),
+           (one, 24, 24, 1, 6, 3, 5, 1, 1, 0), (two, 24, 16, 2, 6, 2, 6, 1, 1, 0)]
+    ref = []
+    for sc, w, h, S, dep, rr, seed, it, nee, mis in mix:
+        img = torch.zeros(w * h * 3, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        if it > 1:
+            st = sc.render_guided(img.data_ptr(), w, h, S, dep, rr, seed, iterations=it, svo_depth=6,
+                                  guide_ratio=0.5, min_samples=16)
+        else:
+            st = sc.render(img.data_ptr(), w, h, S, dep, rr, seed, nee=nee, mis=mis)
+        ref.append((img.cpu().numpy(), st.segments))
+    server = N.Server(ctx, hbm_budget=8 << 30, wave_paths=4096, max_slots=4, use_graphs=graphs)
+    imgs = [torch.full((m[1] * m[2] * 3,), 7.0, dtype=torch.float32, device="cuda") for m in mix]
+    torch.cuda.synchronize()
+    for k, (sc, w, h, S, dep, rr, seed, it, nee, mis) in enumerate(mix):
+        server.submit(sc, imgs[k], w, h, S, dep, rr, seed, iterations=it, svo_depth=6, guide_ratio=0.5,
+                      min_samples=16, tag=k, nee=nee, mis=mis)
+    done = server.run()
+    assert sorted(d["tag"] for d in done) == list(range(len(mix)))
+    for d in done:
+        k = d["tag"]
+        assert d["status"] == 0 and d["segments"] == ref[k][1], (k, d["segments"], ref[k][1])
+        assert np.allclose(imgs[k].cpu().numpy(), ref[k][0], rtol=1e-4, atol=1e-6), k
+    assert server.stats().steps > 10
+    server.close()
+    two.close()
+    one.close()
 
 
 def test_scene_file_instanced_render_matches_oracle(ctx, oracle):

@@ -975,15 +975,36 @@ uint64_t om_render_guided(const om_scene* s, const om_camera* cam, const om_rend
     q.row_offset = 0;
     q.nee = 0;   /* guided frames trace no next-event rays */
     q.mis = 0;
-    /* scene bounds: min / max over all triangle vertices */
+    /* scene bounds: min / max over all triangle vertices; in a two-level
+     * scene over the instances' world boxes (the top tree's extent) */
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = 0; i < s->n; i++)
-        for (int v = 0; v < 3; v++)
+    if (s->two_level) {
+        const om_two_level* tl = s->two_level;
+        for (uint32_t k = 0; k < tl->n_inst; k++) {
+            uint32_t m = tl->inst_mesh[k];
+            float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY}, wb[6];
+            for (uint32_t i = tl->mesh_first[m]; i < tl->mesh_first[m + 1]; i++)
+                for (int v = 0; v < 3; v++)
+                    for (int a = 0; a < 3; a++) {
+                        float x = s->tris[(size_t)i * 9 + 3 * v + a];
+                        box[a] = fminf(box[a], x);
+                        box[3 + a] = fmaxf(box[3 + a], x);
+                    }
+            om_world_box(tl->l2w + (size_t)k * 12, box, wb);
             for (int a = 0; a < 3; a++) {
-                float x = s->tris[(size_t)i * 9 + 3 * v + a];
-                lo[a] = fminf(lo[a], x);
-                hi[a] = fmaxf(hi[a], x);
+                lo[a] = fminf(lo[a], wb[a]);
+                hi[a] = fmaxf(hi[a], wb[3 + a]);
             }
+        }
+    } else {
+        for (uint32_t i = 0; i < s->n; i++)
+            for (int v = 0; v < 3; v++)
+                for (int a = 0; a < 3; a++) {
+                    float x = s->tris[(size_t)i * 9 + 3 * v + a];
+                    lo[a] = fminf(lo[a], x);
+                    hi[a] = fmaxf(hi[a], x);
+                }
+    }
     float span = 0.0f;
     for (int a = 0; a < 3; a++) span = fmaxf(span, hi[a] - lo[a]);
     if (!(span > 0.0f)) span = 1.0f;

@@ -190,3 +190,25 @@ def test_scene_file_instanced_render_matches_oracle(ctx, oracle):
     assert st.segments == seg
     assert ref.mean() > 0.01
     assert relmse(g, ref) <= 1e-4
+
+
+def test_guided_render_two_level_matches_oracle(ctx, oracle):
+    """Path guiding over a two-level scene: the octree is built in the top
+    tree's world bounds from world-space path vertices, as in the oracle."""
+    sc = I.instanced_room()
+    w, h, S, depth, rr, seed = 40, 30, 2, 6, 2, 3
+    it, sd, ratio, ms = 3, 6, 0.5, 32
+    scene = N.Scene(ctx, sc)
+    img = torch.zeros(w * h * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    st = scene.render_guided(img.data_ptr(), w, h, S, depth, rr, seed, iterations=it, svo_depth=sd,
+                             guide_ratio=ratio, min_samples=ms)
+    got = img.cpu().numpy().reshape(h, w, 3)
+    scene.close()
+    c = sc["camera"]
+    cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], w, h)
+    ref, seg, _ = oracle.render_guided(sc, oracle.TwoLevel(sc), cam, w, h, S, depth, rr, seed, it, sd, ratio, ms,
+                                       threads=8, tree_width=ctx.tune_get("trace_width"))
+    assert st.segments == seg
+    assert ref.mean() > 0.02
+    assert relmse(got, ref) <= 1e-4

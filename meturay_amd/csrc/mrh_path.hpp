@@ -418,7 +418,9 @@ __device__ __forceinline__ V3 metal_eval(const Surface& s, V3 wi, const float* _
 // Utility::RGBToLuminance (RayLib/ColorConversion.h:15-22, its coefficients)
 __device__ __forceinline__ float luminance(V3 c) { return (0.2120f * c.x + 0.7150f * c.y) + 0.0722f * c.z; }
 
-// Path-vertex records of the learning pass, indexed [sample * max_depth + bounce]
+// Path-vertex records of the learning pass, bounce-major: [bounce * stride +
+// sample], so the records a wave writes at one bounce (nearby samples) share
+// cache lines
 struct RecBuf {
     uint32_t* code;    // voxel code of the vertex, SVO_NO_NODE = no record
     uint32_t* bin;     // direction bin of the continuing ray
@@ -427,6 +429,7 @@ struct RecBuf {
     float* tb;         // B
     float* rad;        // luminance of the radiance arriving along that ray
     uint32_t max_depth;
+    uint32_t stride;   // samples per frame
 };
 
 struct GuideArgs {
@@ -650,13 +653,13 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
             // path contribution over that vertex's throughput
             // (PathNode.cuh:152-190, per channel, then luminance)
             const V3 cl = mul(v3(le[0], le[1], le[2]), T);
-            const size_t r0 = (size_t)sample * ga.rec.max_depth;
             for (uint32_t k = 0; k + 1 < depth; k++) {
-                if (ga.rec.code[r0 + k] == SVO_NO_NODE) continue;
-                const float fr = ga.rec.tr[r0 + k], fg = ga.rec.tg[r0 + k], fb = ga.rec.tb[r0 + k];
+                const size_t r = (size_t)k * ga.rec.stride + sample;
+                if (ga.rec.code[r] == SVO_NO_NODE) continue;
+                const float fr = ga.rec.tr[r], fg = ga.rec.tg[r], fb = ga.rec.tb[r];
                 const V3 L = v3(fr == 0.0f ? 0.0f : cl.x / fr, fg == 0.0f ? 0.0f : cl.y / fg,
                                 fb == 0.0f ? 0.0f : cl.z / fb);
-                ga.rec.rad[r0 + k] = luminance(L);
+                ga.rec.rad[r] = luminance(L);
             }
         }
         return false;
@@ -773,7 +776,7 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
     if ((terminate && !o.specular) || is_zero(t2)) return false;
     if (LEARN && learn) {
         // path node of this vertex (PPGTracerKC.cuh:605-620)
-        const size_t r = (size_t)sample * ga.rec.max_depth + (depth - 1u);
+        const size_t r = (size_t)(depth - 1u) * ga.rec.stride + sample;
         ga.rec.code[r] = svo_code(s.pos, ga.lox, ga.loy, ga.loz, ga.inv_cell, ga.depth);
         ga.rec.bin[r] = dir_to_bin(normalize(o.d));
         ga.rec.tr[r] = t2.x;

@@ -819,6 +819,7 @@ int32_t guide_init(mrh_context* ctx, const mrh_scene* s, uint64_t total, uint32_
         g.c_bins = (uint32_t*)q; q += g.n_rec * 4;
         g.c_rad = (float*)q;
         g.rec.max_depth = max_depth;
+        g.rec.stride = (uint32_t)total;
     }
     return MRH_OK;
 }

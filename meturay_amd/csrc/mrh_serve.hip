@@ -530,6 +530,7 @@ int32_t admit(mrh_server* s, ServeReq* q, int slot)
         q->c_bins = (uint32_t*)m; m += n_rec * 4;
         q->c_rad = (float*)m;
         q->rec.max_depth = q->r.params.max_depth;
+        q->rec.stride = (uint32_t)q->total;
         MRH_TRY(fill_u32(ctx, q->rec.code, SVO_NO_NODE, n_rec));
         if (!s->free_svos.empty()) {
             q->svo = s->free_svos.back();

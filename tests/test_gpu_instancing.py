@@ -212,3 +212,53 @@ def test_guided_render_two_level_matches_oracle(ctx, oracle):
     assert st.segments == seg
     assert ref.mean() > 0.02
     assert relmse(got, ref) <= 1e-4
+
+
+def test_filtered_render_two_level_matches_oracle(ctx, oracle):
+    """A reconstruction-filtered frame of a two-level scene keeps every
+    sample's radiance exactly, so it equals the oracle's filtered frame."""
+    sc = I.instanced_room()
+    w, h, S = 48, 36, 2
+    d = N.filter_desc("Tent", 1.0)
+    scene = N.Scene(ctx, sc)
+    img = torch.zeros(h * w * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    st = scene.render_filtered(img.data_ptr(), w, h, S, 8, 3, 4, filt=d)
+    got = img.cpu().numpy().reshape(h, w, 3)
+    scene.close()
+    c = sc["camera"]
+    cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], w, h)
+    ref, seg = oracle.render_filtered(sc, oracle.TwoLevel(sc), cam, w, h, S, 8, 3, 4,
+                                      oracle.OmFilterDesc(d.type, d.radius, d.alpha, d.b, d.c), threads=8,
+                                      tree_width=ctx.tune_get("trace_width"))
+    assert st.segments == seg
+    assert np.array_equal(got, ref)
+
+
+def test_incremental_tracer_two_level_matches_render(ctx):
+    """mrh_tracer_* (GenerateWork / Render per bounce / Finalize) over a
+    two-level scene traces the same segments as mrh_render."""
+    import ctypes as C
+    sc = I.instanced_room()
+    scene = N.Scene(ctx, sc)
+    w, h = 40, 30
+    ref = torch.zeros(h * w * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    st = scene.render(ref.data_ptr(), w, h, 2, 8, 3, 0)
+    p = N.RenderParams(w, h, 2, 8, 3, 0, 0, 0, 0)
+    t = C.c_void_p()
+    ctx.check(N.lib().mrh_tracer_create(ctx.h, scene.h, C.byref(p), C.byref(t)))
+    cam = N.camera_desc(sc["camera"])
+    img = torch.zeros_like(ref)
+    ctx.check(N.lib().mrh_tracer_generate_work(t, C.byref(cam)))
+    more = C.c_int32(1)
+    while more.value:
+        ctx.check(N.lib().mrh_tracer_render(t, C.byref(more)))
+    ctx.check(N.lib().mrh_tracer_finalize(t, img.data_ptr()))
+    ts = N.RenderStats()
+    frames = C.c_uint32()
+    ctx.check(N.lib().mrh_tracer_stats(t, C.byref(ts), C.byref(frames)))
+    ctx.check(N.lib().mrh_tracer_destroy(t))
+    assert ts.segments == st.segments
+    assert np.allclose(img.cpu().numpy(), ref.cpu().numpy(), rtol=1e-5, atol=1e-6)
+    scene.close()

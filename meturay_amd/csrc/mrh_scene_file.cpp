@@ -658,6 +658,15 @@ int32_t mrh_scene_file_load(const char* path, uint32_t flags, mrh_scene_file** o
         delete f;
         return MRH_ERR_INVALID;
     }
+    if (f->instanced) {
+        mrh_instancing_desc& I = f->inst_desc;
+        I.mesh_first = f->mesh_first.data();
+        I.n_meshes = (uint32_t)f->mesh_first.size() - 1;
+        I.xform = f->xform.data();
+        I.inst_mesh = f->inst_mesh.data();
+        I.inst_key = f->inst_key.data();
+        I.n_instances = (uint32_t)f->inst_mesh.size();
+    }
     *out = f;
     return MRH_OK;
 }
@@ -680,16 +689,7 @@ int32_t mrh_scene_file_desc(const mrh_scene_file* f, mrh_scene_desc* d)
     d->reflect = f->reflect.data();
     d->n_reflect = (uint32_t)(f->reflect.size() / 4);
     d->morton_bits = 30;
-    if (f->instanced) {
-        mrh_instancing_desc& I = const_cast<mrh_scene_file*>(f)->inst_desc;
-        I.mesh_first = f->mesh_first.data();
-        I.n_meshes = (uint32_t)f->mesh_first.size() - 1;
-        I.xform = f->xform.data();
-        I.inst_mesh = f->inst_mesh.data();
-        I.inst_key = f->inst_key.data();
-        I.n_instances = (uint32_t)f->inst_mesh.size();
-        d->instancing = &I;
-    }
+    if (f->instanced) d->instancing = &f->inst_desc;
     return MRH_OK;
 }
 

@@ -449,7 +449,12 @@ int32_t mrh_svo_view_get(const mrh_svo* svo, mrh_svo_view* out);
  * each non-specular bounce from guide (probability guide_ratio) or BxDF and
  * weights it by the mixture pdf.  d_image = mean of all frames.  `svo` may be
  * NULL; otherwise it keeps the last guide for inspection.  Deterministic:
- * splat sums are fixed point, so equal inputs give equal guides and paths. */
+ * splat sums are fixed point, so equal inputs give equal guides and paths.
+ * With params.nee (and direct_light_mis) each non-specular vertex also sends
+ * a next-event ray as in mrh_render (WFPGTracerKC.cuh:294-373, BxDF pdf in
+ * the NEE weight, the mixture pdf as a path ray's prevPDF); the guide learns
+ * the radiance that path rays bring back, unweighted, and next-event rays do
+ * not propagate into it. */
 typedef struct {
     uint32_t iterations;     /* frames, 1..64 (1 = plain path tracing) */
     uint32_t svo_depth;      /* "OctreeLevel", 1..10 */
@@ -544,8 +549,7 @@ typedef struct {
 typedef struct {
     mrh_scene*        scene;
     mrh_camera_desc   camera;
-    mrh_render_params params;   /* samples = strata per axis of each iteration; nee only
-                                   without guiding */
+    mrh_render_params params;   /* samples = strata per axis of each iteration */
     mrh_guide_params  guide;    /* iterations <= 1: plain path tracing */
     float*            d_image;  /* width*height*3 floats, written when the request completes */
     uint64_t          user_tag;

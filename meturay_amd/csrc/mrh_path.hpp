@@ -602,8 +602,7 @@ __device__ __forceinline__ void add_radiance(const ShadeReq& q, uint32_t sample,
 // and with guide the one-sample MIS of guide and BxDF
 // (WFPGTracerKC.cuh:1480-1553, pdf = r p_guide + (1 - r) p_bxdf,
 // weight = f cos / pdf).  LEARN / GUIDE / NEE compile the three parts in;
-// learn / guide / q.nee switch them per request (NEE is not combined with
-// LEARN or GUIDE).  The ray's depth is dm & 0xFF (1 = camera ray); at depth
+// learn / guide / q.nee switch them per request.  The ray's depth is dm & 0xFF (1 = camera ray); at depth
 // max_depth only emission is collected.  Returns true when the path
 // continues, with its next state in nx; with NEE, has_nee says whether a
 // next-event ray was spawned, in nee (traced with the next bounce).
@@ -635,23 +634,30 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
     if (batch == MRH_BATCH_EMISSIVE) {
         const float* le = mats.emissive + (size_t)id * 3;
         V3 c = v3(le[0] * T.x, le[1] * T.y, le[2] * T.z);
+        bool counts = true;
         if (NEE && q.nee && depth > 1u && !(dm & DM_SPEC)) {
             // a path ray that left a non-specular vertex, which already sent
             // a NEE ray: without MIS the light was counted by that ray, with
             // MIS this hit takes the BxDF strategy's power-heuristic weight
-            if (!q.mis) return false;
-            V3 p0, p1, p2;
-            leaf_corners(q, hit.leaf[rid], hit.inst ? hit.inst[rid] : 0u, p0, p1, p2);
-            const V3 d = normalize(v3(in.dx[rid], in.dy[rid], in.dz[rid]));
-            const float pc = light_pdf_hit(p0, p1, p2, hit.t[rid], d);
-            const float pm = 1.0f / (float)q.n_lights;
-            c = scale(c, power_heuristic(in.pdf[rid], pc * pm));
+            if (!q.mis) {
+                counts = false;
+            } else {
+                V3 p0, p1, p2;
+                leaf_corners(q, hit.leaf[rid], hit.inst ? hit.inst[rid] : 0u, p0, p1, p2);
+                const V3 d = normalize(v3(in.dx[rid], in.dy[rid], in.dz[rid]));
+                const float pc = light_pdf_hit(p0, p1, p2, hit.t[rid], d);
+                const float pm = 1.0f / (float)q.n_lights;
+                c = scale(c, power_heuristic(in.pdf[rid], pc * pm));
+            }
         }
-        add_radiance(q, sample, c);
+        if (counts) add_radiance(q, sample, c);
         if (LEARN && learn) {
             // radiance arriving along each earlier vertex's outgoing ray:
             // path contribution over that vertex's throughput
-            // (PathNode.cuh:152-190, per channel, then luminance)
+            // (PathNode.cuh:152-190, per channel, then luminance).  The
+            // guide learns the radiance that arrived, whatever share of it
+            // the image counted (a NEE / MIS weight is an estimator's
+            // choice); next-event rays do not propagate.
             const V3 cl = mul(v3(le[0], le[1], le[2]), T);
             for (uint32_t k = 0; k + 1 < depth; k++) {
                 const size_t r = (size_t)k * ga.rec.stride + sample;

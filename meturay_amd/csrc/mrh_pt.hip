@@ -615,17 +615,30 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     auto shade_t = [&](auto kernel) {
         return launch(ctx, "shade", dim3(tgrid_s), dim3(256), 0, kernel, in, outb, hb, f.counts, bounce, q, f.ga);
     };
+    const uint32_t v = (f.learn ? 1u : 0u) | (f.guide ? 2u : 0u) | (f.nee ? 4u : 0u);
     if (tiles) {
-        if (f.learn && f.guide) MRH_TRY(shade_t(k_shade_tiles<true, true, false>));
-        else if (f.learn) MRH_TRY(shade_t(k_shade_tiles<true, false, false>));
-        else if (f.guide) MRH_TRY(shade_t(k_shade_tiles<false, true, false>));
-        else if (f.nee) MRH_TRY(shade_t(k_shade_tiles<false, false, true>));
-        else MRH_TRY(shade_t(k_shade_tiles<false, false, false>));
-    } else if (f.learn && f.guide) MRH_TRY(shade(k_shade<true, true, false>));
-    else if (f.learn) MRH_TRY(shade(k_shade<true, false, false>));
-    else if (f.guide) MRH_TRY(shade(k_shade<false, true, false>));
-    else if (f.nee) MRH_TRY(shade(k_shade<false, false, true>));
-    else MRH_TRY(shade(k_shade<false, false, false>));
+        switch (v) {
+        case 0: MRH_TRY(shade_t(k_shade_tiles<false, false, false>)); break;
+        case 1: MRH_TRY(shade_t(k_shade_tiles<true, false, false>)); break;
+        case 2: MRH_TRY(shade_t(k_shade_tiles<false, true, false>)); break;
+        case 3: MRH_TRY(shade_t(k_shade_tiles<true, true, false>)); break;
+        case 4: MRH_TRY(shade_t(k_shade_tiles<false, false, true>)); break;
+        case 5: MRH_TRY(shade_t(k_shade_tiles<true, false, true>)); break;
+        case 6: MRH_TRY(shade_t(k_shade_tiles<false, true, true>)); break;
+        default: MRH_TRY(shade_t(k_shade_tiles<true, true, true>)); break;
+        }
+    } else {
+        switch (v) {
+        case 0: MRH_TRY(shade(k_shade<false, false, false>)); break;
+        case 1: MRH_TRY(shade(k_shade<true, false, false>)); break;
+        case 2: MRH_TRY(shade(k_shade<false, true, false>)); break;
+        case 3: MRH_TRY(shade(k_shade<true, true, false>)); break;
+        case 4: MRH_TRY(shade(k_shade<false, false, true>)); break;
+        case 5: MRH_TRY(shade(k_shade<true, false, true>)); break;
+        case 6: MRH_TRY(shade(k_shade<false, true, true>)); break;
+        default: MRH_TRY(shade(k_shade<true, true, true>)); break;
+        }
+    }
     f.bounce++;
     return MRH_OK;
 }
@@ -872,7 +885,6 @@ int32_t render_guided(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd,
     uint64_t total = 0;
     MRH_TRY(validate_params(ctx, s, p, &total));
     MRH_TRY(guide_validate(ctx, gp, total, p->max_depth));
-    if (p->nee) return fail(ctx, MRH_ERR_UNSUPPORTED, "render_guided: next-event estimation with guiding is not implemented");
     void* arena = nullptr;
     MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(frame_cap(total, p->nee)), &arena));
     const size_t img_count = (size_t)p->width * p->height * 3;
@@ -1110,8 +1122,6 @@ int32_t mrh_tracer_set_guide(mrh_tracer* t, const mrh_guide_params* guide, mrh_s
     if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_set_guide: frame still rendering");
     if (svo && svo_ctx(svo) != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_set_guide: svo of another context");
     MRH_TRY(guide_validate(ctx, guide, t->total, t->p.max_depth));
-    if (t->p.nee)
-        return fail(ctx, MRH_ERR_UNSUPPORTED, "mrh_tracer_set_guide: next-event estimation with guiding is not implemented");
     guide_free(t->g);
     t->g = GuideState{};
     t->guided = false;

@@ -1025,8 +1025,6 @@ int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
     if (p.max_depth == 0 || p.max_depth > 62) return fail(ctx, MRH_ERR_INVALID, "submit: max_depth must be in [1, 62]");
     if (p.nee > 1 || p.direct_light_mis > 1)
         return fail(ctx, MRH_ERR_INVALID, "submit: nee and direct_light_mis are 0 or 1");
-    if (p.nee && r->guide.iterations > 1)
-        return fail(ctx, MRH_ERR_UNSUPPORTED, "submit: next-event estimation with guiding is not implemented");
     if (r->scene->two_level) {
         const uint32_t need = std::max(two_level_stack_need(r->scene->bvh->max_depth, r->scene->mesh_depth, 4),
                                        two_level_stack_need(r->scene->bvh->max_depth, r->scene->mesh_depth, 2));

@@ -59,10 +59,10 @@ private:
 // (TracerCUDA/WFPGTracer.cu) over mrh_tracer
 class PathTracerMRH final : public GPUTracerI {
 public:
-    // "NextEventEstimation" defaults to on, as in PathTracer.h:26-33; the
-    // guided tracer defaults it to off (NEE is not combined with guiding).
+    // "NextEventEstimation" defaults to on, as in PathTracer.h:26-33 and
+    // WFPGTracer.h:62.
     PathTracerMRH(mrh_context* ctx, SceneMRH* scene, const TracerParameters& p, bool guided)
-        : ctx(ctx), scene(scene), params(p), nee(!guided), guided(guided) {}
+        : ctx(ctx), scene(scene), params(p), nee(true), guided(guided) {}
     ~PathTracerMRH() override { Release(); }
 
     void AttachTracerCallbacks(TracerCallbacksI& cb) override { callbacks = &cb; }
@@ -78,9 +78,6 @@ public:
 
     TracerError Initialize() override
     {
-        if (nee && guided)
-            return Report(TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER,
-                                      "NextEventEstimation with guiding is not implemented"));
         Release();
         mrh_render_params rp = {};
         rp.width = width;

@@ -611,10 +611,11 @@ class Scene:
 
     def render_guided(self, d_image, width, height, samples, max_depth, rr_start, seed=0,
                       iterations=4, svo_depth=10, guide_ratio=0.5, min_samples=512, svo=None,
-                      camera=None) -> RenderStats:
-        """Guided render (mrh_render_guided); `svo` (an Svo) keeps the last guide."""
+                      camera=None, nee=0, mis=0) -> RenderStats:
+        """Guided render (mrh_render_guided); `svo` (an Svo) keeps the last guide;
+        nee / mis = "NextEventEstimation" / "DirectLightMIS"."""
         cam = camera_desc(camera or self.camera)
-        p = RenderParams(width, height, samples, max_depth, rr_start, 0, seed, 0)
+        p = RenderParams(width, height, samples, max_depth, rr_start, nee, seed, 0, mis)
         g = GuideParams(iterations, svo_depth, guide_ratio, min_samples)
         st = RenderStats()
         self.ctx.check(lib().mrh_render_guided(self.ctx.h, self.h, C.byref(cam), C.byref(p), C.byref(g),

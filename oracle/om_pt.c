@@ -735,13 +735,19 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
              * and specular rays count in full. */
             const float* le = sc->emissive + (size_t)id * 3;
             om_v3 c = v3(le[0] * T.x, le[1] * T.y, le[2] * T.z);
+            int counts = 1;
             if (nee && depth > 1 && !spec_ray) {
-                if (!mis) break;
-                float pc = light_pdf_hit(tri, tmax, v3_normalize(d));
-                float pm = 1.0f / (float)sc->n_lights;
-                c = v3_scale(c, power_heuristic(prev_pdf, pc * pm));
+                if (!mis) {
+                    counts = 0;
+                } else {
+                    float pc = light_pdf_hit(tri, tmax, v3_normalize(d));
+                    float pm = 1.0f / (float)sc->n_lights;
+                    c = v3_scale(c, power_heuristic(prev_pdf, pc * pm));
+                }
             }
-            contribute(pixel, sacc, c);
+            if (counts) contribute(pixel, sacc, c);
+            /* the guide learns the radiance that arrived, whatever share of
+             * it the image counted; next-event rays do not propagate */
             if (learn) {
                 om_v3 c = v3_mul(v3(le[0], le[1], le[2]), T);
                 for (uint32_t k = 0; k < nrec; k++) {
@@ -973,8 +979,6 @@ uint64_t om_render_guided(const om_scene* s, const om_camera* cam, const om_rend
     om_render_params q = *p;
     q.row_step = 1;
     q.row_offset = 0;
-    q.nee = 0;   /* guided frames trace no next-event rays */
-    q.mis = 0;
     /* scene bounds: min / max over all triangle vertices; in a two-level
      * scene over the instances' world boxes (the top tree's extent) */
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};

@@ -757,12 +757,12 @@ def atan_unit(t: float) -> float:
 
 def render_guided(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples, max_depth, rr_start,
                   seed, iterations, svo_depth, guide_ratio=0.5, min_samples=0, threads=1, keep_svo=False,
-                  tree_width=4):
+                  tree_width=4, nee=0, mis=0):
     """Guided render of every row.  Returns (image float64 [h,w,3] = mean over
     all frames, segments, Svo of the last guide or None)."""
     keep = {}
     sc = _scene_struct(scene, bvh, keep, tree_width)
-    p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, 1, 0, threads, 0, 0)
+    p = OmRenderParams(width, height, samples, max_depth, rr_start, seed, 1, 0, threads, nee, mis)
     g = OmGuideParams(iterations, svo_depth, guide_ratio, min_samples)
     img = np.zeros((height, width, 3), dtype=np.float64)
     last = OmSvo()

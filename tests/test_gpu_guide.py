@@ -274,3 +274,33 @@ def test_incremental_guided_tracer_matches_render_guided(ctx):
     svo_t.close()
     svo_r.close()
     scene.close()
+
+
+@pytest.mark.parametrize("mis", [0, 1])
+def test_guided_render_with_nee_matches_oracle(ctx, oracle, mis):
+    """Guiding with next-event estimation: the oracle's segments (path and
+    next-event rays), its octree bit for bit, and its image."""
+    import nee_scenes as S
+    sc = S.lit_spheres()
+    w, h, S_, depth, rr, seed = 40, 30, 2, 8, 3, 6
+    it, sd, ratio, ms = 3, 6, 0.5, 32
+    scene = N.Scene(ctx, sc)
+    img = torch.zeros(w * h * 3, dtype=torch.float32, device="cuda")
+    svo = N.Svo(ctx)
+    torch.cuda.synchronize()
+    st = scene.render_guided(img.data_ptr(), w, h, S_, depth, rr, seed, iterations=it, svo_depth=sd,
+                             guide_ratio=ratio, min_samples=ms, svo=svo, nee=1, mis=mis)
+    bvh = oracle.LBVH(sc["tris"], 30)
+    c = sc["camera"]
+    cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], w, h)
+    ref, seg, osvo = oracle.render_guided(sc, bvh, cam, w, h, S_, depth, rr, seed, it, sd, ratio, ms, threads=8,
+                                          keep_svo=True, tree_width=ctx.tune_get("trace_width"), nee=1, mis=mis)
+    assert st.segments == seg
+    assert_same_svo(gpu_view(ctx, svo), osvo)
+    got = img.cpu().numpy().reshape(h, w, 3)
+    assert ref.mean() > 0.01
+    err = float(np.mean((got - ref) ** 2 / (ref ** 2 + 1e-2)))
+    assert err < 1e-4, err
+    svo.close()
+    osvo.close()
+    scene.close()

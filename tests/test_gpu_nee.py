@@ -106,9 +106,6 @@ def test_incremental_tracer_with_nee(ctx):
     assert stats.segments == st.segments
     # same paths; the per-pixel float additions may come in another order
     assert np.allclose(img.cpu().numpy(), ref.cpu().numpy(), rtol=1e-5, atol=1e-7)
-    # guiding is not combined with NEE
-    with pytest.raises(N.MrhError):
-        ctx.check(N.lib().mrh_tracer_set_guide(t, C.byref(N.GuideParams(2, 6, 0.5, 16)), None))
     ctx.check(N.lib().mrh_tracer_destroy(t))
     scene.close()
 
@@ -122,9 +119,9 @@ def test_nee_errors(ctx):
         scene.render(img.data_ptr(), 16, 16, 1, 8, 3, 0, nee=2)
     with pytest.raises(N.MrhError):
         scene.render(img.data_ptr(), 16, 16, 1, 8, 3, 0, nee=1, mis=3)
-    p = N.RenderParams(16, 16, 1, 6, 3, 1, 0, 0, 0)
+    p = N.RenderParams(16, 16, 1, 6, 3, 1, 0, 0, 2)
     g = N.GuideParams(2, 6, 0.5, 16)
-    with pytest.raises(N.MrhError):
+    with pytest.raises(N.MrhError):                                   # direct_light_mis = 2
         ctx.check(N.lib().mrh_render_guided(ctx.h, scene.h, C.byref(N.camera_desc(sc["camera"])), C.byref(p),
                                             C.byref(g), None, img.data_ptr(), None))
     # a scene without lights renders black with NEE on

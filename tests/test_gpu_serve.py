@@ -27,6 +27,7 @@ MIX = [
     ("spheres", 80, 60, 1, 5, 2, 6, 2, 0, 0),
     ("lit", 48, 40, 1, 8, 3, 7, 1, 1, 1),
     ("cornell", 24, 24, 2, 6, 3, 8, 1, 1, 1),
+    ("lit", 40, 30, 2, 8, 3, 9, 3, 1, 1),
 ]
 
 
@@ -44,7 +45,7 @@ def solo(ctx, scene, w, h, spp_axis, depth, rr, seed, it, nee, mis):
     torch.cuda.synchronize()
     if it > 1:
         st = scene.render_guided(img.data_ptr(), w, h, spp_axis, depth, rr, seed, iterations=it, svo_depth=6,
-                                 guide_ratio=0.5, min_samples=16)
+                                 guide_ratio=0.5, min_samples=16, nee=nee, mis=mis)
     else:
         st = scene.render(img.data_ptr(), w, h, spp_axis, depth, rr, seed, nee=nee, mis=mis)
     return img.cpu().numpy(), st.segments
@@ -143,8 +144,6 @@ def test_server_errors(ctx, scenes):
         server.submit(scene, img, 16, 16, 1, 0, 1, 0)             # max_depth 0
     with pytest.raises(N.MrhError):
         server.submit(scene, img, 16, 16, 1, 4, 1, 0, iterations=2, svo_depth=11)
-    with pytest.raises(N.MrhError):
-        server.submit(scene, img, 16, 16, 1, 4, 1, 0, iterations=2, svo_depth=6, nee=1)   # NEE + guiding
     with pytest.raises(N.MrhError):
         server.submit(scene, img, 16, 16, 1, 4, 1, 0, nee=2)
     assert server.step() == (0, 0)                                # idle step is a no-op

@@ -242,6 +242,34 @@ def test_guided_render_unbiased_bright_light(oracle):
     assert abs(g[centre].mean() / ref[centre].mean() - 1.0) < 0.03
 
 
+@pytest.mark.parametrize("mis", [0, 1])
+def test_guided_render_with_nee_unbiased(oracle, mis):
+    """Guiding with next-event estimation (and its MIS) estimates the same
+    image as plain path tracing: the bright-light furnace, 3 %."""
+    sc = furnace(20.0)
+    bvh = oracle.LBVH(sc["tris"], 30)
+    c = sc["camera"]
+    cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], 24, 24)
+    ref, _ = oracle.render(sc, bvh, cam, 24, 24, 32, 8, 1000, 5, threads=8)
+    g, seg, svo = oracle.render_guided(sc, bvh, cam, 24, 24, 8, 8, 1000, 9, 4, 5, 0.5, 64, threads=8,
+                                       keep_svo=True, nee=1, mis=mis)
+    assert svo is not None and svo.nodes > 20
+    centre = (slice(9, 15), slice(9, 15))
+    assert abs(g[centre].mean() / ref[centre].mean() - 1.0) < 0.03
+    svo.close()
+
+
+def test_guided_nee_single_iteration_is_plain_nee(oracle):
+    sc = N.gen_cornell(1)
+    bvh = oracle.LBVH(sc["tris"], 30)
+    c = sc["camera"]
+    cam = oracle.camera(c["position"], c["gaze"], c["up"], c["fov_x"], c["near"], c["far"], 16, 16)
+    a, sa = oracle.render(sc, bvh, cam, 16, 16, 2, 5, 2, 3, nee=1, mis=1)
+    b, sb, _ = oracle.render_guided(sc, bvh, cam, 16, 16, 2, 5, 2, 3, 1, 5, nee=1, mis=1)
+    assert sa == sb
+    assert np.array_equal(a, b)
+
+
 def test_guided_render_single_iteration_is_plain(oracle):
     sc = N.gen_cornell(1)
     bvh = oracle.LBVH(sc["tris"], 30)

@@ -163,7 +163,13 @@ __global__ __launch_bounds__(256) void k_lbvh_emit(
         for (int a = 0; a < 6; a++) st_agent((uint32_t*)&node_aabb[(size_t)i * 6 + a], __float_as_uint(box[a]));
 
         uint32_t node = i, l = i, r = i;
-        for (;;) {
+        // a walk is at most tree-depth steps (< 100 for 30/63-bit codes with
+        // the index tie-break); the bound only matters for a corrupt hierarchy
+        for (uint32_t step = 0;; step++) {
+            if (step == 256u) {
+                atomicOr(&meta[8], 4u);
+                break;
+            }
             if (l == 0 && r == n - 1) {
                 st_agent(&parent[node], 0xFFFFFFFFu);
                 meta[0] = node;
@@ -183,10 +189,15 @@ __global__ __launch_bounds__(256) void k_lbvh_emit(
                 st_agent(&ranges[2 * (size_t)p + 1], r);
             }
             st_agent(&parent[node], n + p);
-            __threadfence();
-            const uint32_t old = atomicAdd(&counters[p], 1u);
+            // Every value another thread reads here (child links, ranges,
+            // parent, boxes) is stored and loaded with agent-scope atomics,
+            // which the device keeps coherent across XCDs without flushing
+            // caches.  So the handshake only needs this thread's stores to
+            // have completed before its counter increment.  A __threadfence()
+            // would also write back the whole L2 twice per step.
+            __builtin_amdgcn_s_waitcnt(0);
+            const uint32_t old = __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (old == 0u) break;   // sibling not done yet: it will continue
-            __threadfence();
             // second arrival: both children and both range ends are published
             const uint32_t lc = ld_agent(&left[p]);
             const uint32_t rc = ld_agent(&right[p]);

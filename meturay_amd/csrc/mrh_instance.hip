@@ -133,19 +133,29 @@ int32_t build_meshes(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene* s, st
 {
     const mrh_instancing_desc* I = d->instancing;
     const uint32_t bits = d->morton_bits ? d->morton_bits : 30u;
+    // the triangles and keys are staged only for the mesh builds (each
+    // LBVH copies its triangles, and the leaf records carry corners and keys)
     float* dtris = nullptr;
     uint32_t* dkeys = nullptr;
-    MRH_TRY(upload(s, &dtris, d->tris, (size_t)d->n * 9));
-    MRH_TRY(upload(s, &dkeys, d->prim_key, (size_t)d->n));
+    MRH_TRY(dmalloc(ctx, (void**)&dtris, (size_t)d->n * 36));
+    int32_t rc = dmalloc(ctx, (void**)&dkeys, (size_t)d->n * 4);
+    if (rc == MRH_OK && (hipMemcpy(dtris, d->tris, (size_t)d->n * 36, hipMemcpyHostToDevice) != hipSuccess ||
+                         hipMemcpy(dkeys, d->prim_key, (size_t)d->n * 4, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(ctx, MRH_ERR_HIP, "mrh_scene_create: triangle upload failed");
     size_t nodes = 0;
-    for (uint32_t m = 0; m < I->n_meshes; m++) {
+    for (uint32_t m = 0; m < I->n_meshes && rc == MRH_OK; m++) {
         const uint32_t f = I->mesh_first[m], n = I->mesh_first[m + 1] - f;
         mrh_lbvh* b = nullptr;
-        MRH_TRY(lbvh_create(ctx, dtris + (size_t)f * 9, n, bits, dkeys + f, &b));
+        rc = lbvh_create(ctx, dtris + (size_t)f * 9, n, bits, dkeys + f, &b);
+        if (rc != MRH_OK) break;
         mb[m] = b;
         s->mesh_depth = std::max(s->mesh_depth, b->max_depth);
         nodes += n - 1;
     }
+    (void)hipStreamSynchronize(ctx->stream);
+    if (dtris) (void)hipFree(dtris);
+    if (dkeys) (void)hipFree(dkeys);
+    MRH_TRY(rc);
     void *lv = nullptr, *n2 = nullptr, *n4 = nullptr;
     MRH_TRY(device_buf(s, &lv, (size_t)d->n * 48));
     MRH_TRY(device_buf(s, &n2, nodes * 64));

@@ -262,3 +262,38 @@ def test_incremental_tracer_two_level_matches_render(ctx):
     assert ts.segments == st.segments
     assert np.allclose(img.cpu().numpy(), ref.cpu().numpy(), rtol=1e-5, atol=1e-6)
     scene.close()
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_two_level_edge_shapes(ctx, oracle, bits):
+    """Edge shapes: one-triangle meshes (a leaf root), one instance (a leaf
+    top tree), a mesh used by many instances and 63-bit codes."""
+    rng = np.random.default_rng(21)
+    one = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32)
+    blob = I.uv_sphere(6, 8, 1.0)
+    tris = np.concatenate([one, blob]).astype(np.float32)
+    keys = np.full(len(tris), N.key(1, 0), np.uint32)
+    base = {"tris": tris, "prim_key": keys, "lambert": np.array([[0.5, 0.5, 0.5]], np.float32),
+            "metal": np.zeros((0, 10), np.float32), "dielectric": np.zeros((0, 4), np.float32),
+            "emissive": np.zeros((0, 3), np.float32), "reflect": np.zeros((0, 4), np.float32)}
+    first = np.array([0, 1, len(tris)], np.uint32)
+    cases = {
+        "single instance of the triangle": ([I.trs((0.1, 0.2, 0.3), (0.3, 0.1, 0.2), (2, 2, 2))], [0]),
+        "single instance of the blob": ([I.trs((0, 0, 0), (0.5, 0.2, 0.1), (0.5, 0.7, 0.9))], [1]),
+        "many instances": ([I.trs(rng.uniform(-0.8, 0.8, 3), rng.uniform(-3, 3, 3), rng.uniform(0.05, 0.2, 3))
+                            for _ in range(40)], [int(k % 2) for k in range(40)]),
+    }
+    for name, (xf, mesh) in cases.items():
+        sc = dict(base)
+        sc["instancing"] = {"mesh_first": first, "xform": np.stack(xf).astype(np.float32),
+                            "inst_mesh": np.array(mesh, np.uint32)}
+        scene = N.Scene(ctx, sc, morton_bits=bits)
+        rays = I.rays_into(sc, 4000, seed=7)
+        gi, gp, gt, gb, gs = gpu_trace(ctx, scene, rays)
+        ei, ep, et, eb, es = oracle.TwoLevel(sc, bits).trace(rays, 0.0, stats=True,
+                                                              width=ctx.tune_get("trace_width"))
+        assert (ep != 0xFFFFFFFF).any(), name
+        assert np.array_equal(gp, ep) and np.array_equal(gi, ei), name
+        assert np.array_equal(gt, et) and np.array_equal(gb, eb), name
+        assert np.array_equal(gs, es), name
+        scene.close()

@@ -131,6 +131,42 @@ def nee_leg(N, ctx, scene, img, torch):
             "segments": int(st.segments)}
 
 
+def c2_leg(N, ctx, torch):
+    """The build-and-cast workload: 1,000,000 random small triangles (seed 1)
+    and 4,000,000 random rays (seed 1).  Times the device LBVH rebuild
+    (bounds, Morton codes, sort, hierarchy, refit, 4-wide collapse; mean of 3)
+    and one closest-hit cast of all rays (mean of 3)."""
+    import ctypes as C
+    tris = torch.from_numpy(N.gen_random_tris(1_000_000, 1)).cuda()
+    rays = torch.from_numpy(N.gen_random_rays(4_000_000, 1)).cuda()
+    n, m = 1_000_000, 4_000_000
+    prim = torch.empty(m, dtype=torch.int32, device="cuda")
+    t = torch.empty(m, dtype=torch.float32, device="cuda")
+    bary = torch.empty(2 * m, dtype=torch.float32, device="cuda")
+    h = C.c_void_p()
+    torch.cuda.synchronize()
+    ctx.check(N.lib().mrh_lbvh_build(ctx.h, tris.data_ptr(), n, 30, C.byref(h)))
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.check(N.lib().mrh_lbvh_rebuild(ctx.h, h, tris.data_ptr()))
+    torch.cuda.synchronize()
+    build = (time.perf_counter() - t0) / 3
+    ctx.check(N.lib().mrh_trace_closest(ctx.h, h, rays.data_ptr(), m, m, 0.0, prim.data_ptr(), t.data_ptr(),
+                                        bary.data_ptr(), None))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.check(N.lib().mrh_trace_closest(ctx.h, h, rays.data_ptr(), m, m, 0.0, prim.data_ptr(), t.data_ptr(),
+                                            bary.data_ptr(), None))
+    torch.cuda.synchronize()
+    cast = (time.perf_counter() - t0) / 3
+    N.lib().mrh_lbvh_destroy(h)
+    log(f"c2: LBVH rebuild of {n} triangles {build * 1e3:.2f} ms; {m} rays cast in {cast * 1e3:.2f} ms")
+    return {"workload": "1,000,000 random triangles (seed 1), 4,000,000 random rays (seed 1), closest hit",
+            "lbvh_rebuild_ms": round(build * 1e3, 3), "cast_ms": round(cast * 1e3, 3),
+            "cast_Mrays_per_s": round(m / cast / 1e6, 1)}
+
+
 def instanced_leg(N, ctx, img, torch):
     """The sphere grid as a two-level scene (16 displaced-sphere meshes,
     4096 instances under y rotations, same camera, triangle count and
@@ -291,6 +327,7 @@ def main():
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-tracer leg")
     ap.add_argument("--no-nee", action="store_true", help="skip the next-event-estimation leg")
     ap.add_argument("--no-instanced", action="store_true", help="skip the two-level (instanced) leg")
+    ap.add_argument("--no-c2", action="store_true", help="skip the LBVH build + ray cast leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving leg requests (0 = skip)")
     ap.add_argument("--rate", type=float, default=40.0, help="serving leg arrivals per second")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -402,6 +439,9 @@ def main():
     instanced = None
     if not args.no_instanced:
         instanced = instanced_leg(N, ctx, img, torch)
+    c2 = None
+    if not args.no_c2:
+        c2 = c2_leg(N, ctx, torch)
     serving = None
     if args.requests > 0:
         serving = serving_leg(N, ctx, torch, args.requests, args.rate)
@@ -444,6 +484,7 @@ def main():
             "guided": guided,
             "nee": nee,
             "instanced": instanced,
+            "c2": c2,
             "kernel_ms_per_frame": {"trace": round(trace_ms / args.steps, 3),
                                     "shade": round(shade_ms / args.steps, 3),
                                     "partition": round((sort_ms + hist_ms) / args.steps, 3)},

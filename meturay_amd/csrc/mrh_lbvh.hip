@@ -61,7 +61,12 @@ __global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ tr
 #pragma unroll
         for (int a = 0; a < 6; a++) o[a] = b[a];
     }
-    // wave reduction then one atomic per wave and axis
+    // Wave, then block reduction; one atomic per block and word, skipped
+    // when the word already holds an equal or better bound.  Atomics on the
+    // six words of one line serialise in L2, so one per wave (16 K waves for
+    // 4 M triangles) cost a millisecond.
+    __shared__ uint32_t part[4][6];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -69,11 +74,22 @@ __global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ tr
             mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off, 64));
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            if (mn[a] != INFINITY) atomicMin(&meta[2 + a], enc_ordered(mn[a]));
-            if (mx[a] != -INFINITY) atomicMax(&meta[5 + a], enc_ordered(mx[a]));
+            part[wave][a] = enc_ordered(mn[a]);
+            part[wave][3 + a] = enc_ordered(mx[a]);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t k = threadIdx.x;
+        uint32_t v = part[0][k];
+        for (uint32_t w = 1; w < blockDim.x / 64; w++) v = (k < 3) ? min(v, part[w][k]) : max(v, part[w][k]);
+        if (k < 3) {
+            if (v != enc_ordered(INFINITY) && v < ld_agent(&meta[2 + k])) atomicMin(&meta[2 + k], v);
+        } else {
+            if (v != enc_ordered(-INFINITY) && v > ld_agent(&meta[2 + k])) atomicMax(&meta[2 + k], v);
         }
     }
 }
@@ -251,8 +267,15 @@ __global__ __launch_bounds__(256) void k_lbvh_depth(uint32_t n, const uint32_t* 
         }
         best = max(best, d);
     }
+    // block maximum, then one atomic per block when it raises meta[1]
+    __shared__ uint32_t part[4];
     for (int off = 32; off > 0; off >>= 1) best = max(best, (uint32_t)__shfl_xor((int)best, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(&meta[1], best);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / 64; w++) best = max(best, part[w]);
+        if (best > ld_agent(&meta[1])) atomicMax(&meta[1], best);
+    }
 }
 
 // 4-wide traversal nodes: every internal node at even depth (root = 0)

@@ -260,7 +260,13 @@ int32_t radix_sort_t(mrh_context* ctx, const SortArgs& a)
         }
     }
 
-    MRH_TRY(launch(ctx, "sort_hist", dim3(stride_grid(ctx, a.n_max, 256)), dim3(256), 0,
+    // Each block flushes passes x 256 counters with global atomics, so the
+    // grid is sized for >= 4096 keys per block and at most 4 blocks per CU:
+    // one block per 256 keys put millions of atomics on 1024 words.
+    uint32_t hist_blocks = div_up(a.n_max, 4096u);
+    hist_blocks = std::min<uint32_t>(hist_blocks, ctx->num_cus * 4u);
+    hist_blocks = std::max<uint32_t>(hist_blocks, 1u);
+    MRH_TRY(launch(ctx, "sort_hist", dim3(hist_blocks), dim3(256), 0,
                    k_sort_hist<KeyT>, (const KeyT*)a.keys_in, a.n_max, a.n_dev, pd, meta));
     MRH_TRY(launch(ctx, "sort_scan", dim3(1), dim3(256), 0, k_sort_scan, meta, passes,
                    a.part_offsets, part_digits, a.n_max, a.n_dev));

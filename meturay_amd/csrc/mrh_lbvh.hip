@@ -41,6 +41,15 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p)
     return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// meta[0..1] = root / depth, [2..4] = the encoded extent min (starts at the
+// largest code), [5..7] = the encoded extent max (starts at 0), [8..15] = the
+// hierarchy flags and spare words; one launch instead of four fills
+__global__ void k_meta_init(uint32_t* __restrict__ meta)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < 16) meta[i] = (i >= 2 && i < 5) ? 0xFFFFFFFFu : 0u;
+}
+
 // meta[2..4] = encoded extent min, meta[5..7] = encoded extent max
 __global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ tris, uint32_t n,
                                                     float* __restrict__ aabb, uint32_t* __restrict__ meta)
@@ -355,11 +364,7 @@ int32_t morton_codes(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t
         MRH_TRY(scratch(ctx, SLOT_MORTON_EXT, 64, &p));
         meta = (uint32_t*)p;
     }
-    // meta[0..1] = root/depth, [2..4] = enc(+inf), [5..7] = enc(-inf)
-    MRH_TRY(fill_u32(ctx, meta, 0u, 2));
-    MRH_TRY(fill_u32(ctx, meta + 8, 0u, 8));
-    MRH_TRY(fill_u32(ctx, meta + 2, 0xFFFFFFFFu, 3));
-    MRH_TRY(fill_u32(ctx, meta + 5, 0u, 3));
+    MRH_TRY(launch(ctx, "lbvh_meta_init", dim3(1), dim3(64), 0, k_meta_init, meta));
     MRH_TRY(launch(ctx, "tri_bounds", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_tri_bounds,
                    d_tris, n, aabb, meta));
     if (bits == 30)

@@ -135,7 +135,8 @@ def c2_leg(N, ctx, torch):
     """The build-and-cast workload: 1,000,000 random small triangles (seed 1)
     and 4,000,000 random rays (seed 1).  Times the device LBVH rebuild
     (bounds, Morton codes, sort, hierarchy, refit, 4-wide collapse; mean of 3)
-    and one closest-hit cast of all rays (mean of 3)."""
+    and one closest-hit cast of all rays (mean of 3, at the best of 8, 16 or
+    32 blocks per CU)."""
     import ctypes as C
     tris = torch.from_numpy(N.gen_random_tris(1_000_000, 1)).cuda()
     rays = torch.from_numpy(N.gen_random_rays(4_000_000, 1)).cuda()
@@ -151,20 +152,36 @@ def c2_leg(N, ctx, torch):
         ctx.check(N.lib().mrh_lbvh_rebuild(ctx.h, h, tris.data_ptr()))
     torch.cuda.synchronize()
     build = (time.perf_counter() - t0) / 3
-    ctx.check(N.lib().mrh_trace_closest(ctx.h, h, rays.data_ptr(), m, m, 0.0, prim.data_ptr(), t.data_ptr(),
-                                        bary.data_ptr(), None))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(3):
+
+    def cast_once():
         ctx.check(N.lib().mrh_trace_closest(ctx.h, h, rays.data_ptr(), m, m, 0.0, prim.data_ptr(), t.data_ptr(),
                                             bary.data_ptr(), None))
+
+    # The cast's own launch shape: blocks per CU picked from one timed cast
+    # each (hits are identical for every shape); the frame's knob is restored.
+    frame_bpc = ctx.tune_get("trace_blocks_per_cu")
+    trials = []
+    for bpc in (8, 16, 32):
+        ctx.tune("trace_blocks_per_cu", bpc)
+        cast_once()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cast_once()
+        torch.cuda.synchronize()
+        trials.append((time.perf_counter() - t0, bpc))
+    best_bpc = min(trials)[1]
+    ctx.tune("trace_blocks_per_cu", best_bpc)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        cast_once()
     torch.cuda.synchronize()
     cast = (time.perf_counter() - t0) / 3
+    ctx.tune("trace_blocks_per_cu", frame_bpc)
     N.lib().mrh_lbvh_destroy(h)
     log(f"c2: LBVH rebuild of {n} triangles {build * 1e3:.2f} ms; {m} rays cast in {cast * 1e3:.2f} ms")
     return {"workload": "1,000,000 random triangles (seed 1), 4,000,000 random rays (seed 1), closest hit",
             "lbvh_rebuild_ms": round(build * 1e3, 3), "cast_ms": round(cast * 1e3, 3),
-            "cast_Mrays_per_s": round(m / cast / 1e6, 1)}
+            "cast_Mrays_per_s": round(m / cast / 1e6, 1), "cast_blocks_per_cu": best_bpc}
 
 
 def instanced_leg(N, ctx, img, torch):

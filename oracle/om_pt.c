@@ -239,8 +239,9 @@ void om_camera_setup(const float pos[3], const float gaze[3], const float up[3],
 /* GenerationKernels.cuh:29-108 + GPUCameraPinhole.cuh:209-240: stratified
  * pinhole sample.  Sample ids run pixel-major: sample_id =
  * (y*W + x)*S^2 + (sy*S + sx). */
-static void camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t S,
-                       uint32_t sample_id, om_rng* rng, om_v3* o, om_v3* d, float* coord)
+static void camera_ray_jittered(const om_camera* cam, uint32_t w, uint32_t h, uint32_t S,
+                                uint32_t sample_id, float jx, float jy, om_v3* o, om_v3* d,
+                                float* coord)
 {
     uint32_t spp = S * S;
     uint32_t pix = sample_id / spp;
@@ -249,8 +250,6 @@ static void camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t S,
     uint32_t sx = s % S, sy = s / S;
     float dx = cam->plane_size[0] / (float)(w * S);
     float dy = cam->plane_size[1] / (float)(h * S);
-    float jx = om_uniform(rng);
-    float jy = om_uniform(rng);
     float sdx = (float)(px * S + sx) * dx;
     float sdy = (float)(py * S + sy) * dy;
     sdx += jx * dx;
@@ -273,6 +272,45 @@ static void camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t S,
         coord[0] = cx;
         coord[1] = cy;
     }
+}
+
+/* The stratum jitter is the sample stream's first two uniforms. */
+static void camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t S,
+                       uint32_t sample_id, om_rng* rng, om_v3* o, om_v3* d, float* coord)
+{
+    float jx = om_uniform(rng);
+    float jy = om_uniform(rng);
+    camera_ray_jittered(cam, w, h, S, sample_id, jx, jy, o, d, coord);
+}
+
+/* A camera ray with a given stratum jitter (jx, jy in [0, 1]; 1 reaches the
+ * far edge of the stratum), for closed-form checks of the camera basis. */
+void om_camera_ray_at(const om_camera* cam, uint32_t w, uint32_t h, uint32_t samples,
+                      uint32_t sample_id, float jx, float jy, float o[3], float d[3])
+{
+    om_v3 oo, dd;
+    camera_ray_jittered(cam, w, h, samples, sample_id, jx, jy, &oo, &dd, NULL);
+    o[0] = oo.x; o[1] = oo.y; o[2] = oo.z;
+    d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
+}
+
+/* The raw PCG32 stream of pcg32_srandom_r(initstate, initseq) (O'Neill 2014),
+ * and the SplitMix64 finaliser, exported for known-answer checks. */
+void om_pcg32_stream(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out)
+{
+    om_rng r;
+    om_pcg32_seed(&r, initstate, initseq);
+    for (uint32_t i = 0; i < n; ++i) out[i] = om_pcg32(&r);
+}
+
+uint64_t om_splitmix64(uint64_t z) { return om_mix64(z); }
+
+/* The first n uniforms of a path's stream (seed, global sample id). */
+void om_path_uniforms(uint32_t seed, uint32_t sample_id, uint32_t n, float* out)
+{
+    om_rng r;
+    om_path_rng(&r, seed, sample_id);
+    for (uint32_t i = 0; i < n; ++i) out[i] = om_uniform(&r);
 }
 
 void om_camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t samples,

@@ -306,6 +306,11 @@ uint64_t om_render_guided(const om_scene* s, const om_camera* cam, const om_rend
 void om_camera_ray(const om_camera* cam, uint32_t w, uint32_t h, uint32_t samples,
                    uint32_t seed, uint32_t sample_id, float o[3], float d[3],
                    uint64_t* rng_state);
+void om_camera_ray_at(const om_camera* cam, uint32_t w, uint32_t h, uint32_t samples,
+                      uint32_t sample_id, float jx, float jy, float o[3], float d[3]);
+void om_pcg32_stream(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out);
+uint64_t om_splitmix64(uint64_t z);
+void om_path_uniforms(uint32_t seed, uint32_t sample_id, uint32_t n, float* out);
 
 #ifdef __cplusplus
 }

@@ -163,6 +163,12 @@ def lib():
                                 C.POINTER(OmRenderParams), _P]
         L.om_camera_ray.argtypes = [C.POINTER(OmCamera), _U32, _U32, _U32, _U32, _U32,
                                     _P, _P, C.POINTER(C.c_uint64)]
+        L.om_camera_ray_at.argtypes = [C.POINTER(OmCamera), _U32, _U32, _U32, _U32, _F, _F,
+                                       _P, _P]
+        L.om_pcg32_stream.argtypes = [C.c_uint64, C.c_uint64, _U32, _P]
+        L.om_splitmix64.restype = C.c_uint64
+        L.om_splitmix64.argtypes = [C.c_uint64]
+        L.om_path_uniforms.argtypes = [_U32, _U32, _U32, _P]
         L.om_svo_build.restype = C.c_int
         L.om_svo_build.argtypes = [C.POINTER(OmSvo), _P, _U32, _P, _U32]
         L.om_svo_free.argtypes = [C.POINTER(OmSvo)]
@@ -492,6 +498,30 @@ def camera_ray(cam: OmCamera, w, h, samples, seed, sample_id):
     lib().om_camera_ray(C.byref(cam), w, h, samples, seed, sample_id, _ptr(o), _ptr(d),
                         C.byref(st))
     return o, d, st.value
+
+
+def camera_ray_at(cam: OmCamera, w, h, samples, sample_id, jx, jy):
+    """Camera ray of one sample with a given stratum jitter (jx, jy)."""
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().om_camera_ray_at(C.byref(cam), w, h, samples, sample_id, jx, jy, _ptr(o), _ptr(d))
+    return o, d
+
+
+def pcg32_stream(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().om_pcg32_stream(initstate, initseq, n, _ptr(out))
+    return out
+
+
+def splitmix64(z: int) -> int:
+    return int(lib().om_splitmix64(z))
+
+
+def path_uniforms(seed: int, sample_id: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().om_path_uniforms(seed, sample_id, n, _ptr(out))
+    return out
 
 
 def _scene_struct(scene: dict, bvh: LBVH, keep: dict, width: int = 4) -> OmScene:

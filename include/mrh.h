@@ -27,7 +27,9 @@ typedef enum {
     MRH_ERR_HIP = 2,          /* HIP runtime error */
     MRH_ERR_OOM = 3,          /* device allocation failed (TracerException on OOM, CudaCheck.h:32-42) */
     MRH_ERR_UNSUPPORTED = 4,  /* option or type not implemented */
-    MRH_ERR_BUDGET = 5        /* request exceeds the memory budget */
+    MRH_ERR_BUDGET = 5,       /* request exceeds the memory budget */
+    MRH_ERR_CANCELLED = 6,    /* serving: the request was cancelled (mrh_server_cancel) */
+    MRH_ERR_TIMEOUT = 7       /* serving: the request outlived mrh_server_params.timeout_ms */
 } mrh_status;
 
 typedef struct mrh_context mrh_context;
@@ -74,6 +76,9 @@ int32_t     mrh_profile_get(mrh_context* ctx, const char* kernel, double* total_
  *                         radix pass over the HitKey batch byte (the rays are
  *                         shaded through the sorted order), 1 = each block
  *                         partitions its own tile of 1024 rays in LDS
+ *   "fail_alloc_after"    failure-handling tests: k > 0 makes the k-th
+ *                         following device allocation of this context fail
+ *                         with MRH_ERR_OOM (0 = off)
  * mrh_tune_get reads a knob back. */
 int32_t     mrh_tune(mrh_context* ctx, const char* knob, int64_t value);
 int32_t     mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value);
@@ -538,12 +543,15 @@ int32_t mrh_gen_random_rays(uint32_t n, uint32_t seed, float* rays);
 typedef struct mrh_server mrh_server;
 
 typedef struct {
-    uint64_t hbm_budget;     /* device bytes admitted requests may hold (image, learn records, guide) */
+    uint64_t hbm_budget;     /* device bytes the server may hold: its queues (from creation on),
+                                the scenes of admitted requests (each once) and every admitted
+                                request's image, learn records and guide */
     uint32_t wave_paths;     /* paths in flight per step; each may also have a
                                 next-event ray in flight (queues hold 2x rays) */
     uint32_t max_slots;      /* concurrent requests, 1..64 */
     uint32_t use_graphs;     /* replay each step's launches from a captured hipGraph */
-    uint32_t reserved;
+    uint32_t timeout_ms;     /* 0 = none; otherwise a request not done this long after its
+                                submission ends with MRH_ERR_TIMEOUT */
 } mrh_server_params;
 
 typedef struct {
@@ -568,12 +576,22 @@ typedef struct {
     uint32_t active, queued;
     uint32_t live_paths;        /* rays in flight: paths and their next-event rays */
     uint32_t reserved;
+    uint64_t fixed_bytes;       /* of budget_used: queues and step buffers */
+    uint64_t scene_bytes;       /* of budget_used: scenes of admitted requests */
 } mrh_server_info;
 
 int32_t mrh_server_create(mrh_context* ctx, const mrh_server_params* params, mrh_server** out);
 int32_t mrh_server_context(mrh_server* server, mrh_context** out);
-/* Queues a request (MRH_ERR_BUDGET if it can never fit the budget). */
+/* Queues a request (MRH_ERR_BUDGET if it can never fit the budget).  A
+ * request that fails on its own (its allocations, its learn pass) ends alone
+ * with that status in mrh_server_poll; the others go on. */
 int32_t mrh_server_submit(mrh_server* server, const mrh_request* request, uint64_t* ticket);
+/* Cancels a queued or admitted request: a queued one is reported at once, an
+ * admitted one takes no new samples, its paths in flight end at their next
+ * shading, and it is reported when the last has ended (a step or two later).
+ * Either way mrh_server_poll returns it with MRH_ERR_CANCELLED and its image
+ * is undefined.  MRH_ERR_INVALID for a ticket that is neither. */
+int32_t mrh_server_cancel(mrh_server* server, uint64_t ticket);
 /* Admits what fits, traces one wavefront step, finishes what ended
  * (synchronous).  active / queued may be NULL. */
 int32_t mrh_server_step(mrh_server* server, uint32_t* active, uint32_t* queued);

@@ -23,6 +23,8 @@ int32_t dmalloc(mrh_context* ctx, void** p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
+    if (ctx->fail_alloc_after > 0 && --ctx->fail_alloc_after == 0)
+        return fail(ctx, MRH_ERR_OOM, "hipMalloc(%zu) failed: injected by fail_alloc_after", bytes);
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -213,6 +215,11 @@ int32_t mrh_tune(mrh_context* ctx, const char* knob, int64_t value)
         ctx->shade_partition = (uint32_t)value;
         return MRH_OK;
     }
+    if (k == "fail_alloc_after") {
+        if (value < 0) return fail(ctx, MRH_ERR_INVALID, "fail_alloc_after must be >= 0");
+        ctx->fail_alloc_after = (uint64_t)value;
+        return MRH_OK;
+    }
     return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
 }
 
@@ -224,6 +231,7 @@ int32_t mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value)
     else if (k == "trace_blocks_per_cu") *value = ctx->trace_blocks_per_cu;
     else if (k == "trace_width") *value = ctx->trace_width;
     else if (k == "shade_partition") *value = ctx->shade_partition;
+    else if (k == "fail_alloc_after") *value = (int64_t)ctx->fail_alloc_after;
     else return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
     return MRH_OK;
 }

@@ -58,6 +58,9 @@ struct mrh_context {
     uint32_t trace_width = 4;
     // material partition of a bounce: 0 = global radix pass, 1 = per-block tiles
     uint32_t shade_partition = 1;
+    // failure-handling tests: when k > 0, the k-th following dmalloc on this
+    // context fails with MRH_ERR_OOM (knob "fail_alloc_after"; 0 = off)
+    uint64_t fail_alloc_after = 0;
     // octree reused by guided renders that do not bring their own
     struct mrh_svo* guide_svo = nullptr;
 };

@@ -219,9 +219,18 @@ __global__ __launch_bounds__(256) void k_lbvh_emit(
             // which the device keeps coherent across XCDs without flushing
             // caches.  So the handshake only needs this thread's stores to
             // have completed before its counter increment.  A __threadfence()
-            // would also write back the whole L2 twice per step.
+            // would also write back the whole L2 twice per step.  The
+            // signal fences keep the compiler from moving the stores or the
+            // counter across the wait (s_waitcnt alone is no compiler
+            // barrier, and relaxed atomics on different addresses may be
+            // reordered); the wait keeps the hardware from doing so.  An
+            // agent-scope release/acquire pair would order the same accesses
+            // but costs an L2 write-back per step on this chip.
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
             __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
             const uint32_t old = __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
             if (old == 0u) break;   // sibling not done yet: it will continue
             // second arrival: both children and both range ends are published
             const uint32_t lc = ld_agent(&left[p]);

@@ -521,6 +521,13 @@ int32_t validate_params(mrh_context* ctx, const mrh_scene* s, const mrh_render_p
     const uint64_t t = (uint64_t)p->width * p->height * p->samples * p->samples;
     if (t >= (1ull << 30))
         return fail(ctx, MRH_ERR_INVALID, "render: %llu paths exceed 2^30", (unsigned long long)t);
+    // the global material partition sorts a whole queue: a path and its
+    // next-event ray each, and the sort takes fewer than 2^30 keys
+    if (ctx->shade_partition == 0 && frame_cap(t, p->nee) >= (1ull << 30))
+        return fail(ctx, MRH_ERR_INVALID,
+                    "render: %llu rays per bounce exceed the global partition's 2^30 - 1 keys "
+                    "(use the tile partition, shade_partition = 1, or fewer paths)",
+                    (unsigned long long)frame_cap(t, p->nee));
     if (s->bvh->max_depth > 128)
         return fail(ctx, MRH_ERR_UNSUPPORTED, "tree depth %u exceeds the traversal stack", s->bvh->max_depth);
     *total = t;

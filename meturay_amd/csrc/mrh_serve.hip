@@ -315,6 +315,8 @@ struct ServeReq {
     bool learning = false, have_guide = false;
     bool in_learn = false;     // its learn pass is queued or running on the learner
     bool learned_ok = true;    // result of the last learn pass
+    int32_t learn_rc = MRH_OK;
+    int32_t cancel_status = MRH_OK;   // MRH_ERR_CANCELLED / MRH_ERR_TIMEOUT once cancelled
     uint32_t kept = 0;
     void* recmem = nullptr;
     size_t recmem_bytes = 0;
@@ -348,7 +350,9 @@ struct mrh_server {
     uint32_t parity = 0;           // queue holding the live rays
     uint32_t n_live = 0;           // rays in that queue
     uint32_t n_paths = 0;          // path rays among them
-    uint64_t used = 0;             // budget bytes held by admitted requests
+    uint64_t used = 0;             // budget bytes held: queues, resident scenes, admitted requests
+    uint64_t fixed_bytes = 0;      // the queues and step buffers (charged from creation on)
+    std::map<const mrh_scene*, uint32_t> resident;   // scenes of admitted requests (charged once each)
     uint64_t next_ticket = 1;
     uint64_t steps = 0, segments = 0;
     std::deque<ServeReq*> queue;
@@ -415,9 +419,23 @@ void release(mrh_server* s, ServeReq* q)
         }
     }
     q->recmem = nullptr;
-    if (q->slot >= 0) s->slots[q->slot] = nullptr;
-    s->used -= q->charge;
+    if (q->slot >= 0) {
+        s->slots[q->slot] = nullptr;
+        s->used -= q->charge;
+        auto it = s->resident.find(q->r.scene);
+        if (it != s->resident.end() && --it->second == 0) {
+            s->used -= q->r.scene->bytes;
+            s->resident.erase(it);
+        }
+    }
     delete q;
+}
+
+// budget bytes admitting q adds: its own charge and its scene's bytes when no
+// admitted request holds that scene yet
+uint64_t admit_cost(const mrh_server* s, const ServeReq* q)
+{
+    return q->charge + (s->resident.count(q->r.scene) ? 0u : q->r.scene->bytes);
 }
 
 SlotDev& host_slot(mrh_server* s, const ServeReq* q) { return s->h_slots[q->slot]; }
@@ -486,9 +504,10 @@ void fill_slot(mrh_server* s, ServeReq* q)
 int32_t admit(mrh_server* s, ServeReq* q, int slot)
 {
     mrh_context* ctx = s->ctx;
+    s->used += admit_cost(s, q);
+    s->resident[q->r.scene]++;
     q->slot = slot;
     s->slots[slot] = q;
-    s->used += q->charge;
     q->t_admit = now_s();
     const size_t img = (size_t)q->r.params.width * q->r.params.height * 3;
     MRH_TRY(fill_u32(ctx, (uint32_t*)q->r.d_image, 0u, img));
@@ -581,6 +600,7 @@ void learner_main(mrh_server* s)
         {
             std::lock_guard<std::mutex> lk(s->mu);
             q->learned_ok = rc == MRH_OK;
+            q->learn_rc = rc;
             if (rc != MRH_OK) s->learn_err = s->lctx->err;
             s->learned.push_back(q);
         }
@@ -615,6 +635,22 @@ int32_t iteration_done(mrh_server* s, ServeReq* q)
     }
     if (q->it + 1 < q->iterations) return next_iteration(s, q);
     return finish_request(s, q);
+}
+
+// Ends a request that failed: reported with its error, slot and budget
+// released.  None of its paths may be in flight.
+void end_request(mrh_server* s, ServeReq* q, int32_t status)
+{
+    mrh_request_done d = {};
+    d.ticket = q->ticket;
+    d.user_tag = q->r.user_tag;
+    d.status = status;
+    d.segments = q->segments;
+    d.submit_s = q->t_submit;
+    d.admit_s = q->t_admit;
+    d.done_s = now_s();
+    s->done.push_back(d);
+    release(s, q);
 }
 
 int32_t finish_request(mrh_server* s, ServeReq* q)
@@ -796,9 +832,38 @@ int32_t run_step(mrh_server* s, uint32_t par, uint32_t v)
     return MRH_OK;
 }
 
+// Queued: reported at once.  Admitted: takes no new samples, and its paths in
+// flight end at their next shading (max depth 0 in its slot); it is reported
+// with `status` when the last has ended or its learn pass has returned.
+int32_t cancel_request(mrh_server* s, ServeReq* q, int32_t status)
+{
+    if (q->slot < 0) {
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), q));
+        end_request(s, q, status);
+        return MRH_OK;
+    }
+    if (q->cancel_status != MRH_OK) return MRH_OK;
+    q->cancel_status = status;
+    q->injected = q->total;
+    q->learning = false;
+    fill_slot(s, q);
+    host_slot(s, q).q.max_depth = 0;
+    return upload_slot(s, q);
+}
+
 int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
 {
     mrh_context* ctx = s->ctx;
+    if (s->p.timeout_ms) {
+        const double t = now_s(), lim = s->p.timeout_ms * 1e-3;
+        std::vector<ServeReq*> late;
+        for (ServeReq* q : s->queue)
+            if (t - q->t_submit > lim) late.push_back(q);
+        for (uint32_t k = 0; k < SERVE_MAX_SLOTS; k++)
+            if (s->slots[k] && s->slots[k]->cancel_status == MRH_OK && t - s->slots[k]->t_submit > lim)
+                late.push_back(s->slots[k]);
+        for (ServeReq* q : late) MRH_TRY(cancel_request(s, q, MRH_ERR_TIMEOUT));
+    }
     // admission, first come first served, while slots and budget allow
     while (!s->queue.empty()) {
         ServeReq* q = s->queue.front();
@@ -808,20 +873,12 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
             if (s->slots[k]) n_active++;
             else if (slot < 0) slot = (int)k;
         }
-        if (slot < 0 || (n_active > 0 && s->used + q->charge > s->p.hbm_budget)) break;
+        if (slot < 0 || (n_active > 0 && s->used + admit_cost(s, q) > s->p.hbm_budget)) break;
         s->queue.pop_front();
-        int32_t rc = admit(s, q, slot);
-        if (rc != MRH_OK) {
-            mrh_request_done d = {};
-            d.ticket = q->ticket;
-            d.user_tag = q->r.user_tag;
-            d.status = rc;
-            d.submit_s = q->t_submit;
-            d.done_s = now_s();
-            s->done.push_back(d);
-            release(s, q);
-            return rc;
-        }
+        // a request that cannot be admitted (its record buffer does not
+        // fit, say) ends alone with that error; the others go on
+        const int32_t rc = admit(s, q, slot);
+        if (rc != MRH_OK) end_request(s, q, rc);
     }
     // requests whose learn pass finished: next frame, guided
     std::vector<ServeReq*> ready;
@@ -833,7 +890,17 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
     }
     for (ServeReq* q : ready) {
         q->in_learn = false;
-        if (!q->learned_ok) return fail(ctx, MRH_ERR_HIP, "guide learn pass failed: %s", lerr.c_str());
+        if (q->cancel_status != MRH_OK) {
+            end_request(s, q, q->cancel_status);
+            continue;
+        }
+        if (!q->learned_ok) {
+            // none of its paths is in flight (its frame had ended): it ends
+            // alone with the learn pass's error, the others go on
+            fail(ctx, MRH_ERR_HIP, "guide learn pass failed: %s", lerr.c_str());
+            end_request(s, q, q->learn_rc);
+            continue;
+        }
         if (q->kept) q->have_guide = true;
         MRH_TRY(next_iteration(s, q));
     }
@@ -894,7 +961,9 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
     // iterations / requests whose paths have all ended
     for (uint32_t k = 0; k < s->p.max_slots; k++) {
         ServeReq* q = s->slots[k];
-        if (q && !q->in_learn && q->injected == q->total && q->live == 0) MRH_TRY(iteration_done(s, q));
+        if (!q || q->in_learn || q->injected != q->total || q->live != 0) continue;
+        if (q->cancel_status != MRH_OK) end_request(s, q, q->cancel_status);
+        else MRH_TRY(iteration_done(s, q));
     }
     uint32_t a = 0;
     for (uint32_t k = 0; k < s->p.max_slots; k++) a += s->slots[k] ? 1u : 0u;
@@ -942,6 +1011,10 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
         return fail(user, MRH_ERR_INVALID, "server: max_slots must be in [1, %u]", SERVE_MAX_SLOTS);
     if (p->wave_paths < 1024 || p->wave_paths >= (1u << 30))
         return fail(user, MRH_ERR_INVALID, "server: wave_paths must be in [1024, 2^30)");
+    // a queue holds 2 * wave_paths rays; the global partition sorts it whole
+    if (user->shade_partition == 0 && p->wave_paths >= (1u << 29))
+        return fail(user, MRH_ERR_INVALID,
+                    "server: with the global partition (shade_partition = 0) wave_paths must be below 2^29");
     mrh_server* s = new mrh_server();
     s->user = user;
     s->p = *p;
@@ -968,6 +1041,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
         void* q = nullptr;
         if (rc == MRH_OK) rc = dmalloc(ctx, &q, bytes);
         if (q) s->allocs.push_back(q);
+        s->fixed_bytes += bytes;
         return q;
     };
     const size_t n = s->buf_cap;
@@ -1005,6 +1079,10 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
         rc = fill_u32(ctx, (uint32_t*)s->d_slots, 0u, sizeof(SlotDev) * SERVE_MAX_SLOTS / 4);
     }
     if (rc == MRH_OK) rc = hipStreamSynchronize(ctx->stream) == hipSuccess ? MRH_OK : MRH_ERR_HIP;
+    s->used = s->fixed_bytes;
+    if (rc == MRH_OK && s->fixed_bytes > p->hbm_budget)
+        rc = fail(ctx, MRH_ERR_BUDGET, "server: its queues need %llu bytes, budget is %llu",
+                  (unsigned long long)s->fixed_bytes, (unsigned long long)p->hbm_budget);
     if (rc == MRH_OK) s->learner = std::thread(learner_main, s);
     if (rc != MRH_OK) {
         if (s->user != nullptr) mirror_error(s, rc);
@@ -1050,10 +1128,12 @@ int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
         return fail(ctx, MRH_ERR_INVALID, "submit: request too large");
     }
     q->charge = request_charge(q);
-    if (q->charge > s->p.hbm_budget) {
+    const uint64_t need = s->fixed_bytes + r->scene->bytes + q->charge;
+    if (need > s->p.hbm_budget) {
         delete q;
-        return fail(ctx, MRH_ERR_BUDGET, "submit: request needs %llu bytes, budget is %llu",
-                    (unsigned long long)q->charge, (unsigned long long)s->p.hbm_budget);
+        return fail(ctx, MRH_ERR_BUDGET,
+                    "submit: request needs %llu bytes with its scene and the queues, budget is %llu",
+                    (unsigned long long)need, (unsigned long long)s->p.hbm_budget);
     }
     if (!r->scene->two_level) s->max_tree_depth = std::max(s->max_tree_depth, r->scene->bvh->max_depth);
     q->ticket = s->next_ticket++;
@@ -1103,6 +1183,18 @@ int32_t mrh_server_poll(mrh_server* server, mrh_request_done* out, uint32_t max,
     return MRH_OK;
 }
 
+int32_t mrh_server_cancel(mrh_server* server, uint64_t ticket)
+{
+    if (!server) return MRH_ERR_INVALID;
+    for (ServeReq* q : server->queue)
+        if (q->ticket == ticket) return mirror_error(server, cancel_request(server, q, MRH_ERR_CANCELLED));
+    for (uint32_t k = 0; k < SERVE_MAX_SLOTS; k++)
+        if (server->slots[k] && server->slots[k]->ticket == ticket)
+            return mirror_error(server, cancel_request(server, server->slots[k], MRH_ERR_CANCELLED));
+    return mirror_error(server, fail(server->ctx, MRH_ERR_INVALID, "cancel: no queued or active request %llu",
+                                     (unsigned long long)ticket));
+}
+
 int32_t mrh_server_stats(const mrh_server* server, mrh_server_info* out)
 {
     if (!server || !out) return MRH_ERR_INVALID;
@@ -1110,6 +1202,8 @@ int32_t mrh_server_stats(const mrh_server* server, mrh_server_info* out)
     out->steps = server->steps;
     out->segments = server->segments;
     out->budget_used = server->used;
+    out->fixed_bytes = server->fixed_bytes;
+    for (const auto& r : server->resident) out->scene_bytes += r.first->bytes;
     out->queued = (uint32_t)server->queue.size();
     for (uint32_t k = 0; k < SERVE_MAX_SLOTS; k++) out->active += server->slots[k] ? 1u : 0u;
     out->live_paths = server->n_live;

@@ -93,7 +93,7 @@ class GuideParams(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("wave_paths", _U32), ("max_slots", _U32),
-                ("use_graphs", _U32), ("reserved", _U32)]
+                ("use_graphs", _U32), ("timeout_ms", _U32)]
 
 
 class Request(C.Structure):
@@ -109,7 +109,8 @@ class RequestDone(C.Structure):
 
 class ServerInfo(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("segments", C.c_uint64), ("budget_used", C.c_uint64),
-                ("active", _U32), ("queued", _U32), ("live_paths", _U32), ("reserved", _U32)]
+                ("active", _U32), ("queued", _U32), ("live_paths", _U32), ("reserved", _U32),
+                ("fixed_bytes", C.c_uint64), ("scene_bytes", C.c_uint64)]
 
 
 class InstancingDesc(C.Structure):
@@ -203,6 +204,7 @@ _SIGS = {
     "mrh_server_step": (_I32, [_P, C.POINTER(_U32), C.POINTER(_U32)]),
     "mrh_server_poll": (_I32, [_P, C.POINTER(RequestDone), _U32, C.POINTER(_U32)]),
     "mrh_server_stats": (_I32, [_P, C.POINTER(ServerInfo)]),
+    "mrh_server_cancel": (_I32, [_P, C.c_uint64]),
     "mrh_server_destroy": (_I32, [_P]),
     "mrh_gen_random_tris": (_I32, [_U32, _U32, _P]),
     "mrh_gen_random_rays": (_I32, [_U32, _U32, _P]),
@@ -583,6 +585,12 @@ class Scene:
         self.ctx.check(lib().mrh_scene_trace_closest(self.ctx.h, self.h, ptr(d_rays), n, pitch, tmin, ptr(d_inst),
                                                      ptr(d_prim), ptr(d_t), ptr(d_bary), ptr(d_stats)))
 
+    def bytes(self) -> int:
+        """Device bytes the scene holds (mrh_scene_bytes)."""
+        b = C.c_uint64()
+        self.ctx.check(lib().mrh_scene_bytes(self.h, C.byref(b)))
+        return b.value
+
     def bvh(self) -> Lbvh:
         h = _P()
         self.ctx.check(lib().mrh_scene_bvh(self.h, C.byref(h)))
@@ -711,11 +719,11 @@ class Server:
     """Serving engine (mrh_server): queued render requests traced together."""
 
     def __init__(self, ctx: Context, hbm_budget: int = 200 << 30, wave_paths: int = 1 << 24,
-                 max_slots: int = 64, use_graphs: bool = True):
+                 max_slots: int = 64, use_graphs: bool = True, timeout_ms: int = 0):
         self.ctx = ctx
         ctx.adopt(self)
         self.h = _P()
-        p = ServerParams(hbm_budget, wave_paths, max_slots, 1 if use_graphs else 0, 0)
+        p = ServerParams(hbm_budget, wave_paths, max_slots, 1 if use_graphs else 0, timeout_ms)
         ctx.check(lib().mrh_server_create(ctx.h, C.byref(p), C.byref(self.h)))
         inner = _P()
         ctx.check(lib().mrh_server_context(self.h, C.byref(inner)))
@@ -734,6 +742,9 @@ class Server:
         t = C.c_uint64()
         self.ctx.check(lib().mrh_server_submit(self.h, C.byref(r), C.byref(t)))
         return t.value
+
+    def cancel(self, ticket: int) -> None:
+        self.ctx.check(lib().mrh_server_cancel(self.h, ticket))
 
     def step(self) -> tuple[int, int]:
         a, q = _U32(), _U32()

@@ -204,6 +204,73 @@ def test_lbvh_c2_full_size(ctx, oracle):
     _assert_same_lbvh(g, o, tris.shape[0])
 
 
+def _fetch_view(ctx, b, bits):
+    v = b.view()
+    n = v.n
+    cd = np.uint32 if bits == 30 else np.uint64
+    return {
+        "sorted_codes": fetch(ctx, v.d_sorted_codes, n, cd),
+        "sorted_idx": fetch(ctx, v.d_sorted_idx, n, np.uint32),
+        "left": fetch(ctx, v.d_left, n - 1, np.uint32),
+        "right": fetch(ctx, v.d_right, n - 1, np.uint32),
+        "parent": fetch(ctx, v.d_parent, 2 * n - 1, np.uint32),
+        "node_aabb": fetch(ctx, v.d_node_aabb, (2 * n - 1) * 6, np.float32).reshape(-1, 6),
+        "root": v.root, "max_depth": v.max_depth, "root4": v.root4,
+        "nodes4": fetch(ctx, v.d_nodes4, (n - 1) * 32, np.float32).reshape(n - 1, 32),
+    }
+
+
+def _live4(o):
+    ref = o.nodes4()
+    live, todo = [], [o.root4]
+    while todo:
+        p = todo.pop()
+        live.append(p)
+        todo += [int(c) for c in ref[p, 24:28].view(np.uint32) if c < 0x80000000]
+    return ref, np.array(sorted(live))
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_lbvh_rebuild_resets_state(ctx, oracle, bits):
+    """mrh_lbvh_rebuild over new corners of the same count equals a fresh
+    oracle build: codes, order, topology, boxes, depth, 4-wide nodes and ray
+    casts.  B spans a different extent and clusters toward one corner, so
+    its tree is deeper than A's; rebuilding back to A checks that the depth,
+    the extent and the per-node counters start over."""
+    n = 50_000
+    a = N.gen_random_tris(n, 61)
+    rng = np.random.default_rng(62)
+    c = (rng.random((n, 3)) ** 6) * 40.0 - 7.0                # dense near (-7,-7,-7), extent 40
+    b_tris = (np.repeat(c, 3, axis=1) + rng.uniform(-1e-3, 1e-3, (n, 9))).astype(np.float32)
+    oa, ob = oracle.LBVH(a, bits), oracle.LBVH(b_tris, bits)
+    da, db = _depth(oa.parent, n), _depth(ob.parent, n)
+    assert db > da
+    rays = N.gen_random_rays(20_000, 63)
+    rays_b = rays.copy()
+    rays_b[0:3] = rays_b[0:3] * 40.0 - 7.0
+    d_a, d_b = dev(a), dev(b_tris)
+    torch.cuda.synchronize()
+    old_width = ctx.tune_get("trace_width")
+    ctx.tune("trace_width", 4)
+    bv = N.Lbvh(ctx, d_a.data_ptr(), n, bits)
+    for tris, o, depth, r in ((b_tris, ob, db, rays_b), (a, oa, da, rays), (b_tris, ob, db, rays_b)):
+        bv.rebuild((d_b if tris is b_tris else d_a).data_ptr())
+        ctx.sync()
+        g = _fetch_view(ctx, bv, bits)
+        _assert_same_lbvh(g, o, n)
+        assert g["max_depth"] == depth
+        ref4, live = _live4(o)
+        assert g["root4"] == o.root4
+        assert g["nodes4"][live].tobytes() == ref4[live].tobytes()
+        if bits == 30:
+            p, t, _ = _gpu_trace(ctx, bv, r)
+            rp, rt, _ = o.trace(r, width=4)
+            assert np.array_equal(p, rp)
+            assert t.tobytes() == rt.tobytes()
+    bv.close()
+    ctx.tune("trace_width", old_width)
+
+
 # ------------------------------------------------------------------ traversal
 def _gpu_trace(ctx, b, rays, tmin=0.0, stats=False):
     m = rays.shape[1]

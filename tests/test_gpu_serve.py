@@ -106,11 +106,25 @@ def test_mix_matches_solo_global_partition(ctx, scenes):
     server.close()
 
 
+def _fixed_bytes(ctx, wave):
+    probe = N.Server(ctx, hbm_budget=8 << 30, wave_paths=wave, max_slots=8)
+    fixed = probe.stats().fixed_bytes
+    probe.close()
+    return fixed
+
+
 def test_budget_admission(ctx, scenes):
-    # each plain 64x48 request holds its image (36 KiB): a 100 KiB budget
-    # admits two at a time, the rest wait in the queue
-    server = N.Server(ctx, hbm_budget=100 << 10, wave_paths=1 << 16, max_slots=8, use_graphs=True)
+    """The budget covers the queues, each admitted request's scene (once) and
+    its image: beside the queues and the scene, 100 KiB admits two plain
+    64x48 requests (36 KiB images) at a time, the rest wait in the queue."""
+    wave = 1 << 16
+    fixed = _fixed_bytes(ctx, wave)
     scene = scenes["spheres"][1]
+    sb = scene.bytes()
+    budget = fixed + sb + (100 << 10)
+    server = N.Server(ctx, hbm_budget=budget, wave_paths=wave, max_slots=8, use_graphs=True)
+    st = server.stats()
+    assert st.fixed_bytes == fixed and st.budget_used == fixed and st.scene_bytes == 0
     imgs = [torch.zeros(64 * 48 * 3, dtype=torch.float32, device="cuda") for _ in range(5)]
     torch.cuda.synchronize()
     for k in range(5):
@@ -120,14 +134,120 @@ def test_budget_admission(ctx, scenes):
     while True:
         a, q = server.step()
         peak = max(peak, a)
-        assert server.stats().budget_used <= 100 << 10
+        st = server.stats()
+        assert st.budget_used <= budget
+        if a:
+            assert st.scene_bytes == sb and st.budget_used == fixed + sb + a * 64 * 48 * 12
         done += server.poll()
         if a == 0 and q == 0:
             break
     assert peak == 2
     assert sorted(d["tag"] for d in done) == list(range(5))
+    st = server.stats()
+    assert st.budget_used == fixed and st.scene_bytes == 0        # the scene is released with its last request
     with pytest.raises(N.MrhError):
         server.submit(scene, imgs[0], 1024, 1024, 1, 6, 2, 0)     # needs 12 MiB > budget
+    server.close()
+    with pytest.raises(N.MrhError):
+        N.Server(ctx, hbm_budget=fixed - 1, wave_paths=wave)      # the queues alone do not fit
+
+
+def test_scene_charged_once(ctx, scenes):
+    """Two admitted requests on one scene charge it once; a request on a
+    second scene waits until its scene fits beside the first."""
+    wave = 1 << 16
+    fixed = _fixed_bytes(ctx, wave)
+    a, b = scenes["spheres"][1], scenes["cornell"][1]
+    img = 32 * 32 * 12
+    budget = fixed + max(a.bytes(), b.bytes()) + 3 * img
+    server = N.Server(ctx, hbm_budget=budget, wave_paths=wave, max_slots=8, use_graphs=False)
+    imgs = [torch.zeros(32 * 32 * 3, dtype=torch.float32, device="cuda") for _ in range(3)]
+    torch.cuda.synchronize()
+    server.submit(a, imgs[0], 32, 32, 2, 6, 2, 1, tag=0)
+    server.submit(a, imgs[1], 32, 32, 2, 6, 2, 2, tag=1)
+    server.submit(b, imgs[2], 32, 32, 2, 6, 2, 3, tag=2)
+    act, q = server.step()
+    assert (act, q) == (2, 1)                                     # b's scene does not fit beside a's
+    assert server.stats().scene_bytes == a.bytes()
+    done = server.run()
+    assert sorted(d["tag"] for d in done) == [0, 1, 2] and all(d["status"] == 0 for d in done)
+    server.close()
+
+
+def test_cancel_and_timeout(ctx, scenes):
+    """A cancelled request ends alone with MRH_ERR_CANCELLED, queued or in
+    flight; a request older than timeout_ms ends with MRH_ERR_TIMEOUT; the
+    others still equal their solo renders."""
+    CANCELLED, TIMEOUT = 6, 7
+    jobs = [("cornell", 48, 32, 2, 6, 2, 11, 1, 0, 0), ("spheres", 64, 48, 2, 8, 3, 12, 2, 0, 0),
+            ("lit", 32, 24, 2, 10, 3, 13, 1, 1, 1), ("cornell", 40, 40, 2, 6, 2, 14, 1, 0, 0)]
+    ref = [solo(ctx, scenes[m[0]][1], *m[1:]) for m in jobs]
+    server = N.Server(ctx, hbm_budget=8 << 30, wave_paths=4096, max_slots=2, use_graphs=True)
+    imgs = [torch.zeros(m[1] * m[2] * 3, dtype=torch.float32, device="cuda") for m in jobs]
+    torch.cuda.synchronize()
+    t = []
+    for k, (name, w, h, sa, depth, rr, seed, it, nee, mis) in enumerate(jobs):
+        t.append(server.submit(scenes[name][1], imgs[k], w, h, sa, depth, rr, seed, iterations=it, svo_depth=6,
+                               guide_ratio=0.5, min_samples=16, tag=k, nee=nee, mis=mis))
+    server.step()
+    server.step()
+    server.cancel(t[1])                  # admitted, paths in flight, learning
+    server.cancel(t[3])                  # still queued: reported at once
+    with pytest.raises(N.MrhError):
+        server.cancel(12345)
+    done = server.run()
+    by = {d["tag"]: d for d in done}
+    assert sorted(by) == [0, 1, 2, 3]
+    assert by[1]["status"] == CANCELLED and by[3]["status"] == CANCELLED
+    for k in (0, 2):
+        assert by[k]["status"] == 0 and by[k]["segments"] == ref[k][1]
+        assert np.allclose(imgs[k].cpu().numpy(), ref[k][0], rtol=1e-4, atol=1e-6), k
+    st = server.stats()
+    assert st.active == 0 and st.live_paths == 0 and st.budget_used == st.fixed_bytes
+    with pytest.raises(N.MrhError):
+        server.cancel(t[0])              # already done
+    server.close()
+    # a 1 ms limit: every request times out, none blocks the server
+    server = N.Server(ctx, hbm_budget=8 << 30, wave_paths=4096, max_slots=2, use_graphs=True, timeout_ms=1)
+    for k, (name, w, h, sa, depth, rr, seed, it, nee, mis) in enumerate(jobs):
+        server.submit(scenes[name][1], imgs[k], w, h, sa, depth, rr, seed, iterations=it, svo_depth=6,
+                      guide_ratio=0.5, min_samples=16, tag=k, nee=nee, mis=mis)
+    import time
+    time.sleep(0.01)
+    done = server.run()
+    assert sorted(d["tag"] for d in done) == [0, 1, 2, 3]
+    assert all(d["status"] == TIMEOUT for d in done)
+    assert server.stats().budget_used == server.stats().fixed_bytes
+    server.close()
+
+
+def test_injected_allocation_failure_is_isolated(ctx, scenes):
+    """The guided request's learn-record allocation fails (injected on the
+    server's context): that request alone ends with MRH_ERR_OOM, the step
+    still succeeds, and the plain requests equal their solo renders."""
+    OOM = 3
+    jobs = [("cornell", 48, 32, 2, 6, 2, 21, 1, 0, 0), ("spheres", 40, 30, 2, 8, 3, 22, 3, 0, 0),
+            ("lit", 32, 24, 2, 10, 3, 23, 1, 1, 1)]
+    ref = [solo(ctx, scenes[m[0]][1], *m[1:]) for m in jobs]
+    server = N.Server(ctx, hbm_budget=8 << 30, wave_paths=4096, max_slots=4, use_graphs=True)
+    imgs = [torch.zeros(m[1] * m[2] * 3, dtype=torch.float32, device="cuda") for m in jobs]
+    torch.cuda.synchronize()
+    for k, (name, w, h, sa, depth, rr, seed, it, nee, mis) in enumerate(jobs):
+        server.submit(scenes[name][1], imgs[k], w, h, sa, depth, rr, seed, iterations=it, svo_depth=6,
+                      guide_ratio=0.5, min_samples=16, tag=k, nee=nee, mis=mis)
+    # admission allocates nothing for plain requests: the first allocation
+    # after this point is the guided request's record buffer
+    server.inner.tune("fail_alloc_after", 1)
+    done = server.run()
+    assert server.inner.tune_get("fail_alloc_after") == 0
+    by = {d["tag"]: d for d in done}
+    assert sorted(by) == [0, 1, 2]
+    assert by[1]["status"] == OOM
+    for k in (0, 2):
+        assert by[k]["status"] == 0 and by[k]["segments"] == ref[k][1]
+        assert np.allclose(imgs[k].cpu().numpy(), ref[k][0], rtol=1e-4, atol=1e-6), k
+    st = server.stats()
+    assert st.active == 0 and st.budget_used == st.fixed_bytes
     server.close()
 
 

@@ -121,6 +121,21 @@ int32_t mrh_partition_by_key(mrh_context* ctx, const uint32_t* d_keys, uint32_t 
                              uint32_t sort_begin_bit, uint32_t part_begin_bit, uint32_t end_bit,
                              uint32_t* d_order, uint32_t* d_offsets);
 
+/* mrh_partition_by_key that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, hosts that forbid allocation).  Same
+ * arguments, results and errors as mrh_partition_by_key, bit for bit.  Its
+ * workspace (look-back status, histograms, ping-pong keys and values) is a
+ * fixed 68 MiB array in device memory, one per device, held until the
+ * process ends; n above MRH_PARTITION_NOALLOC_MAX_KEYS fails with
+ * MRH_ERR_INVALID.  Every call on a device shares that device's workspace:
+ * calls that overlap on one device (two contexts, two streams, two threads)
+ * race and must be serialised by the caller.  Records no profiler events. */
+#define MRH_PARTITION_NOALLOC_MAX_KEYS (1u << 22)
+int32_t mrh_partition_by_key_noalloc(mrh_context* ctx, const uint32_t* d_keys, uint32_t n,
+                                     uint32_t sort_begin_bit, uint32_t part_begin_bit,
+                                     uint32_t end_bit, uint32_t* d_order, uint32_t* d_offsets);
+
 /* ------------------------------------------------------------------ */
 /* LBVH.  Replaces LinearBVHCPU::Construct (TracerCUDA/LinearBVH.hpp:423-596)
  * and, as the ray-casting accelerator, GPUAccBVHGroup::ConstructAccelerator

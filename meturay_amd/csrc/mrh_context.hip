@@ -97,9 +97,12 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t count)
         p[i] = v;
 }
 
-int32_t fill_u32(mrh_context* ctx, uint32_t* p, uint32_t value, size_t count)
+int32_t fill_u32(mrh_context* ctx, uint32_t* p, uint32_t value, size_t count, bool plain)
 {
     if (count == 0) return MRH_OK;
+    if (plain)
+        return launch_plain(ctx, "fill_u32", dim3(stride_grid(ctx, count, 256)), dim3(256), 0,
+                            k_fill_u32, p, value, count);
     return launch(ctx, "fill_u32", dim3(stride_grid(ctx, count, 256)), dim3(256), 0, k_fill_u32, p,
                   value, count);
 }
@@ -130,6 +133,8 @@ int32_t mrh_context_create(int32_t device, mrh_context** out)
         delete ctx;
         return MRH_ERR_HIP;
     }
+    // a failed lookup only disables mrh_partition_by_key_noalloc (its error)
+    (void)noalloc_setup(ctx);
     *out = ctx;
     return MRH_OK;
 }

@@ -63,6 +63,10 @@ struct mrh_context {
     uint64_t fail_alloc_after = 0;
     // octree reused by guided renders that do not bring their own
     struct mrh_svo* guide_svo = nullptr;
+    // this device's fixed partition workspace (mrh_partition_by_key_noalloc);
+    // null when its address lookup failed (the reason in noalloc_err)
+    void* noalloc_scratch = nullptr;
+    std::string noalloc_err;
 };
 
 namespace mrh {
@@ -111,8 +115,23 @@ int32_t launch(mrh_context* ctx, const char* name, dim3 grid, dim3 block, size_t
     return MRH_OK;
 }
 
-// Fill kernel launcher (graph-safe replacement for hipMemsetAsync).
-int32_t fill_u32(mrh_context* ctx, uint32_t* p, uint32_t value, size_t count);
+// launch() without the profiler's events: for entry points that must not
+// create runtime objects (mrh_partition_by_key_noalloc).
+template <class Kernel, class... Args>
+int32_t launch_plain(mrh_context* ctx, const char* name, dim3 grid, dim3 block, size_t shmem,
+                     Kernel kernel, Args... args)
+{
+    if (grid.x == 0) return MRH_OK;
+    hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(ctx, MRH_ERR_HIP, "launch of %s failed: %s", name, hipGetErrorString(e));
+    return MRH_OK;
+}
+
+// Fill kernel launcher (graph-safe replacement for hipMemsetAsync); plain =
+// no profiler events.
+int32_t fill_u32(mrh_context* ctx, uint32_t* p, uint32_t value, size_t count, bool plain = false);
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
@@ -144,6 +163,23 @@ struct SortArgs {
     uint32_t* part_offsets = nullptr;
     uint32_t part_begin = 0;
 };
-int32_t radix_sort(mrh_context* ctx, const SortArgs& a);
+// Workspace of a sort: the meta words (digit histograms, partition counts,
+// tile counters, look-back status) and the ping-pong keys/values of the
+// intermediate passes.  radix_sort takes them from the context's grow-only
+// slots when no workspace is given.
+struct SortSpace {
+    uint32_t* meta = nullptr;
+    size_t meta_words = 0;
+    void* keys_a = nullptr;
+    void* keys_b = nullptr;
+    uint32_t* vals_a = nullptr;
+    uint32_t* vals_b = nullptr;
+    uint32_t max_n = 0;          // elements each ping-pong buffer holds
+};
+int32_t radix_sort(mrh_context* ctx, const SortArgs& a, const SortSpace* space = nullptr);
+
+// Device address of this device's fixed partition workspace, looked up once
+// per device (mrh_context_create calls it with the device current).
+int32_t noalloc_setup(mrh_context* ctx);
 
 }  // namespace mrh

@@ -219,7 +219,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_onesweep(
 }
 
 template <class KeyT>
-int32_t radix_sort_t(mrh_context* ctx, const SortArgs& a)
+int32_t radix_sort_t(mrh_context* ctx, const SortArgs& a, const SortSpace* space)
 {
     const uint32_t nbits_total = a.end_bit - a.begin_bit;
     const uint32_t passes = (nbits_total + 7) / 8;
@@ -244,21 +244,36 @@ int32_t radix_sort_t(mrh_context* ctx, const SortArgs& a)
     }
 
     const size_t meta_words = META_STATUS + (size_t)passes * tiles * 256;
-    void* meta_v = nullptr;
-    MRH_TRY(scratch(ctx, SLOT_SORT_META, meta_words * 4, &meta_v));
-    uint32_t* meta = (uint32_t*)meta_v;
-    MRH_TRY(fill_u32(ctx, meta, 0u, meta_words));
-
-    // ping-pong buffers for intermediate passes
+    const bool plain = space != nullptr;   // a fixed workspace: no allocation, no events
+    uint32_t* meta = nullptr;
     void *ka = nullptr, *kb = nullptr, *va = nullptr, *vb = nullptr;
-    if (passes > 1) {
-        MRH_TRY(scratch(ctx, SLOT_SORT_KEYS_A, (size_t)a.n_max * sizeof(KeyT), &ka));
-        MRH_TRY(scratch(ctx, SLOT_SORT_VALS_A, (size_t)a.n_max * 4, &va));
-        if (passes > 2) {
-            MRH_TRY(scratch(ctx, SLOT_SORT_KEYS_B, (size_t)a.n_max * sizeof(KeyT), &kb));
-            MRH_TRY(scratch(ctx, SLOT_SORT_VALS_B, (size_t)a.n_max * 4, &vb));
+    if (space) {
+        if (meta_words > space->meta_words || a.n_max > space->max_n ||
+            (passes > 1 && (!space->keys_a || !space->vals_a)) ||
+            (passes > 2 && (!space->keys_b || !space->vals_b)))
+            return fail(ctx, MRH_ERR_INVALID, "radix_sort: %u keys over %u passes exceed the workspace",
+                        a.n_max, passes);
+        meta = space->meta;
+        ka = space->keys_a;
+        kb = space->keys_b;
+        va = space->vals_a;
+        vb = space->vals_b;
+    } else {
+        void* meta_v = nullptr;
+        MRH_TRY(scratch(ctx, SLOT_SORT_META, meta_words * 4, &meta_v));
+        meta = (uint32_t*)meta_v;
+        // ping-pong buffers for intermediate passes
+        if (passes > 1) {
+            MRH_TRY(scratch(ctx, SLOT_SORT_KEYS_A, (size_t)a.n_max * sizeof(KeyT), &ka));
+            MRH_TRY(scratch(ctx, SLOT_SORT_VALS_A, (size_t)a.n_max * 4, &va));
+            if (passes > 2) {
+                MRH_TRY(scratch(ctx, SLOT_SORT_KEYS_B, (size_t)a.n_max * sizeof(KeyT), &kb));
+                MRH_TRY(scratch(ctx, SLOT_SORT_VALS_B, (size_t)a.n_max * 4, &vb));
+            }
         }
     }
+    // histograms, counters and look-back words start at zero on every call
+    MRH_TRY(fill_u32(ctx, meta, 0u, meta_words, plain));
 
     // Each block flushes passes x 256 counters with global atomics, so the
     // grid is sized for >= 4096 keys per block and at most 4 blocks per CU:
@@ -266,10 +281,15 @@ int32_t radix_sort_t(mrh_context* ctx, const SortArgs& a)
     uint32_t hist_blocks = div_up(a.n_max, 4096u);
     hist_blocks = std::min<uint32_t>(hist_blocks, ctx->num_cus * 4u);
     hist_blocks = std::max<uint32_t>(hist_blocks, 1u);
-    MRH_TRY(launch(ctx, "sort_hist", dim3(hist_blocks), dim3(256), 0,
-                   k_sort_hist<KeyT>, (const KeyT*)a.keys_in, a.n_max, a.n_dev, pd, meta));
-    MRH_TRY(launch(ctx, "sort_scan", dim3(1), dim3(256), 0, k_sort_scan, meta, passes,
-                   a.part_offsets, part_digits, a.n_max, a.n_dev));
+    auto run = [&](const char* name, dim3 grid, auto kernel, auto... args) -> int32_t {
+        return plain ? launch_plain(ctx, name, grid, dim3(256), 0, kernel, args...)
+                     : launch(ctx, name, grid, dim3(256), 0, kernel, args...);
+    };
+    static_assert(SORT_THREADS == 256, "every sort kernel runs 256-thread blocks");
+    MRH_TRY(run("sort_hist", dim3(hist_blocks), k_sort_hist<KeyT>, (const KeyT*)a.keys_in, a.n_max,
+                a.n_dev, pd, meta));
+    MRH_TRY(run("sort_scan", dim3(1), k_sort_scan, meta, passes, a.part_offsets, part_digits, a.n_max,
+                a.n_dev));
 
     const KeyT* kin = (const KeyT*)a.keys_in;
     const uint32_t* vin = a.vals_in;
@@ -277,20 +297,64 @@ int32_t radix_sort_t(mrh_context* ctx, const SortArgs& a)
         const bool last = (p + 1 == passes);
         KeyT* kout = last ? (KeyT*)a.keys_out : (KeyT*)((p & 1) ? kb : ka);
         uint32_t* vout = last ? a.vals_out : (uint32_t*)((p & 1) ? vb : va);
-        MRH_TRY(launch(ctx, "sort_onesweep", dim3(tiles), dim3(SORT_THREADS), 0, k_onesweep<KeyT>,
-                       kin, vin, kout, vout, a.n_max, a.n_dev, pd.shift[p],
-                       (uint32_t)__builtin_popcount(pd.mask[p]), pd.flip[p],
-                       (const uint32_t*)(meta + META_HIST + p * 256),
-                       meta + META_STATUS + (size_t)p * tiles * 256, meta + META_CTR + p));
+        MRH_TRY(run("sort_onesweep", dim3(tiles), k_onesweep<KeyT>, kin, vin, kout, vout, a.n_max,
+                    a.n_dev, pd.shift[p], (uint32_t)__builtin_popcount(pd.mask[p]), pd.flip[p],
+                    (const uint32_t*)(meta + META_HIST + p * 256),
+                    meta + META_STATUS + (size_t)p * tiles * 256, meta + META_CTR + p));
         kin = kout;
         vin = vout;
     }
     return MRH_OK;
 }
 
+// Fixed workspace of mrh_partition_by_key_noalloc: up to 2^22 keys (32-bit
+// keys, so at most 4 passes): the meta words, then keys A, values A, keys B,
+// values B of NOALLOC_MAX_KEYS words each.
+constexpr uint32_t NOALLOC_MAX_KEYS = MRH_PARTITION_NOALLOC_MAX_KEYS;
+constexpr size_t NOALLOC_META_WORDS =
+    (META_STATUS + (size_t)4 * (NOALLOC_MAX_KEYS / SORT_TILE) * 256 + 63) / 64 * 64;
+constexpr size_t NOALLOC_WORDS = NOALLOC_META_WORDS + (size_t)4 * NOALLOC_MAX_KEYS;
+static_assert(NOALLOC_MAX_KEYS % SORT_TILE == 0, "whole tiles");
+
+}  // namespace
+}  // namespace mrh
+
+// One copy per device, held until the process ends.  Host code never uses
+// this name as an address: noalloc_setup looks up the device address of
+// each device's copy, and the kernels receive that.
+static __device__ __align__(256) uint32_t scratch[mrh::NOALLOC_WORDS];
+
+namespace mrh {
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
 }  // namespace
 
-int32_t radix_sort(mrh_context* ctx, const SortArgs& a)
+int32_t noalloc_setup(mrh_context* ctx)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) {
+        ctx->noalloc_err = "device index beyond the fixed workspace table";
+        return MRH_ERR_UNSUPPORTED;
+    }
+    void*& p = g_noalloc_ptr[ctx->device];
+    if (!p) {
+        void* addr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(::scratch));
+        if (e != hipSuccess || !addr) {
+            (void)hipGetLastError();
+            ctx->noalloc_err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+            return MRH_ERR_HIP;
+        }
+        p = addr;
+    }
+    ctx->noalloc_scratch = p;
+    return MRH_OK;
+}
+
+int32_t radix_sort(mrh_context* ctx, const SortArgs& a, const SortSpace* space)
 {
     if (a.n_max == 0) return MRH_OK;
     if (a.n_max >= (1u << 30))
@@ -303,8 +367,8 @@ int32_t radix_sort(mrh_context* ctx, const SortArgs& a)
     if (!a.vals_out) return fail(ctx, MRH_ERR_INVALID, "radix_sort: vals_out is required");
     if (a.part_offsets && (a.end_bit - a.part_begin > 8 || a.part_begin < a.begin_bit))
         return fail(ctx, MRH_ERR_INVALID, "radix_sort: partition bits must be <= 8 wide and inside the sorted range");
-    if (a.key_bytes == 4) return radix_sort_t<uint32_t>(ctx, a);
-    if (a.key_bytes == 8) return radix_sort_t<uint64_t>(ctx, a);
+    if (a.key_bytes == 4) return radix_sort_t<uint32_t>(ctx, a, space);
+    if (a.key_bytes == 8) return radix_sort_t<uint64_t>(ctx, a, space);
     return fail(ctx, MRH_ERR_INVALID, "radix_sort: key_bytes must be 4 or 8");
 }
 
@@ -362,10 +426,54 @@ int32_t mrh_partition_by_key(mrh_context* ctx, const uint32_t* d_keys, uint32_t 
     // intermediate passes need key storage even when the caller wants none
     if ((end_bit - sort_begin_bit + 7) / 8 > 1) {
         void* kout = nullptr;
-        MRH_TRY(scratch(ctx, SLOT_PART_KEYS, (size_t)n * 4, &kout));
+        MRH_TRY(mrh::scratch(ctx, SLOT_PART_KEYS, (size_t)n * 4, &kout));
         a.keys_out = kout;
     }
     return radix_sort(ctx, a);
+}
+
+int32_t mrh_partition_by_key_noalloc(mrh_context* ctx, const uint32_t* d_keys, uint32_t n,
+                                     uint32_t sort_begin_bit, uint32_t part_begin_bit,
+                                     uint32_t end_bit, uint32_t* d_order, uint32_t* d_offsets)
+{
+    if (!ctx) return MRH_ERR_INVALID;
+    if (!d_order || !d_offsets || (n && !d_keys))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_partition_by_key_noalloc: null argument");
+    if (end_bit > 32 || part_begin_bit < sort_begin_bit || end_bit <= part_begin_bit ||
+        end_bit - part_begin_bit > 8)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_partition_by_key_noalloc: bad bit range");
+    if (n > NOALLOC_MAX_KEYS)
+        return fail(ctx, MRH_ERR_INVALID,
+                    "mrh_partition_by_key_noalloc: n = %u exceeds the fixed workspace of %u keys "
+                    "(mrh_partition_by_key has no limit)", n, NOALLOC_MAX_KEYS);
+    if (!ctx->noalloc_scratch)
+        return fail(ctx, MRH_ERR_HIP, "mrh_partition_by_key_noalloc: no workspace on device %d: %s",
+                    ctx->device, ctx->noalloc_err.c_str());
+    if (n == 0) {
+        uint32_t digits = 1u << (end_bit - part_begin_bit);
+        return fill_u32(ctx, d_offsets, 0u, digits + 1u, true);
+    }
+    uint32_t* base = (uint32_t*)ctx->noalloc_scratch;
+    SortSpace ws;
+    ws.meta = base;
+    ws.meta_words = NOALLOC_META_WORDS;
+    ws.keys_a = base + NOALLOC_META_WORDS;
+    ws.vals_a = base + NOALLOC_META_WORDS + (size_t)NOALLOC_MAX_KEYS;
+    ws.keys_b = base + NOALLOC_META_WORDS + (size_t)2 * NOALLOC_MAX_KEYS;
+    ws.vals_b = base + NOALLOC_META_WORDS + (size_t)3 * NOALLOC_MAX_KEYS;
+    ws.max_n = NOALLOC_MAX_KEYS;
+    SortArgs a;
+    a.keys_in = d_keys;
+    a.keys_out = nullptr;     // the last pass writes only the order
+    a.vals_out = d_order;
+    a.n_max = n;
+    a.key_bytes = 4;
+    a.begin_bit = sort_begin_bit;
+    a.end_bit = end_bit;
+    a.descending = true;
+    a.part_offsets = d_offsets;
+    a.part_begin = part_begin_bit;
+    return radix_sort(ctx, a, &ws);
 }
 
 }  // extern "C"

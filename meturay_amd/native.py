@@ -146,6 +146,7 @@ _SIGS = {
     "mrh_morton_codes": (_I32, [_P, _P, _U32, _U32, _P]),
     "mrh_sort_pairs": (_I32, [_P, _P, _P, _P, _P, _U32, _U32, _U32, _U32, _I32]),
     "mrh_partition_by_key": (_I32, [_P, _P, _U32, _U32, _U32, _U32, _P, _P]),
+    "mrh_partition_by_key_noalloc": (_I32, [_P, _P, _U32, _U32, _U32, _U32, _P, _P]),
     "mrh_lbvh_build": (_I32, [_P, _P, _U32, _U32, C.POINTER(_P)]),
     "mrh_scene_trace_closest": (_I32, [_P, _P, _P, _U32, _U32, _F, _P, _P, _P, _P, _P]),
     "mrh_lbvh_view_get": (_I32, [_P, C.POINTER(LbvhView)]),

@@ -90,7 +90,8 @@ struct TreeSel {
 inline int32_t select_tree(mrh_context* ctx, const mrh_lbvh* b, TreeSel& t)
 {
     uint32_t w = ctx->trace_width;
-    if (w == 4 && b->max_depth > 84) w = 2;
+    // the 4-wide stack bound, and 32-bit byte offsets of its node records
+    if (w == 4 && (b->max_depth > 84 || b->n > (1u << 25))) w = 2;
     t.width = w;
     if (w == 4) {
         t.nodes = b->nodes4;
@@ -136,14 +137,23 @@ struct PrivateStack {
     }
 };
 
+// LDS pointers with their address space spelled out, so no access through
+// them is compiled as a generic (flat) access.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) float lds_f32;
+
 template <int L, int N, int BLOCK>
 struct LdsStack {
     static constexpr int CAP = N;
-    uint32_t* s_node;     // [L][BLOCK]
-    float* s_t;           // [L][BLOCK]
+    lds_u32* s_node;      // [L][BLOCK]
+    lds_f32* s_t;         // [L][BLOCK]
     uint32_t slot;        // threadIdx.x
     uint2* overflow;      // [(N-L)][ostride]
     size_t ostride, oslot;
+    __device__ __forceinline__ LdsStack(uint32_t* node, float* t, uint32_t sl, uint2* ovf, size_t ost, size_t osl)
+        : s_node((lds_u32*)node), s_t((lds_f32*)t), slot(sl), overflow(ovf), ostride(ost), oslot(osl)
+    {
+    }
     __device__ __forceinline__ void put(int i, uint32_t n, float d)
     {
         if (i < L) {
@@ -270,13 +280,127 @@ __device__ __forceinline__ TraceResult traverse(Stack& st, const float4* __restr
     }
 }
 
+// Per-ray constants of the 4-wide node test, the slab test with each
+// axis's two planes ordered by the sign of the direction: the near plane
+// (the min plane where the reciprocal direction is non-negative, else the
+// max plane) gives that axis's entry distance and the far plane its exit,
+// each as fma(plane, inv, -o * inv).  A box then costs six fused
+// multiply-adds and no per-axis min/max; the near/far choice is made once
+// per ray, as byte offsets into the node record.  Direction components
+// below 2^-64 in magnitude are replaced by +-2^-64 (sign kept), so every
+// plane distance is finite and no 0 * inf arises.
+struct RaySlab {
+    V3 inv, noi;             // reciprocal direction, -o * inv
+    uint32_t nx, ny, nz;     // byte offsets of the near planes in a 4-wide node (far = near ^ FAR_*)
+};
+// near <-> far plane offsets: x 0 <-> 48, y 16 <-> 64, z 32 <-> 80
+constexpr uint32_t FAR_X = 48u, FAR_Y = 80u, FAR_Z = 112u;
+
+__device__ __forceinline__ float slab_rcp(float d)
+{
+    const float tiny = 0x1p-64f;
+    return 1.0f / (fabsf(d) < tiny ? copysignf(tiny, d) : d);
+}
+
+__device__ __forceinline__ RaySlab make_slab(V3 o, V3 d)
+{
+    RaySlab s;
+    s.inv = v3(slab_rcp(d.x), slab_rcp(d.y), slab_rcp(d.z));
+    s.noi = v3(-(o.x * s.inv.x), -(o.y * s.inv.y), -(o.z * s.inv.z));
+    // node layout: min x/y/z at bytes 0/16/32, max x/y/z at 48/64/80
+    s.nx = signbit(s.inv.x) ? 48u : 0u;
+    s.ny = signbit(s.inv.y) ? 64u : 16u;
+    s.nz = signbit(s.inv.z) ? 80u : 32u;
+    return s;
+}
+
+// The 4 slots of the node at byte offset nb from base: slot j is entered
+// when max(tmin, entry distances) <= min(tmax, exit distances); k.j = its
+// entry distance.  Empty slots hold (+inf, -inf) boxes, whose entry is +inf
+// and exit -inf for every ray, so they are never entered.
+struct Node4Test {
+    float4 k;
+    uint4 c;
+    bool h0, h1, h2, h3;
+};
+
+__device__ __forceinline__ Node4Test node4_test(const char* __restrict__ base, uint32_t nb, const RaySlab& s,
+                                                float tmin, float tmax)
+{
+    // nb has its low 7 bits clear, so | and ^ act on the offsets alone
+    const uint32_t ax = nb | s.nx, ay = nb | s.ny, az = nb | s.nz;
+    const float4 ex = *(const float4*)(base + (size_t)ax);
+    const float4 ey = *(const float4*)(base + (size_t)ay);
+    const float4 ez = *(const float4*)(base + (size_t)az);
+    const float4 xx = *(const float4*)(base + (size_t)(ax ^ FAR_X));
+    const float4 xy = *(const float4*)(base + (size_t)(ay ^ FAR_Y));
+    const float4 xz = *(const float4*)(base + (size_t)(az ^ FAR_Z));
+    const float4 cc = *(const float4*)(base + (size_t)(nb | 96u));
+    Node4Test q;
+    q.c = make_uint4(__float_as_uint(cc.x), __float_as_uint(cc.y), __float_as_uint(cc.z), __float_as_uint(cc.w));
+    auto slot = [&](float ax, float ay, float az, float bx, float by, float bz, float& k) {
+        const float te = fmaxf(fmaxf(fmaxf(__builtin_fmaf(ax, s.inv.x, s.noi.x), __builtin_fmaf(ay, s.inv.y, s.noi.y)),
+                                     __builtin_fmaf(az, s.inv.z, s.noi.z)),
+                               tmin);
+        const float tx = fminf(fminf(fminf(__builtin_fmaf(bx, s.inv.x, s.noi.x), __builtin_fmaf(by, s.inv.y, s.noi.y)),
+                                     __builtin_fmaf(bz, s.inv.z, s.noi.z)),
+                               tmax);
+        k = te;
+        return tx >= te;
+    };
+    q.h0 = slot(ex.x, ey.x, ez.x, xx.x, xy.x, xz.x, q.k.x);
+    q.h1 = slot(ex.y, ey.y, ez.y, xx.y, xy.y, xz.y, q.k.y);
+    q.h2 = slot(ex.z, ey.z, ez.z, xx.z, xy.z, xz.z, q.k.z);
+    q.h3 = slot(ex.w, ey.w, ez.w, xx.w, xy.w, xz.w, q.k.w);
+    return q;
+}
+
+// Orders the entered slots of a node by (entry distance, slot), pushes all
+// but the nearest farthest first, and returns the nearest (NO_HIT if none).
+template <class Stack>
+__device__ __forceinline__ uint32_t node4_order_push(Stack& st, int& sp, const Node4Test& q)
+{
+    const int cnt = (int)q.h0 + (int)q.h1 + (int)q.h2 + (int)q.h3;
+    if (cnt == 0) return NO_HIT;
+    float k0 = q.h0 ? q.k.x : INFINITY, k1 = q.h1 ? q.k.y : INFINITY, k2 = q.h2 ? q.k.z : INFINITY,
+          k3 = q.h3 ? q.k.w : INFINITY;
+    uint32_t c0 = q.c.x, c1 = q.c.y, c2 = q.c.z, c3 = q.c.w;
+    uint32_t s0 = 0, s1 = 1, s2 = 2, s3 = 3;
+    // (key, slot) lexicographic compare-exchange: the order is unique
+    auto cx = [](float& ka, uint32_t& sa, uint32_t& ca, float& kb, uint32_t& sb, uint32_t& cb) {
+        const bool sw = (ka > kb) || (ka == kb && sa > sb);
+        const float tk = sw ? kb : ka;
+        kb = sw ? ka : kb;
+        ka = tk;
+        const uint32_t ts = sw ? sb : sa;
+        sb = sw ? sa : sb;
+        sa = ts;
+        const uint32_t tc = sw ? cb : ca;
+        cb = sw ? ca : cb;
+        ca = tc;
+    };
+    cx(k0, s0, c0, k1, s1, c1);
+    cx(k2, s2, c2, k3, s3, c3);
+    cx(k0, s0, c0, k2, s2, c2);
+    cx(k1, s1, c1, k3, s3, c3);
+    cx(k1, s1, c1, k2, s2, c2);
+    // push farthest first; the guard keeps a malformed tree from writing
+    // past the stack (the host sizes CAP for the tree depth)
+    if (cnt >= 4 && sp < Stack::CAP) st.put(sp++, c3, k3);
+    if (cnt >= 3 && sp < Stack::CAP) st.put(sp++, c2, k2);
+    if (cnt >= 2 && sp < Stack::CAP) st.put(sp++, c1, k1);
+    return c0;
+}
+
 // Closest / any hit over the 4-wide tree.  Node layout (8 float4, 128 B):
 // min x[4], min y[4], min z[4], max x[4], max y[4], max z[4], child code[4]
 // (LEAF_BIT | leaf, internal index, or NO_HIT for an empty slot), pad.  The
-// slots whose boxes the ray enters are ordered by (entry distance, slot)
-// with a 5-comparator network; the nearest is taken next and the others are
-// pushed farthest first, culled on pop once their entry distance exceeds
-// tmax.  Acceptance as traverse().
+// slots whose boxes the ray enters (node4_test) are ordered by (entry
+// distance, slot) with a 5-comparator network; the nearest is taken next
+// and the others are pushed farthest first, culled on pop once their entry
+// distance exceeds tmax.  Acceptance as traverse().  Node records are
+// addressed by 32-bit byte offsets (internal index << 7), so a tree holds
+// at most 2^25 internal nodes (select_tree falls back to the binary tree).
 template <bool ANY, bool STATS, class Stack>
 __device__ __forceinline__ TraceResult traverse4(Stack& st, const float4* __restrict__ nodes4,
                                                  const float4* __restrict__ leaves, uint32_t root,
@@ -287,7 +411,8 @@ __device__ __forceinline__ TraceResult traverse4(Stack& st, const float4* __rest
     r.t = tmax;
     r.ba = r.bb = 0.0f;
     r.visits = r.tests = 0;
-    const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const RaySlab sl = make_slab(o, d);
+    const char* nbase = (const char*)nodes4;
     int sp = 0;
     uint32_t next = root;
     for (;;) {
@@ -336,49 +461,7 @@ __device__ __forceinline__ TraceResult traverse4(Stack& st, const float4* __rest
             break;
         }
         if (STATS) r.visits++;
-        const float4* nd = nodes4 + 8 * (size_t)next;
-        const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5], cc = nd[6];
-        uint32_t c0 = __float_as_uint(cc.x), c1 = __float_as_uint(cc.y), c2 = __float_as_uint(cc.z),
-                 c3 = __float_as_uint(cc.w);
-        float k0, k1, k2, k3;
-        bool h0 = intersect_box(o, inv, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tmin, r.t, k0) && c0 != NO_HIT;
-        bool h1 = intersect_box(o, inv, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tmin, r.t, k1) && c1 != NO_HIT;
-        bool h2 = intersect_box(o, inv, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tmin, r.t, k2) && c2 != NO_HIT;
-        bool h3 = intersect_box(o, inv, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tmin, r.t, k3) && c3 != NO_HIT;
-        const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-        if (cnt == 0) {
-            next = NO_HIT;
-            continue;
-        }
-        k0 = h0 ? k0 : INFINITY;
-        k1 = h1 ? k1 : INFINITY;
-        k2 = h2 ? k2 : INFINITY;
-        k3 = h3 ? k3 : INFINITY;
-        uint32_t s0 = 0, s1 = 1, s2 = 2, s3 = 3;
-        // (key, slot) lexicographic compare-exchange: the order is unique
-        auto cx = [](float& ka, uint32_t& sa, uint32_t& ca, float& kb, uint32_t& sb, uint32_t& cb) {
-            const bool sw = (ka > kb) || (ka == kb && sa > sb);
-            const float tk = sw ? kb : ka;
-            kb = sw ? ka : kb;
-            ka = tk;
-            const uint32_t ts = sw ? sb : sa;
-            sb = sw ? sa : sb;
-            sa = ts;
-            const uint32_t tc = sw ? cb : ca;
-            cb = sw ? ca : cb;
-            ca = tc;
-        };
-        cx(k0, s0, c0, k1, s1, c1);
-        cx(k2, s2, c2, k3, s3, c3);
-        cx(k0, s0, c0, k2, s2, c2);
-        cx(k1, s1, c1, k3, s3, c3);
-        cx(k1, s1, c1, k2, s2, c2);
-        // push farthest first; the guard keeps a malformed tree from
-        // writing past the stack (the host sizes CAP for the tree depth)
-        if (cnt >= 4 && sp < Stack::CAP) st.put(sp++, c3, k3);
-        if (cnt >= 3 && sp < Stack::CAP) st.put(sp++, c2, k2);
-        if (cnt >= 2 && sp < Stack::CAP) st.put(sp++, c1, k1);
-        next = c0;
+        next = node4_order_push(st, sp, node4_test(nbase, next << 7, sl, tmin, r.t));
     }
 }
 
@@ -469,7 +552,8 @@ __device__ __forceinline__ TraceResult traverse_two_level(Stack& st, const float
     r.ba = r.bb = 0.0f;
     r.visits = r.tests = 0;
     hit_inst = NO_HIT;
-    const V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const RaySlab sl = make_slab(o, d);
+    const char* tbase = (const char*)top4;
     int sp = 0;
     uint32_t next = top_root;
     for (;;) {
@@ -535,46 +619,7 @@ __device__ __forceinline__ TraceResult traverse_two_level(Stack& st, const float
             break;
         }
         if (STATS) r.visits++;
-        const float4* nd = top4 + 8 * (size_t)next;
-        const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5], cc = nd[6];
-        uint32_t c0 = __float_as_uint(cc.x), c1 = __float_as_uint(cc.y), c2 = __float_as_uint(cc.z),
-                 c3 = __float_as_uint(cc.w);
-        float k0, k1, k2, k3;
-        bool h0 = intersect_box(o, inv, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tmin, r.t, k0) && c0 != NO_HIT;
-        bool h1 = intersect_box(o, inv, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tmin, r.t, k1) && c1 != NO_HIT;
-        bool h2 = intersect_box(o, inv, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tmin, r.t, k2) && c2 != NO_HIT;
-        bool h3 = intersect_box(o, inv, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tmin, r.t, k3) && c3 != NO_HIT;
-        const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-        if (cnt == 0) {
-            next = NO_HIT;
-            continue;
-        }
-        k0 = h0 ? k0 : INFINITY;
-        k1 = h1 ? k1 : INFINITY;
-        k2 = h2 ? k2 : INFINITY;
-        k3 = h3 ? k3 : INFINITY;
-        uint32_t s0 = 0, s1 = 1, s2 = 2, s3 = 3;
-        auto cx = [](float& ka, uint32_t& sa, uint32_t& ca, float& kb, uint32_t& sb, uint32_t& cb) {
-            const bool sw = (ka > kb) || (ka == kb && sa > sb);
-            const float tk = sw ? kb : ka;
-            kb = sw ? ka : kb;
-            ka = tk;
-            const uint32_t ts = sw ? sb : sa;
-            sb = sw ? sa : sb;
-            sa = ts;
-            const uint32_t tc = sw ? cb : ca;
-            cb = sw ? ca : cb;
-            ca = tc;
-        };
-        cx(k0, s0, c0, k1, s1, c1);
-        cx(k2, s2, c2, k3, s3, c3);
-        cx(k0, s0, c0, k2, s2, c2);
-        cx(k1, s1, c1, k3, s3, c3);
-        cx(k1, s1, c1, k2, s2, c2);
-        if (cnt >= 4 && sp < Stack::CAP) st.put(sp++, c3, k3);
-        if (cnt >= 3 && sp < Stack::CAP) st.put(sp++, c2, k2);
-        if (cnt >= 2 && sp < Stack::CAP) st.put(sp++, c1, k1);
-        next = c0;
+        next = node4_order_push(st, sp, node4_test(tbase, next << 7, sl, tmin, r.t));
     }
 }
 
@@ -600,7 +645,7 @@ __device__ __forceinline__ TraceResult traverse_two_level4(Stack& st, const floa
     r.ba = r.bb = 0.0f;
     r.visits = r.tests = 0;
     hit_inst = NO_HIT;
-    V3 inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    RaySlab sl = make_slab(o, d);
     int sp = 0;
     int base = -1;                  // stack depth at the current instance's entry (-1: top tree)
     uint32_t cur = 0;               // current instance
@@ -616,7 +661,7 @@ __device__ __forceinline__ TraceResult traverse_two_level4(Stack& st, const floa
                 while (!found) {
                     if (sp == base) {   // the instance's tree is done: back to the world ray
                         world(o, d);
-                        inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                        sl = make_slab(o, d);
                         base = -1;
                         nodes = top4;
                         lv = top_leaves;
@@ -665,7 +710,7 @@ __device__ __forceinline__ TraceResult traverse_two_level4(Stack& st, const floa
                 const float4 m6 = rec[6], m7 = rec[7];
                 o = xf_point(w0, w1, w2, o);
                 d = xf_dir(w0, w1, w2, d);
-                inv = v3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                sl = make_slab(o, d);
                 base = sp;
                 cur = ii;
                 n_cur = __float_as_uint(m7.w);
@@ -682,46 +727,7 @@ __device__ __forceinline__ TraceResult traverse_two_level4(Stack& st, const floa
         }
         if (done || (ANY && r.leaf != NO_HIT)) break;
         if (STATS) r.visits++;
-        const float4* nd = nodes + 8 * (size_t)next;
-        const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5], cc = nd[6];
-        uint32_t c0 = __float_as_uint(cc.x), c1 = __float_as_uint(cc.y), c2 = __float_as_uint(cc.z),
-                 c3 = __float_as_uint(cc.w);
-        float k0, k1, k2, k3;
-        bool h0 = intersect_box(o, inv, mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, tmin, r.t, k0) && c0 != NO_HIT;
-        bool h1 = intersect_box(o, inv, mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, tmin, r.t, k1) && c1 != NO_HIT;
-        bool h2 = intersect_box(o, inv, mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, tmin, r.t, k2) && c2 != NO_HIT;
-        bool h3 = intersect_box(o, inv, mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, tmin, r.t, k3) && c3 != NO_HIT;
-        const int cnt = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-        if (cnt == 0) {
-            next = NO_HIT;
-            continue;
-        }
-        k0 = h0 ? k0 : INFINITY;
-        k1 = h1 ? k1 : INFINITY;
-        k2 = h2 ? k2 : INFINITY;
-        k3 = h3 ? k3 : INFINITY;
-        uint32_t s0 = 0, s1 = 1, s2 = 2, s3 = 3;
-        auto cx = [](float& ka, uint32_t& sa, uint32_t& ca, float& kb, uint32_t& sb, uint32_t& cb) {
-            const bool sw = (ka > kb) || (ka == kb && sa > sb);
-            const float tk = sw ? kb : ka;
-            kb = sw ? ka : kb;
-            ka = tk;
-            const uint32_t ts = sw ? sb : sa;
-            sb = sw ? sa : sb;
-            sa = ts;
-            const uint32_t tc = sw ? cb : ca;
-            cb = sw ? ca : cb;
-            ca = tc;
-        };
-        cx(k0, s0, c0, k1, s1, c1);
-        cx(k2, s2, c2, k3, s3, c3);
-        cx(k0, s0, c0, k2, s2, c2);
-        cx(k1, s1, c1, k3, s3, c3);
-        cx(k1, s1, c1, k2, s2, c2);
-        if (cnt >= 4 && sp < Stack::CAP) st.put(sp++, c3, k3);
-        if (cnt >= 3 && sp < Stack::CAP) st.put(sp++, c2, k2);
-        if (cnt >= 2 && sp < Stack::CAP) st.put(sp++, c1, k1);
-        next = c0;
+        next = node4_order_push(st, sp, node4_test((const char*)nodes, next << 7, sl, tmin, r.t));
     }
     // any hit: leave with the hit (the leaf is made global below)
     if (r.leaf != NO_HIT) r.leaf += __float_as_uint(inst[INST_F4 * (size_t)hit_inst + 6].z);

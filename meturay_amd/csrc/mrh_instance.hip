@@ -193,9 +193,15 @@ int32_t two_level_build(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene* s)
         return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: instancing needs meshes and instances");
     if (I->mesh_first[0] != 0 || I->mesh_first[I->n_meshes] != d->n)
         return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: mesh ranges must cover the %u triangles", d->n);
-    for (uint32_t m = 0; m < I->n_meshes; m++)
+    for (uint32_t m = 0; m < I->n_meshes; m++) {
         if (I->mesh_first[m + 1] <= I->mesh_first[m])
             return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: mesh %u is empty or out of order", m);
+        // 4-wide node records are addressed by 32-bit byte offsets per tree
+        if (I->mesh_first[m + 1] - I->mesh_first[m] > (1u << 25))
+            return fail(ctx, MRH_ERR_UNSUPPORTED, "mrh_scene_create: mesh %u has more than 2^25 triangles", m);
+    }
+    if (I->n_instances > (1u << 25))
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "mrh_scene_create: more than 2^25 instances");
     const uint32_t ni = I->n_instances;
     std::vector<float> w2l((size_t)ni * 12);
     for (uint32_t i = 0; i < ni; i++) {

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h,
 constexpr uint32_t COUNTS_WORDS = 128;
 
 template <int W, int CAP, bool STATS, int L>
-__global__ __launch_bounds__(256) void k_trace_pt(const float4* __restrict__ tnodes,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_trace_pt(const float4* __restrict__ tnodes,
                                                   const float4* __restrict__ leaves, uint32_t root, uint32_t n_leaves,
                                                   PathBuf in, uint32_t* __restrict__ counts,
                                                   uint32_t bounce, float tmin, float tmax, HitBuf hit,

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_serve_raygen(const StepHdr* __restrict_
 // Closest hit of every queued ray in its own request's scene; camera rays
 // (depth 1) are clipped to the request's near/far planes.
 template <int W, int CAP, int L>
-__global__ __launch_bounds__(256) void k_serve_trace(const StepHdr* __restrict__ hdr,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_serve_trace(const StepHdr* __restrict__ hdr,
                                                      const SlotDev* __restrict__ slots, PathBuf in,
                                                      const uint32_t* __restrict__ in_slot, HitBuf hit,
                                                      uint2* __restrict__ overflow)
@@ -365,6 +365,7 @@ struct mrh_server {
     void* graph_scratch[8][2][SLOT_COUNT] = {};
     uint32_t graph_cap[8][2] = {}, graph_w[8][2] = {}, graph_lds[8][2] = {}, graph_bpc[8][2] = {};
     uint32_t max_tree_depth = 0;    // deepest LBVH among submitted requests
+    uint32_t max_tree_leaves = 0;   // largest LBVH among submitted requests
     // learner: guide learn passes run on their own context and host thread,
     // overlapping the wavefront steps of other requests
     mrh_context* lctx = nullptr;
@@ -680,7 +681,7 @@ int32_t finish_request(mrh_server* s, ServeReq* q)
 void tree_shape(const mrh_server* s, uint32_t& w, uint32_t& cap)
 {
     const uint32_t d = s->max_tree_depth;
-    w = (s->ctx->trace_width == 4 && d <= 84) ? 4u : 2u;
+    w = (s->ctx->trace_width == 4 && d <= 84 && s->max_tree_leaves <= (1u << 25)) ? 4u : 2u;
     cap = (w == 4) ? (d <= 42 ? 64u : 128u) : (d <= 64 ? 64u : 128u);
 }
 
@@ -1135,7 +1136,10 @@ int32_t server_submit(mrh_server* s, const mrh_request* r, uint64_t* ticket)
                     "submit: request needs %llu bytes with its scene and the queues, budget is %llu",
                     (unsigned long long)need, (unsigned long long)s->p.hbm_budget);
     }
-    if (!r->scene->two_level) s->max_tree_depth = std::max(s->max_tree_depth, r->scene->bvh->max_depth);
+    if (!r->scene->two_level) {
+        s->max_tree_depth = std::max(s->max_tree_depth, r->scene->bvh->max_depth);
+        s->max_tree_leaves = std::max(s->max_tree_leaves, r->scene->bvh->n);
+    }
     q->ticket = s->next_ticket++;
     q->t_submit = now_s();
     s->queue.push_back(q);

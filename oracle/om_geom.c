@@ -349,6 +349,60 @@ typedef struct {
     float    t;
 } om_stack_entry;
 
+/* The box test of the 4-wide walks (the traversal kernels' node4_test): the
+ * slab test with each axis's planes ordered by the direction's sign.  The
+ * near plane (the min plane where the reciprocal direction is non-negative,
+ * else the max plane) gives the axis's entry distance and the far plane its
+ * exit, each as fma(plane, inv, -o * inv); the box is entered when
+ * max(tmin, entries) <= min(tmax, exits).  Direction components below
+ * 2^-64 in magnitude become +-2^-64 (sign kept), so every distance is
+ * finite.  Entry and exit are taken max / min left to right: x, y, z, then
+ * tmin / tmax (exact, in any order, for non-NaN values). */
+typedef struct {
+    float inv[3], noi[3];
+    int   near_max[3];   /* 1: the near plane of the axis is the max plane */
+} om_slab;
+
+static float slab_rcp(float d)
+{
+    const float tiny = 0x1p-64f;
+    return 1.0f / (fabsf(d) < tiny ? copysignf(tiny, d) : d);
+}
+
+static void om_slab_setup(const float o[3], const float d[3], om_slab* s)
+{
+    for (int a = 0; a < 3; a++) {
+        s->inv[a] = slab_rcp(d[a]);
+        s->noi[a] = -(o[a] * s->inv[a]);
+        s->near_max[a] = signbit(s->inv[a]) ? 1 : 0;
+    }
+}
+
+static int om_slab_box(const om_slab* s, const float box[6], float tmin, float tmax, float* t_enter)
+{
+    float te = 0.0f, tx = 0.0f;
+    for (int a = 0; a < 3; a++) {
+        const float np = s->near_max[a] ? box[3 + a] : box[a];
+        const float fp = s->near_max[a] ? box[a] : box[3 + a];
+        const float tn = fmaf(np, s->inv[a], s->noi[a]);
+        const float tf = fmaf(fp, s->inv[a], s->noi[a]);
+        te = (a == 0) ? tn : fmaxf(te, tn);
+        tx = (a == 0) ? tf : fminf(tx, tf);
+    }
+    te = fmaxf(te, tmin);
+    tx = fminf(tx, tmax);
+    *t_enter = te;
+    return tx >= te;
+}
+
+int om_slab_box_test(const float o[3], const float d[3], const float box[6], float tmin, float tmax,
+                     float* t_enter)
+{
+    om_slab s;
+    om_slab_setup(o, d, &s);
+    return om_slab_box(&s, box, tmin, tmax, t_enter);
+}
+
 /* Closest hit over the LBVH.  Semantics follow KCIntersectBVH
  * (GPUAcceleratorBVHKC.cuh:229-346) + AcceptHit
  * (GPUAcceleratorCommonKC.cuh:33-78): accept t in [tmin, tmax), tmax shrinks
@@ -521,7 +575,7 @@ uint32_t om_bvh4_build(uint32_t n, const uint32_t* left, const uint32_t* right, 
 }
 
 /* Closest / any hit over the 4-wide tree.  At a node the occupied slots
- * whose boxes the ray enters (within [tmin, current tmax]) are ordered by
+ * whose boxes the ray enters (om_slab_box, within [tmin, current tmax]) are ordered by
  * (entry distance, slot); the nearest is taken, the others pushed farthest
  * first with their entry distance and culled on pop when it exceeds tmax.
  * Acceptance as om_trace_one. */
@@ -531,7 +585,8 @@ uint32_t om_trace4_one(const float* tris, uint32_t n, const uint32_t* sorted_idx
 {
     uint32_t hit = OM_INVALID;
     float tmax = *tmax_io;
-    float inv_d[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    om_slab sl;
+    om_slab_setup(o, d, &sl);
     uint32_t nv = 0, nt = 0;
     om_stack_entry stack[256];
     int sp = 0;
@@ -571,7 +626,7 @@ uint32_t om_trace4_one(const float* tris, uint32_t n, const uint32_t* sorted_idx
             uint32_t c = om_f2u(q[24 + k]);
             float box[6] = {q[k], q[4 + k], q[8 + k], q[12 + k], q[16 + k], q[20 + k]};
             float e;
-            int h = (c != OM_INVALID) && om_intersect_aabb(o, inv_d, box, tmin, tmax, &e);
+            int h = (c != OM_INVALID) && om_slab_box(&sl, box, tmin, tmax, &e);
             key[k] = h ? e : INFINITY;
             slot[k] = (uint32_t)k;
             code[k] = c;
@@ -756,7 +811,8 @@ uint32_t om_trace2l_one(const om_two_level* tl, const float* tris, uint32_t widt
 {
     uint32_t hit = OM_INVALID, hinst = OM_INVALID;
     float tmax = *tmax_io;
-    float inv_d[3] = {1.0f / d[0], 1.0f / d[1], 1.0f / d[2]};
+    om_slab sl;
+    om_slab_setup(o, d, &sl);
     uint32_t nv = 0, nt = 0;
     om_stack_entry stack[128];
     int sp = 0;
@@ -811,7 +867,7 @@ uint32_t om_trace2l_one(const om_two_level* tl, const float* tris, uint32_t widt
             uint32_t c = om_f2u(q[24 + k]);
             float box[6] = {q[k], q[4 + k], q[8 + k], q[12 + k], q[16 + k], q[20 + k]};
             float e;
-            int h = (c != OM_INVALID) && om_intersect_aabb(o, inv_d, box, tmin, tmax, &e);
+            int h = (c != OM_INVALID) && om_slab_box(&sl, box, tmin, tmax, &e);
             key[k] = h ? e : INFINITY;
             slot[k] = (uint32_t)k;
             code[k] = c;

@@ -64,6 +64,10 @@ int  om_intersect_triangle(const float o[3], const float d[3], const float tri[9
                            float* t, float bary[3]);
 int  om_intersect_aabb(const float o[3], const float inv_d[3], const float box[6],
                        float tmin, float tmax, float* t_enter);
+/* box test of the 4-wide walks: planes ordered by the direction's sign,
+ * distances fma(plane, inv, -o * inv) */
+int  om_slab_box_test(const float o[3], const float d[3], const float box[6], float tmin, float tmax,
+                      float* t_enter);
 /* stats (optional, may be NULL): 2 u32 per ray = internal-node visits, triangle tests */
 void om_trace_closest(const float* tris, uint32_t n, const uint32_t* sorted_idx,
                       const uint32_t* left, const uint32_t* right, const float* node_aabb,

@@ -121,6 +121,8 @@ def lib():
         L.om_intersect_triangle.argtypes = [_P, _P, _P, _P, _P]
         L.om_intersect_aabb.restype = C.c_int
         L.om_intersect_aabb.argtypes = [_P, _P, _P, _F, _F, _P]
+        L.om_slab_box_test.restype = C.c_int
+        L.om_slab_box_test.argtypes = [_P, _P, _P, _F, _F, _P]
         L.om_trace_closest.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F,
                                        _P, _P, _P, _P]
         L.om_trace_any.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F, _P]

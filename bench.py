@@ -209,40 +209,16 @@ def instanced_leg(N, ctx, img, torch):
             "segments": int(st.segments), "build_ms": round(build * 1e3, 2)}
 
 
-SERVE_RES = [(256, 256), (512, 512), (1024, 768), (1920, 1080)]
-SERVE_SPP = [1, 4, 16, 64]
-
-
-def serving_workload(n_req: int, rate: float, seed: int = 3):
-    """The request stream: Poisson arrivals, 8 sphere-grid scenes of
-    log-uniform size in [10K, 4M] triangles, resolution, spp and guiding
-    drawn per request."""
-    rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(np.log(1e4), np.log(4e6), 8))
-    grids = [int(max(5, round(np.sqrt(x / 512.0)))) for x in sizes]
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
-    reqs = []
-    for i in range(n_req):
-        w, h = SERVE_RES[rng.integers(0, 4)]
-        spp = SERVE_SPP[rng.integers(0, 4)]
-        guided = bool(rng.random() < 0.5)
-        it = min(4, spp) if guided else 1
-        S = int(round(np.sqrt(spp // it)))
-        reqs.append({"t": float(arrivals[i]), "scene": int(rng.integers(0, 8)), "w": w, "h": h, "S": S,
-                     "it": it, "spp": spp, "seed": i})
-    return grids, reqs
-
-
 def serving_leg(N, ctx, torch, n_req: int, rate: float):
     """C5: serial one-request-at-a-time launches vs the serving engine, the
     latter both saturated (all requests queued at once) and in real time
     with the Poisson arrivals.  Latency = completion - arrival."""
     import time as _t
+    from meturay_amd.workloads import serving_scene, serving_workload
     grids, reqs = serving_workload(n_req, rate)
     scenes = []
     for k, g in enumerate(grids):
-        sc = N.gen_sphere_grid(g, 16, 16, 100 + k)
-        scenes.append(N.Scene(ctx, sc))
+        scenes.append(N.Scene(ctx, serving_scene(N, g, k)))
     tris = [g * g * 512 for g in grids]
     log(f"serving: {n_req} requests, scenes {tris} triangles")
     depth, rr = DEPTH, RR

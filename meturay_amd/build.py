@@ -60,14 +60,6 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
-    # the C++ host that drives the library through TracerSystemI (tests)
-    host_src = os.path.join(ROOT, "tests", "cpp", "tracer_system_host.cpp")
-    if os.path.exists(host_src):
-        exe = os.path.join(BUILD, "tracer_system_host")
-        cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-I", INCLUDE, host_src, "-o", exe, "-ldl"]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            raise RuntimeError(f"host build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
     return LIB

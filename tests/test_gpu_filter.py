@@ -9,7 +9,6 @@ either order).
 """
 import ctypes as C
 import os
-import subprocess
 
 import numpy as np
 import pytest
@@ -21,7 +20,6 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-HOST = os.path.join(os.path.dirname(N.LIB_PATH), "_build", "tracer_system_host")
 
 
 def descs(kind, r):
@@ -167,20 +165,4 @@ def test_tracer_filter_accumulates_frames(ctx, oracle):
     # detaching the filter goes back to the per-pixel mean
     ctx.check(N.lib().mrh_tracer_set_filter(t, None))
     ctx.check(N.lib().mrh_tracer_destroy(t))
-    scene.close()
-
-
-def test_cpp_host_filtered_frame(ctx, tmp_path):
-    assert os.path.exists(HOST), "tracer_system_host not built (python -m meturay_amd.build)"
-    out, fout = tmp_path / "img.bin", tmp_path / "filtered.bin"
-    r = subprocess.run([HOST, N.LIB_PATH, str(out), "64", "48", "1", str(fout)], capture_output=True, text=True,
-                       timeout=120)
-    print(r.stdout)
-    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
-    got = np.fromfile(fout, np.float32).reshape(48, 64, 3)
-    scene = N.Scene(ctx, N.gen_cornell(4))
-    img = torch.zeros(48 * 64 * 3, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
-    scene.render_filtered(img.data_ptr(), 64, 48, 1, 8, 3, 0, filt=N.filter_desc("Tent", 1.0))
-    assert got.tobytes() == img.cpu().numpy().tobytes()
     scene.close()

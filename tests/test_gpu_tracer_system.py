@@ -4,7 +4,6 @@ drives GenerateWork -> while (Render()) -> Finalize; its image must equal the
 one-call C-ABI render of the same frame bit for bit.  Also: the incremental
 tracer's progressive accumulation, and the full C3 frame against the oracle."""
 import os
-import subprocess
 
 import numpy as np
 import pytest
@@ -14,7 +13,6 @@ from meturay_amd import native as N
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-HOST = os.path.join(os.path.dirname(N.LIB_PATH), "_build", "tracer_system_host")
 
 
 def _render(ctx, sc, w, h, S, depth, rr, seed=0):
@@ -25,18 +23,6 @@ def _render(ctx, sc, w, h, S, depth, rr, seed=0):
     out = img.cpu().numpy().reshape(h, w, 3)
     scene.close()
     return out, st
-
-
-def test_cpp_host_through_tracer_system(ctx, tmp_path):
-    assert os.path.exists(HOST), "tracer_system_host not built (python -m meturay_amd.build)"
-    out = tmp_path / "img.bin"
-    r = subprocess.run([HOST, N.LIB_PATH, str(out), "64", "48", "1"], capture_output=True, text=True,
-                       timeout=120)
-    print(r.stdout)
-    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
-    got = np.fromfile(out, np.float32).reshape(48, 64, 3)
-    ref, _ = _render(ctx, N.gen_cornell(4), 64, 48, 1, 8, 3, 0)
-    assert got.tobytes() == ref.tobytes()
 
 
 def test_incremental_tracer_matches_one_call_render(ctx):

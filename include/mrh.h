@@ -231,6 +231,14 @@ typedef struct {
      * two-level scene: tris / prim_key are the meshes' triangles in their
      * local spaces, and the instances place them in the world. */
     const struct mrh_instancing_desc* instancing;
+    /* NULL: flat shading (every corner carries the face frame).  Otherwise
+     * host float[n][9]: a shading normal per triangle corner (any nonzero
+     * length; normalised).  Each corner gets the world-to-tangent rotation
+     * of its normal with the tangent OrthogonalVector(n), Gram-Schmidt and
+     * bitangent n x t (GPUPrimitiveTriangle.cpp:286-316, TransformGen::Space);
+     * shading blends the three with Quat::BarySLerp
+     * (GPUPrimitiveTriangle.h:626-670).  Single-level scenes only. */
+    const float* normals;
 } mrh_scene_desc;
 
 /* Two-level scenes.  Replaces the base accelerator over per-surface
@@ -368,7 +376,11 @@ int32_t mrh_filter_resolve(mrh_context* ctx, const float* d_weighted, const floa
  * (CameraSurfaces order).  Supported: Null / Primitive lights, Vacuum /
  * Homogeneous media, Identity / Single transforms, LambertC or Lambert with
  * a constant albedo, Reflect, Refract, Metal, inline and OBJ triangles.
- * Anything else fails with MRH_ERR_UNSUPPORTED (malformed files with
+ * Vertex normals (OBJ vn, the "normal" lists of nodeTriangle and
+ * nodeTriangleIndexed) become mrh_scene_desc.normals in world space (inverse
+ * transpose, renormalised); triangles without normals in a scene with some
+ * get their face normal, and the instanced mode below shades flat, noting
+ * it in the warnings.  Anything else fails with MRH_ERR_UNSUPPORTED (malformed files with
  * MRH_ERR_INVALID), the reason in msg; with MRH_SCENE_FILE_SUBSTITUTE it is
  * replaced (a Lambert for other materials, a tessellated sphere for analytic
  * ones) or skipped instead and listed in mrh_scene_file_warnings. */

@@ -43,25 +43,6 @@ MRH_HD V3 octa_to_dir(float u, float v)
     return v3(copysignf(k * c, a), copysignf(k * s, b), z);
 }
 
-// atan(t) for t in [0, 1]: two half-angle reductions,
-// atan(t) = 2 atan(t / (1 + sqrt(1 + t^2))), bring t below tan(pi/16),
-// where the Taylor series to t^13 is exact to about 2e-12.  Written out so
-// that host and device round identically.
-MRH_HD float atan_unit(float t)
-{
-    t = t / (1.0f + sqrtf(1.0f + t * t));
-    t = t / (1.0f + sqrtf(1.0f + t * t));
-    const float t2 = t * t;
-    float p = 0.07692307692307693f;                  //  1/13
-    p = p * t2 - 0.09090909090909091f;               // -1/11
-    p = p * t2 + 0.1111111111111111f;                //  1/9
-    p = p * t2 - 0.14285714285714285f;               // -1/7
-    p = p * t2 + 0.2f;                               //  1/5
-    p = p * t2 - 0.3333333333333333f;                // -1/3
-    p = p * t2 + 1.0f;
-    return 4.0f * (t * p);
-}
-
 // atan2(y, x) / (pi / 2) for y, x >= 0: the angle as a fraction of a quadrant
 MRH_HD float quadrant_angle(float y, float x)
 {

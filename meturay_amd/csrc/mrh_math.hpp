@@ -138,6 +138,135 @@ MRH_HD void sincos_2pi(float u, float& s, float& c)
     else { s = -cp; c = sp; }
 }
 
+// atan(t) for t in [0, 1]: two half-angle reductions,
+// atan(t) = 2 atan(t / (1 + sqrt(1 + t^2))), bring t below tan(pi/16),
+// where the Taylor series to t^13 is exact to about 2e-12.  Written out so
+// that host and device round identically.
+MRH_HD float atan_unit(float t)
+{
+    t = t / (1.0f + sqrtf(1.0f + t * t));
+    t = t / (1.0f + sqrtf(1.0f + t * t));
+    const float t2 = t * t;
+    float p = 0.07692307692307693f;                  //  1/13
+    p = p * t2 - 0.09090909090909091f;               // -1/11
+    p = p * t2 + 0.1111111111111111f;                //  1/9
+    p = p * t2 - 0.14285714285714285f;               // -1/7
+    p = p * t2 + 0.2f;                               //  1/5
+    p = p * t2 - 0.3333333333333333f;                // -1/3
+    p = p * t2 + 1.0f;
+    return 4.0f * (t * p);
+}
+
+// ------------------------------------------------------------ shading frames
+// Per-vertex tangent frames of smooth-shaded triangles and their barycentric
+// interpolation (GPUPrimitiveTriangle.cpp:286-316 at load,
+// GPUPrimitiveTriangle.h:626-670 at shading).  acos and sin are evaluated
+// with atan_unit and sincos_2pi, so host and device round identically.
+
+// RayLib/Vector.hpp:877-886 (pbrt's CoordinateSystem)
+MRH_HD V3 orthogonal_vector(V3 v)
+{
+    if (fabsf(v.x) > fabsf(v.y)) return divs(v3(-v.z, 0.0f, v.x), sqrtf(v.x * v.x + v.z * v.z));
+    return divs(v3(0.0f, v.z, -v.y), sqrtf(v.y * v.y + v.z * v.z));
+}
+
+// Vector<4>::Dot order, component-wise scale and sum, Vector::Normalize
+MRH_HD float qdot(Q4 a, Q4 b)
+{
+    float d = 0.0f;
+    d += a.w * b.w;
+    d += a.x * b.x;
+    d += a.y * b.y;
+    d += a.z * b.z;
+    return d;
+}
+MRH_HD Q4 qscale(Q4 a, float s) { return qmake(a.w * s, a.x * s, a.y * s, a.z * s); }
+MRH_HD Q4 qadd(Q4 a, Q4 b) { return qmake(a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z); }
+MRH_HD Q4 qnormalize(Q4 q) { return qscale(q, 1.0f / sqrtf(qdot(q, q))); }
+
+// TransformGen::Space (RayLib/Quaternion.hpp:370-436): the rotation taking
+// the orthonormal basis (x, y, z) to the standard axes (Mike Day's
+// matrix-to-quaternion conversion, normalised, conjugated)
+MRH_HD Q4 space_quat(V3 x, V3 y, V3 z)
+{
+    const V3 c = cross(x, y) - z;
+    if (fabsf(c.x) > 0.1f && fabsf(c.y) > 0.1f && fabsf(c.z) > 0.1f) x = -x;
+    float t;
+    Q4 q;
+    if (z.z < 0.0f) {
+        if (x.x > y.y) {
+            t = 1.0f + x.x - y.y - z.z;
+            q = qmake(y.z - z.y, t, x.y + y.x, z.x + x.z);
+        } else {
+            t = 1.0f - x.x + y.y - z.z;
+            q = qmake(z.x - x.z, x.y + y.x, t, y.z + z.y);
+        }
+    } else {
+        if (x.x < -y.y) {
+            t = 1.0f - x.x - y.y + z.z;
+            q = qmake(x.y - y.x, z.x + x.z, y.z + z.y, t);
+        } else {
+            t = 1.0f + x.x + y.y + z.z;
+            q = qmake(t, y.z - z.y, z.x - x.z, x.y - y.x);
+        }
+    }
+    q = qscale(q, 0.5f / sqrtf(t));
+    return conj(qnormalize(q));
+}
+
+// World-to-tangent rotation of a vertex with unit shading normal n: the
+// tangent is OrthogonalVector(n) (the reference's tangent for a vertex
+// without texture coordinates), made orthonormal to n (Gram-Schmidt), the
+// bitangent n x t (GPUPrimitiveTriangle.cpp:286-316)
+MRH_HD Q4 vertex_frame(V3 n)
+{
+    V3 t = orthogonal_vector(n);
+    if (dot(t, t) < 1.0e-5f) t = orthogonal_vector(n);
+    t = normalize(t);
+    const V3 tn = normalize(t - scale(n, dot(n, t)));
+    const V3 b = cross(n, tn);
+    return space_quat(tn, b, n);
+}
+
+// acos(x) for x in [0, 1] as 2 atan(sqrt((1 - x) / (1 + x))); sin(x) for
+// x in [0, pi/2] through sincos_2pi
+MRH_HD float acos_unit(float x) { return 2.0f * atan_unit(sqrtf((1.0f - x) / (1.0f + x))); }
+MRH_HD float sin_q(float x)
+{
+    float s, c;
+    sincos_2pi(x * 0.15915494309189534f, s, c);
+    return s;
+}
+
+// Quat::SLerp (RayLib/Quaternion.hpp:256-281): the shorter arc, a plain
+// lerp when the quaternions are within 1e-5 of parallel; not normalised
+MRH_HD Q4 qslerp(Q4 a, Q4 b, float t)
+{
+    const float cos_t = qdot(a, b);
+    const float cos_f = (cos_t >= 0.0f) ? cos_t : -cos_t;
+    float s0, s1;
+    if (cos_f < 1.0f - 1.0e-5f) {
+        const float angle = acos_unit(cos_f);
+        const float sa = sin_q(angle);
+        s0 = sin_q(angle * (1.0f - t)) / sa;
+        s1 = sin_q(angle * t) / sa;
+    } else {
+        s0 = 1.0f - t;
+        s1 = t;
+    }
+    s1 = (cos_t >= 0.0f) ? s1 : -s1;
+    return qadd(qscale(a, s0), qscale(b, s1));
+}
+
+// Quat::BarySLerp (RayLib/Quaternion.hpp:283-318): a, b = weights of q0, q1
+MRH_HD Q4 bary_slerp(Q4 q0, Q4 q1, Q4 q2, float a, float b)
+{
+    const float c = 1.0f - a - b;
+    if (fabsf(a + b) < 1.0e-5f) return q2;
+    const float ab = a / (a + b);
+    return qslerp(qslerp(q1, q0, ab), q2, c);
+}
+
 // ------------------------------------------------------------ Morton codes
 // RayLib/MortonCode.h:45-70 (Expand3D, 32- and 64-bit)
 MRH_HD uint32_t expand10(uint32_t x)

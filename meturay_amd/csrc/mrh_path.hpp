@@ -35,6 +35,9 @@ struct mrh_scene {
     uint32_t n_tris = 0;
     uint32_t mesh_depth = 0;        // deepest mesh tree
     uint32_t* light_inst = nullptr; // instance of every light (lights hold leaf indices)
+    // smooth shading (single-level scenes with per-corner normals): the
+    // corners' world-to-tangent rotations, [n][3] float4 (w, x, y, z), primitive order
+    float4* tbn = nullptr;
 };
 
 namespace mrh {
@@ -241,6 +244,22 @@ __device__ __forceinline__ Surface make_surface(V3 p0, V3 p1, V3 p2, float ba, f
         s.w2t = qmul(qmake(0.0f, 1.0f, 0.0f, 0.0f), s.w2t);
     }
     s.geo_n = n;
+    return s;
+}
+
+// GPUPrimitiveTriangle.h:626-670 for smooth-shaded triangles: the
+// corners' world-to-tangent rotations q0..q2 (vertex_frame) blended by
+// BarySLerp and normalised, flipped by the tangent-axis rotation on the back
+// side; position, geometric normal and side as make_surface.
+__device__ __forceinline__ Surface make_surface_tbn(V3 p0, V3 p1, V3 p2, float ba, float bb, V3 dir,
+                                                    const float4* __restrict__ tbn)
+{
+    Surface s = make_surface(p0, p1, p2, ba, bb, dir);
+    const float4 a = tbn[0], b = tbn[1], c = tbn[2];
+    Q4 w = qnormalize(bary_slerp(qmake(a.x, a.y, a.z, a.w), qmake(b.x, b.y, b.z, b.w), qmake(c.x, c.y, c.z, c.w),
+                                 ba, bb));
+    if (s.back) w = qmul(qmake(0.0f, 1.0f, 0.0f, 0.0f), w);
+    s.w2t = w;
     return s;
 }
 
@@ -491,6 +510,7 @@ struct ShadeReq {
     uint32_t mis;            // power-heuristic MIS of NEE and BxDF rays ("DirectLightMIS"), with nee only
     const float4* inst = nullptr;          // two-level scenes: instance records
     const uint32_t* light_inst = nullptr;  // two-level scenes: instance of every light
+    const float4* tbn = nullptr;           // smooth shading: [primitive][3] corner rotations (w, x, y, z)
 };
 
 // World-space corners of leaf li (of instance ii in a two-level scene: the
@@ -674,7 +694,10 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
     V3 c0, c1, c2;
     leaf_corners(q, hit.leaf[rid], hit.inst ? hit.inst[rid] : 0u, c0, c1, c2);
     const V3 d = v3(in.dx[rid], in.dy[rid], in.dz[rid]);
-    const Surface s = make_surface(c0, c1, c2, hit.ba[rid], hit.bb[rid], d);
+    const uint32_t li = hit.leaf[rid];
+    const Surface s = q.tbn ? make_surface_tbn(c0, c1, c2, hit.ba[rid], hit.bb[rid], d,
+                                               q.tbn + 3 * (size_t)__float_as_uint(q.leaves[3 * (size_t)li].w))
+                            : make_surface(c0, c1, c2, hit.ba[rid], hit.bb[rid], d);
     const V3 wi = -normalize(d);
     Rng rng{in.rng[rid], pcg32_inc(q.sample_base + sample)};
     if (NEE && q.nee && q.n_lights &&

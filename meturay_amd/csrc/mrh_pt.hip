@@ -19,6 +19,7 @@
 #include "mrh_path.hpp"
 
 #include <cfloat>
+#include <cmath>
 #include <cstring>
 
 namespace mrh {
@@ -451,6 +452,25 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
     if (d->n_dielectric > MAX_DIELECTRICS)
         return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: at most %u dielectric materials", MAX_DIELECTRICS);
     const bool two = d->instancing != nullptr;
+    if (d->normals && two)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "mrh_scene_create: per-corner normals in a two-level scene");
+    // smooth shading: every corner's world-to-tangent rotation
+    std::vector<float> tbn;
+    if (d->normals) {
+        tbn.resize((size_t)d->n * 12);
+        for (size_t k = 0; k < (size_t)d->n * 3; k++) {
+            const float* nv = d->normals + 3 * k;
+            const float l2 = nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2];
+            if (!(l2 > 0.0f) || !std::isfinite(l2))
+                return fail(ctx, MRH_ERR_INVALID, "mrh_scene_create: corner normal %zu of triangle %zu is not a direction",
+                            k % 3, k / 3);
+            const Q4 q = vertex_frame(normalize(v3(nv[0], nv[1], nv[2])));
+            tbn[4 * k + 0] = q.w;
+            tbn[4 * k + 1] = q.x;
+            tbn[4 * k + 2] = q.y;
+            tbn[4 * k + 3] = q.z;
+        }
+    }
     std::vector<uint32_t> rank(two ? 0 : d->n);
     uint32_t n_lights = 0;
     for (uint32_t i = 0; i < d->n && !two; i++)
@@ -475,6 +495,11 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
         if (two) {
             rc = two_level_build(ctx, d, s);
             break;
+        }
+        if (!tbn.empty()) {
+            float* dtbn = nullptr;
+            if ((rc = scene_upload(s, &dtbn, tbn.data(), tbn.size())) != MRH_OK) break;
+            s->tbn = (float4*)dtbn;
         }
         if ((rc = scene_upload(s, &dtris, d->tris, (size_t)d->n * 9)) != MRH_OK) break;
         if ((rc = scene_upload(s, &dkeys, d->prim_key, (size_t)d->n)) != MRH_OK) break;
@@ -614,6 +639,7 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     q.mis = f.mis;
     q.inst = s->two_level ? (const float4*)s->inst : nullptr;
     q.light_inst = s->two_level ? s->light_inst : nullptr;
+    q.tbn = s->tbn;
     auto shade = [&](auto kernel) {
         return launch(ctx, "shade", dim3(grid), dim3(256), 0, kernel, in, outb, hb, (const uint32_t*)f.order,
                       f.counts, bounce, q, f.ga);

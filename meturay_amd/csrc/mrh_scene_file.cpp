@@ -14,7 +14,12 @@
 // albedo, Reflect, Refract, Metal; Triangle primitives inline
 // (nodeTriangle, nodeTriangleIndexed) or from OBJ files (innerIndex = mesh
 // in the file; meshes split at o / g / usemtl, polygons fan-triangulated,
-// degenerate triangles dropped).  Everything else (textures, Unreal, analytic
+// degenerate triangles dropped).  Vertex normals (OBJ vn, the JSON "normal"
+// lists) become per-corner shading normals, transformed by the inverse
+// transpose and renormalised; triangles of a scene that has normals
+// elsewhere but none of their own get their face normal (aiProcess_GenNormals
+// of MetaSurfaceLoader.cpp:27), substituted spheres their radial normals.
+// Everything else (textures, Unreal, analytic
 // spheres, other lights and cameras, boundary lights) is an error, or, with
 // MRH_SCENE_FILE_SUBSTITUTE, replaced (grey or constant-albedo Lambert for
 // materials, a 64 x 32 UV tessellation for spheres) or skipped, each noted
@@ -34,6 +39,7 @@
 
 struct mrh_scene_file {
     std::vector<float> tris;
+    std::vector<float> normals;   // [n][9] per-corner shading normals, or empty
     std::vector<uint32_t> keys;
     std::vector<float> lambert, metal, dielectric, emissive, reflect;
     std::vector<mrh_camera_desc> cameras;
@@ -89,20 +95,41 @@ Mat4 rotate(double deg, double x, double y, double z)
     return r;
 }
 
-// Wavefront OBJ: v lines and f lines (v, v/t, v//n, v/t/n; negative indices
-// count back from the vertices read so far, positive ones may refer to
-// vertices further down the file); a new mesh starts at o, g and usemtl once
-// the current one has faces; polygons are fans around their first vertex.
-// Meshes without faces are dropped.
-int32_t obj_read(const char* file, std::vector<std::vector<float>>* meshes)
+// unit face normal of a triangle [9] (zero for a degenerate one)
+void face_normal(const float* t, float n[3])
+{
+    const double ax = t[3] - t[0], ay = t[4] - t[1], az = t[5] - t[2];
+    const double bx = t[6] - t[0], by = t[7] - t[1], bz = t[8] - t[2];
+    const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+    const double l = std::sqrt(cx * cx + cy * cy + cz * cz);
+    n[0] = l > 0 ? (float)(cx / l) : 0.0f;
+    n[1] = l > 0 ? (float)(cy / l) : 0.0f;
+    n[2] = l > 0 ? (float)(cz / l) : 0.0f;
+}
+
+// Wavefront OBJ: v, vn and f lines (v, v/t, v//n, v/t/n; negative indices
+// count back from the vertices / normals read so far, positive ones may
+// refer to entries further down the file); a new mesh starts at o, g and
+// usemtl once the current one has faces; polygons are fans around their
+// first vertex.  Meshes without faces are dropped.  `normals`, when given,
+// receives each mesh's per-corner normals [..][9]: empty for a mesh none of
+// whose corners names one, else the named normal or, for corners naming
+// none, the face normal.
+int32_t obj_read(const char* file, std::vector<std::vector<float>>* meshes,
+                 std::vector<std::vector<float>>* normals = nullptr)
 {
     std::ifstream f(file);
     if (!f) return MRH_ERR_INVALID;
-    std::vector<float> pos;
-    std::vector<std::vector<long>> faces(1);   // vertex indices of each mesh's triangles
+    std::vector<float> pos, nrm;
+    std::vector<std::vector<long>> faces(1), fnrm(1);   // vertex / normal indices of each mesh's corners
+    std::vector<bool> has_nrm(1, false);
     std::string line;
     auto next_mesh = [&]() {
-        if (!faces.back().empty()) faces.emplace_back();
+        if (!faces.back().empty()) {
+            faces.emplace_back();
+            fnrm.emplace_back();
+            has_nrm.push_back(false);
+        }
     };
     while (std::getline(f, line)) {
         if (!line.empty() && line.back() == '\r') line.pop_back();
@@ -117,28 +144,65 @@ int32_t obj_read(const char* file, std::vector<std::vector<float>>* meshes)
             pos.push_back(x);
             pos.push_back(y);
             pos.push_back(z);
+        } else if (tag == "vn") {
+            float x = 0, y = 0, z = 0;
+            ls >> x >> y >> z;
+            nrm.push_back(x);
+            nrm.push_back(y);
+            nrm.push_back(z);
         } else if (tag == "o" || tag == "g" || tag == "usemtl") {
             next_mesh();
         } else if (tag == "f") {
-            std::vector<long> idx;
+            std::vector<long> idx, nidx;
             std::string tok;
             while (ls >> tok) {
-                const long v = std::stol(tok.substr(0, tok.find('/')));
+                const size_t s1 = tok.find('/');
+                const long v = std::stol(tok.substr(0, s1));
                 if (v == 0) return MRH_ERR_INVALID;
                 idx.push_back(v > 0 ? v - 1 : (long)(pos.size() / 3) + v);
+                long n = -1;
+                const size_t s2 = (s1 == std::string::npos) ? s1 : tok.find('/', s1 + 1);
+                if (s2 != std::string::npos && s2 + 1 < tok.size()) {
+                    const long vn = std::stol(tok.substr(s2 + 1));
+                    if (vn == 0) return MRH_ERR_INVALID;
+                    n = vn > 0 ? vn - 1 : (long)(nrm.size() / 3) + vn;
+                    if (n < 0) return MRH_ERR_INVALID;
+                    has_nrm.back() = true;
+                }
+                nidx.push_back(n);
             }
             for (size_t t = 1; t + 1 < idx.size(); t++)
-                for (long k : {idx[0], idx[t], idx[t + 1]}) faces.back().push_back(k);
+                for (size_t k : {(size_t)0, t, t + 1}) {
+                    faces.back().push_back(idx[k]);
+                    fnrm.back().push_back(nidx[k]);
+                }
         }
     }
-    const long nv = (long)(pos.size() / 3);
-    for (const auto& mesh : faces) {
+    const long nv = (long)(pos.size() / 3), nn = (long)(nrm.size() / 3);
+    for (size_t m = 0; m < faces.size(); m++) {
+        const auto& mesh = faces[m];
         if (mesh.empty()) continue;
         std::vector<float> tris;
         tris.reserve(mesh.size() * 3);
         for (long k : mesh) {
             if (k < 0 || k >= nv) return MRH_ERR_INVALID;
             for (int c = 0; c < 3; c++) tris.push_back(pos[3 * (size_t)k + c]);
+        }
+        if (normals) {
+            std::vector<float> cn;
+            if (has_nrm[m]) {
+                cn.resize(tris.size());
+                for (size_t t = 0; t < mesh.size() / 3; t++) {
+                    float fn[3];
+                    face_normal(&tris[9 * t], fn);
+                    for (int c = 0; c < 3; c++) {
+                        const long k = fnrm[m][3 * t + c];
+                        if (k >= nn) return MRH_ERR_INVALID;
+                        for (int a = 0; a < 3; a++) cn[9 * t + 3 * c + a] = k < 0 ? fn[a] : nrm[3 * (size_t)k + a];
+                    }
+                }
+            }
+            normals->push_back(std::move(cn));
         }
         meshes->push_back(std::move(tris));
     }
@@ -197,9 +261,14 @@ private:
     std::map<uint32_t, Mat4> transforms;
     std::map<uint32_t, float> media_ior;
     std::map<uint32_t, MatRef> materials;
-    std::map<uint32_t, std::vector<float>> prims;   // local-space triangles [..][9]
+    std::map<uint32_t, std::vector<float>> prims;     // local-space triangles [..][9]
+    std::map<uint32_t, std::vector<float>> prim_nrm;  // their corner normals [..][9], if any
     std::map<uint32_t, Light> lights;
-    std::map<std::string, std::vector<std::vector<float>>> obj_cache;
+    struct ObjFile {
+        std::vector<std::vector<float>> meshes, normals;
+    };
+    std::map<std::string, ObjFile> obj_cache;
+    bool nrm_dropped = false;
 
     // strict: fail with "what is not supported"; relaxed: note what was done
     void unsupported(const std::string& what, const std::string& action = "skipped")
@@ -373,14 +442,24 @@ private:
         }
     }
 
-    // OBJ meshes of a file (positions only)
-    const std::vector<std::vector<float>>& obj(const std::string& file)
+    // OBJ meshes of a file (positions and corner normals)
+    const ObjFile& obj(const std::string& file)
     {
         auto it = obj_cache.find(file);
         if (it != obj_cache.end()) return it->second;
-        std::vector<std::vector<float>> meshes;
-        if (obj_read(file.c_str(), &meshes) != MRH_OK) throw std::runtime_error("cannot read OBJ " + file);
-        return obj_cache[file] = std::move(meshes);
+        ObjFile o;
+        if (obj_read(file.c_str(), &o.meshes, &o.normals) != MRH_OK) throw std::runtime_error("cannot read OBJ " + file);
+        return obj_cache[file] = std::move(o);
+    }
+
+    // a "normal" list of vec3 into corner normals
+    static void normals_of(const std::vector<Value>& list, std::vector<float>& out3)
+    {
+        for (const Value& p : list) {
+            double v[3];
+            vec3(p, v);
+            for (double c : v) out3.push_back((float)c);
+        }
     }
 
     void read_primitives(const Value& root)
@@ -402,16 +481,29 @@ private:
                         vec3(p, v);
                         for (double c : v) tris.push_back((float)c);
                     }
+                    if (node.get("normal")) {
+                        const auto& nl = per_id(node, "normal", i, n).array();
+                        if (nl.size() != pos.size())
+                            throw std::runtime_error("nodeTriangle needs one normal per position");
+                        normals_of(nl, prim_nrm[id[i]]);
+                    }
                 } else if (t == "Triangle" && name == ".nodeTriangleIndexed") {
                     const auto& pos = need(node, "position").array();
                     const auto& idx = per_id(node, "index", i, n).array();
                     if (idx.size() % 3) throw std::runtime_error("nodeTriangleIndexed indices are not triangles");
+                    const Value* nv = node.get("normal");
+                    if (nv && nv->array().size() != pos.size())
+                        throw std::runtime_error("nodeTriangleIndexed needs one normal per position");
                     for (const Value& e : idx) {
                         const size_t k = (size_t)e.number();
                         if (k >= pos.size()) throw std::runtime_error("nodeTriangleIndexed index out of range");
                         double v[3];
                         vec3(pos[k], v);
                         for (double c : v) tris.push_back((float)c);
+                        if (nv) {
+                            vec3(nv->array()[k], v);
+                            for (double c : v) prim_nrm[id[i]].push_back((float)c);
+                        }
                     }
                 } else if (t == "Triangle" && !name.empty() && name.compare(0, 5, ".node") != 0) {
                     const Value* tag = node.get("tag");
@@ -420,17 +512,18 @@ private:
                         continue;
                     }
                     const std::string file = (name[0] == '/') ? name : dir + name;
-                    const auto& meshes = obj(file);
+                    const ObjFile& o = obj(file);
                     const size_t inner = node.get("innerIndex") ? (size_t)per_id(node, "innerIndex", i, n).number() : 0;
-                    if (inner >= meshes.size())
+                    if (inner >= o.meshes.size())
                         throw std::runtime_error(file + " has no mesh " + std::to_string(inner));
-                    tris = meshes[inner];
+                    tris = o.meshes[inner];
+                    if (!o.normals[inner].empty()) prim_nrm[id[i]] = o.normals[inner];
                 } else if (t == "Sphere" && name == ".nodeSphere") {
                     double c[3];
                     vec3(per_id(node, "position", i, n), c);
                     const double r = per_id(node, "radius", i, n).number();
                     unsupported("primitive " + std::to_string(id[i]) + ", an analytic sphere,", "tessellated (64 x 32)");
-                    tessellate_sphere(c, r, tris);
+                    tessellate_sphere(c, r, tris, prim_nrm[id[i]]);
                 } else {
                     unsupported("primitive " + std::to_string(id[i]) + " of type " + t + " " + name);
                 }
@@ -438,7 +531,7 @@ private:
         }
     }
 
-    static void tessellate_sphere(const double c[3], double r, std::vector<float>& tris)
+    static void tessellate_sphere(const double c[3], double r, std::vector<float>& tris, std::vector<float>& nrm)
     {
         const int slices = 64, stacks = 32;
         auto pt = [&](int st, int sl, double* p) {
@@ -456,10 +549,15 @@ private:
                 pt(st, sl + 1, e);
                 auto tri = [&](const double* p, const double* q, const double* s) {
                     for (const double* v : {p, q, s})
-                        for (int k = 0; k < 3; k++) tris.push_back((float)v[k]);
+                        for (int k = 0; k < 3; k++) {
+                            tris.push_back((float)v[k]);
+                            nrm.push_back((float)((v[k] - c[k]) / r));   // radial
+                        }
                 };
-                if (st != 0) tri(a, b, e);
-                if (st != stacks - 1) tri(b, d, e);
+                // wound counter-clockwise seen from outside: the face
+                // normals agree with the radial shading normals
+                if (st != 0) tri(a, e, b);
+                if (st != stacks - 1) tri(b, e, d);
             }
     }
 
@@ -487,10 +585,40 @@ private:
         }
     }
 
-    void emit(const std::vector<float>& local, const Mat4& m, uint32_t key)
+    // corner normals of the next triangle (world triangle w, normals wn or
+    // NULL); the first triangle with normals gives the flat ones before it
+    // their face normals
+    void add_normals(const float* w, const float* wn)
     {
+        if (!wn && out.normals.empty()) return;
+        float fn[3];
+        while (out.normals.size() < 9 * out.keys.size()) {
+            face_normal(&out.tris[out.normals.size()], fn);
+            for (int c = 0; c < 3; c++) out.normals.insert(out.normals.end(), fn, fn + 3);
+        }
+        if (wn) {
+            out.normals.insert(out.normals.end(), wn, wn + 9);
+        } else {
+            face_normal(w, fn);
+            for (int c = 0; c < 3; c++) out.normals.insert(out.normals.end(), fn, fn + 3);
+        }
+    }
+
+    void emit(const std::vector<float>& local, const std::vector<float>& lnrm, const Mat4& m, uint32_t key)
+    {
+        // normals transform by the inverse transpose: the cofactor matrix,
+        // negated for a mirroring transform, before renormalising
+        double cof[9];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                const int r1 = (r + 1) % 3, r2 = (r + 2) % 3, c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                cof[3 * r + c] = m.m[4 * r1 + c1] * m.m[4 * r2 + c2] - m.m[4 * r1 + c2] * m.m[4 * r2 + c1];
+            }
+        const double det = m.m[0] * cof[0] + m.m[1] * cof[1] + m.m[2] * cof[2];
+        const double sgn = det < 0 ? -1.0 : 1.0;
+        const bool smooth = lnrm.size() == local.size();
         for (size_t t = 0; t + 9 <= local.size(); t += 9) {
-            float w[9];
+            float w[9], wn[9];
             for (int v = 0; v < 3; v++) {
                 const double x = local[t + 3 * v], y = local[t + 3 * v + 1], z = local[t + 3 * v + 2];
                 for (int r = 0; r < 3; r++)
@@ -501,7 +629,23 @@ private:
             const double bx = w[6] - w[0], by = w[7] - w[1], bz = w[8] - w[2];
             const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
             if (cx * cx + cy * cy + cz * cz == 0.0) continue;
+            if (smooth) {
+                float fn[3];
+                face_normal(w, fn);
+                for (int v = 0; v < 3; v++) {
+                    const double x = lnrm[t + 3 * v], y = lnrm[t + 3 * v + 1], z = lnrm[t + 3 * v + 2];
+                    double n[3], l = 0.0;
+                    for (int r = 0; r < 3; r++) {
+                        n[r] = sgn * (cof[3 * r] * x + cof[3 * r + 1] * y + cof[3 * r + 2] * z);
+                        l += n[r] * n[r];
+                    }
+                    l = std::sqrt(l);
+                    // a zero or non-finite normal falls back to the face normal
+                    for (int r = 0; r < 3; r++) wn[3 * v + r] = (l > 0 && std::isfinite(l)) ? (float)(n[r] / l) : fn[r];
+                }
+            }
             out.tris.insert(out.tris.end(), w, w + 9);
+            add_normals(w, smooth ? wn : nullptr);
             out.keys.push_back(key);
         }
     }
@@ -533,7 +677,7 @@ private:
         auto it = mesh_of.find(prim);
         if (it == mesh_of.end()) {
             const uint32_t before = (uint32_t)out.keys.size();
-            emit(local, identity(), key);
+            emit(local, {}, identity(), key);
             if (out.keys.size() == before) return;   // nothing but degenerate triangles
             it = mesh_of.emplace(prim, (uint32_t)out.mesh_first.size()).first;
             out.mesh_first.push_back(before);
@@ -546,8 +690,18 @@ private:
 
     void place(uint32_t prim, const std::vector<float>& local, const Mat4& m, uint32_t key)
     {
-        if (instanced) instance(prim, local, m, key);
-        else emit(local, m, key);
+        auto n = prim_nrm.find(prim);
+        if (instanced) {
+            if (n != prim_nrm.end() && !n->second.empty() && !nrm_dropped) {
+                // the two-level tracer shades flat
+                out.warnings += "vertex normals in instanced mode: flat shading used\n";
+                nrm_dropped = true;
+            }
+            instance(prim, local, m, key);
+        } else {
+            static const std::vector<float> none;
+            emit(local, n != prim_nrm.end() ? n->second : none, m, key);
+        }
     }
 
     void read_surfaces(const Value& root)
@@ -689,6 +843,7 @@ int32_t mrh_scene_file_desc(const mrh_scene_file* f, mrh_scene_desc* d)
     d->reflect = f->reflect.data();
     d->n_reflect = (uint32_t)(f->reflect.size() / 4);
     d->morton_bits = 30;
+    if (f->normals.size() == f->tris.size() && !f->normals.empty()) d->normals = f->normals.data();
     if (f->instanced) d->instancing = &f->inst_desc;
     return MRH_OK;
 }

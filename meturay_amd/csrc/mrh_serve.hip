@@ -464,6 +464,7 @@ void fill_slot(mrh_server* s, ServeReq* q)
     d.n_leaves = b->n;
     d.q.mats = Materials{sc->lambert, sc->metal, sc->dielectric, sc->emissive, sc->reflect};
     d.q.leaves = (const float4*)b->leaves;
+    d.q.tbn = sc->tbn;
     if (sc->two_level) {
         d.two = 1u;
         d.n_leaves = sc->n_inst;

@@ -123,7 +123,7 @@ class SceneDesc(C.Structure):
                 ("lambert", _P), ("n_lambert", _U32), ("metal", _P), ("n_metal", _U32),
                 ("dielectric", _P), ("n_dielectric", _U32), ("emissive", _P),
                 ("n_emissive", _U32), ("morton_bits", _U32), ("reflect", _P), ("n_reflect", _U32),
-                ("instancing", _P)]
+                ("instancing", _P), ("normals", _P)]
 
 
 class SceneBuffers(C.Structure):
@@ -491,6 +491,8 @@ def load_scene_file(path: str, substitute: bool = False, instanced: bool = False
             "emissive": take(d.emissive, d.n_emissive, 3),
             "reflect": take(d.reflect, d.n_reflect, 4),
         }
+        if d.normals:
+            out["normals"] = take(d.normals, d.n, 9)
         if d.instancing:
             idesc = C.cast(d.instancing, C.POINTER(InstancingDesc)).contents
             ni = idesc.n_instances
@@ -561,6 +563,10 @@ class Scene:
             setattr(d, name, a.ctypes.data if a is not None and a.size else None)
             setattr(d, "n_" + name, a.shape[0] if a is not None else 0)
         d.morton_bits = morton_bits
+        nrm = scene.get("normals")
+        if nrm is not None:
+            k["normals"] = np.ascontiguousarray(nrm, np.float32).reshape(-1, 9)
+            d.normals = k["normals"].ctypes.data
         inst = scene.get("instancing")
         self.two_level = inst is not None
         if inst is not None:

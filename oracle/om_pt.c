@@ -359,6 +359,164 @@ static om_surface make_surface(const float* tri, float ba, float bb, om_v3 ray_d
     return s;
 }
 
+/* ------------------------- smooth shading frames ----------------------- */
+/* RayLib/Vector.hpp:877-886 (OrthogonalVector) */
+static om_v3 orthogonal_vector(om_v3 v)
+{
+    if (fabsf(v.x) > fabsf(v.y)) return v3_divs(v3(-v.z, 0.0f, v.x), sqrtf(v.x * v.x + v.z * v.z));
+    return v3_divs(v3(0.0f, v.z, -v.y), sqrtf(v.y * v.y + v.z * v.z));
+}
+
+static float q_dot(om_q a, om_q b)
+{
+    float d = 0.0f;
+    d += a.w * b.w;
+    d += a.x * b.x;
+    d += a.y * b.y;
+    d += a.z * b.z;
+    return d;
+}
+static om_q q_scale(om_q a, float s) { return q_make(a.w * s, a.x * s, a.y * s, a.z * s); }
+static om_q q_add(om_q a, om_q b) { return q_make(a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z); }
+/* Vector::Normalize: times the reciprocal of the length */
+static om_q q_normalize(om_q q) { return q_scale(q, 1.0f / sqrtf(q_dot(q, q))); }
+
+/* TransformGen::Space (RayLib/Quaternion.hpp:370-436): Mike Day's
+ * matrix-to-quaternion conversion of the basis (x, y, z), normalised and
+ * conjugated; x is negated when Cross(x, y) - z exceeds 0.1 in every axis */
+static om_q space_quat(om_v3 x, om_v3 y, om_v3 z)
+{
+    om_v3 c = v3_sub(v3_cross(x, y), z);
+    if (fabsf(c.x) > 0.1f && fabsf(c.y) > 0.1f && fabsf(c.z) > 0.1f) x = v3_neg(x);
+    float t;
+    om_q q;
+    if (z.z < 0.0f) {
+        if (x.x > y.y) {
+            t = 1.0f + x.x - y.y - z.z;
+            q = q_make(y.z - z.y, t, x.y + y.x, z.x + x.z);
+        } else {
+            t = 1.0f - x.x + y.y - z.z;
+            q = q_make(z.x - x.z, x.y + y.x, t, y.z + z.y);
+        }
+    } else {
+        if (x.x < -y.y) {
+            t = 1.0f - x.x - y.y + z.z;
+            q = q_make(x.y - y.x, z.x + x.z, y.z + z.y, t);
+        } else {
+            t = 1.0f + x.x + y.y + z.z;
+            q = q_make(t, y.z - z.y, z.x - x.z, x.y - y.x);
+        }
+    }
+    q = q_scale(q, 0.5f / sqrtf(t));
+    return q_conj(q_normalize(q));
+}
+
+/* GPUPrimitiveTriangle.cpp:286-316 for a vertex without texture
+ * coordinates: tangent OrthogonalVector(n), Gram-Schmidt, bitangent n x t */
+static om_q vertex_frame(om_v3 n)
+{
+    om_v3 t = orthogonal_vector(n);
+    if (v3_dot(t, t) < 1.0e-5f) t = orthogonal_vector(n);
+    t = v3_normalize(t);
+    om_v3 tn = v3_normalize(v3_sub(t, v3_scale(n, v3_dot(n, t))));
+    om_v3 b = v3_cross(n, tn);
+    return space_quat(tn, b, n);
+}
+
+/* acos on [0, 1] and sin on [0, pi/2] from the shared atan and sin/cos */
+static float acos_unit(float x) { return 2.0f * om_atan_unit_f(sqrtf((1.0f - x) / (1.0f + x))); }
+static float sin_q(float x)
+{
+    float s, c;
+    om_sincos_2pi(x * 0.15915494309189534f, &s, &c);
+    return s;
+}
+
+/* Quat::SLerp (RayLib/Quaternion.hpp:256-281) */
+static om_q q_slerp(om_q a, om_q b, float t)
+{
+    float cos_t = q_dot(a, b);
+    float cos_f = (cos_t >= 0.0f) ? cos_t : -cos_t;
+    float s0, s1;
+    if (cos_f < 1.0f - 1.0e-5f) {
+        float angle = acos_unit(cos_f);
+        float sa = sin_q(angle);
+        s0 = sin_q(angle * (1.0f - t)) / sa;
+        s1 = sin_q(angle * t) / sa;
+    } else {
+        s0 = 1.0f - t;
+        s1 = t;
+    }
+    s1 = (cos_t >= 0.0f) ? s1 : -s1;
+    return q_add(q_scale(a, s0), q_scale(b, s1));
+}
+
+/* Quat::BarySLerp (RayLib/Quaternion.hpp:283-318) */
+static om_q q_bary_slerp(om_q q0, om_q q1, om_q q2, float a, float b)
+{
+    float c = 1.0f - a - b;
+    if (fabsf(a + b) < 1.0e-5f) return q2;
+    float ab = a / (a + b);
+    return q_slerp(q_slerp(q1, q0, ab), q2, c);
+}
+
+/* GPUPrimitiveTriangle.h:626-670 with per-vertex frames: the corners'
+ * world-to-tangent rotations blended by BarySLerp and normalised, flipped by
+ * TANGENT_ROT on the back side; position and geometric normal as the flat
+ * surface */
+static om_surface make_surface_tbn(const float* tri, float ba, float bb, om_v3 ray_dir, const float* tq)
+{
+    om_surface s = make_surface(tri, ba, bb, ray_dir);
+    om_q q0 = q_make(tq[0], tq[1], tq[2], tq[3]);
+    om_q q1 = q_make(tq[4], tq[5], tq[6], tq[7]);
+    om_q q2 = q_make(tq[8], tq[9], tq[10], tq[11]);
+    om_q w = q_normalize(q_bary_slerp(q0, q1, q2, ba, bb));
+    if (s.back) w = q_mul(q_make(0.0f, 1.0f, 0.0f, 0.0f), w);
+    s.w2t = w;
+    return s;
+}
+
+void om_orthogonal_vector(const float v[3], float out[3])
+{
+    om_v3 r = orthogonal_vector(v3(v[0], v[1], v[2]));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+void om_space_quat(const float x[3], const float y[3], const float z[3], float q[4])
+{
+    om_q r = space_quat(v3(x[0], x[1], x[2]), v3(y[0], y[1], y[2]), v3(z[0], z[1], z[2]));
+    q[0] = r.w; q[1] = r.x; q[2] = r.y; q[3] = r.z;
+}
+void om_qslerp(const float a[4], const float b[4], float t, float out[4])
+{
+    om_q r = q_slerp(q_make(a[0], a[1], a[2], a[3]), q_make(b[0], b[1], b[2], b[3]), t);
+    out[0] = r.w; out[1] = r.x; out[2] = r.y; out[3] = r.z;
+}
+void om_bary_slerp(const float q0[4], const float q1[4], const float q2[4], float a, float b, float out[4])
+{
+    om_q r = q_bary_slerp(q_make(q0[0], q0[1], q0[2], q0[3]), q_make(q1[0], q1[1], q1[2], q1[3]),
+                          q_make(q2[0], q2[1], q2[2], q2[3]), a, b);
+    out[0] = r.w; out[1] = r.x; out[2] = r.y; out[3] = r.z;
+}
+/* per-corner rotations of n triangles' corner normals [n][9] (normalised
+ * first) -> [n][3][4] (w, x, y, z) */
+void om_corner_frames(const float* normals, uint32_t n, float* out)
+{
+    for (size_t k = 0; k < (size_t)n * 3; k++) {
+        om_v3 nv = v3_normalize(v3(normals[3 * k], normals[3 * k + 1], normals[3 * k + 2]));
+        om_q q = vertex_frame(nv);
+        out[4 * k + 0] = q.w; out[4 * k + 1] = q.x; out[4 * k + 2] = q.y; out[4 * k + 3] = q.z;
+    }
+}
+/* world-to-tangent rotation of a smooth surface point (tests) */
+void om_smooth_frame(const float tri[9], const float tq[12], float ba, float bb, const float dir[3], float q[4],
+                     float normal[3])
+{
+    om_surface s = make_surface_tbn(tri, ba, bb, v3(dir[0], dir[1], dir[2]), tq);
+    q[0] = s.w2t.w; q[1] = s.w2t.x; q[2] = s.w2t.y; q[3] = s.w2t.z;
+    om_v3 nv = q_rotate(q_conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
+    normal[0] = nv.x; normal[1] = nv.y; normal[2] = nv.z;
+}
+
 typedef struct {
     om_v3 o, d;
     om_v3 refl;
@@ -799,7 +957,8 @@ static void trace_path(const om_scene* sc, const om_camera* cam, const om_render
         }
         if (batch == OM_BATCH_NULL) break;
         if (bounce + 1 == p->max_depth) break;   /* the last segment only collects emission */
-        om_surface s = make_surface(tri, b[0], b[1], d);
+        om_surface s = sc->tbn ? make_surface_tbn(tri, b[0], b[1], d, sc->tbn + (size_t)prim * 12)
+                               : make_surface(tri, b[0], b[1], d);
         om_v3 wi = v3_neg(v3_normalize(d));
         if (nee && sc->n_lights &&
             (batch == OM_BATCH_LAMBERT || (batch == OM_BATCH_METAL && (1.0f - sc->metal[(size_t)id * 10]) < 0.99f))) {

@@ -191,6 +191,9 @@ typedef struct {
     const om_two_level* two_level;
     uint32_t        tree_width;    /* two-level: the meshes' tree width, 4 or 2 */
     const uint32_t* light_inst;
+    /* smooth shading (NULL: flat): per-corner world-to-tangent rotations,
+     * [n][3][4] (w, x, y, z) in primitive order (om_corner_frames) */
+    const float*    tbn;
 } om_scene;
 
 typedef struct {
@@ -319,4 +322,13 @@ void om_path_uniforms(uint32_t seed, uint32_t sample_id, uint32_t n, float* out)
 #ifdef __cplusplus
 }
 #endif
+/* smooth shading frames (GPUPrimitiveTriangle.cpp:286-316, RayLib/Quaternion.hpp:256-436) */
+void om_orthogonal_vector(const float v[3], float out[3]);
+void om_space_quat(const float x[3], const float y[3], const float z[3], float q[4]);
+void om_qslerp(const float a[4], const float b[4], float t, float out[4]);
+void om_bary_slerp(const float q0[4], const float q1[4], const float q2[4], float a, float b, float out[4]);
+void om_corner_frames(const float* normals, uint32_t n, float* out);
+void om_smooth_frame(const float tri[9], const float tq[12], float ba, float bb, const float dir[3], float q[4],
+                     float normal[3]);
+
 #endif

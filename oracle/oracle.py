@@ -50,7 +50,8 @@ class OmScene(C.Structure):
                 ("lambert", _P), ("metal", _P), ("dielectric", _P), ("emissive", _P),
                 ("sorted_idx", _P), ("left", _P), ("right", _P), ("node_aabb", _P),
                 ("root", _U32), ("nodes4", _P), ("root4", _U32), ("lights", _P), ("n_lights", _U32),
-                ("reflect", _P), ("two_level", _P), ("tree_width", _U32), ("light_inst", _P)]
+                ("reflect", _P), ("two_level", _P), ("tree_width", _U32), ("light_inst", _P),
+                ("tbn", _P)]
 
 
 class OmTwoLevel(C.Structure):
@@ -123,6 +124,12 @@ def lib():
         L.om_intersect_aabb.argtypes = [_P, _P, _P, _F, _F, _P]
         L.om_slab_box_test.restype = C.c_int
         L.om_slab_box_test.argtypes = [_P, _P, _P, _F, _F, _P]
+        L.om_orthogonal_vector.argtypes = [_P, _P]
+        L.om_space_quat.argtypes = [_P, _P, _P, _P]
+        L.om_qslerp.argtypes = [_P, _P, _F, _P]
+        L.om_bary_slerp.argtypes = [_P, _P, _P, _F, _F, _P]
+        L.om_corner_frames.argtypes = [_P, _U32, _P]
+        L.om_smooth_frame.argtypes = [_P, _P, _F, _F, _P, _P, _P]
         L.om_trace_closest.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F,
                                        _P, _P, _P, _P]
         L.om_trace_any.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F, _P]
@@ -242,6 +249,10 @@ def ref():
         R.ref_aabb_centroid.argtypes = [_P, _P]
         if hasattr(R, "ref_ray_transform"):
             R.ref_ray_transform.argtypes = [_P, _P, _P, _P, _P]
+        R.ref_orthogonal_vector.argtypes = [_P, _P]
+        R.ref_space_quat.argtypes = [_P, _P, _P, _P]
+        R.ref_slerp.argtypes = [_P, _P, _F, _P]
+        R.ref_bary_slerp.argtypes = [_P, _P, _P, _F, _F, _P]
         _ref = R
     return _ref
 
@@ -526,6 +537,15 @@ def path_uniforms(seed: int, sample_id: int, n: int) -> np.ndarray:
     return out
 
 
+def corner_frames(normals) -> np.ndarray:
+    """Per-corner world-to-tangent rotations [n][3][4] (w, x, y, z) of
+    per-corner shading normals [n][9] (om_corner_frames)."""
+    nrm = np.ascontiguousarray(normals, np.float32).reshape(-1, 9)
+    out = np.zeros((nrm.shape[0], 3, 4), np.float32)
+    lib().om_corner_frames(_ptr(nrm), nrm.shape[0], _ptr(out))
+    return out
+
+
 def _scene_struct(scene: dict, bvh: LBVH, keep: dict, width: int = 4) -> OmScene:
 
     def arr(name, dtype, cols):
@@ -556,6 +576,9 @@ def _scene_struct(scene: dict, bvh: LBVH, keep: dict, width: int = 4) -> OmScene
         sc.light_inst = _ptr(keep["light_inst"])
         sc.n_lights = len(bvh.lights)
         return sc
+    if scene.get("normals") is not None:
+        keep["tbn"] = corner_frames(scene["normals"])
+        sc.tbn = _ptr(keep["tbn"])
     sc.sorted_idx = _ptr(bvh.sorted_idx)
     sc.left = _ptr(bvh.left)
     sc.right = _ptr(bvh.right)

@@ -150,4 +150,32 @@ void ref_ray_transform(const float m12[12], const float o[3], const float d[3], 
     out_d[0] = t.getDirection()[0]; out_d[1] = t.getDirection()[1]; out_d[2] = t.getDirection()[2];
 }
 
+// RayLib/Vector.hpp:877-886
+void ref_orthogonal_vector(const float v[3], float out[3])
+{
+    Vector3f r = OrthogonalVector(Vector3f(v[0], v[1], v[2]));
+    out[0] = r[0]; out[1] = r[1]; out[2] = r[2];
+}
+
+// RayLib/Quaternion.hpp:370-436 (TransformGen::Space), q as (w, x, y, z)
+void ref_space_quat(const float x[3], const float y[3], const float z[3], float q[4])
+{
+    QuatF r;
+    TransformGen::Space(r, Vector3f(x[0], x[1], x[2]), Vector3f(y[0], y[1], y[2]), Vector3f(z[0], z[1], z[2]));
+    q[0] = r[0]; q[1] = r[1]; q[2] = r[2]; q[3] = r[3];
+}
+
+// RayLib/Quaternion.hpp:256-318 (Quat::SLerp, Quat::BarySLerp)
+void ref_slerp(const float a[4], const float b[4], float t, float out[4])
+{
+    QuatF r = Quat::SLerp(QuatF(a[0], a[1], a[2], a[3]), QuatF(b[0], b[1], b[2], b[3]), t);
+    out[0] = r[0]; out[1] = r[1]; out[2] = r[2]; out[3] = r[3];
+}
+void ref_bary_slerp(const float q0[4], const float q1[4], const float q2[4], float a, float b, float out[4])
+{
+    QuatF r = Quat::BarySLerp(QuatF(q0[0], q0[1], q0[2], q0[3]), QuatF(q1[0], q1[1], q1[2], q1[3]),
+                              QuatF(q2[0], q2[1], q2[2], q2[3]), a, b);
+    out[0] = r[0]; out[1] = r[1]; out[2] = r[2]; out[3] = r[3];
+}
+
 }  // extern "C"

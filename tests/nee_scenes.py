@@ -84,3 +84,71 @@ def lit_spheres(reflect=False):
         sc["reflect"] = np.array([[0.9, 0.7, 0.5, 0.0], [1.0, 1.0, 1.0, 0.3]], np.float32)
     sc["prim_key"] = keys
     return sc
+
+
+def smooth_normals(tris):
+    """Per-corner shading normals [n][9] of a triangle soup: the area-weighted
+    sum of the face normals of all triangles sharing a vertex position
+    (bit-equal floats), normalised; a corner whose sum vanishes keeps its
+    face normal."""
+    t = np.asarray(tris, np.float32).reshape(-1, 3, 3)
+    face = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]).astype(np.float64)   # 2 x area
+    corners = t.reshape(-1, 3)
+    _, inv = np.unique(corners.view(np.uint32).view([("", np.uint32)] * 3), return_inverse=True)
+    inv = inv.reshape(-1)
+    acc = np.zeros((inv.max() + 1, 3))
+    np.add.at(acc, inv, np.repeat(face, 3, axis=0))
+    n = acc[inv]
+    ln = np.linalg.norm(n, axis=1, keepdims=True)
+    fn = np.repeat(face / np.linalg.norm(face, axis=1, keepdims=True), 3, axis=0)
+    n = np.where(ln > 1e-12, n / np.maximum(ln, 1e-300), fn)
+    return n.reshape(-1, 9).astype(np.float32)
+
+
+def smooth_spheres():
+    """The 8 x 8 displaced-sphere grid with smooth vertex normals."""
+    sc = N.gen_sphere_grid(8, 12, 12, 5)
+    sc["normals"] = smooth_normals(sc["tris"])
+    return sc
+
+
+def uv_sphere(c, r, stacks=16, slices=24):
+    """Triangles [..][9] of a UV sphere and their radial corner normals."""
+    def pt(st, sl):
+        th, ph = math.pi * st / stacks, 2.0 * math.pi * sl / slices
+        return np.array([math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph)])
+    tris, nrm = [], []
+    for st in range(stacks):
+        for sl in range(slices):
+            a, b, d, e = pt(st, sl), pt(st + 1, sl), pt(st + 1, sl + 1), pt(st, sl + 1)
+            quads = []
+            if st != 0:
+                quads.append((a, e, b))      # counter-clockwise seen from outside
+            if st != stacks - 1:
+                quads.append((b, e, d))
+            for q in quads:
+                tris.append(np.concatenate([np.asarray(c) + r * v for v in q]))
+                nrm.append(np.concatenate(q))
+    return np.array(tris, np.float32), np.array(nrm, np.float32)
+
+
+def smooth_cornell():
+    """The Cornell box with three smooth UV spheres: Lambert, rough metal
+    and glass; walls keep their face normals."""
+    sc = N.gen_cornell(2)
+    t = sc["tris"].reshape(-1, 3, 3).astype(np.float64)
+    f = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    f /= np.linalg.norm(f, axis=1, keepdims=True)
+    tris, nrm, keys = [sc["tris"]], [np.repeat(f, 3, axis=0).reshape(-1, 9)], [sc["prim_key"]]
+    for c, r, key in [((0.5, -0.65, -0.3), 0.3, (1 << 24) | 0), ((-0.45, -0.6, -0.1), 0.35, (2 << 24) | 0),
+                      ((0.0, 0.15, -0.45), 0.3, (3 << 24) | 0)]:
+        st, sn = uv_sphere(c, r)
+        tris.append(st)
+        nrm.append(sn)
+        keys.append(np.full(len(st), key, np.uint32))
+    sc["tris"] = np.concatenate(tris).astype(np.float32)
+    sc["normals"] = np.concatenate(nrm).astype(np.float32)
+    sc["prim_key"] = np.concatenate(keys).astype(np.uint32)
+    sc["metal"] = np.array([[0.25, 0.2, 0.4, 1.4, 3.5, 2.4, 1.8, 1.0, 1.0, 1.0]], np.float32)
+    sc["dielectric"] = np.array([[1.0, 1.0, 1.0, 1.5]], np.float32)
+    return sc

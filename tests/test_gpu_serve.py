@@ -28,12 +28,15 @@ MIX = [
     ("lit", 48, 40, 1, 8, 3, 7, 1, 1, 1),
     ("cornell", 24, 24, 2, 6, 3, 8, 1, 1, 1),
     ("lit", 40, 30, 2, 8, 3, 9, 3, 1, 1),
+    ("smooth", 48, 36, 1, 6, 2, 10, 1, 1, 1),
+    ("smooth", 32, 32, 2, 6, 2, 11, 2, 0, 0),
 ]
 
 
 @pytest.fixture(scope="module")
 def scenes(ctx):
-    sc = {"cornell": N.gen_cornell(2), "spheres": N.gen_sphere_grid(4, 8, 8, 5), "lit": S.lit_spheres()}
+    sc = {"cornell": N.gen_cornell(2), "spheres": N.gen_sphere_grid(4, 8, 8, 5), "lit": S.lit_spheres(),
+          "smooth": S.smooth_spheres()}
     res = {k: (v, N.Scene(ctx, v)) for k, v in sc.items()}
     yield res
     for _, s in res.values():

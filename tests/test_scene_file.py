@@ -211,3 +211,118 @@ def test_room_instanced_mode():
     assert len(inst["inst_mesh"]) == 9 and len(inst["mesh_first"]) - 1 == 7
     assert inst["inst_mesh"][6] == inst["inst_mesh"][5] and inst["inst_mesh"][7] == inst["inst_mesh"][4]
     assert ((inst["inst_key"] >> 24) == 4).tolist() == [False] * 8 + [True]
+
+
+SMOOTH_OBJ = """# corner normals, a corner without one, a degenerate face
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+vn 0 0 1
+vn 0.3 0 1
+vn 0 0.3 2
+vn -0.2 -0.2 1
+o a
+f 1//1 2//2 3//3
+f 1/1/4 3//3 4
+f 1//1 2//2 1//1
+"""
+
+SMOOTH_JSON = """{
+  "Transforms": [
+    {"id": 0, "type": "Identity"},
+    {"id": 1, "type": "Single", "layout": "trs", "translate": [0.5, 0.0, -0.25],
+     "rotate": [20.0, 30.0, 40.0], "scale": [0.5, 2.0, 0.6]},
+    {"id": 2, "type": "Single", "layout": "matrix4x4", "major": "row",
+     "matrix": [-1, 0, 0, 0,  0, 1, 0.5, 0,  0, 0, 3, 1,  0, 0, 0, 1]}
+  ],
+  "Materials": [{"id": 1, "type": "LambertC", "albedo": [0.5, 0.5, 0.5]}],
+  "Primitives": [
+    {"id": 0, "type": "Triangle", "name": "smooth.obj", "tag": "assimp_", "innerIndex": 0},
+    {"id": 1, "type": "Triangle", "name": ".nodeTriangle",
+     "position": [[0, 0, 1], [1, 0, 1], [0, 1, 1]], "normal": [[0, 0, 1], [0.1, 0, 1], [0, 0.1, 1]]},
+    {"id": 2, "type": "Triangle", "name": ".nodeTriangleIndexed",
+     "position": [[0, 0, 2], [1, 0, 2], [1, 1, 2], [0, 1, 2]],
+     "normal": [[0, 0, 1], [0, 0.2, 1], [0.2, 0, 1], [0.1, 0.1, 1]], "index": [0, 1, 2, 0, 2, 3]},
+    {"id": 3, "type": "Triangle", "name": ".nodeTriangle",
+     "position": [[0, 0, 3], [1, 0, 3], [0, 1, 3]]}
+  ],
+  "Surfaces": [
+    {"transform": 0, "material": 1, "primitive": 3},
+    {"transform": 1, "material": 1, "primitive": 0},
+    {"transform": 2, "material": 1, "primitive": 1},
+    {"transform": 2, "material": 1, "primitive": 2}
+  ]
+}"""
+
+
+def test_vertex_normals(tmp_path):
+    """Corner normals from OBJ vn and the JSON "normal" lists, moved to world
+    space by the inverse transpose (independently: numpy's inverse) and
+    renormalised; a corner without a normal, and a flat primitive emitted
+    before the first smooth one, get the world face normal; the degenerate
+    face is dropped together with its normals."""
+    write(tmp_path, "smooth.obj", SMOOTH_OBJ)
+    p = write(tmp_path, "smooth.json", SMOOTH_JSON)
+    sc = N.load_scene_file(p)
+    tris = sc["tris"].reshape(-1, 3, 3).astype(np.float64)
+    nrm = sc["normals"].reshape(-1, 3, 3).astype(np.float64)
+    assert len(tris) == 1 + 2 + 1 + 2 and nrm.shape == tris.shape
+    unit = lambda v: v / np.linalg.norm(v)
+    face = lambda t: unit(np.cross(t[1] - t[0], t[2] - t[0]))
+    M1 = matrices(json.loads(strip_comments(SMOOTH_JSON)))[1][:3, :3]
+    M2 = np.array([[-1, 0, 0], [0, 1, 0.5], [0, 0, 3]], np.float64)
+    vn = np.array([[0, 0, 1], [0.3, 0, 1], [0, 0.3, 2], [-0.2, -0.2, 1]], np.float64)
+    world = lambda M, n: unit(np.linalg.inv(M).T @ n)
+    want = [
+        [face(tris[0])] * 3,                                         # flat, before the first smooth one
+        [world(M1, vn[0]), world(M1, vn[1]), world(M1, vn[2])],
+        [world(M1, vn[3]), world(M1, vn[2]), face(tris[2])],        # v/t/n, v//n, a corner without vn
+        [world(M2, n) for n in ([0, 0, 1], [0.1, 0, 1], [0, 0.1, 1])],
+        [world(M2, n) for n in ([0, 0, 1], [0, 0.2, 1], [0.2, 0, 1])],
+        [world(M2, n) for n in ([0, 0, 1], [0.2, 0, 1], [0.1, 0.1, 1])],
+    ]
+    assert np.allclose(nrm, np.array(want), atol=2e-6)
+    assert np.allclose(np.linalg.norm(nrm, axis=2), 1.0, atol=1e-6)
+    # the mirror keeps normals on the side the local normal points to
+    assert np.all(np.einsum("ij,ij->i", nrm[3], [face(tris[3])] * 3) < 0)
+    # instanced: flat shading, noted
+    inst = N.load_scene_file(p, instanced=True)
+    assert "normals" not in inst and "flat shading" in inst["warnings"]
+    # a scene without any normals stays flat
+    flat = json.loads(SMOOTH_JSON)
+    flat["Surfaces"] = flat["Surfaces"][:1]
+    plain = N.load_scene_file(write(tmp_path, "flat.json", json.dumps(flat)))
+    assert "normals" not in plain and len(plain["tris"]) == 1
+    # malformed: a normal list of the wrong length
+    bad = write(tmp_path, "bad_n.json", SMOOTH_JSON.replace('[0, 0.1, 1]]}', '[0, 0.1, 1], [0, 0, 1]]}'))
+    with pytest.raises(N.MrhError) as e:
+        N.load_scene_file(bad)
+    assert "one normal per position" in str(e.value)
+
+
+def test_substituted_sphere_is_smooth():
+    sc = N.load_scene_file(os.path.join(HERE, "room_extras.json"), substitute=True)
+    p = sc["tris"].reshape(-1, 3)
+    n = sc["normals"].reshape(-1, 3)
+    k = len(p) - 6                                                   # the floor's two triangles last
+    assert np.allclose(n[:k], (p[:k] - [0, 1, 0]) / 0.5, atol=2e-6)   # radial
+    t = p[:k].reshape(-1, 3, 3)
+    f = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])                # wound to face outwards
+    assert np.all(np.einsum("ij,ij->i", f, n[:k].reshape(-1, 3, 3).mean(axis=1)) > 0)
+    assert np.allclose(n[k:], [0, -1, 0])                            # the floor's vn
+
+
+def test_room_normals():
+    # room_parts.obj's vn: the floor (mesh 0) and the pentagon (mesh 2) carry
+    # one each, every other triangle its world face normal
+    sc = N.load_scene_file(os.path.join(HERE, "room.json"))
+    tris, _, scene = expected_room()
+    t = tris.reshape(-1, 3, 3)
+    f = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    f /= np.linalg.norm(f, axis=1, keepdims=True)
+    want = np.repeat(f[:, None, :], 3, axis=1)
+    want[0:2] = [0, -1, 0]                                           # floor, identity transform
+    m1 = np.linalg.inv(matrices(scene)[1][:3, :3]).T @ [0, 0, 1]     # pentagon, transform 1
+    want[14:17] = m1 / np.linalg.norm(m1)
+    assert np.allclose(sc["normals"].reshape(-1, 3, 3), want, atol=2e-6)

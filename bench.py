@@ -57,12 +57,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads():
+    """(threads this process may use, hardware threads of the host): the
+    scheduler affinity, capped by OMP_NUM_THREADS where the job sets it
+    (a shared host's per-GPU CPU share), and os.cpu_count()."""
+    hw = os.cpu_count() or 1
+    try:
+        share = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        share = hw
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return max(1, share), hw
+
+
 def cpu_baseline(sc, target_s: float):
-    """The oracle (C restatement, FP32 paths, FP64 accumulation) on host
-    threads over a row subset of the same frame."""
+    """The oracle (C restatement, FP32 paths, FP64 accumulation) on every
+    host thread this job may use, over a row subset of the same frame."""
     from oracle import oracle as O
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads, hw = host_threads()
     t0 = time.perf_counter()
     bvh = O.LBVH(sc["tris"], 30)
     build_s = time.perf_counter() - t0
@@ -86,10 +101,30 @@ def cpu_baseline(sc, target_s: float):
         "value": seg / dt / 1e6,
         "unit": "Mrays/s",
         "cores": threads,
+        "host_hardware_threads": hw,
         "kind": "port",
         "sample": f"{rows} of 1080 rows (every {step}th) x 1920 px x 16 spp of the same frame: "
-                  f"{seg} ray segments in {dt:.2f} s; CPU LBVH build {build_s:.2f} s not included",
+                  f"{seg} ray segments in {dt:.2f} s on {threads} threads (the job's CPU share; the host "
+                  f"has {hw} hardware threads); CPU LBVH build {build_s:.2f} s not included",
     }
+
+
+def stream_copy_gbs(torch, nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """Device-to-device copy bandwidth (bytes read + written per second):
+    the attainable HBM rate the roofline fraction can be read against."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def guided_leg(N, ctx, scene, img, torch):
@@ -395,6 +430,24 @@ def main():
     log(f"timed: {args.steps} frames in {t_total:.3f} s; trace {trace_ms:.1f} ms, shade {shade_ms:.1f} ms, "
         f"partition {sort_ms + hist_ms:.1f} ms")
 
+    # the same frame with the global radix partition (stable onesweep over
+    # material keys) in place of the per-block tile partition, outside the
+    # timed region
+    old_part = ctx.tune_get("shade_partition")
+    ctx.tune("shade_partition", 0)
+    scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        stg = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
+    torch.cuda.synchronize()
+    global_ms = (time.perf_counter() - t0) * 1e3 / 2
+    ctx.tune("shade_partition", old_part)
+    assert stg.segments == st.segments
+    log(f"global-partition frame: {global_ms:.2f} ms")
+    copy_gbs = stream_copy_gbs(torch)
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
+
     # traversal work of the same frame (same seed -> same paths), counted
     # outside the timed region
     width = ctx.tune_get("trace_width")
@@ -481,6 +534,8 @@ def main():
             "kernel_ms_per_frame": {"trace": round(trace_ms / args.steps, 3),
                                     "shade": round(shade_ms / args.steps, 3),
                                     "partition": round((sort_ms + hist_ms) / args.steps, 3)},
+            "global_partition_ms_per_frame": round(global_ms, 3),
+            "global_partition_mrays_s": round(st.segments / (global_ms / 1e3) / 1e6, 3),
         },
         "roofline": {
             "bound": "hbm",
@@ -490,6 +545,16 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic[0] if traffic else None,
             "traffic_source": traffic[1] if traffic else None,
+            "traffic_over_algorithmic": round(traffic[0] / (trace_bytes / max(launches, 1)), 4) if traffic else None,
+            "stream_copy_gbs": round(copy_gbs, 1),
+            "frac_of_stream_copy": round(achieved / copy_gbs, 4),
+            "algorithmic_model": "binary-traversal bytes: 44 B per segment + 32 B per binary LBVH node visit "
+                                 "+ 36 B per triangle test (the specified traversal), over the walked "
+                                 "kernel's time",
+            # measured HBM bytes per launch over the launch time: how busy HBM really is (the tree's
+            # nodes are mostly served from L2, so this sits well below frac)
+            "measured_hbm_frac": round(traffic[0] / (trace_ms / 1e3 / max(trace_launches, 1)) / 1e9
+                                       / HBM_PEAK_GBS, 4) if traffic else None,
             "kernel": "trace_pt",
             "algorithmic_bytes_per_frame": int(trace_bytes),
             "algorithmic_bytes_per_launch": int(trace_bytes / max(launches, 1)),

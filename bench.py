@@ -351,7 +351,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--knobs", default=None,
-                    help="fixed launch shape width,lds,blocks_per_cu instead of autotuning (e.g. 4,8,8)")
+                    help="fixed launch shape width,lds,blocks_per_cu[,trace_block,shade_tile] instead of "
+                         "autotuning (e.g. 4,8,8 or 4,8,8,256,1024)")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-tracer leg")
     ap.add_argument("--no-nee", action="store_true", help="skip the next-event-estimation leg")
     ap.add_argument("--no-instanced", action="store_true", help="skip the two-level (instanced) leg")
@@ -392,19 +393,22 @@ def main():
 
     tune = None
     if args.knobs:
-        w_, l_, b_ = (int(x) for x in args.knobs.split(","))
-        ctx.tune("trace_width", w_)
-        ctx.tune("trace_lds_entries", l_)
-        ctx.tune("trace_blocks_per_cu", b_)
-        tune = {"trace_width": w_, "trace_lds_entries": l_, "trace_blocks_per_cu": b_, "fixed": True}
+        names = ["trace_width", "trace_lds_entries", "trace_blocks_per_cu", "trace_block", "shade_tile"]
+        vals = [int(x) for x in args.knobs.split(",")]
+        tune = {"fixed": True}
+        for k, v in zip(names, vals):
+            ctx.tune(k, v)
+            tune[k] = v
     elif not args.no_autotune:
-        res = scene.autotune(img.data_ptr(), W, H, S, DEPTH, RR, SEED, reps=2)
+        res = scene.autotune(img.data_ptr(), W, H, S, DEPTH, RR, SEED, reps=3)
         tune = {"trace_width": res.best_width, "trace_lds_entries": res.best_lds,
-                "trace_blocks_per_cu": res.best_bpc,
-                "trials": [[res.trial_width[i], res.trial_lds[i], res.trial_bpc[i], round(res.trial_ms[i], 3)]
-                           for i in range(res.trials)]}
+                "trace_blocks_per_cu": res.best_bpc, "trace_block": res.best_block, "shade_tile": res.best_tile,
+                "reps": res.reps,
+                "trials": [[res.trial_width[i], res.trial_lds[i], res.trial_bpc[i], res.trial_block[i],
+                            res.trial_tile[i], round(res.trial_ms[i], 3)] for i in range(res.trials)]}
         log(f"autotune: width {res.best_width}, lds {res.best_lds}, blocks/CU {res.best_bpc}, "
-            f"{res.best_ms:.2f} ms; trials (width, lds, bpc, ms): {tune['trials']}")
+            f"block {res.best_block}, tile {res.best_tile}, {res.best_ms:.2f} ms; "
+            f"trials (width, lds, bpc, block, tile, ms): {tune['trials']}")
     for i in range(args.warmup):
         st = scene.render(img.data_ptr(), W, H, S, DEPTH, RR, SEED)
         log(f"warmup {i}: {st.segments} segments, {st.render_ms:.1f} ms")

@@ -75,7 +75,12 @@ int32_t     mrh_profile_get(mrh_context* ctx, const char* kernel, double* total_
  *   "shade_partition"     material partition of a bounce: 0 = one global
  *                         radix pass over the HitKey batch byte (the rays are
  *                         shaded through the sorted order), 1 = each block
- *                         partitions its own tile of 1024 rays in LDS
+ *                         partitions its own tile of rays in LDS
+ *   "trace_block"         threads per block of the frame renders' traversal
+ *                         kernel (128, 256, 512; two-level scenes and the
+ *                         serving engine use 256)
+ *   "shade_tile"          rays per tile of shade_partition 1 (512, 1024,
+ *                         2048)
  *   "fail_alloc_after"    failure-handling tests: k > 0 makes the k-th
  *                         following device allocation of this context fail
  *                         with MRH_ERR_OOM (0 = off)
@@ -498,11 +503,13 @@ int32_t mrh_render_guided(mrh_context* ctx, mrh_scene* scene, const mrh_camera_d
                           const mrh_render_params* params, const mrh_guide_params* guide, mrh_svo* svo,
                           float* d_image, mrh_render_stats* stats);
 
-/* Autotuner (no counterpart in the reference): times `reps` renders of the
- * given frame per candidate launch shape — traversal tree width, then
- * traversal stack entries in LDS, then blocks per CU of the traversal grid —
- * and leaves the fastest set on the context (mrh_tune).  d_image is scratch
- * for the trial frames. */
+/* Autotuner (no counterpart in the reference): coordinate descent over the
+ * launch shape of frame renders — traversal tree width, traversal stack
+ * entries in LDS, blocks per CU of the traversal grid, traversal block size,
+ * rays per partition tile, each axis swept with the others at their best so
+ * far.  Each candidate renders the given frame `reps` times (at least 3) and
+ * scores the median; the lowest is left set on the context (mrh_tune) and
+ * every trial is listed.  d_image is scratch for the trial frames. */
 typedef struct {
     uint32_t best_lds, best_bpc;
     double   best_ms;
@@ -511,6 +518,9 @@ typedef struct {
     double   trial_ms[32];
     uint32_t best_width;
     uint32_t trial_width[32];
+    uint32_t best_block, best_tile;
+    uint32_t trial_block[32], trial_tile[32];
+    uint32_t reps;               /* renders per candidate actually used */
 } mrh_tune_result;
 int32_t mrh_autotune(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
                      const mrh_render_params* params, float* d_image, uint32_t reps,

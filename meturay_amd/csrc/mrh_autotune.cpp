@@ -1,9 +1,12 @@
-// mrh_autotune.cpp — picks the traversal launch shape by timing real frames.
+// mrh_autotune.cpp — picks the launch shape of frame renders by timing real
+// frames.
 //
-// Coordinate descent: the traversal tree width first, then the LDS share of
-// the traversal stack (at the current grid size), then the persistent grid
-// size at the best LDS depth.
-// Each candidate renders the caller's frame `reps` times; the fastest median
+// Coordinate descent over five axes, each swept with the others held at
+// their current best: the traversal tree width, the LDS share of the
+// traversal stack, the persistent traversal grid (blocks per CU), the
+// traversal block size, then the rays per tile of the per-block material
+// partition.  Each candidate renders the caller's frame `reps` times (at
+// least 3) and is scored by the median; the lowest median over all trials
 // wins and is left set on the context.
 #include "mrh.h"
 
@@ -25,9 +28,15 @@ int32_t time_frame(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* ca
         t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count());
     }
     std::sort(t.begin(), t.end());
-    *ms = t[t.size() / 2];
+    const size_t m = t.size() / 2;
+    *ms = (t.size() % 2) ? t[m] : 0.5 * (t[m - 1] + t[m]);
     return MRH_OK;
 }
+
+struct Axis {
+    const char* knob;
+    std::vector<int64_t> values;
+};
 
 }  // namespace
 
@@ -36,62 +45,54 @@ extern "C" int32_t mrh_autotune(mrh_context* ctx, mrh_scene* scene, const mrh_ca
                                 mrh_tune_result* out)
 {
     if (!ctx || !scene || !cam || !params || !d_image || !out) return MRH_ERR_INVALID;
-    if (reps == 0) reps = 1;
+    reps = std::max(reps, 3u);
     std::memset(out, 0, sizeof(*out));
-    const int64_t lds_set[] = {0, 8, 12, 16, 24};
-    const int64_t bpc_set[] = {4, 8, 16, 32};
-    const int64_t width_set[] = {2, 4};
-    int64_t best_lds = 16, best_bpc = 8, best_w = 4;
-    mrh_tune_get(ctx, "trace_lds_entries", &best_lds);
-    mrh_tune_get(ctx, "trace_blocks_per_cu", &best_bpc);
-    mrh_tune_get(ctx, "trace_width", &best_w);
-    double best = 1e300;
+    const Axis axes[] = {
+        {"trace_width", {2, 4}},
+        {"trace_lds_entries", {0, 8, 12, 16, 24}},
+        {"trace_blocks_per_cu", {4, 8, 16, 32}},
+        {"trace_block", {128, 256, 512}},
+        {"shade_tile", {512, 1024, 2048}},
+    };
+    constexpr int NAX = sizeof(axes) / sizeof(axes[0]);
+    int64_t cur[NAX];
+    for (int a = 0; a < NAX; a++) {
+        int32_t rc = mrh_tune_get(ctx, axes[a].knob, &cur[a]);
+        if (rc != MRH_OK) return rc;
+    }
     // warm the scratch slots once
-    double ms = 0.0;
+    double ms = 0.0, best = 1e300;
     int32_t rc = time_frame(ctx, scene, cam, params, d_image, 1, &ms);
     if (rc != MRH_OK) return rc;
-    for (int64_t w : width_set) {
-        if ((rc = mrh_tune(ctx, "trace_width", w)) != MRH_OK) return rc;
-        if ((rc = time_frame(ctx, scene, cam, params, d_image, reps, &ms)) != MRH_OK) return rc;
-        if (out->trials < 32) {
-            out->trial_width[out->trials] = (uint32_t)w;
-            out->trial_lds[out->trials] = (uint32_t)best_lds;
-            out->trial_bpc[out->trials] = (uint32_t)best_bpc;
-            out->trial_ms[out->trials] = ms;
-            out->trials++;
+    for (int a = 0; a < NAX; a++) {
+        int64_t pick = cur[a];
+        for (int64_t v : axes[a].values) {
+            if ((rc = mrh_tune(ctx, axes[a].knob, v)) != MRH_OK) return rc;
+            if ((rc = time_frame(ctx, scene, cam, params, d_image, reps, &ms)) != MRH_OK) return rc;
+            if (out->trials < 32) {
+                const uint32_t k = out->trials++;
+                cur[a] = v;
+                out->trial_width[k] = (uint32_t)cur[0];
+                out->trial_lds[k] = (uint32_t)cur[1];
+                out->trial_bpc[k] = (uint32_t)cur[2];
+                out->trial_block[k] = (uint32_t)cur[3];
+                out->trial_tile[k] = (uint32_t)cur[4];
+                out->trial_ms[k] = ms;
+            }
+            if (ms < best) {
+                best = ms;
+                pick = v;
+            }
         }
-        if (ms < best) { best = ms; best_w = w; }
+        cur[a] = pick;
+        if ((rc = mrh_tune(ctx, axes[a].knob, pick)) != MRH_OK) return rc;
     }
-    mrh_tune(ctx, "trace_width", best_w);
-    for (int64_t lds : lds_set) {
-        if ((rc = mrh_tune(ctx, "trace_lds_entries", lds)) != MRH_OK) return rc;
-        if ((rc = time_frame(ctx, scene, cam, params, d_image, reps, &ms)) != MRH_OK) return rc;
-        if (out->trials < 32) {
-            out->trial_width[out->trials] = (uint32_t)best_w;
-            out->trial_lds[out->trials] = (uint32_t)lds;
-            out->trial_bpc[out->trials] = (uint32_t)best_bpc;
-            out->trial_ms[out->trials] = ms;
-            out->trials++;
-        }
-        if (ms < best) { best = ms; best_lds = lds; }
-    }
-    mrh_tune(ctx, "trace_lds_entries", best_lds);
-    for (int64_t bpc : bpc_set) {
-        if ((rc = mrh_tune(ctx, "trace_blocks_per_cu", bpc)) != MRH_OK) return rc;
-        if ((rc = time_frame(ctx, scene, cam, params, d_image, reps, &ms)) != MRH_OK) return rc;
-        if (out->trials < 32) {
-            out->trial_width[out->trials] = (uint32_t)best_w;
-            out->trial_lds[out->trials] = (uint32_t)best_lds;
-            out->trial_bpc[out->trials] = (uint32_t)bpc;
-            out->trial_ms[out->trials] = ms;
-            out->trials++;
-        }
-        if (ms < best) { best = ms; best_bpc = bpc; }
-    }
-    mrh_tune(ctx, "trace_blocks_per_cu", best_bpc);
-    out->best_width = (uint32_t)best_w;
-    out->best_lds = (uint32_t)best_lds;
-    out->best_bpc = (uint32_t)best_bpc;
+    out->best_width = (uint32_t)cur[0];
+    out->best_lds = (uint32_t)cur[1];
+    out->best_bpc = (uint32_t)cur[2];
+    out->best_block = (uint32_t)cur[3];
+    out->best_tile = (uint32_t)cur[4];
     out->best_ms = best;
+    out->reps = reps;
     return MRH_OK;
 }

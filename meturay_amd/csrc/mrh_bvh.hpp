@@ -55,10 +55,10 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src);
 void lbvh_free(mrh_lbvh* b);
 
 // Persistent grid of the traversal kernels: a multiple of 8 blocks (one
-// slice per XCD) of 256 threads.
-inline uint32_t trace_grid(mrh_context* ctx, uint64_t rays)
+// slice per XCD) of `block` threads.
+inline uint32_t trace_grid(mrh_context* ctx, uint64_t rays, uint32_t block = 256)
 {
-    uint64_t want = (rays + 255) / 256;
+    uint64_t want = (rays + block - 1) / block;
     uint64_t cap = (uint64_t)ctx->num_cus * ctx->trace_blocks_per_cu;
     if (want > cap) want = cap;
     want = (want + 7) & ~7ull;
@@ -66,12 +66,13 @@ inline uint32_t trace_grid(mrh_context* ctx, uint64_t rays)
 }
 
 // Overflow buffer for stack entries beyond the LDS part (per grid thread).
-inline int32_t trace_overflow(mrh_context* ctx, uint32_t grid, uint32_t cap, uint32_t lds, uint2** out)
+inline int32_t trace_overflow(mrh_context* ctx, uint32_t grid, uint32_t cap, uint32_t lds, uint2** out,
+                              uint32_t block = 256)
 {
     *out = nullptr;
     if (lds == 0 || cap <= lds) return MRH_OK;
     void* p = nullptr;
-    MRH_TRY(scratch(ctx, SLOT_TRACE_OVERFLOW, (size_t)(cap - lds) * grid * 256 * sizeof(uint2), &p));
+    MRH_TRY(scratch(ctx, SLOT_TRACE_OVERFLOW, (size_t)(cap - lds) * grid * block * sizeof(uint2), &p));
     *out = (uint2*)p;
     return MRH_OK;
 }

@@ -220,6 +220,18 @@ int32_t mrh_tune(mrh_context* ctx, const char* knob, int64_t value)
         ctx->shade_partition = (uint32_t)value;
         return MRH_OK;
     }
+    if (k == "trace_block") {
+        if (value != 128 && value != 256 && value != 512)
+            return fail(ctx, MRH_ERR_INVALID, "trace_block must be 128, 256 or 512");
+        ctx->trace_block = (uint32_t)value;
+        return MRH_OK;
+    }
+    if (k == "shade_tile") {
+        if (value != 512 && value != 1024 && value != 2048)
+            return fail(ctx, MRH_ERR_INVALID, "shade_tile must be 512, 1024 or 2048");
+        ctx->shade_tile = (uint32_t)value;
+        return MRH_OK;
+    }
     if (k == "fail_alloc_after") {
         if (value < 0) return fail(ctx, MRH_ERR_INVALID, "fail_alloc_after must be >= 0");
         ctx->fail_alloc_after = (uint64_t)value;
@@ -236,6 +248,8 @@ int32_t mrh_tune_get(mrh_context* ctx, const char* knob, int64_t* value)
     else if (k == "trace_blocks_per_cu") *value = ctx->trace_blocks_per_cu;
     else if (k == "trace_width") *value = ctx->trace_width;
     else if (k == "shade_partition") *value = ctx->shade_partition;
+    else if (k == "trace_block") *value = ctx->trace_block;
+    else if (k == "shade_tile") *value = ctx->shade_tile;
     else if (k == "fail_alloc_after") *value = (int64_t)ctx->fail_alloc_after;
     else return fail(ctx, MRH_ERR_INVALID, "unknown knob %s", knob);
     return MRH_OK;

@@ -58,6 +58,10 @@ struct mrh_context {
     uint32_t trace_width = 4;
     // material partition of a bounce: 0 = global radix pass, 1 = per-block tiles
     uint32_t shade_partition = 1;
+    // threads per block of the frame renders' traversal kernel (128, 256, 512)
+    uint32_t trace_block = 256;
+    // rays per tile of the per-block partition (512, 1024, 2048)
+    uint32_t shade_tile = 1024;
     // failure-handling tests: when k > 0, the k-th following dmalloc on this
     // context fails with MRH_ERR_OOM (knob "fail_alloc_after"; 0 = off)
     uint64_t fail_alloc_after = 0;

@@ -49,15 +49,15 @@ __global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h,
 // [66, 68) u64 triangle tests
 constexpr uint32_t COUNTS_WORDS = 128;
 
-template <int W, int CAP, bool STATS, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_trace_pt(const float4* __restrict__ tnodes,
+template <int W, int CAP, bool STATS, int L, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_trace_pt(const float4* __restrict__ tnodes,
                                                   const float4* __restrict__ leaves, uint32_t root, uint32_t n_leaves,
                                                   PathBuf in, uint32_t* __restrict__ counts,
                                                   uint32_t bounce, float tmin, float tmax, HitBuf hit,
                                                   uint2* __restrict__ overflow)
 {
-    __shared__ uint32_t s_node[L > 0 ? L * 256 : 1];
-    __shared__ float s_t[L > 0 ? L * 256 : 1];
+    __shared__ uint32_t s_node[L > 0 ? L * BS : 1];
+    __shared__ float s_t[L > 0 ? L * BS : 1];
     const uint32_t n = counts[bounce];
     const uint32_t lb = xcd_block();
     const size_t ostride = (size_t)gridDim.x * blockDim.x;
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         const V3 d = v3(in.dx[i], in.dy[i], in.dz[i]);
         TraceResult r;
         if constexpr (L > 0) {
-            LdsStack<L, CAP, 256> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
+            LdsStack<L, CAP, BS> st{s_node, s_t, threadIdx.x, overflow, ostride, oslot};
             r = traverse_w<W, false, STATS>(st, tnodes, leaves, root, n_leaves, o, d, tmin, tmax);
         } else {
             PrivateStack<CAP> st;
@@ -103,22 +103,31 @@ int32_t launch_trace_pt_w(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, co
 {
     const uint32_t root = ts.root;
     const uint32_t lds = ctx->trace_lds_entries;
+    const uint32_t bs = ctx->trace_block;
     uint2* ovf = nullptr;
-    MRH_TRY(trace_overflow(ctx, grid, CAP, lds, &ovf));
+    MRH_TRY(trace_overflow(ctx, grid, CAP, lds, &ovf, bs));
     const char* name = STATS ? "trace_pt_stats" : "trace_pt";
     const float4* tn = ts.nodes;
     const float4* lv = (const float4*)b->leaves;
-#define MRH_TRACE_PT_L(LV)                                                                            \
-    return launch(ctx, name, dim3(grid), dim3(256), 0, k_trace_pt<W, CAP, STATS, LV>, tn, lv, root, b->n, \
+#define MRH_TRACE_PT_L(LV, BS)                                                                            \
+    return launch(ctx, name, dim3(grid), dim3(BS), 0, k_trace_pt<W, CAP, STATS, LV, BS>, tn, lv, root, b->n, \
                   in, counts, bounce, tmin, tmax, hit, ovf)
-    switch (lds) {
-    case 0: MRH_TRACE_PT_L(0);
-    case 8: MRH_TRACE_PT_L(8);
-    case 12: MRH_TRACE_PT_L(12);
-    case 16: MRH_TRACE_PT_L(16);
-    case 24: MRH_TRACE_PT_L(24);
-    default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24");
+#define MRH_TRACE_PT_B(BS)                                                                   \
+    switch (lds) {                                                                           \
+    case 0: MRH_TRACE_PT_L(0, BS);                                                           \
+    case 8: MRH_TRACE_PT_L(8, BS);                                                           \
+    case 12: MRH_TRACE_PT_L(12, BS);                                                         \
+    case 16: MRH_TRACE_PT_L(16, BS);                                                         \
+    case 24: MRH_TRACE_PT_L(24, BS);                                                         \
+    default: return fail(ctx, MRH_ERR_INVALID, "trace_lds_entries must be 0, 8, 12, 16 or 24"); \
     }
+    switch (bs) {
+    case 128: MRH_TRACE_PT_B(128);
+    case 256: MRH_TRACE_PT_B(256);
+    case 512: MRH_TRACE_PT_B(512);
+    default: return fail(ctx, MRH_ERR_INVALID, "trace_block must be 128, 256 or 512");
+    }
+#undef MRH_TRACE_PT_B
 #undef MRH_TRACE_PT_L
 }
 
@@ -270,14 +279,13 @@ __global__ __launch_bounds__(256) void k_shade(PathBuf in, PathBuf out, HitBuf h
 
 
 // The same shade step with the partition done per block: a block takes a
-// tile of SHADE_TILE consecutive rays (coalesced key loads), counts them per
-// HitKey batch in LDS, lays the tile's ray indices out batch by batch and
-// shades them in that order, so a wave nearly always runs one material while
-// all ray-state lines of the tile are fetched by the one block, once.  No
-// global sort, no second pass over the keys.
-constexpr uint32_t SHADE_TILE = 1024;
-
-template <bool LEARN, bool GUIDE, bool NEE>
+// tile of SHADE_TILE consecutive rays (coalesced key loads; 512, 1024 or
+// 2048, the "shade_tile" knob), counts them per HitKey batch in LDS, lays
+// the tile's ray indices out batch by batch and shades them in that order,
+// so a wave nearly always runs one material while all ray-state lines of the
+// tile are fetched by the one block, once.  No global sort, no second pass
+// over the keys.
+template <bool LEARN, bool GUIDE, bool NEE, uint32_t SHADE_TILE>
 __global__ __launch_bounds__(256) void k_shade_tiles(PathBuf in, PathBuf out, HitBuf hit, uint32_t* counts,
                                                      uint32_t bounce, ShadeReq q, GuideArgs ga)
 {
@@ -599,7 +607,7 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     const float tmin = (bounce == 0) ? f.cam.near_d : 0.0f;
     const float tmax = (bounce == 0) ? f.cam.far_d : FLT_MAX;
     const uint32_t grid = stride_grid(ctx, f.cap, 256);
-    const uint32_t tgrid = trace_grid(ctx, f.cap);
+    const uint32_t tgrid = trace_grid(ctx, f.cap, s->two_level ? 256u : ctx->trace_block);
     const float4* lv = s->two_level ? (const float4*)s->leaves : (const float4*)b->leaves;
     HitBuf hb = f.hit;
     if (s->two_level) {
@@ -644,22 +652,30 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
         return launch(ctx, "shade", dim3(grid), dim3(256), 0, kernel, in, outb, hb, (const uint32_t*)f.order,
                       f.counts, bounce, q, f.ga);
     };
-    const uint32_t tgrid_s = stride_grid(ctx, (f.cap + SHADE_TILE - 1) / SHADE_TILE * 256, 256);
-    auto shade_t = [&](auto kernel) {
+    auto shade_t = [&](auto kernel, uint32_t tile) {
+        const uint32_t tgrid_s = stride_grid(ctx, (f.cap + tile - 1) / tile * 256, 256);
         return launch(ctx, "shade", dim3(tgrid_s), dim3(256), 0, kernel, in, outb, hb, f.counts, bounce, q, f.ga);
     };
     const uint32_t v = (f.learn ? 1u : 0u) | (f.guide ? 2u : 0u) | (f.nee ? 4u : 0u);
     if (tiles) {
-        switch (v) {
-        case 0: MRH_TRY(shade_t(k_shade_tiles<false, false, false>)); break;
-        case 1: MRH_TRY(shade_t(k_shade_tiles<true, false, false>)); break;
-        case 2: MRH_TRY(shade_t(k_shade_tiles<false, true, false>)); break;
-        case 3: MRH_TRY(shade_t(k_shade_tiles<true, true, false>)); break;
-        case 4: MRH_TRY(shade_t(k_shade_tiles<false, false, true>)); break;
-        case 5: MRH_TRY(shade_t(k_shade_tiles<true, false, true>)); break;
-        case 6: MRH_TRY(shade_t(k_shade_tiles<false, true, true>)); break;
-        default: MRH_TRY(shade_t(k_shade_tiles<true, true, true>)); break;
+#define MRH_SHADE_TILES(T)                                                                     \
+    switch (v) {                                                                               \
+    case 0: MRH_TRY(shade_t(k_shade_tiles<false, false, false, T>, T)); break;                 \
+    case 1: MRH_TRY(shade_t(k_shade_tiles<true, false, false, T>, T)); break;                  \
+    case 2: MRH_TRY(shade_t(k_shade_tiles<false, true, false, T>, T)); break;                  \
+    case 3: MRH_TRY(shade_t(k_shade_tiles<true, true, false, T>, T)); break;                   \
+    case 4: MRH_TRY(shade_t(k_shade_tiles<false, false, true, T>, T)); break;                  \
+    case 5: MRH_TRY(shade_t(k_shade_tiles<true, false, true, T>, T)); break;                   \
+    case 6: MRH_TRY(shade_t(k_shade_tiles<false, true, true, T>, T)); break;                   \
+    default: MRH_TRY(shade_t(k_shade_tiles<true, true, true, T>, T)); break;                   \
+    }
+        switch (ctx->shade_tile) {
+        case 512: MRH_SHADE_TILES(512u); break;
+        case 1024: MRH_SHADE_TILES(1024u); break;
+        case 2048: MRH_SHADE_TILES(2048u); break;
+        default: return fail(ctx, MRH_ERR_INVALID, "shade_tile must be 512, 1024 or 2048");
         }
+#undef MRH_SHADE_TILES
     } else {
         switch (v) {
         case 0: MRH_TRY(shade(k_shade<false, false, false>)); break;

@@ -64,7 +64,9 @@ class RenderStats(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("best_lds", _U32), ("best_bpc", _U32), ("best_ms", C.c_double), ("trials", _U32),
                 ("trial_lds", _U32 * 32), ("trial_bpc", _U32 * 32), ("trial_ms", C.c_double * 32),
-                ("best_width", _U32), ("trial_width", _U32 * 32)]
+                ("best_width", _U32), ("trial_width", _U32 * 32),
+                ("best_block", _U32), ("best_tile", _U32), ("trial_block", _U32 * 32), ("trial_tile", _U32 * 32),
+                ("reps", _U32)]
 
 
 class FilterDesc(C.Structure):
@@ -638,7 +640,7 @@ class Scene:
                                                C.byref(st)))
         return st
 
-    def autotune(self, d_image, width, height, samples, max_depth, rr_start, seed=0, reps=2,
+    def autotune(self, d_image, width, height, samples, max_depth, rr_start, seed=0, reps=3,
                  camera=None) -> TuneResult:
         cam = camera_desc(camera or self.camera)
         p = RenderParams(width, height, samples, max_depth, rr_start, 0, seed, 0)

@@ -475,6 +475,28 @@ int32_t mrh_svo_sample(mrh_context* ctx, const mrh_svo* svo, const uint32_t* d_n
 int32_t mrh_svo_pdf(mrh_context* ctx, const mrh_svo* svo, const uint32_t* d_nodes,
                     const float* d_normals, const float* d_dirs, uint32_t n, float* d_pdf);
 int32_t mrh_svo_view_get(const mrh_svo* svo, mrh_svo_view* out);
+/* Ray binning by octree node.  Replaces WFPGTracer::AccumulateRayHitsToSVO
+ * (WFPGTracer.cu:488-588: leaf of each ray's hit, sort, run-length encode,
+ * leaf ray counts), AnisoSVOctreeCPU::CollapseRayCounts (AnisoSVO.cu:265-350,
+ * 1485-1529), KCCheckReducedSVOBins (WFPGTracerKC.cuh:1055-1078) and
+ * RayCaster::PartitionRaysWRTCustomData (the partition of the rays by bin).
+ * d_codes: cell codes (mrh_svo_codes at the octree's depth) of the rays'
+ * hit points, 0xFFFFFFFF for rays that take no bin (next-event rays,
+ * misses).  Each ray starts at the deepest existing node of its cell; from
+ * the leaf level up to min_level (clamped to the depth, "MinRayBinLevel") a
+ * node is marked when it is at min_level or holds at least bin_ray_count
+ * rays ("BinRayCount", 1..MRH_BIN_RAY_COUNT_MAX), and hands its rays to its
+ * parent otherwise.  d_bins[i] = the first marked node up from ray i's
+ * start node (the start node if none is marked), 0xFFFFFFFF for rays
+ * without a code.  With d_order: the ray ids sorted by bin (ascending node,
+ * stable), binned rays first; d_part_offsets[k] / d_part_bins[k] = first
+ * sorted position and node of the k-th bin, d_part_offsets[count] = the
+ * number of binned rays, *d_part_count = count (device word; n + 1, n and 1
+ * words).  Everything stays on the device. */
+#define MRH_BIN_RAY_COUNT_MAX 4096u
+int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, uint32_t n, uint32_t min_level,
+                         uint32_t bin_ray_count, uint32_t* d_bins, uint32_t* d_order, uint32_t* d_part_offsets,
+                         uint32_t* d_part_bins, uint32_t* d_part_count);
 
 /* Guided path tracing: replaces WFPGTracer::Render / Finalize
  * (WFPGTracer.cu:1590-1790) and the WFPG path work (WFPGTracerKC.cuh:202-520,

@@ -40,6 +40,9 @@ struct mrh_svo {
     void* work_b = nullptr;
     size_t work_b_bytes = 0;
     uint32_t* counter = nullptr;     // [16]
+    // grow-only workspace of mrh_svo_bin_rays
+    void* bin_work = nullptr;
+    size_t bin_work_bytes = 0;
 };
 
 namespace mrh {
@@ -561,7 +564,7 @@ void svo_free(mrh_svo* s)
     if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
     void* ptrs[] = {s->codes, s->first_child, s->child_mask, s->parent, s->sum,    s->acc,
                     s->cnt,   s->radiance,    s->pack,       s->node_count, s->work_a, s->work_b,
-                    s->counter};
+                    s->counter, s->bin_work};
     for (void* p : ptrs) (void)hipFree(p);
     delete s;
 }
@@ -601,12 +604,206 @@ __global__ void k_svo_pdf(SvoView s, const uint32_t* __restrict__ nodes, const f
     }
 }
 
+// ---------------------------------------------------------------- ray bins
+// Start node of each ray (the deepest existing node of its cell), and the
+// (key, ray) pairs of the first sort; rays without a code get key `skip`.
+__global__ void k_bin_start(SvoView s, const uint32_t* __restrict__ codes, uint32_t n, uint32_t skip,
+                            uint32_t* __restrict__ start, uint32_t* __restrict__ key, uint32_t* __restrict__ val)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t c = codes[i];
+        const uint32_t v = (c == SVO_NO_NODE) ? skip : svo_lookup(s, c);
+        start[i] = v;
+        key[i] = v;
+        val[i] = i;
+    }
+}
+
+// Run-length encoding of the sorted start nodes: first position of each run
+__global__ void k_bin_heads(const uint32_t* __restrict__ skey, uint32_t n, uint32_t skip, uint32_t* __restrict__ first)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t k = skey[p];
+        if (k != skip && (p == 0 || skey[p - 1] != k)) first[k] = p;
+    }
+}
+
+// ... then its length, the node's own ray count
+__global__ void k_bin_own(const uint32_t* __restrict__ skey, uint32_t n, uint32_t skip,
+                          const uint32_t* __restrict__ first, uint32_t* __restrict__ own)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t k = skey[p];
+        if (k != skip && (p == n - 1 || skey[p + 1] != k)) own[k] = p + 1 - first[k];
+    }
+}
+
+constexpr uint32_t BIN_MARK = 1u << 31;
+
+// One level of the collapse (KCCollapseRayCounts / KCCollapseRayCountsLeaf,
+// AnisoSVO.cu:265-350), as a gather: a node's rays are its own and those its
+// unmarked children handed up; it is marked at min_level or with at least
+// min_rays rays.  tot[v] = rays | BIN_MARK.  Integer sums: no atomics, no
+// order dependence.
+__global__ void k_bin_collapse(uint32_t off, uint32_t count, uint32_t at_min, uint32_t min_rays,
+                               const uint32_t* __restrict__ own, const uint32_t* __restrict__ first_child,
+                               const uint32_t* __restrict__ child_mask, uint32_t* __restrict__ tot)
+{
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x) {
+        const uint32_t v = off + t;
+        uint32_t total = own[v];
+        const uint32_t fc = first_child[v];
+        if (fc != SVO_NO_NODE) {
+            const uint32_t nc = (uint32_t)__popc(child_mask[v]);
+            for (uint32_t k = 0; k < nc; k++) {
+                const uint32_t c = tot[fc + k];
+                if (!(c & BIN_MARK)) total += c;
+            }
+        }
+        const bool mark = at_min || total >= min_rays;
+        tot[v] = (total & ~BIN_MARK) | (mark ? BIN_MARK : 0u);
+    }
+}
+
+// Each ray climbs from its start node to the first marked node
+// (AnisoSVOctreeGPU::FindMarkedBin, AnisoSVO.cuh:1244-1262); keys of the
+// partition sort.
+__global__ void k_bin_assign(const uint32_t* __restrict__ start, uint32_t n, uint32_t skip,
+                             const uint32_t* __restrict__ tot, const uint32_t* __restrict__ parent,
+                             uint32_t* __restrict__ bins, uint32_t* __restrict__ key, uint32_t* __restrict__ val)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t v = start[i];
+        uint32_t b = SVO_NO_NODE;
+        if (v != skip) {
+            uint32_t u = v;
+            while (!(tot[u] & BIN_MARK) && parent[u] != SVO_NO_NODE) u = parent[u];
+            b = (tot[u] & BIN_MARK) ? u : v;
+        }
+        bins[i] = b;
+        if (key) {
+            key[i] = (b == SVO_NO_NODE) ? skip : b;
+            val[i] = i;
+        }
+    }
+}
+
+__global__ void k_bin_flags(const uint32_t* __restrict__ skey, uint32_t n, uint32_t skip, uint32_t* __restrict__ flags)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+        flags[p] = (skey[p] != skip && (p == 0 || skey[p - 1] != skey[p])) ? 1u : 0u;
+}
+
+// Partition runs: first position and node of each bin, the end of the
+// binned rays and the run count
+__global__ void k_bin_runs(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ flags,
+                           const uint32_t* __restrict__ pos, uint32_t n, uint32_t skip,
+                           uint32_t* __restrict__ offsets, uint32_t* __restrict__ part_bins,
+                           uint32_t* __restrict__ part_count)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t k = skey[p];
+        if (k == skip) continue;
+        if (flags[p]) {
+            if (offsets) offsets[pos[p]] = p;
+            if (part_bins) part_bins[pos[p]] = k;
+        }
+        if (p == n - 1 || skey[p + 1] == skip) {
+            const uint32_t runs = pos[p] + flags[p];
+            if (offsets) offsets[runs] = p + 1;
+            if (part_count) *part_count = runs;
+        }
+    }
+}
+
+int32_t svo_bin_rays(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, uint32_t n, uint32_t min_level,
+                     uint32_t min_rays, uint32_t* bins, uint32_t* order, uint32_t* offsets, uint32_t* part_bins,
+                     uint32_t* part_count)
+{
+    const uint32_t N = s->nodes;
+    const uint32_t skip = N;                    // sorts after every node
+    const uint32_t bits = 32u - (uint32_t)__builtin_clz(N);   // holds 0..N
+    if (min_level > s->depth) min_level = s->depth;   // WFPGTracer.cu:612-613
+    // workspace: start, key, val, skey, sval [n]; own, first, tot [N]; flags, pos [n]; scan tmp
+    const size_t tmp_words = scan_tmp_words(n ? n : 1);
+    const size_t words = 7 * (size_t)n + 3 * (size_t)N + tmp_words + 64;
+    MRH_TRY(grow(ctx, &s->bin_work, &s->bin_work_bytes, words * 4));
+    uint32_t* w = (uint32_t*)s->bin_work;
+    uint32_t* start = w;
+    uint32_t* key = start + n;
+    uint32_t* val = key + n;
+    uint32_t* skey = val + n;
+    uint32_t* sval = skey + n;
+    uint32_t* flags = sval + n;
+    uint32_t* pos = flags + n;
+    uint32_t* own = pos + n;
+    uint32_t* first = own + N;
+    uint32_t* tot = first + N;
+    uint32_t* tmp = tot + N;
+    if (order) {
+        if (part_count) MRH_TRY(fill_u32(ctx, part_count, 0u, 1));
+        if (offsets) MRH_TRY(fill_u32(ctx, offsets, 0u, 1));
+    }
+    if (n == 0) return MRH_OK;
+    const uint32_t g = stride_grid(ctx, n, 256);
+    const SvoView view = svo_view(s);
+    MRH_TRY(launch(ctx, "bin_start", dim3(g), dim3(256), 0, k_bin_start, view, codes, n, skip, start, key, val));
+    SortArgs sa;
+    sa.keys_in = key;
+    sa.vals_in = val;
+    sa.keys_out = skey;
+    sa.vals_out = sval;
+    sa.n_max = n;
+    sa.key_bytes = 4;
+    sa.begin_bit = 0;
+    sa.end_bit = bits;
+    MRH_TRY(radix_sort(ctx, sa));
+    MRH_TRY(fill_u32(ctx, own, 0u, N));
+    MRH_TRY(fill_u32(ctx, tot, 0u, N));
+    MRH_TRY(launch(ctx, "bin_heads", dim3(g), dim3(256), 0, k_bin_heads, (const uint32_t*)skey, n, skip, first));
+    MRH_TRY(launch(ctx, "bin_own", dim3(g), dim3(256), 0, k_bin_own, (const uint32_t*)skey, n, skip,
+                   (const uint32_t*)first, own));
+    for (int32_t l = (int32_t)s->depth; l >= (int32_t)min_level; l--) {
+        const uint32_t cnt = s->lvl_count[l];
+        if (!cnt) continue;
+        MRH_TRY(launch(ctx, "bin_collapse", dim3(stride_grid(ctx, cnt, 256)), dim3(256), 0, k_bin_collapse,
+                       s->lvl_off[l], cnt, (uint32_t)((uint32_t)l == min_level), min_rays, (const uint32_t*)own,
+                       (const uint32_t*)s->first_child, (const uint32_t*)s->child_mask, tot));
+    }
+    MRH_TRY(launch(ctx, "bin_assign", dim3(g), dim3(256), 0, k_bin_assign, (const uint32_t*)start, n, skip,
+                   (const uint32_t*)tot, (const uint32_t*)s->parent, bins, order ? key : nullptr, val));
+    if (!order) return MRH_OK;
+    sa.vals_out = order;
+    MRH_TRY(radix_sort(ctx, sa));
+    MRH_TRY(launch(ctx, "bin_flags", dim3(g), dim3(256), 0, k_bin_flags, (const uint32_t*)skey, n, skip, flags));
+    MRH_TRY(scan_exclusive(ctx, flags, pos, n, tmp));
+    return launch(ctx, "bin_runs", dim3(g), dim3(256), 0, k_bin_runs, (const uint32_t*)skey, (const uint32_t*)flags,
+                  (const uint32_t*)pos, n, skip, offsets, part_bins, part_count);
+}
+
 }  // namespace
 }  // namespace mrh
 
 using namespace mrh;
 
 extern "C" {
+
+int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, uint32_t n, uint32_t min_level,
+                         uint32_t bin_ray_count, uint32_t* d_bins, uint32_t* d_order, uint32_t* d_part_offsets,
+                         uint32_t* d_part_bins, uint32_t* d_part_count)
+{
+    if (!ctx || !svo || svo->ctx != ctx) return MRH_ERR_INVALID;
+    if (!svo->nodes) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_bin_rays: the octree is not built");
+    if (bin_ray_count < 1 || bin_ray_count > MRH_BIN_RAY_COUNT_MAX)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_bin_rays: bin_ray_count must be in [1, %u]",
+                    (unsigned)MRH_BIN_RAY_COUNT_MAX);
+    if (n >= (1u << 30)) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_bin_rays: at most 2^30 - 1 rays");
+    if (n && (!d_codes || !d_bins)) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_bin_rays: null codes or bins");
+    if (!d_order && (d_part_offsets || d_part_bins || d_part_count))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_bin_rays: the partition needs d_order");
+    return svo_bin_rays(ctx, svo, d_codes, n, min_level, bin_ray_count, d_bins, d_order, d_part_offsets,
+                        d_part_bins, d_part_count);
+}
 
 int32_t mrh_svo_create(mrh_context* ctx, mrh_svo** out)
 {

@@ -50,9 +50,10 @@ struct mrh_context {
     std::vector<hipEvent_t> free_events;
     std::map<std::string, std::pair<double, uint64_t>> totals;
     // tuning knobs (mrh_tune): traversal stack entries kept in LDS per
-    // thread (0 = whole stack in private scratch) and blocks per CU of the
-    // persistent trace grid
-    uint32_t trace_lds_entries = 16;
+    // thread (0 = whole stack in private scratch; 8 is what the autotuner
+    // picks on MI355X for 1080p frames of a 2 M-triangle scene) and blocks
+    // per CU of the persistent trace grid
+    uint32_t trace_lds_entries = 8;
     uint32_t trace_blocks_per_cu = 8;
     // traversal tree: 2 = binary LBVH, 4 = 4-wide tree collapsed from it
     uint32_t trace_width = 4;

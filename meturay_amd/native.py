@@ -199,6 +199,7 @@ _SIGS = {
     "mrh_svo_sample": (_I32, [_P, _P, _P, _P, _P, _U32, _P, _P]),
     "mrh_svo_pdf": (_I32, [_P, _P, _P, _P, _P, _U32, _P]),
     "mrh_svo_view_get": (_I32, [_P, C.POINTER(SvoView)]),
+    "mrh_svo_bin_rays": (_I32, [_P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
     "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                  C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
     "mrh_server_create": (_I32, [_P, C.POINTER(ServerParams), C.POINTER(_P)]),
@@ -686,6 +687,13 @@ class Svo:
 
     def lookup(self, d_codes, n: int, d_nodes, min_count: int = 0) -> None:
         self.ctx.check(lib().mrh_svo_lookup(self.ctx.h, self.h, ptr(d_codes), n, min_count, ptr(d_nodes)))
+
+    def bin_rays(self, d_codes, n: int, min_level: int, bin_ray_count: int, d_bins, d_order=None,
+                 d_part_offsets=None, d_part_bins=None, d_part_count=None) -> None:
+        """mrh_svo_bin_rays: each ray's bin node and, with d_order, the rays partitioned by bin."""
+        self.ctx.check(lib().mrh_svo_bin_rays(self.ctx.h, self.h, ptr(d_codes), n, min_level, bin_ray_count,
+                                              ptr(d_bins), ptr(d_order), ptr(d_part_offsets), ptr(d_part_bins),
+                                              ptr(d_part_count)))
 
     def sample(self, d_nodes, d_normals, d_xi, n: int, d_dirs, d_pdf) -> None:
         self.ctx.check(lib().mrh_svo_sample(self.ctx.h, self.h, ptr(d_nodes), ptr(d_normals), ptr(d_xi), n,

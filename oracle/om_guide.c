@@ -148,6 +148,83 @@ uint32_t om_svo_lookup_guide(const om_svo* s, uint32_t code, uint32_t min_count)
     return node;
 }
 
+/* Ray binning (WFPGTracer::AccumulateRayHitsToSVO, WFPGTracer.cu:488-588;
+ * AnisoSVOctreeCPU::CollapseRayCounts + KCCollapseRayCounts(Leaf),
+ * AnisoSVO.cu:265-350, 1485-1529; KCCheckReducedSVOBins +
+ * AnisoSVOctreeGPU::FindMarkedBin, WFPGTracerKC.cuh:1055-1078,
+ * AnisoSVO.cuh:1244-1262; RayCaster::PartitionRaysWRTCustomData).
+ * Each ray starts at the deepest existing node of its cell (the reference's
+ * leaf); the ray counts of the nodes are collapsed bottom-up from the leaf
+ * level to min_level: a node at min_level is marked, a node with at least
+ * min_rays rays is marked, any other node hands its rays to its parent.  A
+ * ray's bin is the first marked node on the way up from its start node (its
+ * start node when none is marked: a start node above min_level).  Rays with
+ * code OM_SVO_NO_NODE get no bin.  order: ray ids stably sorted by bin,
+ * binned rays first; part_offsets[k] / part_bins[k] the first sorted
+ * position and node of bin run k, part_offsets[count] the end of the binned
+ * rays. */
+void om_svo_bin_rays(const om_svo* s, const uint32_t* codes, uint32_t n, uint32_t min_level, uint32_t min_rays,
+                     uint32_t* bins, uint32_t* order, uint32_t* part_offsets, uint32_t* part_bins,
+                     uint32_t* part_count)
+{
+    const uint32_t N = s->nodes;
+    uint32_t* start = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t* cnt = (uint32_t*)calloc(N ? N : 1, sizeof(uint32_t));
+    uint8_t* marked = (uint8_t*)calloc(N ? N : 1, 1);
+    if (min_level > s->depth) min_level = s->depth;   /* WFPGTracer.cu:612-613 */
+    /* start nodes and their ray counts (sort + run-length encode) */
+    for (uint32_t i = 0; i < n; i++) {
+        start[i] = (codes[i] == OM_SVO_NO_NODE) ? OM_SVO_NO_NODE : om_svo_lookup(s, codes[i]);
+        if (start[i] != OM_SVO_NO_NODE) cnt[start[i]]++;
+    }
+    /* collapse, leaf level first */
+    for (int32_t l = (int32_t)s->depth; l >= (int32_t)min_level; l--) {
+        for (uint32_t k = 0; k < s->level_count[l]; k++) {
+            const uint32_t v = s->level_offset[l] + k;
+            if ((uint32_t)l == min_level) {
+                marked[v] = 1;
+                continue;
+            }
+            if (cnt[v] < min_rays) {
+                if (s->parent[v] != OM_SVO_NO_NODE) cnt[s->parent[v]] += cnt[v];
+            } else {
+                marked[v] = 1;
+            }
+        }
+    }
+    /* each ray climbs to its first marked node */
+    for (uint32_t i = 0; i < n; i++) {
+        uint32_t v = start[i];
+        if (v != OM_SVO_NO_NODE) {
+            uint32_t u = v;
+            while (!marked[u] && s->parent[u] != OM_SVO_NO_NODE) u = s->parent[u];
+            if (!marked[u]) u = v;
+            v = u;
+        }
+        bins[i] = v;
+    }
+    /* partition by bin: stable counting sort, binned rays first */
+    if (order) {
+        uint32_t* first = (uint32_t*)calloc((size_t)N + 2, sizeof(uint32_t));
+        for (uint32_t i = 0; i < n; i++) first[(bins[i] == OM_SVO_NO_NODE ? N : bins[i]) + 1]++;
+        for (uint32_t v = 0; v <= N; v++) first[v + 1] += first[v];
+        uint32_t runs = 0;
+        for (uint32_t v = 0; v < N; v++)
+            if (first[v + 1] > first[v]) {
+                if (part_offsets) part_offsets[runs] = first[v];
+                if (part_bins) part_bins[runs] = v;
+                runs++;
+            }
+        if (part_offsets) part_offsets[runs] = first[N];
+        if (part_count) *part_count = runs;
+        for (uint32_t i = 0; i < n; i++) order[first[bins[i] == OM_SVO_NO_NODE ? N : bins[i]]++] = i;
+        free(first);
+    }
+    free(start);
+    free(cnt);
+    free(marked);
+}
+
 void om_svo_splat(om_svo* s, const uint32_t* codes, const uint32_t* bins, const float* values, uint32_t n)
 {
     for (uint32_t i = 0; i < n; i++) {

@@ -289,6 +289,11 @@ uint32_t om_svo_lookup_guide(const om_svo* s, uint32_t code, uint32_t min_count)
 void     om_svo_splat(om_svo* s, const uint32_t* codes, const uint32_t* bins, const float* values,
                       uint32_t n);
 void     om_svo_filter(om_svo* s);
+/* ray binning by octree node (BinRayCount collapse) and the partition of
+ * the rays by bin; order / part_* may be NULL */
+void     om_svo_bin_rays(const om_svo* s, const uint32_t* codes, uint32_t n, uint32_t min_level, uint32_t min_rays,
+                         uint32_t* bins, uint32_t* order, uint32_t* part_offsets, uint32_t* part_bins,
+                         uint32_t* part_count);
 /* pts / dirs / normals / xi: n x 3 floats; uv: n x 2 */
 void     om_svo_codes(const float* pts, uint32_t n, const float cube[4], uint32_t depth, uint32_t* codes);
 void     om_dir_to_bins(const float* dirs, uint32_t n, uint32_t* bins);

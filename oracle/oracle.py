@@ -187,6 +187,8 @@ def lib():
         L.om_svo_lookup_guide.argtypes = [C.POINTER(OmSvo), _U32, _U32]
         L.om_svo_splat.argtypes = [C.POINTER(OmSvo), _P, _P, _P, _U32]
         L.om_svo_filter.argtypes = [C.POINTER(OmSvo)]
+        L.om_svo_bin_rays.argtypes = [C.POINTER(OmSvo), _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]
+        L.om_svo_bin_rays.restype = None
         L.om_svo_codes.argtypes = [_P, _U32, _P, _U32, _P]
         L.om_dir_to_bins.argtypes = [_P, _U32, _P]
         L.om_octa_to_dirs.argtypes = [_P, _U32, _P]
@@ -745,6 +747,20 @@ class Svo:
     def filter(self):
         lib().om_svo_filter(C.byref(self._s))
         self.refresh()
+
+    def bin_rays(self, codes, min_level: int, min_rays: int):
+        """om_svo_bin_rays: (bins, order, part_offsets[count + 1], part_bins[count])."""
+        codes = np.ascontiguousarray(codes, dtype=np.uint32)
+        n = len(codes)
+        bins = np.zeros(n, np.uint32)
+        order = np.zeros(n, np.uint32)
+        offsets = np.zeros(n + 1, np.uint32)
+        pbins = np.zeros(max(n, 1), np.uint32)
+        count = np.zeros(1, np.uint32)
+        lib().om_svo_bin_rays(C.byref(self._s), _ptr(codes), n, min_level, min_rays, _ptr(bins), _ptr(order),
+                              _ptr(offsets), _ptr(pbins), _ptr(count))
+        k = int(count[0])
+        return bins, order, offsets[:k + 1].copy(), pbins[:k].copy()
 
     def sample(self, nodes, normals, xi):
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)

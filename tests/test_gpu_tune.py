@@ -91,7 +91,7 @@ def test_autotune_leaves_a_valid_choice(ctx, knobs):
     # the chosen shape renders the default shape's frame
     seg, got = render(scene, img)
     for kk in SETS:
-        ctx.tune(kk, {"trace_width": 4, "trace_lds_entries": 16, "trace_blocks_per_cu": 8, "trace_block": 256,
+        ctx.tune(kk, {"trace_width": 4, "trace_lds_entries": 8, "trace_blocks_per_cu": 8, "trace_block": 256,
                       "shade_tile": 1024}[kk])
     seg0, ref = render(scene, img)
     assert seg == seg0 and np.allclose(got, ref, rtol=1e-5, atol=1e-6)

@@ -28,7 +28,7 @@ struct mrh_lbvh {
     uint32_t* right = nullptr;      // [n-1]
     uint32_t* parent = nullptr;     // [2n-1]
     float* node_aabb = nullptr;     // [2n-1][6]
-    uint32_t* ranges = nullptr;     // [n-1][2]
+    uint32_t* halves = nullptr;     // [n-1][16] build handshake: each child's box, node and range end
     uint32_t* counters = nullptr;   // [n-1]
     float4* tnodes = nullptr;       // [n-1][4]
     float4* leaves = nullptr;       // [n][3]

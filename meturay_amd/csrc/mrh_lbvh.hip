@@ -11,7 +11,8 @@
 //                    continues, so hierarchy, AABB refit and the traversal
 //                    records are written in a single pass with no relaunch
 //                    loop and no host round trip.
-//   k_lbvh_depth   : deepest leaf, bounds the traversal stack.
+//   k_bvh_depth_collapse : node depths -> the deepest leaf (bounds the
+//                    traversal stack) and the 4-wide traversal nodes.
 #include "mrh_bvh.hpp"
 
 #include <cstring>
@@ -163,159 +164,248 @@ __device__ __forceinline__ uint32_t child_code(uint32_t node, uint32_t n)
     return (node < n) ? (node | LEAF_BIT) : (node - n);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// 32 coherent bytes: two 16-byte stores / loads that go through to memory
+// past this XCD's L2 (cache policy sc1, what agent-scope atomic stores and
+// loads of a word use), so the thread that reads a half-record on another
+// XCD sees it once the writer's counter increment is visible.  The loads
+// wait for their data inside the asm block.
+__device__ __forceinline__ void st_half(uint32_t* p, u32x4 a, u32x4 b)
+{
+    asm volatile("global_store_dwordx4 %0, %1, off sc1\n\t"
+                 "global_store_dwordx4 %0, %2, off offset:16 sc1"
+                 :
+                 : "v"(p), "v"(a), "v"(b)
+                 : "memory");
+}
+
+__device__ __forceinline__ void ld_half(const uint32_t* p, u32x4& a, u32x4& b)
+{
+    asm volatile("global_load_dwordx4 %0, %2, off sc1\n\t"
+                 "global_load_dwordx4 %1, %2, off offset:16 sc1\n\t"
+                 "s_waitcnt vmcnt(0)"
+                 : "=&v"(a), "=&v"(b)
+                 : "v"(p)
+                 : "memory");
+}
+
+// One thread per sorted leaf walks up (Apetrei 2014) and the second child
+// to reach a parent continues.  A child reaching parent p publishes one
+// 32-byte half of p's 64-byte record (left child: words 0-7, right child:
+// 8-15): its box, its node index and its range end (l for a left child, r
+// for a right one), then increments p's counter.  The second arrival reads
+// its sibling's half and has everything p needs: both children, both boxes
+// and p's range.
+//
+// A block owns 256 consecutive leaves [lo, hi].  A node whose range lies
+// inside them is reached only by the block's own threads, so its record and
+// counter live in LDS; the others use the global record and counter, with
+// stores and loads that go through to memory.  Node p (split between p and
+// p + 1) has range [l, r] with l - 1 the last j < p and r the first j > p
+// where delta(j) < delta(p) (delta 0 past either end), so p is local when
+// such j exist in [lo - 1, p - 1] and in [p + 1, hi]: a prefix and a suffix
+// minimum of the block's deltas.  The arrays the view reads (links,
+// parents, boxes, binary traversal nodes) are plain stores, read only after
+// the kernel.
+constexpr uint32_t EMIT_BLOCK = 256;
+
 template <class CodeT>
-__global__ __launch_bounds__(256) void k_lbvh_emit(
+__global__ __launch_bounds__(EMIT_BLOCK) void k_lbvh_emit(
     uint32_t n, const CodeT* __restrict__ scodes, const uint32_t* __restrict__ sidx,
     const float* __restrict__ leaf_aabb, const float* __restrict__ tris,
-    const uint32_t* __restrict__ prim_key, uint32_t* left, uint32_t* right, uint32_t* parent,
-    float* node_aabb, uint32_t* ranges, uint32_t* counters, float4* __restrict__ tnodes,
-    float4* __restrict__ leaves, uint32_t* meta)
+    const uint32_t* __restrict__ prim_key, uint32_t* __restrict__ left, uint32_t* __restrict__ right,
+    uint32_t* __restrict__ parent, float* __restrict__ node_aabb, uint32_t* halves, uint32_t* counters,
+    float4* __restrict__ tnodes, float4* __restrict__ leaves, uint32_t* meta)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t prim = sidx[i];
-        float box[6];
+    __shared__ uint32_t s_delta[EMIT_BLOCK + 1];   // delta(lo - 1 + k)
+    __shared__ uint32_t s_suf[EMIT_BLOCK];         // min of s_delta[k .. EMIT_BLOCK]
+    __shared__ uint32_t s_wave[8];
+    __shared__ uint32_t s_local[EMIT_BLOCK];
+    __shared__ uint32_t s_cnt[EMIT_BLOCK];
+    __shared__ u32x4 s_rec[EMIT_BLOCK * 4];        // [node - lo][4 x 16 B]
+    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const uint32_t lo = blockIdx.x * EMIT_BLOCK;
+    const uint32_t hi = min(lo + EMIT_BLOCK - 1u, n - 1u);
+    auto delta_or0 = [&](uint32_t k) -> uint32_t {   // delta(lo - 1 + k), 0 past the ends
+        if (k == 0 && lo == 0) return 0u;
+        const uint32_t j = lo - 1u + k;
+        return (j + 1u >= n) ? 0u : delta_at(scodes, j);
+    };
+    s_delta[t] = delta_or0(t);
+    if (t == 0) s_delta[EMIT_BLOCK] = delta_or0(EMIT_BLOCK);
+    s_cnt[t] = 0u;
+    __syncthreads();
+    // inclusive prefix minimum of s_delta[0 .. t]
+    uint32_t pm = s_delta[t];
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)pm, off, 64);
+        if (lane >= off) pm = min(pm, y);
+    }
+    // suffix minimum of s_delta[t .. EMIT_BLOCK - 1]
+    uint32_t sm = s_delta[t];
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_down((int)sm, off, 64);
+        if (lane + off < 64) sm = min(sm, y);
+    }
+    if (lane == 63) s_wave[wave] = pm;
+    if (lane == 0) s_wave[4 + wave] = sm;
+    __syncthreads();
+    for (uint32_t w = 0; w < wave; w++) pm = min(pm, s_wave[w]);
+    for (uint32_t w = wave + 1; w < EMIT_BLOCK / 64; w++) sm = min(sm, s_wave[4 + w]);
+    s_suf[t] = min(sm, s_delta[EMIT_BLOCK]);
+    __syncthreads();
+    {
+        // node p = lo + t: local when p < hi and both nearest smaller deltas are in the block
+        const uint32_t dp = s_delta[t + 1u];
+        const uint32_t right_min = (t + 2u < EMIT_BLOCK) ? s_suf[t + 2u]
+                                 : (t + 2u == EMIT_BLOCK ? s_delta[EMIT_BLOCK] : 0xFFFFFFFFu);
+        s_local[t] = (lo + t < hi && pm < dp && right_min < dp) ? 1u : 0u;
+    }
+    __syncthreads();
+    const uint32_t i = lo + t;
+    if (i >= n) return;
+    const uint32_t prim = sidx[i];
+    float box[6];
 #pragma unroll
-        for (int a = 0; a < 6; a++) box[a] = leaf_aabb[(size_t)prim * 6 + a];
-        // leaf records
-        {
-            const float* t = tris + (size_t)prim * 9;
-            const uint32_t key = prim_key ? prim_key[prim] : 0u;
-            leaves[3 * (size_t)i + 0] = make_float4(t[0], t[1], t[2], __uint_as_float(prim));
-            leaves[3 * (size_t)i + 1] = make_float4(t[3], t[4], t[5], __uint_as_float(key));
-            leaves[3 * (size_t)i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
-        }
+    for (int a = 0; a < 6; a++) box[a] = leaf_aabb[(size_t)prim * 6 + a];
+    // leaf records
+    {
+        const float* tr = tris + (size_t)prim * 9;
+        const uint32_t key = prim_key ? prim_key[prim] : 0u;
+        leaves[3 * (size_t)i + 0] = make_float4(tr[0], tr[1], tr[2], __uint_as_float(prim));
+        leaves[3 * (size_t)i + 1] = make_float4(tr[3], tr[4], tr[5], __uint_as_float(key));
+        leaves[3 * (size_t)i + 2] = make_float4(tr[6], tr[7], tr[8], 0.0f);
+    }
 #pragma unroll
-        for (int a = 0; a < 6; a++) st_agent((uint32_t*)&node_aabb[(size_t)i * 6 + a], __float_as_uint(box[a]));
+    for (int a = 0; a < 6; a++) node_aabb[(size_t)i * 6 + a] = box[a];
 
-        uint32_t node = i, l = i, r = i;
-        // a walk is at most tree-depth steps (< 100 for 30/63-bit codes with
-        // the index tie-break); the bound only matters for a corrupt hierarchy
-        for (uint32_t step = 0;; step++) {
-            if (step == 256u) {
-                atomicOr(&meta[8], 4u);
-                break;
-            }
-            if (l == 0 && r == n - 1) {
-                st_agent(&parent[node], 0xFFFFFFFFu);
-                meta[0] = node;
-                break;
-            }
-            // ChooseParentAndUpdate (LinearBVH.hpp:43-100)
-            const uint32_t ld = (l == 0) ? 0u : delta_at(scodes, l - 1);
-            const uint32_t rd = (r == n - 1) ? 0u : delta_at(scodes, r);
-            uint32_t p;
-            if (rd > ld) {
-                p = r;
-                st_agent(&left[p], node);
-                st_agent(&ranges[2 * (size_t)p + 0], l);
-            } else {
-                p = l - 1;
-                st_agent(&right[p], node);
-                st_agent(&ranges[2 * (size_t)p + 1], r);
-            }
-            st_agent(&parent[node], n + p);
-            // Every value another thread reads here (child links, ranges,
-            // parent, boxes) is stored and loaded with agent-scope atomics,
-            // which the device keeps coherent across XCDs without flushing
-            // caches.  So the handshake only needs this thread's stores to
-            // have completed before its counter increment.  A __threadfence()
-            // would also write back the whole L2 twice per step.  The
-            // signal fences keep the compiler from moving the stores or the
-            // counter across the wait (s_waitcnt alone is no compiler
-            // barrier, and relaxed atomics on different addresses may be
-            // reordered); the wait keeps the hardware from doing so.  An
-            // agent-scope release/acquire pair would order the same accesses
-            // but costs an L2 write-back per step on this chip.
+    uint32_t node = i, l = i, r = i;
+    // a walk is at most tree-depth steps (< 100 for 30/63-bit codes with
+    // the index tie-break); the bound only matters for a corrupt hierarchy
+    for (uint32_t step = 0;; step++) {
+        if (step == 256u) {
+            atomicOr(&meta[8], 4u);
+            break;
+        }
+        if (l == 0 && r == n - 1) {
+            parent[node] = 0xFFFFFFFFu;
+            meta[0] = node;
+            break;
+        }
+        // ChooseParentAndUpdate (LinearBVH.hpp:43-100)
+        const uint32_t ld = (l == 0) ? 0u : (l >= lo ? s_delta[l - lo] : delta_at(scodes, l - 1));
+        const uint32_t rd = (r == n - 1) ? 0u : (r <= hi ? s_delta[r - lo + 1u] : delta_at(scodes, r));
+        const bool is_left = rd > ld;
+        const uint32_t p = is_left ? r : l - 1;
+        parent[node] = n + p;
+        const u32x4 ha_out = u32x4{__float_as_uint(box[0]), __float_as_uint(box[1]), __float_as_uint(box[2]),
+                                   __float_as_uint(box[3])};
+        const u32x4 hb_out = u32x4{__float_as_uint(box[4]), __float_as_uint(box[5]), node, is_left ? l : r};
+        u32x4 ha, hb;
+        const bool local = p >= lo && p < hi && s_local[p - lo];
+        if (local) {
+            // LDS handshake: the half is written before the counter
+            // increment, and LDS serves one wave's requests in order, so a
+            // sibling that sees the increment reads the half
+            u32x4* rec = s_rec + 4 * (p - lo);
+            rec[is_left ? 0 : 2] = ha_out;
+            rec[is_left ? 1 : 3] = hb_out;
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            const uint32_t old = __hip_atomic_fetch_add(&s_cnt[p - lo], 1u, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (old == 0u) break;   // sibling not done yet: it will continue
+            ha = rec[is_left ? 2 : 0];
+            hb = rec[is_left ? 3 : 1];
+        } else {
+            uint32_t* rec = halves + 16 * (size_t)p;
+            st_half(rec + (is_left ? 0 : 8), ha_out, hb_out);
+            // The half-record went through to memory (sc1) and the counter
+            // is an agent-scope atomic, so the handshake only needs the
+            // stores to have completed before the increment.  A
+            // __threadfence() would also write back the whole L2 twice per
+            // step; an agent-scope release/acquire pair costs an L2
+            // write-back per step on this chip.  The signal fences keep the
+            // compiler from moving memory operations across the wait
+            // (s_waitcnt alone is no compiler barrier); the wait keeps the
+            // hardware from doing so.
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             __builtin_amdgcn_s_waitcnt(0);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             const uint32_t old = __hip_atomic_fetch_add(&counters[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             if (old == 0u) break;   // sibling not done yet: it will continue
-            // second arrival: both children and both range ends are published
-            const uint32_t lc = ld_agent(&left[p]);
-            const uint32_t rc = ld_agent(&right[p]);
-            const uint32_t sib = (lc == node) ? rc : lc;
-            if (sib >= 2 * n - 1) {   // never taken on a correct build; reported to the host
-                atomicOr(&meta[8], 1u);
-                break;
-            }
-            float sbox[6];
+            ld_half(rec + (is_left ? 8 : 0), ha, hb);
+        }
+        // second arrival: the sibling's half holds its box, node and end
+        const uint32_t sib = hb.z;
+        if (sib >= 2 * n - 1) {   // never taken on a correct build; reported to the host
+            atomicOr(&meta[8], 1u);
+            break;
+        }
+        const float sbox[6] = {__uint_as_float(ha.x), __uint_as_float(ha.y), __uint_as_float(ha.z),
+                               __uint_as_float(ha.w), __uint_as_float(hb.x), __uint_as_float(hb.y)};
+        if (is_left) r = hb.w;
+        else l = hb.w;
+        if (l > r || r >= n) {
+            atomicOr(&meta[8], 2u);
+            break;
+        }
+        const uint32_t lc = is_left ? node : sib;
+        const uint32_t rc = is_left ? sib : node;
+        left[p] = lc;
+        right[p] = rc;
+        const float* lb = is_left ? box : sbox;
+        const float* rb = is_left ? sbox : box;
+        float4* tn = tnodes + 4 * (size_t)p;
+        tn[0] = make_float4(lb[0], lb[1], lb[2], lb[3]);
+        tn[1] = make_float4(lb[4], lb[5], rb[0], rb[1]);
+        tn[2] = make_float4(rb[2], rb[3], rb[4], rb[5]);
+        tn[3] = make_float4(__uint_as_float(child_code(lc, n)), __uint_as_float(child_code(rc, n)), 0.0f, 0.0f);
+        float ub[6];
 #pragma unroll
-            for (int a = 0; a < 6; a++) sbox[a] = __uint_as_float(ld_agent((const uint32_t*)&node_aabb[(size_t)sib * 6 + a]));
-            const float* lb = (lc == node) ? box : sbox;
-            const float* rb = (lc == node) ? sbox : box;
-            float4* tn = tnodes + 4 * (size_t)p;
-            tn[0] = make_float4(lb[0], lb[1], lb[2], lb[3]);
-            tn[1] = make_float4(lb[4], lb[5], rb[0], rb[1]);
-            tn[2] = make_float4(rb[2], rb[3], rb[4], rb[5]);
-            tn[3] = make_float4(__uint_as_float(child_code(lc, n)), __uint_as_float(child_code(rc, n)),
-                                0.0f, 0.0f);
-            float ub[6];
+        for (int a = 0; a < 3; a++) {
+            ub[a] = fminf(lb[a], rb[a]);
+            ub[3 + a] = fmaxf(lb[3 + a], rb[3 + a]);
+        }
+        node = n + p;
 #pragma unroll
-            for (int a = 0; a < 3; a++) {
-                ub[a] = fminf(lb[a], rb[a]);
-                ub[3 + a] = fmaxf(lb[3 + a], rb[3 + a]);
-            }
-            node = n + p;
-#pragma unroll
-            for (int a = 0; a < 6; a++) {
-                box[a] = ub[a];
-                st_agent((uint32_t*)&node_aabb[(size_t)node * 6 + a], __float_as_uint(ub[a]));
-            }
-            l = ld_agent(&ranges[2 * (size_t)p + 0]);
-            r = ld_agent(&ranges[2 * (size_t)p + 1]);
-            if (l > r || r >= n) {
-                atomicOr(&meta[8], 2u);
-                break;
-            }
+        for (int a = 0; a < 6; a++) {
+            box[a] = ub[a];
+            node_aabb[(size_t)node * 6 + a] = ub[a];
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_lbvh_depth(uint32_t n, const uint32_t* __restrict__ parent,
-                                                    uint32_t* __restrict__ meta)
+// Depth of every internal node (a walk to the root), then both uses of it:
+// the deepest leaf (a child of a node at depth d is at d + 1; one atomic per
+// block when it raises meta[1]) bounds the traversal stack, and every node
+// at even depth (root = 0) becomes a 4-wide traversal node gathering its
+// children, an internal child replaced by that child's two children, from
+// the binary nodes' child boxes (layout in traverse4).
+__global__ __launch_bounds__(256) void k_bvh_depth_collapse(uint32_t n, const float4* __restrict__ tnodes,
+                                                            const uint32_t* __restrict__ parent,
+                                                            float4* __restrict__ nodes4, uint32_t* __restrict__ meta)
 {
     uint32_t best = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t d = 0, x = i;
-        while (parent[x] != 0xFFFFFFFFu && d < 4096u) {
-            x = parent[x];
-            d++;
-        }
-        best = max(best, d);
-    }
-    // block maximum, then one atomic per block when it raises meta[1]
-    __shared__ uint32_t part[4];
-    for (int off = 32; off > 0; off >>= 1) best = max(best, (uint32_t)__shfl_xor((int)best, off, 64));
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < blockDim.x / 64; w++) best = max(best, part[w]);
-        if (best > ld_agent(&meta[1])) atomicMax(&meta[1], best);
-    }
-}
-
-// 4-wide traversal nodes: every internal node at even depth (root = 0)
-// gathers its children, an internal child replaced by that child's two
-// children, from the binary nodes' child boxes (layout in traverse4).
-__global__ __launch_bounds__(256) void k_bvh4_collapse(uint32_t n, const float4* __restrict__ tnodes,
-                                                       const uint32_t* __restrict__ parent,
-                                                       float4* __restrict__ nodes4)
-{
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p + 1 < n; p += gridDim.x * blockDim.x) {
         uint32_t d = 0, x = n + p;
         while (parent[x] != 0xFFFFFFFFu && d < 4096u) {
             x = parent[x];
             d++;
         }
+        const float4* tn = tnodes + 4 * (size_t)p;
+        const float4 a3 = tn[3];
+        const uint32_t code[2] = {__float_as_uint(a3.x), __float_as_uint(a3.y)};
+        if ((code[0] | code[1]) & LEAF_BIT) best = max(best, d + 1u);
         if (d & 1u) continue;
+        const float4 a0 = tn[0], a1 = tn[1], a2 = tn[2];
         float* q = (float*)(nodes4 + 8 * (size_t)p);
         uint32_t k = 0;
-        const float4* tn = tnodes + 4 * (size_t)p;
-        const float4 a0 = tn[0], a1 = tn[1], a2 = tn[2], a3 = tn[3];
         const float box[2][6] = {{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, {a1.z, a1.w, a2.x, a2.y, a2.z, a2.w}};
-        const uint32_t code[2] = {__float_as_uint(a3.x), __float_as_uint(a3.y)};
         for (int c = 0; c < 2; c++) {
             if (code[c] & LEAF_BIT) {
                 for (int a = 0; a < 6; a++) q[a * 4 + k] = box[c][a];
@@ -341,6 +431,14 @@ __global__ __launch_bounds__(256) void k_bvh4_collapse(uint32_t n, const float4*
             q[24 + k] = __uint_as_float(0xFFFFFFFFu);
         }
         for (int j = 28; j < 32; j++) q[j] = 0.0f;
+    }
+    __shared__ uint32_t part[4];
+    for (int off = 32; off > 0; off >>= 1) best = max(best, (uint32_t)__shfl_xor((int)best, off, 64));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / 64; w++) best = max(best, part[w]);
+        if (best > ld_agent(&meta[1])) atomicMax(&meta[1], best);
     }
 }
 
@@ -402,23 +500,22 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src)
     MRH_TRY(radix_sort(ctx, sa));
     if (n > 1) MRH_TRY(fill_u32(ctx, b->counters, 0u, n - 1));
     const uint32_t grid = stride_grid(ctx, n, 256);
+    const uint32_t egrid = div_up(n, EMIT_BLOCK);   // one thread per leaf
     if (b->bits == 30)
-        MRH_TRY(launch(ctx, "lbvh_emit", dim3(grid), dim3(256), 0, k_lbvh_emit<uint32_t>, n,
+        MRH_TRY(launch(ctx, "lbvh_emit", dim3(egrid), dim3(EMIT_BLOCK), 0, k_lbvh_emit<uint32_t>, n,
                        (const uint32_t*)b->sorted_codes, (const uint32_t*)b->sorted_idx,
                        (const float*)b->leaf_aabb, (const float*)b->tris,
                        (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
-                       b->ranges, b->counters, b->tnodes, b->leaves, b->meta));
+                       b->halves, b->counters, b->tnodes, b->leaves, b->meta));
     else
-        MRH_TRY(launch(ctx, "lbvh_emit", dim3(grid), dim3(256), 0, k_lbvh_emit<uint64_t>, n,
+        MRH_TRY(launch(ctx, "lbvh_emit", dim3(egrid), dim3(EMIT_BLOCK), 0, k_lbvh_emit<uint64_t>, n,
                        (const uint64_t*)b->sorted_codes, (const uint32_t*)b->sorted_idx,
                        (const float*)b->leaf_aabb, (const float*)b->tris,
                        (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
-                       b->ranges, b->counters, b->tnodes, b->leaves, b->meta));
-    MRH_TRY(launch(ctx, "lbvh_depth", dim3(grid), dim3(256), 0, k_lbvh_depth, n,
-                   (const uint32_t*)b->parent, b->meta));
+                       b->halves, b->counters, b->tnodes, b->leaves, b->meta));
     if (n > 1)
-        MRH_TRY(launch(ctx, "bvh4_collapse", dim3(grid), dim3(256), 0, k_bvh4_collapse, n,
-                       (const float4*)b->tnodes, (const uint32_t*)b->parent, b->nodes4));
+        MRH_TRY(launch(ctx, "bvh_depth_collapse", dim3(grid), dim3(256), 0, k_bvh_depth_collapse, n,
+                       (const float4*)b->tnodes, (const uint32_t*)b->parent, b->nodes4, b->meta));
     uint32_t host_meta[9] = {};
     MRH_HIP_CHECK(ctx, hipMemcpyAsync(host_meta, b->meta, sizeof(host_meta), hipMemcpyDeviceToHost,
                                       ctx->stream));
@@ -464,7 +561,7 @@ int32_t lbvh_create(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t 
         if ((rc = alloc_into(b, &b->right, ni)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->parent, 2 * (size_t)n - 1)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->node_aabb, (2 * (size_t)n - 1) * 6)) != MRH_OK) break;
-        if ((rc = alloc_into(b, &b->ranges, 2 * ni)) != MRH_OK) break;
+        if ((rc = alloc_into(b, &b->halves, 16 * ni)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->counters, ni)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->tnodes, 4 * ni)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->nodes4, 8 * ni)) != MRH_OK) break;

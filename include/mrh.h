@@ -172,13 +172,14 @@ typedef struct {
     uint32_t        reserved;
 } mrh_lbvh_view;
 
-/* d_tris is copied; the caller may free it after the call returns.
- * Synchronous (returns after the build). */
+/* d_tris is read during the call only; the caller may free it after the
+ * call returns.  Synchronous (returns after the build). */
 int32_t mrh_lbvh_build(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t bits,
                        mrh_lbvh** out);
 int32_t mrh_lbvh_view_get(const mrh_lbvh* bvh, mrh_lbvh_view* out);
 int32_t mrh_lbvh_destroy(mrh_lbvh* bvh);
-/* Rebuilds in place from new corners of the same count (timed builds). */
+/* Rebuilds in place from new corners of the same count (timed builds);
+ * d_tris as for mrh_lbvh_build (NULL is invalid). */
 int32_t mrh_lbvh_rebuild(mrh_context* ctx, mrh_lbvh* bvh, const float* d_tris);
 
 /* ------------------------------------------------------------------ */

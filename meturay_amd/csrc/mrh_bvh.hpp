@@ -19,7 +19,6 @@
 struct mrh_lbvh {
     mrh_context* ctx = nullptr;
     uint32_t n = 0, bits = 30, root = 0, max_depth = 0;
-    float* tris = nullptr;          // [n][9] primitive order
     float* leaf_aabb = nullptr;     // [n][6] primitive order
     void* codes = nullptr;          // unsorted codes
     void* sorted_codes = nullptr;
@@ -51,7 +50,7 @@ int32_t morton_codes(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t
 // the leaf records.
 int32_t lbvh_create(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t bits,
                     const uint32_t* d_prim_key, mrh_lbvh** out);
-int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src);
+int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris);
 void lbvh_free(mrh_lbvh* b);
 
 // Persistent grid of the traversal kernels: a multiple of 8 blocks (one

@@ -30,6 +30,7 @@ enum MrhSlot : int {
     SLOT_GUIDE_REC,
     SLOT_FILTER,
     SLOT_SAMPLES,
+    SLOT_BOUNDS_PART,
     SLOT_COUNT
 };
 

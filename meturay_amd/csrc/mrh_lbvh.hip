@@ -1,8 +1,9 @@
 // mrh_lbvh.hip — Morton codes and the one-pass LBVH builder.
 //
-// Build = 4 kernels + the onesweep sort:
-//   k_tri_bounds   : triangle AABBs + extent (order-preserving integer
-//                    atomicMin/Max, exact and order independent)
+// Build = 5 kernels + the onesweep sort:
+//   k_tri_bounds   : triangle AABBs + per-block extents, reduced by
+//                    k_extent_reduce (order-preserving integer min / max,
+//                    exact and order independent)
 //   k_morton       : quantised centroids -> 30- or 63-bit codes
 //   radix sort     : (code, primitive) ascending, stable
 //   k_lbvh_emit    : one thread per sorted leaf walks up (Apetrei 2014): the
@@ -51,30 +52,52 @@ __global__ void k_meta_init(uint32_t* __restrict__ meta)
     if (i < 16) meta[i] = (i >= 2 && i < 5) ? 0xFFFFFFFFu : 0u;
 }
 
-// meta[2..4] = encoded extent min, meta[5..7] = encoded extent max
+// meta[2..4] = encoded extent min, meta[5..7] = encoded extent max.  A
+// block stages a tile of 256 triangles through LDS: the corners come in as
+// consecutive words (every load instruction covers 1 KiB of the array) and
+// the boxes leave the same way, instead of 36- and 24-byte strided accesses.
+constexpr uint32_t BOUNDS_TILE = 256;
+
 __global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ tris, uint32_t n,
-                                                    float* __restrict__ aabb, uint32_t* __restrict__ meta)
+                                                    float* __restrict__ aabb, uint32_t* __restrict__ part_out)
 {
+    __shared__ float s_in[BOUNDS_TILE * 9];
+    __shared__ float s_out[BOUNDS_TILE * 6];
     float mn[3] = {INFINITY, INFINITY, INFINITY};
     float mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const float* t = tris + (size_t)i * 9;
-        float b[6];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t base = blockIdx.x * BOUNDS_TILE; base < n; base += gridDim.x * BOUNDS_TILE) {
+        const uint32_t cnt = min(BOUNDS_TILE, n - base);
+        const float* src = tris + (size_t)base * 9;
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            b[a] = fminf(fminf(t[a], t[3 + a]), t[6 + a]);
-            b[3 + a] = fmaxf(fmaxf(t[a], t[3 + a]), t[6 + a]);
-            mn[a] = fminf(mn[a], b[a]);
-            mx[a] = fmaxf(mx[a], b[3 + a]);
+        for (uint32_t j = 0; j < 9; j++) {
+            const uint32_t k = t + j * BOUNDS_TILE;
+            if (k < cnt * 9) s_in[k] = src[k];
         }
-        float* o = aabb + (size_t)i * 6;
+        __syncthreads();
+        if (t < cnt) {
+            const float* c = s_in + 9 * t;
 #pragma unroll
-        for (int a = 0; a < 6; a++) o[a] = b[a];
+            for (int a = 0; a < 3; a++) {
+                const float lo = fminf(fminf(c[a], c[3 + a]), c[6 + a]);
+                const float hi = fmaxf(fmaxf(c[a], c[3 + a]), c[6 + a]);
+                s_out[6 * t + a] = lo;
+                s_out[6 * t + 3 + a] = hi;
+                mn[a] = fminf(mn[a], lo);
+                mx[a] = fmaxf(mx[a], hi);
+            }
+        }
+        __syncthreads();
+        float* dst = aabb + (size_t)base * 6;
+#pragma unroll
+        for (uint32_t j = 0; j < 6; j++) {
+            const uint32_t k = t + j * BOUNDS_TILE;
+            if (k < cnt * 6) dst[k] = s_out[k];
+        }
+        __syncthreads();
     }
-    // Wave, then block reduction; one atomic per block and word, skipped
-    // when the word already holds an equal or better bound.  Atomics on the
-    // six words of one line serialise in L2, so one per wave (16 K waves for
-    // 4 M triangles) cost a millisecond.
+    // wave, then block reduction into this block's six encoded words
+    // (reduced by k_extent_reduce: no atomics on one line from every block)
     __shared__ uint32_t part[4][6];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -96,11 +119,41 @@ __global__ __launch_bounds__(256) void k_tri_bounds(const float* __restrict__ tr
         const uint32_t k = threadIdx.x;
         uint32_t v = part[0][k];
         for (uint32_t w = 1; w < blockDim.x / 64; w++) v = (k < 3) ? min(v, part[w][k]) : max(v, part[w][k]);
-        if (k < 3) {
-            if (v != enc_ordered(INFINITY) && v < ld_agent(&meta[2 + k])) atomicMin(&meta[2 + k], v);
-        } else {
-            if (v != enc_ordered(-INFINITY) && v > ld_agent(&meta[2 + k])) atomicMax(&meta[2 + k], v);
+        part_out[6 * (size_t)blockIdx.x + k] = v;
+    }
+}
+
+// meta[2..4] / meta[5..7] = the encoded extent over the blocks' partials
+// (one block; order-preserving integer min / max, exact)
+__global__ __launch_bounds__(256) void k_extent_reduce(const uint32_t* __restrict__ part, uint32_t blocks,
+                                                       uint32_t* __restrict__ meta)
+{
+    __shared__ uint32_t s[6][4];
+    uint32_t v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = (k < 3) ? 0xFFFFFFFFu : 0u;
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const uint32_t x = part[6 * (size_t)b + k];
+            v[k] = (k < 3) ? min(v[k], x) : max(v[k], x);
         }
+    }
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint32_t y = (uint32_t)__shfl_xor((int)v[k], off, 64);
+            v[k] = (k < 3) ? min(v[k], y) : max(v[k], y);
+        }
+        if (lane == 0) s[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t k = threadIdx.x;
+        uint32_t r = s[k][0];
+        for (uint32_t w = 1; w < blockDim.x / 64; w++) r = (k < 3) ? min(r, s[k][w]) : max(r, s[k][w]);
+        meta[2 + k] = r;
     }
 }
 
@@ -471,9 +524,12 @@ int32_t morton_codes(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t
         MRH_TRY(scratch(ctx, SLOT_MORTON_EXT, 64, &p));
         meta = (uint32_t*)p;
     }
+    const uint32_t bgrid = stride_grid(ctx, n, 256);
+    void* part = nullptr;
+    MRH_TRY(scratch(ctx, SLOT_BOUNDS_PART, (size_t)bgrid * 24, &part));
     MRH_TRY(launch(ctx, "lbvh_meta_init", dim3(1), dim3(64), 0, k_meta_init, meta));
-    MRH_TRY(launch(ctx, "tri_bounds", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_tri_bounds,
-                   d_tris, n, aabb, meta));
+    MRH_TRY(launch(ctx, "tri_bounds", dim3(bgrid), dim3(256), 0, k_tri_bounds, d_tris, n, aabb, (uint32_t*)part));
+    MRH_TRY(launch(ctx, "extent_reduce", dim3(1), dim3(256), 0, k_extent_reduce, (const uint32_t*)part, bgrid, meta));
     if (bits == 30)
         return launch(ctx, "morton", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_morton<uint32_t>,
                       (const float*)aabb, n, (const uint32_t*)meta, 10u, (uint32_t*)codes);
@@ -481,14 +537,14 @@ int32_t morton_codes(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t
                   (const float*)aabb, n, (const uint32_t*)meta, 21u, (uint64_t*)codes);
 }
 
-int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src)
+// The triangles are read during the build only (bounds, codes, leaf
+// records), so they are not copied.
+int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris)
 {
     mrh_context* ctx = b->ctx;
     const uint32_t n = b->n;
-    if (d_tris_src && d_tris_src != b->tris)
-        MRH_HIP_CHECK(ctx, hipMemcpyAsync(b->tris, d_tris_src, (size_t)n * 36,
-                                          hipMemcpyDeviceToDevice, ctx->stream));
-    MRH_TRY(morton_codes(ctx, b->tris, n, b->bits, b->codes, b->leaf_aabb, b->meta));
+    if (!d_tris) return fail(ctx, MRH_ERR_INVALID, "lbvh build: null triangles");
+    MRH_TRY(morton_codes(ctx, d_tris, n, b->bits, b->codes, b->leaf_aabb, b->meta));
     SortArgs sa;
     sa.keys_in = b->codes;
     sa.keys_out = b->sorted_codes;
@@ -504,13 +560,13 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris_src)
     if (b->bits == 30)
         MRH_TRY(launch(ctx, "lbvh_emit", dim3(egrid), dim3(EMIT_BLOCK), 0, k_lbvh_emit<uint32_t>, n,
                        (const uint32_t*)b->sorted_codes, (const uint32_t*)b->sorted_idx,
-                       (const float*)b->leaf_aabb, (const float*)b->tris,
+                       (const float*)b->leaf_aabb, d_tris,
                        (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
                        b->halves, b->counters, b->tnodes, b->leaves, b->meta));
     else
         MRH_TRY(launch(ctx, "lbvh_emit", dim3(egrid), dim3(EMIT_BLOCK), 0, k_lbvh_emit<uint64_t>, n,
                        (const uint64_t*)b->sorted_codes, (const uint32_t*)b->sorted_idx,
-                       (const float*)b->leaf_aabb, (const float*)b->tris,
+                       (const float*)b->leaf_aabb, d_tris,
                        (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
                        b->halves, b->counters, b->tnodes, b->leaves, b->meta));
     if (n > 1)
@@ -552,7 +608,6 @@ int32_t lbvh_create(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t 
     const size_t cb = (bits == 30) ? 4 : 8;
     int32_t rc = MRH_OK;
     do {
-        if ((rc = alloc_into(b, &b->tris, (size_t)n * 9)) != MRH_OK) break;
         if ((rc = alloc_into(b, &b->leaf_aabb, (size_t)n * 6)) != MRH_OK) break;
         if ((rc = alloc_into(b, (char**)&b->codes, (size_t)n * cb)) != MRH_OK) break;
         if ((rc = alloc_into(b, (char**)&b->sorted_codes, (size_t)n * cb)) != MRH_OK) break;

@@ -433,64 +433,108 @@ __global__ __launch_bounds__(EMIT_BLOCK) void k_lbvh_emit(
     }
 }
 
-// Depth of every internal node (a walk to the root), then both uses of it:
-// the deepest leaf (a child of a node at depth d is at d + 1; one atomic per
-// block when it raises meta[1]) bounds the traversal stack, and every node
-// at even depth (root = 0) becomes a 4-wide traversal node gathering its
-// children, an internal child replaced by that child's two children, from
-// the binary nodes' child boxes (layout in traverse4).
-__global__ __launch_bounds__(256) void k_bvh_depth_collapse(uint32_t n, const float4* __restrict__ tnodes,
-                                                            const uint32_t* __restrict__ parent,
-                                                            float4* __restrict__ nodes4, uint32_t* __restrict__ meta)
+// Depth of every internal node, then both uses of it: the deepest leaf (a
+// child of a node at depth d is at d + 1; one atomic per block when it
+// raises meta[1]) bounds the traversal stack, and every node at even depth
+// (root = 0) becomes a 4-wide traversal node gathering its children, an
+// internal child replaced by that child's two children, from the binary
+// nodes' child boxes (layout in traverse4).
+//
+// A block takes 256 consecutive internal nodes.  Those whose parent is
+// among them form a forest in LDS; pointer jumping (8 rounds) gives every
+// node its forest root and the distance to it, and only the forest roots
+// walk the parent links to the tree's root.
+constexpr uint32_t DEPTH_BLOCK = 256;
+
+__global__ __launch_bounds__(DEPTH_BLOCK) void k_bvh_depth_collapse(uint32_t n, const float4* __restrict__ tnodes,
+                                                                    const uint32_t* __restrict__ parent,
+                                                                    float4* __restrict__ nodes4,
+                                                                    uint32_t* __restrict__ meta)
 {
-    uint32_t best = 0;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p + 1 < n; p += gridDim.x * blockDim.x) {
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    __shared__ uint32_t s_jump[DEPTH_BLOCK], s_dist[DEPTH_BLOCK], s_done[DEPTH_BLOCK], s_root_depth[DEPTH_BLOCK];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lo = blockIdx.x * DEPTH_BLOCK;
+    const uint32_t p = lo + t;
+    const bool valid = p + 1 < n;
+    uint32_t up = NONE;   // parent's slot in this block
+    if (valid) {
+        const uint32_t par = parent[n + p];
+        if (par != NONE && par - n - lo < DEPTH_BLOCK) up = par - n - lo;
+    }
+    uint32_t jump = (up != NONE) ? up : t, dist = (up != NONE) ? 1u : 0u, done = (up == NONE) ? 1u : 0u;
+    s_jump[t] = jump;
+    s_dist[t] = dist;
+    s_done[t] = done;
+    __syncthreads();
+    for (int round = 0; round < 8; round++) {
+        if (!done) {
+            dist += s_dist[jump];
+            done = s_done[jump];
+            jump = s_jump[jump];
+        }
+        __syncthreads();
+        s_jump[t] = jump;
+        s_dist[t] = dist;
+        s_done[t] = done;
+        __syncthreads();
+    }
+    // forest roots: depth by walking the parent links
+    if (valid && up == NONE) {
         uint32_t d = 0, x = n + p;
-        while (parent[x] != 0xFFFFFFFFu && d < 4096u) {
+        while (parent[x] != NONE && d < 4096u) {
             x = parent[x];
             d++;
         }
+        s_root_depth[t] = d;
+    }
+    __syncthreads();
+    uint32_t best = 0;
+    if (valid) {
+        const uint32_t d = dist + s_root_depth[jump];
         const float4* tn = tnodes + 4 * (size_t)p;
         const float4 a3 = tn[3];
         const uint32_t code[2] = {__float_as_uint(a3.x), __float_as_uint(a3.y)};
-        if ((code[0] | code[1]) & LEAF_BIT) best = max(best, d + 1u);
-        if (d & 1u) continue;
-        const float4 a0 = tn[0], a1 = tn[1], a2 = tn[2];
-        float* q = (float*)(nodes4 + 8 * (size_t)p);
-        uint32_t k = 0;
-        const float box[2][6] = {{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, {a1.z, a1.w, a2.x, a2.y, a2.z, a2.w}};
-        for (int c = 0; c < 2; c++) {
-            if (code[c] & LEAF_BIT) {
-                for (int a = 0; a < 6; a++) q[a * 4 + k] = box[c][a];
-                q[24 + k] = __uint_as_float(code[c]);
-                k++;
-            } else {
-                const float4* tq = tnodes + 4 * (size_t)code[c];
-                const float4 b0 = tq[0], b1 = tq[1], b2 = tq[2], b3 = tq[3];
-                const float gb[2][6] = {{b0.x, b0.y, b0.z, b0.w, b1.x, b1.y}, {b1.z, b1.w, b2.x, b2.y, b2.z, b2.w}};
-                const uint32_t gc[2] = {__float_as_uint(b3.x), __float_as_uint(b3.y)};
-                for (int g = 0; g < 2; g++) {
-                    for (int a = 0; a < 6; a++) q[a * 4 + k] = gb[g][a];
-                    q[24 + k] = __uint_as_float(gc[g]);
+        if ((code[0] | code[1]) & LEAF_BIT) best = d + 1u;
+        if (!(d & 1u)) {
+            const float4 a0 = tn[0], a1 = tn[1], a2 = tn[2];
+            float* q = (float*)(nodes4 + 8 * (size_t)p);
+            uint32_t k = 0;
+            const float box[2][6] = {{a0.x, a0.y, a0.z, a0.w, a1.x, a1.y}, {a1.z, a1.w, a2.x, a2.y, a2.z, a2.w}};
+            for (int c = 0; c < 2; c++) {
+                if (code[c] & LEAF_BIT) {
+                    for (int a = 0; a < 6; a++) q[a * 4 + k] = box[c][a];
+                    q[24 + k] = __uint_as_float(code[c]);
                     k++;
+                } else {
+                    const float4* tq = tnodes + 4 * (size_t)code[c];
+                    const float4 b0 = tq[0], b1 = tq[1], b2 = tq[2], b3 = tq[3];
+                    const float gb[2][6] = {{b0.x, b0.y, b0.z, b0.w, b1.x, b1.y},
+                                            {b1.z, b1.w, b2.x, b2.y, b2.z, b2.w}};
+                    const uint32_t gc[2] = {__float_as_uint(b3.x), __float_as_uint(b3.y)};
+                    for (int g = 0; g < 2; g++) {
+                        for (int a = 0; a < 6; a++) q[a * 4 + k] = gb[g][a];
+                        q[24 + k] = __uint_as_float(gc[g]);
+                        k++;
+                    }
                 }
             }
-        }
-        for (; k < 4; k++) {
-            for (int a = 0; a < 3; a++) {
-                q[a * 4 + k] = INFINITY;
-                q[(a + 3) * 4 + k] = -INFINITY;
+            for (; k < 4; k++) {
+                for (int a = 0; a < 3; a++) {
+                    q[a * 4 + k] = INFINITY;
+                    q[(a + 3) * 4 + k] = -INFINITY;
+                }
+                q[24 + k] = __uint_as_float(0xFFFFFFFFu);
             }
-            q[24 + k] = __uint_as_float(0xFFFFFFFFu);
+            for (int j = 28; j < 32; j++) q[j] = 0.0f;
         }
-        for (int j = 28; j < 32; j++) q[j] = 0.0f;
     }
-    __shared__ uint32_t part[4];
+    __shared__ uint32_t part[DEPTH_BLOCK / 64];
     for (int off = 32; off > 0; off >>= 1) best = max(best, (uint32_t)__shfl_xor((int)best, off, 64));
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = best;
+    if ((t & 63) == 0) part[t >> 6] = best;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < blockDim.x / 64; w++) best = max(best, part[w]);
+    if (t == 0) {
+        for (uint32_t w = 1; w < DEPTH_BLOCK / 64; w++) best = max(best, part[w]);
         if (best > ld_agent(&meta[1])) atomicMax(&meta[1], best);
     }
 }
@@ -570,7 +614,8 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris)
                        (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
                        b->halves, b->counters, b->tnodes, b->leaves, b->meta));
     if (n > 1)
-        MRH_TRY(launch(ctx, "bvh_depth_collapse", dim3(grid), dim3(256), 0, k_bvh_depth_collapse, n,
+        MRH_TRY(launch(ctx, "bvh_depth_collapse", dim3(div_up(n - 1, DEPTH_BLOCK)), dim3(DEPTH_BLOCK), 0,
+                       k_bvh_depth_collapse, n,
                        (const float4*)b->tnodes, (const uint32_t*)b->parent, b->nodes4, b->meta));
     uint32_t host_meta[9] = {};
     MRH_HIP_CHECK(ctx, hipMemcpyAsync(host_meta, b->meta, sizeof(host_meta), hipMemcpyDeviceToHost,

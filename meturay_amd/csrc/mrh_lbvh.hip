@@ -266,7 +266,7 @@ constexpr uint32_t EMIT_BLOCK = 256;
 template <class CodeT>
 __global__ __launch_bounds__(EMIT_BLOCK) void k_lbvh_emit(
     uint32_t n, const CodeT* __restrict__ scodes, const uint32_t* __restrict__ sidx,
-    const float* __restrict__ leaf_aabb, const float* __restrict__ tris,
+    const float* __restrict__ tris,
     const uint32_t* __restrict__ prim_key, uint32_t* __restrict__ left, uint32_t* __restrict__ right,
     uint32_t* __restrict__ parent, float* __restrict__ node_aabb, uint32_t* halves, uint32_t* counters,
     float4* __restrict__ tnodes, float4* __restrict__ leaves, uint32_t* meta)
@@ -319,16 +319,23 @@ __global__ __launch_bounds__(EMIT_BLOCK) void k_lbvh_emit(
     const uint32_t i = lo + t;
     if (i >= n) return;
     const uint32_t prim = sidx[i];
+    // leaf record, and the leaf's box from the same corners (the values
+    // k_tri_bounds computed: one gather instead of two)
     float box[6];
-#pragma unroll
-    for (int a = 0; a < 6; a++) box[a] = leaf_aabb[(size_t)prim * 6 + a];
-    // leaf records
     {
         const float* tr = tris + (size_t)prim * 9;
+        float c[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) c[k] = tr[k];
         const uint32_t key = prim_key ? prim_key[prim] : 0u;
-        leaves[3 * (size_t)i + 0] = make_float4(tr[0], tr[1], tr[2], __uint_as_float(prim));
-        leaves[3 * (size_t)i + 1] = make_float4(tr[3], tr[4], tr[5], __uint_as_float(key));
-        leaves[3 * (size_t)i + 2] = make_float4(tr[6], tr[7], tr[8], 0.0f);
+        leaves[3 * (size_t)i + 0] = make_float4(c[0], c[1], c[2], __uint_as_float(prim));
+        leaves[3 * (size_t)i + 1] = make_float4(c[3], c[4], c[5], __uint_as_float(key));
+        leaves[3 * (size_t)i + 2] = make_float4(c[6], c[7], c[8], 0.0f);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            box[a] = fminf(fminf(c[a], c[3 + a]), c[6 + a]);
+            box[3 + a] = fmaxf(fmaxf(c[a], c[3 + a]), c[6 + a]);
+        }
     }
 #pragma unroll
     for (int a = 0; a < 6; a++) node_aabb[(size_t)i * 6 + a] = box[a];
@@ -604,13 +611,13 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris)
     if (b->bits == 30)
         MRH_TRY(launch(ctx, "lbvh_emit", dim3(egrid), dim3(EMIT_BLOCK), 0, k_lbvh_emit<uint32_t>, n,
                        (const uint32_t*)b->sorted_codes, (const uint32_t*)b->sorted_idx,
-                       (const float*)b->leaf_aabb, d_tris,
+                       d_tris,
                        (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
                        b->halves, b->counters, b->tnodes, b->leaves, b->meta));
     else
         MRH_TRY(launch(ctx, "lbvh_emit", dim3(egrid), dim3(EMIT_BLOCK), 0, k_lbvh_emit<uint64_t>, n,
                        (const uint64_t*)b->sorted_codes, (const uint32_t*)b->sorted_idx,
-                       (const float*)b->leaf_aabb, d_tris,
+                       d_tris,
                        (const uint32_t*)b->prim_key, b->left, b->right, b->parent, b->node_aabb,
                        b->halves, b->counters, b->tnodes, b->leaves, b->meta));
     if (n > 1)

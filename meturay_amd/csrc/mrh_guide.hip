@@ -428,7 +428,7 @@ static int32_t grow(mrh_context* ctx, void** p, size_t* have, size_t want)
 // and the parent index of the level below; then links, lookup records and
 // fresh radiance.
 int32_t svo_build(mrh_context* ctx, mrh_svo* s, const uint32_t* d_codes, uint32_t n, const float cube[4],
-                  uint32_t depth)
+                  uint32_t depth, bool filtered)
 {
     if (depth < 1 || depth > 10) return fail(ctx, MRH_ERR_INVALID, "svo depth must be in [1, 10]");
     if (n == 0) return fail(ctx, MRH_ERR_INVALID, "svo build: no points");
@@ -495,9 +495,10 @@ int32_t svo_build(mrh_context* ctx, mrh_svo* s, const uint32_t* d_codes, uint32_
                    (const uint32_t*)s->first_child, (const uint32_t*)s->child_mask, (const uint32_t*)nullptr,
                    s->pack));
     // fresh radiance
-    MRH_TRY(fill_u32(ctx, (uint32_t*)s->sum, 0u, (size_t)s->nodes * GUIDE_BINS));
     MRH_TRY(fill_u32(ctx, (uint32_t*)s->acc, 0u, (size_t)s->nodes * GUIDE_BINS * 2));
     MRH_TRY(fill_u32(ctx, s->cnt, 0u, (size_t)s->nodes * GUIDE_BINS));
+    if (filtered) return MRH_OK;
+    MRH_TRY(fill_u32(ctx, (uint32_t*)s->sum, 0u, (size_t)s->nodes * GUIDE_BINS));
     uint32_t eps_bits;
     std::memcpy(&eps_bits, &GUIDE_EPS, 4);
     MRH_TRY(fill_u32(ctx, (uint32_t*)s->radiance, eps_bits, (size_t)s->nodes * GUIDE_BINS));

@@ -295,8 +295,11 @@ __device__ __forceinline__ float guide_pdf(const SvoView& s, uint32_t node, V3 n
 SvoView svo_view(const mrh_svo* s);
 int32_t svo_codes(mrh_context* ctx, const float* d_points, uint32_t n, uint32_t pitch, const float cube[4],
                   uint32_t depth, uint32_t* d_codes);
+// filtered = true: a splat and a filter follow, which rewrite the means,
+// the filtered radiance and the node counts, so only the splat
+// accumulators are cleared
 int32_t svo_build(mrh_context* ctx, mrh_svo* s, const uint32_t* d_codes, uint32_t n, const float cube[4],
-                  uint32_t depth);
+                  uint32_t depth, bool filtered = false);
 int32_t svo_splat(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, const uint32_t* bins, const float* values,
                   uint32_t n);
 int32_t svo_filter(mrh_context* ctx, mrh_svo* s);

@@ -914,7 +914,7 @@ int32_t guide_learn(mrh_context* ctx, GuideState& g)
     MRH_TRY(svo_compact_records(ctx, g.svo, g.rec.code, g.rec.bin, g.rec.rad, (uint32_t)g.n_rec, g.c_codes, g.c_bins,
                                 g.c_rad, &kept));
     if (kept == 0) return MRH_OK;   // nothing learned: keep the previous guide
-    MRH_TRY(svo_build(ctx, g.svo, g.c_codes, kept, g.cube, g.gp.svo_depth));
+    MRH_TRY(svo_build(ctx, g.svo, g.c_codes, kept, g.cube, g.gp.svo_depth, true));
     MRH_TRY(svo_splat(ctx, g.svo, g.c_codes, g.c_bins, g.c_rad, kept));
     MRH_TRY(svo_filter(ctx, g.svo));
     g.have_guide = true;

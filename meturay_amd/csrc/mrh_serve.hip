@@ -576,7 +576,7 @@ int32_t learn_pass(mrh_server* s, ServeReq* q)
     MRH_TRY(svo_compact_records(lc, q->svo, q->rec.code, q->rec.bin, q->rec.rad, n_rec, q->c_codes, q->c_bins,
                                 q->c_rad, &kept));
     if (kept) {
-        MRH_TRY(svo_build(lc, q->svo, q->c_codes, kept, q->cube, q->r.guide.svo_depth));
+        MRH_TRY(svo_build(lc, q->svo, q->c_codes, kept, q->cube, q->r.guide.svo_depth, true));
         MRH_TRY(svo_splat(lc, q->svo, q->c_codes, q->c_bins, q->c_rad, kept));
         MRH_TRY(svo_filter(lc, q->svo));
     }

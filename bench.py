@@ -244,10 +244,12 @@ def instanced_leg(N, ctx, img, torch):
             "segments": int(st.segments), "build_ms": round(build * 1e3, 2)}
 
 
-def serving_leg(N, ctx, torch, n_req: int, rate: float):
+def serving_leg(N, ctx, torch, n_req: int, rate: float, passes=("serial", "saturated", "realtime")):
     """C5: serial one-request-at-a-time launches vs the serving engine, the
     latter both saturated (all requests queued at once) and in real time
-    with the Poisson arrivals.  Latency = completion - arrival."""
+    with the Poisson arrivals.  Latency = completion - arrival.  `passes`
+    selects which of the three run (all three for the reported line; one,
+    to profile it alone)."""
     import time as _t
     from meturay_amd.workloads import serving_scene, serving_workload
     grids, reqs = serving_workload(n_req, rate)
@@ -261,6 +263,8 @@ def serving_leg(N, ctx, torch, n_req: int, rate: float):
     def image(r):
         return torch.empty(r["w"] * r["h"] * 3, dtype=torch.float32, device="cuda")
 
+    if "serial" not in passes or "saturated" not in passes or "realtime" not in passes:
+        return serving_passes(N, ctx, torch, n_req, reqs, scenes, depth, rr, passes)
     # serial: each request alone, back to back
     service = np.zeros(n_req)
     seg_serial = 0
@@ -342,6 +346,51 @@ def serving_leg(N, ctx, torch, n_req: int, rate: float):
     }
 
 
+def serving_passes(N, ctx, torch, n_req, reqs, scenes, depth, rr, passes):
+    """Some of the serving leg's passes alone (for profiling): serial launches
+    and / or the saturated engine; reports their rates only."""
+    import time as _t
+    out = {"passes": list(passes)}
+    buf = torch.empty(1920 * 1080 * 3, dtype=torch.float32, device="cuda")
+    if "serial" in passes:
+        torch.cuda.synchronize()
+        t0, segs = _t.perf_counter(), 0
+        for r in reqs:
+            sc = scenes[r["scene"]]
+            if r["it"] > 1:
+                st = sc.render_guided(buf.data_ptr(), r["w"], r["h"], r["S"], depth, rr, r["seed"],
+                                      iterations=r["it"], svo_depth=10, guide_ratio=0.5, min_samples=512)
+            else:
+                st = sc.render(buf.data_ptr(), r["w"], r["h"], r["S"], depth, rr, r["seed"])
+            segs += st.segments
+        out["serial_Mrays_s"] = round(segs / (_t.perf_counter() - t0) / 1e6, 1)
+        log(f"serial: {out['serial_Mrays_s']} Mrays/s")
+    if "saturated" in passes:
+        srv = N.Server(ctx, hbm_budget=200 << 30, wave_paths=1 << 24, max_slots=64, use_graphs=True)
+        imgs, segs, n_done = {}, 0, 0
+        t0 = _t.monotonic()
+        for i, r in enumerate(reqs):
+            imgs[i] = torch.empty(r["w"] * r["h"] * 3, dtype=torch.float32, device="cuda")
+            srv.submit(scenes[r["scene"]], imgs[i], r["w"], r["h"], r["S"], depth, rr, r["seed"],
+                       iterations=r["it"], svo_depth=10, guide_ratio=0.5, min_samples=512, tag=i)
+        while n_done < n_req:
+            srv.step()
+            for d in srv.poll():
+                if d["status"] != 0:
+                    raise RuntimeError(f"request {d['tag']} failed with status {d['status']}")
+                segs += d["segments"]
+                del imgs[d["tag"]]
+                n_done += 1
+        wall = _t.monotonic() - t0
+        out["saturated_Mrays_s"] = round(segs / wall / 1e6, 1)
+        out["steps"] = int(srv.stats().steps)
+        srv.close()
+        log(f"engine saturated: {out['saturated_Mrays_s']} Mrays/s over {wall:.1f} s, {out['steps']} steps")
+    for sc in scenes:
+        sc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +408,8 @@ def main():
     ap.add_argument("--no-c2", action="store_true", help="skip the LBVH build + ray cast leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving leg requests (0 = skip)")
     ap.add_argument("--rate", type=float, default=40.0, help="serving leg arrivals per second")
+    ap.add_argument("--serving-passes", default="serial,saturated,realtime",
+                    help="serving passes to run (all three for the reported figures; one to profile it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 512 MiB device-to-device first: the reference kernel for scaling "
                          "FETCH_SIZE/WRITE_SIZE counters")
@@ -502,7 +553,8 @@ def main():
         c2 = c2_leg(N, ctx, torch)
     serving = None
     if args.requests > 0:
-        serving = serving_leg(N, ctx, torch, args.requests, args.rate)
+        serving = serving_leg(N, ctx, torch, args.requests, args.rate,
+                              tuple(x for x in args.serving_passes.split(",") if x))
     cpu = None
     if not args.no_cpu:
         log("CPU baseline ...")
@@ -535,7 +587,7 @@ def main():
             "bvh_build_ms": round(build_ms, 2),
             "bvh_build_first_create_ms": round(build_cold_ms, 2),
             "latency_ms": {"p50": serving["p50_ms"], "p99": serving["p99_ms"], "of": "serving requests"}
-                          if serving else
+                          if serving and "p50_ms" in serving else
                           {"p50": round(float(np.percentile(lat_sorted, 50)), 3),
                            "p99": round(float(np.percentile(lat_sorted, 99)), 3), "of": "frames"},
             "serving": serving,

@@ -147,9 +147,11 @@ int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, cons
 
 // Two-level scenes: the top tree over the instances, then the hit mesh's
 // tree in local space (traverse_two_level); writes the instance too, and the
-// instance's HitKey override where it has one.
+// instance's HitKey override where it has one.  Five waves per SIMD: the
+// walk needs 90 registers; capped at 80 (six waves) it spills and runs 3 %
+// slower, at 72 or 64 it spills far more (1.4x and 2.9x the frame time).
 template <int W, int CAP, bool STATS, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_trace_pt2(const float4* __restrict__ top4,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_pt2(const float4* __restrict__ top4,
                                                    const float4* __restrict__ top_leaves, uint32_t top_root,
                                                    uint32_t n_inst, const float4* __restrict__ inst,
                                                    const float4* __restrict__ mesh_nodes,

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 // The rays of two-level requests (launched only when one is admitted): the
 // top tree, then the hit mesh's tree in local space (two_level_w).
 template <int W, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_serve_trace2(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_serve_trace2(
     const StepHdr* __restrict__ hdr, const SlotDev* __restrict__ slots, PathBuf in,
     const uint32_t* __restrict__ in_slot, HitBuf hit, uint2* __restrict__ overflow)
 {

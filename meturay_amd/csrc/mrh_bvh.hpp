@@ -123,6 +123,7 @@ struct TraceResult {
 template <int N>
 struct PrivateStack {
     static constexpr int CAP = N;
+    static constexpr int LDS_ENTRIES = 0;
     uint32_t node[N];
     float t[N];
     __device__ __forceinline__ void put(int i, uint32_t n, float d)
@@ -145,6 +146,7 @@ typedef __attribute__((address_space(3))) float lds_f32;
 template <int L, int N, int BLOCK>
 struct LdsStack {
     static constexpr int CAP = N;
+    static constexpr int LDS_ENTRIES = L;
     lds_u32* s_node;      // [L][BLOCK]
     lds_f32* s_t;         // [L][BLOCK]
     uint32_t slot;        // threadIdx.x
@@ -162,6 +164,12 @@ struct LdsStack {
         } else {
             overflow[(size_t)(i - L) * ostride + oslot] = make_uint2(n, __float_as_uint(d));
         }
+    }
+    // entry i < L, no check
+    __device__ __forceinline__ void put_lds(int i, uint32_t n, float d)
+    {
+        s_node[i * BLOCK + slot] = n;
+        s_t[i * BLOCK + slot] = d;
     }
     __device__ __forceinline__ void get(int i, uint32_t& n, float& d) const
     {
@@ -384,6 +392,20 @@ __device__ __forceinline__ uint32_t node4_order_push(Stack& st, int& sp, const N
     cx(k0, s0, c0, k2, s2, c2);
     cx(k1, s1, c1, k3, s3, c3);
     cx(k1, s1, c1, k2, s2, c2);
+    if constexpr (Stack::LDS_ENTRIES >= 3) {
+        // Room for three more entries in the LDS part (nearly always): write
+        // three entries without branches, the pushed ones farthest first
+        // and then whatever; only the cnt - 1 pushed ones are below the new
+        // top, so the rest is never read.
+        if (sp + 3 <= Stack::LDS_ENTRIES) {
+            const bool four = cnt >= 4, three = cnt == 3;
+            st.put_lds(sp, four ? c3 : (three ? c2 : c1), four ? k3 : (three ? k2 : k1));
+            st.put_lds(sp + 1, four ? c2 : c1, four ? k2 : k1);
+            st.put_lds(sp + 2, c1, k1);
+            sp += cnt - 1;
+            return c0;
+        }
+    }
     // push farthest first; the guard keeps a malformed tree from writing
     // past the stack (the host sizes CAP for the tree depth)
     if (cnt >= 4 && sp < Stack::CAP) st.put(sp++, c3, k3);
@@ -512,6 +534,7 @@ __device__ __forceinline__ V3 xf_dir(float4 r0, float4 r1, float4 r2, V3 d)
 template <class P>
 struct OffsetStack {
     static constexpr int CAP = P::CAP;
+    static constexpr int LDS_ENTRIES = 0;
     P& p;
     int base;
     __device__ __forceinline__ void put(int i, uint32_t n, float d)

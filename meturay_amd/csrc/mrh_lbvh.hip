@@ -251,7 +251,7 @@ __device__ __forceinline__ void ld_half(const uint32_t* p, u32x4& a, u32x4& b)
 // its sibling's half and has everything p needs: both children, both boxes
 // and p's range.
 //
-// A block owns 256 consecutive leaves [lo, hi].  A node whose range lies
+// A block owns 1024 consecutive leaves [lo, hi].  A node whose range lies
 // inside them is reached only by the block's own threads, so its record and
 // counter live in LDS; the others use the global record and counter, with
 // stores and loads that go through to memory.  Node p (split between p and
@@ -261,7 +261,8 @@ __device__ __forceinline__ void ld_half(const uint32_t* p, u32x4& a, u32x4& b)
 // minimum of the block's deltas.  The arrays the view reads (links,
 // parents, boxes, binary traversal nodes) are plain stores, read only after
 // the kernel.
-constexpr uint32_t EMIT_BLOCK = 256;
+constexpr uint32_t EMIT_BLOCK = 1024;
+constexpr uint32_t EMIT_WAVES = EMIT_BLOCK / 64;
 
 template <class CodeT>
 __global__ __launch_bounds__(EMIT_BLOCK) void k_lbvh_emit(
@@ -273,8 +274,8 @@ __global__ __launch_bounds__(EMIT_BLOCK) void k_lbvh_emit(
 {
     __shared__ uint32_t s_delta[EMIT_BLOCK + 1];   // delta(lo - 1 + k)
     __shared__ uint32_t s_suf[EMIT_BLOCK];         // min of s_delta[k .. EMIT_BLOCK]
-    __shared__ uint32_t s_wave[8];
-    __shared__ uint32_t s_local[EMIT_BLOCK];
+    __shared__ uint32_t s_wave[2 * EMIT_WAVES];
+    __shared__ uint8_t s_local[EMIT_BLOCK];
     __shared__ uint32_t s_cnt[EMIT_BLOCK];
     __shared__ u32x4 s_rec[EMIT_BLOCK * 4];        // [node - lo][4 x 16 B]
     const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
@@ -302,10 +303,10 @@ __global__ __launch_bounds__(EMIT_BLOCK) void k_lbvh_emit(
         if (lane + off < 64) sm = min(sm, y);
     }
     if (lane == 63) s_wave[wave] = pm;
-    if (lane == 0) s_wave[4 + wave] = sm;
+    if (lane == 0) s_wave[EMIT_WAVES + wave] = sm;
     __syncthreads();
     for (uint32_t w = 0; w < wave; w++) pm = min(pm, s_wave[w]);
-    for (uint32_t w = wave + 1; w < EMIT_BLOCK / 64; w++) sm = min(sm, s_wave[4 + w]);
+    for (uint32_t w = wave + 1; w < EMIT_WAVES; w++) sm = min(sm, s_wave[EMIT_WAVES + w]);
     s_suf[t] = min(sm, s_delta[EMIT_BLOCK]);
     __syncthreads();
     {
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(EMIT_BLOCK) void k_lbvh_emit(
         const uint32_t dp = s_delta[t + 1u];
         const uint32_t right_min = (t + 2u < EMIT_BLOCK) ? s_suf[t + 2u]
                                  : (t + 2u == EMIT_BLOCK ? s_delta[EMIT_BLOCK] : 0xFFFFFFFFu);
-        s_local[t] = (lo + t < hi && pm < dp && right_min < dp) ? 1u : 0u;
+        s_local[t] = (lo + t < hi && pm < dp && right_min < dp) ? 1 : 0;
     }
     __syncthreads();
     const uint32_t i = lo + t;

@@ -612,7 +612,13 @@ typedef struct {
     uint32_t use_graphs;     /* replay each step's launches from a captured hipGraph */
     uint32_t timeout_ms;     /* 0 = none; otherwise a request not done this long after its
                                 submission ends with MRH_ERR_TIMEOUT */
+    uint32_t policy;         /* order in which admitted requests inject new paths while the
+                                queue has room: MRH_SERVE_SMALLEST_FIRST (0, least remaining
+                                work first: lowest mean latency) or MRH_SERVE_ARRIVAL (1, in
+                                submission order: the oldest request goes first) */
 } mrh_server_params;
+#define MRH_SERVE_SMALLEST_FIRST 0u
+#define MRH_SERVE_ARRIVAL        1u
 
 typedef struct {
     mrh_scene*        scene;

@@ -910,11 +910,15 @@ int32_t server_step(mrh_server* s, uint32_t* active, uint32_t* queued)
     std::vector<ServeReq*> act;
     for (uint32_t k = 0; k < s->p.max_slots; k++)
         if (s->slots[k] && s->slots[k]->injected < s->slots[k]->total) act.push_back(s->slots[k]);
-    std::sort(act.begin(), act.end(), [](const ServeReq* a, const ServeReq* b) {
-        const uint64_t ra = (a->total - a->injected) + (uint64_t)(a->iterations - a->it - 1) * a->total;
-        const uint64_t rb = (b->total - b->injected) + (uint64_t)(b->iterations - b->it - 1) * b->total;
-        return ra != rb ? ra < rb : a->ticket < b->ticket;
-    });
+    if (s->p.policy == MRH_SERVE_ARRIVAL) {
+        std::sort(act.begin(), act.end(), [](const ServeReq* a, const ServeReq* b) { return a->ticket < b->ticket; });
+    } else {
+        std::sort(act.begin(), act.end(), [](const ServeReq* a, const ServeReq* b) {
+            const uint64_t ra = (a->total - a->injected) + (uint64_t)(a->iterations - a->it - 1) * a->total;
+            const uint64_t rb = (b->total - b->injected) + (uint64_t)(b->iterations - b->it - 1) * b->total;
+            return ra != rb ? ra < rb : a->ticket < b->ticket;
+        });
+    }
     StepHdr& h = *s->h_hdr;
     h.n_live = s->n_live;
     h.n_chunks = 0;
@@ -1011,6 +1015,7 @@ int32_t server_create(mrh_context* user, const mrh_server_params* p, mrh_server*
 {
     if (p->max_slots == 0 || p->max_slots > SERVE_MAX_SLOTS)
         return fail(user, MRH_ERR_INVALID, "server: max_slots must be in [1, %u]", SERVE_MAX_SLOTS);
+    if (p->policy > MRH_SERVE_ARRIVAL) return fail(user, MRH_ERR_INVALID, "server: unknown policy %u", p->policy);
     if (p->wave_paths < 1024 || p->wave_paths >= (1u << 30))
         return fail(user, MRH_ERR_INVALID, "server: wave_paths must be in [1024, 2^30)");
     // a queue holds 2 * wave_paths rays; the global partition sorts it whole

@@ -95,7 +95,7 @@ class GuideParams(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("wave_paths", _U32), ("max_slots", _U32),
-                ("use_graphs", _U32), ("timeout_ms", _U32)]
+                ("use_graphs", _U32), ("timeout_ms", _U32), ("policy", _U32)]
 
 
 class Request(C.Structure):
@@ -736,11 +736,11 @@ class Server:
     """Serving engine (mrh_server): queued render requests traced together."""
 
     def __init__(self, ctx: Context, hbm_budget: int = 200 << 30, wave_paths: int = 1 << 24,
-                 max_slots: int = 64, use_graphs: bool = True, timeout_ms: int = 0):
+                 max_slots: int = 64, use_graphs: bool = True, timeout_ms: int = 0, policy: int = 0):
         self.ctx = ctx
         ctx.adopt(self)
         self.h = _P()
-        p = ServerParams(hbm_budget, wave_paths, max_slots, 1 if use_graphs else 0, timeout_ms)
+        p = ServerParams(hbm_budget, wave_paths, max_slots, 1 if use_graphs else 0, timeout_ms, policy)
         ctx.check(lib().mrh_server_create(ctx.h, C.byref(p), C.byref(self.h)))
         inner = _P()
         ctx.check(lib().mrh_server_context(self.h, C.byref(inner)))

@@ -254,7 +254,26 @@ def test_injected_allocation_failure_is_isolated(ctx, scenes):
     server.close()
 
 
+def test_mix_matches_solo_in_arrival_order(ctx, scenes):
+    """Injection in submission order (policy MRH_SERVE_ARRIVAL) changes only
+    which request's samples enter a step first, never their paths."""
+    ref = [solo(ctx, scenes[m[0]][1], *m[1:]) for m in MIX]
+    server = N.Server(ctx, hbm_budget=8 << 30, wave_paths=4096, max_slots=4, use_graphs=True, policy=1)
+    imgs = [torch.full((m[1] * m[2] * 3,), 7.0, dtype=torch.float32, device="cuda") for m in MIX]
+    torch.cuda.synchronize()
+    submit_mix(server, scenes, imgs)
+    done = server.run()
+    assert sorted(d["tag"] for d in done) == list(range(len(MIX)))
+    for d in done:
+        k = d["tag"]
+        assert d["status"] == 0 and d["segments"] == ref[k][1]
+        assert np.allclose(imgs[k].cpu().numpy(), ref[k][0], rtol=1e-4, atol=1e-6), k
+    server.close()
+
+
 def test_server_errors(ctx, scenes):
+    with pytest.raises(N.MrhError):
+        N.Server(ctx, policy=2)
     with pytest.raises(N.MrhError):
         N.Server(ctx, wave_paths=100)
     with pytest.raises(N.MrhError):

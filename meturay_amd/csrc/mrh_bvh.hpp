@@ -143,6 +143,18 @@ struct PrivateStack {
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) float lds_f32;
 
+// Waves per SIMD a trace kernel asks the register allocator for: `want`,
+// or fewer when its LDS stack (L entries of 8 bytes per thread, BLOCK
+// threads per block, 160 KiB of LDS per CU, four SIMDs) already holds the
+// CU below that; asking for more would only cap registers for nothing.
+constexpr int lds_stack_waves(int L, int BLOCK, int want)
+{
+    if (L <= 0) return want;
+    const int blocks = (160 * 1024) / (L * BLOCK * 8);
+    const int waves = blocks * (BLOCK / 64) / 4;
+    return waves < 1 ? 1 : (waves < want ? waves : want);
+}
+
 template <int L, int N, int BLOCK>
 struct LdsStack {
     static constexpr int CAP = N;

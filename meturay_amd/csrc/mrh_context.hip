@@ -61,7 +61,7 @@ hipEvent_t prof_begin(mrh_context* ctx)
     } else if (hipEventCreate(&e) != hipSuccess) {
         return nullptr;
     }
-    hipEventRecord(e, ctx->stream);
+    (void)hipEventRecord(e, ctx->stream);
     return e;
 }
 
@@ -142,14 +142,14 @@ int32_t mrh_context_create(int32_t device, mrh_context** out)
 int32_t mrh_context_destroy(mrh_context* ctx)
 {
     if (!ctx) return MRH_ERR_INVALID;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     if (ctx->guide_svo) mrh_svo_destroy(ctx->guide_svo);
     for (int i = 0; i < SLOT_COUNT; i++)
-        if (ctx->slot_ptr[i]) hipFree(ctx->slot_ptr[i]);
-    for (auto& p : ctx->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
-    for (auto e : ctx->free_events) hipEventDestroy(e);
-    hipStreamDestroy(ctx->stream);
+        if (ctx->slot_ptr[i]) (void)hipFree(ctx->slot_ptr[i]);
+    for (auto& p : ctx->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MRH_OK;
 }

@@ -282,7 +282,7 @@ int32_t two_level_build(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene* s)
     if (hipMemcpy(dboxes, boxes.data(), boxes.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(ctx, MRH_ERR_HIP, "mrh_scene_create: instance box upload failed");
     if (rc == MRH_OK) rc = lbvh_create(ctx, dboxes, ni, d->morton_bits ? d->morton_bits : 30u, nullptr, &s->bvh);
-    hipFree(dboxes);
+    (void)hipFree(dboxes);
     if (rc != MRH_OK) return rc;
     s->bytes += (size_t)ni * (36 + 24 + 8 + 8 + 4 + 12 + 8 + 48 + 64 + 4);
     if (!lights.empty()) {

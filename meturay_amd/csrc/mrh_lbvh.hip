@@ -607,7 +607,6 @@ int32_t lbvh_build_into(mrh_lbvh* b, const float* d_tris)
     sa.end_bit = b->bits;
     MRH_TRY(radix_sort(ctx, sa));
     if (n > 1) MRH_TRY(fill_u32(ctx, b->counters, 0u, n - 1));
-    const uint32_t grid = stride_grid(ctx, n, 256);
     const uint32_t egrid = div_up(n, EMIT_BLOCK);   // one thread per leaf
     if (b->bits == 30)
         MRH_TRY(launch(ctx, "lbvh_emit", dim3(egrid), dim3(EMIT_BLOCK), 0, k_lbvh_emit<uint32_t>, n,
@@ -686,8 +685,8 @@ int32_t lbvh_create(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t 
         rc = lbvh_build_into(b, d_tris);
     } while (0);
     if (rc != MRH_OK) {
-        hipStreamSynchronize(ctx->stream);
-        for (void* p : b->allocs) hipFree(p);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (void* p : b->allocs) (void)hipFree(p);
         delete b;
         return rc;
     }
@@ -698,8 +697,8 @@ int32_t lbvh_create(mrh_context* ctx, const float* d_tris, uint32_t n, uint32_t 
 void lbvh_free(mrh_lbvh* b)
 {
     if (!b) return;
-    if (b->ctx) hipStreamSynchronize(b->ctx->stream);
-    for (void* p : b->allocs) hipFree(p);
+    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    for (void* p : b->allocs) (void)hipFree(p);
     delete b;
 }
 

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_raygen(Cam cam, uint32_t w, uint32_t h,
 constexpr uint32_t COUNTS_WORDS = 128;
 
 template <int W, int CAP, bool STATS, int L, int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(8))) void k_trace_pt(const float4* __restrict__ tnodes,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(lds_stack_waves(L, BS, 8)))) void k_trace_pt(const float4* __restrict__ tnodes,
                                                   const float4* __restrict__ leaves, uint32_t root, uint32_t n_leaves,
                                                   PathBuf in, uint32_t* __restrict__ counts,
                                                   uint32_t bounce, float tmin, float tmax, HitBuf hit,
@@ -151,7 +151,7 @@ int32_t launch_trace_pt(mrh_context* ctx, uint32_t grid, const mrh_lbvh* b, cons
 // walk needs 90 registers; capped at 80 (six waves) it spills and runs 3 %
 // slower, at 72 or 64 it spills far more (1.4x and 2.9x the frame time).
 template <int W, int CAP, bool STATS, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_trace_pt2(const float4* __restrict__ top4,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_stack_waves(L, 256, 5)))) void k_trace_pt2(const float4* __restrict__ top4,
                                                    const float4* __restrict__ top_leaves, uint32_t top_root,
                                                    uint32_t n_inst, const float4* __restrict__ inst,
                                                    const float4* __restrict__ mesh_nodes,
@@ -437,9 +437,9 @@ int32_t scene_upload(mrh_scene* s, T** dst, const T* src, size_t count)
 void scene_free(mrh_scene* s)
 {
     if (!s) return;
-    if (s->ctx) hipStreamSynchronize(s->ctx->stream);
+    if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
     if (s->bvh) lbvh_free(s->bvh);
-    for (void* p : s->allocs) hipFree(p);
+    for (void* p : s->allocs) (void)hipFree(p);
     delete s;
 }
 
@@ -521,7 +521,7 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
             if ((rc = dmalloc(ctx, (void**)&drank, (size_t)d->n * 4)) != MRH_OK) break;
             void* lp = nullptr;
             if ((rc = dmalloc(ctx, &lp, (size_t)n_lights * 4)) != MRH_OK) {
-                hipFree(drank);
+                (void)hipFree(drank);
                 break;
             }
             s->allocs.push_back(lp);
@@ -534,7 +534,7 @@ int32_t scene_create(mrh_context* ctx, const mrh_scene_desc* d, mrh_scene** out)
                             (const float4*)s->bvh->leaves, d->n, (const uint32_t*)drank, s->lights);
             if (rc == MRH_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
                 rc = fail(ctx, MRH_ERR_HIP, "mrh_scene_create: light table failed");
-            hipFree(drank);
+            (void)hipFree(drank);
         }
     } while (0);
     if (rc != MRH_OK) {
@@ -766,7 +766,10 @@ int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const 
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MRH_HIP_CHECK(ctx, hipEventCreate(&e0));
-    MRH_HIP_CHECK(ctx, hipEventCreate(&e1));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(ctx, MRH_ERR_HIP, "mrh_render: event create failed");
+    }
     MRH_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
     Frame f;
     int32_t rc = frame_begin(ctx, cd, p, arena, 0u, f, fd ? sb.coords : nullptr);
@@ -779,22 +782,23 @@ int32_t render(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const 
         rc = launch(ctx, "finalize", dim3(stride_grid(ctx, img_count, 256)), dim3(256), 0, k_scale, d_image,
                     img_count, 1.0f / (float)f.spp);
     }
+    if (rc == MRH_OK && hipEventRecord(e1, ctx->stream) != hipSuccess)
+        rc = fail(ctx, MRH_ERR_HIP, "mrh_render: event record failed");
     if (rc == MRH_OK) {
-        hipEventRecord(e1, ctx->stream);
         mrh_render_stats local;
         memset(&local, 0, sizeof(local));
         rc = frame_counts(ctx, f, &local);
         if (rc == MRH_OK && stats) {
             float ms = 0.0f;
-            hipEventElapsedTime(&ms, e0, e1);
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) ms = 0.0f;
             local.render_ms = ms;
             *stats = local;
         }
     } else {
-        hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return rc;
 }
 
@@ -1104,8 +1108,8 @@ int32_t mrh_tracer_create(mrh_context* ctx, mrh_scene* scene, const mrh_render_p
     if (rc == MRH_OK) rc = dmalloc(ctx, (void**)&t->accum, img * 4);
     if (rc == MRH_OK) rc = fill_u32(ctx, (uint32_t*)t->accum, 0u, img);
     if (rc != MRH_OK) {
-        hipFree(t->arena);
-        hipFree(t->accum);
+        (void)hipFree(t->arena);
+        (void)hipFree(t->accum);
         delete t;
         return rc;
     }
@@ -1229,10 +1233,10 @@ int32_t mrh_tracer_stats(const mrh_tracer* t, mrh_render_stats* out, uint32_t* f
 int32_t mrh_tracer_destroy(mrh_tracer* t)
 {
     if (!t) return MRH_ERR_INVALID;
-    hipStreamSynchronize(t->ctx->stream);
-    hipFree(t->arena);
-    hipFree(t->accum);
-    if (t->smem) hipFree(t->smem);
+    (void)hipStreamSynchronize(t->ctx->stream);
+    (void)hipFree(t->arena);
+    (void)hipFree(t->accum);
+    if (t->smem) (void)hipFree(t->smem);
     guide_free(t->g);
     delete t;
     return MRH_OK;

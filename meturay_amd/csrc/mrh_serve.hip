@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_serve_raygen(const StepHdr* __restrict_
 // Closest hit of every queued ray in its own request's scene; camera rays
 // (depth 1) are clipped to the request's near/far planes.
 template <int W, int CAP, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_serve_trace(const StepHdr* __restrict__ hdr,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_stack_waves(L, 256, 8)))) void k_serve_trace(const StepHdr* __restrict__ hdr,
                                                      const SlotDev* __restrict__ slots, PathBuf in,
                                                      const uint32_t* __restrict__ in_slot, HitBuf hit,
                                                      uint2* __restrict__ overflow)
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 // The rays of two-level requests (launched only when one is admitted): the
 // top tree, then the hit mesh's tree in local space (two_level_w).
 template <int W, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_serve_trace2(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_stack_waves(L, 256, 5)))) void k_serve_trace2(
     const StepHdr* __restrict__ hdr, const SlotDev* __restrict__ slots, PathBuf in,
     const uint32_t* __restrict__ in_slot, HitBuf hit, uint2* __restrict__ overflow)
 {

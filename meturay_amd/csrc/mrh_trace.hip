@@ -9,7 +9,7 @@ namespace mrh {
 namespace {
 
 template <int W, bool ANY, bool STATS, int CAP, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(const float4* __restrict__ tnodes,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_stack_waves(L, 256, 8)))) void k_trace(const float4* __restrict__ tnodes,
                                                const float4* __restrict__ leaves, uint32_t root,
                                                uint32_t n_leaves, const float* __restrict__ rays,
                                                uint32_t n, uint32_t pitch, float tmin,

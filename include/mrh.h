@@ -499,6 +499,49 @@ int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes
                          uint32_t bin_ray_count, uint32_t* d_bins, uint32_t* d_order, uint32_t* d_part_offsets,
                          uint32_t* d_part_bins, uint32_t* d_part_count);
 
+/* Per-bin radiance fields of the reference's wavefront path guiding: the
+ * field-to-sample half of KCGenAndSampleDistribution (WFPGTracerKC.cuh:
+ * 1433-1545, with GenerateRadianceField's radiance query and filter,
+ * :1312-1340).  Each bin brings a field_y x field_x field of incoming
+ * radiance over the concentric octahedral map (row v, column u; the reference
+ * fills it by cone tracing its voxel octree, :1282-1310).  Per bin: texels of
+ * exactly 0 become 1e-4; the 11-tap Gaussian of variance filter_alpha
+ * ("RFieldFilterAlpha", StaticGaussianFilter1D<11>, BlockTextureFilter.cuh:
+ * 84-121) runs along u then v, folding taps outside the map back with the
+ * map's edge mirroring (Utility::CocentricOctohedralWrapInt,
+ * CoordinateConversion.h:213-235); a 2D piecewise-constant distribution is
+ * built over the result (BlockPWCDistribution2D, GPUBlockPWCDistribution.cuh:
+ * 107-200); then every ray of the bin draws (u,v) from it with its two
+ * uniforms xi (BlockPWCDistribution2D::Sample, :202-275) and gets the guide
+ * pdf of that sample in solid angle (pdf of the texel / 4 pi, :277-300 and
+ * WFPGTracerKC.cuh:1511-1512).  With eval_uv each ray also gets the guide pdf
+ * at that (u,v): the pdf the balance heuristic needs for a BxDF-sampled
+ * direction (:1518-1533; the BxDF half of that MIS stays with the material
+ * code).  out_dir receives the sampled direction with y up, as the reference
+ * hands it to the path (NormProjectionFunc, :1420-1424).  Bins are the
+ * partition that mrh_svo_bin_rays writes: rays ray_ids[bin_offsets[b] ..
+ * bin_offsets[b+1]) use field b; per-ray arrays are indexed by ray id, ids
+ * >= ray_count are skipped, rays of no bin are left untouched.  Results do not
+ * depend on the launch shape: each ray brings its own uniforms.  All pointers
+ * are device pointers, read or written during the call only. */
+typedef struct {
+    uint32_t        field_x, field_y;  /* texels along u and v: 8, 16, 32 or 64 each */
+    uint32_t        bin_count;
+    uint32_t        ray_count;         /* rays the per-ray arrays hold */
+    float           filter_alpha;      /* Gaussian variance, finite and > 0 (reference default 1) */
+    const float*    fields;            /* [bin_count][field_y][field_x] */
+    const uint32_t* bin_offsets;       /* [bin_count + 1], non-decreasing */
+    const uint32_t* ray_ids;           /* [bin_offsets[bin_count]] */
+    const float*    xi;                /* [ray_count][2] */
+    const float*    eval_uv;           /* optional [ray_count][2] */
+    float*          out_uv;            /* [ray_count][2] */
+    float*          out_dir;           /* optional [ray_count][3] */
+    float*          out_pdf;           /* [ray_count] */
+    float*          out_eval_pdf;      /* [ray_count], given iff eval_uv is */
+    float*          out_filtered;      /* optional [bin_count][field_y][field_x] */
+} mrh_wfpg_fields_desc;
+int32_t mrh_wfpg_sample_fields(mrh_context* ctx, const mrh_wfpg_fields_desc* desc);
+
 /* Guided path tracing: replaces WFPGTracer::Render / Finalize
  * (WFPGTracer.cu:1590-1790) and the WFPG path work (WFPGTracerKC.cuh:202-520,
  * options WFPGTracer.h:25-96).  `iterations` frames of samples^2 spp each;

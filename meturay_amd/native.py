@@ -81,6 +81,13 @@ def filter_desc(kind: str, radius: float, alpha: float = 0.0, b: float = 0.0, c:
     return FilterDesc(FILTER_TYPES[kind], radius, alpha, b, c)
 
 
+class WfpgFieldsDesc(C.Structure):
+    _fields_ = [("field_x", _U32), ("field_y", _U32), ("bin_count", _U32), ("ray_count", _U32),
+                ("filter_alpha", _F), ("fields", _P), ("bin_offsets", _P), ("ray_ids", _P), ("xi", _P),
+                ("eval_uv", _P), ("out_uv", _P), ("out_dir", _P), ("out_pdf", _P), ("out_eval_pdf", _P),
+                ("out_filtered", _P)]
+
+
 class SvoView(C.Structure):
     _fields_ = [("depth", _U32), ("nodes", _U32), ("level_count", _U32 * 11),
                 ("level_offset", _U32 * 12), ("cube", _F * 4), ("d_codes", _P),
@@ -200,6 +207,7 @@ _SIGS = {
     "mrh_svo_pdf": (_I32, [_P, _P, _P, _P, _P, _U32, _P]),
     "mrh_svo_view_get": (_I32, [_P, C.POINTER(SvoView)]),
     "mrh_svo_bin_rays": (_I32, [_P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
+    "mrh_wfpg_sample_fields": (_I32, [_P, C.POINTER(WfpgFieldsDesc)]),
     "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                  C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
     "mrh_server_create": (_I32, [_P, C.POINTER(ServerParams), C.POINTER(_P)]),
@@ -320,6 +328,16 @@ class Context:
 
     def filter_resolve(self, d_weighted, d_weights, width: int, height: int, d_image) -> None:
         self.check(lib().mrh_filter_resolve(self.h, ptr(d_weighted), ptr(d_weights), width, height, ptr(d_image)))
+
+    def wfpg_sample_fields(self, d_fields, field_x: int, field_y: int, bin_count: int, d_bin_offsets, d_ray_ids,
+                           ray_count: int, d_xi, d_out_uv, d_out_pdf, alpha: float = 1.0, d_eval_uv=None,
+                           d_out_eval_pdf=None, d_out_dir=None, d_out_filtered=None) -> None:
+        """mrh_wfpg_sample_fields: filter each bin's radiance field, build its
+        distribution and draw one guided (u,v) per ray of the bin."""
+        d = WfpgFieldsDesc(field_x, field_y, bin_count, ray_count, alpha, ptr(d_fields), ptr(d_bin_offsets),
+                           ptr(d_ray_ids), ptr(d_xi), ptr(d_eval_uv), ptr(d_out_uv), ptr(d_out_dir), ptr(d_out_pdf),
+                           ptr(d_out_eval_pdf), ptr(d_out_filtered))
+        self.check(lib().mrh_wfpg_sample_fields(self.h, C.byref(d)))
 
 
 class Lbvh:

@@ -324,9 +324,6 @@ void om_pcg32_stream(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t*
 uint64_t om_splitmix64(uint64_t z);
 void om_path_uniforms(uint32_t seed, uint32_t sample_id, uint32_t n, float* out);
 
-#ifdef __cplusplus
-}
-#endif
 /* smooth shading frames (GPUPrimitiveTriangle.cpp:286-316, RayLib/Quaternion.hpp:256-436) */
 void om_orthogonal_vector(const float v[3], float out[3]);
 void om_space_quat(const float x[3], const float y[3], const float z[3], float q[4]);
@@ -335,5 +332,36 @@ void om_bary_slerp(const float q0[4], const float q1[4], const float q2[4], floa
 void om_corner_frames(const float* normals, uint32_t n, float* out);
 void om_smooth_frame(const float tri[9], const float tq[12], float ba, float bb, const float dir[3], float q[4],
                      float normal[3]);
+
+/* ---- Per-bin radiance fields of the reference's wavefront path guiding
+ * (WFPGTracerKC.cuh:1252-1545): filter, piecewise-constant distribution,
+ * one guided sample per ray.  Fields are [bin][y][x]; cdf arrays hold the
+ * end of each texel; uv / xi are 2 floats per ray, dirs 3 (y up). */
+typedef struct {
+    uint32_t        field_x, field_y, bin_count, ray_count;
+    float           filter_alpha;
+    const float*    fields;
+    const uint32_t* bin_offsets;      /* bin_count + 1 */
+    const uint32_t* ray_ids;
+    const float*    xi;
+    const float*    eval_uv;          /* may be NULL */
+    float*          out_uv;
+    float*          out_dir;          /* may be NULL */
+    float*          out_pdf;
+    float*          out_eval_pdf;     /* written iff eval_uv */
+    float*          out_filtered;     /* may be NULL */
+} om_wfpg_desc;
+void  om_wfpg_gauss_weights(float alpha, float w[11]);
+void  om_octa_wrap_int(int32_t x, int32_t y, int32_t X, int32_t Y, int32_t out[2]);
+void  om_wfpg_filter(const float* field, uint32_t X, uint32_t Y, const float w[11], float* out);
+void  om_wfpg_pwc(const float* f, uint32_t X, uint32_t Y, float* pdfx, float* cdfx, float* pdfy, float* cdfy);
+void  om_wfpg_pwc_sample(const float* cdfx, const float* cdfy, uint32_t X, uint32_t Y, float xi0, float xi1,
+                         float uv[2]);
+float om_wfpg_pwc_pdf(const float* pdfx, const float* pdfy, uint32_t X, uint32_t Y, float u, float v);
+int   om_wfpg_sample_fields(const om_wfpg_desc* d);
+
+#ifdef __cplusplus
+}
+#endif
 
 #endif

@@ -91,6 +91,13 @@ class OmGuideParams(C.Structure):
     _fields_ = [("iterations", _U32), ("svo_depth", _U32), ("guide_ratio", _F), ("min_samples", _U32)]
 
 
+class OmWfpgDesc(C.Structure):
+    _fields_ = [("field_x", _U32), ("field_y", _U32), ("bin_count", _U32), ("ray_count", _U32),
+                ("filter_alpha", _F), ("fields", _P), ("bin_offsets", _P), ("ray_ids", _P), ("xi", _P),
+                ("eval_uv", _P), ("out_uv", _P), ("out_dir", _P), ("out_pdf", _P), ("out_eval_pdf", _P),
+                ("out_filtered", _P)]
+
+
 _lib = None
 _ref = None
 
@@ -130,6 +137,14 @@ def lib():
         L.om_bary_slerp.argtypes = [_P, _P, _P, _F, _F, _P]
         L.om_corner_frames.argtypes = [_P, _U32, _P]
         L.om_smooth_frame.argtypes = [_P, _P, _F, _F, _P, _P, _P]
+        L.om_wfpg_gauss_weights.argtypes = [_F, _P]
+        L.om_octa_wrap_int.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P]
+        L.om_wfpg_filter.argtypes = [_P, _U32, _U32, _P, _P]
+        L.om_wfpg_pwc.argtypes = [_P, _U32, _U32, _P, _P, _P, _P]
+        L.om_wfpg_pwc_sample.argtypes = [_P, _P, _U32, _U32, _F, _F, _P]
+        L.om_wfpg_pwc_pdf.restype = _F
+        L.om_wfpg_pwc_pdf.argtypes = [_P, _P, _U32, _U32, _F, _F]
+        L.om_wfpg_sample_fields.argtypes = [C.POINTER(OmWfpgDesc)]
         L.om_trace_closest.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F,
                                        _P, _P, _P, _P]
         L.om_trace_any.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F, _P]
@@ -255,6 +270,9 @@ def ref():
         R.ref_space_quat.argtypes = [_P, _P, _P, _P]
         R.ref_slerp.argtypes = [_P, _P, _F, _P]
         R.ref_bary_slerp.argtypes = [_P, _P, _P, _F, _F, _P]
+        R.ref_octa_to_dir.argtypes = [_F, _F, _P]
+        R.ref_dir_to_octa.argtypes = [_P, _P]
+        R.ref_octa_wrap_int.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _P]
         _ref = R
     return _ref
 
@@ -846,3 +864,72 @@ def render_guided(scene: dict, bvh: LBVH, cam: OmCamera, width, height, samples,
         else:
             lib().om_svo_free(C.byref(last))
     return img / float(samples * samples * iterations), int(seg), svo
+
+
+# ---- per-bin radiance fields of the reference's wavefront path guiding
+
+def wfpg_gauss_weights(alpha: float) -> np.ndarray:
+    w = np.zeros(11, np.float32)
+    lib().om_wfpg_gauss_weights(alpha, _ptr(w))
+    return w
+
+
+def octa_wrap_int(x: int, y: int, X: int, Y: int) -> tuple[int, int]:
+    out = np.zeros(2, np.int32)
+    lib().om_octa_wrap_int(x, y, X, Y, _ptr(out))
+    return int(out[0]), int(out[1])
+
+
+def wfpg_filter(field: np.ndarray, alpha: float) -> np.ndarray:
+    """field [Y][X] -> Gaussian-filtered field (no empty-texel raise)."""
+    field = np.ascontiguousarray(field, dtype=np.float32)
+    Y, X = field.shape
+    out = np.zeros_like(field)
+    lib().om_wfpg_filter(_ptr(field), X, Y, _ptr(wfpg_gauss_weights(alpha)), _ptr(out))
+    return out
+
+
+class Pwc2D:
+    """om_wfpg_pwc over a [Y][X] field: pdfx/cdfx [Y][X], pdfy/cdfy [Y]."""
+
+    def __init__(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, dtype=np.float32)
+        self.Y, self.X = f.shape
+        self.pdfx = np.zeros_like(f)
+        self.cdfx = np.zeros_like(f)
+        self.pdfy = np.zeros(self.Y, np.float32)
+        self.cdfy = np.zeros(self.Y, np.float32)
+        lib().om_wfpg_pwc(_ptr(f), self.X, self.Y, _ptr(self.pdfx), _ptr(self.cdfx), _ptr(self.pdfy),
+                          _ptr(self.cdfy))
+
+    def sample(self, xi0: float, xi1: float) -> np.ndarray:
+        uv = np.zeros(2, np.float32)
+        lib().om_wfpg_pwc_sample(_ptr(self.cdfx), _ptr(self.cdfy), self.X, self.Y, xi0, xi1, _ptr(uv))
+        return uv
+
+    def pdf(self, u: float, v: float) -> float:
+        return float(lib().om_wfpg_pwc_pdf(_ptr(self.pdfx), _ptr(self.pdfy), self.X, self.Y, u, v))
+
+
+def wfpg_sample_fields(fields: np.ndarray, bin_offsets: np.ndarray, ray_ids: np.ndarray, xi: np.ndarray,
+                       alpha: float = 1.0, eval_uv: np.ndarray | None = None, ray_count: int | None = None):
+    """om_wfpg_sample_fields.  fields [bins][Y][X].  Returns dict of uv [n][2],
+    dir [n][3], pdf [n], eval_pdf [n] (or None), filtered [bins][Y][X]; rays
+    outside every bin keep NaN."""
+    fields = np.ascontiguousarray(fields, dtype=np.float32)
+    bins, Y, X = fields.shape
+    bin_offsets = np.ascontiguousarray(bin_offsets, dtype=np.uint32)
+    ray_ids = np.ascontiguousarray(ray_ids, dtype=np.uint32)
+    xi = np.ascontiguousarray(xi, dtype=np.float32)
+    n = len(xi) if ray_count is None else ray_count
+    out = {"uv": np.full((n, 2), np.nan, np.float32), "dir": np.full((n, 3), np.nan, np.float32),
+           "pdf": np.full(n, np.nan, np.float32), "filtered": np.zeros_like(fields), "eval_pdf": None}
+    ev = None
+    if eval_uv is not None:
+        ev = np.ascontiguousarray(eval_uv, dtype=np.float32)
+        out["eval_pdf"] = np.full(n, np.nan, np.float32)
+    d = OmWfpgDesc(X, Y, bins, n, alpha, _ptr(fields), _ptr(bin_offsets), _ptr(ray_ids), _ptr(xi), _ptr(ev),
+                   _ptr(out["uv"]), _ptr(out["dir"]), _ptr(out["pdf"]), _ptr(out["eval_pdf"]),
+                   _ptr(out["filtered"]))
+    lib().om_wfpg_sample_fields(C.byref(d))
+    return out

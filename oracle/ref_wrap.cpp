@@ -21,6 +21,7 @@
 #include "RayLib/Quaternion.h"
 #include "RayLib/Triangle.h"
 #include "RayLib/Matrix.h"
+#include "RayLib/CoordinateConversion.h"
 
 #include <cstdint>
 
@@ -41,7 +42,7 @@ int ref_intersect_triangle(const float o[3], const float d[3], const float tri[9
                            float* t, float bary[3])
 {
     RayF r(Vector3f(d[0], d[1], d[2]), Vector3f(o[0], o[1], o[2]));
-    Vector3f b; float tt = 0.0f;
+    Vector3f b(0.0f, 0.0f, 0.0f); float tt = 0.0f;   // misses leave b as given
     bool h = r.IntersectsTriangle(b, tt, Vector3f(tri[0], tri[1], tri[2]),
                                   Vector3f(tri[3], tri[4], tri[5]),
                                   Vector3f(tri[6], tri[7], tri[8]), false);
@@ -176,6 +177,23 @@ void ref_bary_slerp(const float q0[4], const float q1[4], const float q2[4], flo
     QuatF r = Quat::BarySLerp(QuatF(q0[0], q0[1], q0[2], q0[3]), QuatF(q1[0], q1[1], q1[2], q1[3]),
                               QuatF(q2[0], q2[1], q2[2], q2[3]), a, b);
     out[0] = r[0]; out[1] = r[1]; out[2] = r[2]; out[3] = r[3];
+}
+
+// RayLib/CoordinateConversion.h:112-235 (concentric octahedral map)
+void ref_octa_to_dir(float u, float v, float out[3])
+{
+    Vector3f d = Utility::CocentricOctohedralToDirection(Vector2f(u, v));
+    out[0] = d[0]; out[1] = d[1]; out[2] = d[2];
+}
+void ref_dir_to_octa(const float d[3], float out[2])
+{
+    Vector2f st = Utility::DirectionToCocentricOctohedral(Vector3f(d[0], d[1], d[2]));
+    out[0] = st[0]; out[1] = st[1];
+}
+void ref_octa_wrap_int(int32_t x, int32_t y, int32_t X, int32_t Y, int32_t out[2])
+{
+    Vector2i w = Utility::CocentricOctohedralWrapInt(Vector2i(x, y), Vector2i(X, Y));
+    out[0] = w[0]; out[1] = w[1];
 }
 
 }  // extern "C"

@@ -112,6 +112,28 @@ def main():
     out.update(reflect_wi=rw, reflect_n=nq, reflect_out=rr, trinorm_out=tn, triaabb_out=ta,
                centroid_out=tc)
 
+    # Concentric octahedral map and its integer wrap (CoordinateConversion.h)
+    r2 = np.random.default_rng(20261203)
+    ouv = r2.random((m, 2)).astype(np.float32)
+    ouv[:8] = [[0, 0], [1, 1], [0.5, 0.5], [0, 1], [1, 0], [0.5, 0], [0.25, 0.75], [0.5, 1]]
+    odir = np.zeros((m, 3), np.float32)
+    for i in range(m):
+        R.ref_octa_to_dir(float(ouv[i, 0]), float(ouv[i, 1]), p(odir[i]))
+    dd = r2.normal(size=(m, 3)); dd = (dd / np.linalg.norm(dd, axis=1, keepdims=True)).astype(np.float32)
+    dd[:4] = [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0, -1, 0]]
+    duv = np.zeros((m, 2), np.float32)
+    for i in range(m):
+        R.ref_dir_to_octa(p(dd[i]), p(duv[i]))
+    wq = []
+    for X, Y in ((8, 8), (16, 8), (64, 64), (32, 16)):
+        for x in range(-6, X + 6):
+            for y in (-6, -1, 0, Y // 2, Y - 1, Y, Y + 5):
+                o2 = np.zeros(2, np.int32)
+                R.ref_octa_wrap_int(x, y, X, Y, p(o2))
+                wq.append((x, y, X, Y, int(o2[0]), int(o2[1])))
+    out.update(octa_uv=ouv, octa_dir=odir, octa_in_dir=dd, octa_out_uv=duv,
+               octa_wrap=np.array(wq, np.int32))
+
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -219,6 +219,115 @@ def c2_leg(N, ctx, torch):
             "cast_Mrays_per_s": round(m / cast / 1e6, 1), "cast_blocks_per_cu": best_bpc}
 
 
+def wfpg_leg(N, ctx, torch, cpu_bins: int):
+    """The guided tracer's per-bin field stage (filter, piecewise-constant
+    distribution, one guided sample per ray): 4096 bins of 64x64 fields (the
+    reference's first-bounce size) and one ray per 1080p pixel spread evenly
+    over the bins; then the smaller fields of deeper bounces (32, 16, 8).
+    Kernel time from the library's stream events, mean of 5 launches.
+    Algorithmic bytes: the fields (4 B per texel), the bin offsets, and per
+    ray its id, two uniforms, the sampled (u,v) and pdf (24 B)."""
+    rng = np.random.default_rng(5)
+    bins, n = 4096, 1920 * 1080
+    bin_of = np.sort(rng.integers(0, bins, n))
+    offsets = torch.from_numpy(np.searchsorted(bin_of, np.arange(bins + 1)).astype(np.int32)).cuda()
+    ids = torch.from_numpy(rng.permutation(n).astype(np.int32)).cuda()
+    xi = torch.rand((n, 2), device="cuda")
+    uv = torch.empty((n, 2), device="cuda")
+    pdf = torch.empty(n, device="cuda")
+    out = {"workload": f"{bins} bins, {n} rays (one per 1080p pixel), Gaussian alpha 1, (u,v) + pdf per ray"}
+    ctx.profile(True)
+    for side in (64, 32, 16, 8):
+        fields = (torch.rand((bins, side, side), device="cuda") * 4.0).contiguous()
+        torch.cuda.synchronize()
+        ctx.wfpg_sample_fields(fields, side, side, bins, offsets, ids, n, xi, uv, pdf)
+        ctx.sync()
+        ctx.profile_reset()
+        for _ in range(5):
+            ctx.wfpg_sample_fields(fields, side, side, bins, offsets, ids, n, xi, uv, pdf)
+        ctx.sync()
+        ms, cnt = ctx.profile_get("wfpg_fields")
+        per = ms / max(cnt, 1)
+        nbytes = bins * side * side * 4 + (bins + 1) * 4 + n * 24
+        out[f"field_{side}"] = {"ms_per_launch": round(per, 4), "Mrays_per_s": round(n / (per / 1e3) / 1e6, 1),
+                                "algorithmic_bytes": nbytes,
+                                "achieved_GBs": round(nbytes / (per / 1e3) / 1e9, 1),
+                                "frac_of_hbm_peak": round(nbytes / (per / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"wfpg fields {side}x{side}: {per * 1e3:.1f} us per launch ({nbytes / 1e6:.1f} MB)")
+        if side == 64:
+            # the same launch with each bin's rays stored contiguously (ids in
+            # storage order): the per-ray gathers and scatters become coalesced
+            seq = torch.arange(n, dtype=torch.int32, device="cuda")
+            torch.cuda.synchronize()
+            ctx.profile_reset()
+            for _ in range(5):
+                ctx.wfpg_sample_fields(fields, side, side, bins, offsets, seq, n, xi, uv, pdf)
+            ctx.sync()
+            ms, cnt = ctx.profile_get("wfpg_fields")
+            per = ms / max(cnt, 1)
+            out["field_64_rays_in_bin_order"] = {
+                "ms_per_launch": round(per, 4), "achieved_GBs": round(nbytes / (per / 1e3) / 1e9, 1),
+                "frac_of_hbm_peak": round(nbytes / (per / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+            log(f"wfpg fields 64x64, rays in bin order: {per * 1e3:.1f} us per launch")
+            del seq
+    ctx.profile(False)
+    if cpu_bins:
+        # the oracle on the same kind of input: a few 64x64 bins, their rays
+        from oracle import oracle as O
+        f = rng.gamma(0.6, 2.0, (cpu_bins, 64, 64)).astype(np.float32)
+        per_bin = n // bins
+        off = (np.arange(cpu_bins + 1) * per_bin).astype(np.uint32)
+        rid = np.arange(cpu_bins * per_bin, dtype=np.uint32)
+        x = rng.random((cpu_bins * per_bin, 2)).astype(np.float32)
+        t0 = time.perf_counter()
+        O.wfpg_sample_fields(f, off, rid, x, alpha=1.0)
+        dt = time.perf_counter() - t0
+        out["cpu_oracle"] = {"bins": cpu_bins, "rays": int(cpu_bins * per_bin), "threads": 1,
+                             "ms_per_bin": round(dt * 1e3 / cpu_bins, 3),
+                             "ms_for_4096_bins_scaled": round(dt * 1e3 / cpu_bins * bins, 1)}
+    return out
+
+
+def voxel_leg(N, ctx, sc, torch):
+    """The guiding octree's geometry: voxelise the frame's 1,966,080
+    triangles at depth 10 (1024^3 leaves; the reference's cube: scene bounds
+    padded by one leaf), then build the octree over the records.  Mean of 3
+    (count pass, scan, write pass; octree build)."""
+    tris = np.ascontiguousarray(sc["tris"], np.float32).reshape(-1, 9)
+    depth = 10
+    p = tris.reshape(-1, 3)
+    lo, hi = p.min(0), p.max(0)
+    w = np.float32((hi - lo).max())
+    pad = np.float32(w / np.float32(1 << depth))
+    cube = [float(lo[0] - pad), float(lo[1] - pad), float(lo[2] - pad), float(w + np.float32(2) * pad)]
+    d_t = torch.from_numpy(tris.reshape(-1)).cuda()
+    n = len(tris)
+    torch.cuda.synchronize()
+    count = ctx.svo_voxelize(d_t, n, cube, depth)
+    codes = torch.empty(count, dtype=torch.int32, device="cuda")
+    svo = N.Svo(ctx)
+    ctx.svo_voxelize(d_t, n, cube, depth, codes, count)
+    svo.build(codes, count, cube, depth)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.svo_voxelize(d_t, n, cube, depth, codes, count)
+    ctx.sync()
+    vox = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    for _ in range(3):
+        svo.build(codes, count, cube, depth)
+    ctx.sync()
+    build = (time.perf_counter() - t0) / 3
+    nodes = svo.view().nodes
+    svo.close()
+    log(f"voxelise: {n} triangles -> {count} records in {vox * 1e3:.2f} ms; octree of {nodes} nodes "
+        f"in {build * 1e3:.2f} ms")
+    return {"workload": f"{n} triangles of the frame's scene, depth {depth}", "records": int(count),
+            "octree_nodes": int(nodes), "voxelize_ms": round(vox * 1e3, 3), "octree_build_ms": round(build * 1e3, 3),
+            "Mtris_per_s": round(n / vox / 1e6, 1)}
+
+
 def instanced_leg(N, ctx, img, torch):
     """The sphere grid as a two-level scene (16 displaced-sphere meshes,
     4096 instances under y rotations, same camera, triangle count and
@@ -406,6 +515,8 @@ def main():
     ap.add_argument("--no-nee", action="store_true", help="skip the next-event-estimation leg")
     ap.add_argument("--no-instanced", action="store_true", help="skip the two-level (instanced) leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the LBVH build + ray cast leg")
+    ap.add_argument("--no-wfpg", action="store_true", help="skip the guided-path field stage leg")
+    ap.add_argument("--no-voxel", action="store_true", help="skip the voxeliser + octree build leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving leg requests (0 = skip)")
     ap.add_argument("--rate", type=float, default=40.0, help="serving leg arrivals per second")
     ap.add_argument("--serving-passes", default="serial,saturated,realtime",
@@ -551,6 +662,12 @@ def main():
     c2 = None
     if not args.no_c2:
         c2 = c2_leg(N, ctx, torch)
+    wfpg = None
+    if not args.no_wfpg:
+        wfpg = wfpg_leg(N, ctx, torch, 0 if args.no_cpu else 24)
+    voxel = None
+    if not args.no_voxel:
+        voxel = voxel_leg(N, ctx, sc, torch)
     serving = None
     if args.requests > 0:
         serving = serving_leg(N, ctx, torch, args.requests, args.rate,
@@ -596,6 +713,8 @@ def main():
             "nee": nee,
             "instanced": instanced,
             "c2": c2,
+            "wfpg_fields": wfpg,
+            "voxel_octree": voxel,
             "kernel_ms_per_frame": {"trace": round(trace_ms / args.steps, 3),
                                     "shade": round(shade_ms / args.steps, 3),
                                     "partition": round((sort_ms + hist_ms) / args.steps, 3)},

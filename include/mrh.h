@@ -499,6 +499,25 @@ int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes
                          uint32_t bin_ray_count, uint32_t* d_bins, uint32_t* d_order, uint32_t* d_part_offsets,
                          uint32_t* d_part_bins, uint32_t* d_part_count);
 
+/* Geometry of the reference's guiding octree: its triangle voxeliser
+ * (TriFunctions::Voxelize, GPUPrimitiveTriangle.h:384-620, with
+ * EachPrimVoxelCount / VoxelizeSurfaces, GPUAcceleratorCommonKC.cuh:439-542,
+ * as AnisoSVOctreeCPU::Constrcut calls them, AnisoSVO.cu:657-760).  Each
+ * triangle (d_tris: n x 9 floats, world space) is projected onto the plane
+ * of its normal's dominant axis with the cube (lo xyz, size) mapped to
+ * [-1,1]^3, rasterised conservatively at the leaf resolution 2^depth (edges
+ * pushed out by half a pixel), and every covered pixel gives one record: the
+ * cell code (as mrh_svo_codes writes it at `depth`) of the leaf holding the
+ * triangle's plane point under the pixel centre.  Records come triangle by
+ * triangle, pixels row by row; shared edges give duplicates, which
+ * mrh_svo_build folds.  *out_count = the number of records; they are written
+ * to d_codes only when it holds them all (count <= capacity), so a call with
+ * d_codes NULL sizes the buffer.  The reference pads the scene bounds by one
+ * leaf on every side of its largest span to get the cube
+ * (AnisoSVO.cu:670-678).  Synchronises the context's stream. */
+int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, const float cube[4], uint32_t depth,
+                         uint32_t* d_codes, uint32_t capacity, uint32_t* out_count);
+
 /* Per-bin radiance fields of the reference's wavefront path guiding: the
  * field-to-sample half of KCGenAndSampleDistribution (WFPGTracerKC.cuh:
  * 1433-1545, with GenerateRadianceField's radiance query and filter,

@@ -16,6 +16,7 @@
 #include "mrh_bvh.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -354,6 +355,175 @@ __global__ void k_svo_blur(uint32_t nodes, const float* __restrict__ mean, float
     }
 }
 
+
+// ---------------------------------------------------------------- voxeliser
+// The geometry of the reference's guiding octree (TriFunctions::Voxelize,
+// GPUPrimitiveTriangle.h:384-620): each triangle is projected onto the plane
+// of its normal's dominant axis (the cube mapped to [-1,1]^3, the dominant
+// axis turned to +z by the exact axis permutation that the reference's
+// quarter- or half-turn quaternion stands for), rasterised conservatively at
+// the leaf resolution (edges pushed out by half a pixel along their L1
+// normal), and every covered pixel emits the leaf holding the triangle's
+// plane point under the pixel centre.  One wave per triangle: the pixels of
+// its rectangle 64 at a time, row by row, coverage by ballot; a count pass,
+// a scan, and a write pass that places each record at its triangle's offset
+// plus its rank, so records come out triangle by triangle, row by row.
+struct VoxSetup {
+    V3 p[3];
+    float q[3][2];
+    float c0[2], ec0[2], ec1[2], denom_c;
+    float e0[2], e1[2], denom;
+    int x0, x1, y0, y1;
+};
+
+__device__ inline void vox_setup(const float* __restrict__ tri, const float cube[4], uint32_t R, VoxSetup& s)
+{
+    for (int k = 0; k < 3; k++) s.p[k] = v3(tri[3 * k], tri[3 * k + 1], tri[3 * k + 2]);
+    const float ea[3] = {s.p[1].x - s.p[0].x, s.p[1].y - s.p[0].y, s.p[1].z - s.p[0].z};
+    const float eb[3] = {s.p[2].x - s.p[0].x, s.p[2].y - s.p[0].y, s.p[2].z - s.p[0].z};
+    float n[3] = {ea[1] * eb[2] - ea[2] * eb[1], ea[2] * eb[0] - ea[0] * eb[2], ea[0] * eb[1] - ea[1] * eb[0]};
+    const float inv = 1.0f / sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    n[0] *= inv;
+    n[1] *= inv;
+    n[2] *= inv;
+    int dom = 0;
+    float best = fabsf(n[0]);
+    if (fabsf(n[1]) > best) {
+        dom = 1;
+        best = fabsf(n[1]);
+    }
+    if (fabsf(n[2]) > best) dom = 2;
+    const bool neg = signbit(n[dom]);
+    const float P[3][3] = {{s.p[0].x, s.p[0].y, s.p[0].z}, {s.p[1].x, s.p[1].y, s.p[1].z},
+                           {s.p[2].x, s.p[2].y, s.p[2].z}};
+    for (int k = 0; k < 3; k++) {
+        float ndc[3];
+        for (int a = 0; a < 3; a++) {
+            const float lo = cube[a], hi = cube[a] + cube[3];
+            const float sc = 2.0f / (hi - lo), tr = -((hi + lo) / (hi - lo));
+            ndc[a] = sc * P[k][a] + tr;
+        }
+        float u, v;
+        if (dom == 0) {
+            u = neg ? ndc[2] : -ndc[2];
+            v = ndc[1];
+        } else if (dom == 1) {
+            u = ndc[0];
+            v = neg ? ndc[2] : -ndc[2];
+        } else {
+            u = neg ? -ndc[0] : ndc[0];
+            v = ndc[1];
+        }
+        s.q[k][0] = u;
+        s.q[k][1] = v;
+    }
+    float mn[2] = {s.q[0][0], s.q[0][1]}, mx[2] = {s.q[0][0], s.q[0][1]};
+    for (int k = 1; k < 3; k++)
+        for (int a = 0; a < 2; a++) {
+            mn[a] = fminf(mn[a], s.q[k][a]);
+            mx[a] = fmaxf(mx[a], s.q[k][a]);
+        }
+    const float Rf = (float)R;
+    int r[4] = {(int)floorf((0.5f + 0.5f * mn[0]) * Rf), (int)ceilf((0.5f + 0.5f * mx[0]) * Rf),
+                (int)floorf((0.5f + 0.5f * mn[1]) * Rf), (int)ceilf((0.5f + 0.5f * mx[1]) * Rf)};
+    for (int i = 0; i < 4; i++) r[i] = r[i] < 0 ? 0 : (r[i] > (int)R ? (int)R : r[i]);
+    s.x0 = r[0];
+    s.x1 = r[1];
+    s.y0 = r[2];
+    s.y1 = r[3];
+    const float hp = 1.0f / Rf;
+    float pl[3][3];
+    for (int k = 0; k < 3; k++) {
+        const int k1 = (k + 1) % 3;
+        const float dx = s.q[k1][0] - s.q[k][0], dy = s.q[k1][1] - s.q[k][1];
+        pl[k][0] = dy;
+        pl[k][1] = -dx;
+        pl[k][2] = dx * s.q[k][1] - dy * s.q[k][0];
+        pl[k][2] -= hp * fabsf(pl[k][0]) + hp * fabsf(pl[k][1]);
+    }
+    const int pair[3][2] = {{0, 2}, {0, 1}, {1, 2}};
+    float cc[3][2];
+    for (int k = 0; k < 3; k++) {
+        const float* A = pl[pair[k][0]];
+        const float* B = pl[pair[k][1]];
+        const float X = A[1] * B[2] - A[2] * B[1], Y = A[2] * B[0] - A[0] * B[2], Z = A[0] * B[1] - A[1] * B[0];
+        cc[k][0] = X / Z;
+        cc[k][1] = Y / Z;
+    }
+    s.c0[0] = cc[0][0];
+    s.c0[1] = cc[0][1];
+    s.ec0[0] = cc[1][0] - cc[0][0];
+    s.ec0[1] = cc[1][1] - cc[0][1];
+    s.ec1[0] = cc[2][0] - cc[0][0];
+    s.ec1[1] = cc[2][1] - cc[0][1];
+    s.denom_c = 1.0f / (s.ec0[0] * s.ec1[1] - s.ec1[0] * s.ec0[1]);
+    s.e0[0] = s.q[1][0] - s.q[0][0];
+    s.e0[1] = s.q[1][1] - s.q[0][1];
+    s.e1[0] = s.q[2][0] - s.q[0][0];
+    s.e1[1] = s.q[2][1] - s.q[0][1];
+    s.denom = 1.0f / (s.e0[0] * s.e1[1] - s.e1[0] * s.e0[1]);
+}
+
+// GPUPrimitiveTriangle.h:548-612: conservative barycentrics of the pixel
+// centre in range -> the leaf of the original triangle's plane point there
+__device__ inline bool vox_pixel(const VoxSetup& s, const float cube[4], uint32_t R, int x, int y, uint32_t& code)
+{
+    const float dp = 2.0f / (float)R;
+    const float px = ((float)x + 0.5f) * dp - 1.0f, py = ((float)y + 0.5f) * dp - 1.0f;
+    const float g0 = px - s.c0[0], g1 = py - s.c0[1];
+    const float b = (g0 * s.ec1[1] - s.ec1[0] * g1) * s.denom_c;
+    const float c = (s.ec0[0] * g1 - g0 * s.ec0[1]) * s.denom_c;
+    const float a = 1.0f - b - c;
+    if (!(b >= 0.0f && b <= 1.0f && c >= 0.0f && c <= 1.0f && a >= 0.0f && a <= 1.0f)) return false;
+    const float h0 = px - s.q[0][0], h1 = py - s.q[0][1];
+    const float B = (h0 * s.e1[1] - s.e1[0] * h1) * s.denom;
+    const float C = (s.e0[0] * h1 - h0 * s.e0[1]) * s.denom;
+    const float A = 1.0f - B - C;
+    const float pos[3] = {s.p[0].x * A + s.p[1].x * B + s.p[2].x * C, s.p[0].y * A + s.p[1].y * B + s.p[2].y * C,
+                          s.p[0].z * A + s.p[1].z * B + s.p[2].z * C};
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        const float f = ((pos[k] - cube[k]) / cube[3]) * (float)R;
+        q[k] = !(f > 0.0f) ? 0u : (f >= (float)(R - 1) ? R - 1 : (uint32_t)f);
+    }
+    code = expand10(q[0]) | (expand10(q[1]) << 1) | (expand10(q[2]) << 2);
+    return true;
+}
+
+// offsets == nullptr: out[t] = records of triangle t; else out[offsets[t] +
+// rank] = the records
+__global__ __launch_bounds__(256) void k_voxelize(const float* __restrict__ tris, uint32_t n, float4 cube4,
+                                                  uint32_t depth, const uint32_t* __restrict__ offsets,
+                                                  uint32_t* __restrict__ out, unsigned long long* __restrict__ total)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t R = 1u << depth;
+    const float cube[4] = {cube4.x, cube4.y, cube4.z, cube4.w};
+    const uint64_t below = (1ull << lane) - 1ull;
+    unsigned long long wave_total = 0ull;   // one atomic per wave, not per triangle
+    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += nw) {
+        VoxSetup s;
+        vox_setup(tris + 9 * (size_t)t, cube, R, s);
+        const int w = s.x1 - s.x0, h = s.y1 - s.y0;
+        const uint32_t area = (w > 0 && h > 0) ? (uint32_t)(w * h) : 0u;
+        uint32_t run = offsets ? offsets[t] : 0u;
+        for (uint32_t base = 0; base < area; base += 64u) {
+            const uint32_t k = base + lane;
+            uint32_t code = 0u;
+            bool hit = false;
+            if (k < area) hit = vox_pixel(s, cube, R, s.x0 + (int)(k % (uint32_t)w), s.y0 + (int)(k / (uint32_t)w), code);
+            const uint64_t m = __ballot(hit);
+            if (offsets && hit) out[run + (uint32_t)__popcll(m & below)] = code;
+            run += (uint32_t)__popcll(m);
+        }
+        if (!offsets && lane == 0) {
+            out[t] = run;
+            wave_total += run;
+        }
+    }
+    if (!offsets && lane == 0 && wave_total) atomicAdd(total, wave_total);
+}
 }  // namespace
 
 SvoView svo_view(const mrh_svo* s)
@@ -788,6 +958,43 @@ int32_t svo_bin_rays(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, uint32
 using namespace mrh;
 
 extern "C" {
+
+int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, const float cube[4], uint32_t depth,
+                         uint32_t* d_codes, uint32_t capacity, uint32_t* out_count)
+{
+    if (!ctx || !cube || !out_count) return MRH_ERR_INVALID;
+    *out_count = 0;
+    if (depth < 1 || depth > 10) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_voxelize: depth must be in [1, 10]");
+    for (int a = 0; a < 4; a++)
+        if (!std::isfinite(cube[a])) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_voxelize: cube must be finite");
+    if (!(cube[3] > 0.0f)) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_voxelize: cube size must be > 0");
+    if (n == 0) return MRH_OK;
+    if (!d_tris) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_voxelize: null triangles");
+    if (n >= (1u << 30)) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_voxelize: at most 2^30 - 1 triangles");
+    // counts[n + 1] (scanned in place into offsets), scan scratch, a 64-bit total
+    const size_t words = 2 + (size_t)n + 1 + scan_tmp_words(n + 1);
+    void* mem = nullptr;
+    MRH_TRY(scratch(ctx, SLOT_VOXEL, words * 4, &mem));
+    unsigned long long* total = (unsigned long long*)mem;
+    uint32_t* counts = (uint32_t*)mem + 2;
+    uint32_t* tmp = counts + n + 1;
+    const float4 c4 = make_float4(cube[0], cube[1], cube[2], cube[3]);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(div_up(n, 4), (uint64_t)ctx->num_cus * 8u);
+    MRH_TRY(fill_u32(ctx, (uint32_t*)mem, 0u, 2));
+    MRH_TRY(fill_u32(ctx, counts + n, 0u, 1));
+    MRH_TRY(launch(ctx, "voxelize_count", dim3(grid), dim3(256), 0, k_voxelize, d_tris, n, c4, depth,
+                   (const uint32_t*)nullptr, counts, total));
+    unsigned long long h_total = 0;
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(&h_total, total, 8, hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h_total > 0xFFFFFFFFull)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_voxelize: %llu voxel records exceed 2^32 - 1", h_total);
+    *out_count = (uint32_t)h_total;
+    if (!d_codes || h_total > capacity || h_total == 0) return MRH_OK;   // a size query
+    MRH_TRY(scan_exclusive(ctx, counts, counts, n + 1, tmp));
+    return launch(ctx, "voxelize", dim3(grid), dim3(256), 0, k_voxelize, d_tris, n, c4, depth,
+                  (const uint32_t*)counts, d_codes, (unsigned long long*)nullptr);
+}
 
 int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, uint32_t n, uint32_t min_level,
                          uint32_t bin_ray_count, uint32_t* d_bins, uint32_t* d_order, uint32_t* d_part_offsets,

@@ -31,6 +31,7 @@ enum MrhSlot : int {
     SLOT_FILTER,
     SLOT_SAMPLES,
     SLOT_BOUNDS_PART,
+    SLOT_VOXEL,
     SLOT_COUNT
 };
 

@@ -4,10 +4,9 @@
 // per ray (KCGenAndSampleDistribution, WFPGTracerKC.cuh:1433-1545).
 //
 // One 256-thread workgroup per bin (grid-stride over bins).  The field lives
-// in LDS for the whole bin: rows padded to 65 floats so that the
-// one-lane-per-row prefix walks hit 64 distinct banks.  Float operation
-// order is fixed and launch-independent: taps accumulate with fma from -5 to
-// 5, prefix sums run left to right, and every ray brings its own uniforms.
+// in LDS for the whole bin, rows padded by one float.  Float operation order
+// is fixed and launch-independent: taps accumulate with fma from -5 to 5,
+// prefix sums run left to right, and every ray brings its own uniforms.
 #include "mrh_guide.hpp"
 
 #include <cmath>
@@ -15,9 +14,7 @@
 namespace mrh {
 namespace {
 
-constexpr uint32_t WFPG_BLOCK = 256;
-constexpr uint32_t WFPG_MAX = 64;           // texels along a field side
-constexpr uint32_t WFPG_LS = WFPG_MAX + 1;  // LDS row stride
+constexpr int WFPG_BLOCK = 256;
 constexpr int WFPG_HALF = 5;                // 11 taps
 
 struct WfpgTaps {
@@ -63,146 +60,200 @@ __device__ inline void octa_fold(int x, int y, int X, int Y, int& ox, int& oy)
 // (BlockPWCDistribution2D::Sample with BinarySearchInBetween): c holds the
 // cdf at the END of each texel.  Found interval k (c[k] <= xi < c[k+1]) ->
 // k + frac + 1; below c[0] -> xi / c[0]; at or above the last entry ->
-// size - 1e-5.
-__device__ inline float pwc_axis(const float* c, int size, float xi)
+// size - 1e-5.  The interval is found as the count of entries <= xi, in
+// log2(size) fixed steps: for a non-decreasing cdf (non-negative radiance)
+// that is the one interval the reference's bisection stops at.
+template <int SIZE>
+__device__ inline float pwc_axis(const float* c, float xi)
 {
-    int lo = 0, hi = size - 2;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const float cm = c[mid], cn = c[mid + 1];
-        if (cm <= xi && cn > xi) return ((float)mid + (xi - cm) / (cn - cm)) + 1.0f;
-        if (xi < cm) hi = mid - 1;
-        else lo = mid + 1;
-    }
-    if (xi < c[size - 1]) return xi / c[0];
-    return (float)size - 1.0e-5f;
+    int pos = 0;
+#pragma unroll
+    for (int step = SIZE / 2; step >= 1; step >>= 1)
+        pos = (c[pos + step - 1] <= xi) ? pos + step : pos;
+    if (pos == SIZE - 1 && c[SIZE - 1] <= xi) pos = SIZE;   // the steps reach at most SIZE - 1
+    if (pos == 0) return xi / c[0];
+    if (pos == SIZE) return (float)SIZE - 1.0e-5f;
+    const float cm = c[pos - 1], cn = c[pos];
+    return ((float)(pos - 1) + (xi - cm) / (cn - cm)) + 1.0f;
 }
 
 // pdf of the texel under (u,v) times 1/(4 pi) (solid angle of the map);
 // coordinates outside [0,1) (or NaN) clamp to the border texels
-__device__ inline float pwc_pdf(const float* s_pdf, const float* s_py, uint32_t X, uint32_t Y, float u, float v)
+template <int X, int Y, int LS>
+__device__ inline float pwc_pdf(const float* s_pdf, const float* s_py, float u, float v)
 {
     const float fx = u * (float)X, fy = v * (float)Y;
     uint32_t ix = (fx > 0.0f) ? (uint32_t)fminf(fx, 65536.0f) : 0u;
     uint32_t iy = (fy > 0.0f) ? (uint32_t)fminf(fy, 65536.0f) : 0u;
-    ix = ix >= X ? X - 1 : ix;
-    iy = iy >= Y ? Y - 1 : iy;
-    const float p = s_pdf[iy * WFPG_LS + ix] * s_py[iy];
+    ix = ix >= (uint32_t)X ? X - 1 : ix;
+    iy = iy >= (uint32_t)Y ? Y - 1 : iy;
+    const float p = s_pdf[iy * LS + ix] * s_py[iy];
     return p * (0.25f * 0.318309886183790671538f);
 }
 
-__global__ __launch_bounds__(WFPG_BLOCK) void k_wfpg_fields(
-    const float* __restrict__ fields, uint32_t X, uint32_t Y, uint32_t lx, uint32_t bins, WfpgTaps taps,
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ray_ids, uint32_t ray_count,
-    const float2* __restrict__ xi, const float2* __restrict__ eval_uv, float2* __restrict__ out_uv,
-    float* __restrict__ out_dir, float* __restrict__ out_pdf, float* __restrict__ out_eval_pdf,
-    float* __restrict__ out_filtered)
+// One bin per workgroup iteration; X, Y compile-time so that texel
+// indexing is shifts and each row's prefix runs in registers.  Four waves
+// per SIMD (128 registers): a 64x64 field's LDS holds the CU to four
+// workgroups anyway.
+template <int X, int Y>
+__global__ __launch_bounds__(WFPG_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_wfpg_fields(
+    const float* __restrict__ fields, uint32_t bins, WfpgTaps taps, const uint32_t* __restrict__ offsets,
+    const uint32_t* __restrict__ ray_ids, uint32_t ray_count, const float2* __restrict__ xi,
+    const float2* __restrict__ eval_uv, float2* __restrict__ out_uv, float* __restrict__ out_dir,
+    float* __restrict__ out_pdf, float* __restrict__ out_eval_pdf, float* __restrict__ out_filtered)
 {
-    __shared__ float s_a[WFPG_MAX * WFPG_LS];  // field -> filtered field -> row pdfs
-    __shared__ float s_b[WFPG_MAX * WFPG_LS];  // u-pass result -> row cdfs
-    __shared__ float s_row[WFPG_MAX];          // row totals -> marginal pdf
-    __shared__ float s_cy[WFPG_MAX];           // marginal cdf
-    const uint32_t n = X * Y, tid = threadIdx.x, xm = X - 1;
-    const int Xi = (int)X, Yi = (int)Y;
+    constexpr int N = X * Y;
+    constexpr int LS = X + 1;            // padded row stride: lane-per-row walks hit distinct banks
+    __shared__ float s_a[Y * LS];        // field -> filtered field -> row pdfs
+    __shared__ float s_b[Y * LS];        // u-pass result -> row cdfs
+    __shared__ float s_row[Y];           // marginal pdf
+    __shared__ float s_cy[Y];            // marginal cdf
+    constexpr int PER = (N + WFPG_BLOCK - 1) / WFPG_BLOCK;   // texels per thread
+    // filter runs: K consecutive texels from one window of K + 10
+    constexpr int K = (N / WFPG_BLOCK > 8) ? 8 : ((N / WFPG_BLOCK > 1) ? N / WFPG_BLOCK : 1);
+    constexpr int RUNS = N / K;
+    static_assert(X % K == 0 && Y % K == 0, "filter runs tile the rows and columns");
+    const int tid = (int)threadIdx.x;
     const float dX = 1.0f / (float)X, dY = 1.0f / (float)Y;
+    // the next bin's field is loaded into registers while the current bin's
+    // rays are sampled
+    float pre[PER];
+    auto fetch = [&](uint32_t b) {
+        const float* src = fields + (size_t)b * N;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            const int p = i * WFPG_BLOCK + tid;
+            pre[i] = (p < N) ? src[p] : 0.0f;
+        }
+    };
+    if (blockIdx.x < bins) fetch(blockIdx.x);
     for (uint32_t bin = blockIdx.x; bin < bins; bin += gridDim.x) {
-        const float* src = fields + (size_t)bin * n;
+        // this thread's first ray of the bin, loaded before the field work
+        const uint32_t lo = offsets[bin], hi = offsets[bin + 1];
+        uint32_t r0 = 0xFFFFFFFFu;
+        float2 x0 = make_float2(0.0f, 0.0f);
+        if (lo + (uint32_t)tid < hi) {
+            r0 = ray_ids[lo + tid];
+            if (r0 < ray_count) x0 = xi[r0];
+        }
         // radiance query result: an empty texel keeps a small weight
         // (WFPGTracerKC.cuh:1316-1317)
-        for (uint32_t p = tid; p < n; p += WFPG_BLOCK) {
-            const float v = src[p];
-            s_a[(p >> lx) * WFPG_LS + (p & xm)] = (v == 0.0f) ? 1.0e-4f : v;
-        }
-        __syncthreads();
-        // filter along u (BlockTextureFilter.cuh:157-174)
-        for (uint32_t p = tid; p < n; p += WFPG_BLOCK) {
-            const int x = (int)(p & xm), y = (int)(p >> lx);
-            float acc = 0.0f;
 #pragma unroll
-            for (int j = -WFPG_HALF; j <= WFPG_HALF; j++) {
-                int qx = x + j, qy = y;
-                if (qx < 0 || qx >= Xi) octa_fold(qx, y, Xi, Yi, qx, qy);
-                acc = fmaf(s_a[qy * WFPG_LS + qx], taps.w[j + WFPG_HALF], acc);
-            }
-            s_b[y * WFPG_LS + x] = acc;
+        for (int i = 0; i < PER; i++) {
+            const int p = i * WFPG_BLOCK + tid;
+            if (p < N) s_a[(p / X) * LS + (p % X)] = (pre[i] == 0.0f) ? 1.0e-4f : pre[i];
         }
         __syncthreads();
-        // filter along v (:178-195)
-        for (uint32_t p = tid; p < n; p += WFPG_BLOCK) {
-            const int x = (int)(p & xm), y = (int)(p >> lx);
-            float acc = 0.0f;
+        // filter along u (BlockTextureFilter.cuh:157-174): runs of K texels
+        // of one row
+        for (int q = tid; q < RUNS; q += WFPG_BLOCK) {
+            const int y = q / (X / K), x0 = (q % (X / K)) * K;
+            float w[K + 2 * WFPG_HALF];
 #pragma unroll
-            for (int j = -WFPG_HALF; j <= WFPG_HALF; j++) {
-                int qx = x, qy = y + j;
-                if (qy < 0 || qy >= Yi) octa_fold(x, qy, Xi, Yi, qx, qy);
-                acc = fmaf(s_b[qy * WFPG_LS + qx], taps.w[j + WFPG_HALF], acc);
+            for (int i = 0; i < K + 2 * WFPG_HALF; i++) {
+                int qx = x0 - WFPG_HALF + i, qy = y;
+                if (qx < 0 || qx >= X) octa_fold(qx, y, X, Y, qx, qy);
+                w[i] = s_a[qy * LS + qx];
             }
-            s_a[y * WFPG_LS + x] = acc;
-            if (out_filtered) out_filtered[(size_t)bin * n + p] = acc;
+#pragma unroll
+            for (int o = 0; o < K; o++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 2 * WFPG_HALF + 1; j++) acc = fmaf(w[o + j], taps.w[j], acc);
+                s_b[y * LS + x0 + o] = acc;
+            }
         }
         __syncthreads();
-        // conditional distributions, one lane per row
-        // (GPUBlockPWCDistribution.cuh:107-156)
+        // filter along v (:178-195): runs of K texels of one column
+        for (int q = tid; q < RUNS; q += WFPG_BLOCK) {
+            const int x = q % X, y0 = (q / X) * K;
+            float w[K + 2 * WFPG_HALF];
+#pragma unroll
+            for (int i = 0; i < K + 2 * WFPG_HALF; i++) {
+                int qx = x, qy = y0 - WFPG_HALF + i;
+                if (qy < 0 || qy >= Y) octa_fold(x, qy, X, Y, qx, qy);
+                w[i] = s_b[qy * LS + qx];
+            }
+#pragma unroll
+            for (int o = 0; o < K; o++) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 2 * WFPG_HALF + 1; j++) acc = fmaf(w[o + j], taps.w[j], acc);
+                s_a[(y0 + o) * LS + x] = acc;
+                if (out_filtered) out_filtered[(size_t)bin * N + (y0 + o) * X + x] = acc;
+            }
+        }
+        __syncthreads();
+        // conditional distributions, one lane of the first wave per row
+        // (GPUBlockPWCDistribution.cuh:107-156): the row total first, then
+        // the same left-to-right prefix again while the row is normalised
+        float m = 0.0f;                  // this lane's row total * dY
         if (tid < Y) {
-            float* ra = s_a + tid * WFPG_LS;
-            float* rb = s_b + tid * WFPG_LS;
+            float* ra = s_a + tid * LS;
+            float* rb = s_b + tid * LS;
             float s = 0.0f;
-            for (uint32_t x = 0; x < X; x++) {
-                s += ra[x];
-                rb[x] = s;
-            }
-            s_row[tid] = s;
+#pragma unroll 16
+            for (int x = 0; x < X; x++) s += ra[x];
             const float sdx = s * dX;
-            if (sdx != 0.0f) {
-                const float inv = 1.0f / sdx;
-                for (uint32_t x = 0; x < X; x++) {
-                    ra[x] = ra[x] * inv;
-                    rb[x] = (rb[x] * dX) * inv;
-                }
-            } else {
-                for (uint32_t x = 0; x < X; x++) rb[x] = rb[x] * dX;
-            }
-        }
-        __syncthreads();
-        // marginal over rows (:158-199)
-        if (tid == 0) {
+            const bool norm = (sdx != 0.0f);
+            const float inv = norm ? 1.0f / sdx : 1.0f;
             float c = 0.0f;
-            for (uint32_t y = 0; y < Y; y++) {
-                c += s_row[y] * dY;
-                s_cy[y] = c;
+#pragma unroll 16
+            for (int x = 0; x < X; x++) {
+                const float v = ra[x];
+                c += v;
+                if (norm) {
+                    ra[x] = v * inv;
+                    rb[x] = (c * dX) * inv;
+                } else {
+                    rb[x] = c * dX;
+                }
             }
-            const float inv = (c != 0.0f) ? 1.0f / c : 1.0f;
-            for (uint32_t y = 0; y < Y; y++) {
-                float p = s_row[y] * dY, cc = s_cy[y];
-                if (c != 0.0f) {
+            m = s * dY;
+        }
+        // marginal over rows (:158-199) in the first wave: lane y adds the
+        // row terms 0..y in order, so it holds exactly the left-to-right prefix
+        if (tid < 64) {
+            float c = 0.0f;
+#pragma unroll
+            for (int k = 0; k < Y; k++) {
+                const float mk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), k));
+                if (k <= tid) c += mk;
+            }
+            const float T = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(c), Y - 1));
+            if (tid < Y) {
+                float p = m, cc = c;
+                if (T != 0.0f) {
+                    const float inv = 1.0f / T;
                     p = p * inv;
                     cc = cc * inv;
                 }
-                s_row[y] = p * (float)Y;
-                s_cy[y] = cc;
+                s_row[tid] = p * (float)Y;
+                s_cy[tid] = cc;
             }
         }
         __syncthreads();
+        if (bin + gridDim.x < bins) fetch(bin + gridDim.x);
         // one guided sample per ray of the bin (WFPGTracerKC.cuh:1466-1533)
-        const uint32_t lo = offsets[bin], hi = offsets[bin + 1];
         for (uint32_t k = lo + tid; k < hi; k += WFPG_BLOCK) {
-            const uint32_t r = ray_ids[k];
+            const bool first = (k == lo + (uint32_t)tid);
+            const uint32_t r = first ? r0 : ray_ids[k];
             if (r >= ray_count) continue;
-            const float2 x2 = xi[r];
+            const float2 x2 = first ? x0 : xi[r];
             float u, v;
             if (s_cy[Y - 1] == 0.0f) {
                 u = x2.x;
                 v = x2.y;
             } else {
-                const float iy = pwc_axis(s_cy, Yi, x2.y);
+                const float iy = pwc_axis<Y>(s_cy, x2.y);
                 int row = (int)iy;
-                row = row >= Yi ? Yi - 1 : (row < 0 ? 0 : row);
-                const float ix = pwc_axis(s_b + row * WFPG_LS, Xi, x2.x);
+                row = row >= Y ? Y - 1 : (row < 0 ? 0 : row);
+                const float ix = pwc_axis<X>(s_b + row * LS, x2.x);
                 u = ix * dX;
                 v = iy * dY;
             }
             out_uv[r] = make_float2(u, v);
-            out_pdf[r] = pwc_pdf(s_a, s_row, X, Y, u, v);
+            out_pdf[r] = pwc_pdf<X, Y, LS>(s_a, s_row, u, v);
             if (out_dir) {
                 const V3 d = octa_to_dir(u, v);   // z up; the path uses y up
                 out_dir[3 * (size_t)r] = d.y;
@@ -211,10 +262,41 @@ __global__ __launch_bounds__(WFPG_BLOCK) void k_wfpg_fields(
             }
             if (eval_uv) {
                 const float2 e = eval_uv[r];
-                out_eval_pdf[r] = pwc_pdf(s_a, s_row, X, Y, e.x, e.y);
+                out_eval_pdf[r] = pwc_pdf<X, Y, LS>(s_a, s_row, e.x, e.y);
             }
         }
         __syncthreads();
+    }
+}
+
+// Workgroups per CU the kernel's LDS leaves room for (at most 8).
+template <int X, int Y>
+constexpr uint32_t wfpg_blocks_per_cu()
+{
+    const uint32_t lds = (uint32_t)(2 * Y * (X + 1) + 2 * Y) * 4u;
+    const uint32_t b = (160u * 1024u) / lds;
+    return b > 8u ? 8u : b;
+}
+
+template <int X, int Y>
+int32_t launch_wfpg(mrh_context* ctx, const mrh_wfpg_fields_desc* d)
+{
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(d->bin_count,
+                                                        (uint64_t)ctx->num_cus * wfpg_blocks_per_cu<X, Y>());
+    return launch(ctx, "wfpg_fields", dim3(grid), dim3(WFPG_BLOCK), 0, k_wfpg_fields<X, Y>, d->fields,
+                  d->bin_count, gauss_taps(d->filter_alpha), d->bin_offsets, d->ray_ids, d->ray_count,
+                  (const float2*)d->xi, (const float2*)d->eval_uv, (float2*)d->out_uv, d->out_dir, d->out_pdf,
+                  d->out_eval_pdf, d->out_filtered);
+}
+
+template <int X>
+int32_t launch_wfpg_x(mrh_context* ctx, const mrh_wfpg_fields_desc* d)
+{
+    switch (d->field_y) {
+    case 8: return launch_wfpg<X, 8>(ctx, d);
+    case 16: return launch_wfpg<X, 16>(ctx, d);
+    case 32: return launch_wfpg<X, 32>(ctx, d);
+    default: return launch_wfpg<X, 64>(ctx, d);
     }
 }
 
@@ -239,10 +321,10 @@ int32_t mrh_wfpg_sample_fields(mrh_context* ctx, const mrh_wfpg_fields_desc* d)
         return fail(ctx, MRH_ERR_INVALID, "mrh_wfpg_sample_fields: null fields, bin_offsets or ray_ids");
     if (d->ray_count && (!d->xi || !d->out_uv || !d->out_pdf))
         return fail(ctx, MRH_ERR_INVALID, "mrh_wfpg_sample_fields: null xi, out_uv or out_pdf");
-    const uint32_t lx = (uint32_t)__builtin_ctz(d->field_x);
-    const uint32_t grid = std::min<uint64_t>(d->bin_count, (uint64_t)ctx->num_cus * 4u);
-    return launch(ctx, "wfpg_fields", dim3(grid), dim3(WFPG_BLOCK), 0, k_wfpg_fields, d->fields, d->field_x,
-                  d->field_y, lx, d->bin_count, gauss_taps(d->filter_alpha), d->bin_offsets, d->ray_ids,
-                  d->ray_count, (const float2*)d->xi, (const float2*)d->eval_uv, (float2*)d->out_uv, d->out_dir,
-                  d->out_pdf, d->out_eval_pdf, d->out_filtered);
+    switch (d->field_x) {
+    case 8: return launch_wfpg_x<8>(ctx, d);
+    case 16: return launch_wfpg_x<16>(ctx, d);
+    case 32: return launch_wfpg_x<32>(ctx, d);
+    default: return launch_wfpg_x<64>(ctx, d);
+    }
 }

@@ -208,6 +208,7 @@ _SIGS = {
     "mrh_svo_view_get": (_I32, [_P, C.POINTER(SvoView)]),
     "mrh_svo_bin_rays": (_I32, [_P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
     "mrh_wfpg_sample_fields": (_I32, [_P, C.POINTER(WfpgFieldsDesc)]),
+    "mrh_svo_voxelize": (_I32, [_P, _P, _U32, C.POINTER(_F * 4), _U32, _P, _U32, C.POINTER(_U32)]),
     "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                  C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
     "mrh_server_create": (_I32, [_P, C.POINTER(ServerParams), C.POINTER(_P)]),
@@ -328,6 +329,15 @@ class Context:
 
     def filter_resolve(self, d_weighted, d_weights, width: int, height: int, d_image) -> None:
         self.check(lib().mrh_filter_resolve(self.h, ptr(d_weighted), ptr(d_weights), width, height, ptr(d_image)))
+
+    def svo_voxelize(self, d_tris, n: int, cube, depth: int, d_codes=None, capacity: int = 0) -> int:
+        """mrh_svo_voxelize: the number of voxel records of n triangles; the
+        records go to d_codes when it holds them all."""
+        c = (_F * 4)(*[float(x) for x in cube])
+        cnt = _U32(0)
+        self.check(lib().mrh_svo_voxelize(self.h, ptr(d_tris), n, C.byref(c), depth, ptr(d_codes), capacity,
+                                          C.byref(cnt)))
+        return int(cnt.value)
 
     def wfpg_sample_fields(self, d_fields, field_x: int, field_y: int, bin_count: int, d_bin_offsets, d_ray_ids,
                            ray_count: int, d_xi, d_out_uv, d_out_pdf, alpha: float = 1.0, d_eval_uv=None,

@@ -1,0 +1,180 @@
+/*
+ * om_vox.c — oracle for the reference's triangle voxeliser, the geometry
+ * source of its path-guiding octree (TriFunctions::Voxelize,
+ * TracerCUDA/GPUPrimitiveTriangle.h:384-620, driven by KCGetVoxelCount /
+ * KCVoxelizePrims, GPUAcceleratorCommonKC.cuh:439-542, and
+ * AnisoSVOctreeCPU::Constrcut, AnisoSVO.cu:657-760).
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Per triangle: project onto the plane of the dominant axis of its normal
+ * (the octree cube mapped to [-1,1]^3, then turned so that the dominant
+ * axis faces +z), rasterise conservatively at the leaf resolution R (each
+ * edge pushed out by half a pixel along its normal's L1 direction), and for
+ * every covered pixel emit the leaf voxel that holds the triangle's plane
+ * point under the pixel centre (barycentrics of the original triangle,
+ * unclamped; voxel index clamped to [0, R-1]).  Records come out triangle by
+ * triangle, pixels row by row.
+ *
+ * The reference turns the projected points with a float quaternion of angle
+ * pi/2 or pi; here the turn is the exact axis permutation it approximates
+ * (the pixel grid is symmetric under the sign flips, so only which two axes
+ * span the raster matters).  The two agree except within float rounding of a
+ * pixel edge.  Products are not fused.
+ */
+#include <math.h>
+#include <stdlib.h>
+
+#include "oracle.h"
+
+static uint32_t vox_expand10(uint32_t v)
+{
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+typedef struct {
+    float p[3][3];             /* world positions */
+    float q[3][2];             /* projected 2D positions */
+    float c0[2], ec0[2], ec1[2], denom_c;   /* conservative triangle */
+    float e0[2], e1[2], denom;              /* original triangle */
+    int32_t x0, x1, y0, y1;                 /* pixel rectangle [x0,x1) x [y0,y1) */
+} om_vox_setup;
+
+/* GPUPrimitiveTriangle.h:431-540 */
+static void vox_setup(const float* tri, const float cube[4], uint32_t R, om_vox_setup* s)
+{
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) s->p[k][a] = tri[3 * k + a];
+    /* Triangle::Normal: normalised cross of the edges (RayLib/Triangle.h:203-210) */
+    const float ea[3] = {s->p[1][0] - s->p[0][0], s->p[1][1] - s->p[0][1], s->p[1][2] - s->p[0][2]};
+    const float eb[3] = {s->p[2][0] - s->p[0][0], s->p[2][1] - s->p[0][1], s->p[2][2] - s->p[0][2]};
+    float n[3] = {ea[1] * eb[2] - ea[2] * eb[1], ea[2] * eb[0] - ea[0] * eb[2], ea[0] * eb[1] - ea[1] * eb[0]};
+    const float inv = 1.0f / sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    n[0] *= inv;
+    n[1] *= inv;
+    n[2] *= inv;
+    /* dominant axis: the first largest |component| (Vector::Max) */
+    int dom = 0;
+    float best = fabsf(n[0]);
+    if (fabsf(n[1]) > best) { dom = 1; best = fabsf(n[1]); }
+    if (fabsf(n[2]) > best) dom = 2;
+    const int neg = signbit(n[dom]) ? 1 : 0;
+    /* cube -> [-1,1]^3 (TransformGen::Ortogonal, RayLib/Matrix.hpp:1021-1039) */
+    for (int k = 0; k < 3; k++) {
+        float ndc[3];
+        for (int a = 0; a < 3; a++) {
+            const float lo = cube[a], hi = cube[a] + cube[3];
+            const float sc = 2.0f / (hi - lo), tr = -((hi + lo) / (hi - lo));
+            ndc[a] = sc * s->p[k][a] + tr;
+        }
+        /* turn the dominant axis to +z; keep the raster axes */
+        float u, v;
+        if (dom == 0) { u = neg ? ndc[2] : -ndc[2]; v = ndc[1]; }
+        else if (dom == 1) { u = ndc[0]; v = neg ? ndc[2] : -ndc[2]; }
+        else { u = neg ? -ndc[0] : ndc[0]; v = ndc[1]; }
+        s->q[k][0] = u;
+        s->q[k][1] = v;
+    }
+    float mn[2] = {s->q[0][0], s->q[0][1]}, mx[2] = {s->q[0][0], s->q[0][1]};
+    for (int k = 1; k < 3; k++)
+        for (int a = 0; a < 2; a++) {
+            mn[a] = fminf(mn[a], s->q[k][a]);
+            mx[a] = fmaxf(mx[a], s->q[k][a]);
+        }
+    const float Rf = (float)R;
+    int32_t r[4] = {(int32_t)floorf((0.5f + 0.5f * mn[0]) * Rf), (int32_t)ceilf((0.5f + 0.5f * mx[0]) * Rf),
+                    (int32_t)floorf((0.5f + 0.5f * mn[1]) * Rf), (int32_t)ceilf((0.5f + 0.5f * mx[1]) * Rf)};
+    for (int i = 0; i < 4; i++) r[i] = r[i] < 0 ? 0 : (r[i] > (int32_t)R ? (int32_t)R : r[i]);
+    s->x0 = r[0];
+    s->x1 = r[1];
+    s->y0 = r[2];
+    s->y1 = r[3];
+    /* conservative edges (GPU Gems 2 ch. 42): line through q_k along the
+     * edge, moved out by half a pixel */
+    const float hp = 1.0f / Rf;
+    float pl[3][3];
+    for (int k = 0; k < 3; k++) {
+        const float* a = s->q[k];
+        const float* b = s->q[(k + 1) % 3];
+        const float dx = b[0] - a[0], dy = b[1] - a[1];
+        pl[k][0] = dy;
+        pl[k][1] = -dx;
+        pl[k][2] = dx * a[1] - dy * a[0];
+        pl[k][2] -= hp * fabsf(pl[k][0]) + hp * fabsf(pl[k][1]);
+    }
+    /* corners: crossings of adjacent moved lines */
+    const int pair[3][2] = {{0, 2}, {0, 1}, {1, 2}};
+    float cc[3][2];
+    for (int k = 0; k < 3; k++) {
+        const float* A = pl[pair[k][0]];
+        const float* B = pl[pair[k][1]];
+        const float X = A[1] * B[2] - A[2] * B[1], Y = A[2] * B[0] - A[0] * B[2], Z = A[0] * B[1] - A[1] * B[0];
+        cc[k][0] = X / Z;
+        cc[k][1] = Y / Z;
+    }
+    s->c0[0] = cc[0][0];
+    s->c0[1] = cc[0][1];
+    s->ec0[0] = cc[1][0] - cc[0][0];
+    s->ec0[1] = cc[1][1] - cc[0][1];
+    s->ec1[0] = cc[2][0] - cc[0][0];
+    s->ec1[1] = cc[2][1] - cc[0][1];
+    s->denom_c = 1.0f / (s->ec0[0] * s->ec1[1] - s->ec1[0] * s->ec0[1]);
+    s->e0[0] = s->q[1][0] - s->q[0][0];
+    s->e0[1] = s->q[1][1] - s->q[0][1];
+    s->e1[0] = s->q[2][0] - s->q[0][0];
+    s->e1[1] = s->q[2][1] - s->q[0][1];
+    s->denom = 1.0f / (s->e0[0] * s->e1[1] - s->e1[0] * s->e0[1]);
+}
+
+/* one pixel: 1 and the voxel code when the conservative triangle covers its
+ * centre (GPUPrimitiveTriangle.h:548-612) */
+static int vox_pixel(const om_vox_setup* s, const float cube[4], uint32_t R, int32_t x, int32_t y, uint32_t* code)
+{
+    const float dp = 2.0f / (float)R;
+    const float px = ((float)x + 0.5f) * dp - 1.0f, py = ((float)y + 0.5f) * dp - 1.0f;
+    const float g0 = px - s->c0[0], g1 = py - s->c0[1];
+    const float b = (g0 * s->ec1[1] - s->ec1[0] * g1) * s->denom_c;
+    const float c = (s->ec0[0] * g1 - g0 * s->ec0[1]) * s->denom_c;
+    const float a = 1.0f - b - c;
+    if (!(b >= 0.0f && b <= 1.0f && c >= 0.0f && c <= 1.0f && a >= 0.0f && a <= 1.0f)) return 0;
+    const float h0 = px - s->q[0][0], h1 = py - s->q[0][1];
+    const float B = (h0 * s->e1[1] - s->e1[0] * h1) * s->denom;
+    const float C = (s->e0[0] * h1 - h0 * s->e0[1]) * s->denom;
+    const float A = 1.0f - B - C;
+    uint32_t q[3];
+    for (int k = 0; k < 3; k++) {
+        const float pos = s->p[0][k] * A + s->p[1][k] * B + s->p[2][k] * C;
+        const float f = ((pos - cube[k]) / cube[3]) * (float)R;
+        q[k] = !(f > 0.0f) ? 0u : (f >= (float)(R - 1) ? R - 1 : (uint32_t)f);
+    }
+    *code = vox_expand10(q[0]) | (vox_expand10(q[1]) << 1) | (vox_expand10(q[2]) << 2);
+    return 1;
+}
+
+/* Voxel records of n triangles (float[n][9]) in the cube (lo xyz, size) at
+ * depth (R = 2^depth <= 1024): returns the record count; codes (capacity
+ * cap, may be NULL) receive the first min(count, cap) records. */
+uint64_t om_voxelize(const float* tris, uint32_t n, const float cube[4], uint32_t depth, uint32_t* codes,
+                     uint64_t cap)
+{
+    const uint32_t R = 1u << depth;
+    uint64_t w = 0;
+    for (uint32_t t = 0; t < n; t++) {
+        om_vox_setup s;
+        vox_setup(tris + 9 * (size_t)t, cube, R, &s);
+        for (int32_t y = s.y0; y < s.y1; y++)
+            for (int32_t x = s.x0; x < s.x1; x++) {
+                uint32_t code;
+                if (vox_pixel(&s, cube, R, x, y, &code)) {
+                    if (codes && w < cap) codes[w] = code;
+                    w++;
+                }
+            }
+    }
+    return w;
+}

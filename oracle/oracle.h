@@ -360,6 +360,12 @@ void  om_wfpg_pwc_sample(const float* cdfx, const float* cdfy, uint32_t X, uint3
 float om_wfpg_pwc_pdf(const float* pdfx, const float* pdfy, uint32_t X, uint32_t Y, float u, float v);
 int   om_wfpg_sample_fields(const om_wfpg_desc* d);
 
+/* ---- Triangle voxeliser of the guiding octree (GPUPrimitiveTriangle.h:
+ * 384-620): records of every conservatively covered pixel, triangle by
+ * triangle, row by row; codes are cell codes at `depth` of the cube. */
+uint64_t om_voxelize(const float* tris, uint32_t n, const float cube[4], uint32_t depth, uint32_t* codes,
+                     uint64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

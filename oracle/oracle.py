@@ -145,6 +145,8 @@ def lib():
         L.om_wfpg_pwc_pdf.restype = _F
         L.om_wfpg_pwc_pdf.argtypes = [_P, _P, _U32, _U32, _F, _F]
         L.om_wfpg_sample_fields.argtypes = [C.POINTER(OmWfpgDesc)]
+        L.om_voxelize.restype = C.c_uint64
+        L.om_voxelize.argtypes = [_P, _U32, _P, _U32, _P, C.c_uint64]
         L.om_trace_closest.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F,
                                        _P, _P, _P, _P]
         L.om_trace_any.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F, _P]
@@ -933,3 +935,24 @@ def wfpg_sample_fields(fields: np.ndarray, bin_offsets: np.ndarray, ray_ids: np.
                    _ptr(out["filtered"]))
     lib().om_wfpg_sample_fields(C.byref(d))
     return out
+
+
+def voxelize(tris: np.ndarray, cube, depth: int) -> np.ndarray:
+    """om_voxelize: voxel cell codes of every covered pixel (duplicates kept)."""
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    cube = np.ascontiguousarray(cube, dtype=np.float32)
+    n = int(lib().om_voxelize(_ptr(tris), len(tris), _ptr(cube), depth, None, 0))
+    out = np.zeros(max(n, 1), np.uint32)
+    lib().om_voxelize(_ptr(tris), len(tris), _ptr(cube), depth, _ptr(out), n)
+    return out[:n]
+
+
+def svo_cube(lo, hi, depth: int) -> np.ndarray:
+    """The guiding octree's cube for scene bounds [lo, hi] at `depth`
+    (AnisoSVOctreeCPU::Constrcut, AnisoSVO.cu:670-678): the largest span,
+    padded by one leaf on every side."""
+    lo = np.asarray(lo, np.float32)
+    span = np.asarray(hi, np.float32) - lo
+    w = np.float32(span.max())
+    pad = np.float32(w / np.float32(1 << depth))
+    return np.array([lo[0] - pad, lo[1] - pad, lo[2] - pad, np.float32(w + np.float32(2) * pad)], np.float32)

@@ -518,6 +518,22 @@ int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes
 int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, const float cube[4], uint32_t depth,
                          uint32_t* d_codes, uint32_t capacity, uint32_t* out_count);
 
+/* Cone trace through the octree.  Replaces AnisoSVOctreeGPU::ConeTraceRay
+ * (AnisoSVO.cuh:767-1032; as GenerateRadianceField's batched trace calls it,
+ * WFPGTracerKC.cuh:1282-1310).  d_rays: n x 7 floats (origin, direction,
+ * tmin), world space.  Each ray walks the octree front to back and stops on
+ * entering an existing node that is a leaf, whose voxel the cone's disc is
+ * narrower than (diameter^2 = 4 t^2 aperture / pi > voxel size^2;
+ * aperture = the cone's solid angle), or that lies max_query_level_offset or
+ * fewer levels above the leaves.  d_nodes[i] = that node's global index
+ * (0xFFFFFFFF when the ray leaves the cube first), d_t[i] = the t reached
+ * (the node's entry), d_is_leaf[i] (may be NULL) = 1 for a leaf.  Directions
+ * closer than 1e-5 to an axis plane are pushed to 1e-5, as the reference
+ * does.  A walk longer than 65536 steps ends as a miss (the reference has no
+ * bound). */
+int32_t mrh_svo_cone_trace(mrh_context* ctx, const mrh_svo* svo, const float* d_rays, uint32_t n, float aperture,
+                           uint32_t max_query_level_offset, uint32_t* d_nodes, float* d_t, uint32_t* d_is_leaf);
+
 /* Per-bin radiance fields of the reference's wavefront path guiding: the
  * field-to-sample half of KCGenAndSampleDistribution (WFPGTracerKC.cuh:
  * 1433-1545, with GenerateRadianceField's radiance query and filter,

@@ -524,6 +524,101 @@ __global__ __launch_bounds__(256) void k_voxelize(const float* __restrict__ tris
     }
     if (!offsets && lane == 0 && wave_total) atomicAdd(total, wave_total);
 }
+
+// ---------------------------------------------------------------- cone trace
+// AnisoSVOctreeGPU::ConeTraceRay (AnisoSVO.cuh:767-1032) on this octree: the
+// ray in the cube's unit space, mirrored to a positive direction; a Morton
+// bit stack of the child indices taken and the parent links instead of a
+// node stack; descending into an existing child stops at a leaf, where the
+// cone's disc (diameter^2 = 4 t^2 aperture / pi) is wider than the child's
+// voxel, or `offset` levels above the leaves; a missing child is crossed
+// past its far face.  A walk longer than CONE_MAX_STEPS steps ends as a miss.
+constexpr uint32_t CONE_MAX_STEPS = 1u << 16;
+
+__device__ inline float neg_exp2i(uint32_t p) { return p == 0 ? 1.0f : __int_as_float((int)((127u - p) << 23)); }
+
+__global__ __launch_bounds__(256) void k_svo_cone_trace(const uint32_t* __restrict__ first_child,
+                                                        const uint32_t* __restrict__ child_mask,
+                                                        const uint32_t* __restrict__ parent, uint32_t depth,
+                                                        float4 cube, const float* __restrict__ rays, uint32_t n,
+                                                        float aperture, uint32_t offset, uint32_t* __restrict__ out_node,
+                                                        float* __restrict__ out_t, uint32_t* __restrict__ out_leaf)
+{
+    const float EPS = 1.0e-5f;
+    const float lo[3] = {cube.x, cube.y, cube.z};
+    const float scale = 1.0f / cube.w;
+    const float leaf = cube.w / (float)(1u << depth);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float* r = rays + 7 * (size_t)i;
+        float d[3], p[3], coeff[3], bias[3];
+        uint32_t mirror = 0;
+        for (int a = 0; a < 3; a++) {
+            d[a] = (fabsf(r[3 + a]) < EPS) ? copysignf(EPS, r[3 + a]) : r[3 + a];
+            d[a] = d[a] * scale;
+            p[a] = (r[a] - lo[a]) * scale;
+            if (d[a] < 0.0f) {
+                p[a] = 1.0f - p[a];
+                mirror |= 1u << a;
+            }
+            coeff[a] = 1.0f / fabsf(d[a]);
+            bias[a] = p[a] * coeff[a];
+        }
+        float tmin = r[6];
+        const float te0 = -bias[0], te1 = -bias[1], te2 = -bias[2];
+        const float tsvo = fmaxf(fmaxf(te0, te1), te2);
+        if (!(te0 < tmin && te1 < tmin && te2 < tmin)) tmin = nextafterf(tsvo, INFINITY);
+        uint64_t stack = 0;
+        uint32_t cnt = 0, nid = 0, node = SVO_NO_NODE, is_leaf = 0;
+        float corner[3] = {0.0f, 0.0f, 0.0f};
+        bool empty = false;
+        for (uint32_t step = 0; nid != SVO_NO_NODE; step++) {
+            if (step >= CONE_MAX_STEPS) break;
+            const float cs = neg_exp2i(cnt + 1), vs = neg_exp2i(cnt);
+            float tmax = INFINITY;
+            for (int a = 0; a < 3; a++) tmax = fminf(tmax, (corner[a] + vs) * coeff[a] - bias[a]);
+            if (empty || tmax < tmin) {
+                if (empty) tmin = nextafterf(tmax, INFINITY);
+                const uint32_t bits = (uint32_t)(stack & 7u);
+                for (int a = 0; a < 3; a++)
+                    if ((bits >> a) & 1u) corner[a] -= vs;
+                stack >>= 3;
+                if (!empty) {
+                    if (cnt == 0) break;
+                    nid = parent[nid];
+                }
+                cnt--;
+                empty = false;
+                continue;
+            }
+            uint32_t child = 0;
+            for (int a = 0; a < 3; a++)
+                if ((corner[a] + cs) * coeff[a] - bias[a] < tmin) child |= 1u << a;
+            const uint32_t mir = child ^ mirror;
+            const uint32_t mask = child_mask[nid];
+            const bool has = (mask >> mir) & 1u;
+            if (has) {
+                nid = first_child[nid] + (uint32_t)__popc(mask & ((1u << mir) - 1u));
+                const uint32_t level = cnt + 1;
+                const float vsz = leaf * (float)(1u << (depth - level));
+                const float diam2 = 4.0f * ((tmin * tmin * aperture) * 0.318309886183790671538f);
+                const bool child_leaf = (level == depth);
+                if (child_leaf || diam2 > vsz * vsz || (depth - level) <= offset) {
+                    node = nid;
+                    is_leaf = child_leaf ? 1u : 0u;
+                    break;
+                }
+            }
+            empty = !has;
+            for (int a = 0; a < 3; a++)
+                if ((child >> a) & 1u) corner[a] += cs;
+            stack = (stack << 3) | child;
+            cnt++;
+        }
+        out_node[i] = node;
+        out_t[i] = tmin;
+        if (out_leaf) out_leaf[i] = is_leaf;
+    }
+}
 }  // namespace
 
 SvoView svo_view(const mrh_svo* s)
@@ -994,6 +1089,21 @@ int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, cons
     MRH_TRY(scan_exclusive(ctx, counts, counts, n + 1, tmp));
     return launch(ctx, "voxelize", dim3(grid), dim3(256), 0, k_voxelize, d_tris, n, c4, depth,
                   (const uint32_t*)counts, d_codes, (unsigned long long*)nullptr);
+}
+
+int32_t mrh_svo_cone_trace(mrh_context* ctx, const mrh_svo* svo, const float* d_rays, uint32_t n, float aperture,
+                           uint32_t max_query_level_offset, uint32_t* d_nodes, float* d_t, uint32_t* d_is_leaf)
+{
+    if (!ctx || !svo || svo->ctx != ctx) return MRH_ERR_INVALID;
+    if (!svo->nodes) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_cone_trace: the octree is not built");
+    if (!(aperture >= 0.0f) || !std::isfinite(aperture))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_cone_trace: aperture must be finite and >= 0");
+    if (n == 0) return MRH_OK;
+    if (!d_rays || !d_nodes || !d_t) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_cone_trace: null rays or outputs");
+    return launch(ctx, "svo_cone_trace", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_svo_cone_trace,
+                  (const uint32_t*)svo->first_child, (const uint32_t*)svo->child_mask, (const uint32_t*)svo->parent,
+                  svo->depth, make_float4(svo->cube[0], svo->cube[1], svo->cube[2], svo->cube[3]), d_rays, n, aperture,
+                  max_query_level_offset, d_nodes, d_t, d_is_leaf);
 }
 
 int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, uint32_t n, uint32_t min_level,

@@ -209,6 +209,7 @@ _SIGS = {
     "mrh_svo_bin_rays": (_I32, [_P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
     "mrh_wfpg_sample_fields": (_I32, [_P, C.POINTER(WfpgFieldsDesc)]),
     "mrh_svo_voxelize": (_I32, [_P, _P, _U32, C.POINTER(_F * 4), _U32, _P, _U32, C.POINTER(_U32)]),
+    "mrh_svo_cone_trace": (_I32, [_P, _P, _P, _U32, _F, _U32, _P, _P, _P]),
     "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                  C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
     "mrh_server_create": (_I32, [_P, C.POINTER(ServerParams), C.POINTER(_P)]),
@@ -722,6 +723,12 @@ class Svo:
         self.ctx.check(lib().mrh_svo_bin_rays(self.ctx.h, self.h, ptr(d_codes), n, min_level, bin_ray_count,
                                               ptr(d_bins), ptr(d_order), ptr(d_part_offsets), ptr(d_part_bins),
                                               ptr(d_part_count)))
+
+    def cone_trace(self, d_rays, n: int, aperture: float, max_query_level_offset: int, d_nodes, d_t,
+                   d_is_leaf=None) -> None:
+        """mrh_svo_cone_trace over rays [n][7] (origin, direction, tmin)."""
+        self.ctx.check(lib().mrh_svo_cone_trace(self.ctx.h, self.h, ptr(d_rays), n, aperture, max_query_level_offset,
+                                                ptr(d_nodes), ptr(d_t), ptr(d_is_leaf)))
 
     def sample(self, d_nodes, d_normals, d_xi, n: int, d_dirs, d_pdf) -> None:
         self.ctx.check(lib().mrh_svo_sample(self.ctx.h, self.h, ptr(d_nodes), ptr(d_normals), ptr(d_xi), n,

@@ -26,6 +26,7 @@
 #include <stdlib.h>
 
 #include "oracle.h"
+#include "om.h"
 
 static uint32_t vox_expand10(uint32_t v)
 {
@@ -177,4 +178,117 @@ uint64_t om_voxelize(const float* tris, uint32_t n, const float cube[4], uint32_
             }
     }
     return w;
+}
+
+/* ---- Cone trace through the guiding octree (AnisoSVOctreeGPU::ConeTraceRay,
+ * TracerCUDA/AnisoSVO.cuh:767-1032), on the octree of om_svo_build (levels
+ * root first; a node's children are first_child + rank in child_mask).
+ *
+ * The ray is taken to the cube's unit space and mirrored so that every
+ * direction component is positive (the child index is mirrored back when a
+ * child is looked up); a Morton bit stack of the child indices taken on the
+ * way down and the parent links replace a node stack.  Descending into an
+ * existing child stops at a leaf, at a level whose voxel the cone's disc
+ * (diameter^2 = 4 t^2 aperture / pi at the current t) is wider than, or at
+ * `max_query_level_offset` levels above the leaves; a missing child is
+ * crossed without a node and the walk moves on past its far face.  Returns
+ * the t reached; *node = the node stopped at (OM_SVO_NO_NODE: the ray left
+ * the cube), *is_leaf whether it is a leaf.  A step limit (the reference has
+ * none) ends a walk that does not finish within OM_CONE_MAX_STEPS steps as a
+ * miss. */
+#define OM_CONE_MAX_STEPS (1u << 16)
+
+static float neg_exp2i(uint32_t p) { return p == 0 ? 1.0f : ldexpf(1.0f, -(int)p); }
+
+float om_svo_cone_trace(const om_svo* s, const float o[3], const float dir[3], float tmin, float aperture,
+                        uint32_t max_query_level_offset, uint32_t* node_out, int* is_leaf)
+{
+    const float EPS = 1.0e-5f;
+    *node_out = OM_SVO_NO_NODE;
+    *is_leaf = 0;
+    float d[3], p[3], coeff[3], bias[3];
+    const float scale = 1.0f / s->cube[3];
+    uint32_t mirror = 0;
+    for (int a = 0; a < 3; a++) {
+        d[a] = (fabsf(dir[a]) < EPS) ? copysignf(EPS, dir[a]) : dir[a];
+        d[a] = d[a] * scale;
+        p[a] = (o[a] - s->cube[a]) * scale;
+        if (d[a] < 0.0f) {
+            p[a] = 1.0f - p[a];
+            mirror |= 1u << a;
+        }
+        coeff[a] = 1.0f / fabsf(d[a]);
+        bias[a] = p[a] * coeff[a];
+    }
+    const float te[3] = {-bias[0], -bias[1], -bias[2]};
+    const float tsvo = fmaxf(fmaxf(te[0], te[1]), te[2]);
+    if (!(te[0] < tmin && te[1] < tmin && te[2] < tmin)) tmin = nextafterf(tsvo, INFINITY);
+    const float leaf = s->cube[3] / (float)(1u << s->depth);
+    uint64_t stack = 0;
+    uint32_t cnt = 0;
+    uint32_t nid = 0;
+    float corner[3] = {0.0f, 0.0f, 0.0f};
+    int empty = 0;
+    for (uint32_t step = 0; nid != OM_SVO_NO_NODE; step++) {
+        if (step >= OM_CONE_MAX_STEPS) {
+            *node_out = OM_SVO_NO_NODE;
+            *is_leaf = 0;
+            return tmin;
+        }
+        const float cs = neg_exp2i(cnt + 1), vs = neg_exp2i(cnt);
+        float tmax = INFINITY;
+        for (int a = 0; a < 3; a++) tmax = fminf(tmax, (corner[a] + vs) * coeff[a] - bias[a]);
+        if (empty || tmax < tmin) {
+            if (empty) tmin = nextafterf(tmax, INFINITY);
+            const uint32_t bits = (uint32_t)(stack & 7u);
+            for (int a = 0; a < 3; a++)
+                if ((bits >> a) & 1u) corner[a] -= vs;
+            stack >>= 3;
+            if (!empty) {
+                if (cnt == 0) break;                    /* above the root: left the cube */
+                nid = s->parent[nid];
+            }
+            cnt--;
+            empty = 0;
+            continue;
+        }
+        uint32_t child = 0;
+        for (int a = 0; a < 3; a++)
+            if ((corner[a] + cs) * coeff[a] - bias[a] < tmin) child |= 1u << a;
+        const uint32_t mir = child ^ mirror;
+        const uint32_t mask = s->child_mask[nid];
+        const int has = (mask >> mir) & 1u;
+        if (has) {
+            nid = s->first_child[nid] + (uint32_t)__builtin_popcount(mask & ((1u << mir) - 1u));
+            const uint32_t level = cnt + 1;
+            const float vsz = leaf * (float)(1u << (s->depth - level));
+            const float diam2 = 4.0f * ((tmin * tmin * aperture) * 0.318309886183790671538f);
+            const int child_leaf = (level == s->depth);
+            const int term = (diam2 > vsz * vsz) || ((s->depth - level) <= max_query_level_offset);
+            if (child_leaf || term) {
+                *is_leaf = child_leaf;
+                *node_out = nid;
+                return tmin;
+            }
+        }
+        empty = !has;
+        for (int a = 0; a < 3; a++)
+            if ((child >> a) & 1u) corner[a] += cs;
+        stack = (stack << 3) | child;
+        cnt++;
+    }
+    *node_out = OM_SVO_NO_NODE;
+    return tmin;
+}
+
+/* rays: n x 7 floats (origin, direction, tmin) */
+void om_svo_cone_trace_batch(const om_svo* s, const float* rays, uint32_t n, float aperture, uint32_t offset,
+                             uint32_t* nodes, float* t, uint32_t* leaf)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        const float* r = rays + 7 * (size_t)i;
+        int lf;
+        t[i] = om_svo_cone_trace(s, r, r + 3, r[6], aperture, offset, &nodes[i], &lf);
+        leaf[i] = (uint32_t)lf;
+    }
 }

@@ -365,6 +365,13 @@ int   om_wfpg_sample_fields(const om_wfpg_desc* d);
  * triangle, row by row; codes are cell codes at `depth` of the cube. */
 uint64_t om_voxelize(const float* tris, uint32_t n, const float cube[4], uint32_t depth, uint32_t* codes,
                      uint64_t cap);
+/* Cone trace through an om_svo_build octree (AnisoSVO.cuh:767-1032):
+ * returns t; *node = node stopped at (0xFFFFFFFF: left the cube).
+ * rays for the batch: n x 7 floats (origin, direction, tmin). */
+float    om_svo_cone_trace(const om_svo* s, const float o[3], const float dir[3], float tmin, float aperture,
+                           uint32_t max_query_level_offset, uint32_t* node, int* is_leaf);
+void     om_svo_cone_trace_batch(const om_svo* s, const float* rays, uint32_t n, float aperture, uint32_t offset,
+                                 uint32_t* nodes, float* t, uint32_t* leaf);
 
 #ifdef __cplusplus
 }

@@ -146,6 +146,7 @@ def lib():
         L.om_wfpg_pwc_pdf.argtypes = [_P, _P, _U32, _U32, _F, _F]
         L.om_wfpg_sample_fields.argtypes = [C.POINTER(OmWfpgDesc)]
         L.om_voxelize.restype = C.c_uint64
+        L.om_svo_cone_trace_batch.argtypes = [C.POINTER(OmSvo), _P, _U32, _F, _U32, _P, _P, _P]
         L.om_voxelize.argtypes = [_P, _U32, _P, _U32, _P, C.c_uint64]
         L.om_trace_closest.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F,
                                        _P, _P, _P, _P]
@@ -749,6 +750,18 @@ class Svo:
         if rc != 0:
             raise ValueError(f"om_svo_build failed ({rc})")
         return cls(s)
+
+    def cone_trace(self, rays: np.ndarray, aperture: float, max_query_level_offset: int = 0):
+        """om_svo_cone_trace over rays [n][7] (origin, direction, tmin):
+        (nodes, t, is_leaf)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 7)
+        n = len(rays)
+        nodes = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        leaf = np.zeros(n, np.uint32)
+        lib().om_svo_cone_trace_batch(C.byref(self._s), _ptr(rays), n, aperture, max_query_level_offset,
+                                      _ptr(nodes), _ptr(t), _ptr(leaf))
+        return nodes, t, leaf
 
     def lookup(self, codes: np.ndarray) -> np.ndarray:
         return np.array([lib().om_svo_lookup(C.byref(self._s), int(c)) for c in codes], dtype=np.uint32)

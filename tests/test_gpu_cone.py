@@ -1,0 +1,124 @@
+"""Cone tracing through the guiding octree on the GPU (mrh_svo_cone_trace)
+against the oracle (om_svo_cone_trace): nodes, t and leaf flags bit for bit
+over octrees voxelised from triangles (depth 5, 8 and 10) and from points,
+for rays inside and outside the cube, axis-parallel rays, rays with a tmin,
+apertures from a ray to a wide cone, and level offsets; then the trace the
+field stage makes: 64x64 concentric-octahedral directions (the first-bounce
+field size) from points inside the voxelised Cornell box at depth 10."""
+import math
+
+import numpy as np
+import pytest
+
+from meturay_amd import native as N
+from test_gpu_guide import dev, host
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def build_both(ctx, oracle, codes, cube, depth):
+    svo = N.Svo(ctx)
+    d = dev(codes)
+    torch.cuda.synchronize()
+    svo.build(d.data_ptr(), len(codes), cube.tolist(), depth)
+    ctx.sync()
+    return svo, oracle.Svo.build(codes, cube.tolist(), depth)
+
+
+def trace_gpu(ctx, svo, rays, aperture, offset):
+    n = len(rays)
+    d_r = dev(np.ascontiguousarray(rays, np.float32).reshape(-1))
+    d_n = torch.empty(n, dtype=torch.int32, device="cuda")
+    d_t = torch.empty(n, dtype=torch.float32, device="cuda")
+    d_l = torch.empty(n, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    svo.cone_trace(d_r, n, aperture, offset, d_n, d_t, d_l)
+    ctx.sync()
+    return host(d_n, np.uint32), d_t.cpu().numpy(), host(d_l, np.uint32)
+
+
+def random_rays(rng, n, span=1.6):
+    o = rng.uniform(-span, span, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    k = len(d[::9])
+    d[::9] = np.eye(3)[rng.integers(0, 3, k)] * rng.choice([-1, 1], (k, 1))
+    d[4::13, rng.integers(0, 3)] = 0.0
+    tmin = np.where(rng.random(n) < 0.25, rng.uniform(0, 0.4, n), 0.0)
+    return np.hstack([o, d, tmin[:, None]]).astype(np.float32)
+
+
+@pytest.mark.parametrize("depth", [5, 8, 10])
+def test_matches_oracle_on_voxelised_triangles(ctx, oracle, depth):
+    rng = np.random.default_rng(depth)
+    cube = oracle.svo_cube([-1, -1, -1], [1, 1, 1], depth)
+    c = rng.uniform(-0.8, 0.8, (300, 1, 3))
+    tris = (c + 0.25 * rng.normal(size=(300, 3, 3))).clip(-1, 1).astype(np.float32).reshape(-1, 9)
+    codes = oracle.voxelize(tris, cube, depth)
+    svo, o = build_both(ctx, oracle, codes, cube, depth)
+    rays = random_rays(rng, 20000)
+    for aperture, offset in ((0.0, 0), (1e-3, 0), (3e-2, 0), (0.0, 2), (5e-3, 1)):
+        print(f"[cone] depth {depth} aperture {aperture} offset {offset}", flush=True)
+        gn, gt, gl = trace_gpu(ctx, svo, rays, aperture, offset)
+        on, ot, ol = o.cone_trace(rays, aperture, offset)
+        assert np.array_equal(gn, on) and np.array_equal(gl, ol)
+        assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
+        assert (gn != 0xFFFFFFFF).sum() > 1000
+
+
+def test_matches_oracle_on_point_octree(ctx, oracle):
+    rng = np.random.default_rng(41)
+    depth = 7
+    cube = oracle.svo_cube([-1, -1, -1], [1, 1, 1], depth)
+    codes = oracle.svo_codes(rng.uniform(-1, 1, (5000, 3)).astype(np.float32), cube, depth)
+    svo, o = build_both(ctx, oracle, codes, cube, depth)
+    rays = random_rays(rng, 10000)
+    gn, gt, gl = trace_gpu(ctx, svo, rays, 2e-3, 0)
+    on, ot, ol = o.cone_trace(rays, 2e-3, 0)
+    assert np.array_equal(gn, on) and np.array_equal(gl, ol)
+    assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
+
+
+def test_field_directions_in_cornell_box(ctx, oracle):
+    """What a bin's field needs: 64x64 octahedral texel directions from a
+    point, cone aperture = the texel's solid angle (4 pi / 4096)."""
+    depth = 10
+    tris = np.ascontiguousarray(N.gen_cornell()["tris"], np.float32).reshape(-1, 9)
+    p = tris.reshape(-1, 3)
+    cube = oracle.svo_cube(p.min(0), p.max(0), depth)
+    codes = oracle.voxelize(tris, cube, depth)
+    svo, o = build_both(ctx, oracle, codes, cube, depth)
+    u = (np.arange(64) + 0.5) / 64
+    uv = np.stack(np.meshgrid(u, u), -1).reshape(-1, 2).astype(np.float32)
+    dirs = oracle.octa_to_dirs(uv)
+    rng = np.random.default_rng(5)
+    lo, hi = p.min(0), p.max(0)
+    origins = lo + (hi - lo) * rng.uniform(0.2, 0.8, (8, 3))
+    rays = np.concatenate([np.hstack([np.repeat(org[None], len(dirs), 0), dirs, np.zeros((len(dirs), 1))])
+                           for org in origins]).astype(np.float32)
+    aperture = 4 * math.pi / 4096
+    gn, gt, gl = trace_gpu(ctx, svo, rays, aperture, 0)
+    on, ot, ol = o.cone_trace(rays, aperture, 0)
+    assert np.array_equal(gn, on) and np.array_equal(gl, ol)
+    assert np.array_equal(gt.view(np.uint32), ot.view(np.uint32))
+    assert (gn != 0xFFFFFFFF).mean() > 0.95          # closed box: almost every cone lands
+
+
+def test_rejects_bad_arguments(ctx, oracle):
+    depth = 4
+    cube = oracle.svo_cube([0, 0, 0], [1, 1, 1], depth)
+    codes = oracle.svo_codes(np.random.default_rng(0).random((50, 3)).astype(np.float32), cube, depth)
+    svo, _ = build_both(ctx, oracle, codes, cube, depth)
+    r = torch.zeros(7, device="cuda")
+    nn = torch.zeros(1, dtype=torch.int32, device="cuda")
+    tt = torch.zeros(1, device="cuda")
+    torch.cuda.synchronize()
+    for ap in (-1.0, float("nan"), float("inf")):
+        with pytest.raises(N.MrhError):
+            svo.cone_trace(r, 1, ap, 0, nn, tt)
+    with pytest.raises(N.MrhError):
+        svo.cone_trace(r, 1, 0.0, 0, None, tt)
+    empty = N.Svo(ctx)
+    with pytest.raises(N.MrhError):
+        empty.cone_trace(r, 1, 0.0, 0, nn, tt)

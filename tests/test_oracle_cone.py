@@ -1,0 +1,175 @@
+"""Pins the oracle's cone trace through the guiding octree (om_svo_cone_trace;
+AnisoSVOctreeGPU::ConeTraceRay, AnisoSVO.cuh:767-1032).
+
+The reference's trace is device code with no host build, so it is pinned by
+what the method fixes, against float64 brute force over the octree's boxes:
+- a zero-aperture cone stopping only at leaves returns the first occupied
+  leaf along the ray, entered at its slab entry t (or the ray's tmin when it
+  starts inside), and a miss when no leaf lies on the ray;
+- stopping `k` levels above the leaves returns the first existing node of
+  that level along the ray;
+- with an aperture, the node returned is one whose voxel the cone's disc is
+  wider than at the t returned (or a leaf), and its box holds the ray point
+  at that t.
+Near ties (two boxes entered within 1e-5, grazing hits) are skipped."""
+import numpy as np
+import pytest
+
+EPS = np.float32(1e-5)
+
+
+def node_boxes(svo, cube, depth):
+    """level, lo corner and size (world) of every node."""
+    lo = np.array(cube[:3], np.float64)
+    size = float(cube[3])
+    lv = np.zeros(svo.nodes, np.int64)
+    offs = list(svo.level_offset)
+    for l in range(depth + 1):
+        lv[offs[l]:offs[l + 1]] = l
+    codes = svo.codes.astype(np.int64)
+
+    def compact(v):
+        v = v & 0x09249249
+        v = (v | (v >> 2)) & 0x030C30C3
+        v = (v | (v >> 4)) & 0x0300F00F
+        v = (v | (v >> 8)) & 0x030000FF
+        v = (v | (v >> 16)) & 0x000003FF
+        return v
+
+    xyz = np.stack([compact(codes), compact(codes >> 1), compact(codes >> 2)], 1).astype(np.float64)
+    cell = size / (2.0 ** lv)
+    return lv, lo + xyz * cell[:, None], cell
+
+
+def slab(o, d, blo, bsz):
+    inv = 1.0 / d
+    t0 = (blo - o) * inv
+    t1 = (blo + bsz[:, None] - o) * inv
+    tn = np.minimum(t0, t1).max(1)
+    tf = np.maximum(t0, t1).min(1)
+    return tn, tf
+
+
+def clamp_dir(d):
+    d = d.astype(np.float32).copy()
+    small = np.abs(d) < EPS
+    d[small] = np.copysign(EPS, d[small])
+    return d.astype(np.float64)
+
+
+def make_tree(oracle, rng, depth, kind):
+    cube = oracle.svo_cube([-1, -1, -1], [1, 1, 1], depth)
+    if kind == "tris":
+        c = rng.uniform(-0.8, 0.8, (40, 1, 3))
+        tris = (c + 0.3 * rng.normal(size=(40, 3, 3))).clip(-1, 1).astype(np.float32).reshape(-1, 9)
+        codes = oracle.voxelize(tris, cube, depth)
+    else:
+        pts = rng.uniform(-1, 1, (300, 3)).astype(np.float32)
+        codes = oracle.svo_codes(pts, cube, depth)
+    return oracle.Svo.build(codes, cube.tolist(), depth), cube
+
+
+def rays(rng, n):
+    o = rng.uniform(-1.6, 1.6, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d[::11] = np.eye(3)[rng.integers(0, 3, len(d[::11]))] * rng.choice([-1, 1], (len(d[::11]), 1))
+    tmin = np.where(rng.random(n) < 0.3, rng.uniform(0, 0.5, n), 0.0)
+    return np.hstack([o, d, tmin[:, None]]).astype(np.float32)
+
+
+def first_box(o, d, tmin, blo, bsz):
+    tn, tf = slab(o, d, blo, bsz)
+    ok = (tf >= np.maximum(tn, tmin)) & (tf - np.maximum(tn, tmin) > 1e-6)
+    if not ok.any():
+        return None, None, None
+    te = np.maximum(tn, tmin)
+    te[~ok] = np.inf
+    k = int(np.argmin(te))
+    srt = np.sort(te)
+    gap = srt[1] - srt[0] if np.isfinite(srt[1]) else np.inf
+    return k, te[k], gap
+
+
+@pytest.mark.parametrize("kind,depth", [("tris", 5), ("points", 4), ("tris", 6)])
+def test_zero_aperture_hits_first_leaf(oracle, kind, depth):
+    rng = np.random.default_rng(depth * 7 + len(kind))
+    svo, cube = make_tree(oracle, rng, depth, kind)
+    lv, blo, bsz = node_boxes(svo, cube, depth)
+    leaves = np.where(lv == depth)[0]
+    R = rays(rng, 400)
+    nodes, t, leaf = svo.cone_trace(R, 0.0, 0)
+    checked = hits = 0
+    for i, r in enumerate(R):
+        o, d, tmin = r[:3].astype(np.float64), clamp_dir(r[3:6]), float(r[6])
+        k, te, gap = first_box(o, d, tmin, blo[leaves], bsz[leaves])
+        if k is None:
+            assert nodes[i] == 0xFFFFFFFF, i
+            checked += 1
+            continue
+        if gap < 1e-5:
+            continue
+        checked += 1
+        hits += 1
+        assert leaf[i] == 1 and nodes[i] == leaves[k], (i, nodes[i], leaves[k])
+        assert abs(float(t[i]) - te) < 1e-5 * max(1.0, te), (i, t[i], te)
+    assert checked > 300 and hits > 40
+
+
+@pytest.mark.parametrize("offset", [1, 2])
+def test_offset_stops_at_level(oracle, offset):
+    rng = np.random.default_rng(offset)
+    depth = 6
+    svo, cube = make_tree(oracle, rng, depth, "tris")
+    lv, blo, bsz = node_boxes(svo, cube, depth)
+    lvl = np.where(lv == depth - offset)[0]
+    R = rays(rng, 800)
+    nodes, t, leaf = svo.cone_trace(R, 0.0, offset)
+    checked = 0
+    for i, r in enumerate(R):
+        o, d, tmin = r[:3].astype(np.float64), clamp_dir(r[3:6]), float(r[6])
+        k, te, gap = first_box(o, d, tmin, blo[lvl], bsz[lvl])
+        if k is None:
+            assert nodes[i] == 0xFFFFFFFF
+            continue
+        if gap < 1e-5:
+            continue
+        checked += 1
+        assert nodes[i] == lvl[k] and leaf[i] == 0
+    assert checked > 100
+
+
+@pytest.mark.parametrize("aperture", [1e-3, 2e-2, 0.3])
+def test_aperture_stops_where_the_cone_covers_the_voxel(oracle, aperture):
+    rng = np.random.default_rng(int(aperture * 1e4))
+    depth = 7
+    svo, cube = make_tree(oracle, rng, depth, "tris")
+    lv, blo, bsz = node_boxes(svo, cube, depth)
+    R = rays(rng, 300)
+    nodes, t, leaf = svo.cone_trace(R, aperture, 0)
+    leafsize = float(cube[3]) / (1 << depth)
+    stops = 0
+    for i, r in enumerate(R):
+        if nodes[i] == 0xFFFFFFFF:
+            continue
+        n = int(nodes[i])
+        L = int(lv[n])
+        tt = float(t[i])
+        diam2 = 4.0 * tt * tt * aperture / np.pi
+        assert leaf[i] == (L == depth)
+        if L < depth:
+            stops += 1
+            assert diam2 > (leafsize * 2 ** (depth - L)) ** 2 * (1 - 1e-5)
+        p = r[:3].astype(np.float64) + tt * clamp_dir(r[3:6])
+        tol = 1e-4 * float(cube[3])
+        assert ((p >= blo[n] - tol) & (p <= blo[n] + bsz[n] + tol)).all(), (i, p, blo[n], bsz[n])
+    if aperture >= 2e-2:
+        assert stops > 20
+
+
+def test_rays_leaving_the_cube_miss(oracle):
+    rng = np.random.default_rng(3)
+    svo, cube = make_tree(oracle, rng, 5, "points")
+    R = np.array([[2, 2, 2, 1, 1, 1, 0], [-3, 0, 0, -1, 0, 0, 0], [0, 0, 0, 0, 0, 1, 5.0]], np.float32)
+    nodes, t, leaf = svo.cone_trace(R, 0.0, 0)
+    assert (nodes == 0xFFFFFFFF).all() and (leaf == 0).all()

@@ -534,6 +534,38 @@ int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, cons
 int32_t mrh_svo_cone_trace(mrh_context* ctx, const mrh_svo* svo, const float* d_rays, uint32_t n, float aperture,
                            uint32_t max_query_level_offset, uint32_t* d_nodes, float* d_t, uint32_t* d_is_leaf);
 
+/* The radiance fields of the bins, by cone tracing.  Replaces
+ * GenerateRadianceField's trace and radiance query (WFPGTracerKC.cuh:
+ * 1252-1318: BatchConeTracer::BatchedConeTraceRay, RecursiveConeTrace.cuh:
+ * 219-247; AnisoSVOctreeGPU::ReadRadiance, AnisoSVO.cuh:1035-1053;
+ * VoxelPayload::ReadRadiance, :433-489).  Texel (x, y) of bin b looks along
+ * the concentric-octahedral direction of ((x, y) + jitter[b]) / (X, Y),
+ * y up, with a cone of the texel's solid angle (4 pi / field_x^2, as
+ * OctohedralConeAperture forms it, WFPGTracer.cu:134-141) from
+ * bin_origins[b] (xyz, tmin: the reference takes a random ray of the bin,
+ * its hit point pushed off the surface, and tmin = 0.15 leaf + 1e-5,
+ * WFPGTracerKC.cuh:1132-1192).  The texel's radiance is the stopped node's
+ * node_irradiance[node][0] when the cone arrives on the side its
+ * node_normals[node] faces (normal . -dir >= 0), [1] otherwise, and
+ * boundary_radiance when the cone leaves the cube (a constant stands for
+ * the reference's boundary light).  out_fields feeds
+ * mrh_wfpg_sample_fields.  The node payload (VoxelPayload's average
+ * front/back radiance and normal per node) is the caller's. */
+typedef struct {
+    uint32_t       field_x, field_y;       /* 8, 16, 32 or 64 each */
+    uint32_t       bin_count;
+    uint32_t       max_query_level_offset; /* 0 in the reference's field trace */
+    float          boundary_radiance;
+    const float*   bin_origins;            /* [bin_count][4] */
+    const float*   bin_jitter;             /* [bin_count][2] in [0,1) */
+    const float*   node_irradiance;        /* [nodes][2] front, back */
+    const float*   node_normals;           /* [nodes][3] */
+    float*         out_fields;             /* [bin_count][field_y][field_x] */
+    uint32_t*      out_nodes;              /* optional, same shape */
+    float*         out_t;                  /* optional, same shape */
+} mrh_wfpg_trace_desc;
+int32_t mrh_wfpg_trace_fields(mrh_context* ctx, const mrh_svo* svo, const mrh_wfpg_trace_desc* desc);
+
 /* Per-bin radiance fields of the reference's wavefront path guiding: the
  * field-to-sample half of KCGenAndSampleDistribution (WFPGTracerKC.cuh:
  * 1433-1545, with GenerateRadianceField's radiance query and filter,

@@ -537,86 +537,142 @@ constexpr uint32_t CONE_MAX_STEPS = 1u << 16;
 
 __device__ inline float neg_exp2i(uint32_t p) { return p == 0 ? 1.0f : __int_as_float((int)((127u - p) << 23)); }
 
-__global__ __launch_bounds__(256) void k_svo_cone_trace(const uint32_t* __restrict__ first_child,
-                                                        const uint32_t* __restrict__ child_mask,
-                                                        const uint32_t* __restrict__ parent, uint32_t depth,
-                                                        float4 cube, const float* __restrict__ rays, uint32_t n,
+struct ConeTree {
+    const uint32_t* first_child;
+    const uint32_t* child_mask;
+    const uint32_t* parent;
+    uint32_t depth;
+    float lo[3], size;
+};
+
+// One cone from o along dir (world space) starting at tmin: the node it
+// stops at (SVO_NO_NODE: left the cube), whether it is a leaf; returns t.
+__device__ inline float cone_trace_one(const ConeTree& tr, const float o[3], const float dir[3], float tmin,
+                                       float aperture, uint32_t offset, uint32_t& node, uint32_t& is_leaf)
+{
+    const float EPS = 1.0e-5f;
+    const float scale = 1.0f / tr.size;
+    const float leaf = tr.size / (float)(1u << tr.depth);
+    float coeff[3], bias[3];
+    uint32_t mirror = 0;
+    for (int a = 0; a < 3; a++) {
+        float d = (fabsf(dir[a]) < EPS) ? copysignf(EPS, dir[a]) : dir[a];
+        d = d * scale;
+        float p = (o[a] - tr.lo[a]) * scale;
+        if (d < 0.0f) {
+            p = 1.0f - p;
+            mirror |= 1u << a;
+        }
+        coeff[a] = 1.0f / fabsf(d);
+        bias[a] = p * coeff[a];
+    }
+    const float te0 = -bias[0], te1 = -bias[1], te2 = -bias[2];
+    const float tsvo = fmaxf(fmaxf(te0, te1), te2);
+    if (!(te0 < tmin && te1 < tmin && te2 < tmin)) tmin = nextafterf(tsvo, INFINITY);
+    uint64_t stack = 0;
+    uint32_t cnt = 0, nid = 0;
+    node = SVO_NO_NODE;
+    is_leaf = 0;
+    float corner[3] = {0.0f, 0.0f, 0.0f};
+    bool empty = false;
+    for (uint32_t step = 0; nid != SVO_NO_NODE; step++) {
+        if (step >= CONE_MAX_STEPS) break;
+        const float cs = neg_exp2i(cnt + 1), vs = neg_exp2i(cnt);
+        float tmax = INFINITY;
+        for (int a = 0; a < 3; a++) tmax = fminf(tmax, (corner[a] + vs) * coeff[a] - bias[a]);
+        if (empty || tmax < tmin) {
+            if (empty) tmin = nextafterf(tmax, INFINITY);
+            const uint32_t bits = (uint32_t)(stack & 7u);
+            for (int a = 0; a < 3; a++)
+                if ((bits >> a) & 1u) corner[a] -= vs;
+            stack >>= 3;
+            if (!empty) {
+                if (cnt == 0) break;
+                nid = tr.parent[nid];
+            }
+            cnt--;
+            empty = false;
+            continue;
+        }
+        uint32_t child = 0;
+        for (int a = 0; a < 3; a++)
+            if ((corner[a] + cs) * coeff[a] - bias[a] < tmin) child |= 1u << a;
+        const uint32_t mir = child ^ mirror;
+        const uint32_t mask = tr.child_mask[nid];
+        const bool has = (mask >> mir) & 1u;
+        if (has) {
+            nid = tr.first_child[nid] + (uint32_t)__popc(mask & ((1u << mir) - 1u));
+            const uint32_t level = cnt + 1;
+            const float vsz = leaf * (float)(1u << (tr.depth - level));
+            const float diam2 = 4.0f * ((tmin * tmin * aperture) * 0.318309886183790671538f);
+            const bool child_leaf = (level == tr.depth);
+            if (child_leaf || diam2 > vsz * vsz || (tr.depth - level) <= offset) {
+                node = nid;
+                is_leaf = child_leaf ? 1u : 0u;
+                return tmin;
+            }
+        }
+        empty = !has;
+        for (int a = 0; a < 3; a++)
+            if ((child >> a) & 1u) corner[a] += cs;
+        stack = (stack << 3) | child;
+        cnt++;
+    }
+    node = SVO_NO_NODE;
+    is_leaf = 0;
+    return tmin;
+}
+
+__global__ __launch_bounds__(256) void k_svo_cone_trace(ConeTree tr, const float* __restrict__ rays, uint32_t n,
                                                         float aperture, uint32_t offset, uint32_t* __restrict__ out_node,
                                                         float* __restrict__ out_t, uint32_t* __restrict__ out_leaf)
 {
-    const float EPS = 1.0e-5f;
-    const float lo[3] = {cube.x, cube.y, cube.z};
-    const float scale = 1.0f / cube.w;
-    const float leaf = cube.w / (float)(1u << depth);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float* r = rays + 7 * (size_t)i;
-        float d[3], p[3], coeff[3], bias[3];
-        uint32_t mirror = 0;
-        for (int a = 0; a < 3; a++) {
-            d[a] = (fabsf(r[3 + a]) < EPS) ? copysignf(EPS, r[3 + a]) : r[3 + a];
-            d[a] = d[a] * scale;
-            p[a] = (r[a] - lo[a]) * scale;
-            if (d[a] < 0.0f) {
-                p[a] = 1.0f - p[a];
-                mirror |= 1u << a;
-            }
-            coeff[a] = 1.0f / fabsf(d[a]);
-            bias[a] = p[a] * coeff[a];
-        }
-        float tmin = r[6];
-        const float te0 = -bias[0], te1 = -bias[1], te2 = -bias[2];
-        const float tsvo = fmaxf(fmaxf(te0, te1), te2);
-        if (!(te0 < tmin && te1 < tmin && te2 < tmin)) tmin = nextafterf(tsvo, INFINITY);
-        uint64_t stack = 0;
-        uint32_t cnt = 0, nid = 0, node = SVO_NO_NODE, is_leaf = 0;
-        float corner[3] = {0.0f, 0.0f, 0.0f};
-        bool empty = false;
-        for (uint32_t step = 0; nid != SVO_NO_NODE; step++) {
-            if (step >= CONE_MAX_STEPS) break;
-            const float cs = neg_exp2i(cnt + 1), vs = neg_exp2i(cnt);
-            float tmax = INFINITY;
-            for (int a = 0; a < 3; a++) tmax = fminf(tmax, (corner[a] + vs) * coeff[a] - bias[a]);
-            if (empty || tmax < tmin) {
-                if (empty) tmin = nextafterf(tmax, INFINITY);
-                const uint32_t bits = (uint32_t)(stack & 7u);
-                for (int a = 0; a < 3; a++)
-                    if ((bits >> a) & 1u) corner[a] -= vs;
-                stack >>= 3;
-                if (!empty) {
-                    if (cnt == 0) break;
-                    nid = parent[nid];
-                }
-                cnt--;
-                empty = false;
-                continue;
-            }
-            uint32_t child = 0;
-            for (int a = 0; a < 3; a++)
-                if ((corner[a] + cs) * coeff[a] - bias[a] < tmin) child |= 1u << a;
-            const uint32_t mir = child ^ mirror;
-            const uint32_t mask = child_mask[nid];
-            const bool has = (mask >> mir) & 1u;
-            if (has) {
-                nid = first_child[nid] + (uint32_t)__popc(mask & ((1u << mir) - 1u));
-                const uint32_t level = cnt + 1;
-                const float vsz = leaf * (float)(1u << (depth - level));
-                const float diam2 = 4.0f * ((tmin * tmin * aperture) * 0.318309886183790671538f);
-                const bool child_leaf = (level == depth);
-                if (child_leaf || diam2 > vsz * vsz || (depth - level) <= offset) {
-                    node = nid;
-                    is_leaf = child_leaf ? 1u : 0u;
-                    break;
-                }
-            }
-            empty = !has;
-            for (int a = 0; a < 3; a++)
-                if ((child >> a) & 1u) corner[a] += cs;
-            stack = (stack << 3) | child;
-            cnt++;
-        }
+        const float o[3] = {r[0], r[1], r[2]}, d[3] = {r[3], r[4], r[5]};
+        uint32_t node, leaf;
+        const float t = cone_trace_one(tr, o, d, r[6], aperture, offset, node, leaf);
         out_node[i] = node;
-        out_t[i] = tmin;
-        if (out_leaf) out_leaf[i] = is_leaf;
+        out_t[i] = t;
+        if (out_leaf) out_leaf[i] = leaf;
+    }
+}
+
+// GenerateRadianceField (WFPGTracerKC.cuh:1252-1318) for every texel of
+// every bin: the texel's direction with the bin's jitter on the concentric
+// octahedral map (y up), a cone of the texel's solid angle from the bin's
+// origin, and the radiance the stopped node sends back along the cone
+// (front or back side by the node's normal; the boundary radiance on a miss).
+__global__ __launch_bounds__(256) void k_wfpg_trace_fields(ConeTree tr, uint32_t X, uint32_t Y, uint32_t lx,
+                                                           uint32_t bins, const float4* __restrict__ origins,
+                                                           const float2* __restrict__ jitter, float aperture,
+                                                           uint32_t offset, const float2* __restrict__ irr,
+                                                           const float* __restrict__ normals, float boundary,
+                                                           float* __restrict__ fields, uint32_t* __restrict__ out_node,
+                                                           float* __restrict__ out_t)
+{
+    const uint32_t per = X * Y;
+    const uint64_t total = (uint64_t)bins * per;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = (uint32_t)(i / per), p = (uint32_t)(i % per);
+        const uint32_t x = p & (X - 1u), y = p >> lx;
+        const float4 og = origins[b];
+        const float2 jt = jitter[b];
+        const float u = ((float)x + jt.x) / (float)X, v = ((float)y + jt.y) / (float)Y;
+        const V3 dz = octa_to_dir(u, v);
+        const float o[3] = {og.x, og.y, og.z}, d[3] = {dz.y, dz.z, dz.x};
+        uint32_t node, leaf;
+        const float t = cone_trace_one(tr, o, d, og.w, aperture, offset, node, leaf);
+        float rad = boundary;
+        if (node != SVO_NO_NODE) {
+            const float* nrm = normals + 3 * (size_t)node;
+            const bool front = (nrm[0] * -d[0] + nrm[1] * -d[1] + nrm[2] * -d[2]) >= 0.0f;
+            const float2 two = irr[node];
+            rad = front ? two.x : two.y;
+        }
+        fields[i] = rad;
+        if (out_node) out_node[i] = node;
+        if (out_t) out_t[i] = t;
     }
 }
 }  // namespace
@@ -1091,6 +1147,20 @@ int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, cons
                   (const uint32_t*)counts, d_codes, (unsigned long long*)nullptr);
 }
 
+static ConeTree cone_tree(const mrh_svo* s)
+{
+    ConeTree t;
+    t.first_child = s->first_child;
+    t.child_mask = s->child_mask;
+    t.parent = s->parent;
+    t.depth = s->depth;
+    t.lo[0] = s->cube[0];
+    t.lo[1] = s->cube[1];
+    t.lo[2] = s->cube[2];
+    t.size = s->cube[3];
+    return t;
+}
+
 int32_t mrh_svo_cone_trace(mrh_context* ctx, const mrh_svo* svo, const float* d_rays, uint32_t n, float aperture,
                            uint32_t max_query_level_offset, uint32_t* d_nodes, float* d_t, uint32_t* d_is_leaf)
 {
@@ -1101,9 +1171,30 @@ int32_t mrh_svo_cone_trace(mrh_context* ctx, const mrh_svo* svo, const float* d_
     if (n == 0) return MRH_OK;
     if (!d_rays || !d_nodes || !d_t) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_cone_trace: null rays or outputs");
     return launch(ctx, "svo_cone_trace", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_svo_cone_trace,
-                  (const uint32_t*)svo->first_child, (const uint32_t*)svo->child_mask, (const uint32_t*)svo->parent,
-                  svo->depth, make_float4(svo->cube[0], svo->cube[1], svo->cube[2], svo->cube[3]), d_rays, n, aperture,
-                  max_query_level_offset, d_nodes, d_t, d_is_leaf);
+                  cone_tree(svo), d_rays, n, aperture, max_query_level_offset, d_nodes, d_t, d_is_leaf);
+}
+
+int32_t mrh_wfpg_trace_fields(mrh_context* ctx, const mrh_svo* svo, const mrh_wfpg_trace_desc* d)
+{
+    if (!ctx || !svo || !d || svo->ctx != ctx) return MRH_ERR_INVALID;
+    if (!svo->nodes) return fail(ctx, MRH_ERR_INVALID, "mrh_wfpg_trace_fields: the octree is not built");
+    const auto side_ok = [](uint32_t s) { return s == 8 || s == 16 || s == 32 || s == 64; };
+    if (!side_ok(d->field_x) || !side_ok(d->field_y))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_wfpg_trace_fields: field sides must be 8, 16, 32 or 64");
+    if (!std::isfinite(d->boundary_radiance))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_wfpg_trace_fields: boundary radiance must be finite");
+    if (d->bin_count == 0) return MRH_OK;
+    if (!d->bin_origins || !d->bin_jitter || !d->node_irradiance || !d->node_normals || !d->out_fields)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_wfpg_trace_fields: null input or output");
+    // OctohedralConeAperture (WFPGTracer.cu:134-141): 4 pi over the texel
+    // count, which the reference forms as field_x * field_x
+    const float aperture = (4.0f * 3.14159265358979323846f) / ((float)d->field_x * (float)d->field_x);
+    const uint64_t total = (uint64_t)d->bin_count * d->field_x * d->field_y;
+    return launch(ctx, "wfpg_trace_fields", dim3(stride_grid(ctx, total, 256)), dim3(256), 0, k_wfpg_trace_fields,
+                  cone_tree(svo), d->field_x, d->field_y, (uint32_t)__builtin_ctz(d->field_x), d->bin_count,
+                  (const float4*)d->bin_origins, (const float2*)d->bin_jitter, aperture, d->max_query_level_offset,
+                  (const float2*)d->node_irradiance, d->node_normals, d->boundary_radiance, d->out_fields,
+                  d->out_nodes, d->out_t);
 }
 
 int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes, uint32_t n, uint32_t min_level,

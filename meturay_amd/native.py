@@ -88,6 +88,12 @@ class WfpgFieldsDesc(C.Structure):
                 ("out_filtered", _P)]
 
 
+class WfpgTraceDesc(C.Structure):
+    _fields_ = [("field_x", _U32), ("field_y", _U32), ("bin_count", _U32), ("max_query_level_offset", _U32),
+                ("boundary_radiance", _F), ("bin_origins", _P), ("bin_jitter", _P), ("node_irradiance", _P),
+                ("node_normals", _P), ("out_fields", _P), ("out_nodes", _P), ("out_t", _P)]
+
+
 class SvoView(C.Structure):
     _fields_ = [("depth", _U32), ("nodes", _U32), ("level_count", _U32 * 11),
                 ("level_offset", _U32 * 12), ("cube", _F * 4), ("d_codes", _P),
@@ -210,6 +216,7 @@ _SIGS = {
     "mrh_wfpg_sample_fields": (_I32, [_P, C.POINTER(WfpgFieldsDesc)]),
     "mrh_svo_voxelize": (_I32, [_P, _P, _U32, C.POINTER(_F * 4), _U32, _P, _U32, C.POINTER(_U32)]),
     "mrh_svo_cone_trace": (_I32, [_P, _P, _P, _U32, _F, _U32, _P, _P, _P]),
+    "mrh_wfpg_trace_fields": (_I32, [_P, _P, C.POINTER(WfpgTraceDesc)]),
     "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                  C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
     "mrh_server_create": (_I32, [_P, C.POINTER(ServerParams), C.POINTER(_P)]),
@@ -729,6 +736,13 @@ class Svo:
         """mrh_svo_cone_trace over rays [n][7] (origin, direction, tmin)."""
         self.ctx.check(lib().mrh_svo_cone_trace(self.ctx.h, self.h, ptr(d_rays), n, aperture, max_query_level_offset,
                                                 ptr(d_nodes), ptr(d_t), ptr(d_is_leaf)))
+
+    def trace_fields(self, field_x: int, field_y: int, bin_count: int, d_origins, d_jitter, d_irradiance, d_normals,
+                     d_fields, boundary: float = 0.0, offset: int = 0, d_nodes=None, d_t=None) -> None:
+        """mrh_wfpg_trace_fields: cone-traced radiance fields of the bins."""
+        d = WfpgTraceDesc(field_x, field_y, bin_count, offset, boundary, ptr(d_origins), ptr(d_jitter),
+                          ptr(d_irradiance), ptr(d_normals), ptr(d_fields), ptr(d_nodes), ptr(d_t))
+        self.ctx.check(lib().mrh_wfpg_trace_fields(self.ctx.h, self.h, C.byref(d)))
 
     def sample(self, d_nodes, d_normals, d_xi, n: int, d_dirs, d_pdf) -> None:
         self.ctx.check(lib().mrh_svo_sample(self.ctx.h, self.h, ptr(d_nodes), ptr(d_normals), ptr(d_xi), n,

@@ -292,3 +292,40 @@ void om_svo_cone_trace_batch(const om_svo* s, const float* rays, uint32_t n, flo
         leaf[i] = (uint32_t)lf;
     }
 }
+
+/* GenerateRadianceField's trace and radiance query (WFPGTracerKC.cuh:
+ * 1252-1318, RecursiveConeTrace.cuh:219-247, AnisoSVO.cuh:433-489,
+ * 1035-1053) for every texel of every bin: the jittered concentric-
+ * octahedral direction (y up), a cone of 4 pi / X^2 (OctohedralConeAperture,
+ * WFPGTracer.cu:134-141) from the bin's origin and tmin, and the stopped
+ * node's front or back radiance by its normal (boundary radiance on a
+ * miss).  origins: [bin][4]; jitter: [bin][2]; irr: [node][2]; normals:
+ * [node][3]; fields / nodes / t: [bin][Y][X] (nodes, t may be NULL). */
+void om_wfpg_trace_fields(const om_svo* s, uint32_t X, uint32_t Y, uint32_t bins, const float* origins,
+                          const float* jitter, const float* irr, const float* normals, float boundary,
+                          uint32_t offset, float* fields, uint32_t* nodes, float* t)
+{
+    const float aperture = (4.0f * 3.14159265358979323846f) / ((float)X * (float)X);
+    for (uint32_t b = 0; b < bins; b++)
+        for (uint32_t y = 0; y < Y; y++)
+            for (uint32_t x = 0; x < X; x++) {
+                const size_t i = ((size_t)b * Y + y) * X + x;
+                const float uv[2] = {((float)x + jitter[2 * b]) / (float)X, ((float)y + jitter[2 * b + 1]) / (float)Y};
+                float z[3];
+                om_octa_to_dirs(uv, 1, z);
+                const float d[3] = {z[1], z[2], z[0]};
+                uint32_t node;
+                int leaf;
+                const float tt = om_svo_cone_trace(s, origins + 4 * (size_t)b, d, origins[4 * (size_t)b + 3], aperture,
+                                                   offset, &node, &leaf);
+                float rad = boundary;
+                if (node != OM_SVO_NO_NODE) {
+                    const float* nrm = normals + 3 * (size_t)node;
+                    const int front = (nrm[0] * -d[0] + nrm[1] * -d[1] + nrm[2] * -d[2]) >= 0.0f;
+                    rad = front ? irr[2 * (size_t)node] : irr[2 * (size_t)node + 1];
+                }
+                fields[i] = rad;
+                if (nodes) nodes[i] = node;
+                if (t) t[i] = tt;
+            }
+}

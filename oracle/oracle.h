@@ -372,6 +372,10 @@ float    om_svo_cone_trace(const om_svo* s, const float o[3], const float dir[3]
                            uint32_t max_query_level_offset, uint32_t* node, int* is_leaf);
 void     om_svo_cone_trace_batch(const om_svo* s, const float* rays, uint32_t n, float aperture, uint32_t offset,
                                  uint32_t* nodes, float* t, uint32_t* leaf);
+/* cone-traced radiance fields of bins (WFPGTracerKC.cuh:1252-1318) */
+void     om_wfpg_trace_fields(const om_svo* s, uint32_t X, uint32_t Y, uint32_t bins, const float* origins,
+                              const float* jitter, const float* irr, const float* normals, float boundary,
+                              uint32_t offset, float* fields, uint32_t* nodes, float* t);
 
 #ifdef __cplusplus
 }

@@ -147,6 +147,8 @@ def lib():
         L.om_wfpg_sample_fields.argtypes = [C.POINTER(OmWfpgDesc)]
         L.om_voxelize.restype = C.c_uint64
         L.om_svo_cone_trace_batch.argtypes = [C.POINTER(OmSvo), _P, _U32, _F, _U32, _P, _P, _P]
+        L.om_wfpg_trace_fields.argtypes = [C.POINTER(OmSvo), _U32, _U32, _U32, _P, _P, _P, _P, _F, _U32, _P, _P,
+                                           _P]
         L.om_voxelize.argtypes = [_P, _U32, _P, _U32, _P, C.c_uint64]
         L.om_trace_closest.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F,
                                        _P, _P, _P, _P]
@@ -762,6 +764,20 @@ class Svo:
         lib().om_svo_cone_trace_batch(C.byref(self._s), _ptr(rays), n, aperture, max_query_level_offset,
                                       _ptr(nodes), _ptr(t), _ptr(leaf))
         return nodes, t, leaf
+
+    def trace_fields(self, X: int, Y: int, origins, jitter, irr, normals, boundary: float = 0.0, offset: int = 0):
+        """om_wfpg_trace_fields: (fields [bins][Y][X], nodes, t)."""
+        origins = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 4)
+        jitter = np.ascontiguousarray(jitter, dtype=np.float32).reshape(-1, 2)
+        irr = np.ascontiguousarray(irr, dtype=np.float32).reshape(-1, 2)
+        normals = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 3)
+        bins = len(origins)
+        fields = np.zeros((bins, Y, X), np.float32)
+        nodes = np.zeros((bins, Y, X), np.uint32)
+        t = np.zeros((bins, Y, X), np.float32)
+        lib().om_wfpg_trace_fields(C.byref(self._s), X, Y, bins, _ptr(origins), _ptr(jitter), _ptr(irr),
+                                   _ptr(normals), boundary, offset, _ptr(fields), _ptr(nodes), _ptr(t))
+        return fields, nodes, t
 
     def lookup(self, codes: np.ndarray) -> np.ndarray:
         return np.array([lib().om_svo_lookup(C.byref(self._s), int(c)) for c in codes], dtype=np.uint32)

@@ -122,3 +122,49 @@ def test_rejects_bad_arguments(ctx, oracle):
     empty = N.Svo(ctx)
     with pytest.raises(N.MrhError):
         empty.cone_trace(r, 1, 0.0, 0, nn, tt)
+
+
+@pytest.mark.parametrize("side", [64, 16])
+def test_trace_fields_match_oracle_and_feed_sampling(ctx, oracle, side):
+    """Cone-traced fields of 24 bins in the voxelised Cornell box (depth 10),
+    then one guided sample per ray from them: both against the oracle."""
+    depth = 10
+    tris = np.ascontiguousarray(N.gen_cornell()["tris"], np.float32).reshape(-1, 9)
+    p = tris.reshape(-1, 3)
+    cube = oracle.svo_cube(p.min(0), p.max(0), depth)
+    codes = oracle.voxelize(tris, cube, depth)
+    svo, o = build_both(ctx, oracle, codes, cube, depth)
+    rng = np.random.default_rng(side)
+    n = o.nodes
+    irr = rng.gamma(0.7, 1.5, (n, 2)).astype(np.float32)
+    irr[rng.random((n, 2)) < 0.2] = 0.0
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    bins = 24
+    lo, hi = p.min(0), p.max(0)
+    origins = np.hstack([lo + (hi - lo) * rng.uniform(0.1, 0.9, (bins, 3)),
+                         np.full((bins, 1), float(cube[3]) / 1024 * 0.15 + 1e-5)]).astype(np.float32)
+    jit = rng.random((bins, 2)).astype(np.float32)
+    d_o, d_j, d_i, d_n = dev(origins.reshape(-1)), dev(jit.reshape(-1)), dev(irr.reshape(-1)), dev(nrm.reshape(-1))
+    d_f = torch.empty(bins * side * side, device="cuda")
+    d_nodes = torch.empty(bins * side * side, dtype=torch.int32, device="cuda")
+    d_t = torch.empty(bins * side * side, device="cuda")
+    torch.cuda.synchronize()
+    svo.trace_fields(side, side, bins, d_o, d_j, d_i, d_n, d_f, boundary=0.5, d_nodes=d_nodes, d_t=d_t)
+    ctx.sync()
+    of, on, ot = o.trace_fields(side, side, origins, jit, irr, nrm, boundary=0.5)
+    assert np.array_equal(host(d_nodes, np.uint32), on.ravel())
+    assert np.array_equal(d_t.cpu().numpy().view(np.uint32), ot.ravel().view(np.uint32))
+    assert np.array_equal(d_f.cpu().numpy(), of.ravel())
+    # the fields drive the guided samples of each bin's rays
+    n_rays = 6000
+    offsets = np.linspace(0, n_rays, bins + 1).astype(np.uint32)
+    ids = rng.permutation(n_rays).astype(np.uint32)
+    xi = rng.random((n_rays, 2)).astype(np.float32)
+    d_uv = torch.full((n_rays, 2), float("nan"), device="cuda")
+    d_pdf = torch.full((n_rays,), float("nan"), device="cuda")
+    d_off, d_ids, d_xi = dev(offsets), dev(ids), dev(xi)
+    torch.cuda.synchronize()
+    ctx.wfpg_sample_fields(d_f, side, side, bins, d_off, d_ids, n_rays, d_xi, d_uv, d_pdf)
+    ctx.sync()
+    want = oracle.wfpg_sample_fields(of, offsets, ids, xi, alpha=1.0, ray_count=n_rays)
+    assert np.array_equal(d_uv.cpu().numpy(), want["uv"]) and np.array_equal(d_pdf.cpu().numpy(), want["pdf"])

@@ -173,3 +173,60 @@ def test_rays_leaving_the_cube_miss(oracle):
     R = np.array([[2, 2, 2, 1, 1, 1, 0], [-3, 0, 0, -1, 0, 0, 0], [0, 0, 0, 0, 0, 1, 5.0]], np.float32)
     nodes, t, leaf = svo.cone_trace(R, 0.0, 0)
     assert (nodes == 0xFFFFFFFF).all() and (leaf == 0).all()
+
+
+def _room(oracle, depth):
+    """A closed unit cube of 12 triangles, voxelised, and its octree."""
+    c = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float32)
+    quads = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (3, 2, 6, 7), (0, 3, 7, 4), (1, 2, 6, 5)]
+    tris = []
+    for a, b, cc, d in quads:
+        tris += [np.concatenate([c[a], c[b], c[cc]]), np.concatenate([c[a], c[cc], c[d]])]
+    tris = np.array(tris, np.float32)
+    cube = oracle.svo_cube([0, 0, 0], [1, 1, 1], depth)
+    svo = oracle.Svo.build(oracle.voxelize(tris, cube, depth), cube.tolist(), depth)
+    return svo, cube
+
+
+def test_trace_fields_compose_the_cone_trace(oracle):
+    depth, X, Y = 6, 16, 8
+    svo, cube = _room(oracle, depth)
+    rng = np.random.default_rng(9)
+    n = svo.nodes
+    irr = rng.uniform(0, 5, (n, 2)).astype(np.float32)
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    origins = np.hstack([rng.uniform(0.2, 0.8, (3, 3)), np.full((3, 1), 0.01)]).astype(np.float32)
+    jit = rng.random((3, 2)).astype(np.float32)
+    fields, nodes, t = svo.trace_fields(X, Y, origins, jit, irr, nrm, boundary=0.25)
+    aperture = np.float32(4 * np.pi) / np.float32(X * X)
+    for b in range(3):
+        u = ((np.arange(X, dtype=np.float32)[None, :] + jit[b, 0]) / np.float32(X)).repeat(Y, 0)
+        v = ((np.arange(Y, dtype=np.float32)[:, None] + jit[b, 1]) / np.float32(Y)).repeat(X, 1)
+        z = oracle.octa_to_dirs(np.stack([u.ravel(), v.ravel()], 1))
+        d = np.stack([z[:, 1], z[:, 2], z[:, 0]], 1)                  # y up
+        rays = np.hstack([np.repeat(origins[b:b + 1, :3], X * Y, 0), d, np.repeat(origins[b:b + 1, 3:], X * Y, 0)])
+        on, ot, _ = svo.cone_trace(rays, float(aperture), 0)
+        assert np.array_equal(nodes[b].ravel(), on) and np.array_equal(t[b].ravel(), ot)
+        for k in range(X * Y):
+            want = np.float32(0.25) if on[k] == 0xFFFFFFFF else \
+                irr[on[k], 0 if float(np.dot(nrm[on[k]], -d[k])) >= 0 else 1]
+            assert fields[b].ravel()[k] == want
+
+
+def test_closed_room_walls_fill_a_sixth_of_the_field_each(oracle):
+    """From the centre of a cube room, each wall covers 4 pi / 6 of the sphere;
+    the concentric octahedral map is equal-area, so about a sixth of the
+    texels see each wall."""
+    depth, X = 7, 64
+    svo, cube = _room(oracle, depth)
+    lv, blo, bsz = node_boxes(svo, cube, depth)
+    ctr = blo + bsz[:, None] / 2
+    dist = np.stack([ctr[:, 0], 1 - ctr[:, 0], ctr[:, 1], 1 - ctr[:, 1], ctr[:, 2], 1 - ctr[:, 2]], 1)
+    wall = np.argmin(dist, 1).astype(np.float32) + 1
+    irr = np.stack([wall, wall], 1)
+    nrm = np.tile(np.array([[0, 0, 1]], np.float32), (svo.nodes, 1))
+    fields, nodes, _ = svo.trace_fields(X, X, np.array([[0.5, 0.5, 0.5, 0.0]], np.float32),
+                                        np.array([[0.5, 0.5]], np.float32), irr, nrm)
+    assert (nodes != 0xFFFFFFFF).all()
+    frac = np.array([(fields == k).mean() for k in range(1, 7)])
+    assert np.allclose(frac, 1 / 6, atol=0.02), frac

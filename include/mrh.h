@@ -512,11 +512,12 @@ int32_t mrh_svo_bin_rays(mrh_context* ctx, mrh_svo* svo, const uint32_t* d_codes
  * triangle, pixels row by row; shared edges give duplicates, which
  * mrh_svo_build folds.  *out_count = the number of records; they are written
  * to d_codes only when it holds them all (count <= capacity), so a call with
- * d_codes NULL sizes the buffer.  The reference pads the scene bounds by one
+ * d_codes NULL sizes the buffer; d_prims (may be NULL) receives each record's
+ * triangle index (the payload's normals come from it).  The reference pads the scene bounds by one
  * leaf on every side of its largest span to get the cube
  * (AnisoSVO.cu:670-678).  Synchronises the context's stream. */
 int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, const float cube[4], uint32_t depth,
-                         uint32_t* d_codes, uint32_t capacity, uint32_t* out_count);
+                         uint32_t* d_codes, uint32_t* d_prims, uint32_t capacity, uint32_t* out_count);
 
 /* Cone trace through the octree.  Replaces AnisoSVOctreeGPU::ConeTraceRay
  * (AnisoSVO.cuh:767-1032; as GenerateRadianceField's batched trace calls it,
@@ -533,6 +534,42 @@ int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, cons
  * bound). */
 int32_t mrh_svo_cone_trace(mrh_context* ctx, const mrh_svo* svo, const float* d_rays, uint32_t n, float aperture,
                            uint32_t max_query_level_offset, uint32_t* d_nodes, float* d_t, uint32_t* d_is_leaf);
+
+/* The octree's node payload for the field trace.  Replaces the reference's
+ * VoxelPayload set-up (AnisoSVO.cuh:23-115): leaf normals from its voxel
+ * records (KCReduceVoxelPayload, AnisoSVOKC.cuh:8-230), internal normals
+ * bottom-up (KCAverageNormals, AnisoSVO.cu:419-520), the path vertices'
+ * radiance into the nearest leaf's front or back side
+ * (KCAccumulateRadianceToLeaf, :225-263, NearestNodeIndex, AnisoSVO.cuh:
+ * 1168-1240, WriteLeafRadiance, :491-536) and NormalizeAndFilterRadiance
+ * (:1415-1483, 530-656).  Normals: a 2-means clustering (4 rounds, seeded
+ * with the first normal and its negation) of the unit normals of a leaf's
+ * records' triangles, in record order, and of the children's average
+ * normals for a node; a node's normal is the larger cluster's average
+ * (node_normals: its direction; node_normal_length, optional: its length,
+ * below 1 where the normals spread).  Radiance: each vertex (7 floats:
+ * position, wo toward the previous path vertex, luminance) adds to the side
+ * of the nearest existing leaf (its own cell, else the neighbour cells
+ * nearest first) that wo faces; sums are 48.16 fixed point, so the result does
+ * not depend on the order of the adds (the reference adds floats).  Leaf
+ * sides average their samples; node sides average their children's sides
+ * that have samples (a child whose normal opposes the node's swaps its two
+ * sides); a side still without samples takes its parent's value, top down;
+ * one without samples anywhere up to the root keeps 65504 (the reference's
+ * half-float maximum).  The octree must have been built from the records'
+ * codes (mrh_svo_voxelize with d_prims). */
+typedef struct {
+    const float*    tris;               /* [n_tris][9]: the voxelised triangles */
+    const uint32_t* codes;              /* [n_records] voxel records (mrh_svo_voxelize) */
+    const uint32_t* prims;              /* [n_records] their triangles */
+    uint32_t        n_records;
+    const float*    vertices;           /* [n_vertices][7] path vertices */
+    uint32_t        n_vertices;
+    float*          node_irradiance;    /* [nodes][2] front, back */
+    float*          node_normals;       /* [nodes][3] */
+    float*          node_normal_length; /* optional [nodes] */
+} mrh_svo_payload_desc;
+int32_t mrh_svo_payload(mrh_context* ctx, const mrh_svo* svo, const mrh_svo_payload_desc* desc);
 
 /* The radiance fields of the bins, by cone tracing.  Replaces
  * GenerateRadianceField's trace and radiance query (WFPGTracerKC.cuh:

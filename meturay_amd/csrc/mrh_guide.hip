@@ -494,7 +494,8 @@ __device__ inline bool vox_pixel(const VoxSetup& s, const float cube[4], uint32_
 // rank] = the records
 __global__ __launch_bounds__(256) void k_voxelize(const float* __restrict__ tris, uint32_t n, float4 cube4,
                                                   uint32_t depth, const uint32_t* __restrict__ offsets,
-                                                  uint32_t* __restrict__ out, unsigned long long* __restrict__ total)
+                                                  uint32_t* __restrict__ out, uint32_t* __restrict__ prims,
+                                                  unsigned long long* __restrict__ total)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
@@ -514,7 +515,11 @@ __global__ __launch_bounds__(256) void k_voxelize(const float* __restrict__ tris
             bool hit = false;
             if (k < area) hit = vox_pixel(s, cube, R, s.x0 + (int)(k % (uint32_t)w), s.y0 + (int)(k / (uint32_t)w), code);
             const uint64_t m = __ballot(hit);
-            if (offsets && hit) out[run + (uint32_t)__popcll(m & below)] = code;
+            if (offsets && hit) {
+                const uint32_t at = run + (uint32_t)__popcll(m & below);
+                out[at] = code;
+                if (prims) prims[at] = t;
+            }
             run += (uint32_t)__popcll(m);
         }
         if (!offsets && lane == 0) {
@@ -673,6 +678,263 @@ __global__ __launch_bounds__(256) void k_wfpg_trace_fields(ConeTree tr, uint32_t
         fields[i] = rad;
         if (out_node) out_node[i] = node;
         if (out_t) out_t[i] = t;
+    }
+}
+
+// ---------------------------------------------------------------- payload
+// The octree's node payload as the reference forms it (VoxelPayload,
+// AnisoSVO.cuh:23-115): normals from a 2-means clustering of the voxel
+// records' triangle normals (KCReduceVoxelPayload, AnisoSVOKC.cuh:8-230) and
+// of the children's average normals bottom-up (KCAverageNormals,
+// AnisoSVO.cu:419-520); path-vertex radiance splatted into the nearest
+// leaf's front or back side (KCAccumulateRadianceToLeaf, :225-263; 48.16
+// fixed point, order independent), averaged, averaged bottom-up over the
+// children's valid sides (swapping a child whose normal opposes) and filled
+// top-down (NormalizeAndFilterRadiance, :1415-1483).
+constexpr float PAYLOAD_INVALID = 65504.0f;   // the reference's half maximum
+
+struct Vec3 {
+    float v[3];
+};
+
+// 2-means of up to n vectors (zero vectors skipped), 4 rounds seeded with
+// the first non-zero vector and its negation; a vector joins the mean it is
+// farther from (the reference's rule: the labels swap each round).  Writes
+// the larger cluster's average as unit normal + length.
+template <class Get>
+__device__ inline float4 kmeans2_dev(uint32_t n, Get get)
+{
+    uint32_t first = n;
+    for (uint32_t i = 0; i < n; i++) {
+        const Vec3 a = get(i);
+        if (!(a.v[0] == 0.0f && a.v[1] == 0.0f && a.v[2] == 0.0f)) {
+            first = i;
+            break;
+        }
+    }
+    if (first == n) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const Vec3 f = get(first);
+    float m[2][3] = {{f.v[0], f.v[1], f.v[2]}, {-f.v[0], -f.v[1], -f.v[2]}};
+    uint32_t cnt[2] = {0, 0};
+    for (int it = 0; it < 4; it++) {
+        for (int k = 0; k < 2; k++) {
+            const float inv = 1.0f / sqrtf(m[k][0] * m[k][0] + m[k][1] * m[k][1] + m[k][2] * m[k][2]);
+            m[k][0] *= inv;
+            m[k][1] *= inv;
+            m[k][2] *= inv;
+        }
+        float acc[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        cnt[0] = cnt[1] = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            const Vec3 a = get(i);
+            if (a.v[0] == 0.0f && a.v[1] == 0.0f && a.v[2] == 0.0f) continue;
+            const float d0 = 1.0f - (m[0][0] * a.v[0] + m[0][1] * a.v[1] + m[0][2] * a.v[2]);
+            const float d1 = 1.0f - (m[1][0] * a.v[0] + m[1][1] * a.v[1] + m[1][2] * a.v[2]);
+            const int k = (d0 >= d1) ? 0 : 1;
+            acc[k][0] += a.v[0];
+            acc[k][1] += a.v[1];
+            acc[k][2] += a.v[2];
+            cnt[k]++;
+        }
+        for (int k = 0; k < 2; k++)
+            for (int c = 0; c < 3; c++) m[k][c] = acc[k][c];
+    }
+    const int k = (cnt[0] >= cnt[1]) ? 0 : 1;
+    const float r = 1.0f / (float)cnt[k];
+    const float mean[3] = {m[k][0] * r, m[k][1] * r, m[k][2] * r};
+    const float len = sqrtf(mean[0] * mean[0] + mean[1] * mean[1] + mean[2] * mean[2]);
+    const float il = 1.0f / len;
+    return make_float4(mean[0] * il, mean[1] * il, mean[2] * il, len);
+}
+
+// the leaf of a cell code, SVO_NO_NODE when the tree lacks it
+__device__ inline uint32_t leaf_of_code(const uint32_t* __restrict__ first_child, const uint32_t* __restrict__ mask,
+                                        uint32_t depth, uint32_t code)
+{
+    uint32_t node = 0;
+    for (uint32_t l = 0; l < depth; l++) {
+        const uint32_t oct = (code >> (3u * (depth - 1u - l))) & 7u;
+        const uint32_t m = mask[node];
+        if (!((m >> oct) & 1u)) return SVO_NO_NODE;
+        node = first_child[node] + (uint32_t)__popc(m & ((1u << oct) - 1u));
+    }
+    return node;
+}
+
+__global__ void k_pl_keys(const uint32_t* __restrict__ codes, uint32_t n, const uint32_t* __restrict__ first_child,
+                          const uint32_t* __restrict__ mask, uint32_t depth, uint32_t leaf0, uint32_t* __restrict__ keys)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t lf = leaf_of_code(first_child, mask, depth, codes[i]);
+        keys[i] = (lf == SVO_NO_NODE) ? 0xFFFFFFFFu : lf - leaf0;
+    }
+}
+
+// [begin, end) of each leaf's run in the sorted keys
+__global__ void k_pl_runs(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ begin,
+                          uint32_t* __restrict__ end)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t k = keys[i];
+        if (k == 0xFFFFFFFFu) continue;
+        if (i == 0 || keys[i - 1] != k) begin[k] = i;
+        if (i == n - 1 || keys[i + 1] != k) end[k] = i + 1;
+    }
+}
+
+__global__ void k_pl_leaf_normals(uint32_t leaves, uint32_t leaf0, const uint32_t* __restrict__ begin,
+                                  const uint32_t* __restrict__ end, const uint32_t* __restrict__ order,
+                                  const uint32_t* __restrict__ prims, const float* __restrict__ tris,
+                                  float4* __restrict__ nrm)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < leaves; i += gridDim.x * blockDim.x) {
+        const uint32_t b = begin[i], e = end[i];
+        nrm[leaf0 + i] = kmeans2_dev(e - b, [&](uint32_t k) {
+            const float* t = tris + 9 * (size_t)prims[order[b + k]];
+            const float ea[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
+            const float eb[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
+            const float nn[3] = {ea[1] * eb[2] - ea[2] * eb[1], ea[2] * eb[0] - ea[0] * eb[2],
+                                 ea[0] * eb[1] - ea[1] * eb[0]};
+            const float inv = 1.0f / sqrtf(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            Vec3 r;
+            r.v[0] = nn[0] * inv;
+            r.v[1] = nn[1] * inv;
+            r.v[2] = nn[2] * inv;
+            return r;
+        });
+    }
+}
+
+__global__ void k_pl_node_normals(uint32_t off, uint32_t count, const uint32_t* __restrict__ first_child,
+                                  const uint32_t* __restrict__ mask, float4* __restrict__ nrm)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t nd = off + i, fc = first_child[nd], k = (uint32_t)__popc(mask[nd]);
+        nrm[nd] = kmeans2_dev(k, [&](uint32_t c) {
+            const float4 q = nrm[fc + c];
+            Vec3 r;
+            r.v[0] = q.x * q.w;
+            r.v[1] = q.y * q.w;
+            r.v[2] = q.z * q.w;
+            return r;
+        });
+    }
+}
+
+__global__ void k_pl_splat(const float* __restrict__ verts, uint32_t n, const uint32_t* __restrict__ first_child,
+                           const uint32_t* __restrict__ mask, uint32_t depth, float4 cube, const float4* __restrict__ nrm,
+                           unsigned long long* __restrict__ tot, uint32_t* __restrict__ num)
+{
+    const uint32_t R = 1u << depth;
+    const float lo[3] = {cube.x, cube.y, cube.z};
+    const float leaf = cube.w / (float)R;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        const float* x = verts + 7 * (size_t)v;
+        bool inside = true;
+        for (int a = 0; a < 3; a++) inside = inside && !(x[a] < lo[a] || x[a] > lo[a] + cube.w);
+        if (!inside) continue;
+        float fi[3], dist[3];
+        int inc[3];
+        for (int a = 0; a < 3; a++) {
+            const float l = (x[a] - lo[a]) / leaf;
+            fi[a] = floorf(l);
+            const float fr = l - fi[a];
+            inc[a] = (fr < 0.5f) ? -1 : 1;
+            dist[a] = 0.5f - fabsf(fr - 0.5f);
+        }
+        int m0 = 0;
+        if (dist[1] < dist[m0]) m0 = 1;
+        if (dist[2] < dist[m0]) m0 = 2;
+        const int o1 = (m0 + 1) % 3, o2 = (m0 + 2) % 3;
+        const int m1 = (dist[o2] < dist[o1]) ? o2 : o1;
+        const int m2 = 3 - m0 - m1;
+        uint32_t lf = SVO_NO_NODE;
+        for (uint32_t i = 0; i < 8 && lf == SVO_NO_NODE; i++) {
+            int sh[3];
+            sh[m0] = (int)(i & 1u);
+            sh[m1] = (int)((i >> 1) & 1u);
+            sh[m2] = (int)((i >> 2) & 1u);
+            long long q[3];
+            bool ok = true;
+            for (int a = 0; a < 3; a++) {
+                q[a] = (long long)fi[a] + sh[a] * inc[a];
+                ok = ok && q[a] >= 0 && q[a] < (long long)R;
+            }
+            if (!ok) continue;
+            const uint32_t code =
+                expand10((uint32_t)q[0]) | (expand10((uint32_t)q[1]) << 1) | (expand10((uint32_t)q[2]) << 2);
+            lf = leaf_of_code(first_child, mask, depth, code);
+        }
+        if (lf == SVO_NO_NODE) continue;
+        const float4 nn = nrm[lf];
+        const int side = (nn.x * x[3] + nn.y * x[4] + nn.z * x[5] >= 0.0f) ? 0 : 1;
+        const float lum = x[6];
+        const unsigned long long f =
+            (lum > 0.0f) ? (unsigned long long)(fminf(lum, 2147483647.0f) * 65536.0f) : 0ull;
+        atomicAdd(&tot[2 * (size_t)lf + side], f);
+        atomicAdd(&num[2 * (size_t)lf + side], 1u);
+    }
+}
+
+__global__ void k_pl_leaf_avg(uint32_t leaf0, uint32_t leaves, const unsigned long long* __restrict__ tot,
+                              const uint32_t* __restrict__ num, float2* __restrict__ irr)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < leaves; i += gridDim.x * blockDim.x) {
+        const size_t lf = leaf0 + i;
+        float r[2];
+        for (int j = 0; j < 2; j++) {
+            const uint32_t c = num[2 * lf + j];
+            r[j] = c ? ((float)tot[2 * lf + j] * (1.0f / 65536.0f)) / (float)c : PAYLOAD_INVALID;
+        }
+        irr[lf] = make_float2(r[0], r[1]);
+    }
+}
+
+__global__ void k_pl_bottom_up(uint32_t off, uint32_t count, const uint32_t* __restrict__ first_child,
+                               const uint32_t* __restrict__ mask, const float4* __restrict__ nrm, float2* __restrict__ irr)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t nd = off + i, fc = first_child[nd], k = (uint32_t)__popc(mask[nd]);
+        const float4 pn = nrm[nd];
+        float sum[2] = {0.0f, 0.0f};
+        uint32_t c[2] = {0, 0};
+        for (uint32_t q = 0; q < k; q++) {
+            const float4 cn = nrm[fc + q];
+            const bool swap = (cn.x * pn.x + cn.y * pn.y + cn.z * pn.z) < 0.0f;
+            const float2 ci = irr[fc + q];
+            const float vals[2] = {swap ? ci.y : ci.x, swap ? ci.x : ci.y};
+            for (int j = 0; j < 2; j++)
+                if (vals[j] != PAYLOAD_INVALID) {
+                    sum[j] += vals[j];
+                    c[j]++;
+                }
+        }
+        irr[nd] = make_float2(c[0] ? sum[0] * (1.0f / (float)c[0]) : PAYLOAD_INVALID,
+                              c[1] ? sum[1] * (1.0f / (float)c[1]) : PAYLOAD_INVALID);
+    }
+}
+
+__global__ void k_pl_top_down(uint32_t off, uint32_t count, const uint32_t* __restrict__ parent, float2* __restrict__ irr)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint32_t nd = off + i;
+        float2 v = irr[nd];
+        const float2 p = irr[parent[nd]];
+        if (v.x == PAYLOAD_INVALID) v.x = p.x;
+        if (v.y == PAYLOAD_INVALID) v.y = p.y;
+        irr[nd] = v;
+    }
+}
+
+__global__ void k_pl_out(uint32_t nodes, const float4* __restrict__ nrm, float* __restrict__ normals,
+                         float* __restrict__ lengths)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nodes; i += gridDim.x * blockDim.x) {
+        const float4 q = nrm[i];
+        normals[3 * (size_t)i] = q.x;
+        normals[3 * (size_t)i + 1] = q.y;
+        normals[3 * (size_t)i + 2] = q.z;
+        if (lengths) lengths[i] = q.w;
     }
 }
 }  // namespace
@@ -1111,7 +1373,7 @@ using namespace mrh;
 extern "C" {
 
 int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, const float cube[4], uint32_t depth,
-                         uint32_t* d_codes, uint32_t capacity, uint32_t* out_count)
+                         uint32_t* d_codes, uint32_t* d_prims, uint32_t capacity, uint32_t* out_count)
 {
     if (!ctx || !cube || !out_count) return MRH_ERR_INVALID;
     *out_count = 0;
@@ -1134,7 +1396,7 @@ int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, cons
     MRH_TRY(fill_u32(ctx, (uint32_t*)mem, 0u, 2));
     MRH_TRY(fill_u32(ctx, counts + n, 0u, 1));
     MRH_TRY(launch(ctx, "voxelize_count", dim3(grid), dim3(256), 0, k_voxelize, d_tris, n, c4, depth,
-                   (const uint32_t*)nullptr, counts, total));
+                   (const uint32_t*)nullptr, counts, (uint32_t*)nullptr, total));
     unsigned long long h_total = 0;
     MRH_HIP_CHECK(ctx, hipMemcpyAsync(&h_total, total, 8, hipMemcpyDeviceToHost, ctx->stream));
     MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1144,7 +1406,7 @@ int32_t mrh_svo_voxelize(mrh_context* ctx, const float* d_tris, uint32_t n, cons
     if (!d_codes || h_total > capacity || h_total == 0) return MRH_OK;   // a size query
     MRH_TRY(scan_exclusive(ctx, counts, counts, n + 1, tmp));
     return launch(ctx, "voxelize", dim3(grid), dim3(256), 0, k_voxelize, d_tris, n, c4, depth,
-                  (const uint32_t*)counts, d_codes, (unsigned long long*)nullptr);
+                  (const uint32_t*)counts, d_codes, d_prims, (unsigned long long*)nullptr);
 }
 
 static ConeTree cone_tree(const mrh_svo* s)
@@ -1172,6 +1434,83 @@ int32_t mrh_svo_cone_trace(mrh_context* ctx, const mrh_svo* svo, const float* d_
     if (!d_rays || !d_nodes || !d_t) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_cone_trace: null rays or outputs");
     return launch(ctx, "svo_cone_trace", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_svo_cone_trace,
                   cone_tree(svo), d_rays, n, aperture, max_query_level_offset, d_nodes, d_t, d_is_leaf);
+}
+
+int32_t mrh_svo_payload(mrh_context* ctx, const mrh_svo* svo, const mrh_svo_payload_desc* d)
+{
+    if (!ctx || !svo || !d || svo->ctx != ctx) return MRH_ERR_INVALID;
+    if (!svo->nodes) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_payload: the octree is not built");
+    if (!d->node_irradiance || !d->node_normals)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_payload: null outputs");
+    if (d->n_records && (!d->tris || !d->codes || !d->prims))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_payload: null records");
+    if (d->n_vertices && !d->vertices) return fail(ctx, MRH_ERR_INVALID, "mrh_svo_payload: null vertices");
+    if (d->n_records >= (1u << 30) || d->n_vertices >= (1u << 30))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_svo_payload: at most 2^30 - 1 records and vertices");
+    const uint32_t nodes = svo->nodes, D = svo->depth, leaf0 = svo->lvl_off[D], leaves = nodes - leaf0;
+    const uint32_t nr = d->n_records;
+    // workspace: keys, sorted keys, order (nr each), begin/end (leaves each),
+    // normals (4 per node), sums (2 x u64 per node), counts (2 per node)
+    const size_t w_keys = nr ? nr : 1, w_leaves = leaves ? leaves : 1;
+    const size_t words = 3 * w_keys + 2 * w_leaves + 4 * (size_t)nodes + 4 * (size_t)nodes + 2 * (size_t)nodes + 8;
+    void* mem = nullptr;
+    MRH_TRY(scratch(ctx, SLOT_VOXEL, words * 4, &mem));
+    unsigned long long* tot = (unsigned long long*)mem;                  // 8-byte aligned first
+    float4* nrm = (float4*)(tot + 2 * (size_t)nodes);
+    uint32_t* num = (uint32_t*)(nrm + nodes);
+    uint32_t* keys = num + 2 * (size_t)nodes;
+    uint32_t* skeys = keys + w_keys;
+    uint32_t* order = skeys + w_keys;
+    uint32_t* begin = order + w_keys;
+    uint32_t* end = begin + w_leaves;
+    MRH_TRY(fill_u32(ctx, begin, 0u, 2 * w_leaves));
+    if (nr) {
+        MRH_TRY(launch(ctx, "payload_keys", dim3(stride_grid(ctx, nr, 256)), dim3(256), 0, k_pl_keys, d->codes, nr,
+                       (const uint32_t*)svo->first_child, (const uint32_t*)svo->child_mask, D, leaf0, keys));
+        SortArgs sa;
+        sa.keys_in = keys;
+        sa.keys_out = skeys;
+        sa.vals_out = order;
+        sa.n_max = nr;
+        sa.key_bytes = 4;
+        sa.begin_bit = 0;
+        sa.end_bit = 32;
+        MRH_TRY(radix_sort(ctx, sa));
+        MRH_TRY(launch(ctx, "payload_runs", dim3(stride_grid(ctx, nr, 256)), dim3(256), 0, k_pl_runs,
+                       (const uint32_t*)skeys, nr, begin, end));
+    }
+    MRH_TRY(launch(ctx, "payload_leaf_normals", dim3(stride_grid(ctx, leaves, 256)), dim3(256), 0, k_pl_leaf_normals,
+                   leaves, leaf0, (const uint32_t*)begin, (const uint32_t*)end, (const uint32_t*)order, d->prims,
+                   d->tris, nrm));
+    for (int l = (int)D - 1; l >= 0; l--) {
+        const uint32_t off = svo->lvl_off[l], cnt = svo->lvl_off[l + 1] - off;
+        MRH_TRY(launch(ctx, "payload_node_normals", dim3(stride_grid(ctx, cnt, 256)), dim3(256), 0,
+                       k_pl_node_normals, off, cnt, (const uint32_t*)svo->first_child,
+                       (const uint32_t*)svo->child_mask, nrm));
+    }
+    MRH_TRY(fill_u32(ctx, (uint32_t*)tot, 0u, 4 * (size_t)nodes));
+    MRH_TRY(fill_u32(ctx, num, 0u, 2 * (size_t)nodes));
+    if (d->n_vertices)
+        MRH_TRY(launch(ctx, "payload_splat", dim3(stride_grid(ctx, d->n_vertices, 256)), dim3(256), 0, k_pl_splat,
+                       d->vertices, d->n_vertices, (const uint32_t*)svo->first_child, (const uint32_t*)svo->child_mask,
+                       D, make_float4(svo->cube[0], svo->cube[1], svo->cube[2], svo->cube[3]), (const float4*)nrm, tot,
+                       num));
+    float2* irr = (float2*)d->node_irradiance;
+    MRH_TRY(launch(ctx, "payload_leaf_avg", dim3(stride_grid(ctx, leaves, 256)), dim3(256), 0, k_pl_leaf_avg, leaf0,
+                   leaves, (const unsigned long long*)tot, (const uint32_t*)num, irr));
+    for (int l = (int)D - 1; l >= 0; l--) {
+        const uint32_t off = svo->lvl_off[l], cnt = svo->lvl_off[l + 1] - off;
+        MRH_TRY(launch(ctx, "payload_bottom_up", dim3(stride_grid(ctx, cnt, 256)), dim3(256), 0, k_pl_bottom_up, off,
+                       cnt, (const uint32_t*)svo->first_child, (const uint32_t*)svo->child_mask, (const float4*)nrm,
+                       irr));
+    }
+    for (uint32_t l = 1; l <= D; l++) {
+        const uint32_t off = svo->lvl_off[l], cnt = svo->lvl_off[l + 1] - off;
+        MRH_TRY(launch(ctx, "payload_top_down", dim3(stride_grid(ctx, cnt, 256)), dim3(256), 0, k_pl_top_down, off,
+                       cnt, (const uint32_t*)svo->parent, irr));
+    }
+    return launch(ctx, "payload_out", dim3(stride_grid(ctx, nodes, 256)), dim3(256), 0, k_pl_out, nodes,
+                  (const float4*)nrm, d->node_normals, d->node_normal_length);
 }
 
 int32_t mrh_wfpg_trace_fields(mrh_context* ctx, const mrh_svo* svo, const mrh_wfpg_trace_desc* d)

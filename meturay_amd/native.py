@@ -88,6 +88,11 @@ class WfpgFieldsDesc(C.Structure):
                 ("out_filtered", _P)]
 
 
+class SvoPayloadDesc(C.Structure):
+    _fields_ = [("tris", _P), ("codes", _P), ("prims", _P), ("n_records", _U32), ("vertices", _P),
+                ("n_vertices", _U32), ("node_irradiance", _P), ("node_normals", _P), ("node_normal_length", _P)]
+
+
 class WfpgTraceDesc(C.Structure):
     _fields_ = [("field_x", _U32), ("field_y", _U32), ("bin_count", _U32), ("max_query_level_offset", _U32),
                 ("boundary_radiance", _F), ("bin_origins", _P), ("bin_jitter", _P), ("node_irradiance", _P),
@@ -214,9 +219,10 @@ _SIGS = {
     "mrh_svo_view_get": (_I32, [_P, C.POINTER(SvoView)]),
     "mrh_svo_bin_rays": (_I32, [_P, _P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
     "mrh_wfpg_sample_fields": (_I32, [_P, C.POINTER(WfpgFieldsDesc)]),
-    "mrh_svo_voxelize": (_I32, [_P, _P, _U32, C.POINTER(_F * 4), _U32, _P, _U32, C.POINTER(_U32)]),
+    "mrh_svo_voxelize": (_I32, [_P, _P, _U32, C.POINTER(_F * 4), _U32, _P, _P, _U32, C.POINTER(_U32)]),
     "mrh_svo_cone_trace": (_I32, [_P, _P, _P, _U32, _F, _U32, _P, _P, _P]),
     "mrh_wfpg_trace_fields": (_I32, [_P, _P, C.POINTER(WfpgTraceDesc)]),
+    "mrh_svo_payload": (_I32, [_P, _P, C.POINTER(SvoPayloadDesc)]),
     "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                  C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
     "mrh_server_create": (_I32, [_P, C.POINTER(ServerParams), C.POINTER(_P)]),
@@ -338,13 +344,14 @@ class Context:
     def filter_resolve(self, d_weighted, d_weights, width: int, height: int, d_image) -> None:
         self.check(lib().mrh_filter_resolve(self.h, ptr(d_weighted), ptr(d_weights), width, height, ptr(d_image)))
 
-    def svo_voxelize(self, d_tris, n: int, cube, depth: int, d_codes=None, capacity: int = 0) -> int:
+    def svo_voxelize(self, d_tris, n: int, cube, depth: int, d_codes=None, capacity: int = 0, d_prims=None) -> int:
         """mrh_svo_voxelize: the number of voxel records of n triangles; the
-        records go to d_codes when it holds them all."""
+        records (and their triangles, with d_prims) go out when d_codes holds
+        them all."""
         c = (_F * 4)(*[float(x) for x in cube])
         cnt = _U32(0)
-        self.check(lib().mrh_svo_voxelize(self.h, ptr(d_tris), n, C.byref(c), depth, ptr(d_codes), capacity,
-                                          C.byref(cnt)))
+        self.check(lib().mrh_svo_voxelize(self.h, ptr(d_tris), n, C.byref(c), depth, ptr(d_codes), ptr(d_prims),
+                                          capacity, C.byref(cnt)))
         return int(cnt.value)
 
     def wfpg_sample_fields(self, d_fields, field_x: int, field_y: int, bin_count: int, d_bin_offsets, d_ray_ids,
@@ -736,6 +743,13 @@ class Svo:
         """mrh_svo_cone_trace over rays [n][7] (origin, direction, tmin)."""
         self.ctx.check(lib().mrh_svo_cone_trace(self.ctx.h, self.h, ptr(d_rays), n, aperture, max_query_level_offset,
                                                 ptr(d_nodes), ptr(d_t), ptr(d_is_leaf)))
+
+    def payload(self, d_tris, d_codes, d_prims, n_records: int, d_vertices, n_vertices: int, d_irradiance,
+                d_normals, d_lengths=None) -> None:
+        """mrh_svo_payload: normals and front/back radiance of every node."""
+        d = SvoPayloadDesc(ptr(d_tris), ptr(d_codes), ptr(d_prims), n_records, ptr(d_vertices), n_vertices,
+                           ptr(d_irradiance), ptr(d_normals), ptr(d_lengths))
+        self.ctx.check(lib().mrh_svo_payload(self.ctx.h, self.h, C.byref(d)))
 
     def trace_fields(self, field_x: int, field_y: int, bin_count: int, d_origins, d_jitter, d_irradiance, d_normals,
                      d_fields, boundary: float = 0.0, offset: int = 0, d_nodes=None, d_t=None) -> None:

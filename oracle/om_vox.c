@@ -158,10 +158,11 @@ static int vox_pixel(const om_vox_setup* s, const float cube[4], uint32_t R, int
 }
 
 /* Voxel records of n triangles (float[n][9]) in the cube (lo xyz, size) at
- * depth (R = 2^depth <= 1024): returns the record count; codes (capacity
- * cap, may be NULL) receive the first min(count, cap) records. */
+ * depth (R = 2^depth <= 1024): returns the record count; codes and prims
+ * (capacity cap, may be NULL) receive the first min(count, cap) records and
+ * their triangle indices. */
 uint64_t om_voxelize(const float* tris, uint32_t n, const float cube[4], uint32_t depth, uint32_t* codes,
-                     uint64_t cap)
+                     uint32_t* prims, uint64_t cap)
 {
     const uint32_t R = 1u << depth;
     uint64_t w = 0;
@@ -173,6 +174,7 @@ uint64_t om_voxelize(const float* tris, uint32_t n, const float cube[4], uint32_
                 uint32_t code;
                 if (vox_pixel(&s, cube, R, x, y, &code)) {
                     if (codes && w < cap) codes[w] = code;
+                    if (prims && w < cap) prims[w] = t;
                     w++;
                 }
             }
@@ -328,4 +330,231 @@ void om_wfpg_trace_fields(const om_svo* s, uint32_t X, uint32_t Y, uint32_t bins
                 if (nodes) nodes[i] = node;
                 if (t) t[i] = tt;
             }
+}
+
+/* ---- Node payload of the guiding octree (VoxelPayload, AnisoSVO.cuh:23-115):
+ * per node a normal and the average radiance leaving its front and back
+ * side, as the reference forms them:
+ * - leaf normals: a 2-means clustering (4 rounds, seeded with the first
+ *   record's normal and its negation; a normal joins the mean it is FARTHER
+ *   from, the reference's rule, which only swaps the two labels each round)
+ *   of the unit normals of the leaf's voxel records' triangles, in record
+ *   order; the normal is the average of the larger cluster (its length < 1
+ *   says how much the normals spread) (KCReduceVoxelPayload,
+ *   AnisoSVOKC.cuh:8-230);
+ * - internal normals: the same clustering over the children's average
+ *   normals (KCAverageNormals, AnisoSVO.cu:419-520);
+ * - radiance: each path vertex (position, wo toward the previous vertex,
+ *   luminance) goes to the nearest existing leaf (its own cell, else the
+ *   neighbours nearest first, NearestNodeIndex, AnisoSVO.cuh:1168-1240) and
+ *   adds to the side its wo faces (WriteLeafRadiance, :491-536); sums are
+ *   48.16 fixed point so that they do not depend on the order of the adds;
+ * - leaf averages, then node averages bottom-up over the children's valid
+ *   sides (a child whose normal opposes the node's swaps its sides), then
+ *   sides without samples take the parent's value top-down
+ *   (NormalizeAndFilterRadiance, AnisoSVO.cu:1415-1483, 530-656).  A side
+ *   with no sample anywhere above keeps the marker 65504 (the reference's
+ *   half maximum).
+ * vertices: [n][7] (position, wo, luminance).  Outputs [nodes][2] and
+ * [nodes][4] (unit normal xyz, length of the average normal). */
+#define OM_PAYLOAD_INVALID 65504.0f
+
+static void kmeans2(const float (*v)[3], uint32_t n, float out[4])
+{
+    /* the first non-zero vector seeds the means */
+    uint32_t first = n;
+    for (uint32_t i = 0; i < n; i++)
+        if (!(v[i][0] == 0.0f && v[i][1] == 0.0f && v[i][2] == 0.0f)) { first = i; break; }
+    if (first == n) {
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return;
+    }
+    float m[2][3] = {{v[first][0], v[first][1], v[first][2]}, {-v[first][0], -v[first][1], -v[first][2]}};
+    uint32_t cnt[2] = {0, 0};
+    for (int it = 0; it < 4; it++) {
+        for (int k = 0; k < 2; k++) {
+            const float inv = 1.0f / sqrtf(m[k][0] * m[k][0] + m[k][1] * m[k][1] + m[k][2] * m[k][2]);
+            m[k][0] *= inv;
+            m[k][1] *= inv;
+            m[k][2] *= inv;
+        }
+        float acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        cnt[0] = cnt[1] = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (v[i][0] == 0.0f && v[i][1] == 0.0f && v[i][2] == 0.0f) continue;
+            const float d0 = 1.0f - (m[0][0] * v[i][0] + m[0][1] * v[i][1] + m[0][2] * v[i][2]);
+            const float d1 = 1.0f - (m[1][0] * v[i][0] + m[1][1] * v[i][1] + m[1][2] * v[i][2]);
+            const int k = (d0 >= d1) ? 0 : 1;
+            acc[k][0] += v[i][0];
+            acc[k][1] += v[i][1];
+            acc[k][2] += v[i][2];
+            cnt[k]++;
+        }
+        for (int k = 0; k < 2; k++)
+            for (int a = 0; a < 3; a++) m[k][a] = acc[k][a];
+    }
+    const int k = (cnt[0] >= cnt[1]) ? 0 : 1;
+    const float r = 1.0f / (float)cnt[k];
+    const float mean[3] = {m[k][0] * r, m[k][1] * r, m[k][2] * r};
+    const float len = sqrtf(mean[0] * mean[0] + mean[1] * mean[1] + mean[2] * mean[2]);
+    const float il = 1.0f / len;
+    out[0] = mean[0] * il;
+    out[1] = mean[1] * il;
+    out[2] = mean[2] * il;
+    out[3] = len;
+}
+
+/* deepest node on the path of `code` that is a leaf, or OM_SVO_NO_NODE */
+static uint32_t leaf_of(const om_svo* s, uint32_t code)
+{
+    uint32_t node = 0;
+    for (uint32_t l = 0; l < s->depth; l++) {
+        const uint32_t oct = (code >> (3u * (s->depth - 1u - l))) & 7u;
+        const uint32_t m = s->child_mask[node];
+        if (!((m >> oct) & 1u)) return OM_SVO_NO_NODE;
+        node = s->first_child[node] + (uint32_t)__builtin_popcount(m & ((1u << oct) - 1u));
+    }
+    return node;
+}
+
+/* NearestNodeIndex (AnisoSVO.cuh:1168-1240): the point's own leaf cell, then
+ * its neighbours towards the nearest faces first; cells outside the grid are
+ * skipped */
+static uint32_t nearest_leaf(const om_svo* s, const float p[3])
+{
+    const uint32_t R = 1u << s->depth;
+    for (int a = 0; a < 3; a++)
+        if (p[a] < s->cube[a] || p[a] > s->cube[a] + s->cube[3]) return OM_SVO_NO_NODE;
+    const float leaf = s->cube[3] / (float)R;
+    float fi[3], fr[3], dist[3];
+    int inc[3];
+    for (int a = 0; a < 3; a++) {
+        const float l = (p[a] - s->cube[a]) / leaf;
+        fi[a] = floorf(l);
+        fr[a] = l - fi[a];
+        inc[a] = (fr[a] < 0.5f) ? -1 : 1;
+        dist[a] = 0.5f - fabsf(fr[a] - 0.5f);
+    }
+    int m0 = 0;
+    if (dist[1] < dist[m0]) m0 = 1;
+    if (dist[2] < dist[m0]) m0 = 2;
+    const int o1 = (m0 + 1) % 3, o2 = (m0 + 2) % 3;
+    const int m1 = (dist[o2] < dist[o1]) ? o2 : o1;
+    const int m2 = 3 - m0 - m1;
+    for (uint32_t i = 0; i < 8; i++) {
+        int sh[3];
+        sh[m0] = (int)(i & 1u);
+        sh[m1] = (int)((i >> 1) & 1u);
+        sh[m2] = (int)((i >> 2) & 1u);
+        int64_t q[3];
+        int ok = 1;
+        for (int a = 0; a < 3; a++) {
+            q[a] = (int64_t)fi[a] + sh[a] * inc[a];
+            if (q[a] < 0 || q[a] >= (int64_t)R) ok = 0;
+        }
+        if (!ok) continue;
+        const uint32_t code = vox_expand10((uint32_t)q[0]) | (vox_expand10((uint32_t)q[1]) << 1) |
+                              (vox_expand10((uint32_t)q[2]) << 2);
+        const uint32_t leafn = leaf_of(s, code);
+        if (leafn != OM_SVO_NO_NODE) return leafn;
+    }
+    return OM_SVO_NO_NODE;
+}
+
+void om_svo_payload(const om_svo* s, const float* tris, const uint32_t* codes, const uint32_t* prims, uint32_t n_rec,
+                    const float* vertices, uint32_t n_vert, float* irr, float* normals)
+{
+    const uint32_t nodes = s->nodes, D = s->depth;
+    const uint32_t leaf0 = s->level_offset[D];
+    float (*nv)[4] = (float (*)[4])normals;
+    /* leaf normals: records grouped by leaf in record order (counting sort) */
+    uint32_t* cnt = (uint32_t*)calloc(nodes + 1, sizeof(uint32_t));
+    uint32_t* leafof = (uint32_t*)malloc((size_t)(n_rec ? n_rec : 1) * sizeof(uint32_t));
+    for (uint32_t r = 0; r < n_rec; r++) {
+        leafof[r] = leaf_of(s, codes[r]);
+        if (leafof[r] != OM_SVO_NO_NODE) cnt[leafof[r] + 1]++;
+    }
+    for (uint32_t i = 0; i < nodes; i++) cnt[i + 1] += cnt[i];
+    uint32_t* pos = (uint32_t*)malloc((size_t)(nodes + 1) * sizeof(uint32_t));
+    for (uint32_t i = 0; i <= nodes; i++) pos[i] = cnt[i];
+    uint32_t* order = (uint32_t*)malloc((size_t)(n_rec ? n_rec : 1) * sizeof(uint32_t));
+    for (uint32_t r = 0; r < n_rec; r++)
+        if (leafof[r] != OM_SVO_NO_NODE) order[pos[leafof[r]]++] = r;
+    float (*buf)[3] = (float (*)[3])malloc((size_t)(n_rec ? n_rec : 8) * 3 * sizeof(float));
+    for (uint32_t lf = leaf0; lf < nodes; lf++) {
+        const uint32_t b = cnt[lf], e = cnt[lf + 1];
+        for (uint32_t k = b; k < e; k++) {
+            const float* t = tris + 9 * (size_t)prims[order[k]];
+            const float ea[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
+            const float eb[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
+            float nn[3] = {ea[1] * eb[2] - ea[2] * eb[1], ea[2] * eb[0] - ea[0] * eb[2], ea[0] * eb[1] - ea[1] * eb[0]};
+            const float inv = 1.0f / sqrtf(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+            buf[k - b][0] = nn[0] * inv;
+            buf[k - b][1] = nn[1] * inv;
+            buf[k - b][2] = nn[2] * inv;
+        }
+        kmeans2((const float (*)[3])buf, e - b, nv[lf]);
+    }
+    /* internal normals bottom-up over the children's average normals */
+    for (int l = (int)D - 1; l >= 0; l--)
+        for (uint32_t nd = s->level_offset[l]; nd < s->level_offset[l + 1]; nd++) {
+            const uint32_t fc = s->first_child[nd], k = (uint32_t)__builtin_popcount(s->child_mask[nd]);
+            float ch[8][3];
+            for (uint32_t i = 0; i < k; i++)
+                for (int a = 0; a < 3; a++) ch[i][a] = nv[fc + i][a] * nv[fc + i][3];
+            kmeans2((const float (*)[3])ch, k, nv[nd]);
+        }
+    /* splat in 48.16 fixed point */
+    uint64_t* tot = (uint64_t*)calloc((size_t)nodes * 2, sizeof(uint64_t));
+    uint32_t* num = (uint32_t*)calloc((size_t)nodes * 2, sizeof(uint32_t));
+    for (uint32_t v = 0; v < n_vert; v++) {
+        const float* x = vertices + 7 * (size_t)v;
+        const uint32_t lf = nearest_leaf(s, x);
+        if (lf == OM_SVO_NO_NODE) continue;
+        const float nd = nv[lf][0] * x[3] + nv[lf][1] * x[4] + nv[lf][2] * x[5];
+        const int side = (nd >= 0.0f) ? 0 : 1;
+        const float lum = x[6];
+        const uint64_t f = (lum > 0.0f) ? (uint64_t)(fminf(lum, 2147483647.0f) * 65536.0f) : 0ull;
+        tot[2 * (size_t)lf + side] += f;
+        num[2 * (size_t)lf + side]++;
+    }
+    for (uint32_t lf = leaf0; lf < nodes; lf++)
+        for (int j = 0; j < 2; j++) {
+            const uint32_t c = num[2 * (size_t)lf + j];
+            irr[2 * (size_t)lf + j] = c ? ((float)tot[2 * (size_t)lf + j] * (1.0f / 65536.0f)) / (float)c
+                                        : OM_PAYLOAD_INVALID;
+        }
+    /* bottom-up over the valid sides, swapping a child whose normal opposes */
+    for (int l = (int)D - 1; l >= 0; l--)
+        for (uint32_t nd = s->level_offset[l]; nd < s->level_offset[l + 1]; nd++) {
+            const uint32_t fc = s->first_child[nd], k = (uint32_t)__builtin_popcount(s->child_mask[nd]);
+            float sum[2] = {0.0f, 0.0f};
+            uint32_t c[2] = {0, 0};
+            for (uint32_t i = 0; i < k; i++) {
+                const float* cn = nv[fc + i];
+                const int swap = (cn[0] * nv[nd][0] + cn[1] * nv[nd][1] + cn[2] * nv[nd][2]) < 0.0f;
+                for (int j = 0; j < 2; j++) {
+                    const float val = irr[2 * (size_t)(fc + i) + (swap ? 1 - j : j)];
+                    if (val != OM_PAYLOAD_INVALID) {
+                        sum[j] += val;
+                        c[j]++;
+                    }
+                }
+            }
+            for (int j = 0; j < 2; j++) irr[2 * (size_t)nd + j] = c[j] ? sum[j] * (1.0f / (float)c[j]) : OM_PAYLOAD_INVALID;
+        }
+    /* top-down: sides without samples take the parent's */
+    for (uint32_t l = 1; l <= D; l++)
+        for (uint32_t nd = s->level_offset[l]; nd < s->level_offset[l + 1]; nd++) {
+            const uint32_t p = s->parent[nd];
+            for (int j = 0; j < 2; j++)
+                if (irr[2 * (size_t)nd + j] == OM_PAYLOAD_INVALID) irr[2 * (size_t)nd + j] = irr[2 * (size_t)p + j];
+        }
+    free(cnt);
+    free(leafof);
+    free(pos);
+    free(order);
+    free(buf);
+    free(tot);
+    free(num);
 }

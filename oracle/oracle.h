@@ -364,7 +364,7 @@ int   om_wfpg_sample_fields(const om_wfpg_desc* d);
  * 384-620): records of every conservatively covered pixel, triangle by
  * triangle, row by row; codes are cell codes at `depth` of the cube. */
 uint64_t om_voxelize(const float* tris, uint32_t n, const float cube[4], uint32_t depth, uint32_t* codes,
-                     uint64_t cap);
+                     uint32_t* prims, uint64_t cap);
 /* Cone trace through an om_svo_build octree (AnisoSVO.cuh:767-1032):
  * returns t; *node = node stopped at (0xFFFFFFFF: left the cube).
  * rays for the batch: n x 7 floats (origin, direction, tmin). */
@@ -372,6 +372,12 @@ float    om_svo_cone_trace(const om_svo* s, const float o[3], const float dir[3]
                            uint32_t max_query_level_offset, uint32_t* node, int* is_leaf);
 void     om_svo_cone_trace_batch(const om_svo* s, const float* rays, uint32_t n, float aperture, uint32_t offset,
                                  uint32_t* nodes, float* t, uint32_t* leaf);
+/* node payload of the guiding octree (AnisoSVO.cu:225-263, 419-656,
+ * 1415-1483; AnisoSVOKC.cuh:8-230): vertices [n][7] (position, wo,
+ * luminance); irr [nodes][2] (front, back); normals [nodes][4] (unit xyz,
+ * length of the average normal) */
+void     om_svo_payload(const om_svo* s, const float* tris, const uint32_t* codes, const uint32_t* prims,
+                        uint32_t n_rec, const float* vertices, uint32_t n_vert, float* irr, float* normals);
 /* cone-traced radiance fields of bins (WFPGTracerKC.cuh:1252-1318) */
 void     om_wfpg_trace_fields(const om_svo* s, uint32_t X, uint32_t Y, uint32_t bins, const float* origins,
                               const float* jitter, const float* irr, const float* normals, float boundary,

@@ -147,9 +147,10 @@ def lib():
         L.om_wfpg_sample_fields.argtypes = [C.POINTER(OmWfpgDesc)]
         L.om_voxelize.restype = C.c_uint64
         L.om_svo_cone_trace_batch.argtypes = [C.POINTER(OmSvo), _P, _U32, _F, _U32, _P, _P, _P]
+        L.om_svo_payload.argtypes = [C.POINTER(OmSvo), _P, _P, _P, _U32, _P, _U32, _P, _P]
         L.om_wfpg_trace_fields.argtypes = [C.POINTER(OmSvo), _U32, _U32, _U32, _P, _P, _P, _P, _F, _U32, _P, _P,
                                            _P]
-        L.om_voxelize.argtypes = [_P, _U32, _P, _U32, _P, C.c_uint64]
+        L.om_voxelize.argtypes = [_P, _U32, _P, _U32, _P, _P, C.c_uint64]
         L.om_trace_closest.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F,
                                        _P, _P, _P, _P]
         L.om_trace_any.argtypes = [_P, _U32, _P, _P, _P, _P, _U32, _P, _U32, _U32, _F, _P]
@@ -765,6 +766,18 @@ class Svo:
                                       _ptr(nodes), _ptr(t), _ptr(leaf))
         return nodes, t, leaf
 
+    def payload(self, tris, codes, prims, vertices):
+        """om_svo_payload: (irr [nodes][2], normals [nodes][4])."""
+        tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        codes = np.ascontiguousarray(codes, dtype=np.uint32)
+        prims = np.ascontiguousarray(prims, dtype=np.uint32)
+        vertices = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 7)
+        irr = np.zeros((self.nodes, 2), np.float32)
+        nrm = np.zeros((self.nodes, 4), np.float32)
+        lib().om_svo_payload(C.byref(self._s), _ptr(tris), _ptr(codes), _ptr(prims), len(codes), _ptr(vertices),
+                             len(vertices), _ptr(irr), _ptr(nrm))
+        return irr, nrm
+
     def trace_fields(self, X: int, Y: int, origins, jitter, irr, normals, boundary: float = 0.0, offset: int = 0):
         """om_wfpg_trace_fields: (fields [bins][Y][X], nodes, t)."""
         origins = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 4)
@@ -966,14 +979,16 @@ def wfpg_sample_fields(fields: np.ndarray, bin_offsets: np.ndarray, ray_ids: np.
     return out
 
 
-def voxelize(tris: np.ndarray, cube, depth: int) -> np.ndarray:
-    """om_voxelize: voxel cell codes of every covered pixel (duplicates kept)."""
+def voxelize(tris: np.ndarray, cube, depth: int, prims: bool = False):
+    """om_voxelize: voxel cell codes of every covered pixel (duplicates kept);
+    with prims=True also each record's triangle index."""
     tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
     cube = np.ascontiguousarray(cube, dtype=np.float32)
-    n = int(lib().om_voxelize(_ptr(tris), len(tris), _ptr(cube), depth, None, 0))
+    n = int(lib().om_voxelize(_ptr(tris), len(tris), _ptr(cube), depth, None, None, 0))
     out = np.zeros(max(n, 1), np.uint32)
-    lib().om_voxelize(_ptr(tris), len(tris), _ptr(cube), depth, _ptr(out), n)
-    return out[:n]
+    pr = np.zeros(max(n, 1), np.uint32)
+    lib().om_voxelize(_ptr(tris), len(tris), _ptr(cube), depth, _ptr(out), _ptr(pr), n)
+    return (out[:n], pr[:n]) if prims else out[:n]
 
 
 def svo_cube(lo, hi, depth: int) -> np.ndarray:

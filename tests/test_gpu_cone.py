@@ -168,3 +168,55 @@ def test_trace_fields_match_oracle_and_feed_sampling(ctx, oracle, side):
     ctx.sync()
     want = oracle.wfpg_sample_fields(of, offsets, ids, xi, alpha=1.0, ray_count=n_rays)
     assert np.array_equal(d_uv.cpu().numpy(), want["uv"]) and np.array_equal(d_pdf.cpu().numpy(), want["pdf"])
+
+
+def surface_vertices(rng, tris, n):
+    """points on random triangles, random unit wo, luminance; plus a few in
+    empty space and outside the cube"""
+    t = tris[rng.integers(0, len(tris), n)].reshape(n, 3, 3).astype(np.float64)
+    w = rng.dirichlet([1, 1, 1], n)
+    p = (w[:, :, None] * t).sum(1)
+    p[: n // 20] += rng.normal(scale=0.05, size=(n // 20, 3))
+    p[n // 20: n // 10] = rng.uniform(-10, 10, (n // 10 - n // 20, 3))
+    wo = rng.normal(size=(n, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    lum = rng.gamma(0.8, 1.0, n)
+    return np.hstack([p, wo, lum[:, None]]).astype(np.float32)
+
+
+def test_payload_matches_oracle_and_drives_the_fields(ctx, oracle):
+    """Payload of the voxelised Cornell box (depth 10) from 200k path
+    vertices, then the fields it gives and the samples from them: every stage
+    against the oracle, bit for bit."""
+    depth = 10
+    tris = np.ascontiguousarray(N.gen_cornell()["tris"], np.float32).reshape(-1, 9)
+    p = tris.reshape(-1, 3)
+    cube = oracle.svo_cube(p.min(0), p.max(0), depth)
+    codes, prims = oracle.voxelize(tris, cube, depth, prims=True)
+    svo, o = build_both(ctx, oracle, codes, cube, depth)
+    rng = np.random.default_rng(77)
+    verts = surface_vertices(rng, tris, 200000)
+    n = o.nodes
+    d_t, d_c, d_p, d_v = dev(tris.reshape(-1)), dev(codes), dev(prims), dev(verts.reshape(-1))
+    d_irr = torch.empty(2 * n, device="cuda")
+    d_nrm = torch.empty(3 * n, device="cuda")
+    d_len = torch.empty(n, device="cuda")
+    torch.cuda.synchronize()
+    svo.payload(d_t, d_c, d_p, len(codes), d_v, len(verts), d_irr, d_nrm, d_len)
+    ctx.sync()
+    oi, on = o.payload(tris, codes, prims, verts)
+    assert np.array_equal(d_irr.cpu().numpy().view(np.uint32), oi.ravel().view(np.uint32))
+    assert np.array_equal(d_nrm.cpu().numpy().view(np.uint32), on[:, :3].ravel().view(np.uint32))
+    assert np.array_equal(d_len.cpu().numpy().view(np.uint32), on[:, 3].view(np.uint32))
+    assert (oi[o.level_offset[depth]:] != 65504.0).mean() > 0.5
+    # fields from that payload at bins on the surfaces, then samples
+    bins, side = 16, 32
+    org = verts[:bins, :3] + 0.01 * verts[:bins, 3:6]
+    origins = np.hstack([org, np.full((bins, 1), float(cube[3]) / 1024 * 0.15 + 1e-5)]).astype(np.float32)
+    jit = rng.random((bins, 2)).astype(np.float32)
+    d_f = torch.empty(bins * side * side, device="cuda")
+    torch.cuda.synchronize()
+    svo.trace_fields(side, side, bins, dev(origins.reshape(-1)), dev(jit.reshape(-1)), d_irr, d_nrm, d_f)
+    ctx.sync()
+    of, _, _ = o.trace_fields(side, side, origins, jit, oi, on[:, :3])
+    assert np.array_equal(d_f.cpu().numpy(), of.ravel())

@@ -20,16 +20,19 @@ def cube_of(oracle, tris, depth):
     return oracle.svo_cube(p.min(0), p.max(0), depth)
 
 
-def voxelize_gpu(ctx, tris, cube, depth):
+def voxelize_gpu(ctx, tris, cube, depth, prims=False):
     d_t = dev(np.ascontiguousarray(tris, np.float32).reshape(-1))
     torch.cuda.synchronize()
     n = len(tris)
     count = ctx.svo_voxelize(d_t, n, cube, depth)
     d_c = torch.full((max(count, 1),), -1, dtype=torch.int32, device="cuda")
+    d_p = torch.full((max(count, 1),), -1, dtype=torch.int32, device="cuda") if prims else None
     torch.cuda.synchronize()
-    again = ctx.svo_voxelize(d_t, n, cube, depth, d_c, count)
+    again = ctx.svo_voxelize(d_t, n, cube, depth, d_c, count, d_prims=d_p)
     ctx.sync()
     assert again == count
+    if prims:
+        return host(d_c, np.uint32)[:count], host(d_p, np.uint32)[:count]
     return host(d_c, np.uint32)[:count]
 
 
@@ -48,9 +51,10 @@ def test_matches_oracle_on_triangle_soup(ctx, oracle, depth):
     tris = soup(rng, 3000)
     cube = cube_of(oracle, tris, depth)
     print(f"[vox] soup depth {depth}", flush=True)
-    got = voxelize_gpu(ctx, tris, cube, depth)
-    want = oracle.voxelize(tris, cube, depth)
+    got, gprim = voxelize_gpu(ctx, tris, cube, depth, prims=True)
+    want, wprim = oracle.voxelize(tris, cube, depth, prims=True)
     assert len(got) == len(want) and np.array_equal(got, want)
+    assert np.array_equal(gprim, wprim)
     assert len(want) > 3000
 
 

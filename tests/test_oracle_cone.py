@@ -230,3 +230,111 @@ def test_closed_room_walls_fill_a_sixth_of_the_field_each(oracle):
     assert (nodes != 0xFFFFFFFF).all()
     frac = np.array([(fields == k).mean() for k in range(1, 7)])
     assert np.allclose(frac, 1 / 6, atol=0.02), frac
+
+
+def _wall_tris(z=0.5, flip=False):
+    a, b = 0.1, 0.9
+    t = np.array([[a, a, z, b, a, z, b, b, z], [a, a, z, b, b, z, a, b, z]], np.float32)
+    if flip:
+        t = t.reshape(2, 3, 3)[:, ::-1].reshape(2, 9).copy()
+    return t
+
+
+def _cell_code(ix, iy, iz):
+    def e(v):
+        v &= 0x3FF
+        v = (v | (v << 16)) & 0x030000FF
+        v = (v | (v << 8)) & 0x0300F00F
+        v = (v | (v << 4)) & 0x030C30C3
+        v = (v | (v << 2)) & 0x09249249
+        return v
+    return e(ix) | (e(iy) << 1) | (e(iz) << 2)
+
+
+def test_payload_flat_wall_sides_and_averages(oracle):
+    """Wall z = 0.5 facing +z: every node's normal is +z (length 1); a vertex
+    whose wo faces +z lands on the front side; leaf sides average their
+    vertices, parents average their leaves, unsampled sides take the
+    parent's, and a side unsampled everywhere keeps 65504."""
+    depth = 5
+    tris = _wall_tris()
+    cube = oracle.svo_cube([0, 0, 0], [1, 1, 1], depth)
+    codes, prims = oracle.voxelize(tris, cube, depth, prims=True)
+    svo = oracle.Svo.build(codes, cube.tolist(), depth)
+    L = svo.level_offset[depth]
+    leaf = float(cube[3]) / (1 << depth)
+    lv, blo, bsz = node_boxes(svo, cube, depth)
+    rng = np.random.default_rng(2)
+    leaves = np.arange(L, svo.nodes)
+    pick = rng.choice(leaves, len(leaves) // 2, replace=False)        # half the leaves get samples
+    verts, want = [], {}
+    for lf in pick:
+        ctr = blo[lf] + bsz[lf] / 2
+        lum = rng.uniform(0.5, 4.0, 3).astype(np.float32)
+        for x in lum:
+            verts.append([*ctr, 0, 0, 1, x])
+        want[int(lf)] = lum.astype(np.float64).mean()
+    irr, nrm = svo.payload(tris, codes, prims, np.array(verts, np.float32))
+    assert np.allclose(nrm[:, :3], [0, 0, 1]) and np.allclose(nrm[:, 3], 1.0)
+    assert (irr[:, 1] == 65504.0).all()                              # nothing behind the wall
+    for lf in leaves:
+        if int(lf) in want:
+            # 48.16 fixed point: each sample truncated to 2^-16
+            assert abs(irr[lf, 0] - want[int(lf)]) < 2.0 ** -16 + 1e-6 * want[int(lf)]
+    # parents of leaves: the mean of their sampled children; unsampled leaves
+    # take their parent's value
+    for nd in range(svo.level_offset[depth - 1], L):
+        fc, k = svo.first_child[nd], bin(int(svo.child_mask[nd])).count("1")
+        kids = [want[c] for c in range(fc, fc + k) if c in want]
+        if kids:
+            assert abs(irr[nd, 0] - np.mean(kids)) < 2.0 ** -16 + 1e-6 * np.mean(kids)
+            for c in range(fc, fc + k):
+                if c not in want:
+                    assert irr[c, 0] == irr[nd, 0]
+
+
+def test_payload_back_side_and_opposed_children(oracle):
+    """Vertices whose wo points away from the normal go to the back side; a
+    wall facing -z swaps its sides when averaged into a parent whose
+    normal is +z."""
+    depth = 4
+    up = _wall_tris(0.25)
+    down = _wall_tris(0.75, flip=True)                   # faces -z
+    tris = np.concatenate([up, up, up, down])             # +z is the majority overall
+    cube = oracle.svo_cube([0, 0, 0], [1, 1, 1], depth)
+    codes, prims = oracle.voxelize(tris, cube, depth, prims=True)
+    svo = oracle.Svo.build(codes, cube.tolist(), depth)
+    lv, blo, bsz = node_boxes(svo, cube, depth)
+    L = svo.level_offset[depth]
+    verts = []
+    for lf in range(L, svo.nodes):
+        ctr = blo[lf] + bsz[lf] / 2
+        verts.append([*ctr, 0, 0, 1, 2.0])                  # wo +z
+    irr, nrm = svo.payload(tris, codes, prims, np.array(verts, np.float32))
+    low = [lf for lf in range(L, svo.nodes) if blo[lf][2] < 0.5]
+    high = [lf for lf in range(L, svo.nodes) if blo[lf][2] > 0.5]
+    assert all(np.allclose(nrm[lf, :3], [0, 0, 1]) for lf in low)
+    assert all(np.allclose(nrm[lf, :3], [0, 0, -1]) for lf in high)
+    assert all(irr[lf, 0] == 2.0 for lf in low)            # wo . n >= 0: front
+    assert all(irr[lf, 1] == 2.0 for lf in high)           # wo . n < 0: back
+    # the root holds both walls; its normal follows the majority (+z) and the
+    # -z wall's back side arrives on the root's front
+    assert np.allclose(nrm[0, :3], [0, 0, 1]) and irr[0, 0] == 2.0
+
+
+def test_payload_vertex_in_empty_cell_goes_to_nearest_leaf(oracle):
+    depth = 4
+    tris = _wall_tris(0.5)
+    cube = oracle.svo_cube([0, 0, 0], [1, 1, 1], depth)
+    codes, prims = oracle.voxelize(tris, cube, depth, prims=True)
+    svo = oracle.Svo.build(codes, cube.tolist(), depth)
+    leaf = float(cube[3]) / 16
+    lv, blo, bsz = node_boxes(svo, cube, depth)
+    lf = svo.level_offset[depth] + 5
+    ctr = blo[lf] + bsz[lf] / 2
+    p = ctr + np.array([0, 0, 0.8 * leaf])                  # in the empty cell above, near its floor
+    irr, _ = svo.payload(tris, codes, prims, np.array([[*p, 0, 0, 1, 3.0]], np.float32))
+    assert irr[lf, 0] == 3.0
+    far = np.array([[0.5, 0.5, 0.95, 0, 0, 1, 7.0]], np.float32)   # no leaf among its neighbours
+    irr2, _ = svo.payload(tris, codes, prims, far)
+    assert (irr2 != 7.0).all()

@@ -269,7 +269,19 @@ def wfpg_leg(N, ctx, torch, cpu_bins: int):
                 "ms_per_launch": round(per, 4), "achieved_GBs": round(nbytes / (per / 1e3) / 1e9, 1),
                 "frac_of_hbm_peak": round(nbytes / (per / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
             log(f"wfpg fields 64x64, rays in bin order: {per * 1e3:.1f} us per launch")
-            del seq
+            # the 8x8 product mode: per ray a BxDF cell field (64 floats)
+            bx = torch.rand((n, 64), device="cuda")
+            torch.cuda.synchronize()
+            ctx.profile_reset()
+            for _ in range(3):
+                ctx.wfpg_sample_fields(fields, side, side, bins, offsets, seq, n, None, uv, pdf, d_bxdf=bx, seed=7)
+            ctx.sync()
+            ms, cnt = ctx.profile_get("wfpg_fields_product")
+            per = ms / max(cnt, 1)
+            out["field_64_product_rays_in_bin_order"] = {"ms_per_launch": round(per, 4),
+                                                         "Mrays_per_s": round(n / (per / 1e3) / 1e6, 1)}
+            log(f"wfpg fields 64x64, product mode, rays in bin order: {per * 1e3:.1f} us per launch")
+            del seq, bx
     ctx.profile(False)
     if cpu_bins:
         # the oracle on the same kind of input: a few 64x64 bins, their rays

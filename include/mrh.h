@@ -643,6 +643,20 @@ typedef struct {
     float*          out_pdf;           /* [ray_count] */
     float*          out_eval_pdf;      /* [ray_count], given iff eval_uv is */
     float*          out_filtered;      /* optional [bin_count][field_y][field_x] */
+    /* Product mode (ProductSampler, ProductSampler.cuh:16-725, pure guiding as
+     * KCGenAndSampleDistributionProduct uses it, WFPGTracerKC.cuh:1562-1766)
+     * when bxdf_field is given: per ray the luminance of its BxDF towards the
+     * 8x8 cell directions (cell (i, j) = row i, column j at index 8 i + j; the
+     * reference evaluates them at the cell centres' octahedral directions,
+     * y up, tilted by 4 pi / 64 radians about cross(direction, normal),
+     * WFPGTracerKC.cuh:1602-1615).  The filtered field is reduced to 8x8
+     * cells (average, maximum); each ray draws a cell from max(1e-3, bxdf *
+     * cell average) and a texel inside it by rejection (24 rounds of 64
+     * tries, accepted with probability texel / cell maximum); xi is not read;
+     * every draw is a hash of (seed, ray id, draw index).  out_pdf and
+     * out_eval_pdf are the product pdfs (solid angle). */
+    const float*    bxdf_field;        /* optional [ray_count][64] */
+    uint32_t        seed;
 } mrh_wfpg_fields_desc;
 int32_t mrh_wfpg_sample_fields(mrh_context* ctx, const mrh_wfpg_fields_desc* desc);
 

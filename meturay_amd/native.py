@@ -85,7 +85,7 @@ class WfpgFieldsDesc(C.Structure):
     _fields_ = [("field_x", _U32), ("field_y", _U32), ("bin_count", _U32), ("ray_count", _U32),
                 ("filter_alpha", _F), ("fields", _P), ("bin_offsets", _P), ("ray_ids", _P), ("xi", _P),
                 ("eval_uv", _P), ("out_uv", _P), ("out_dir", _P), ("out_pdf", _P), ("out_eval_pdf", _P),
-                ("out_filtered", _P)]
+                ("out_filtered", _P), ("bxdf_field", _P), ("seed", _U32)]
 
 
 class SvoPayloadDesc(C.Structure):
@@ -356,12 +356,13 @@ class Context:
 
     def wfpg_sample_fields(self, d_fields, field_x: int, field_y: int, bin_count: int, d_bin_offsets, d_ray_ids,
                            ray_count: int, d_xi, d_out_uv, d_out_pdf, alpha: float = 1.0, d_eval_uv=None,
-                           d_out_eval_pdf=None, d_out_dir=None, d_out_filtered=None) -> None:
+                           d_out_eval_pdf=None, d_out_dir=None, d_out_filtered=None, d_bxdf=None,
+                           seed: int = 0) -> None:
         """mrh_wfpg_sample_fields: filter each bin's radiance field, build its
         distribution and draw one guided (u,v) per ray of the bin."""
         d = WfpgFieldsDesc(field_x, field_y, bin_count, ray_count, alpha, ptr(d_fields), ptr(d_bin_offsets),
                            ptr(d_ray_ids), ptr(d_xi), ptr(d_eval_uv), ptr(d_out_uv), ptr(d_out_dir), ptr(d_out_pdf),
-                           ptr(d_out_eval_pdf), ptr(d_out_filtered))
+                           ptr(d_out_eval_pdf), ptr(d_out_filtered), ptr(d_bxdf), seed)
         self.check(lib().mrh_wfpg_sample_fields(self.h, C.byref(d)))
 
 

@@ -185,6 +185,145 @@ float om_wfpg_pwc_pdf(const float* pdfx, const float* pdfy, uint32_t X, uint32_t
     return p * (0.25f * 0.318309886183790671538f);
 }
 
+/* ---- 8x8 product sampling (ProductSampler, TracerCUDA/ProductSampler.cuh:
+ * 16-725, as KCGenAndSampleDistributionProduct uses it in pure guiding,
+ * WFPGTracerKC.cuh:1562-1766).  The filtered field is reduced to 8x8 cells
+ * (average and maximum over each cell's texels, row by row); per ray the
+ * product max(1e-3, bxdf[cell] * average[cell]) gets a piecewise-constant
+ * distribution over the cells (rows: prefix left to right; the marginal over
+ * the row totals), a cell is drawn (row: first with xi0 <= cdf, column: first
+ * with xi1 <= cdf), and a texel inside it by rejection: up to 24 rounds of 64
+ * tries (the wave's lanes), each a uniform point of the cell accepted with
+ * probability texel / cell maximum, the lowest accepting try winning; with
+ * none, try 0 of the last round is kept as a uniform inner sample.  pdf =
+ * pdfY * pdfX * (texel / cell average, 1 for a uniform inner sample) / 4 pi.
+ * The reference's rounds are 32-lane warps and its uniforms come from a
+ * per-thread generator; here a round has 64 tries and every draw is a hash
+ * of (seed, ray, draw index). */
+static uint64_t wf_mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+float om_wfpg_uniform(uint32_t seed, uint32_t ray, uint32_t k)
+{
+    const uint64_t h = wf_mix64(wf_mix64(((uint64_t)seed << 32) | ray) + k);
+    return (float)(uint32_t)(h >> 40) * (1.0f / 16777216.0f);
+}
+
+#define OM_PS 8
+
+static void om_small_field(const float* f, uint32_t X, uint32_t Y, float* avg, float* mx)
+{
+    const uint32_t DX = X / OM_PS, DY = Y / OM_PS;
+    for (uint32_t c = 0; c < OM_PS * OM_PS; c++) {
+        const uint32_t cx = c % OM_PS, cy = c / OM_PS;
+        float s = 0.0f, m = 0.0f;
+        for (uint32_t y = 0; y < DY; y++)
+            for (uint32_t x = 0; x < DX; x++) {
+                const float v = f[(size_t)(cy * DY + y) * X + cx * DX + x];
+                s += v;
+                m = fmaxf(m, v);
+            }
+        avg[c] = s / (float)(DX * DY);
+        mx[c] = m;
+    }
+}
+
+/* the product distribution of one ray: pdfx/cdfx [64], pdfy/cdfy [8] */
+static void om_product_pwc(const float* avg, const float* bxdf, float* pdfx, float* cdfx, float* pdfy, float* cdfy)
+{
+    float prod[OM_PS * OM_PS], rows[OM_PS];
+    for (int c = 0; c < OM_PS * OM_PS; c++) prod[c] = fmaxf(1.0e-3f, bxdf[c] * avg[c]);
+    for (int r = 0; r < OM_PS; r++) {
+        float acc = 0.0f;
+        for (int x = 0; x < OM_PS; x++) {
+            acc += prod[r * OM_PS + x];
+            cdfx[r * OM_PS + x] = acc;
+        }
+        const float tot = acc, rec = 1.0f / tot;
+        rows[r] = tot;
+        for (int x = 0; x < OM_PS; x++) {
+            cdfx[r * OM_PS + x] *= rec;
+            pdfx[r * OM_PS + x] = prod[r * OM_PS + x] * (float)OM_PS * rec;
+        }
+    }
+    float acc = 0.0f;
+    for (int r = 0; r < OM_PS; r++) {
+        acc += rows[r];
+        cdfy[r] = acc;
+    }
+    const float rec = 1.0f / acc;
+    for (int r = 0; r < OM_PS; r++) {
+        cdfy[r] *= rec;
+        pdfy[r] = rows[r] * ((float)OM_PS * rec);
+    }
+}
+
+static int om_first_at_or_above(const float* c, float xi)
+{
+    int k = 0;
+    for (int i = 0; i < OM_PS; i++) k += (xi > c[i]) ? 1 : 0;
+    return k > OM_PS - 1 ? OM_PS - 1 : k;
+}
+
+static float om_product_pdf(const float* f, uint32_t X, uint32_t Y, const float* avg, const float* pdfx,
+                            const float* pdfy, float u, float v)
+{
+    const uint32_t DX = X / OM_PS, DY = Y / OM_PS;
+    const float ox = u * (float)OM_PS, oy = v * (float)OM_PS;
+    const float fx = floorf(ox), fy = floorf(oy);
+    int cx = (int)fx, cy = (int)fy;
+    int ix = (int)floorf((ox - fx) * (float)DX), iy = (int)floorf((oy - fy) * (float)DY);
+    cx = cx < 0 ? 0 : (cx > OM_PS - 1 ? OM_PS - 1 : cx);
+    cy = cy < 0 ? 0 : (cy > OM_PS - 1 ? OM_PS - 1 : cy);
+    ix = ix < 0 ? 0 : (ix > (int)DX - 1 ? (int)DX - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > (int)DY - 1 ? (int)DY - 1 : iy);
+    const float val = f[(size_t)(cy * (int)DY + iy) * X + cx * (int)DX + ix];
+    const float inner = val / avg[cy * OM_PS + cx];
+    return (pdfx[cy * OM_PS + cx] * pdfy[cy]) * inner * (0.25f * 0.318309886183790671538f);
+}
+
+static void om_product_sample(const float* f, uint32_t X, uint32_t Y, const float* avg, const float* mx,
+                              const float* pdfx, const float* cdfx, const float* pdfy, const float* cdfy,
+                              uint32_t seed, uint32_t ray, float uv[2], float* pdf)
+{
+    const uint32_t DX = X / OM_PS, DY = Y / OM_PS;
+    const float xi0 = om_wfpg_uniform(seed, ray, 0), xi1 = om_wfpg_uniform(seed, ray, 1);
+    const int row = om_first_at_or_above(cdfy, xi0);
+    const int col = om_first_at_or_above(cdfx + row * OM_PS, xi1);
+    const int cell = row * OM_PS + col;
+    float u0 = 0.0f, u1 = 0.0f, tex = 0.0f;
+    int found = 0;
+    for (uint32_t it = 0; it < 24 && !found; it++)
+        for (uint32_t lane = 0; lane < 64; lane++) {
+            const uint32_t k = 2u + 3u * (it * 64u + lane);
+            const float a0 = om_wfpg_uniform(seed, ray, k), a1 = om_wfpg_uniform(seed, ray, k + 1);
+            const float acc = om_wfpg_uniform(seed, ray, k + 2);
+            const int ix = (int)(a0 * (float)DX), iy = (int)(a1 * (float)DY);
+            const float val = f[(size_t)(row * (int)DY + iy) * X + col * (int)DX + ix];
+            if (lane == 0) {
+                u0 = a0;
+                u1 = a1;
+                tex = val;
+            }
+            if (acc <= val / mx[cell]) {
+                u0 = a0;
+                u1 = a1;
+                tex = val;
+                found = 1;
+                break;
+            }
+        }
+    const float inner = found ? tex / avg[cell] : 1.0f;
+    *pdf = ((pdfy[row] * pdfx[cell]) * inner) * (0.25f * 0.318309886183790671538f);
+    uv[0] = ((float)col * (float)DX + u0 * (float)DX) * (1.0f / (float)X);
+    uv[1] = ((float)row * (float)DY + u1 * (float)DY) * (1.0f / (float)Y);
+}
+
 /* The whole stage for every bin (KCGenAndSampleDistribution,
  * WFPGTracerKC.cuh:1433-1545, radiance query :1312-1318). */
 int om_wfpg_sample_fields(const om_wfpg_desc* d)
@@ -202,16 +341,33 @@ int om_wfpg_sample_fields(const om_wfpg_desc* d)
         for (uint32_t p = 0; p < n; p++) raw[p] = (src[p] == 0.0f) ? 1.0e-4f : src[p];
         om_wfpg_filter(raw, X, Y, w, filt);
         if (d->out_filtered) memcpy(d->out_filtered + (size_t)b * n, filt, (size_t)n * sizeof(float));
-        om_wfpg_pwc(filt, X, Y, pdfx, cdfx, pdfy, cdfy);
+        float savg[64], smax[64], ppx[64], pcx[64], ppy[8], pcy[8];
+        if (d->bxdf_field) om_small_field(filt, X, Y, savg, smax);
+        else om_wfpg_pwc(filt, X, Y, pdfx, cdfx, pdfy, cdfy);
         const uint32_t lo = d->bin_offsets[b], hi = d->bin_offsets[b + 1];
         for (uint32_t k = lo; k < hi; k++) {
             const uint32_t r = d->ray_ids[k];
             if (r >= d->ray_count) continue;
             float uv[2];
-            om_wfpg_pwc_sample(cdfx, cdfy, X, Y, d->xi[2 * (size_t)r], d->xi[2 * (size_t)r + 1], uv);
-            d->out_uv[2 * (size_t)r] = uv[0];
-            d->out_uv[2 * (size_t)r + 1] = uv[1];
-            d->out_pdf[r] = om_wfpg_pwc_pdf(pdfx, pdfy, X, Y, uv[0], uv[1]);
+            if (d->bxdf_field) {
+                om_product_pwc(savg, d->bxdf_field + 64 * (size_t)r, ppx, pcx, ppy, pcy);
+                float pdf;
+                om_product_sample(filt, X, Y, savg, smax, ppx, pcx, ppy, pcy, d->seed, r, uv, &pdf);
+                d->out_uv[2 * (size_t)r] = uv[0];
+                d->out_uv[2 * (size_t)r + 1] = uv[1];
+                d->out_pdf[r] = pdf;
+                if (d->eval_uv)
+                    d->out_eval_pdf[r] = om_product_pdf(filt, X, Y, savg, ppx, ppy, d->eval_uv[2 * (size_t)r],
+                                                        d->eval_uv[2 * (size_t)r + 1]);
+            } else {
+                om_wfpg_pwc_sample(cdfx, cdfy, X, Y, d->xi[2 * (size_t)r], d->xi[2 * (size_t)r + 1], uv);
+                d->out_uv[2 * (size_t)r] = uv[0];
+                d->out_uv[2 * (size_t)r + 1] = uv[1];
+                d->out_pdf[r] = om_wfpg_pwc_pdf(pdfx, pdfy, X, Y, uv[0], uv[1]);
+                if (d->eval_uv)
+                    d->out_eval_pdf[r] = om_wfpg_pwc_pdf(pdfx, pdfy, X, Y, d->eval_uv[2 * (size_t)r],
+                                                         d->eval_uv[2 * (size_t)r + 1]);
+            }
             if (d->out_dir) {
                 float z[3];
                 om_octa_to_dirs(uv, 1, z);
@@ -219,9 +375,6 @@ int om_wfpg_sample_fields(const om_wfpg_desc* d)
                 d->out_dir[3 * (size_t)r + 1] = z[2];
                 d->out_dir[3 * (size_t)r + 2] = z[0];
             }
-            if (d->eval_uv)
-                d->out_eval_pdf[r] = om_wfpg_pwc_pdf(pdfx, pdfy, X, Y, d->eval_uv[2 * (size_t)r],
-                                                     d->eval_uv[2 * (size_t)r + 1]);
         }
     }
     free(raw);

@@ -350,6 +350,8 @@ typedef struct {
     float*          out_pdf;
     float*          out_eval_pdf;     /* written iff eval_uv */
     float*          out_filtered;     /* may be NULL */
+    const float*    bxdf_field;       /* product mode when not NULL: [ray][64] */
+    uint32_t        seed;             /* product mode's random stream */
 } om_wfpg_desc;
 void  om_wfpg_gauss_weights(float alpha, float w[11]);
 void  om_octa_wrap_int(int32_t x, int32_t y, int32_t X, int32_t Y, int32_t out[2]);
@@ -359,6 +361,7 @@ void  om_wfpg_pwc_sample(const float* cdfx, const float* cdfy, uint32_t X, uint3
                          float uv[2]);
 float om_wfpg_pwc_pdf(const float* pdfx, const float* pdfy, uint32_t X, uint32_t Y, float u, float v);
 int   om_wfpg_sample_fields(const om_wfpg_desc* d);
+float om_wfpg_uniform(uint32_t seed, uint32_t ray, uint32_t k);
 
 /* ---- Triangle voxeliser of the guiding octree (GPUPrimitiveTriangle.h:
  * 384-620): records of every conservatively covered pixel, triangle by

@@ -95,7 +95,7 @@ class OmWfpgDesc(C.Structure):
     _fields_ = [("field_x", _U32), ("field_y", _U32), ("bin_count", _U32), ("ray_count", _U32),
                 ("filter_alpha", _F), ("fields", _P), ("bin_offsets", _P), ("ray_ids", _P), ("xi", _P),
                 ("eval_uv", _P), ("out_uv", _P), ("out_dir", _P), ("out_pdf", _P), ("out_eval_pdf", _P),
-                ("out_filtered", _P)]
+                ("out_filtered", _P), ("bxdf_field", _P), ("seed", _U32)]
 
 
 _lib = None
@@ -956,7 +956,8 @@ class Pwc2D:
 
 
 def wfpg_sample_fields(fields: np.ndarray, bin_offsets: np.ndarray, ray_ids: np.ndarray, xi: np.ndarray,
-                       alpha: float = 1.0, eval_uv: np.ndarray | None = None, ray_count: int | None = None):
+                       alpha: float = 1.0, eval_uv: np.ndarray | None = None, ray_count: int | None = None,
+                       bxdf: np.ndarray | None = None, seed: int = 0):
     """om_wfpg_sample_fields.  fields [bins][Y][X].  Returns dict of uv [n][2],
     dir [n][3], pdf [n], eval_pdf [n] (or None), filtered [bins][Y][X]; rays
     outside every bin keep NaN."""
@@ -972,9 +973,10 @@ def wfpg_sample_fields(fields: np.ndarray, bin_offsets: np.ndarray, ray_ids: np.
     if eval_uv is not None:
         ev = np.ascontiguousarray(eval_uv, dtype=np.float32)
         out["eval_pdf"] = np.full(n, np.nan, np.float32)
+    bx = np.ascontiguousarray(bxdf, dtype=np.float32) if bxdf is not None else None
     d = OmWfpgDesc(X, Y, bins, n, alpha, _ptr(fields), _ptr(bin_offsets), _ptr(ray_ids), _ptr(xi), _ptr(ev),
                    _ptr(out["uv"]), _ptr(out["dir"]), _ptr(out["pdf"]), _ptr(out["eval_pdf"]),
-                   _ptr(out["filtered"]))
+                   _ptr(out["filtered"]), _ptr(bx), seed)
     lib().om_wfpg_sample_fields(C.byref(d))
     return out
 

@@ -202,3 +202,30 @@ def test_rejects_bad_arguments(ctx):
         ctx.wfpg_sample_fields(f, 8, 8, 2, off, ids, 2, None, uv, pdf)
     ctx.wfpg_sample_fields(None, 8, 8, 0, None, None, 0, None, None, None)   # no bins: nothing to do
     ctx.sync()
+
+
+@pytest.mark.parametrize("X,Y", [(64, 64), (32, 32), (16, 16), (8, 8), (64, 32)])
+def test_product_mode_matches_oracle(ctx, oracle, X, Y):
+    """The 8x8 product sampler: (u,v), direction, pdf and the pdf at a second
+    (u,v), bit for bit, with per-ray BxDF cells (some near zero, so the 1e-3
+    floor and long rejection runs occur)."""
+    rng = np.random.default_rng(X + 3 * Y)
+    n_rays = 3000
+    fields, offsets, ids, xi, ev = make_case(rng, X, Y, 12, n_rays)
+    bx = rng.gamma(0.5, 1.0, (n_rays, 64)).astype(np.float32)
+    bx[rng.random((n_rays, 64)) < 0.2] = 0.0
+    d_f, d_off, d_ids, d_ev, d_bx = dev(fields), dev(offsets), dev(ids), dev(ev), dev(bx.reshape(-1))
+    nan = float("nan")
+    d_uv = torch.full((n_rays, 2), nan, device="cuda")
+    d_pdf = torch.full((n_rays,), nan, device="cuda")
+    d_epdf = torch.full((n_rays,), nan, device="cuda")
+    d_dir = torch.full((n_rays, 3), nan, device="cuda")
+    torch.cuda.synchronize()
+    ctx.wfpg_sample_fields(d_f, X, Y, 12, d_off, d_ids, n_rays, None, d_uv, d_pdf, alpha=1.0, d_eval_uv=d_ev,
+                           d_out_eval_pdf=d_epdf, d_out_dir=d_dir, d_bxdf=d_bx, seed=1234)
+    ctx.sync()
+    want = oracle.wfpg_sample_fields(fields, offsets, ids, xi, alpha=1.0, eval_uv=ev, ray_count=n_rays, bxdf=bx,
+                                     seed=1234)
+    for k, t in (("uv", d_uv), ("pdf", d_pdf), ("eval_pdf", d_epdf), ("dir", d_dir)):
+        assert np.array_equal(t.cpu().numpy(), want[k], equal_nan=True), k
+    assert np.isfinite(want["pdf"]).sum() > 0.9 * len(ids)

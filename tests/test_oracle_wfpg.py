@@ -265,3 +265,55 @@ def test_stage_is_the_composition_of_its_parts(oracle):
     # an all-empty field is uniform after the raise
     r = ids[offsets[2]]
     assert abs(out["pdf"][r] * 4 * np.pi - 1) < 1e-5
+
+
+# ---------------------------------------------------------------- product mode
+
+def _one_bin(field, n, bxdf_row, seed=3, ev=None):
+    import numpy as _np
+    bins = field[None]
+    off = _np.array([0, n], _np.uint32)
+    ids = _np.arange(n, dtype=_np.uint32)
+    bx = _np.tile(bxdf_row.astype(_np.float32), (n, 1))
+    return bins, off, ids, bx
+
+
+def test_product_uniform_field_and_bxdf(oracle):
+    f = np.ones((16, 16), np.float32)
+    bins, off, ids, bx = _one_bin(f, 400, np.ones(64))
+    out = oracle.wfpg_sample_fields(bins, off, ids, np.zeros((400, 2), np.float32), alpha=1e-3, bxdf=bx, seed=5)
+    assert np.allclose(out["pdf"] * 4 * np.pi, 1.0, atol=1e-5)
+    assert ((out["uv"] >= 0) & (out["uv"] < 1)).all()
+
+
+def test_product_samples_follow_bxdf_times_field(oracle):
+    """Texel frequencies match bxdf[cell] x field[texel] (chi-square), and each
+    sample's pdf equals the pdf evaluated at it."""
+    rng = np.random.default_rng(12)
+    X = Y = 16
+    f = rng.uniform(0.2, 3.0, (Y, X)).astype(np.float32)
+    bxdf = rng.uniform(0.5, 2.0, 64).astype(np.float32)
+    n = 60000
+    bins, off, ids, bx = _one_bin(f, n, bxdf)
+    # alpha 1e-3 makes the filter the identity, so the sampled field is f
+    out = oracle.wfpg_sample_fields(bins, off, ids, np.zeros((n, 2), np.float32), alpha=1e-3, bxdf=bx, seed=9)
+    uv = out["uv"]
+    ev = oracle.wfpg_sample_fields(bins, off, ids, np.zeros((n, 2), np.float32), alpha=1e-3, bxdf=bx, seed=9,
+                                   eval_uv=uv)
+    assert np.allclose(ev["eval_pdf"], out["pdf"], rtol=1e-6)
+    tx = np.minimum((uv[:, 0] * X).astype(int), X - 1)
+    ty = np.minimum((uv[:, 1] * Y).astype(int), Y - 1)
+    counts = np.zeros((Y, X))
+    np.add.at(counts, (ty, tx), 1)
+    cell = (np.arange(Y)[:, None] // 2) * 8 + (np.arange(X)[None, :] // 2)
+    want = bxdf[cell].astype(np.float64) * f.astype(np.float64)
+    want /= want.sum()
+    assert stats.chisquare(counts.ravel(), want.ravel() * n).pvalue > 1e-4
+    # the pdf integrates to one over the sphere
+    cu = ((np.arange(X) + 0.5) / X).astype(np.float32)
+    grid = np.stack(np.meshgrid(cu, cu), -1).reshape(-1, 2).astype(np.float32)
+    m = len(grid)
+    b2, o2, i2, x2 = _one_bin(f, m, bxdf)
+    g = oracle.wfpg_sample_fields(b2, o2, i2, np.zeros((m, 2), np.float32), alpha=1e-3, bxdf=x2, seed=9,
+                                  eval_uv=grid)
+    assert abs(float(g["eval_pdf"].astype(np.float64).sum()) * 4 * np.pi / m - 1.0) < 1e-4

@@ -317,8 +317,9 @@ def voxel_leg(N, ctx, sc, torch):
     torch.cuda.synchronize()
     count = ctx.svo_voxelize(d_t, n, cube, depth)
     codes = torch.empty(count, dtype=torch.int32, device="cuda")
+    prims = torch.empty(count, dtype=torch.int32, device="cuda")
     svo = N.Svo(ctx)
-    ctx.svo_voxelize(d_t, n, cube, depth, codes, count)
+    ctx.svo_voxelize(d_t, n, cube, depth, codes, count, d_prims=prims)
     svo.build(codes, count, cube, depth)
     ctx.sync()
     t0 = time.perf_counter()
@@ -332,32 +333,51 @@ def voxel_leg(N, ctx, sc, torch):
     ctx.sync()
     build = (time.perf_counter() - t0) / 3
     nodes = svo.view().nodes
+    # node payload from one path vertex per 1080p pixel: triangle centroids
+    # with random wo and luminance
+    nv = 1920 * 1080
+    rng = np.random.default_rng(4)
+    pick = rng.integers(0, n, nv)
+    cen = tris[pick].reshape(nv, 3, 3).mean(1)
+    wo = rng.normal(size=(nv, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    verts = torch.from_numpy(np.hstack([cen, wo, rng.gamma(0.8, 1.0, (nv, 1))]).astype(np.float32).reshape(-1)).cuda()
+    d_irr = torch.empty(2 * nodes, device="cuda")
+    d_nrm = torch.empty(3 * nodes, device="cuda")
+    torch.cuda.synchronize()
+    svo.payload(d_t, codes, prims, count, verts, nv, d_irr, d_nrm)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        svo.payload(d_t, codes, prims, count, verts, nv, d_irr, d_nrm)
+    ctx.sync()
+    payload = (time.perf_counter() - t0) / 3
     # cone-traced 64x64 fields of 1024 bins from points inside the scene,
-    # the octree's node payload random (the field trace reads it once per cone)
+    # reading that payload
     rng = np.random.default_rng(3)
     bins, side = 1024, 64
     org = np.hstack([lo + (hi - lo) * rng.uniform(0.05, 0.95, (bins, 3)),
                      np.full((bins, 1), cube[3] / 1024 * 0.15 + 1e-5)]).astype(np.float32)
     d_o = torch.from_numpy(org.reshape(-1)).cuda()
     d_j = torch.rand(bins * 2, device="cuda")
-    d_i = torch.rand(nodes * 2, device="cuda")
-    d_n = torch.nn.functional.normalize(torch.randn(nodes, 3, device="cuda"), dim=1).contiguous()
     d_f = torch.empty(bins * side * side, device="cuda")
     torch.cuda.synchronize()
-    svo.trace_fields(side, side, bins, d_o, d_j, d_i, d_n, d_f)
+    svo.trace_fields(side, side, bins, d_o, d_j, d_irr, d_nrm, d_f)
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(3):
-        svo.trace_fields(side, side, bins, d_o, d_j, d_i, d_n, d_f)
+        svo.trace_fields(side, side, bins, d_o, d_j, d_irr, d_nrm, d_f)
     ctx.sync()
     trace = (time.perf_counter() - t0) / 3
     svo.close()
     cones = bins * side * side
     log(f"voxelise: {n} triangles -> {count} records in {vox * 1e3:.2f} ms; octree of {nodes} nodes "
-        f"in {build * 1e3:.2f} ms; {cones} field cones in {trace * 1e3:.2f} ms")
+        f"in {build * 1e3:.2f} ms; payload from {nv} vertices in {payload * 1e3:.2f} ms; {cones} field cones in "
+        f"{trace * 1e3:.2f} ms")
     return {"workload": f"{n} triangles of the frame's scene, depth {depth}", "records": int(count),
             "octree_nodes": int(nodes), "voxelize_ms": round(vox * 1e3, 3), "octree_build_ms": round(build * 1e3, 3),
             "Mtris_per_s": round(n / vox / 1e6, 1),
+            "payload": {"vertices": nv, "ms": round(payload * 1e3, 3)},
             "field_trace": {"bins": bins, "field": f"{side}x{side}", "cones": cones, "ms": round(trace * 1e3, 3),
                             "Mcones_per_s": round(cones / trace / 1e6, 1)}}
 

@@ -94,25 +94,38 @@ __device__ inline float pwc_pdf(const float* s_pdf, const float* s_py, float u, 
 // ---- 8x8 product sampling (ProductSampler, ProductSampler.cuh:16-725, pure
 // guiding as KCGenAndSampleDistributionProduct uses it, WFPGTracerKC.cuh:
 // 1562-1766): one wave per ray, lane c = cell c of the 8x8 grid.  Every
-// random draw is a hash of (seed, ray, draw index), so the result does not
-// depend on the launch; a rejection round has 64 tries (the wave's lanes).
+// random draw is an integer hash of (seed, ray, draw index), so the result
+// does not depend on the launch; a rejection round has 64 tries (the wave's
+// lanes).  Broadcasts from a wave-uniform lane use readlane, the row
+// prefixes (a different source lane per lane) the LDS crossbar.
 constexpr int PS = 8;
 
-__device__ inline uint64_t wf_mix64(uint64_t z)
+// lowbias32 (C. Wellons' integer hash): a bijection of 32-bit words
+__device__ inline uint32_t wf_hash(uint32_t x)
 {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
 }
 
-__device__ inline float wf_uniform(uint32_t seed, uint32_t ray, uint32_t k)
+// the ray's key: every draw k of the ray is hash(key ^ (k * odd + c))
+__device__ inline uint32_t wf_key(uint32_t seed, uint32_t ray) { return wf_hash((seed * 0x9E3779B9u) ^ wf_hash(ray + 0x632BE5ABu)); }
+
+__device__ inline float wf_uniform(uint32_t key, uint32_t k)
 {
-    const uint64_t h = wf_mix64(wf_mix64(((uint64_t)seed << 32) | ray) + k);
-    return (float)(uint32_t)(h >> 40) * (1.0f / 16777216.0f);
+    return (float)(wf_hash(key ^ (k * 0x85EBCA6Bu + 0x27D4EB2Fu)) >> 8) * (1.0f / 16777216.0f);
 }
 
+// lane broadcasts: any source lane (LDS crossbar), or a wave-uniform one
 __device__ inline float lane_f(float v, int lane) { return __shfl(v, lane, 64); }
+__device__ inline float lane_u(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 template <int X, int Y, int LS>
 __device__ void product_rays(const float* s_f, const float* s_cavg, const float* s_cmax, uint32_t lo, uint32_t hi,
@@ -124,8 +137,9 @@ __device__ void product_rays(const float* s_f, const float* s_cavg, const float*
     const int lane = (int)(threadIdx.x & 63u), wave = (int)(threadIdx.x >> 6);
     const int cx = lane % PS, cy = lane / PS;
     for (uint32_t k = lo + (uint32_t)wave; k < hi; k += WFPG_BLOCK / 64) {
-        const uint32_t r = ray_ids[k];
+        const uint32_t r = (uint32_t)uni((int)ray_ids[k]);
         if (r >= ray_count) continue;
+        const uint32_t key = (uint32_t)uni((int)wf_key(seed, r));
         // product field and its distribution (GeneratePDFAndCDF, :268-330)
         const float prod = fmaxf(1.0e-3f, bxdf[64 * (size_t)r + lane] * s_cavg[lane]);
         float acc = 0.0f, mycdf = 0.0f;
@@ -140,7 +154,7 @@ __device__ void product_rays(const float* s_f, const float* s_cavg, const float*
         float cacc = 0.0f;
 #pragma unroll
         for (int q = 0; q < PS; q++) {
-            rowt[q] = lane_f(total, q * PS);
+            rowt[q] = lane_u(total, q * PS);
             cacc += rowt[q];
             cyv[q] = cacc;
         }
@@ -148,35 +162,35 @@ __device__ void product_rays(const float* s_f, const float* s_cavg, const float*
 #pragma unroll
         for (int q = 0; q < PS; q++) cyv[q] *= crec;
         // cell (Sample, :333-420)
-        const float xi0 = wf_uniform(seed, r, 0u), xi1 = wf_uniform(seed, r, 1u);
+        const float xi0 = wf_uniform(key, 0u), xi1 = wf_uniform(key, 1u);
         int row = 0;
 #pragma unroll
         for (int q = 0; q < PS; q++) row += (xi0 > cyv[q]) ? 1 : 0;
-        row = row > PS - 1 ? PS - 1 : row;
+        row = uni(row > PS - 1 ? PS - 1 : row);
         int col = 0;
 #pragma unroll
-        for (int q = 0; q < PS; q++) col += (xi1 > lane_f(cdfx, row * PS + q)) ? 1 : 0;
-        col = col > PS - 1 ? PS - 1 : col;
+        for (int q = 0; q < PS; q++) col += (xi1 > lane_u(cdfx, row * PS + q)) ? 1 : 0;
+        col = uni(col > PS - 1 ? PS - 1 : col);
         const int cell = row * PS + col;
         float pY = 0.0f;
 #pragma unroll
         for (int q = 0; q < PS; q++)
             if (q == row) pY = rowt[q] * ((float)PS * crec);
-        const float pX = lane_f(pdfx, cell);
+        const float pX = lane_u(pdfx, cell);
         // texel inside the cell by rejection (:421-460)
         float u0 = 0.0f, u1 = 0.0f, tex = 0.0f;
         bool found = false;
         for (uint32_t it = 0; it < 24u; it++) {
             const uint32_t kk = 2u + 3u * (it * 64u + (uint32_t)lane);
-            const float a0 = wf_uniform(seed, r, kk), a1 = wf_uniform(seed, r, kk + 1u);
-            const float ac = wf_uniform(seed, r, kk + 2u);
+            const float a0 = wf_uniform(key, kk), a1 = wf_uniform(key, kk + 1u);
+            const float ac = wf_uniform(key, kk + 2u);
             const int ix = (int)(a0 * (float)DX), iy = (int)(a1 * (float)DY);
             const float val = s_f[(row * DY + iy) * LS + col * DX + ix];
             const uint64_t m = __ballot(ac <= val / s_cmax[cell]);
-            const int pick = m ? (int)__builtin_ctzll(m) : 0;
-            u0 = lane_f(a0, pick);
-            u1 = lane_f(a1, pick);
-            tex = lane_f(val, pick);
+            const int pick = uni(m ? (int)__builtin_ctzll(m) : 0);
+            u0 = lane_u(a0, pick);
+            u1 = lane_u(a1, pick);
+            tex = lane_u(val, pick);
             if (m) {
                 found = true;
                 break;
@@ -200,7 +214,7 @@ __device__ void product_rays(const float* s_f, const float* s_cavg, const float*
             iy = iy < 0 ? 0 : (iy > DY - 1 ? DY - 1 : iy);
             const float val = s_f[(ey * DY + iy) * LS + ex * DX + ix];
             const float in2 = val / s_cavg[ey * PS + ex];
-            const float ex_pdfx = lane_f(pdfx, ey * PS + ex);
+            const float ex_pdfx = lane_u(pdfx, uni(ey * PS + ex));
             float ex_pdfy = 0.0f;
 #pragma unroll
             for (int q = 0; q < PS; q++)

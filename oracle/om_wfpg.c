@@ -198,20 +198,25 @@ float om_wfpg_pwc_pdf(const float* pdfx, const float* pdfy, uint32_t X, uint32_t
  * none, try 0 of the last round is kept as a uniform inner sample.  pdf =
  * pdfY * pdfX * (texel / cell average, 1 for a uniform inner sample) / 4 pi.
  * The reference's rounds are 32-lane warps and its uniforms come from a
- * per-thread generator; here a round has 64 tries and every draw is a hash
- * of (seed, ray, draw index). */
-static uint64_t wf_mix64(uint64_t z)
+ * per-thread generator; here a round has 64 tries and every draw is an
+ * integer hash of (seed, ray, draw index). */
+/* lowbias32 (C. Wellons' integer hash); draw k of a ray is
+ * hash(key ^ (k * 0x85EBCA6B + 0x27D4EB2F)), key = hash((seed * 0x9E3779B9)
+ * ^ hash(ray + 0x632BE5AB)), its top 24 bits over 2^24 */
+static uint32_t wf_hash(uint32_t x)
 {
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
 }
 
 float om_wfpg_uniform(uint32_t seed, uint32_t ray, uint32_t k)
 {
-    const uint64_t h = wf_mix64(wf_mix64(((uint64_t)seed << 32) | ray) + k);
-    return (float)(uint32_t)(h >> 40) * (1.0f / 16777216.0f);
+    const uint32_t key = wf_hash((seed * 0x9E3779B9u) ^ wf_hash(ray + 0x632BE5ABu));
+    return (float)(wf_hash(key ^ (k * 0x85EBCA6Bu + 0x27D4EB2Fu)) >> 8) * (1.0f / 16777216.0f);
 }
 
 #define OM_PS 8

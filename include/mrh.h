@@ -687,6 +687,51 @@ int32_t mrh_render_guided(mrh_context* ctx, mrh_scene* scene, const mrh_camera_d
                           const mrh_render_params* params, const mrh_guide_params* guide, mrh_svo* svo,
                           float* d_image, mrh_render_stats* stats);
 
+/* Guided render with the reference's own guide (WFPGTracer: Render,
+ * GenerateGuidedDirections and Finalize, WFPGTracer.cu:590-760, 1590-1790;
+ * the path work's use of the guided direction, WFPGTracerKC.cuh:380-430).
+ * Replaces the octree guide of mrh_render_guided with the stages of the
+ * voxelised octree: the scene's triangles are voxelised at svo_depth
+ * (mrh_svo_voxelize, mrh_svo_build), `iterations` frames follow as in
+ * mrh_render_guided, every frame but the last records its path vertices and
+ * forms the octree's node payload from them when it ends (mrh_svo_payload),
+ * and every later frame, at every bounce: draws the one-sample mixture
+ * choice for each non-specular hit (guide with probability guide_ratio,
+ * else the BxDF's own sample), bins the rays by octree node
+ * (mrh_svo_bin_rays with min_bin_level / bin_ray_count), cone-traces one
+ * field per bin from the hit point of its ray of the smallest path id
+ * (mrh_wfpg_trace_fields:
+ * 64^2 texels at the first bounce, 32^2, 16^2, then 8^2), filters and
+ * samples it (mrh_wfpg_sample_fields with filter_alpha, the guide pdf at
+ * the BxDF's direction for the balance heuristic), and weights the path by
+ * f cos / (r p_guide + (1 - r) p_bxdf).  Each recorded path vertex takes
+ * 64 B of device memory.  Single-level scenes only
+ * (MRH_ERR_UNSUPPORTED otherwise).  NEE and MIS as in mrh_render_guided.
+ * Differences from the reference: the bin origin is the bin's ray of the
+ * smallest path id, not a random one (so the render does not depend on the
+ * order of a bounce's rays); the product sampler is not used (the reference's
+ * non-product path, productPG = false); the payload is formed from the last
+ * learning frame's vertices. */
+typedef struct {
+    uint32_t iterations;        /* frames, 1..64 (1 = plain path tracing) */
+    uint32_t svo_depth;         /* "OctreeLevel", 1..10 */
+    float    guide_ratio;       /* "MISRatio" */
+    uint32_t min_bin_level;     /* "MinRayBinLevel", 0..10 */
+    uint32_t bin_ray_count;     /* "BinRayCount", 1..MRH_BIN_RAY_COUNT_MAX */
+    float    filter_alpha;      /* "RFieldFilterAlpha", > 0 */
+    float    boundary_radiance; /* radiance of cones that leave the octree, >= 0 */
+} mrh_wfpg_params;
+typedef struct {
+    uint64_t bins;              /* bins over every guided bounce */
+    uint64_t binned_rays;       /* rays sampled from a field, over every guided bounce */
+    uint32_t guided_bounces;    /* bounces whose fields were traced */
+    uint32_t octree_nodes;      /* nodes of the voxel octree */
+    uint32_t voxel_records;     /* voxel records of the scene's triangles */
+} mrh_wfpg_stats;
+int32_t mrh_render_wfpg(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                        const mrh_render_params* params, const mrh_wfpg_params* wfpg, float* d_image,
+                        mrh_render_stats* stats, mrh_wfpg_stats* wstats);
+
 /* Autotuner (no counterpart in the reference): coordinate descent over the
  * launch shape of frame renders — traversal tree width, traversal stack
  * entries in LDS, blocks per CU of the traversal grid, traversal block size,

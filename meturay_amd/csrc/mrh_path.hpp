@@ -447,6 +447,8 @@ struct RecBuf {
     float* tg;         // G
     float* tb;         // B
     float* rad;        // luminance of the radiance arriving along that ray
+    float* vtx;        // optional [stride * max_depth][7]: position, direction toward the
+                       // previous vertex, rad (the octree payload's path vertices)
     uint32_t max_depth;
     uint32_t stride;   // samples per frame
 };
@@ -458,6 +460,16 @@ struct GuideArgs {
     uint32_t depth;
     float ratio;        // probability of sampling the guide
     uint32_t min_samples;   // guide lookup threshold
+    // Per-ray guided directions of the field-traced guide, indexed like the
+    // rays of the bounce (set when the bounce's fields were sampled):
+    // wf_flag 0 = not guided, 1 = the guide's sample, 2 = the BxDF's sample;
+    // wf_gdir / wf_gpdf the guide's sample and pdf, wf_bdir / wf_epdf the
+    // BxDF's sample and the guide pdf there.  Null: the octree guide.
+    const uint32_t* wf_flag;
+    const float* wf_gdir;
+    const float* wf_gpdf;
+    const float* wf_bdir;
+    const float* wf_epdf;
 };
 
 // Camera ray of sample i of a frame (GenerationKernels.cuh:29-108 +
@@ -686,6 +698,7 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
                 const V3 L = v3(fr == 0.0f ? 0.0f : cl.x / fr, fg == 0.0f ? 0.0f : cl.y / fg,
                                 fb == 0.0f ? 0.0f : cl.z / fb);
                 ga.rec.rad[r] = luminance(L);
+                if (ga.rec.vtx) ga.rec.vtx[7 * r + 6] = ga.rec.rad[r];
             }
         }
         return false;
@@ -740,7 +753,11 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
     float gtotal = 0.0f;
     GuideBins gbins;
     V3 ns = v3(0.0f, 0.0f, 1.0f);
-    if (GUIDE && guide &&
+    uint32_t wfl = 0;
+    if (GUIDE && guide && ga.wf_flag) {
+        wfl = ga.wf_flag[rid];
+        guided = wfl != 0u;
+    } else if (GUIDE && guide &&
         (batch == MRH_BATCH_LAMBERT || (batch == MRH_BATCH_METAL && (1.0f - mats.metal[(size_t)id * 10]) < 0.99f))) {
         ns = rotate(conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
         gnode = svo_lookup_guide(ga.svo,
@@ -751,11 +768,28 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
         guided = gtotal > 0.0f;
     }
     if (GUIDE && guided) {
-        const float xi = rng.uniform();
         V3 dir;
         float pg = 0.0f, pb = 0.0f;
         V3 fcos;
-        if (xi < ga.ratio) {
+        if (ga.wf_flag) {
+            // the direction was drawn before the bounce's fields were
+            // sampled (WFPGTracerKC.cuh:1490-1553); the BxDF is evaluated
+            // there as the path work does (:389-430)
+            const bool bx = wfl == 2u;
+            const float* dd = (bx ? ga.wf_bdir : ga.wf_gdir) + 3 * (size_t)rid;
+            dir = v3(dd[0], dd[1], dd[2]);
+            pg = bx ? ga.wf_epdf[rid] : ga.wf_gpdf[rid];
+            if (wfl == 3u) {
+                // the BxDF's own sample failed: the path ends, as without the guide
+                fcos = v3(0.0f, 0.0f, 0.0f);
+                pg = 0.0f;
+            } else if (batch == MRH_BATCH_LAMBERT) {
+                const float* a = mats.lambert + (size_t)id * 3;
+                fcos = lambert_eval(s, v3(a[0], a[1], a[2]), dir, pb);
+            } else {
+                fcos = metal_eval(s, wi, mats.metal + (size_t)id * 10, dir, pb);
+            }
+        } else if (rng.uniform() < ga.ratio) {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
             dir = guide_sample(gbins, ns, gtotal, u0, u1, u2, pg);
             if (batch == MRH_BATCH_LAMBERT) {
@@ -812,6 +846,12 @@ __device__ __forceinline__ bool shade_ray(const PathBuf& in, const HitBuf& hit, 
         ga.rec.tg[r] = t2.y;
         ga.rec.tb[r] = t2.z;
         ga.rec.rad[r] = 0.0f;
+        if (ga.rec.vtx) {
+            float* x = ga.rec.vtx + 7 * r;
+            x[0] = s.pos.x; x[1] = s.pos.y; x[2] = s.pos.z;
+            x[3] = wi.x; x[4] = wi.y; x[5] = wi.z;
+            x[6] = 0.0f;
+        }
     }
     nx.o = o.o;
     nx.d = o.d;

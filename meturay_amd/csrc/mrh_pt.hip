@@ -376,6 +376,7 @@ struct Frame {
     uint32_t count_traversal = 0, bounce = 0;
     uint32_t learn = 0, guide = 0;   // record path vertices / sample the guide
     GuideArgs ga = {};
+    struct WfState* wf = nullptr;    // field-traced guide (render_wfpg), sampled on guide frames
 };
 
 void carve_frame(void* base, uint64_t cap, Frame& f)
@@ -599,6 +600,288 @@ int32_t frame_begin(mrh_context* ctx, const mrh_camera_desc* cd, const mrh_rende
                   f.w, f.h, f.S, f.seed, f.sample_base, (uint32_t)f.total, f.bufs[0], coords);
 }
 
+// ------------------------------------------------- field-traced guiding
+// State of one render_wfpg call: the scene's voxel octree and its payload,
+// the per-ray arrays of a guided bounce and the per-bin fields.
+struct WfState {
+    mrh_wfpg_params wp = {};
+    mrh_svo* svo = nullptr;
+    float cube[4] = {0, 0, 0, 1};
+    float* tris = nullptr;           // [n_tris][9] the scene's triangles
+    uint32_t n_tris = 0;
+    uint32_t *vcodes = nullptr, *vprims = nullptr;   // voxel records
+    uint32_t n_vox = 0, nodes = 0;
+    float *irr = nullptr, *nrm = nullptr;            // node payload
+    bool have_payload = false;
+    // per ray of a bounce (cap entries)
+    uint32_t *codes = nullptr, *flag = nullptr, *bins = nullptr, *order = nullptr, *offs = nullptr,
+             *pbins = nullptr, *pcount = nullptr;
+    float4* org = nullptr;
+    float2 *xi = nullptr, *euv = nullptr, *guv = nullptr;
+    float *bdir = nullptr, *gdir = nullptr, *gpdf = nullptr, *epdf = nullptr;
+    // per bin
+    float4* origins = nullptr;
+    float2* jitter = nullptr;
+    float* fields = nullptr;
+    size_t field_floats = 0, bin_cap = 0;
+    unsigned long long* minkey = nullptr;   // per node: the bin origin's (sample << 32 | ray)
+    float* vtx = nullptr;            // recorded path vertices, [records][7]
+    mrh_wfpg_stats st = {};
+    std::vector<void*> allocs;
+};
+
+// Per-ray parameters of the bounce's guide preparation
+struct WfPrep {
+    float lox, loy, loz, inv_cell;   // leaf grid of the voxel octree
+    uint32_t depth;
+    float tmin;                      // field cone start distance
+    float ratio;                     // probability of the guide's sample
+    uint32_t *codes, *flag;
+    float4* org;
+    float2 *xi, *euv;
+    float* bdir;
+};
+
+__device__ __forceinline__ uint32_t wf_hash(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ float wf_unit(uint32_t h) { return (float)(h >> 8) * (1.0f / 16777216.0f); }
+
+// The half of KCGenAndSampleDistribution that happens per ray before the
+// field is sampled (WFPGTracerKC.cuh:1480-1522): for a non-specular surface
+// hit (Lambert, or a conductor of roughness >= 0.01 — the vertices the
+// guide of this library serves) the ray's cell code at the hit point, the
+// one-sample mixture choice (guide with probability ratio, else the BxDF's
+// own sample, whose direction and concentric-octahedral (u,v) are kept for
+// the guide pdf there), the guide sample's two uniforms and the bin origin
+// candidate (hit point pushed off the surface, cone start tmin).  Draws come
+// from the path's stream, which is advanced in place.  Flag 3 = the BxDF
+// sample failed (the path ends as it would without the guide).
+__global__ __launch_bounds__(256) void k_wf_prep(PathBuf in, HitBuf hit, uint32_t n, ShadeReq q, WfPrep w)
+{
+    for (uint32_t rid = blockIdx.x * blockDim.x + threadIdx.x; rid < n; rid += gridDim.x * blockDim.x) {
+        const uint32_t key = hit.key[rid];
+        const uint32_t batch = key >> 24, id = key & 0xFFFFFFu;
+        const uint32_t dm = in.dm[rid];
+        const uint32_t depth = dm & 0xFFu;
+        const bool ok = !(dm & DM_NEE) && depth < q.max_depth &&
+                        (batch == MRH_BATCH_LAMBERT ||
+                         (batch == MRH_BATCH_METAL && (1.0f - q.mats.metal[(size_t)id * 10]) < 0.99f));
+        if (!ok) {
+            w.codes[rid] = SVO_NO_NODE;
+            w.flag[rid] = 0u;
+            continue;
+        }
+        V3 c0, c1, c2;
+        const uint32_t li = hit.leaf[rid];
+        leaf_corners(q, li, hit.inst ? hit.inst[rid] : 0u, c0, c1, c2);
+        const V3 d = v3(in.dx[rid], in.dy[rid], in.dz[rid]);
+        const Surface s = q.tbn ? make_surface_tbn(c0, c1, c2, hit.ba[rid], hit.bb[rid], d,
+                                                   q.tbn + 3 * (size_t)__float_as_uint(q.leaves[3 * (size_t)li].w))
+                                : make_surface(c0, c1, c2, hit.ba[rid], hit.bb[rid], d);
+        const V3 wi = -normalize(d);
+        Rng rng{in.rng[rid], pcg32_inc(q.sample_base + in.sample[rid])};
+        uint32_t fl = 1u;
+        float2 e = make_float2(0.5f, 0.5f);
+        if (rng.uniform() >= w.ratio) {
+            BsdfOut o;
+            o.medium = 0u;
+            if (batch == MRH_BATCH_LAMBERT) {
+                const float* a = q.mats.lambert + (size_t)id * 3;
+                lambert_sample(s, v3(a[0], a[1], a[2]), rng, o);
+            } else {
+                metal_sample(s, wi, q.mats.metal + (size_t)id * 10, rng, o);
+            }
+            const float len = sqrtf(dot(o.d, o.d));
+            if (o.pdf > 0.0f && len > 0.0f && isfinite(len)) {
+                const V3 bd = scale(o.d, 1.0f / len);
+                dir_to_octa(v3(bd.z, bd.x, bd.y), e.x, e.y);   // y-up direction -> the map's z-up frame
+                w.bdir[3 * (size_t)rid] = o.d.x;
+                w.bdir[3 * (size_t)rid + 1] = o.d.y;
+                w.bdir[3 * (size_t)rid + 2] = o.d.z;
+                fl = 2u;
+            } else {
+                fl = 3u;
+            }
+        }
+        const float x0 = rng.uniform(), x1 = rng.uniform();
+        w.xi[rid] = make_float2(x0, x1);
+        w.euv[rid] = e;
+        in.rng[rid] = rng.state;
+        w.flag[rid] = fl;
+        w.codes[rid] = svo_code(s.pos, w.lox, w.loy, w.loz, w.inv_cell, w.depth);
+        const V3 p = nudge(s.pos, s.geo_n);
+        w.org[rid] = make_float4(p.x, p.y, p.z, w.tmin);
+    }
+}
+
+// Per node of the octree, the binned ray of the smallest path (sample) id:
+// (sample << 32 | ray) by a 64-bit atomic minimum, so the choice does not
+// depend on the order of the rays in the bounce's buffer.
+__global__ void k_wf_minray(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ bins,
+                            const uint32_t* __restrict__ sample, uint32_t n, unsigned long long* __restrict__ minkey)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (codes[i] == SVO_NO_NODE) continue;
+        atomicMin(minkey + bins[i], ((unsigned long long)sample[i] << 32) | i);
+    }
+}
+
+// Origin and jitter of every bin: the pushed-off hit point of its ray of the
+// smallest path id (the reference takes a random ray of the bin,
+// WFPGTracerKC.cuh:1132-1192) and two hashed uniforms of (key, bin).
+__global__ void k_wf_bins(const uint32_t* __restrict__ count, const uint32_t* __restrict__ part_bins,
+                          const unsigned long long* __restrict__ minkey, const float4* __restrict__ org,
+                          float4* __restrict__ origins, float2* __restrict__ jitter, uint32_t key)
+{
+    const uint32_t nb = *count;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        origins[b] = org[(uint32_t)minkey[part_bins[b]]];
+        const uint32_t h = wf_hash(key ^ wf_hash(2u * b + 1u));
+        jitter[b] = make_float2(wf_unit(h), wf_unit(wf_hash(h ^ 0x9e3779b9u)));
+    }
+}
+
+__global__ void k_wf_tris(const float4* __restrict__ leaves, uint32_t n, float* __restrict__ tris)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int c = 0; c < 3; c++) {
+            const float4 l = leaves[3 * (size_t)i + c];
+            tris[9 * (size_t)i + 3 * c] = l.x;
+            tris[9 * (size_t)i + 3 * c + 1] = l.y;
+            tris[9 * (size_t)i + 3 * c + 2] = l.z;
+        }
+    }
+}
+
+int32_t wf_alloc(mrh_context* ctx, WfState& w, void** p, size_t bytes)
+{
+    MRH_TRY(dmalloc(ctx, p, bytes ? bytes : 4));
+    w.allocs.push_back(*p);
+    return MRH_OK;
+}
+
+void wf_free(WfState& w)
+{
+    for (void* p : w.allocs) (void)hipFree(p);
+    w.allocs.clear();
+    if (w.fields) (void)hipFree(w.fields);
+    w.fields = nullptr;
+    if (w.origins) (void)hipFree(w.origins);
+    w.origins = nullptr;
+    if (w.jitter) (void)hipFree(w.jitter);
+    w.jitter = nullptr;
+    if (w.svo) (void)mrh_svo_destroy(w.svo);
+    w.svo = nullptr;
+}
+
+// GenerateGuidedDirections (WFPGTracer.cu:590-760) for the rays of one
+// bounce: prepare every ray, bin the rays by octree node
+// (mrh_svo_bin_rays), cone-trace each bin's field from its origin
+// (mrh_wfpg_trace_fields; 64^2 texels at the first bounce, then 32^2,
+// 16^2 and 8^2 from the fourth on, as the reference's kernel list by depth,
+// WFPGTracer.cu:676-692) and sample every binned ray
+// (mrh_wfpg_sample_fields).  Host reads: the bounce's ray count, the bin
+// count and the number of binned rays.
+int32_t wf_bounce(mrh_context* ctx, Frame& f, const PathBuf& in, const HitBuf& hb, const ShadeReq& q)
+{
+    WfState& w = *f.wf;
+    uint32_t n = 0;
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(&n, f.counts + f.bounce, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (n == 0) return MRH_OK;
+    const uint32_t D = w.wp.svo_depth;
+    WfPrep pr;
+    pr.lox = w.cube[0];
+    pr.loy = w.cube[1];
+    pr.loz = w.cube[2];
+    pr.inv_cell = (float)(1u << D) / w.cube[3];
+    pr.depth = D;
+    pr.tmin = 0.15f * (w.cube[3] / (float)(1u << D)) + 1e-5f;
+    pr.ratio = w.wp.guide_ratio;
+    pr.codes = w.codes;
+    pr.flag = w.flag;
+    pr.org = w.org;
+    pr.xi = w.xi;
+    pr.euv = w.euv;
+    pr.bdir = w.bdir;
+    MRH_TRY(launch(ctx, "wfpg_prep", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_wf_prep, in, hb, n, q, pr));
+    MRH_TRY(mrh_svo_bin_rays(ctx, w.svo, w.codes, n, w.wp.min_bin_level, w.wp.bin_ray_count, w.bins, w.order,
+                             w.offs, w.pbins, w.pcount));
+    uint32_t nb = 0;
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(&nb, w.pcount, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (nb == 0) return MRH_OK;
+    uint32_t binned = 0;
+    MRH_HIP_CHECK(ctx, hipMemcpyAsync(&binned, w.offs + nb, 4, hipMemcpyDeviceToHost, ctx->stream));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t side = 64u >> (f.bounce < 3u ? f.bounce : 3u);
+    const size_t need = (size_t)nb * side * side;
+    if (need > w.field_floats) {
+        if (w.fields) MRH_HIP_CHECK(ctx, hipFree(w.fields));
+        w.fields = nullptr;
+        w.field_floats = 0;
+        MRH_TRY(dmalloc(ctx, (void**)&w.fields, need * 4));
+        w.field_floats = need;
+    }
+    if (nb > w.bin_cap) {
+        if (w.origins) MRH_HIP_CHECK(ctx, hipFree(w.origins));
+        if (w.jitter) MRH_HIP_CHECK(ctx, hipFree(w.jitter));
+        w.origins = nullptr;
+        w.jitter = nullptr;
+        w.bin_cap = 0;
+        MRH_TRY(dmalloc(ctx, (void**)&w.origins, (size_t)nb * 16));
+        MRH_TRY(dmalloc(ctx, (void**)&w.jitter, (size_t)nb * 8));
+        w.bin_cap = nb;
+    }
+    const uint32_t key = (f.seed * 0x9e3779b9u) ^ (f.sample_base * 0x85ebca6bu) ^ (f.bounce * 0xc2b2ae35u);
+    MRH_TRY(fill_u32(ctx, (uint32_t*)w.minkey, 0xFFFFFFFFu, 2 * (size_t)w.nodes));
+    MRH_TRY(launch(ctx, "wfpg_minray", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_wf_minray,
+                   (const uint32_t*)w.codes, (const uint32_t*)w.bins, (const uint32_t*)in.sample, n, w.minkey));
+    MRH_TRY(launch(ctx, "wfpg_bins", dim3(stride_grid(ctx, nb, 256)), dim3(256), 0, k_wf_bins,
+                   (const uint32_t*)w.pcount, (const uint32_t*)w.pbins, (const unsigned long long*)w.minkey,
+                   (const float4*)w.org, w.origins, w.jitter, key));
+    mrh_wfpg_trace_desc td;
+    memset(&td, 0, sizeof(td));
+    td.field_x = td.field_y = side;
+    td.bin_count = nb;
+    td.max_query_level_offset = 0;
+    td.boundary_radiance = w.wp.boundary_radiance;
+    td.bin_origins = (const float*)w.origins;
+    td.bin_jitter = (const float*)w.jitter;
+    td.node_irradiance = w.irr;
+    td.node_normals = w.nrm;
+    td.out_fields = w.fields;
+    MRH_TRY(mrh_wfpg_trace_fields(ctx, w.svo, &td));
+    mrh_wfpg_fields_desc fd;
+    memset(&fd, 0, sizeof(fd));
+    fd.field_x = fd.field_y = side;
+    fd.bin_count = nb;
+    fd.ray_count = n;
+    fd.filter_alpha = w.wp.filter_alpha;
+    fd.fields = w.fields;
+    fd.bin_offsets = w.offs;
+    fd.ray_ids = w.order;
+    fd.xi = (const float*)w.xi;
+    fd.eval_uv = (const float*)w.euv;
+    fd.out_uv = (float*)w.guv;
+    fd.out_dir = w.gdir;
+    fd.out_pdf = w.gpdf;
+    fd.out_eval_pdf = w.epdf;
+    MRH_TRY(mrh_wfpg_sample_fields(ctx, &fd));
+    w.st.bins += nb;
+    w.st.binned_rays += binned;
+    w.st.guided_bounces++;
+    return MRH_OK;
+}
+
 // One iteration of PathTracer::Render: trace, partition, shade
 int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* image)
 {
@@ -650,6 +933,7 @@ int32_t frame_bounce(mrh_context* ctx, const mrh_scene* s, Frame& f, float* imag
     q.inst = s->two_level ? (const float4*)s->inst : nullptr;
     q.light_inst = s->two_level ? s->light_inst : nullptr;
     q.tbn = s->tbn;
+    if (f.wf && f.guide) MRH_TRY(wf_bounce(ctx, f, in, hb, q));
     auto shade = [&](auto kernel) {
         return launch(ctx, "shade", dim3(grid), dim3(256), 0, kernel, in, outb, hb, (const uint32_t*)f.order,
                       f.counts, bounce, q, f.ga);
@@ -979,6 +1263,153 @@ int32_t render_guided(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd,
     return rc;
 }
 
+// WFPGTracer with its own guide (WFPGTracer.cu:590-760, 1590-1790): the
+// scene's triangles are voxelised into an octree once; `iterations` frames
+// of the same view follow, each with samples^2 spp and its own sample range.
+// Every frame but the last records its path vertices (position, direction
+// toward the previous vertex, luminance of the radiance that came back), and
+// the octree's payload is formed from them when it ends; every frame after
+// the first guides its non-specular vertices with the fields cone-traced
+// through that payload (wf_bounce), one-sample MIS with the BxDF.  The image
+// is the mean over all frames.
+int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const mrh_render_params* p,
+                    const mrh_wfpg_params* wp, float* d_image, mrh_render_stats* stats, mrh_wfpg_stats* wstats)
+{
+    if (s->two_level)
+        return fail(ctx, MRH_ERR_UNSUPPORTED, "mrh_render_wfpg: two-level scenes are not supported");
+    if (wp->min_bin_level > 10)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: min_bin_level must be in [0, 10]");
+    if (wp->bin_ray_count == 0 || wp->bin_ray_count > MRH_BIN_RAY_COUNT_MAX)
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: bin_ray_count must be in [1, %u]",
+                    MRH_BIN_RAY_COUNT_MAX);
+    if (!(wp->filter_alpha > 0.0f) || !std::isfinite(wp->filter_alpha))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: filter_alpha must be finite and > 0");
+    if (!(wp->boundary_radiance >= 0.0f) || !std::isfinite(wp->boundary_radiance))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: boundary_radiance must be finite and >= 0");
+    uint64_t total = 0;
+    MRH_TRY(validate_params(ctx, s, p, &total));
+    mrh_guide_params gp;
+    memset(&gp, 0, sizeof(gp));
+    gp.iterations = wp->iterations;
+    gp.svo_depth = wp->svo_depth;
+    gp.guide_ratio = wp->guide_ratio;
+    MRH_TRY(guide_validate(ctx, &gp, total, p->max_depth));
+    void* arena = nullptr;
+    MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(frame_cap(total, p->nee)), &arena));
+    const size_t img_count = (size_t)p->width * p->height * 3;
+    MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
+    GuideState g;
+    WfState w;
+    w.wp = *wp;
+    int32_t rc = guide_init(ctx, s, total, p->max_depth, &gp, nullptr, false, g);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == MRH_OK && hipEventCreate(&e0) != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "event create");
+    if (rc == MRH_OK && hipEventCreate(&e1) != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "event create");
+    if (rc == MRH_OK) (void)hipEventRecord(e0, ctx->stream);
+    // the voxel octree over the scene's triangles, in the records' cube
+    auto setup = [&]() -> int32_t {
+        // AnisoSVOctreeCPU::Constrcut (AnisoSVO.cu:670-678): the scene's
+        // largest span padded by one leaf on every side
+        const mrh_lbvh* b = s->bvh;
+        float span = 0.0f;
+        for (int a = 0; a < 3; a++) span = fmaxf(span, b->hi[a] - b->lo[a]);
+        const float pad = span / (float)(1u << wp->svo_depth);
+        for (int a = 0; a < 3; a++) w.cube[a] = b->lo[a] - pad;
+        w.cube[3] = span + 2.0f * pad;
+        w.n_tris = b->n;
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.tris, (size_t)w.n_tris * 36));
+        MRH_TRY(launch(ctx, "wfpg_tris", dim3(stride_grid(ctx, w.n_tris, 256)), dim3(256), 0, k_wf_tris,
+                       (const float4*)b->leaves, w.n_tris, w.tris));
+        MRH_TRY(mrh_svo_voxelize(ctx, w.tris, w.n_tris, w.cube, wp->svo_depth, nullptr, nullptr, 0, &w.n_vox));
+        if (w.n_vox == 0) return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: the scene voxelises to nothing");
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.vcodes, (size_t)w.n_vox * 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.vprims, (size_t)w.n_vox * 4));
+        uint32_t again = 0;
+        MRH_TRY(mrh_svo_voxelize(ctx, w.tris, w.n_tris, w.cube, wp->svo_depth, w.vcodes, w.vprims, w.n_vox,
+                                 &again));
+        MRH_TRY(mrh_svo_create(ctx, &w.svo));
+        MRH_TRY(mrh_svo_build(ctx, w.svo, w.vcodes, w.n_vox, w.cube, wp->svo_depth));
+        const SvoView v = svo_view(w.svo);
+        w.nodes = v.lvl_off[v.depth + 1];
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.irr, (size_t)w.nodes * 8));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.nrm, (size_t)w.nodes * 12));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.minkey, (size_t)w.nodes * 8));
+        const uint64_t cap = frame_cap(total, p->nee);
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.codes, cap * 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.flag, cap * 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.bins, cap * 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.order, cap * 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.offs, (cap + 1) * 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.pbins, cap * 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.pcount, 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.org, cap * 16));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.xi, cap * 8));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.euv, cap * 8));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.guv, cap * 8));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.bdir, cap * 12));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.gdir, cap * 12));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.gpdf, cap * 4));
+        MRH_TRY(wf_alloc(ctx, w, (void**)&w.epdf, cap * 4));
+        if (g.n_rec) MRH_TRY(wf_alloc(ctx, w, (void**)&w.vtx, (size_t)g.n_rec * 28));
+        return MRH_OK;
+    };
+    if (rc == MRH_OK) rc = setup();
+    mrh_render_stats local;
+    memset(&local, 0, sizeof(local));
+    for (uint32_t it = 0; it < wp->iterations && rc == MRH_OK; it++) {
+        Frame f;
+        rc = frame_begin(ctx, cd, p, arena, (uint32_t)(it * total), f);
+        if (rc == MRH_OK) rc = guide_frame(ctx, g, it, f);
+        if (rc != MRH_OK) break;
+        f.wf = &w;
+        f.guide = w.have_payload ? 1u : 0u;
+        f.ga.rec.vtx = w.vtx;
+        f.ga.wf_flag = w.flag;
+        f.ga.wf_gdir = w.gdir;
+        f.ga.wf_gpdf = w.gpdf;
+        f.ga.wf_bdir = w.bdir;
+        f.ga.wf_epdf = w.epdf;
+        if (f.learn) rc = fill_u32(ctx, (uint32_t*)w.vtx, 0x7F800000u, (size_t)g.n_rec * 7);   // +inf: no vertex
+        while (rc == MRH_OK && f.bounce < f.max_depth) rc = frame_bounce(ctx, s, f, d_image);
+        if (rc == MRH_OK) rc = frame_counts(ctx, f, &local);
+        if (rc == MRH_OK && f.learn) {
+            mrh_svo_payload_desc pd;
+            memset(&pd, 0, sizeof(pd));
+            pd.tris = w.tris;
+            pd.codes = w.vcodes;
+            pd.prims = w.vprims;
+            pd.n_records = w.n_vox;
+            pd.vertices = w.vtx;
+            pd.n_vertices = (uint32_t)g.n_rec;
+            pd.node_irradiance = w.irr;
+            pd.node_normals = w.nrm;
+            rc = mrh_svo_payload(ctx, w.svo, &pd);
+            if (rc == MRH_OK) w.have_payload = true;
+        }
+    }
+    if (rc == MRH_OK)
+        rc = launch(ctx, "finalize", dim3(stride_grid(ctx, img_count, 256)), dim3(256), 0, k_scale, d_image,
+                    img_count, 1.0f / ((float)((uint64_t)p->samples * p->samples) * (float)wp->iterations));
+    if (rc == MRH_OK) {
+        (void)hipEventRecord(e1, ctx->stream);
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "render_wfpg: %s", hipGetErrorString(e));
+        float ms = 0.0f;
+        if (rc == MRH_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+        local.render_ms = ms;
+        if (rc == MRH_OK && stats) *stats = local;
+        w.st.octree_nodes = w.nodes;
+        w.st.voxel_records = w.n_vox;
+        if (rc == MRH_OK && wstats) *wstats = w.st;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    wf_free(w);
+    guide_free(g);
+    return rc;
+}
+
 __global__ void k_mean(const float* __restrict__ acc, float* __restrict__ out, size_t count, float s)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
@@ -1037,6 +1468,15 @@ int32_t mrh_render_filtered(mrh_context* ctx, mrh_scene* scene, const mrh_camera
     if (scene->ctx != ctx)
         return fail(ctx, MRH_ERR_INVALID, "mrh_render_filtered: scene belongs to another context");
     return render(ctx, scene, cam, params, filter, d_image, stats);
+}
+
+int32_t mrh_render_wfpg(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,
+                        const mrh_render_params* params, const mrh_wfpg_params* wfpg, float* d_image,
+                        mrh_render_stats* stats, mrh_wfpg_stats* wstats)
+{
+    if (!ctx || !scene || !cam || !params || !wfpg || !d_image) return MRH_ERR_INVALID;
+    if (scene->ctx != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: scene belongs to another context");
+    return render_wfpg(ctx, scene, cam, params, wfpg, d_image, stats, wstats);
 }
 
 int32_t mrh_render_guided(mrh_context* ctx, mrh_scene* scene, const mrh_camera_desc* cam,

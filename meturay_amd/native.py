@@ -111,6 +111,16 @@ class GuideParams(C.Structure):
                 ("min_samples", _U32)]
 
 
+class WfpgParams(C.Structure):
+    _fields_ = [("iterations", _U32), ("svo_depth", _U32), ("guide_ratio", _F), ("min_bin_level", _U32),
+                ("bin_ray_count", _U32), ("filter_alpha", _F), ("boundary_radiance", _F)]
+
+
+class WfpgStats(C.Structure):
+    _fields_ = [("bins", C.c_uint64), ("binned_rays", C.c_uint64), ("guided_bounces", _U32),
+                ("octree_nodes", _U32), ("voxel_records", _U32)]
+
+
 class ServerParams(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("wave_paths", _U32), ("max_slots", _U32),
                 ("use_graphs", _U32), ("timeout_ms", _U32), ("policy", _U32)]
@@ -223,6 +233,8 @@ _SIGS = {
     "mrh_svo_cone_trace": (_I32, [_P, _P, _P, _U32, _F, _U32, _P, _P, _P]),
     "mrh_wfpg_trace_fields": (_I32, [_P, _P, C.POINTER(WfpgTraceDesc)]),
     "mrh_svo_payload": (_I32, [_P, _P, C.POINTER(SvoPayloadDesc)]),
+    "mrh_render_wfpg": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams), C.POINTER(WfpgParams),
+                               _P, C.POINTER(RenderStats), C.POINTER(WfpgStats)]),
     "mrh_render_guided": (_I32, [_P, _P, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                  C.POINTER(GuideParams), _P, _P, C.POINTER(RenderStats)]),
     "mrh_server_create": (_I32, [_P, C.POINTER(ServerParams), C.POINTER(_P)]),
@@ -684,6 +696,20 @@ class Scene:
                                                svo.h if svo is not None else None, ptr(d_image),
                                                C.byref(st)))
         return st
+
+    def render_wfpg(self, d_image, width, height, samples, max_depth, rr_start, seed=0, iterations=4,
+                    svo_depth=10, guide_ratio=0.5, min_bin_level=5, bin_ray_count=512, filter_alpha=1.0,
+                    boundary_radiance=0.0, camera=None, nee=0, mis=0):
+        """Guided render with the field-traced guide of the voxel octree
+        (mrh_render_wfpg); returns (RenderStats, WfpgStats)."""
+        cam = camera_desc(camera or self.camera)
+        p = RenderParams(width, height, samples, max_depth, rr_start, nee, seed, 0, mis)
+        w = WfpgParams(iterations, svo_depth, guide_ratio, min_bin_level, bin_ray_count, filter_alpha,
+                       boundary_radiance)
+        st, ws = RenderStats(), WfpgStats()
+        self.ctx.check(lib().mrh_render_wfpg(self.ctx.h, self.h, C.byref(cam), C.byref(p), C.byref(w),
+                                             ptr(d_image), C.byref(st), C.byref(ws)))
+        return st, ws
 
     def autotune(self, d_image, width, height, samples, max_depth, rr_start, seed=0, reps=3,
                  camera=None) -> TuneResult:

@@ -1,0 +1,82 @@
+"""The guided render with the field-traced guide of the voxel octree
+(mrh_render_wfpg; WFPGTracer::Render / GenerateGuidedDirections / Finalize,
+WFPGTracer.cu:590-760, 1590-1790).  Its stages are bit-exact against the
+oracle one by one (test_gpu_vox.py, test_gpu_cone.py, test_gpu_wfpg.py,
+test_gpu_guide.py for the ray binning); the render is checked for
+unbiasedness against the oracle's 4096-spp converged images in
+test_gpu_converged.py.  Here: one iteration is plain path tracing, the
+voxel octree is the oracle's, the guided frames bin and sample rays, a
+render is deterministic per path (equal path counts and bin counts when
+repeated; the bin origin does not depend on the order of a bounce's rays),
+and the argument checks."""
+import numpy as np
+import pytest
+
+import instancing_scenes as I
+from meturay_amd import native as N
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def test_one_iteration_is_plain_path_tracing(ctx):
+    scene = N.Scene(ctx, N.gen_cornell(2))
+    a = torch.zeros(32 * 32 * 3, dtype=torch.float32, device="cuda")
+    b = torch.zeros_like(a)
+    torch.cuda.synchronize()
+    sa = scene.render(a.data_ptr(), 32, 32, 2, 6, 2, 4)
+    sb, ws = scene.render_wfpg(b.data_ptr(), 32, 32, 2, 6, 2, 4, iterations=1)
+    assert sa.segments == sb.segments
+    assert torch.allclose(a, b, rtol=1e-5, atol=1e-6)
+    assert ws.guided_bounces == 0 and ws.bins == 0
+    scene.close()
+
+
+@pytest.mark.parametrize("nee", [0, 1])
+def test_guided_frames_bin_and_sample_deterministically(ctx, oracle, nee):
+    sc = N.gen_cornell(2)
+    w, h, S, depth, rr, seed, sd = 64, 48, 2, 6, 2, 5, 8
+    print(f"[wfpg render] cornell {w}x{h} S={S} nee={nee}", flush=True)
+    scene = N.Scene(ctx, sc)
+    img = torch.zeros(w * h * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    kw = dict(iterations=3, svo_depth=sd, guide_ratio=0.5, min_bin_level=3, bin_ray_count=64, nee=nee, mis=nee)
+    st, ws = scene.render_wfpg(img.data_ptr(), w, h, S, depth, rr, seed, **kw)
+    first = img.cpu().numpy().copy()
+    assert np.isfinite(first).all() and first.mean() > 0
+    # the octree is the voxel octree of the scene's triangles in the
+    # guide's cube, record for record and node for node
+    tris = np.ascontiguousarray(sc["tris"], np.float32).reshape(-1, 9)
+    p = tris.reshape(-1, 3)
+    cube = oracle.svo_cube(p.min(0), p.max(0), sd)
+    recs = oracle.voxelize(tris, cube, sd)
+    assert ws.voxel_records == len(recs)
+    o = oracle.Svo.build(recs, cube.tolist(), sd)
+    assert ws.octree_nodes == o.nodes
+    o.close()
+    # two guided frames, each bounce with surface hits binned and sampled
+    assert 2 <= ws.guided_bounces <= 2 * depth
+    assert 0 < ws.bins <= ws.binned_rays
+    assert ws.binned_rays < st.segments
+    st2, ws2 = scene.render_wfpg(img.data_ptr(), w, h, S, depth, rr, seed, **kw)
+    assert st2.segments == st.segments
+    assert (ws2.bins, ws2.binned_rays, ws2.guided_bounces) == (ws.bins, ws.binned_rays, ws.guided_bounces)
+    assert np.allclose(img.cpu().numpy(), first, rtol=1e-4, atol=1e-6)
+    scene.close()
+
+
+def test_rejects_bad_arguments(ctx):
+    scene = N.Scene(ctx, N.gen_cornell(2))
+    b = torch.zeros(16 * 16 * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for kw in (dict(min_bin_level=11), dict(bin_ray_count=0), dict(bin_ray_count=4097), dict(filter_alpha=0.0),
+               dict(filter_alpha=float("nan")), dict(boundary_radiance=-1.0), dict(iterations=0),
+               dict(svo_depth=11), dict(guide_ratio=1.5)):
+        with pytest.raises(N.MrhError):
+            scene.render_wfpg(b.data_ptr(), 16, 16, 1, 4, 2, 1, **kw)
+    scene.close()
+    two = N.Scene(ctx, I.instanced_room(n_spheres=4))
+    torch.cuda.synchronize()
+    with pytest.raises(N.MrhError):
+        two.render_wfpg(b.data_ptr(), 16, 16, 1, 4, 2, 1, iterations=2)
+    two.close()

@@ -150,6 +150,45 @@ def guided_leg(N, ctx, scene, img, torch):
             "segments": int(st.segments), "octree_nodes": int(v.nodes)}
 
 
+def wfpg_render_leg(N, ctx, scene, img, torch):
+    """The same scene and iteration split as the guided leg through the
+    field-traced guide (mrh_render_wfpg): voxel octree of the scene at depth
+    10, bins of >= 512 rays from level 5, per-bin cone-traced fields (64^2 at
+    the first bounce down to 8^2 from the fourth), Gaussian alpha 1, 50/50
+    mixture.  One warm-up render with per-stage event timing, one timed
+    render without it."""
+    it, S, depth, svo_depth = 4, 4, 12, 10
+    kw = dict(iterations=it, svo_depth=svo_depth, guide_ratio=0.5, min_bin_level=5, bin_ray_count=512,
+              filter_alpha=1.0)
+    stages = {"trace": ["trace_pt"], "shade": ["shade"], "prep": ["wfpg_prep"],
+              "binning": ["bin_start", "bin_heads", "bin_own", "bin_collapse", "bin_assign", "bin_flags",
+                          "bin_runs", "wfpg_minray", "wfpg_bins", "sort_hist", "sort_scan", "sort_onesweep",
+                          "scan_tiles", "scan_add"],
+              "field_trace": ["wfpg_trace_fields"], "field_sample": ["wfpg_fields"],
+              "payload": ["payload_keys", "payload_runs", "payload_leaf_normals", "payload_node_normals",
+                          "payload_splat", "payload_leaf_avg", "payload_bottom_up", "payload_top_down",
+                          "payload_out"]}
+    ctx.profile(True)
+    ctx.profile_reset()
+    scene.render_wfpg(img.data_ptr(), W, H, S, depth, RR, SEED + 1, **kw)
+    torch.cuda.synchronize()
+    split = {k: round(sum(ctx.profile_get(n)[0] for n in names), 3) for k, names in stages.items()}
+    ctx.profile(False)
+    t0 = time.perf_counter()
+    st, ws = scene.render_wfpg(img.data_ptr(), W, H, S, depth, RR, SEED + 1, **kw)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"wfpg render: {st.segments} segments in {dt * 1e3:.1f} ms, {ws.bins} bins / {ws.binned_rays} binned rays "
+        f"over {ws.guided_bounces} bounces, octree {ws.octree_nodes} nodes; stage ms {split}")
+    return {"workload": f"same scene, 1920x1080, {it} learn-then-render iterations x {S * S} spp "
+                        f"({it * S * S} spp), max depth {depth}, voxel octree depth {svo_depth}, bins of >= 512 rays "
+                        f"from level 5, cone-traced fields 64^2..8^2, 50/50 mixture",
+            "Mrays_per_s": round(st.segments / dt / 1e6, 3), "ms": round(dt * 1e3, 3),
+            "segments": int(st.segments), "bins": int(ws.bins), "binned_rays": int(ws.binned_rays),
+            "guided_bounces": int(ws.guided_bounces), "octree_nodes": int(ws.octree_nodes),
+            "voxel_records": int(ws.voxel_records), "stage_ms_profiled_render": split}
+
+
 def nee_leg(N, ctx, scene, img, torch):
     """The same frame with next-event estimation and direct-light MIS: one
     light sample per non-specular vertex, traced with the next bounce.  One
@@ -570,6 +609,7 @@ def main():
     ap.add_argument("--no-instanced", action="store_true", help="skip the two-level (instanced) leg")
     ap.add_argument("--no-c2", action="store_true", help="skip the LBVH build + ray cast leg")
     ap.add_argument("--no-wfpg", action="store_true", help="skip the guided-path field stage leg")
+    ap.add_argument("--no-wfpg-render", action="store_true", help="skip the field-traced guided render leg")
     ap.add_argument("--no-voxel", action="store_true", help="skip the voxeliser + octree build leg")
     ap.add_argument("--requests", type=int, default=2000, help="serving leg requests (0 = skip)")
     ap.add_argument("--rate", type=float, default=40.0, help="serving leg arrivals per second")
@@ -707,6 +747,9 @@ def main():
     guided = None
     if not args.no_guided:
         guided = guided_leg(N, ctx, scene, img, torch)
+    wfpg_render = None
+    if not args.no_wfpg_render:
+        wfpg_render = wfpg_render_leg(N, ctx, scene, img, torch)
     nee = None
     if not args.no_nee:
         nee = nee_leg(N, ctx, scene, img, torch)
@@ -764,6 +807,7 @@ def main():
             "serving": serving,
             "autotune": tune,
             "guided": guided,
+            "wfpg_render": wfpg_render,
             "nee": nee,
             "instanced": instanced,
             "c2": c2,

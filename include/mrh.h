@@ -568,6 +568,8 @@ typedef struct {
     float*          node_irradiance;    /* [nodes][2] front, back */
     float*          node_normals;       /* [nodes][3] */
     float*          node_normal_length; /* optional [nodes] */
+    const uint32_t* vertex_mask;        /* optional [n_vertices]: a vertex whose entry is
+                                           0xFFFFFFFF is skipped (path-vertex slots left empty) */
 } mrh_svo_payload_desc;
 int32_t mrh_svo_payload(mrh_context* ctx, const mrh_svo* svo, const mrh_svo_payload_desc* desc);
 

@@ -821,7 +821,8 @@ __global__ void k_pl_node_normals(uint32_t off, uint32_t count, const uint32_t* 
     }
 }
 
-__global__ void k_pl_splat(const float* __restrict__ verts, uint32_t n, const uint32_t* __restrict__ first_child,
+__global__ void k_pl_splat(const float* __restrict__ verts, uint32_t n, const uint32_t* __restrict__ vmask,
+                           const uint32_t* __restrict__ first_child,
                            const uint32_t* __restrict__ mask, uint32_t depth, float4 cube, const float4* __restrict__ nrm,
                            unsigned long long* __restrict__ tot, uint32_t* __restrict__ num)
 {
@@ -829,6 +830,7 @@ __global__ void k_pl_splat(const float* __restrict__ verts, uint32_t n, const ui
     const float lo[3] = {cube.x, cube.y, cube.z};
     const float leaf = cube.w / (float)R;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        if (vmask && vmask[v] == 0xFFFFFFFFu) continue;
         const float* x = verts + 7 * (size_t)v;
         bool inside = true;
         for (int a = 0; a < 3; a++) inside = inside && !(x[a] < lo[a] || x[a] > lo[a] + cube.w);
@@ -1192,33 +1194,23 @@ __global__ void k_svo_pdf(SvoView s, const uint32_t* __restrict__ nodes, const f
 // Start node of each ray (the deepest existing node of its cell), and the
 // (key, ray) pairs of the first sort; rays without a code get key `skip`.
 __global__ void k_bin_start(SvoView s, const uint32_t* __restrict__ codes, uint32_t n, uint32_t skip,
-                            uint32_t* __restrict__ start, uint32_t* __restrict__ key, uint32_t* __restrict__ val)
+                            uint32_t* __restrict__ start, uint32_t* __restrict__ own)
 {
+    const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t c = codes[i];
         const uint32_t v = (c == SVO_NO_NODE) ? skip : svo_lookup(s, c);
         start[i] = v;
-        key[i] = v;
-        val[i] = i;
-    }
-}
-
-// Run-length encoding of the sorted start nodes: first position of each run
-__global__ void k_bin_heads(const uint32_t* __restrict__ skey, uint32_t n, uint32_t skip, uint32_t* __restrict__ first)
-{
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t k = skey[p];
-        if (k != skip && (p == 0 || skey[p - 1] != k)) first[k] = p;
-    }
-}
-
-// ... then its length, the node's own ray count
-__global__ void k_bin_own(const uint32_t* __restrict__ skey, uint32_t n, uint32_t skip,
-                          const uint32_t* __restrict__ first, uint32_t* __restrict__ own)
-{
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t k = skey[p];
-        if (k != skip && (p == n - 1 || skey[p + 1] != k)) own[k] = p + 1 - first[k];
+        // the node's own ray count: one atomic per distinct node of the wave
+        const bool has = v != skip;
+        uint64_t pend = __ballot(has);
+        while (pend) {
+            const int l = __ffsll((unsigned long long)pend) - 1;
+            const uint32_t lv = __shfl(v, l, 64);
+            const uint64_t same = __ballot(has && v == lv);
+            if ((int)lane == l) atomicAdd(own + lv, (uint32_t)__popcll(same));
+            pend &= ~same;
+        }
     }
 }
 
@@ -1308,21 +1300,19 @@ int32_t svo_bin_rays(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, uint32
     const uint32_t skip = N;                    // sorts after every node
     const uint32_t bits = 32u - (uint32_t)__builtin_clz(N);   // holds 0..N
     if (min_level > s->depth) min_level = s->depth;   // WFPGTracer.cu:612-613
-    // workspace: start, key, val, skey, sval [n]; own, first, tot [N]; flags, pos [n]; scan tmp
+    // workspace: start, key, val, skey, flags, pos [n]; own, tot [N]; scan tmp
     const size_t tmp_words = scan_tmp_words(n ? n : 1);
-    const size_t words = 7 * (size_t)n + 3 * (size_t)N + tmp_words + 64;
+    const size_t words = 6 * (size_t)n + 2 * (size_t)N + tmp_words + 64;
     MRH_TRY(grow(ctx, &s->bin_work, &s->bin_work_bytes, words * 4));
     uint32_t* w = (uint32_t*)s->bin_work;
     uint32_t* start = w;
     uint32_t* key = start + n;
     uint32_t* val = key + n;
     uint32_t* skey = val + n;
-    uint32_t* sval = skey + n;
-    uint32_t* flags = sval + n;
+    uint32_t* flags = skey + n;
     uint32_t* pos = flags + n;
     uint32_t* own = pos + n;
-    uint32_t* first = own + N;
-    uint32_t* tot = first + N;
+    uint32_t* tot = own + N;
     uint32_t* tmp = tot + N;
     if (order) {
         if (part_count) MRH_TRY(fill_u32(ctx, part_count, 0u, 1));
@@ -1331,22 +1321,9 @@ int32_t svo_bin_rays(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, uint32
     if (n == 0) return MRH_OK;
     const uint32_t g = stride_grid(ctx, n, 256);
     const SvoView view = svo_view(s);
-    MRH_TRY(launch(ctx, "bin_start", dim3(g), dim3(256), 0, k_bin_start, view, codes, n, skip, start, key, val));
-    SortArgs sa;
-    sa.keys_in = key;
-    sa.vals_in = val;
-    sa.keys_out = skey;
-    sa.vals_out = sval;
-    sa.n_max = n;
-    sa.key_bytes = 4;
-    sa.begin_bit = 0;
-    sa.end_bit = bits;
-    MRH_TRY(radix_sort(ctx, sa));
     MRH_TRY(fill_u32(ctx, own, 0u, N));
     MRH_TRY(fill_u32(ctx, tot, 0u, N));
-    MRH_TRY(launch(ctx, "bin_heads", dim3(g), dim3(256), 0, k_bin_heads, (const uint32_t*)skey, n, skip, first));
-    MRH_TRY(launch(ctx, "bin_own", dim3(g), dim3(256), 0, k_bin_own, (const uint32_t*)skey, n, skip,
-                   (const uint32_t*)first, own));
+    MRH_TRY(launch(ctx, "bin_start", dim3(g), dim3(256), 0, k_bin_start, view, codes, n, skip, start, own));
     for (int32_t l = (int32_t)s->depth; l >= (int32_t)min_level; l--) {
         const uint32_t cnt = s->lvl_count[l];
         if (!cnt) continue;
@@ -1357,7 +1334,15 @@ int32_t svo_bin_rays(mrh_context* ctx, mrh_svo* s, const uint32_t* codes, uint32
     MRH_TRY(launch(ctx, "bin_assign", dim3(g), dim3(256), 0, k_bin_assign, (const uint32_t*)start, n, skip,
                    (const uint32_t*)tot, (const uint32_t*)s->parent, bins, order ? key : nullptr, val));
     if (!order) return MRH_OK;
+    SortArgs sa;
+    sa.keys_in = key;
+    sa.vals_in = val;
+    sa.keys_out = skey;
     sa.vals_out = order;
+    sa.n_max = n;
+    sa.key_bytes = 4;
+    sa.begin_bit = 0;
+    sa.end_bit = bits;
     MRH_TRY(radix_sort(ctx, sa));
     MRH_TRY(launch(ctx, "bin_flags", dim3(g), dim3(256), 0, k_bin_flags, (const uint32_t*)skey, n, skip, flags));
     MRH_TRY(scan_exclusive(ctx, flags, pos, n, tmp));
@@ -1492,7 +1477,7 @@ int32_t mrh_svo_payload(mrh_context* ctx, const mrh_svo* svo, const mrh_svo_payl
     MRH_TRY(fill_u32(ctx, num, 0u, 2 * (size_t)nodes));
     if (d->n_vertices)
         MRH_TRY(launch(ctx, "payload_splat", dim3(stride_grid(ctx, d->n_vertices, 256)), dim3(256), 0, k_pl_splat,
-                       d->vertices, d->n_vertices, (const uint32_t*)svo->first_child, (const uint32_t*)svo->child_mask,
+                       d->vertices, d->n_vertices, d->vertex_mask, (const uint32_t*)svo->first_child, (const uint32_t*)svo->child_mask,
                        D, make_float4(svo->cube[0], svo->cube[1], svo->cube[2], svo->cube[3]), (const float4*)nrm, tot,
                        num));
     float2* irr = (float2*)d->node_irradiance;

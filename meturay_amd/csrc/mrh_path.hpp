@@ -448,7 +448,7 @@ struct RecBuf {
     float* tb;         // B
     float* rad;        // luminance of the radiance arriving along that ray
     float* vtx;        // optional [stride * max_depth][7]: position, direction toward the
-                       // previous vertex, rad (the octree payload's path vertices)
+                       // previous vertex, rad (the octree payload's path vertices; code marks the valid ones)
     uint32_t max_depth;
     uint32_t stride;   // samples per frame
 };

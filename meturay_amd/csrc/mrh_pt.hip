@@ -625,7 +625,7 @@ struct WfState {
     float* fields = nullptr;
     size_t field_floats = 0, bin_cap = 0;
     unsigned long long* minkey = nullptr;   // per node: the bin origin's (sample << 32 | ray)
-    float* vtx = nullptr;            // recorded path vertices, [records][7]
+    float* vtx = nullptr;            // recorded path vertices, [record slots][7]
     mrh_wfpg_stats st = {};
     std::vector<void*> allocs;
 };
@@ -724,13 +724,46 @@ __global__ __launch_bounds__(256) void k_wf_prep(PathBuf in, HitBuf hit, uint32_
 
 // Per node of the octree, the binned ray of the smallest path (sample) id:
 // (sample << 32 | ray) by a 64-bit atomic minimum, so the choice does not
-// depend on the order of the rays in the bounce's buffer.
-__global__ void k_wf_minray(const uint32_t* __restrict__ codes, const uint32_t* __restrict__ bins,
-                            const uint32_t* __restrict__ sample, uint32_t n, unsigned long long* __restrict__ minkey)
+// depend on the order of the rays in the bounce's buffer.  The rays are read
+// in bin order, so a wave holds one or two bins: each distinct bin of the
+// wave is reduced across the wave first and takes one atomic.  The loop
+// bound is uniform over the block, so every lane joins the shuffles.
+__global__ __launch_bounds__(256) void k_wf_minray(const uint32_t* __restrict__ count,
+                                                   const uint32_t* __restrict__ offs,
+                                                   const uint32_t* __restrict__ order,
+                                                   const uint32_t* __restrict__ bins,
+                                                   const uint32_t* __restrict__ sample,
+                                                   unsigned long long* __restrict__ minkey)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (codes[i] == SVO_NO_NODE) continue;
-        atomicMin(minkey + bins[i], ((unsigned long long)sample[i] << 32) | i);
+    const uint32_t m = offs[*count];   // binned rays
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < m; base += gridDim.x * blockDim.x) {
+        const uint32_t p = base + threadIdx.x;
+        const bool valid = p < m;
+        uint32_t node = SVO_NO_NODE, lo = 0xFFFFFFFFu, hi = 0xFFFFFFFFu;
+        if (valid) {
+            const uint32_t r = order[p];
+            node = bins[r];
+            hi = sample[r];
+            lo = r;
+        }
+        uint64_t pend = __ballot(valid);
+        while (pend) {
+            const int l = __ffsll((unsigned long long)pend) - 1;
+            const uint32_t ln = __shfl(node, l, 64);
+            const bool mine = valid && node == ln;
+            const uint64_t same = __ballot(mine);
+            uint32_t h = mine ? hi : 0xFFFFFFFFu, w = mine ? lo : 0xFFFFFFFFu;
+            for (int off = 32; off > 0; off >>= 1) {
+                const uint32_t oh = __shfl_xor(h, off, 64), ow = __shfl_xor(w, off, 64);
+                if (oh < h || (oh == h && ow < w)) {
+                    h = oh;
+                    w = ow;
+                }
+            }
+            if ((int)lane == l) atomicMin(minkey + ln, ((unsigned long long)h << 32) | w);
+            pend &= ~same;
+        }
     }
 }
 
@@ -843,8 +876,9 @@ int32_t wf_bounce(mrh_context* ctx, Frame& f, const PathBuf& in, const HitBuf& h
     }
     const uint32_t key = (f.seed * 0x9e3779b9u) ^ (f.sample_base * 0x85ebca6bu) ^ (f.bounce * 0xc2b2ae35u);
     MRH_TRY(fill_u32(ctx, (uint32_t*)w.minkey, 0xFFFFFFFFu, 2 * (size_t)w.nodes));
-    MRH_TRY(launch(ctx, "wfpg_minray", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_wf_minray,
-                   (const uint32_t*)w.codes, (const uint32_t*)w.bins, (const uint32_t*)in.sample, n, w.minkey));
+    MRH_TRY(launch(ctx, "wfpg_minray", dim3(stride_grid(ctx, binned, 256)), dim3(256), 0, k_wf_minray,
+                   (const uint32_t*)w.pcount, (const uint32_t*)w.offs, (const uint32_t*)w.order,
+                   (const uint32_t*)w.bins, (const uint32_t*)in.sample, w.minkey));
     MRH_TRY(launch(ctx, "wfpg_bins", dim3(stride_grid(ctx, nb, 256)), dim3(256), 0, k_wf_bins,
                    (const uint32_t*)w.pcount, (const uint32_t*)w.pbins, (const unsigned long long*)w.minkey,
                    (const float4*)w.org, w.origins, w.jitter, key));
@@ -1294,6 +1328,9 @@ int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, c
     gp.svo_depth = wp->svo_depth;
     gp.guide_ratio = wp->guide_ratio;
     MRH_TRY(guide_validate(ctx, &gp, total, p->max_depth));
+    if (wp->iterations > 1 && total * p->max_depth >= (1ull << 30))
+        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: %llu path-vertex slots exceed the payload's 2^30",
+                    (unsigned long long)(total * p->max_depth));
     void* arena = nullptr;
     MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(frame_cap(total, p->nee)), &arena));
     const size_t img_count = (size_t)p->width * p->height * 3;
@@ -1369,7 +1406,6 @@ int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, c
         f.ga.wf_gpdf = w.gpdf;
         f.ga.wf_bdir = w.bdir;
         f.ga.wf_epdf = w.epdf;
-        if (f.learn) rc = fill_u32(ctx, (uint32_t*)w.vtx, 0x7F800000u, (size_t)g.n_rec * 7);   // +inf: no vertex
         while (rc == MRH_OK && f.bounce < f.max_depth) rc = frame_bounce(ctx, s, f, d_image);
         if (rc == MRH_OK) rc = frame_counts(ctx, f, &local);
         if (rc == MRH_OK && f.learn) {
@@ -1381,6 +1417,7 @@ int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, c
             pd.n_records = w.n_vox;
             pd.vertices = w.vtx;
             pd.n_vertices = (uint32_t)g.n_rec;
+            pd.vertex_mask = g.rec.code;   // slots without a vertex keep SVO_NO_NODE
             pd.node_irradiance = w.irr;
             pd.node_normals = w.nrm;
             rc = mrh_svo_payload(ctx, w.svo, &pd);

@@ -90,7 +90,8 @@ class WfpgFieldsDesc(C.Structure):
 
 class SvoPayloadDesc(C.Structure):
     _fields_ = [("tris", _P), ("codes", _P), ("prims", _P), ("n_records", _U32), ("vertices", _P),
-                ("n_vertices", _U32), ("node_irradiance", _P), ("node_normals", _P), ("node_normal_length", _P)]
+                ("n_vertices", _U32), ("node_irradiance", _P), ("node_normals", _P), ("node_normal_length", _P),
+                ("vertex_mask", _P)]
 
 
 class WfpgTraceDesc(C.Structure):
@@ -772,10 +773,11 @@ class Svo:
                                                 ptr(d_nodes), ptr(d_t), ptr(d_is_leaf)))
 
     def payload(self, d_tris, d_codes, d_prims, n_records: int, d_vertices, n_vertices: int, d_irradiance,
-                d_normals, d_lengths=None) -> None:
-        """mrh_svo_payload: normals and front/back radiance of every node."""
+                d_normals, d_lengths=None, d_mask=None) -> None:
+        """mrh_svo_payload: normals and front/back radiance of every node;
+        vertices whose d_mask entry is 0xFFFFFFFF are skipped."""
         d = SvoPayloadDesc(ptr(d_tris), ptr(d_codes), ptr(d_prims), n_records, ptr(d_vertices), n_vertices,
-                           ptr(d_irradiance), ptr(d_normals), ptr(d_lengths))
+                           ptr(d_irradiance), ptr(d_normals), ptr(d_lengths), ptr(d_mask))
         self.ctx.check(lib().mrh_svo_payload(self.ctx.h, self.h, C.byref(d)))
 
     def trace_fields(self, field_x: int, field_y: int, bin_count: int, d_origins, d_jitter, d_irradiance, d_normals,

@@ -209,6 +209,17 @@ def test_payload_matches_oracle_and_drives_the_fields(ctx, oracle):
     assert np.array_equal(d_nrm.cpu().numpy().view(np.uint32), on[:, :3].ravel().view(np.uint32))
     assert np.array_equal(d_len.cpu().numpy().view(np.uint32), on[:, 3].view(np.uint32))
     assert (oi[o.level_offset[depth]:] != 65504.0).mean() > 0.5
+    # a vertex mask (0xFFFFFFFF = skipped) is the payload of the kept vertices
+    keep = rng.random(len(verts)) < 0.6
+    mask = np.where(keep, 0, 0xFFFFFFFF).astype(np.uint32)
+    d_irr2 = torch.empty(2 * n, device="cuda")
+    d_m = dev(mask)
+    torch.cuda.synchronize()
+    svo.payload(d_t, d_c, d_p, len(codes), d_v, len(verts), d_irr2, d_nrm, d_mask=d_m)
+    ctx.sync()
+    oi2, _ = o.payload(tris, codes, prims, np.ascontiguousarray(verts[keep]))
+    assert np.array_equal(d_irr2.cpu().numpy().view(np.uint32), oi2.ravel().view(np.uint32))
+    assert not np.array_equal(oi2, oi)
     # fields from that payload at bins on the surfaces, then samples
     bins, side = 16, 32
     org = verts[:bins, :3] + 0.01 * verts[:bins, 3:6]

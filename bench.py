@@ -182,13 +182,23 @@ def wfpg_render_leg(N, ctx, scene, img, torch):
     dt = time.perf_counter() - t0
     log(f"wfpg render: {st.segments} segments in {dt * 1e3:.1f} ms, {ws.bins} bins / {ws.binned_rays} binned rays "
         f"over {ws.guided_bounces} bounces, octree {ws.octree_nodes} nodes; stage ms {split}")
+    # the reference's default: the guide's half samples field x BxDF
+    scene.render_wfpg(img.data_ptr(), W, H, S, depth, RR, SEED + 1, product=1, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stp, wsp = scene.render_wfpg(img.data_ptr(), W, H, S, depth, RR, SEED + 1, product=1, **kw)
+    torch.cuda.synchronize()
+    dtp = time.perf_counter() - t0
+    log(f"wfpg render, product guide: {stp.segments} segments in {dtp * 1e3:.1f} ms")
     return {"workload": f"same scene, 1920x1080, {it} learn-then-render iterations x {S * S} spp "
                         f"({it * S * S} spp), max depth {depth}, voxel octree depth {svo_depth}, bins of >= 512 rays "
                         f"from level 5, cone-traced fields 64^2..8^2, 50/50 mixture",
             "Mrays_per_s": round(st.segments / dt / 1e6, 3), "ms": round(dt * 1e3, 3),
             "segments": int(st.segments), "bins": int(ws.bins), "binned_rays": int(ws.binned_rays),
             "guided_bounces": int(ws.guided_bounces), "octree_nodes": int(ws.octree_nodes),
-            "voxel_records": int(ws.voxel_records), "stage_ms_profiled_render": split}
+            "voxel_records": int(ws.voxel_records), "stage_ms_profiled_render": split,
+            "product": {"Mrays_per_s": round(stp.segments / dtp / 1e6, 3), "ms": round(dtp * 1e3, 3),
+                        "segments": int(stp.segments), "bins": int(wsp.bins)}}
 
 
 def nee_leg(N, ctx, scene, img, torch):

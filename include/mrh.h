@@ -711,9 +711,18 @@ int32_t mrh_render_guided(mrh_context* ctx, mrh_scene* scene, const mrh_camera_d
  * (MRH_ERR_UNSUPPORTED otherwise).  NEE and MIS as in mrh_render_guided.
  * Differences from the reference: the bin origin is the bin's ray of the
  * smallest path id, not a random one (so the render does not depend on the
- * order of a bounce's rays); the product sampler is not used (the reference's
- * non-product path, productPG = false); the payload is formed from the last
- * learning frame's vertices. */
+ * order of a bounce's rays); the payload is formed from the last learning
+ * frame's vertices.  With product = 1 (the reference's default, productPG
+ * with purePG off: KCGenAndSampleDistributionProduct and
+ * ProductSampler::SampleMIS, WFPGTracerKC.cuh:1562-1766) the guide's half of
+ * the mixture samples the product of the bin's field and the ray's BxDF:
+ * one wave per ray evaluates the BxDF's luminance (f cos) at the 8x8
+ * cell-centre directions tilted toward the shading normal by 4 pi / 64
+ * radians (the reference's BxDFProjFunc, :1604-1629), the field stage's
+ * product sampler draws the guide direction and gives the product pdf at
+ * the BxDF's direction, and the weight is f cos / (r p_product + (1 - r)
+ * p_bxdf).  The product sampler's draws are hashed by the ray's place in the
+ * bounce, so product renders are not reproducible path for path. */
 typedef struct {
     uint32_t iterations;        /* frames, 1..64 (1 = plain path tracing) */
     uint32_t svo_depth;         /* "OctreeLevel", 1..10 */
@@ -722,6 +731,10 @@ typedef struct {
     uint32_t bin_ray_count;     /* "BinRayCount", 1..MRH_BIN_RAY_COUNT_MAX */
     float    filter_alpha;      /* "RFieldFilterAlpha", > 0 */
     float    boundary_radiance; /* radiance of cones that leave the octree, >= 0 */
+    uint32_t product;           /* "ProductPG": 1 = the guide samples the product of the field and
+                                   the BxDF (the field stage's product mode), 0 = the field alone;
+                                   either way mixed with the BxDF by guide_ratio ("PurePG" =
+                                   guide_ratio 1) */
 } mrh_wfpg_params;
 typedef struct {
     uint64_t bins;              /* bins over every guided bounce */

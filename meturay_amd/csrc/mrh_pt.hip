@@ -626,6 +626,7 @@ struct WfState {
     size_t field_floats = 0, bin_cap = 0;
     unsigned long long* minkey = nullptr;   // per node: the bin origin's (sample << 32 | ray)
     float* vtx = nullptr;            // recorded path vertices, [record slots][7]
+    float* bfield = nullptr;         // product mode: per ray the BxDF's 8x8 cell luminances
     mrh_wfpg_stats st = {};
     std::vector<void*> allocs;
 };
@@ -782,6 +783,55 @@ __global__ void k_wf_bins(const uint32_t* __restrict__ count, const uint32_t* __
     }
 }
 
+// Product mode: per guided ray the luminance of its BxDF (f cos) toward the
+// 64 cell centres of the 8x8 concentric-octahedral map, each direction tilted
+// toward the shading normal by the cell's solid angle in radians (4 pi / 64)
+// about cross(direction, normal) (BxDFProjFunc, WFPGTracerKC.cuh:1604-1629;
+// ProductSampler.cuh:270-308).  One wave per ray, lane = cell (row-major,
+// 8 per row), so the 256 bytes of a ray are one coalesced store.
+__global__ __launch_bounds__(256) void k_wf_bxdf_field(PathBuf in, HitBuf hit, uint32_t n, ShadeReq q,
+                                                       const uint32_t* __restrict__ flag, float* __restrict__ bfield)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t rid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; rid < n; rid += waves) {
+        if (flag[rid] == 0u) continue;   // wave-uniform
+        const uint32_t key = hit.key[rid];
+        const uint32_t batch = key >> 24, id = key & 0xFFFFFFu;
+        V3 c0, c1, c2;
+        const uint32_t li = hit.leaf[rid];
+        leaf_corners(q, li, hit.inst ? hit.inst[rid] : 0u, c0, c1, c2);
+        const V3 d = v3(in.dx[rid], in.dy[rid], in.dz[rid]);
+        const Surface s = q.tbn ? make_surface_tbn(c0, c1, c2, hit.ba[rid], hit.bb[rid], d,
+                                                   q.tbn + 3 * (size_t)__float_as_uint(q.leaves[3 * (size_t)li].w))
+                                : make_surface(c0, c1, c2, hit.ba[rid], hit.bb[rid], d);
+        const V3 wi = -normalize(d);
+        const V3 ns = rotate(conj(s.w2t), v3(0.0f, 0.0f, 1.0f));
+        const float u = ((float)(lane & 7u) + 0.5f) * 0.125f, v = ((float)(lane >> 3) + 0.5f) * 0.125f;
+        const V3 dz = octa_to_dir(u, v);
+        V3 w = v3(dz.y, dz.z, dz.x);
+        V3 k = cross(w, ns);
+        const float kl = sqrtf(dot(k, k));
+        if (kl > 0.0f) {
+            // Rodrigues: w cos a + (k x w) sin a + k (k . w)(1 - cos a), k unit
+            k = scale(k, 1.0f / kl);
+            const float a = 4.0f * PI_F / 64.0f;
+            const float ca = cosf(a), sa = sinf(a);
+            w = (scale(w, ca) + scale(cross(k, w), sa)) + scale(k, dot(k, w) * (1.0f - ca));
+        }
+        float pb;
+        V3 fc;
+        if (batch == MRH_BATCH_LAMBERT) {
+            const float* al = q.mats.lambert + (size_t)id * 3;
+            fc = lambert_eval(s, v3(al[0], al[1], al[2]), w, pb);
+        } else {
+            fc = metal_eval(s, wi, q.mats.metal + (size_t)id * 10, w, pb);
+        }
+        const float g = luminance(fc);
+        bfield[64 * (size_t)rid + lane] = isfinite(g) ? g : 0.0f;
+    }
+}
+
 __global__ void k_wf_tris(const float4* __restrict__ leaves, uint32_t n, float* __restrict__ tris)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -846,6 +896,9 @@ int32_t wf_bounce(mrh_context* ctx, Frame& f, const PathBuf& in, const HitBuf& h
     pr.euv = w.euv;
     pr.bdir = w.bdir;
     MRH_TRY(launch(ctx, "wfpg_prep", dim3(stride_grid(ctx, n, 256)), dim3(256), 0, k_wf_prep, in, hb, n, q, pr));
+    if (w.wp.product)
+        MRH_TRY(launch(ctx, "wfpg_bxdf_field", dim3(stride_grid(ctx, (uint64_t)n * 64, 256)), dim3(256), 0,
+                       k_wf_bxdf_field, in, hb, n, q, (const uint32_t*)w.flag, w.bfield));
     MRH_TRY(mrh_svo_bin_rays(ctx, w.svo, w.codes, n, w.wp.min_bin_level, w.wp.bin_ray_count, w.bins, w.order,
                              w.offs, w.pbins, w.pcount));
     uint32_t nb = 0;
@@ -904,11 +957,15 @@ int32_t wf_bounce(mrh_context* ctx, Frame& f, const PathBuf& in, const HitBuf& h
     fd.bin_offsets = w.offs;
     fd.ray_ids = w.order;
     fd.xi = (const float*)w.xi;
-    fd.eval_uv = (const float*)w.euv;
     fd.out_uv = (float*)w.guv;
     fd.out_dir = w.gdir;
     fd.out_pdf = w.gpdf;
+    fd.eval_uv = (const float*)w.euv;
     fd.out_eval_pdf = w.epdf;
+    if (w.wp.product) {   // the product distribution takes the field's place in the mixture
+        fd.bxdf_field = w.bfield;
+        fd.seed = key;
+    }
     MRH_TRY(mrh_wfpg_sample_fields(ctx, &fd));
     w.st.bins += nb;
     w.st.binned_rays += binned;
@@ -1318,6 +1375,7 @@ int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, c
                     MRH_BIN_RAY_COUNT_MAX);
     if (!(wp->filter_alpha > 0.0f) || !std::isfinite(wp->filter_alpha))
         return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: filter_alpha must be finite and > 0");
+    if (wp->product > 1) return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: product must be 0 or 1");
     if (!(wp->boundary_radiance >= 0.0f) || !std::isfinite(wp->boundary_radiance))
         return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: boundary_radiance must be finite and >= 0");
     uint64_t total = 0;
@@ -1387,6 +1445,7 @@ int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, c
         MRH_TRY(wf_alloc(ctx, w, (void**)&w.gdir, cap * 12));
         MRH_TRY(wf_alloc(ctx, w, (void**)&w.gpdf, cap * 4));
         MRH_TRY(wf_alloc(ctx, w, (void**)&w.epdf, cap * 4));
+        if (wp->product) MRH_TRY(wf_alloc(ctx, w, (void**)&w.bfield, cap * 256));
         if (g.n_rec) MRH_TRY(wf_alloc(ctx, w, (void**)&w.vtx, (size_t)g.n_rec * 28));
         return MRH_OK;
     };

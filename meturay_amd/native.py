@@ -114,7 +114,7 @@ class GuideParams(C.Structure):
 
 class WfpgParams(C.Structure):
     _fields_ = [("iterations", _U32), ("svo_depth", _U32), ("guide_ratio", _F), ("min_bin_level", _U32),
-                ("bin_ray_count", _U32), ("filter_alpha", _F), ("boundary_radiance", _F)]
+                ("bin_ray_count", _U32), ("filter_alpha", _F), ("boundary_radiance", _F), ("product", _U32)]
 
 
 class WfpgStats(C.Structure):
@@ -700,13 +700,13 @@ class Scene:
 
     def render_wfpg(self, d_image, width, height, samples, max_depth, rr_start, seed=0, iterations=4,
                     svo_depth=10, guide_ratio=0.5, min_bin_level=5, bin_ray_count=512, filter_alpha=1.0,
-                    boundary_radiance=0.0, camera=None, nee=0, mis=0):
+                    boundary_radiance=0.0, product=0, camera=None, nee=0, mis=0):
         """Guided render with the field-traced guide of the voxel octree
         (mrh_render_wfpg); returns (RenderStats, WfpgStats)."""
         cam = camera_desc(camera or self.camera)
         p = RenderParams(width, height, samples, max_depth, rr_start, nee, seed, 0, mis)
         w = WfpgParams(iterations, svo_depth, guide_ratio, min_bin_level, bin_ray_count, filter_alpha,
-                       boundary_radiance)
+                       boundary_radiance, product)
         st, ws = RenderStats(), WfpgStats()
         self.ctx.check(lib().mrh_render_wfpg(self.ctx.h, self.h, C.byref(cam), C.byref(p), C.byref(w),
                                              ptr(d_image), C.byref(st), C.byref(ws)))

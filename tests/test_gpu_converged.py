@@ -8,8 +8,9 @@ Plain path tracing, next-event estimation with MIS, the guided tracer
 (4 learn-then-render iterations, octree depth 10, 50/50 mixture, 512-sample
 node threshold) and the field-traced guided tracer (mrh_render_wfpg: voxel
 octree depth 10, bins of 512 rays from level 5, per-bin cone-traced fields,
-50/50 mixture) must all converge to the plain oracle's image: each is an
-unbiased estimator of the same radiance."""
+50/50 mixture; and its product mode, the guide's half sampling field x
+BxDF) must all converge to the plain oracle's image: each is an unbiased estimator of
+the same radiance."""
 import os
 
 import numpy as np
@@ -50,6 +51,11 @@ def _gpu_batches(ctx, name, mode):
             _, ws = scene.render_wfpg(img.data_ptr(), w, h, strata // 2, depth, 3, SEED0 + b, iterations=4,
                                       svo_depth=10, guide_ratio=0.5, min_bin_level=5, bin_ray_count=512)
             assert ws.guided_bounces > 0 and ws.bins > 0 and ws.binned_rays > 0
+        elif mode == "wfpg_product":   # the same, the guide sampling field x BxDF
+            _, ws = scene.render_wfpg(img.data_ptr(), w, h, strata // 2, depth, 3, SEED0 + b, iterations=4,
+                                      svo_depth=10, guide_ratio=0.5, min_bin_level=5, bin_ray_count=512,
+                                      product=1)
+            assert ws.guided_bounces > 0 and ws.bins > 0 and ws.binned_rays > 0
         else:   # guided: 4 iterations of (strata/2)^2 spp = the same spp per batch
             scene.render_guided(img.data_ptr(), w, h, strata // 2, depth, 3, SEED0 + b, iterations=4,
                                 svo_depth=10, guide_ratio=0.5, min_samples=512)
@@ -58,7 +64,7 @@ def _gpu_batches(ctx, name, mode):
     return CS.batch_mean_se(out), tile
 
 
-@pytest.mark.parametrize("mode", ["plain", "nee_mis", "guided", "wfpg"])
+@pytest.mark.parametrize("mode", ["plain", "nee_mis", "guided", "wfpg", "wfpg_product"])
 @pytest.mark.parametrize("name", ["c1", "grid8"])
 def test_converges_to_fp64_oracle(ctx, name, mode):
     print(f"[converged] {name} {mode}", flush=True)

@@ -65,6 +65,29 @@ def test_guided_frames_bin_and_sample_deterministically(ctx, oracle, nee):
     scene.close()
 
 
+def test_product_mode_renders_the_same_image(ctx):
+    """Product mode: the guide's half of the mixture samples field x BxDF.
+    Its draws are hashed by the ray's place in the bounce, so only what does
+    not depend on them is compared: the octree, and the image mean."""
+    scene = N.Scene(ctx, N.gen_cornell(2))
+    w, h = 64, 48
+    img = torch.zeros(w * h * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    kw = dict(iterations=2, svo_depth=8, min_bin_level=3, bin_ray_count=64)
+    st_m, ws_m = scene.render_wfpg(img.data_ptr(), w, h, 2, 6, 2, 9, **kw)
+    mix = img.cpu().numpy().copy()
+    st_p, ws_p = scene.render_wfpg(img.data_ptr(), w, h, 2, 6, 2, 9, product=1, **kw)
+    prod = img.cpu().numpy()
+    assert np.isfinite(prod).all() and prod.mean() > 0
+    assert (ws_p.octree_nodes, ws_p.voxel_records) == (ws_m.octree_nodes, ws_m.voxel_records)
+    assert ws_p.guided_bounces >= 2 and 0 < ws_p.bins <= ws_p.binned_rays < st_p.segments
+    # both estimate the same image: the means agree to a few per cent at 4 spp x 2
+    assert abs(prod.mean() / mix.mean() - 1.0) < 0.1
+    with pytest.raises(N.MrhError):
+        scene.render_wfpg(img.data_ptr(), w, h, 1, 4, 2, 1, product=2)
+    scene.close()
+
+
 def test_rejects_bad_arguments(ctx):
     scene = N.Scene(ctx, N.gen_cornell(2))
     b = torch.zeros(16 * 16 * 3, dtype=torch.float32, device="cuda")

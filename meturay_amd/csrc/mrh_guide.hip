@@ -873,8 +873,28 @@ __global__ void k_pl_splat(const float* __restrict__ verts, uint32_t n, const ui
         const float lum = x[6];
         const unsigned long long f =
             (lum > 0.0f) ? (unsigned long long)(fminf(lum, 2147483647.0f) * 65536.0f) : 0ull;
-        atomicAdd(&tot[2 * (size_t)lf + side], f);
-        atomicAdd(&num[2 * (size_t)lf + side], 1u);
+        // the lanes still here that splat into the same leaf side add up in
+        // the wave first (path vertices of neighbouring samples share
+        // leaves), one atomic pair per distinct side; integer sums, so the
+        // result is the same
+        const uint32_t slot = 2u * lf + (uint32_t)side;
+        const uint32_t flo = (uint32_t)f, fhi = (uint32_t)(f >> 32);
+        uint64_t pend = __ballot(1);
+        while (pend) {
+            const int l = __ffsll((unsigned long long)pend) - 1;
+            const uint32_t ls = __shfl(slot, l, 64);
+            const uint64_t same = __ballot(slot == ls);
+            unsigned long long sum = 0;
+            for (uint64_t it = same; it; it &= it - 1ull) {
+                const int j = __ffsll((unsigned long long)it) - 1;
+                sum += ((unsigned long long)(uint32_t)__shfl(fhi, j, 64) << 32) | (uint32_t)__shfl(flo, j, 64);
+            }
+            if ((int)(threadIdx.x & 63u) == l) {
+                atomicAdd(&tot[ls], sum);
+                atomicAdd(&num[ls], (uint32_t)__popcll(same));
+            }
+            pend &= ~same;
+        }
     }
 }
 

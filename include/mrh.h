@@ -792,6 +792,14 @@ int32_t mrh_tracer_stats(const mrh_tracer* tracer, mrh_render_stats* out, uint32
  * the first samples from the mixture of the latest guide and the BxDF.
  * `iterations` frames reproduce mrh_render_guided.  svo may be NULL. */
 int32_t mrh_tracer_set_guide(mrh_tracer* tracer, const mrh_guide_params* guide, mrh_svo* svo);
+/* mrh_tracer_set_guide with the reference's own guide, the field-traced one
+ * of mrh_render_wfpg (WFPGTracer, WFPGTracer.cu:590-760, 1590-1790): the
+ * scene is voxelised now; frame k (counted from this call) records its path
+ * vertices while k + 1 < wfpg->iterations and forms the octree's payload
+ * from them when it ends; every frame after the first guides its
+ * non-specular hits with per-bin cone-traced fields.  `iterations` frames
+ * reproduce mrh_render_wfpg.  Replaces any guide set before. */
+int32_t mrh_tracer_set_wfpg(mrh_tracer* tracer, const mrh_wfpg_params* wfpg);
 /* GPUTracerI::AttachReconFilter (RayLib/GPUTracerI.h:69): frames from the
  * next GenerateWork on are filtered (filter NULL: per-pixel mean again);
  * resets the accumulated image. */

@@ -256,11 +256,13 @@ public:
             if ((e = o.GetUInt(u, "SVODebugRenderIter")) != TracerError::OK) return e;
             if ((e = o.GetUInt(u, "SVORenderLevel")) != TracerError::OK) return e;
             if ((e = o.GetString(s, "InitialSVO")) != TracerError::OK) return e;
-            if ((e = o.GetFloat(f, "RFieldFilterAlpha")) != TracerError::OK) return e;
+            float alpha = 1.0f;
+            bool product = true;
+            if ((e = o.GetFloat(alpha, "RFieldFilterAlpha")) != TracerError::OK) return e;
             if ((e = o.GetBool(skipPG, "SkipPG")) != TracerError::OK) return e;
             if ((e = o.GetBool(purePG, "PurePG")) != TracerError::OK) return e;
             if ((e = o.GetFloat(ratio, "BXDF-GuideMISRatio")) != TracerError::OK) return e;
-            if ((e = o.GetBool(b, "DoProductPathGuiding")) != TracerError::OK) return e;
+            if ((e = o.GetBool(product, "DoProductPathGuiding")) != TracerError::OK) return e;
             if ((e = o.GetBool(b, "OptiXTraceSVO")) != TracerError::OK) return e;
             if ((e = o.GetBool(b, "UseSceneAccelerator")) != TracerError::OK) return e;
             if ((e = o.GetUInt(u, "OptixTraceBufferSize")) != TracerError::OK) return e;
@@ -274,8 +276,15 @@ public:
             if ((e = o.GetFloat(f, "GuidingDisableRatio")) != TracerError::OK) return e;
             if (level < 1 || level > 10)
                 return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "OctreeLevel must be in [1, 10]");
+            if (minLevel > 10)
+                return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "MinRayBinLevel must be in [0, 10]");
+            if (!(alpha > 0.0f))
+                return TracerError(TracerError::UNABLE_TO_INITIALIZE_TRACER, "RFieldFilterAlpha must be > 0");
             octreeLevel = level;
             binRayCount = binRays;
+            minRayBinLevel = minLevel;
+            filterAlpha = alpha;
+            productPG = product;
             // SkipPG: BxDF sampling only; PurePG: guide sampling only
             misRatio = skipPG ? 0.0f : (purePG ? 1.0f : ratio);
         }
@@ -309,6 +318,9 @@ public:
             list.emplace("OctreeLevel", OptionVariable((int64_t)octreeLevel));
             list.emplace("BinRayCount", OptionVariable((int64_t)binRayCount));
             list.emplace("BXDF-GuideMISRatio", OptionVariable(misRatio));
+            list.emplace("MinRayBinLevel", OptionVariable((int64_t)minRayBinLevel));
+            list.emplace("RFieldFilterAlpha", OptionVariable(filterAlpha));
+            list.emplace("DoProductPathGuiding", OptionVariable(productPG));
         } else {
             list.emplace("RunAsMegaKernel", OptionVariable(false));
         }
@@ -497,14 +509,19 @@ private:
             int32_t rc = mrh_tracer_create(ctx, scene->Handle(), &rp, &tracer);
             if (rc == MRH_OK && filter) rc = mrh_tracer_set_filter(tracer, &filter->desc);
             if (rc == MRH_OK && guided) {
-                // the guide learns over the first 16 frames (the octree of
-                // frame k guides frame k + 1) and then stays fixed
-                mrh_guide_params gp = {};
-                gp.iterations = 16;
-                gp.svo_depth = octreeLevel;
-                gp.guide_ratio = misRatio;
-                gp.min_samples = binRayCount;
-                rc = mrh_tracer_set_guide(tracer, &gp, nullptr);
+                // the reference's own guide (voxel octree, per-bin cone-traced
+                // fields): its payload learns over the first 16 frames (the
+                // vertices of frame k guide frame k + 1) and then stays fixed
+                mrh_wfpg_params wp = {};
+                wp.iterations = 16;
+                wp.svo_depth = octreeLevel;
+                wp.guide_ratio = misRatio;
+                wp.min_bin_level = minRayBinLevel;
+                wp.bin_ray_count = binRayCount;
+                wp.filter_alpha = filterAlpha;
+                wp.boundary_radiance = 0.0f;
+                wp.product = productPG ? 1u : 0u;
+                rc = mrh_tracer_set_wfpg(tracer, &wp);
             }
             void* img = nullptr;
             if (rc == MRH_OK) rc = mrh_device_alloc(ctx, &img, (size_t)width * height * 3 * sizeof(float));
@@ -545,8 +562,9 @@ private:
     TracerParameters params;
     bool guided;
     Opt opt;
-    uint32_t octreeLevel = 10, binRayCount = 512;
-    float misRatio = 0.5f;
+    uint32_t octreeLevel = 10, binRayCount = 512, minRayBinLevel = 5;
+    float misRatio = 0.5f, filterAlpha = 1.0f;
+    bool productPG = true;
     TracerCallbacksI* callbacks = nullptr;
     FilterMRH* filter = nullptr;
     mrh_tracer* tracer = nullptr;

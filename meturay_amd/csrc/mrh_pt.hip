@@ -1363,93 +1363,136 @@ int32_t render_guided(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd,
 // the first guides its non-specular vertices with the fields cone-traced
 // through that payload (wf_bounce), one-sample MIS with the BxDF.  The image
 // is the mean over all frames.
-int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const mrh_render_params* p,
-                    const mrh_wfpg_params* wp, float* d_image, mrh_render_stats* stats, mrh_wfpg_stats* wstats)
+// Checks of the field-traced guide's parameters (render and tracer)
+int32_t wf_validate(mrh_context* ctx, const mrh_scene* s, const mrh_wfpg_params* wp, uint64_t total,
+                    uint32_t max_depth, mrh_guide_params& gp)
 {
-    if (s->two_level)
-        return fail(ctx, MRH_ERR_UNSUPPORTED, "mrh_render_wfpg: two-level scenes are not supported");
-    if (wp->min_bin_level > 10)
-        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: min_bin_level must be in [0, 10]");
+    if (s->two_level) return fail(ctx, MRH_ERR_UNSUPPORTED, "wfpg: two-level scenes are not supported");
+    if (wp->min_bin_level > 10) return fail(ctx, MRH_ERR_INVALID, "wfpg: min_bin_level must be in [0, 10]");
     if (wp->bin_ray_count == 0 || wp->bin_ray_count > MRH_BIN_RAY_COUNT_MAX)
-        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: bin_ray_count must be in [1, %u]",
-                    MRH_BIN_RAY_COUNT_MAX);
+        return fail(ctx, MRH_ERR_INVALID, "wfpg: bin_ray_count must be in [1, %u]", MRH_BIN_RAY_COUNT_MAX);
     if (!(wp->filter_alpha > 0.0f) || !std::isfinite(wp->filter_alpha))
-        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: filter_alpha must be finite and > 0");
-    if (wp->product > 1) return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: product must be 0 or 1");
+        return fail(ctx, MRH_ERR_INVALID, "wfpg: filter_alpha must be finite and > 0");
+    if (wp->product > 1) return fail(ctx, MRH_ERR_INVALID, "wfpg: product must be 0 or 1");
     if (!(wp->boundary_radiance >= 0.0f) || !std::isfinite(wp->boundary_radiance))
-        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: boundary_radiance must be finite and >= 0");
-    uint64_t total = 0;
-    MRH_TRY(validate_params(ctx, s, p, &total));
-    mrh_guide_params gp;
+        return fail(ctx, MRH_ERR_INVALID, "wfpg: boundary_radiance must be finite and >= 0");
     memset(&gp, 0, sizeof(gp));
     gp.iterations = wp->iterations;
     gp.svo_depth = wp->svo_depth;
     gp.guide_ratio = wp->guide_ratio;
-    MRH_TRY(guide_validate(ctx, &gp, total, p->max_depth));
-    if (wp->iterations > 1 && total * p->max_depth >= (1ull << 30))
-        return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: %llu path-vertex slots exceed the payload's 2^30",
-                    (unsigned long long)(total * p->max_depth));
+    MRH_TRY(guide_validate(ctx, &gp, total, max_depth));
+    if (wp->iterations > 1 && total * max_depth >= (1ull << 30))
+        return fail(ctx, MRH_ERR_INVALID, "wfpg: %llu path-vertex slots exceed the payload's 2^30",
+                    (unsigned long long)(total * max_depth));
+    return MRH_OK;
+}
+
+// The voxel octree over the scene's triangles and the per-ray, per-node and
+// per-vertex buffers of the field-traced guide (rays per bounce `cap`,
+// path-vertex slots `n_rec`)
+int32_t wf_setup(mrh_context* ctx, const mrh_scene* s, const mrh_wfpg_params* wp, uint64_t cap, uint64_t n_rec,
+                 WfState& w)
+{
+    w.wp = *wp;
+    // AnisoSVOctreeCPU::Constrcut (AnisoSVO.cu:670-678): the scene's
+    // largest span padded by one leaf on every side
+    const mrh_lbvh* b = s->bvh;
+    float span = 0.0f;
+    for (int a = 0; a < 3; a++) span = fmaxf(span, b->hi[a] - b->lo[a]);
+    const float pad = span / (float)(1u << wp->svo_depth);
+    for (int a = 0; a < 3; a++) w.cube[a] = b->lo[a] - pad;
+    w.cube[3] = span + 2.0f * pad;
+    w.n_tris = b->n;
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.tris, (size_t)w.n_tris * 36));
+    MRH_TRY(launch(ctx, "wfpg_tris", dim3(stride_grid(ctx, w.n_tris, 256)), dim3(256), 0, k_wf_tris,
+                   (const float4*)b->leaves, w.n_tris, w.tris));
+    MRH_TRY(mrh_svo_voxelize(ctx, w.tris, w.n_tris, w.cube, wp->svo_depth, nullptr, nullptr, 0, &w.n_vox));
+    if (w.n_vox == 0) return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: the scene voxelises to nothing");
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.vcodes, (size_t)w.n_vox * 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.vprims, (size_t)w.n_vox * 4));
+    uint32_t again = 0;
+    MRH_TRY(mrh_svo_voxelize(ctx, w.tris, w.n_tris, w.cube, wp->svo_depth, w.vcodes, w.vprims, w.n_vox,
+                             &again));
+    MRH_TRY(mrh_svo_create(ctx, &w.svo));
+    MRH_TRY(mrh_svo_build(ctx, w.svo, w.vcodes, w.n_vox, w.cube, wp->svo_depth));
+    const SvoView v = svo_view(w.svo);
+    w.nodes = v.lvl_off[v.depth + 1];
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.irr, (size_t)w.nodes * 8));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.nrm, (size_t)w.nodes * 12));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.minkey, (size_t)w.nodes * 8));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.codes, cap * 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.flag, cap * 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.bins, cap * 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.order, cap * 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.offs, (cap + 1) * 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.pbins, cap * 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.pcount, 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.org, cap * 16));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.xi, cap * 8));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.euv, cap * 8));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.guv, cap * 8));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.bdir, cap * 12));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.gdir, cap * 12));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.gpdf, cap * 4));
+    MRH_TRY(wf_alloc(ctx, w, (void**)&w.epdf, cap * 4));
+    if (wp->product) MRH_TRY(wf_alloc(ctx, w, (void**)&w.bfield, cap * 256));
+    if (n_rec) MRH_TRY(wf_alloc(ctx, w, (void**)&w.vtx, (size_t)n_rec * 28));
+    return MRH_OK;
+    return MRH_OK;
+}
+
+// Points a frame at the field-traced guide: guided once a payload exists,
+// its path vertices recorded beside the records
+void wf_frame(WfState& w, Frame& f)
+{
+    f.wf = &w;
+    f.guide = w.have_payload ? 1u : 0u;
+    f.ga.rec.vtx = w.vtx;
+    f.ga.wf_flag = w.flag;
+    f.ga.wf_gdir = w.gdir;
+    f.ga.wf_gpdf = w.gpdf;
+    f.ga.wf_bdir = w.bdir;
+    f.ga.wf_epdf = w.epdf;
+}
+
+// After a learning frame: the octree's payload from its path vertices
+int32_t wf_learn(mrh_context* ctx, WfState& w, const GuideState& g)
+{
+    mrh_svo_payload_desc pd;
+    memset(&pd, 0, sizeof(pd));
+    pd.tris = w.tris;
+    pd.codes = w.vcodes;
+    pd.prims = w.vprims;
+    pd.n_records = w.n_vox;
+    pd.vertices = w.vtx;
+    pd.n_vertices = (uint32_t)g.n_rec;
+    pd.vertex_mask = g.rec.code;   // slots without a vertex keep SVO_NO_NODE
+    pd.node_irradiance = w.irr;
+    pd.node_normals = w.nrm;
+    MRH_TRY(mrh_svo_payload(ctx, w.svo, &pd));
+    w.have_payload = true;
+    return MRH_OK;
+}
+
+int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, const mrh_render_params* p,
+                    const mrh_wfpg_params* wp, float* d_image, mrh_render_stats* stats, mrh_wfpg_stats* wstats)
+{
+    uint64_t total = 0;
+    MRH_TRY(validate_params(ctx, s, p, &total));
+    mrh_guide_params gp;
+    MRH_TRY(wf_validate(ctx, s, wp, total, p->max_depth, gp));
     void* arena = nullptr;
     MRH_TRY(scratch(ctx, SLOT_FRAME, frame_bytes(frame_cap(total, p->nee)), &arena));
     const size_t img_count = (size_t)p->width * p->height * 3;
     MRH_TRY(fill_u32(ctx, (uint32_t*)d_image, 0u, img_count));
     GuideState g;
     WfState w;
-    w.wp = *wp;
     int32_t rc = guide_init(ctx, s, total, p->max_depth, &gp, nullptr, false, g);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == MRH_OK && hipEventCreate(&e0) != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "event create");
     if (rc == MRH_OK && hipEventCreate(&e1) != hipSuccess) rc = fail(ctx, MRH_ERR_HIP, "event create");
     if (rc == MRH_OK) (void)hipEventRecord(e0, ctx->stream);
-    // the voxel octree over the scene's triangles, in the records' cube
-    auto setup = [&]() -> int32_t {
-        // AnisoSVOctreeCPU::Constrcut (AnisoSVO.cu:670-678): the scene's
-        // largest span padded by one leaf on every side
-        const mrh_lbvh* b = s->bvh;
-        float span = 0.0f;
-        for (int a = 0; a < 3; a++) span = fmaxf(span, b->hi[a] - b->lo[a]);
-        const float pad = span / (float)(1u << wp->svo_depth);
-        for (int a = 0; a < 3; a++) w.cube[a] = b->lo[a] - pad;
-        w.cube[3] = span + 2.0f * pad;
-        w.n_tris = b->n;
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.tris, (size_t)w.n_tris * 36));
-        MRH_TRY(launch(ctx, "wfpg_tris", dim3(stride_grid(ctx, w.n_tris, 256)), dim3(256), 0, k_wf_tris,
-                       (const float4*)b->leaves, w.n_tris, w.tris));
-        MRH_TRY(mrh_svo_voxelize(ctx, w.tris, w.n_tris, w.cube, wp->svo_depth, nullptr, nullptr, 0, &w.n_vox));
-        if (w.n_vox == 0) return fail(ctx, MRH_ERR_INVALID, "mrh_render_wfpg: the scene voxelises to nothing");
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.vcodes, (size_t)w.n_vox * 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.vprims, (size_t)w.n_vox * 4));
-        uint32_t again = 0;
-        MRH_TRY(mrh_svo_voxelize(ctx, w.tris, w.n_tris, w.cube, wp->svo_depth, w.vcodes, w.vprims, w.n_vox,
-                                 &again));
-        MRH_TRY(mrh_svo_create(ctx, &w.svo));
-        MRH_TRY(mrh_svo_build(ctx, w.svo, w.vcodes, w.n_vox, w.cube, wp->svo_depth));
-        const SvoView v = svo_view(w.svo);
-        w.nodes = v.lvl_off[v.depth + 1];
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.irr, (size_t)w.nodes * 8));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.nrm, (size_t)w.nodes * 12));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.minkey, (size_t)w.nodes * 8));
-        const uint64_t cap = frame_cap(total, p->nee);
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.codes, cap * 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.flag, cap * 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.bins, cap * 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.order, cap * 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.offs, (cap + 1) * 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.pbins, cap * 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.pcount, 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.org, cap * 16));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.xi, cap * 8));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.euv, cap * 8));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.guv, cap * 8));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.bdir, cap * 12));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.gdir, cap * 12));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.gpdf, cap * 4));
-        MRH_TRY(wf_alloc(ctx, w, (void**)&w.epdf, cap * 4));
-        if (wp->product) MRH_TRY(wf_alloc(ctx, w, (void**)&w.bfield, cap * 256));
-        if (g.n_rec) MRH_TRY(wf_alloc(ctx, w, (void**)&w.vtx, (size_t)g.n_rec * 28));
-        return MRH_OK;
-    };
-    if (rc == MRH_OK) rc = setup();
+    if (rc == MRH_OK) rc = wf_setup(ctx, s, wp, frame_cap(total, p->nee), g.n_rec, w);
     mrh_render_stats local;
     memset(&local, 0, sizeof(local));
     for (uint32_t it = 0; it < wp->iterations && rc == MRH_OK; it++) {
@@ -1457,31 +1500,10 @@ int32_t render_wfpg(mrh_context* ctx, mrh_scene* s, const mrh_camera_desc* cd, c
         rc = frame_begin(ctx, cd, p, arena, (uint32_t)(it * total), f);
         if (rc == MRH_OK) rc = guide_frame(ctx, g, it, f);
         if (rc != MRH_OK) break;
-        f.wf = &w;
-        f.guide = w.have_payload ? 1u : 0u;
-        f.ga.rec.vtx = w.vtx;
-        f.ga.wf_flag = w.flag;
-        f.ga.wf_gdir = w.gdir;
-        f.ga.wf_gpdf = w.gpdf;
-        f.ga.wf_bdir = w.bdir;
-        f.ga.wf_epdf = w.epdf;
+        wf_frame(w, f);
         while (rc == MRH_OK && f.bounce < f.max_depth) rc = frame_bounce(ctx, s, f, d_image);
         if (rc == MRH_OK) rc = frame_counts(ctx, f, &local);
-        if (rc == MRH_OK && f.learn) {
-            mrh_svo_payload_desc pd;
-            memset(&pd, 0, sizeof(pd));
-            pd.tris = w.tris;
-            pd.codes = w.vcodes;
-            pd.prims = w.vprims;
-            pd.n_records = w.n_vox;
-            pd.vertices = w.vtx;
-            pd.n_vertices = (uint32_t)g.n_rec;
-            pd.vertex_mask = g.rec.code;   // slots without a vertex keep SVO_NO_NODE
-            pd.node_irradiance = w.irr;
-            pd.node_normals = w.nrm;
-            rc = mrh_svo_payload(ctx, w.svo, &pd);
-            if (rc == MRH_OK) w.have_payload = true;
-        }
+        if (rc == MRH_OK && f.learn) rc = wf_learn(ctx, w, g);
     }
     if (rc == MRH_OK)
         rc = launch(ctx, "finalize", dim3(stride_grid(ctx, img_count, 256)), dim3(256), 0, k_scale, d_image,
@@ -1616,6 +1638,8 @@ struct mrh_tracer {
     bool guided = false;
     uint32_t guide_frames = 0;   // frames rendered since guiding was set
     mrh::GuideState g;
+    bool wfpg = false;           // the guide is the field-traced one (mrh_tracer_set_wfpg)
+    mrh::WfState w;
     // attached reconstruction filter: per-sample buffers of the frame in
     // flight and the accumulated filter sums
     bool filtered = false;
@@ -1662,6 +1686,7 @@ int32_t mrh_tracer_generate_work(mrh_tracer* t, const mrh_camera_desc* cam)
     MRH_TRY(frame_begin(ctx, cam, &t->p, t->arena, (uint32_t)(t->frames * t->total), t->f,
                         t->filtered ? t->sb.coords : nullptr));
     if (t->guided) MRH_TRY(guide_frame(ctx, t->g, t->guide_frames, t->f));
+    if (t->wfpg) wf_frame(t->w, t->f);
     t->active = true;
     return MRH_OK;
 }
@@ -1684,7 +1709,7 @@ int32_t mrh_tracer_render(mrh_tracer* t, int32_t* more)
     t->frames++;
     if (t->guided) {
         t->guide_frames++;
-        if (t->f.learn) MRH_TRY(guide_learn(t->ctx, t->g));
+        if (t->f.learn) MRH_TRY(t->wfpg ? wf_learn(t->ctx, t->w, t->g) : guide_learn(t->ctx, t->g));
     }
     return MRH_OK;
 }
@@ -1713,11 +1738,42 @@ int32_t mrh_tracer_set_guide(mrh_tracer* t, const mrh_guide_params* guide, mrh_s
     if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_set_guide: frame still rendering");
     if (svo && svo_ctx(svo) != ctx) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_set_guide: svo of another context");
     MRH_TRY(guide_validate(ctx, guide, t->total, t->p.max_depth));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     guide_free(t->g);
     t->g = GuideState{};
-    t->guided = false;
+    wf_free(t->w);
+    t->w = WfState{};
+    t->guided = t->wfpg = false;
     MRH_TRY(guide_init(ctx, t->scene, t->total, t->p.max_depth, guide, svo, true, t->g));
     t->guided = true;
+    t->guide_frames = 0;
+    return MRH_OK;
+}
+
+int32_t mrh_tracer_set_wfpg(mrh_tracer* t, const mrh_wfpg_params* wfpg)
+{
+    if (!t || !wfpg) return MRH_ERR_INVALID;
+    mrh_context* ctx = t->ctx;
+    if (t->active) return fail(ctx, MRH_ERR_INVALID, "mrh_tracer_set_wfpg: frame still rendering");
+    mrh_guide_params gp;
+    MRH_TRY(wf_validate(ctx, t->scene, wfpg, t->total, t->p.max_depth, gp));
+    MRH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    guide_free(t->g);
+    t->g = GuideState{};
+    wf_free(t->w);
+    t->w = WfState{};
+    t->guided = t->wfpg = false;
+    MRH_TRY(guide_init(ctx, t->scene, t->total, t->p.max_depth, &gp, nullptr, true, t->g));
+    const int32_t rc = wf_setup(ctx, t->scene, wfpg, frame_cap(t->total, t->p.nee), t->g.n_rec, t->w);
+    if (rc != MRH_OK) {
+        (void)hipStreamSynchronize(ctx->stream);
+        wf_free(t->w);
+        t->w = WfState{};
+        guide_free(t->g);
+        t->g = GuideState{};
+        return rc;
+    }
+    t->guided = t->wfpg = true;
     t->guide_frames = 0;
     return MRH_OK;
 }
@@ -1774,6 +1830,7 @@ int32_t mrh_tracer_destroy(mrh_tracer* t)
     (void)hipFree(t->accum);
     if (t->smem) (void)hipFree(t->smem);
     guide_free(t->g);
+    wf_free(t->w);
     delete t;
     return MRH_OK;
 }

@@ -206,6 +206,7 @@ _SIGS = {
     "mrh_tracer_stats": (_I32, [_P, C.POINTER(RenderStats), C.POINTER(_U32)]),
     "mrh_tracer_destroy": (_I32, [_P]),
     "mrh_tracer_set_guide": (_I32, [_P, C.POINTER(GuideParams), _P]),
+    "mrh_tracer_set_wfpg": (_I32, [_P, C.POINTER(WfpgParams)]),
     "mrh_tracer_set_filter": (_I32, [_P, C.POINTER(FilterDesc)]),
     "mrh_image_write": (_I32, [C.c_char_p, _U32, _U32, _U32, _P]),
     "mrh_scene_file_load": (_I32, [C.c_char_p, _U32, C.POINTER(_P), C.c_char_p, _U32]),

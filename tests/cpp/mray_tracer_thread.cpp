@@ -150,7 +150,7 @@ VariableList WFPGOptions(int64_t samples)
     v.emplace("SkipPG", OptionVariable(false));
     v.emplace("PurePG", OptionVariable(false));
     v.emplace("BXDF-GuideMISRatio", OptionVariable(0.5f));
-    v.emplace("DoProductPathGuiding", OptionVariable(true));
+    v.emplace("DoProductPathGuiding", OptionVariable(false));   // reproducible path for path
     v.emplace("OptiXTraceSVO", OptionVariable(false));
     v.emplace("UseSceneAccelerator", OptionVariable(false));
     v.emplace("OptixTraceBufferSize", OptionVariable((int64_t)0));

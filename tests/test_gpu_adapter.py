@@ -52,7 +52,8 @@ def test_path_tracer_through_mray_interfaces(ctx, tmp_path, filt):
 
 def _incremental(ctx, scene, cam, frames, samples, seed, guide=None):
     """frames of the C-ABI incremental tracer, Box radius 0 filter; guide =
-    (iterations, octree depth, ratio, min samples) or None"""
+    mrh_wfpg_params fields (iterations, octree depth, ratio, min bin level,
+    bin ray count, filter alpha, boundary radiance, product) or None"""
     import ctypes as C
     L = N.lib()
     p = N.RenderParams(W, H, samples, 8, 3, 0, seed, 0, 0)
@@ -61,8 +62,8 @@ def _incremental(ctx, scene, cam, frames, samples, seed, guide=None):
     f = N.filter_desc("Box", 0.0)
     ctx.check(L.mrh_tracer_set_filter(t, C.byref(f)))
     if guide:
-        g = N.GuideParams(*guide)
-        ctx.check(L.mrh_tracer_set_guide(t, C.byref(g), None))
+        g = N.WfpgParams(*guide)
+        ctx.check(L.mrh_tracer_set_wfpg(t, C.byref(g)))
     cd = N.camera_desc(cam)
     more = C.c_int32(1)
     for _ in range(frames):
@@ -86,9 +87,10 @@ def test_frames_accumulate_and_guided_tracer(ctx, tmp_path):
     ref = _incremental(ctx, scene, cam, 3, 1, 9)
     assert ref.mean() > 0.005
     assert np.allclose(got, ref, rtol=1e-6, atol=1e-7)
-    # WFPGTracer (octree level 8, 64-sample bins, ratio 0.5): the guide learns
-    # over the first 16 frames, the octree of frame k guiding frame k + 1
+    # WFPGTracer (octree level 8, bins of 64 rays from level 4, ratio 0.5,
+    # alpha 1, no product): the field-traced guide, its payload learning over
+    # the first 16 frames, the vertices of frame k guiding frame k + 1
     got = _host(tmp_path, "WFPGTracer", 3, 1, 9, "box0")
-    ref = _incremental(ctx, scene, cam, 3, 1, 9, guide=(16, 8, 0.5, 64))
+    ref = _incremental(ctx, scene, cam, 3, 1, 9, guide=(16, 8, 0.5, 4, 64, 1.0, 0.0, 0))
     scene.close()
     assert np.allclose(got, ref, rtol=1e-6, atol=1e-7)

@@ -88,6 +88,45 @@ def test_product_mode_renders_the_same_image(ctx):
     scene.close()
 
 
+def test_incremental_tracer_reproduces_the_render(ctx):
+    """mrh_tracer_set_wfpg + `iterations` frames of the incremental tracer
+    (the GPUTracerI protocol) = mrh_render_wfpg."""
+    import ctypes as C
+    sc = N.gen_cornell(2)
+    scene = N.Scene(ctx, sc)
+    w, h, S, depth, rr, seed = 48, 32, 2, 6, 2, 11
+    wp = N.WfpgParams(3, 8, 0.5, 3, 64, 1.0, 0.0, 0)
+    p = N.RenderParams(w, h, S, depth, rr, 0, seed, 0)
+    t = C.c_void_p()
+    ctx.check(N.lib().mrh_tracer_create(ctx.h, scene.h, C.byref(p), C.byref(t)))
+    ctx.check(N.lib().mrh_tracer_set_wfpg(t, C.byref(wp)))
+    cam = N.camera_desc(sc["camera"])
+    img = torch.zeros(h * w * 3, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(3):
+        ctx.check(N.lib().mrh_tracer_generate_work(t, C.byref(cam)))
+        more = C.c_int32(1)
+        while more.value:
+            ctx.check(N.lib().mrh_tracer_render(t, C.byref(more)))
+    ctx.check(N.lib().mrh_tracer_finalize(t, img.data_ptr()))
+    st = N.RenderStats()
+    frames = C.c_uint32()
+    ctx.check(N.lib().mrh_tracer_stats(t, C.byref(st), C.byref(frames)))
+    assert frames.value == 3
+    ref = torch.zeros_like(img)
+    st_r, ws_r = scene.render_wfpg(ref.data_ptr(), w, h, S, depth, rr, seed, iterations=3, svo_depth=8,
+                                   guide_ratio=0.5, min_bin_level=3, bin_ray_count=64)
+    assert ws_r.guided_bounces > 0
+    assert st.segments == st_r.segments
+    assert torch.allclose(img, ref, rtol=1e-5, atol=1e-6)
+    # a bad parameter set leaves no guide half set up
+    bad = N.WfpgParams(3, 8, 0.5, 3, 0, 1.0, 0.0, 0)
+    with pytest.raises(N.MrhError):
+        ctx.check(N.lib().mrh_tracer_set_wfpg(t, C.byref(bad)))
+    ctx.check(N.lib().mrh_tracer_destroy(t))
+    scene.close()
+
+
 def test_rejects_bad_arguments(ctx):
     scene = N.Scene(ctx, N.gen_cornell(2))
     b = torch.zeros(16 * 16 * 3, dtype=torch.float32, device="cuda")
